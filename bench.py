@@ -1,0 +1,200 @@
+"""Throughput of the gfx950 wavefront path tracer on the procedural
+1M-triangle scene at 1080p.
+
+    python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--no-cpu]
+
+A step renders one full frame: 1920x1080, 64 samples per pixel, max depth 8
+(Russian roulette from depth 3), 4096 Disney-shaded instances of a 256-triangle
+displaced sphere (1,048,576 triangles), 64 sphere mesh lights and a constant
+environment. Scene, BVH and path pool are resident in HBM before timing.
+
+Prints one JSON line: value = Mrays/s (extension + shadow rays per second of
+wall time over the timed frames), plus samples/s, the roofline of the
+dominant kernel (extension-ray traversal) and the CPU baseline (the
+reference's lib/bvh traversal on the same rays, on this host's cores).
+"""
+import argparse
+import json
+import os
+import re
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Mrays/s & samples/s @1080p; equal-time relMSE vs FP64 GT, NRC vs plain PT"
+HBM_PEAK_GBS = 8000.0
+WIDTH, HEIGHT, SPP = 1920, 1080, 64
+MIN_DEPTH, MAX_DEPTH = 3, 8
+STATE_COUNT = 1 << 21
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def algorithmic_bytes_per_ray(steps, isects):
+    """32 B ray in + 20 B hit out + 64 B per traversal step (two 32-byte
+    binary nodes) + 36 B per triangle test (three FP32 vertices), with
+    lib/bvh's SingleRayTraverser statistics on the same rays."""
+    return 32.0 + 20.0 + 64.0 * float(np.mean(steps)) + 36.0 * float(np.mean(isects))
+
+
+def read_traffic(path):
+    """HBM bytes per traced ray from the committed rocprofv3 counter summary
+    (FETCH_SIZE + WRITE_SIZE of the extension-trace kernel), if present."""
+    if not os.path.exists(path):
+        return None
+    m = re.search(r"hbm_bytes_per_ray:\s*([0-9.eE+-]+)", open(path).read())
+    return float(m.group(1)) if m else None
+
+
+def cpu_baseline(rays, world, budget_s, threads):
+    from oracle import oracle
+    if not oracle.ref_available():
+        return None, None
+    ref = oracle.RefBvh(world)
+    t0 = time.perf_counter()
+    prim, t, u, v, steps, isects = ref.trace(rays[: min(len(rays), 1 << 16)], threads=threads)
+    dt = time.perf_counter() - t0
+    per_ray = dt / min(len(rays), 1 << 16)
+    n = int(min(len(rays), max(1 << 16, budget_s / max(per_ray, 1e-9))))
+    t0 = time.perf_counter()
+    prim, t, u, v, steps, isects = ref.trace(rays[:n], threads=threads)
+    dt = time.perf_counter() - t0
+    return dict(value=n / dt / 1e6, unit="Mrays/s", cores=threads, kind="reference",
+                sample=f"{n} extension rays of the first two bounces of this frame, traced by lib/bvh "
+                       f"(BinnedSAH16 + SingleRayTraverser, closest hit) on {threads} host threads "
+                       f"({dt:.1f} s)"), (steps, isects)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--spp", type=int, default=SPP)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
+              f"the multi-GPU split of the frame is described in DESIGN.md", file=sys.stderr)
+        sys.exit(2)
+
+    import btrc_amd
+    from btrc_amd import scenes, _abi
+
+    log("bench: building the 1M-triangle scene")
+    scene, cam = scenes.sphere_field()
+    ctx = btrc_amd.Context(0)
+    t0 = time.perf_counter()
+    ctx.upload_scene(scene)
+    info = ctx.scene_info()
+    log(f"bench: scene uploaded ({info['triangles']} tris, {info['nodes']} BVH4 nodes, depth {info['depth']}, "
+        f"stack {info['stack']}) in {time.perf_counter() - t0:.1f} s on {ctx.device_name()}")
+    ctx.set_camera(cam, WIDTH, HEIGHT)
+
+    p = _abi.RenderParams()
+    p.width, p.height, p.spp = WIDTH, HEIGHT, args.spp
+    p.min_depth, p.max_depth = MIN_DEPTH, MAX_DEPTH
+    p.rr_threshold, p.rr_cont_prob = 0.1, 0.5
+    p.state_count = STATE_COUNT
+    p.time_stages = 1
+
+    for i in range(args.warmup):
+        st = ctx.render(p)
+        log(f"bench: warmup {i}: {st.seconds:.3f} s")
+
+    totals = dict(seconds=0.0, ext=0, shadow=0, paths=0, stage_ms=np.zeros(4), launches=np.zeros(4))
+    for i in range(args.steps):
+        st = ctx.render(p)
+        totals["seconds"] += st.seconds
+        totals["ext"] += st.extension_rays
+        totals["shadow"] += st.shadow_rays
+        totals["paths"] += st.paths
+        totals["stage_ms"] += np.array(st.stage_ms[:4])
+        totals["launches"] += np.array(st.stage_launches[:4])
+        log(f"bench: step {i}: {st.seconds:.3f} s, {(st.extension_rays + st.shadow_rays) / st.seconds / 1e6:.1f} "
+            f"Mrays/s, stage ms {[round(x, 1) for x in st.stage_ms]}")
+
+    secs = totals["seconds"]
+    mrays = (totals["ext"] + totals["shadow"]) / secs / 1e6
+    samples = totals["paths"] / secs
+    trace_ms = totals["stage_ms"][1]
+
+    # rays of this frame for the work model and the CPU baseline
+    log("bench: capturing rays of the first two bounces")
+    pc = _abi.RenderParams.from_buffer_copy(p)
+    pc.time_stages = 0
+    rays = ctx.capture_rays(pc, 2, 1 << 22)
+    rng = np.random.default_rng(0)
+    rays = rays[rng.permutation(len(rays))]
+
+    cpu = None
+    bytes_per_ray = None
+    if not args.no_cpu:
+        # world-space triangles in flattened order, as the GPU traverses them
+        world = flatten_instances(scene)
+        threads = min(16, os.cpu_count() or 1)
+        cpu, stats = cpu_baseline(rays, world, args.cpu_seconds, threads)
+        if stats is not None:
+            bytes_per_ray = algorithmic_bytes_per_ray(*stats)
+            cpu["steps_per_ray"] = float(np.mean(stats[0]))
+            cpu["isects_per_ray"] = float(np.mean(stats[1]))
+
+    roofline = None
+    if bytes_per_ray is not None and trace_ms > 0:
+        achieved = totals["ext"] * bytes_per_ray / (trace_ms * 1e-3) / 1e9
+        traffic_per_ray = read_traffic(os.path.join(ROOT, "profiles", "trace_extension_traffic.md"))
+        roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                        frac=round(achieved / HBM_PEAK_GBS, 4),
+                        traffic=None if traffic_per_ray is None else round(traffic_per_ray, 1),
+                        kernel="k_trace_extension", algorithmic_bytes_per_ray=round(bytes_per_ray, 1),
+                        kernel_ms_per_launch=round(trace_ms / max(1, totals["launches"][1]), 4))
+
+    out = {
+        "metric": METRIC,
+        "value": round(mrays, 2),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(secs / args.steps * 1e3, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "procedural 1M-triangle field (4096 instances x 256-tri displaced sphere, "
+                               "Disney per instance, 64 sphere lights + constant env), 1920x1080, "
+                               f"{args.spp} spp, max depth {MAX_DEPTH}",
+                   "width": WIDTH, "height": HEIGHT, "spp": args.spp, "max_depth": MAX_DEPTH,
+                   "min_depth": MIN_DEPTH, "triangles": info["triangles"], "state_count": STATE_COUNT},
+        "samples_per_s": round(samples, 1),
+        "extension_rays": int(totals["ext"]),
+        "shadow_rays": int(totals["shadow"]),
+        "stage_ms": {k: round(float(v), 2) for k, v in zip(("generate", "trace", "shade", "shadow"),
+                                                           totals["stage_ms"])},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def flatten_instances(scene):
+    parts = []
+    for inst in scene.instances:
+        p = scene.geometries[inst["geometry"]]["positions"].reshape(-1, 3).astype(np.float32)
+        m = inst["mat"].astype(np.float32)
+        # CTransform3D::apply_to_point operation order
+        w = ((m[:, 0] * p[:, :1] + m[:, 1] * p[:, 1:2]) + m[:, 2] * p[:, 2:3]) + m[:, 3]
+        parts.append(w[:, :3].reshape(-1, 9))
+    return np.concatenate(parts).astype(np.float32)
+
+
+if __name__ == "__main__":
+    main()

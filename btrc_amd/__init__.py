@@ -1,0 +1,239 @@
+"""MI355X (gfx950) wavefront path tracer with Btrc's render-path interface.
+
+The compute lives in the native library libbtrc_hip.so (HIP kernels for
+gfx950 behind the C ABI of include/btrc_hip.h). This module loads it with
+ctypes and mirrors the reference's renderer surface for Python callers:
+
+    WavefrontPathTracer  - the "wfpt" renderer (builtin/renderer/wavefront.h:9-60),
+                           same parameters and defaults (wavefront.h:13-27)
+    Scene, Camera        - the scene factory's output (factory/scene.cpp:5-94)
+                           and PinholeCamera (builtin/camera/pinhole.cpp)
+
+Errors surface as BtrcException, as in the reference (utils/exception.h).
+There is no fallback: without the built library every entry point raises.
+"""
+import ctypes as C
+import os
+import threading
+
+import numpy as np
+
+from . import _abi
+from .scenes import Camera, Scene  # noqa: F401
+
+_LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libbtrc_hip.so")
+_lib = None
+_lib_lock = threading.Lock()
+
+
+class BtrcException(RuntimeError):
+    """Failure reported by the native library (status code + message)."""
+
+    def __init__(self, status, message):
+        super().__init__(f"btrc_hip error {status}: {message}")
+        self.status = status
+
+
+def library_path():
+    return _LIB_PATH
+
+
+def load_library():
+    """Load libbtrc_hip.so; raises if it has not been built."""
+    global _lib
+    with _lib_lock:
+        if _lib is None:
+            if not os.path.exists(_LIB_PATH):
+                raise BtrcException(_abi.BTRC_ERR_STATE,
+                                    f"{_LIB_PATH} is missing: build it with __graft_entry__.build()")
+            _lib = _abi.declare(C.CDLL(_LIB_PATH))
+        return _lib
+
+
+class Context:
+    """One device context: a HIP stream, device memory and the uploaded scene."""
+
+    def __init__(self, device=0):
+        self.lib = load_library()
+        h = C.c_void_p()
+        st = self.lib.btrc_hip_create(device, C.byref(h))
+        if st != _abi.BTRC_OK:
+            raise BtrcException(st, f"cannot create a context on device {device}")
+        self.h = h
+        self._keep = None
+
+    def close(self):
+        if self.h:
+            self.lib.btrc_hip_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, st):
+        if st != _abi.BTRC_OK:
+            raise BtrcException(st, self.lib.btrc_hip_last_error(self.h).decode(errors="replace"))
+
+    def device_name(self):
+        buf = C.create_string_buffer(128)
+        self._check(self.lib.btrc_hip_device_name(self.h, buf, 128))
+        return buf.value.decode()
+
+    def upload_scene(self, scene):
+        desc, holder = scene.to_desc()
+        self._check(self.lib.btrc_hip_upload_scene(self.h, C.byref(desc)))
+        self._keep = holder
+
+    def scene_info(self):
+        n, d, s, t = C.c_int64(), C.c_int32(), C.c_int32(), C.c_int64()
+        self._check(self.lib.btrc_hip_scene_info(self.h, C.byref(n), C.byref(d), C.byref(s), C.byref(t)))
+        return dict(nodes=n.value, depth=d.value, stack=s.value, triangles=t.value)
+
+    def set_camera(self, camera, width, height):
+        self._check(self.lib.btrc_hip_set_camera(self.h, C.byref(camera.to_desc(width, height))))
+
+    def render(self, params):
+        stats = _abi.RenderStats()
+        self._check(self.lib.btrc_hip_render(self.h, C.byref(params), C.byref(stats)))
+        return stats
+
+    def film(self, width, height, fp64=False):
+        rgba = np.empty((height, width, 4), dtype=np.float32)
+        rgb64 = np.empty((height, width, 3), dtype=np.float64) if fp64 else None
+        self._check(self.lib.btrc_hip_film_readback(
+            self.h, rgba.ctypes.data, rgb64.ctypes.data if fp64 else None))
+        return rgba, rgb64
+
+    def request_stop(self):
+        self._check(self.lib.btrc_hip_request_stop(self.h))
+
+    # ---- traversal on ray batches
+    def trace_closest(self, rays, tri_test=_abi.TRI_WATERTIGHT):
+        rays = np.ascontiguousarray(rays, dtype=_abi.ray_dtype())
+        hits = np.empty(len(rays), dtype=_abi.hit_dtype())
+        self._check(self.lib.btrc_hip_trace_closest(self.h, rays.ctypes.data, len(rays), tri_test,
+                                                    hits.ctypes.data))
+        return hits
+
+    def trace_any(self, rays, tri_test=_abi.TRI_WATERTIGHT):
+        rays = np.ascontiguousarray(rays, dtype=_abi.ray_dtype())
+        occ = np.empty(len(rays), dtype=np.uint8)
+        self._check(self.lib.btrc_hip_trace_any(self.h, rays.ctypes.data, len(rays), tri_test, occ.ctypes.data))
+        return occ.astype(bool)
+
+    def batch_upload(self, rays):
+        rays = np.ascontiguousarray(rays, dtype=_abi.ray_dtype())
+        self._check(self.lib.btrc_hip_batch_upload(self.h, rays.ctypes.data, len(rays)))
+        self._batch_n = len(rays)
+
+    def batch_trace(self, tri_test=_abi.TRI_WATERTIGHT, any_hit=False, repeats=1):
+        ms = C.c_double()
+        self._check(self.lib.btrc_hip_batch_trace(self.h, tri_test, 1 if any_hit else 0, repeats, C.byref(ms)))
+        return ms.value
+
+    def batch_download(self):
+        hits = np.empty(self._batch_n, dtype=_abi.hit_dtype())
+        self._check(self.lib.btrc_hip_batch_download(self.h, hits.ctypes.data))
+        return hits
+
+    def capture_rays(self, params, bounces, max_rays):
+        rays = np.empty(max_rays, dtype=_abi.ray_dtype())
+        n = C.c_int64()
+        self._check(self.lib.btrc_hip_capture_rays(self.h, C.byref(params), bounces, rays.ctypes.data,
+                                                   max_rays, C.byref(n)))
+        return rays[:n.value]
+
+    # ---- device building blocks, for parity tests
+    def eval_halton(self, width, height, px, py, sample_index, dims):
+        px = np.ascontiguousarray(px, dtype=np.uint32)
+        py = np.ascontiguousarray(py, dtype=np.uint32)
+        si = np.ascontiguousarray(sample_index, dtype=np.int32)
+        out = np.empty((len(px), dims), dtype=np.float32)
+        self._check(self.lib.btrc_hip_eval_halton(self.h, width, height, px.ctypes.data, py.ctypes.data,
+                                                  si.ctypes.data, len(px), dims, out.ctypes.data))
+        return out
+
+    def eval_detmath(self, fn, x, y=None):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        y = np.ascontiguousarray(np.ones_like(x) if y is None else y, dtype=np.float32)
+        out = np.empty_like(x)
+        self._check(self.lib.btrc_hip_eval_detmath(self.h, fn, x.ctypes.data, y.ctypes.data, len(x),
+                                                   out.ctypes.data))
+        return out
+
+
+class RenderResult:
+    """Renderer::RenderResult (core/renderer.h:19-24): the resolved colour
+    image as host memory, plus the run's statistics."""
+
+    def __init__(self, color, color64, stats):
+        self.color = color
+        self.color64 = color64
+        self.stats = stats
+
+
+class WavefrontPathTracer:
+    """The "wfpt" renderer. Parameters and defaults follow
+    WavefrontPathTracer::Params (builtin/renderer/wavefront.h:13-27) and its
+    creator (wavefront.cpp:375-398)."""
+
+    DEFAULTS = dict(spp=128, min_depth=5, max_depth=10, rr_threshold=0.1, rr_cont_prob=0.5,
+                    state_count=1000000)
+
+    def __init__(self, context=None, **params):
+        unknown = set(params) - set(self.DEFAULTS) - {"film_fp64", "time_stages"}
+        if unknown:
+            raise BtrcException(_abi.BTRC_ERR_INVALID, f"unknown wfpt parameters: {sorted(unknown)}")
+        self.ctx = context or Context()
+        self.params = dict(self.DEFAULTS)
+        self.params.update(params)
+        self.scene = None
+        self.camera = None
+        self.width, self.height = 512, 512
+        self._committed = False
+
+    def set_scene(self, scene):
+        self.scene = scene
+        self._committed = False
+
+    def set_camera(self, camera):
+        self.camera = camera
+        self._committed = False
+
+    def set_film(self, width, height):
+        self.width, self.height = int(width), int(height)
+        self._committed = False
+
+    def commit(self):
+        if self.scene is None or self.camera is None:
+            raise BtrcException(_abi.BTRC_ERR_STATE, "commit needs a scene and a camera")
+        self.ctx.upload_scene(self.scene)
+        self.ctx.set_camera(self.camera, self.width, self.height)
+        self._committed = True
+
+    def render_params(self):
+        p = _abi.RenderParams()
+        p.width, p.height = self.width, self.height
+        p.spp = int(self.params["spp"])
+        p.min_depth = int(self.params["min_depth"])
+        p.max_depth = int(self.params["max_depth"])
+        p.rr_threshold = float(self.params["rr_threshold"])
+        p.rr_cont_prob = float(self.params["rr_cont_prob"])
+        p.state_count = int(self.params["state_count"])
+        p.film_fp64 = 1 if self.params.get("film_fp64") else 0
+        p.time_stages = 1 if self.params.get("time_stages") else 0
+        return p
+
+    def render(self):
+        if not self._committed:
+            self.commit()
+        stats = self.ctx.render(self.render_params())
+        color, color64 = self.ctx.film(self.width, self.height, fp64=bool(self.params.get("film_fp64")))
+        return RenderResult(color, color64, stats)
+
+    def stop_async(self):
+        """Renderer::stop_async (core/renderer.cpp:38-48)"""
+        self.ctx.request_stop()

@@ -1,0 +1,134 @@
+"""ctypes mirror of include/btrc_hip.h (plain C structs, no HIP types)."""
+import ctypes as C
+
+BTRC_OK = 0
+BTRC_ERR_INVALID = -1
+BTRC_ERR_HIP = -2
+BTRC_ERR_OOM = -3
+BTRC_ERR_STATE = -4
+BTRC_ERR_STOPPED = -5
+
+MATERIAL_DIFFUSE = 0
+MATERIAL_DISNEY = 1
+ENVIR_NONE = 0
+ENVIR_GRADIENT_SKY = 1
+TRI_WATERTIGHT = 0
+TRI_MOLLER_TRUMBORE = 1
+
+F3 = C.c_float * 3
+F16 = C.c_float * 16
+
+
+class GeometryDesc(C.Structure):
+    _fields_ = [("triangle_count", C.c_int32),
+                ("positions", C.POINTER(C.c_float)),
+                ("normals", C.POINTER(C.c_float)),
+                ("tex_coords", C.POINTER(C.c_float))]
+
+
+class MaterialDesc(C.Structure):
+    _fields_ = [("type", C.c_int32), ("shadow_terminator", C.c_int32), ("color", F3),
+                ("metallic", C.c_float), ("roughness", C.c_float), ("specular", C.c_float),
+                ("specular_tint", C.c_float), ("anisotropic", C.c_float),
+                ("sheen", C.c_float), ("sheen_tint", C.c_float), ("clearcoat", C.c_float),
+                ("clearcoat_gloss", C.c_float), ("transmission", C.c_float),
+                ("transmission_roughness", C.c_float), ("ior", C.c_float)]
+
+
+class InstanceDesc(C.Structure):
+    _fields_ = [("geometry", C.c_int32), ("material", C.c_int32),
+                ("local_to_world", F16), ("world_to_local", F16),
+                ("has_light", C.c_int32), ("light_intensity", F3),
+                ("inner_medium", C.c_int32), ("outer_medium", C.c_int32)]
+
+
+class EnvirLightDesc(C.Structure):
+    _fields_ = [("type", C.c_int32), ("up", F3), ("lower", F3), ("upper", F3)]
+
+
+class SceneDesc(C.Structure):
+    _fields_ = [("geometry_count", C.c_int32), ("geometries", C.POINTER(GeometryDesc)),
+                ("material_count", C.c_int32), ("materials", C.POINTER(MaterialDesc)),
+                ("instance_count", C.c_int32), ("instances", C.POINTER(InstanceDesc)),
+                ("envir_light", EnvirLightDesc)]
+
+
+class CameraDesc(C.Structure):
+    _fields_ = [("eye", F3), ("dst", F3), ("up", F3), ("fov_y_deg", C.c_float), ("w_over_h", C.c_float)]
+
+
+class RenderParams(C.Structure):
+    _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("spp", C.c_int32),
+                ("min_depth", C.c_int32), ("max_depth", C.c_int32),
+                ("rr_threshold", C.c_float), ("rr_cont_prob", C.c_float),
+                ("state_count", C.c_int32), ("film_fp64", C.c_int32), ("time_stages", C.c_int32),
+                ("reserved", C.c_int32 * 6)]
+
+
+class RenderStats(C.Structure):
+    _fields_ = [("seconds", C.c_double), ("extension_rays", C.c_uint64), ("shadow_rays", C.c_uint64),
+                ("paths", C.c_uint64), ("iterations", C.c_uint64),
+                ("stage_ms", C.c_double * 4), ("stage_launches", C.c_uint64 * 4)]
+
+
+class Ray(C.Structure):
+    _fields_ = [("o", F3), ("tmin", C.c_float), ("d", F3), ("tmax", C.c_float)]
+
+
+class Hit(C.Structure):
+    _fields_ = [("t", C.c_float), ("instance", C.c_uint32), ("prim", C.c_uint32),
+                ("u", C.c_float), ("v", C.c_float)]
+
+
+# numpy dtypes with the same layout
+def ray_dtype():
+    import numpy as np
+    return np.dtype([("o", "<f4", 3), ("tmin", "<f4"), ("d", "<f4", 3), ("tmax", "<f4")])
+
+
+def hit_dtype():
+    import numpy as np
+    return np.dtype([("t", "<f4"), ("instance", "<u4"), ("prim", "<u4"), ("u", "<f4"), ("v", "<f4")])
+
+
+# every symbol include/btrc_hip.h declares, in declaration order
+EXPORTS = [
+    "btrc_hip_create", "btrc_hip_destroy", "btrc_hip_last_error", "btrc_hip_device_name",
+    "btrc_hip_upload_scene", "btrc_hip_set_camera", "btrc_hip_scene_info",
+    "btrc_hip_render", "btrc_hip_film_readback", "btrc_hip_request_stop",
+    "btrc_hip_trace_closest", "btrc_hip_trace_any",
+    "btrc_hip_batch_upload", "btrc_hip_batch_trace", "btrc_hip_batch_download",
+    "btrc_hip_capture_rays", "btrc_hip_eval_halton", "btrc_hip_eval_detmath",
+]
+
+
+def declare(lib):
+    """Attach argtypes/restype to the library's exported functions."""
+    P = C.POINTER
+    ctx = C.c_void_p
+    sig = {
+        "btrc_hip_create": (C.c_int, [C.c_int, P(C.c_void_p)]),
+        "btrc_hip_destroy": (None, [ctx]),
+        "btrc_hip_last_error": (C.c_char_p, [ctx]),
+        "btrc_hip_device_name": (C.c_int, [ctx, C.c_char_p, C.c_size_t]),
+        "btrc_hip_upload_scene": (C.c_int, [ctx, P(SceneDesc)]),
+        "btrc_hip_set_camera": (C.c_int, [ctx, P(CameraDesc)]),
+        "btrc_hip_scene_info": (C.c_int, [ctx, P(C.c_int64), P(C.c_int32), P(C.c_int32), P(C.c_int64)]),
+        "btrc_hip_render": (C.c_int, [ctx, P(RenderParams), P(RenderStats)]),
+        "btrc_hip_film_readback": (C.c_int, [ctx, C.c_void_p, C.c_void_p]),
+        "btrc_hip_request_stop": (C.c_int, [ctx]),
+        "btrc_hip_trace_closest": (C.c_int, [ctx, C.c_void_p, C.c_int64, C.c_int32, C.c_void_p]),
+        "btrc_hip_trace_any": (C.c_int, [ctx, C.c_void_p, C.c_int64, C.c_int32, C.c_void_p]),
+        "btrc_hip_batch_upload": (C.c_int, [ctx, C.c_void_p, C.c_int64]),
+        "btrc_hip_batch_trace": (C.c_int, [ctx, C.c_int32, C.c_int32, C.c_int32, P(C.c_double)]),
+        "btrc_hip_batch_download": (C.c_int, [ctx, C.c_void_p]),
+        "btrc_hip_capture_rays": (C.c_int, [ctx, P(RenderParams), C.c_int32, C.c_void_p, C.c_int64, P(C.c_int64)]),
+        "btrc_hip_eval_halton": (C.c_int, [ctx, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p,
+                                           C.c_int32, C.c_int32, C.c_void_p]),
+        "btrc_hip_eval_detmath": (C.c_int, [ctx, C.c_int32, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib

@@ -1,0 +1,1025 @@
+// Host side of the C ABI (include/btrc_hip.h): scene commit, device memory,
+// the wavefront render loop and the traversal entry points.
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <new>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../../include/btrc_hip.h"
+#include "../kernels/wavefront.h"
+#include "bvh_build.h"
+
+using namespace btrc;
+
+namespace {
+
+struct HipError : std::runtime_error
+{
+    using std::runtime_error::runtime_error;
+};
+struct InvalidArg : std::runtime_error
+{
+    using std::runtime_error::runtime_error;
+};
+struct StateError : std::runtime_error
+{
+    using std::runtime_error::runtime_error;
+};
+struct Stopped : std::runtime_error
+{
+    using std::runtime_error::runtime_error;
+};
+
+#define BTRC_HIP_CHECK(expr)                                                                        \
+    do                                                                                              \
+    {                                                                                               \
+        hipError_t e_ = (expr);                                                                     \
+        if(e_ != hipSuccess)                                                                        \
+            throw HipError(std::string(#expr) + ": " + hipGetErrorString(e_));                      \
+    } while(0)
+
+// RAII device allocation
+template<typename T>
+struct DevBuf
+{
+    T *ptr = nullptr;
+    size_t count = 0;
+
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() { release(); }
+
+    void release()
+    {
+        if(ptr)
+            (void)hipFree(ptr);
+        ptr = nullptr;
+        count = 0;
+    }
+    void alloc(size_t n)
+    {
+        if(n == count && ptr)
+            return;
+        release();
+        if(n == 0)
+            return;
+        if(hipMalloc(&ptr, n * sizeof(T)) != hipSuccess)
+        {
+            ptr = nullptr;
+            throw std::bad_alloc();
+        }
+        count = n;
+    }
+    void upload(const T *src, size_t n, hipStream_t s)
+    {
+        alloc(n);
+        if(n)
+            BTRC_HIP_CHECK(hipMemcpyAsync(ptr, src, n * sizeof(T), hipMemcpyHostToDevice, s));
+    }
+};
+
+struct PoolSide
+{
+    DevBuf<float4> ray_o, ray_d, beta, rad, bsdf_le;
+    DevBuf<uint4> smp;
+
+    void alloc(size_t n)
+    {
+        ray_o.alloc(n); ray_d.alloc(n); beta.alloc(n); rad.alloc(n); bsdf_le.alloc(n); smp.alloc(n);
+    }
+    PathSOA view()
+    {
+        return PathSOA{ ray_o.ptr, ray_d.ptr, beta.ptr, rad.ptr, bsdf_le.ptr, smp.ptr };
+    }
+};
+
+int64_t multiplicative_inverse(int64_t a, int64_t n)
+{
+    // extended Euclid, sampler/halton.cpp:18-37
+    int64_t old_r = a, r = n, old_s = 1, s = 0;
+    while(r != 0)
+    {
+        const int64_t q = old_r / r;
+        int64_t t = old_r - q * r; old_r = r; r = t;
+        t = old_s - q * s; old_s = s; s = t;
+    }
+    int64_t x = old_s % n;
+    return x < 0 ? x + n : x;
+}
+
+HaltonConsts make_halton_consts(int32_t width, int32_t height)
+{
+    // HaltonSampler::initialize_consts, sampler/halton.cpp:118-133
+    HaltonConsts hc;
+    const int32_t res[2] = { width, height };
+    for(int i = 0; i < 2; ++i)
+    {
+        const int32_t base = i == 0 ? 2 : 3;
+        int32_t scale = 1, exp = 0;
+        while(scale < std::min(res[i], MAX_HALTON_RESOLUTION))
+        {
+            scale *= base;
+            ++exp;
+        }
+        hc.base_scales[i] = scale;
+        hc.base_exponents[i] = exp;
+    }
+    hc.mult_inverse[0] = (int32_t)multiplicative_inverse(hc.base_scales[1], hc.base_scales[0]);
+    hc.mult_inverse[1] = (int32_t)multiplicative_inverse(hc.base_scales[0], hc.base_scales[1]);
+    return hc;
+}
+
+Xform make_xform(const float *mat, const float *inv)
+{
+    Xform x;
+    for(int k = 0; k < 12; ++k)
+    {
+        x.m[k] = mat[k];
+        x.n[k] = inv[k];
+    }
+    return x;
+}
+
+// triangle_dpdu, utils/math/triangle.h:8-23
+V3 triangle_dpdu(V3 B_A, V3 C_A, float bax, float bay, float cax, float cay, V3 nor)
+{
+    const float m00 = bax, m01 = bay;
+    const float m10 = cax, m11 = cay;
+    const float det = m00 * m11 - m01 * m10;
+    if(det == 0.0f)
+        return frame_from_z(nor).x;
+    const float inv_det = 1.0f / det;
+    return normalize(m11 * inv_det * B_A - m01 * inv_det * C_A);
+}
+
+} // namespace
+
+struct btrc_ctx
+{
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string last_error;
+    std::atomic<bool> stop{ false };
+    int cu_count = 256;
+    std::string device_name;
+
+    // scene
+    bool has_scene = false;
+    DevBuf<BvhNode4> nodes;
+    DevBuf<float4> tris;
+    DevBuf<InstanceRec> instances;
+    DevBuf<GeometryRec> geometries;
+    DevBuf<MaterialRec> materials;
+    DevBuf<LightRec> lights;
+    DevBuf<float4> gx_u_a, gy_u_ba, gz_u_ca, sn_v_a, sn_v_ba, sn_v_ca, light_pos;
+    DevBuf<AliasUnit> alias;
+    int32_t light_count = 0;
+    int32_t envir_light_index = -1;
+    int32_t instance_count = 0;
+    int64_t triangle_count = 0;
+    int64_t node_count = 0;
+    int32_t bvh_depth = 0;
+    int32_t stack_size = 0;
+
+    // camera
+    bool has_camera = false;
+    CameraConsts camera{};
+
+    // render state
+    PoolSide pool[2];
+    size_t pool_size = 0;
+    DevBuf<uint32_t> hit_flag;
+    DevBuf<float4> hit_tuv;
+    DevBuf<float4> sh_o, sh_d, sh_beta;
+    DevBuf<WaveCounters> counters;
+    WaveCounters *host_counters = nullptr; // pinned
+    DevBuf<float> film;
+    DevBuf<double> film64;
+    DevBuf<float4> resolved;
+    int32_t film_w = 0, film_h = 0, film_spp = 0;
+    bool film_has64 = false;
+
+    // resident ray batch
+    DevBuf<float4> batch_rays;
+    DevBuf<uint32_t> batch_out;
+    int64_t batch_n = 0;
+
+    TraceScene trace_scene() const { return TraceScene{ nodes.ptr, tris.ptr }; }
+    SceneView scene_view() const
+    {
+        SceneView v;
+        v.instances = instances.ptr;
+        v.geometries = geometries.ptr;
+        v.materials = materials.ptr;
+        v.lights = lights.ptr;
+        v.gx_u_a = gx_u_a.ptr; v.gy_u_ba = gy_u_ba.ptr; v.gz_u_ca = gz_u_ca.ptr;
+        v.sn_v_a = sn_v_a.ptr; v.sn_v_ba = sn_v_ba.ptr; v.sn_v_ca = sn_v_ca.ptr;
+        v.light_pos = light_pos.ptr;
+        v.alias = alias.ptr;
+        v.light_count = light_count;
+        v.envir_light_index = envir_light_index;
+        v.instance_count = instance_count;
+        v.pad = 0;
+        return v;
+    }
+    int persistent_grid(int threads_per_block, int64_t work) const
+    {
+        const int64_t want = (work + threads_per_block - 1) / threads_per_block;
+        const int64_t cap = (int64_t)cu_count * 16;
+        return (int)std::max<int64_t>(1, std::min(want, cap));
+    }
+};
+
+namespace {
+
+template<typename F>
+int guarded(btrc_ctx *ctx, F &&f)
+{
+    if(!ctx)
+        return BTRC_ERR_INVALID;
+    try
+    {
+        f();
+        ctx->last_error.clear();
+        return BTRC_OK;
+    }
+    catch(const InvalidArg &e)
+    {
+        ctx->last_error = e.what();
+        return BTRC_ERR_INVALID;
+    }
+    catch(const StateError &e)
+    {
+        ctx->last_error = e.what();
+        return BTRC_ERR_STATE;
+    }
+    catch(const Stopped &e)
+    {
+        ctx->last_error = e.what();
+        return BTRC_ERR_STOPPED;
+    }
+    catch(const HipError &e)
+    {
+        ctx->last_error = e.what();
+        return BTRC_ERR_HIP;
+    }
+    catch(const std::bad_alloc &)
+    {
+        ctx->last_error = "out of memory";
+        return BTRC_ERR_OOM;
+    }
+    catch(const std::exception &e)
+    {
+        ctx->last_error = e.what();
+        return BTRC_ERR_INVALID;
+    }
+}
+
+void upload_scene(btrc_ctx *ctx, const btrc_scene_desc *sd)
+{
+    if(!sd)
+        throw InvalidArg("scene is null");
+    if(sd->geometry_count < 0 || sd->material_count < 0 || sd->instance_count < 0)
+        throw InvalidArg("negative element count");
+    if(sd->instance_count > 0 && (!sd->instances || !sd->geometries || !sd->materials))
+        throw InvalidArg("missing scene arrays");
+    if((uint32_t)sd->instance_count > PATH_FLAG_INSTANCE_ID_MASK)
+        throw InvalidArg("too many instances");
+
+    hipStream_t s = ctx->stream;
+
+    // geometries: TriangleMesh::commit per-triangle records (triangle_mesh.cpp:24-171)
+    std::vector<GeometryRec> geos(sd->geometry_count);
+    std::vector<float4> gx, gy, gz, sa, sba, sca, lpos;
+    std::vector<AliasUnit> alias_units;
+    uint32_t prim_offset = 0;
+    for(int32_t gi = 0; gi < sd->geometry_count; ++gi)
+    {
+        const btrc_geometry_desc &g = sd->geometries[gi];
+        if(g.triangle_count < 0 || (g.triangle_count > 0 && !g.positions))
+            throw InvalidArg("geometry " + std::to_string(gi) + " has no positions");
+        geos[gi].prim_offset = prim_offset;
+        geos[gi].prim_count = (uint32_t)g.triangle_count;
+        std::vector<float> areas(g.triangle_count);
+        for(int32_t t = 0; t < g.triangle_count; ++t)
+        {
+            const float *p = g.positions + 9 * (size_t)t;
+            const V3 a = v3(p[0], p[1], p[2]), b = v3(p[3], p[4], p[5]), c = v3(p[6], p[7], p[8]);
+            const V3 geo_z = normalize(cross(b - a, c - a));
+            V3 sz[3];
+            for(int k = 0; k < 3; ++k)
+            {
+                if(g.normals)
+                {
+                    const float *n = g.normals + 9 * (size_t)t + 3 * k;
+                    sz[k] = v3(n[0], n[1], n[2]);
+                    if(sz[k].x == 0.0f && sz[k].y == 0.0f && sz[k].z == 0.0f)
+                        sz[k] = geo_z;
+                }
+                else
+                    sz[k] = geo_z;
+            }
+            float uv[6] = { 0, 0, 0, 0, 0, 0 };
+            if(g.tex_coords)
+                std::memcpy(uv, g.tex_coords + 6 * (size_t)t, sizeof(uv));
+            const V3 ex = triangle_dpdu(b - a, c - a, uv[2] - uv[0], uv[3] - uv[1], uv[4] - uv[0], uv[5] - uv[1], geo_z);
+            const V3 ey = normalize(cross(geo_z, ex));
+            gx.push_back(make_float4(ex.x, ex.y, ex.z, uv[0]));
+            gy.push_back(make_float4(ey.x, ey.y, ey.z, uv[2] - uv[0]));
+            gz.push_back(make_float4(geo_z.x, geo_z.y, geo_z.z, uv[4] - uv[0]));
+            sa.push_back(make_float4(sz[0].x, sz[0].y, sz[0].z, uv[1]));
+            const V3 dba = sz[1] - sz[0], dca = sz[2] - sz[0];
+            sba.push_back(make_float4(dba.x, dba.y, dba.z, uv[3] - uv[1]));
+            sca.push_back(make_float4(dca.x, dca.y, dca.z, uv[5] - uv[1]));
+            const V3 ba = b - a, ca = c - a;
+            lpos.push_back(make_float4(a.x, a.y, a.z, 0.0f));
+            lpos.push_back(make_float4(ba.x, ba.y, ba.z, 0.0f));
+            lpos.push_back(make_float4(ca.x, ca.y, ca.z, 0.0f));
+            areas[t] = 0.5f * length(cross(ba, ca));
+        }
+        // AliasTable::initialize (utils/math/alias.cpp:13-58), sum in index order
+        float sum = 0.0f;
+        for(float a : areas)
+            sum += a;
+        geos[gi].total_area = sum;
+        const size_t n = areas.size();
+        std::vector<AliasUnit> units(n);
+        if(n)
+        {
+            const float ratio = (float)n / sum;
+            std::vector<uint32_t> overs, unders;
+            for(uint32_t i = 0; i < (uint32_t)n; ++i)
+            {
+                const float pr = areas[i] * ratio;
+                units[i].accept_prob = pr;
+                units[i].another_idx = i;
+                if(pr > 1)
+                    overs.push_back(i);
+                else if(pr < 1)
+                    unders.push_back(i);
+            }
+            while(!overs.empty() && !unders.empty())
+            {
+                const uint32_t over = overs.back();
+                const uint32_t under = unders.back();
+                overs.pop_back();
+                unders.pop_back();
+                units[over].accept_prob -= 1 - units[under].accept_prob;
+                units[under].another_idx = over;
+                if(units[over].accept_prob > 1)
+                    overs.push_back(over);
+                else if(units[over].accept_prob < 1)
+                    unders.push_back(over);
+            }
+            for(auto i : overs)
+                units[i].accept_prob = 1;
+            for(auto i : unders)
+                units[i].accept_prob = 1;
+        }
+        alias_units.insert(alias_units.end(), units.begin(), units.end());
+        prim_offset += (uint32_t)g.triangle_count;
+    }
+
+    // materials
+    std::vector<MaterialRec> mats(sd->material_count);
+    for(int32_t i = 0; i < sd->material_count; ++i)
+    {
+        const btrc_material_desc &m = sd->materials[i];
+        if(m.type != BTRC_MATERIAL_DIFFUSE && m.type != BTRC_MATERIAL_DISNEY)
+            throw InvalidArg("material " + std::to_string(i) + ": unknown type");
+        MaterialRec r{};
+        r.type = m.type;
+        r.shadow_terminator = m.shadow_terminator;
+        for(int k = 0; k < 3; ++k)
+            r.color[k] = m.color[k];
+        r.metallic = m.metallic; r.roughness = m.roughness; r.specular = m.specular;
+        r.specular_tint = m.specular_tint; r.anisotropic = m.anisotropic; r.sheen = m.sheen;
+        r.sheen_tint = m.sheen_tint; r.clearcoat = m.clearcoat; r.clearcoat_gloss = m.clearcoat_gloss;
+        r.transmission = m.transmission; r.transmission_roughness = m.transmission_roughness; r.ior = m.ior;
+        mats[i] = r;
+    }
+
+    // instances, lights and the flattened world-space triangle list
+    std::vector<InstanceRec> insts(sd->instance_count);
+    std::vector<LightRec> lights;
+    std::vector<float> world;
+    std::vector<uint32_t> tri_inst, tri_prim;
+    uint32_t prim_base = 0;
+    for(int32_t i = 0; i < sd->instance_count; ++i)
+    {
+        const btrc_instance_desc &d = sd->instances[i];
+        if(d.geometry < 0 || d.geometry >= sd->geometry_count)
+            throw InvalidArg("instance " + std::to_string(i) + ": bad geometry index");
+        if(d.material < 0 || d.material >= sd->material_count)
+            throw InvalidArg("instance " + std::to_string(i) + ": bad material index");
+        if(d.inner_medium >= 0 || d.outer_medium >= 0)
+            throw InvalidArg("instance " + std::to_string(i) + ": participating media are not supported by this build");
+        InstanceRec r{};
+        r.geometry_id = d.geometry;
+        r.material_id = d.material;
+        r.light_id = -1;
+        r.inner_medium = 0;
+        r.outer_medium = 0;
+        r.flag = 0;
+        r.prim_base = prim_base;
+        r.xf = make_xform(d.local_to_world, d.world_to_local);
+        if(d.has_light)
+        {
+            LightRec l{};
+            l.type = LIGHT_MESH;
+            l.instance = i;
+            l.geometry = d.geometry;
+            l.scale = length(xf_point(r.xf, v3(1, 0, 0)) - xf_point(r.xf, v3(0, 0, 0)));
+            for(int k = 0; k < 3; ++k)
+                l.intensity[k] = d.light_intensity[k];
+            r.light_id = (int32_t)lights.size();
+            lights.push_back(l);
+        }
+        insts[i] = r;
+        const btrc_geometry_desc &g = sd->geometries[d.geometry];
+        for(int32_t t = 0; t < g.triangle_count; ++t)
+        {
+            const float *p = g.positions + 9 * (size_t)t;
+            for(int k = 0; k < 3; ++k)
+            {
+                const V3 w = xf_point(r.xf, v3(p[3 * k], p[3 * k + 1], p[3 * k + 2]));
+                world.push_back(w.x);
+                world.push_back(w.y);
+                world.push_back(w.z);
+            }
+            tri_inst.push_back((uint32_t)i);
+            tri_prim.push_back((uint32_t)t);
+        }
+        prim_base += (uint32_t)g.triangle_count;
+    }
+    int32_t envir_index = -1;
+    if(sd->envir_light.type == BTRC_ENVIR_GRADIENT_SKY)
+    {
+        LightRec l{};
+        l.type = LIGHT_GRADIENT_SKY;
+        l.instance = -1;
+        l.geometry = -1;
+        const V3 up = normalize(v3(sd->envir_light.up[0], sd->envir_light.up[1], sd->envir_light.up[2]));
+        l.up[0] = up.x; l.up[1] = up.y; l.up[2] = up.z;
+        for(int k = 0; k < 3; ++k)
+        {
+            l.lower[k] = sd->envir_light.lower[k];
+            l.upper[k] = sd->envir_light.upper[k];
+        }
+        envir_index = (int32_t)lights.size();
+        lights.push_back(l);
+    }
+    else if(sd->envir_light.type != BTRC_ENVIR_NONE)
+        throw InvalidArg("unknown environment light type");
+
+    const uint32_t ntri = prim_base;
+    const Bvh4 bvh = build_bvh4(world.data(), ntri, BvhBuildOptions{});
+    std::vector<float4> tri_data(3 * (size_t)ntri);
+    for(uint32_t k = 0; k < ntri; ++k)
+    {
+        const uint32_t gid = bvh.prim_order[k];
+        const float *w = world.data() + 9 * (size_t)gid;
+        tri_data[3 * k + 0] = make_float4(w[0], w[1], w[2], u2f(gid));
+        tri_data[3 * k + 1] = make_float4(w[3], w[4], w[5], u2f(tri_inst[gid]));
+        tri_data[3 * k + 2] = make_float4(w[6], w[7], w[8], u2f(tri_prim[gid]));
+    }
+
+    ctx->has_scene = false;
+    ctx->nodes.upload(bvh.nodes.data(), bvh.nodes.size(), s);
+    ctx->tris.upload(tri_data.data(), tri_data.size(), s);
+    ctx->instances.upload(insts.data(), insts.size(), s);
+    ctx->geometries.upload(geos.data(), geos.size(), s);
+    ctx->materials.upload(mats.data(), mats.size(), s);
+    ctx->lights.upload(lights.data(), lights.size(), s);
+    ctx->gx_u_a.upload(gx.data(), gx.size(), s);
+    ctx->gy_u_ba.upload(gy.data(), gy.size(), s);
+    ctx->gz_u_ca.upload(gz.data(), gz.size(), s);
+    ctx->sn_v_a.upload(sa.data(), sa.size(), s);
+    ctx->sn_v_ba.upload(sba.data(), sba.size(), s);
+    ctx->sn_v_ca.upload(sca.data(), sca.size(), s);
+    ctx->light_pos.upload(lpos.data(), lpos.size(), s);
+    ctx->alias.upload(alias_units.data(), alias_units.size(), s);
+    BTRC_HIP_CHECK(hipStreamSynchronize(s));
+
+    ctx->light_count = (int32_t)lights.size();
+    ctx->envir_light_index = envir_index;
+    ctx->instance_count = sd->instance_count;
+    ctx->triangle_count = ntri;
+    ctx->node_count = (int64_t)bvh.nodes.size();
+    ctx->bvh_depth = bvh.depth;
+    ctx->stack_size = bvh.stack_bound + 1;
+    if(ctx->stack_size * 128 * 4 > 64 * 1024)
+        throw InvalidArg("BVH too deep for the LDS traversal stack (bound " + std::to_string(bvh.stack_bound) + ")");
+    ctx->has_scene = true;
+}
+
+// PinholeCamera::commit, camera/pinhole.cpp:55-77
+void set_camera(btrc_ctx *ctx, const btrc_camera_desc *cd)
+{
+    if(!cd)
+        throw InvalidArg("camera is null");
+    const V3 eye = v3(cd->eye[0], cd->eye[1], cd->eye[2]);
+    const V3 dst = v3(cd->dst[0], cd->dst[1], cd->dst[2]);
+    const V3 up = v3(cd->up[0], cd->up[1], cd->up[2]);
+    const V3 forward = normalize(dst - eye);
+    const V3 ex = normalize(cross(forward, up));
+    const V3 ey = normalize(cross(forward, ex));
+    const float y_len = 2.0f * std::tan(0.5f * cd->fov_y_deg * PI / 180.0f);
+    const float x_len = cd->w_over_h * y_len;
+    const V3 lbc = eye + forward - 0.5f * (x_len * ex + y_len * ey);
+    const V3 fx = x_len * ex;
+    const V3 fy = y_len * ey;
+    CameraConsts c;
+    c.eye[0] = eye.x; c.eye[1] = eye.y; c.eye[2] = eye.z;
+    c.lbc[0] = lbc.x; c.lbc[1] = lbc.y; c.lbc[2] = lbc.z;
+    c.film_x[0] = fx.x; c.film_x[1] = fx.y; c.film_x[2] = fx.z;
+    c.film_y[0] = fy.x; c.film_y[1] = fy.y; c.film_y[2] = fy.z;
+    ctx->camera = c;
+    ctx->has_camera = true;
+}
+
+struct StageTimer
+{
+    bool enabled = false;
+    hipStream_t stream = nullptr;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> events[STAGE_COUNT];
+    std::vector<hipEvent_t> pool;
+    size_t used = 0;
+
+    hipEvent_t get()
+    {
+        if(used == pool.size())
+        {
+            hipEvent_t e;
+            BTRC_HIP_CHECK(hipEventCreate(&e));
+            pool.push_back(e);
+        }
+        return pool[used++];
+    }
+    template<typename F>
+    void run(int stage, F &&f)
+    {
+        if(!enabled)
+        {
+            f();
+            return;
+        }
+        hipEvent_t a = get(), b = get();
+        BTRC_HIP_CHECK(hipEventRecord(a, stream));
+        f();
+        BTRC_HIP_CHECK(hipEventRecord(b, stream));
+        events[stage].push_back({ a, b });
+    }
+    void collect(double *ms, uint64_t *launches)
+    {
+        for(int s = 0; s < STAGE_COUNT; ++s)
+        {
+            double total = 0;
+            for(auto &p : events[s])
+            {
+                float t = 0;
+                BTRC_HIP_CHECK(hipEventElapsedTime(&t, p.first, p.second));
+                total += t;
+            }
+            ms[s] = total;
+            launches[s] = events[s].size();
+        }
+    }
+    ~StageTimer()
+    {
+        for(auto e : pool)
+            (void)hipEventDestroy(e);
+    }
+};
+
+void render(btrc_ctx *ctx, const btrc_render_params *p, btrc_render_stats *stats, int32_t capture_bounces,
+            btrc_ray *capture, int64_t capture_max, int64_t *capture_count)
+{
+    if(!p)
+        throw InvalidArg("params is null");
+    if(!ctx->has_scene)
+        throw StateError("render before btrc_hip_upload_scene");
+    if(!ctx->has_camera)
+        throw StateError("render before btrc_hip_set_camera");
+    if(p->width <= 0 || p->height <= 0 || p->spp <= 0 || p->state_count <= 0)
+        throw InvalidArg("width, height, spp and state_count must be positive");
+    if(p->min_depth < 0 || p->max_depth < 0)
+        throw InvalidArg("depths must be non-negative");
+    const uint64_t pixels = (uint64_t)p->width * (uint64_t)p->height;
+    const uint64_t total = pixels * (uint64_t)p->spp;
+    if(pixels > 0x7fffffffull || (uint64_t)p->spp * 31104ull > 0x7fffffffull)
+        throw InvalidArg("film or spp too large");
+
+    hipStream_t s = ctx->stream;
+    const size_t P = (size_t)p->state_count;
+    if(ctx->pool_size != P)
+    {
+        ctx->pool[0].alloc(P);
+        ctx->pool[1].alloc(P);
+        ctx->hit_flag.alloc(P);
+        ctx->hit_tuv.alloc(P);
+        ctx->sh_o.alloc(P);
+        ctx->sh_d.alloc(P);
+        ctx->sh_beta.alloc(P);
+        ctx->pool_size = P;
+    }
+    ctx->counters.alloc(1);
+    if(!ctx->host_counters)
+        BTRC_HIP_CHECK(hipHostMalloc((void **)&ctx->host_counters, sizeof(WaveCounters), hipHostMallocDefault));
+    ctx->film.alloc(4 * pixels);
+    if(p->film_fp64)
+        ctx->film64.alloc(3 * pixels);
+    BTRC_HIP_CHECK(hipMemsetAsync(ctx->film.ptr, 0, 4 * pixels * sizeof(float), s));
+    if(p->film_fp64)
+        BTRC_HIP_CHECK(hipMemsetAsync(ctx->film64.ptr, 0, 3 * pixels * sizeof(double), s));
+    ctx->film_w = p->width;
+    ctx->film_h = p->height;
+    ctx->film_spp = p->spp;
+    ctx->film_has64 = p->film_fp64 != 0;
+
+    WaveCounters init{};
+    init.total = total;
+    *ctx->host_counters = init;
+    BTRC_HIP_CHECK(hipMemcpyAsync(ctx->counters.ptr, ctx->host_counters, sizeof(WaveCounters), hipMemcpyHostToDevice, s));
+
+    RenderConsts rc{};
+    rc.width = p->width;
+    rc.height = p->height;
+    rc.spp = p->spp;
+    rc.min_depth = p->min_depth;
+    rc.max_depth = p->max_depth;
+    rc.rr_threshold = p->rr_threshold;
+    rc.rr_cont_prob = p->rr_cont_prob;
+    rc.state_count = p->state_count;
+    rc.vol_prim_medium = 0;
+    rc.film_fp64 = p->film_fp64;
+    rc.halton = make_halton_consts(p->width, p->height);
+    rc.camera = ctx->camera;
+
+    const TraceScene ts = ctx->trace_scene();
+    const SceneView sv = ctx->scene_view();
+    FilmView fv{ ctx->film.ptr, p->film_fp64 ? ctx->film64.ptr : nullptr };
+    HitSOA hit{ ctx->hit_flag.ptr, ctx->hit_tuv.ptr };
+    ShadowSOA sh{ ctx->sh_o.ptr, ctx->sh_d.ptr, ctx->sh_beta.ptr };
+
+    const int grid256 = ctx->persistent_grid(256, (int64_t)P);
+    const int grid128 = ctx->persistent_grid(128, (int64_t)P);
+
+    StageTimer timer;
+    timer.enabled = p->time_stages != 0;
+    timer.stream = s;
+
+    ctx->stop = false;
+    int64_t captured = 0;
+    const auto t0 = std::chrono::steady_clock::now();
+    int cur = 0;
+    const int check_every = 4;
+    for(uint64_t iter = 0;; ++iter)
+    {
+        if(ctx->stop.load())
+            throw Stopped("render stopped");
+        PathSOA path = ctx->pool[cur].view();
+        PathSOA next = ctx->pool[cur ^ 1].view();
+        launch_plan(s, ctx->counters.ptr, p->state_count);
+        timer.run(STAGE_GENERATE, [&] { launch_generate(s, grid256, rc, ctx->counters.ptr, path); });
+        timer.run(STAGE_TRACE, [&] { launch_trace_extension(s, grid128, ctx->stack_size, ts, ctx->counters.ptr, path, hit); });
+        if(capture && (int64_t)iter < capture_bounces && captured < capture_max)
+        {
+            BTRC_HIP_CHECK(hipMemcpyAsync(ctx->host_counters, ctx->counters.ptr, sizeof(WaveCounters), hipMemcpyDeviceToHost, s));
+            BTRC_HIP_CHECK(hipStreamSynchronize(s));
+            const int64_t n = std::min<int64_t>(ctx->host_counters->active, capture_max - captured);
+            std::vector<float4> o(n), d(n);
+            if(n > 0)
+            {
+                BTRC_HIP_CHECK(hipMemcpy(o.data(), path.ray_o, n * sizeof(float4), hipMemcpyDeviceToHost));
+                BTRC_HIP_CHECK(hipMemcpy(d.data(), path.ray_d, n * sizeof(float4), hipMemcpyDeviceToHost));
+            }
+            for(int64_t k = 0; k < n; ++k)
+            {
+                btrc_ray &r = capture[captured + k];
+                r.o[0] = o[k].x; r.o[1] = o[k].y; r.o[2] = o[k].z; r.tmin = 0.0f;
+                r.d[0] = d[k].x; r.d[1] = d[k].y; r.d[2] = d[k].z; r.tmax = d[k].w;
+            }
+            captured += n;
+        }
+        timer.run(STAGE_SHADE, [&] { launch_shade(s, grid256, rc, sv, ctx->counters.ptr, path, hit, next, sh, fv); });
+        timer.run(STAGE_SHADOW, [&] { launch_shadow(s, grid128, ctx->stack_size, ts, ctx->counters.ptr, sh, fv, p->width); });
+        launch_end_iteration(s, ctx->counters.ptr);
+        BTRC_HIP_CHECK(hipGetLastError());
+        cur ^= 1;
+        if((iter + 1) % check_every == 0)
+        {
+            BTRC_HIP_CHECK(hipMemcpyAsync(ctx->host_counters, ctx->counters.ptr, sizeof(WaveCounters), hipMemcpyDeviceToHost, s));
+            BTRC_HIP_CHECK(hipStreamSynchronize(s));
+            const WaveCounters &hc = *ctx->host_counters;
+            if(hc.generated == hc.total && hc.active == 0)
+                break;
+        }
+    }
+    BTRC_HIP_CHECK(hipStreamSynchronize(s));
+    const auto t1 = std::chrono::steady_clock::now();
+    if(capture_count)
+        *capture_count = captured;
+    if(stats)
+    {
+        std::memset(stats, 0, sizeof(*stats));
+        const WaveCounters &hc = *ctx->host_counters;
+        stats->seconds = std::chrono::duration<double>(t1 - t0).count();
+        stats->extension_rays = hc.ext_rays;
+        stats->shadow_rays = hc.shadow_rays;
+        stats->paths = hc.generated;
+        stats->iterations = hc.iterations;
+        timer.collect(stats->stage_ms, stats->stage_launches);
+    }
+}
+
+void film_readback(btrc_ctx *ctx, float *rgba, double *rgb64)
+{
+    if(ctx->film_w <= 0)
+        throw StateError("no film: render first");
+    const int32_t pixels = ctx->film_w * ctx->film_h;
+    hipStream_t s = ctx->stream;
+    if(rgba)
+    {
+        ctx->resolved.alloc(pixels);
+        launch_resolve(s, FilmView{ ctx->film.ptr, nullptr }, pixels, 1.0f / (float)ctx->film_spp, ctx->resolved.ptr);
+        BTRC_HIP_CHECK(hipGetLastError());
+        BTRC_HIP_CHECK(hipMemcpyAsync(rgba, ctx->resolved.ptr, pixels * sizeof(float4), hipMemcpyDeviceToHost, s));
+    }
+    if(rgb64)
+    {
+        if(!ctx->film_has64)
+            throw StateError("the last render had no FP64 film");
+        BTRC_HIP_CHECK(hipMemcpyAsync(rgb64, ctx->film64.ptr, 3 * (size_t)pixels * sizeof(double), hipMemcpyDeviceToHost, s));
+    }
+    BTRC_HIP_CHECK(hipStreamSynchronize(s));
+    if(rgb64)
+    {
+        const double inv = 1.0 / (double)ctx->film_spp;
+        for(size_t i = 0; i < 3 * (size_t)pixels; ++i)
+            rgb64[i] *= inv;
+    }
+}
+
+void check_batch_args(btrc_ctx *ctx, const void *rays, int64_t n, int32_t tri_test)
+{
+    if(!ctx->has_scene)
+        throw StateError("trace before btrc_hip_upload_scene");
+    if(n < 0 || (n > 0 && !rays))
+        throw InvalidArg("bad ray batch");
+    if(tri_test != BTRC_TRI_WATERTIGHT && tri_test != BTRC_TRI_MOLLER_TRUMBORE)
+        throw InvalidArg("unknown triangle test");
+}
+
+} // namespace
+
+extern "C" {
+
+int btrc_hip_create(int device, btrc_ctx **out)
+{
+    if(!out)
+        return BTRC_ERR_INVALID;
+    *out = nullptr;
+    int count = 0;
+    if(hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count)
+        return BTRC_ERR_HIP;
+    btrc_ctx *ctx = new(std::nothrow) btrc_ctx;
+    if(!ctx)
+        return BTRC_ERR_OOM;
+    ctx->device = device;
+    if(hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess)
+    {
+        delete ctx;
+        return BTRC_ERR_HIP;
+    }
+    hipDeviceProp_t prop;
+    if(hipGetDeviceProperties(&prop, device) == hipSuccess)
+    {
+        ctx->cu_count = prop.multiProcessorCount;
+        ctx->device_name = prop.gcnArchName;
+    }
+    *out = ctx;
+    return BTRC_OK;
+}
+
+void btrc_hip_destroy(btrc_ctx *ctx)
+{
+    if(!ctx)
+        return;
+    (void)hipSetDevice(ctx->device);
+    if(ctx->stream)
+        (void)hipStreamSynchronize(ctx->stream);
+    if(ctx->host_counters)
+        (void)hipHostFree(ctx->host_counters);
+    if(ctx->stream)
+        (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+const char *btrc_hip_last_error(const btrc_ctx *ctx)
+{
+    return ctx ? ctx->last_error.c_str() : "null context";
+}
+
+int btrc_hip_device_name(btrc_ctx *ctx, char *buf, size_t len)
+{
+    return guarded(ctx, [&] {
+        if(!buf || len == 0)
+            throw InvalidArg("bad buffer");
+        std::strncpy(buf, ctx->device_name.c_str(), len - 1);
+        buf[len - 1] = 0;
+    });
+}
+
+int btrc_hip_upload_scene(btrc_ctx *ctx, const btrc_scene_desc *scene)
+{
+    return guarded(ctx, [&] { upload_scene(ctx, scene); });
+}
+
+int btrc_hip_set_camera(btrc_ctx *ctx, const btrc_camera_desc *camera)
+{
+    return guarded(ctx, [&] { set_camera(ctx, camera); });
+}
+
+int btrc_hip_scene_info(btrc_ctx *ctx, int64_t *nodes, int32_t *depth, int32_t *stack_bound, int64_t *triangles)
+{
+    return guarded(ctx, [&] {
+        if(!ctx->has_scene)
+            throw StateError("no scene");
+        if(nodes) *nodes = ctx->node_count;
+        if(depth) *depth = ctx->bvh_depth;
+        if(stack_bound) *stack_bound = ctx->stack_size;
+        if(triangles) *triangles = ctx->triangle_count;
+    });
+}
+
+int btrc_hip_render(btrc_ctx *ctx, const btrc_render_params *params, btrc_render_stats *stats)
+{
+    return guarded(ctx, [&] { render(ctx, params, stats, 0, nullptr, 0, nullptr); });
+}
+
+int btrc_hip_capture_rays(btrc_ctx *ctx, const btrc_render_params *params, int32_t bounces,
+                          btrc_ray *rays, int64_t max_rays, int64_t *count)
+{
+    return guarded(ctx, [&] {
+        if(!rays || max_rays < 0 || bounces < 0)
+            throw InvalidArg("bad capture buffer");
+        render(ctx, params, nullptr, bounces, rays, max_rays, count);
+    });
+}
+
+int btrc_hip_film_readback(btrc_ctx *ctx, float *rgba, double *rgb64)
+{
+    return guarded(ctx, [&] { film_readback(ctx, rgba, rgb64); });
+}
+
+int btrc_hip_request_stop(btrc_ctx *ctx)
+{
+    if(!ctx)
+        return BTRC_ERR_INVALID;
+    ctx->stop = true;
+    return BTRC_OK;
+}
+
+int btrc_hip_trace_closest(btrc_ctx *ctx, const btrc_ray *rays, int64_t n, int32_t tri_test, btrc_hit *hits)
+{
+    return guarded(ctx, [&] {
+        check_batch_args(ctx, rays, n, tri_test);
+        if(n > 0 && !hits)
+            throw InvalidArg("hits is null");
+        if(n == 0)
+            return;
+        DevBuf<float4> r;
+        DevBuf<uint32_t> o;
+        r.upload(reinterpret_cast<const float4 *>(rays), 2 * (size_t)n, ctx->stream);
+        o.alloc(5 * (size_t)n);
+        launch_trace_batch(ctx->stream, ctx->persistent_grid(128, n), ctx->stack_size, ctx->trace_scene(),
+                           tri_test, false, r.ptr, n, o.ptr);
+        BTRC_HIP_CHECK(hipGetLastError());
+        BTRC_HIP_CHECK(hipMemcpyAsync(hits, o.ptr, 5 * (size_t)n * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+        BTRC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int btrc_hip_trace_any(btrc_ctx *ctx, const btrc_ray *rays, int64_t n, int32_t tri_test, uint8_t *occluded)
+{
+    return guarded(ctx, [&] {
+        check_batch_args(ctx, rays, n, tri_test);
+        if(n > 0 && !occluded)
+            throw InvalidArg("occluded is null");
+        if(n == 0)
+            return;
+        DevBuf<float4> r;
+        DevBuf<uint32_t> o;
+        r.upload(reinterpret_cast<const float4 *>(rays), 2 * (size_t)n, ctx->stream);
+        o.alloc((size_t)n);
+        launch_trace_batch(ctx->stream, ctx->persistent_grid(128, n), ctx->stack_size, ctx->trace_scene(),
+                           tri_test, true, r.ptr, n, o.ptr);
+        BTRC_HIP_CHECK(hipGetLastError());
+        std::vector<uint32_t> tmp(n);
+        BTRC_HIP_CHECK(hipMemcpyAsync(tmp.data(), o.ptr, (size_t)n * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+        BTRC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        for(int64_t i = 0; i < n; ++i)
+            occluded[i] = (uint8_t)tmp[i];
+    });
+}
+
+int btrc_hip_batch_upload(btrc_ctx *ctx, const btrc_ray *rays, int64_t n)
+{
+    return guarded(ctx, [&] {
+        if(n < 0 || (n > 0 && !rays))
+            throw InvalidArg("bad ray batch");
+        ctx->batch_rays.upload(reinterpret_cast<const float4 *>(rays), 2 * (size_t)n, ctx->stream);
+        ctx->batch_out.alloc(5 * (size_t)std::max<int64_t>(n, 1));
+        ctx->batch_n = n;
+        BTRC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int btrc_hip_batch_trace(btrc_ctx *ctx, int32_t tri_test, int32_t any_hit, int32_t repeats, double *kernel_ms)
+{
+    return guarded(ctx, [&] {
+        check_batch_args(ctx, ctx->batch_rays.ptr, ctx->batch_n, tri_test);
+        if(repeats <= 0)
+            throw InvalidArg("repeats must be positive");
+        hipEvent_t a, b;
+        BTRC_HIP_CHECK(hipEventCreate(&a));
+        BTRC_HIP_CHECK(hipEventCreate(&b));
+        double total = 0;
+        for(int32_t k = 0; k < repeats; ++k)
+        {
+            BTRC_HIP_CHECK(hipEventRecord(a, ctx->stream));
+            launch_trace_batch(ctx->stream, ctx->persistent_grid(128, ctx->batch_n), ctx->stack_size,
+                               ctx->trace_scene(), tri_test, any_hit != 0, ctx->batch_rays.ptr, ctx->batch_n,
+                               ctx->batch_out.ptr);
+            BTRC_HIP_CHECK(hipEventRecord(b, ctx->stream));
+            BTRC_HIP_CHECK(hipEventSynchronize(b));
+            float ms = 0;
+            BTRC_HIP_CHECK(hipEventElapsedTime(&ms, a, b));
+            total += ms;
+        }
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+        if(kernel_ms)
+            *kernel_ms = total / repeats;
+    });
+}
+
+int btrc_hip_batch_download(btrc_ctx *ctx, btrc_hit *hits)
+{
+    return guarded(ctx, [&] {
+        if(!hits)
+            throw InvalidArg("hits is null");
+        BTRC_HIP_CHECK(hipMemcpy(hits, ctx->batch_out.ptr, 5 * (size_t)ctx->batch_n * sizeof(uint32_t),
+                                 hipMemcpyDeviceToHost));
+    });
+}
+
+int btrc_hip_eval_halton(btrc_ctx *ctx, int32_t width, int32_t height, const uint32_t *px, const uint32_t *py,
+                         const int32_t *sample_index, int32_t n, int32_t dims, float *out)
+{
+    return guarded(ctx, [&] {
+        if(n < 0 || dims <= 0 || width <= 0 || height <= 0 || (n > 0 && (!px || !py || !sample_index || !out)))
+            throw InvalidArg("bad halton query");
+        if(n == 0)
+            return;
+        DevBuf<uint32_t> dx, dy;
+        DevBuf<int32_t> ds;
+        DevBuf<float> dout;
+        dx.upload(px, n, ctx->stream);
+        dy.upload(py, n, ctx->stream);
+        ds.upload(sample_index, n, ctx->stream);
+        dout.alloc((size_t)n * dims);
+        launch_halton_eval(ctx->stream, make_halton_consts(width, height), dx.ptr, dy.ptr, ds.ptr, n, dims, dout.ptr);
+        BTRC_HIP_CHECK(hipGetLastError());
+        BTRC_HIP_CHECK(hipMemcpyAsync(out, dout.ptr, (size_t)n * dims * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+        BTRC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int btrc_hip_eval_detmath(btrc_ctx *ctx, int32_t fn, const float *x, const float *y, int32_t n, float *out)
+{
+    return guarded(ctx, [&] {
+        if(n < 0 || fn < 0 || fn > 6 || (n > 0 && (!x || !y || !out)))
+            throw InvalidArg("bad detmath query");
+        if(n == 0)
+            return;
+        DevBuf<float> dx, dy, dout;
+        dx.upload(x, n, ctx->stream);
+        dy.upload(y, n, ctx->stream);
+        dout.alloc(n);
+        launch_detmath_eval(ctx->stream, fn, dx.ptr, dy.ptr, n, dout.ptr);
+        BTRC_HIP_CHECK(hipGetLastError());
+        BTRC_HIP_CHECK(hipMemcpyAsync(out, dout.ptr, (size_t)n * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+        BTRC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+} // extern "C"

@@ -1,0 +1,116 @@
+// Device data layout of the wavefront path tracer and the launch wrappers of
+// its stages. Mirrors the reference's SOA pools (renderer/wavefront/soa.h:
+// 8-147) with compact records, and replaces its per-bounce host readback of
+// the queue counters (renderer/wavefront.cpp:283-284) with counters that stay
+// in device memory: every stage reads its own work count on the GPU.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "../core/sampler.h"
+#include "../core/scene.h"
+#include "trace.h"
+
+namespace btrc {
+
+// Path state pool, one side of the ping-pong pair.
+struct PathSOA
+{
+    float4 *ray_o;    // origin.xyz, medium id bits
+    float4 *ray_d;    // direction.xyz, t1
+    float4 *beta;     // throughput.xyz, depth bits
+    float4 *rad;      // path radiance.xyz, unused
+    float4 *bsdf_le;  // beta_le.xyz, bsdf pdf (negative: delta lobe)
+    uint4 *smp;       // pixel index, Halton dimension, Halton index lo, hi
+};
+
+struct HitSOA
+{
+    uint32_t *flag;   // bit 30: intersected, bit 31: scattered, low 30 bits: instance
+    float4 *tuv;      // t, b1, b2, prim bits
+};
+
+struct ShadowSOA
+{
+    float4 *o;        // origin.xyz, pixel index bits
+    float4 *d;        // direction.xyz, t1
+    float4 *beta;     // beta_li.xyz, medium id bits
+};
+
+constexpr uint32_t PATH_FLAG_HAS_INTERSECTION = 1u << 30; // wavefront/common.h:12
+constexpr uint32_t PATH_FLAG_HAS_SCATTERING = 1u << 31;
+constexpr uint32_t PATH_FLAG_INSTANCE_ID_MASK = ~0u >> 2;
+
+struct WaveCounters
+{
+    int32_t active;       // live states in the current pool side
+    int32_t next_active;  // states pushed into the other side this bounce
+    int32_t shadow;       // shadow rays pushed this bounce
+    int32_t gen_count;    // states generated this iteration
+    int32_t gen_offset;   // first slot of the generated states
+    int32_t pad0[3];
+    uint64_t generated;   // paths generated so far
+    uint64_t total;       // spp * width * height
+    uint64_t gen_base;    // path index of the first generated state
+    uint64_t ext_rays;    // extension rays traced
+    uint64_t shadow_rays; // shadow rays traced
+    uint64_t iterations;
+};
+
+struct CameraConsts
+{
+    float eye[3];
+    float lbc[3];    // left_bottom_corner, pinhole.cpp:79
+    float film_x[3];
+    float film_y[3];
+};
+
+struct RenderConsts
+{
+    int32_t width, height, spp;
+    int32_t min_depth, max_depth;
+    float rr_threshold, rr_cont_prob;
+    int32_t state_count;
+    uint32_t vol_prim_medium;
+    int32_t film_fp64;
+    HaltonConsts halton;
+    CameraConsts camera;
+};
+
+struct FilmView
+{
+    float *rgb;      // 4 floats per pixel (radiance xyz, unused)
+    double *rgb64;   // 3 doubles per pixel, or null
+};
+
+// kernel timing buckets
+enum : int
+{
+    STAGE_GENERATE = 0,
+    STAGE_TRACE = 1,
+    STAGE_SHADE = 2,
+    STAGE_SHADOW = 3,
+    STAGE_COUNT = 4,
+};
+
+void launch_plan(hipStream_t s, WaveCounters *c, int32_t state_count);
+void launch_generate(hipStream_t s, int grid, const RenderConsts &rc, const WaveCounters *c, PathSOA path);
+void launch_trace_extension(hipStream_t s, int grid, int stack, const TraceScene &ts, const WaveCounters *c,
+                            PathSOA path, HitSOA hit);
+void launch_shade(hipStream_t s, int grid, const RenderConsts &rc, const SceneView &sv, WaveCounters *c,
+                  PathSOA path, HitSOA hit, PathSOA next, ShadowSOA shadow, FilmView film);
+void launch_shadow(hipStream_t s, int grid, int stack, const TraceScene &ts, const WaveCounters *c,
+                   ShadowSOA shadow, FilmView film, int32_t width);
+void launch_end_iteration(hipStream_t s, WaveCounters *c);
+void launch_resolve(hipStream_t s, const FilmView &film, int32_t pixels, float inv_weight, float4 *out);
+
+// ray batches of the traversal entry points: o.xyz/tmin and d.xyz/tmax
+void launch_trace_batch(hipStream_t s, int grid, int stack, const TraceScene &ts, int tri_test, bool any_hit,
+                        const float4 *rays, int64_t n, uint32_t *out);
+
+// test hooks
+void launch_halton_eval(hipStream_t s, const HaltonConsts &hc, const uint32_t *px, const uint32_t *py,
+                        const int32_t *sample_index, int32_t n, int32_t dims, float *out);
+void launch_detmath_eval(hipStream_t s, int32_t fn, const float *x, const float *y, int32_t n, float *out);
+
+} // namespace btrc

@@ -1,0 +1,499 @@
+// Wavefront path-tracing stages for gfx950.
+//
+// Stage semantics follow the reference kernels: generate_kernel
+// (renderer/wavefront/generate.cpp:33-87, uniform mode), the OptiX trace
+// programs (trace.cpp:38-78), shade_kernel (shade.cpp:29-236) and the shadow
+// programs (shadow.cpp:24-76). Work counts are read from device memory and
+// every kernel is a grid-stride loop over a persistent grid, so the host
+// launches a fixed sequence per bounce and never waits on the GPU between
+// bounces. Queue pushes are aggregated per wavefront: one atomic per wave.
+#include "wavefront.h"
+
+namespace btrc {
+
+__constant__ PrimeTable c_primes = make_prime_table();
+
+namespace {
+
+__device__ __forceinline__ int lane_id() { return (int)__lane_id(); }
+
+// Wave-aggregated slot reservation: every lane of the wavefront calls this
+// with its predicate; lanes with pred receive consecutive slots.
+__device__ __forceinline__ int32_t wave_push(int32_t *counter, bool pred)
+{
+    const uint64_t mask = __ballot(pred);
+    if(mask == 0)
+        return -1;
+    const int leader = __ffsll((unsigned long long)mask) - 1;
+    int32_t base = 0;
+    if(lane_id() == leader)
+        base = atomicAdd(counter, (int32_t)__popcll(mask));
+    base = __shfl(base, leader);
+    const uint64_t below = mask & ((1ull << lane_id()) - 1ull);
+    return pred ? base + (int32_t)__popcll(below) : -1;
+}
+
+__device__ __forceinline__ void film_splat(const FilmView &film, uint32_t pixel, V3 v)
+{
+    // Film::splat_atomic drops non-finite values (core/film.cpp:145-163)
+    if(!isfinite3(v))
+        return;
+    float *p = film.rgb + 4 * (size_t)pixel;
+    atomicAdd(p + 0, v.x);
+    atomicAdd(p + 1, v.y);
+    atomicAdd(p + 2, v.z);
+    if(film.rgb64)
+    {
+        double *q = film.rgb64 + 3 * (size_t)pixel;
+        atomicAdd(q + 0, (double)v.x);
+        atomicAdd(q + 1, (double)v.y);
+        atomicAdd(q + 2, (double)v.z);
+    }
+}
+
+__device__ __forceinline__ V3 xyz(float4 a) { return v3(a.x, a.y, a.z); }
+__device__ __forceinline__ float4 f4(V3 a, float w) { return make_float4(a.x, a.y, a.z, w); }
+
+__device__ __forceinline__ HaltonState load_sampler(uint4 s)
+{
+    HaltonState h;
+    h.dimension = s.y;
+    h.index = (uint64_t)s.z | ((uint64_t)s.w << 32);
+    return h;
+}
+
+__device__ __forceinline__ uint4 pack_sampler(uint32_t pixel, const HaltonState &h)
+{
+    return make_uint4(pixel, h.dimension, (uint32_t)h.index, (uint32_t)(h.index >> 32));
+}
+
+// ---------------------------------------------------------------- plan
+
+// GeneratePipeline::generate's refill rule (generate.cpp:140-176), evaluated
+// on the device: refill when at least half of the pool is free or when all
+// remaining paths fit.
+__global__ void k_plan(WaveCounters *c, int32_t state_count)
+{
+    const int32_t active = c->active;
+    const int64_t rest = (int64_t)state_count - active;
+    int64_t fresh = 0;
+    if(c->generated < c->total && rest > 0)
+    {
+        const int64_t unfinished = (int64_t)(c->total - c->generated);
+        if(!(unfinished > rest && rest + rest < state_count))
+            fresh = unfinished < rest ? unfinished : rest;
+    }
+    c->gen_offset = active;
+    c->gen_count = (int32_t)fresh;
+    c->gen_base = c->generated;
+    c->generated += (uint64_t)fresh;
+    c->active = active + (int32_t)fresh;
+    c->ext_rays += (uint64_t)(active + fresh);
+}
+
+// ---------------------------------------------------------------- generate
+
+__global__ void __launch_bounds__(256) k_generate(RenderConsts rc, const WaveCounters *c, PathSOA path)
+{
+    const int32_t n = c->gen_count;
+    const int32_t offset = c->gen_offset;
+    const uint64_t base = c->gen_base;
+    const uint64_t pixels = (uint64_t)rc.width * (uint64_t)rc.height;
+    for(int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    {
+        const uint64_t path_index = base + (uint64_t)i;
+        const uint32_t pixel_index = (uint32_t)(path_index % pixels);
+        const int32_t sample_index = (int32_t)(path_index / pixels);
+        const uint32_t px = pixel_index % (uint32_t)rc.width;
+        const uint32_t py = pixel_index / (uint32_t)rc.width;
+
+        HaltonState hs = halton_start(rc.halton, px, py, sample_index);
+        // BoxFilter::sample, film_filter/box.cpp:5-9
+        const float fx = halton_get1d(c_primes.p, hs) - 0.5f;
+        const float fy = halton_get1d(c_primes.p, hs) - 0.5f;
+        const float film_x = ((float)px + 0.5f + fx) / (float)rc.width;
+        const float film_y = ((float)py + 0.5f + fy) / (float)rc.height;
+        (void)halton_get1d(c_primes.p, hs); // time sample, generate.cpp:70
+
+        // PinholeCamera::generate_ray_inline, camera/pinhole.cpp:84-99
+        const CameraConsts &cam = rc.camera;
+        const V3 x = film_x * v3(cam.film_x[0], cam.film_x[1], cam.film_x[2]);
+        const V3 y = film_y * v3(cam.film_y[0], cam.film_y[1], cam.film_y[2]);
+        const V3 dst = v3(cam.lbc[0], cam.lbc[1], cam.lbc[2]) + x + y;
+        const V3 eye = v3(cam.eye[0], cam.eye[1], cam.eye[2]);
+        const V3 dir = normalize(dst - eye);
+
+        const int32_t s = offset + i;
+        path.ray_o[s] = f4(eye, __uint_as_float(rc.vol_prim_medium));
+        path.ray_d[s] = f4(dir, MAX_FLOAT);
+        path.beta[s] = make_float4(1.0f, 1.0f, 1.0f, __int_as_float(0));
+        path.rad[s] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+        path.bsdf_le[s] = make_float4(1.0f, 1.0f, 1.0f, -1.0f);
+        path.smp[s] = pack_sampler(pixel_index, hs);
+    }
+}
+
+// ---------------------------------------------------------------- trace
+
+__global__ void __launch_bounds__(128) k_trace_extension(TraceScene ts, const WaveCounters *c, PathSOA path, HitSOA hit)
+{
+    extern __shared__ int32_t s_stack[];
+    int32_t *stack = s_stack + threadIdx.x;
+    const int32_t n = c->active;
+    for(int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    {
+        const float4 o = path.ray_o[i];
+        const float4 d = path.ray_d[i];
+        const HitRecord h = trace_ray<TRI_WATERTIGHT, false>(ts, stack, blockDim.x, xyz(o), xyz(d), 0.0f, d.w);
+        if(h.gid != 0xffffffffu)
+        {
+            hit.flag[i] = PATH_FLAG_HAS_INTERSECTION | h.inst;
+            hit.tuv[i] = make_float4(h.t, h.b1, h.b2, __uint_as_float(h.prim));
+        }
+        else
+            hit.flag[i] = 0;
+    }
+}
+
+// ---------------------------------------------------------------- shade
+
+__global__ void __launch_bounds__(256) k_shade(
+    RenderConsts rc, SceneView sv, WaveCounters *c, PathSOA path, HitSOA hit,
+    PathSOA next, ShadowSOA shadow, FilmView film)
+{
+    const int32_t n = c->active;
+    const int32_t stride = gridDim.x * blockDim.x;
+    for(int32_t base = blockIdx.x * blockDim.x; base < n; base += stride)
+    {
+        const int32_t i = base + (int32_t)threadIdx.x;
+        bool push_next = false, push_shadow = false;
+        // values carried to the pushes
+        float4 nx_o, nx_d, nx_beta, nx_rad, nx_le, sh_o, sh_d, sh_beta;
+        uint4 nx_smp;
+
+        if(i < n)
+        {
+            const uint32_t flag = hit.flag[i];
+            const float4 ro = path.ray_o[i];
+            const float4 rd = path.ray_d[i];
+            const float4 beta_depth = path.beta[i];
+            const float4 rad4 = path.rad[i];
+            const float4 le4 = path.bsdf_le[i];
+            const uint4 smp = path.smp[i];
+            const uint32_t pixel = smp.x;
+            HaltonState hs = load_sampler(smp);
+
+            const V3 o = xyz(ro), d = xyz(rd);
+            V3 beta = xyz(beta_depth);
+            const int32_t depth = __float_as_int(beta_depth.w);
+            V3 path_rad = xyz(rad4);
+            const V3 beta_le = xyz(le4);
+            const float bsdf_pdf = le4.w;
+
+            if(!(flag & PATH_FLAG_HAS_INTERSECTION))
+            {
+                V3 splat = eval_miss_le(sv, d, beta_le, bsdf_pdf);
+                if(!(flag & PATH_FLAG_HAS_SCATTERING))
+                    splat = splat + path_rad;
+                film_splat(film, pixel, splat);
+            }
+            else
+            {
+                const float4 tuv = hit.tuv[i];
+                const uint32_t inst_id = flag & PATH_FLAG_INSTANCE_ID_MASK;
+                const InstanceRec &inst = sv.instances[inst_id];
+                const SurfacePoint sp = get_hitinfo(sv, o, d, inst, __float_as_uint(tuv.w), tuv.x, tuv.y, tuv.z);
+
+                const V3 le_rad = handle_intersected_light(sv, o, d, sp, beta_le, bsdf_pdf, inst.light_id);
+                bool done = false;
+                if(flag & PATH_FLAG_HAS_SCATTERING)
+                {
+                    film_splat(film, pixel, le_rad);
+                    done = true;
+                }
+                if(!done)
+                {
+                    path_rad = path_rad + le_rad;
+                    // simple_russian_roulette, helper.cpp:67-97
+                    bool rr_exit = false;
+                    if(depth >= rc.min_depth)
+                    {
+                        if(depth >= rc.max_depth)
+                            rr_exit = true;
+                        else
+                        {
+                            const float u = halton_get1d(c_primes.p, hs);
+                            if(luminance(beta) < rc.rr_threshold)
+                            {
+                                if(u > rc.rr_cont_prob)
+                                    rr_exit = true;
+                                else
+                                    beta = beta / rc.rr_cont_prob;
+                            }
+                        }
+                    }
+                    if(rr_exit)
+                    {
+                        film_splat(film, pixel, path_rad);
+                        done = true;
+                    }
+                }
+                if(!done)
+                {
+                    float l0, l1, l2;
+                    halton_get3d(c_primes.p, hs, &l0, &l1, &l2);
+                    const float lsel = halton_get1d(c_primes.p, hs);
+                    ShadowSample ss = sample_surface_light_li(sv, sp.position, sp.frame.z, l0, l1, l2, lsel);
+
+                    const Shader shader = make_shader(sv.materials[inst.material_id], sp.frame, sp.interp_z);
+                    float s0, s1, s2;
+                    halton_get3d(c_primes.p, hs, &s0, &s1, &s2);
+                    BSDFSample bs;
+                    shader_sample(shader, -d, s0, s1, s2, bs);
+
+                    V3 shadow_f = v3s(0.0f);
+                    float shadow_pdf = 0.0f;
+                    if(ss.success)
+                    {
+                        shadow_f = shader_eval(shader, ss.d, -d);
+                        ss.success = !is_zero(shadow_f);
+                        if(ss.success)
+                            shadow_pdf = shader_pdf(shader, ss.d, -d);
+                    }
+                    if(ss.success)
+                    {
+                        // vacuum: the shadow ray keeps the medium of its side
+                        const bool shadow_out = dot(sp.frame.z, ss.d) > 0.0f;
+                        const bool last_out = dot(sp.frame.z, d) < 0.0f;
+                        const uint32_t med = shadow_out == last_out ? __float_as_uint(ro.w)
+                                           : (shadow_out ? inst.outer_medium : inst.inner_medium);
+                        const float cs = fabs_(dot(normalize(ss.d), sp.frame.z));
+                        const V3 beta_li = ss.li * beta * shadow_f * cs / (shadow_pdf + ss.light_pdf);
+                        push_shadow = true;
+                        sh_o = f4(ss.o, __uint_as_float(pixel));
+                        sh_d = f4(ss.d, ss.t1);
+                        sh_beta = f4(beta_li, __uint_as_float(med));
+                    }
+
+                    if(!is_zero(bs.bsdf))
+                    {
+                        const V3 dir = normalize(bs.dir);
+                        const float cs = fabs_(dot(dir, sp.frame.z));
+                        const V3 new_beta_le = beta * cs * bs.bsdf;
+                        beta = new_beta_le / bs.pdf;
+                        const V3 next_o = intersection_offset(sp.position, sp.frame.z, dir);
+                        const bool next_out = dot(sp.frame.z, dir) > 0.0f;
+                        const uint32_t med = next_out ? inst.outer_medium : inst.inner_medium;
+                        const float stored_pdf = bs.is_delta ? -bs.pdf : bs.pdf;
+                        push_next = true;
+                        nx_o = f4(next_o, __uint_as_float(med));
+                        nx_d = f4(dir, MAX_FLOAT);
+                        nx_beta = f4(beta, __int_as_float(depth + 1));
+                        nx_rad = f4(path_rad, 1.0f);
+                        nx_le = f4(new_beta_le, stored_pdf);
+                        nx_smp = pack_sampler(pixel, hs);
+                    }
+                    else
+                        film_splat(film, pixel, path_rad);
+                }
+            }
+        }
+
+        const int32_t si = wave_push(&c->shadow, push_shadow);
+        if(push_shadow)
+        {
+            shadow.o[si] = sh_o;
+            shadow.d[si] = sh_d;
+            shadow.beta[si] = sh_beta;
+        }
+        const int32_t ni = wave_push(&c->next_active, push_next);
+        if(push_next)
+        {
+            next.ray_o[ni] = nx_o;
+            next.ray_d[ni] = nx_d;
+            next.beta[ni] = nx_beta;
+            next.rad[ni] = nx_rad;
+            next.bsdf_le[ni] = nx_le;
+            next.smp[ni] = nx_smp;
+        }
+    }
+}
+
+// ---------------------------------------------------------------- shadow
+
+__global__ void __launch_bounds__(128) k_shadow(TraceScene ts, const WaveCounters *c, ShadowSOA shadow, FilmView film)
+{
+    extern __shared__ int32_t s_stack[];
+    int32_t *stack = s_stack + threadIdx.x;
+    const int32_t n = c->shadow;
+    for(int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    {
+        const float4 o = shadow.o[i];
+        const float4 d = shadow.d[i];
+        const HitRecord h = trace_ray<TRI_WATERTIGHT, true>(ts, stack, blockDim.x, xyz(o), xyz(d), 0.0f, d.w);
+        if(h.gid == 0xffffffffu)
+        {
+            // __miss__shadow, shadow.cpp:43-71; transmittance is 1 in vacuum
+            const float4 b = shadow.beta[i];
+            film_splat(film, __float_as_uint(o.w), xyz(b));
+        }
+    }
+}
+
+// ---------------------------------------------------------------- bookkeeping
+
+__global__ void k_end_iteration(WaveCounters *c)
+{
+    c->shadow_rays += (uint64_t)c->shadow;
+    c->active = c->next_active;
+    c->next_active = 0;
+    c->shadow = 0;
+    c->iterations += 1;
+}
+
+// preview resolve, wavefront/preview.cpp:28-51 (weight = samples per pixel)
+__global__ void k_resolve(FilmView film, int32_t pixels, float inv_weight, float4 *out)
+{
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if(i >= pixels)
+        return;
+    const float *p = film.rgb + 4 * (size_t)i;
+    out[i] = make_float4(p[0] * inv_weight, p[1] * inv_weight, p[2] * inv_weight, 1.0f);
+}
+
+// ---------------------------------------------------------------- ray batches
+
+template<int TRI, bool ANY>
+__global__ void __launch_bounds__(128) k_trace_batch(TraceScene ts, const float4 *rays, int64_t n, uint32_t *out)
+{
+    extern __shared__ int32_t s_stack[];
+    int32_t *stack = s_stack + threadIdx.x;
+    for(int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    {
+        const float4 a = rays[2 * i + 0];
+        const float4 b = rays[2 * i + 1];
+        const HitRecord h = trace_ray<TRI, ANY>(ts, stack, blockDim.x, xyz(a), xyz(b), a.w, b.w);
+        if(ANY)
+            out[i] = h.gid != 0xffffffffu ? 1u : 0u;
+        else
+        {
+            uint32_t *o = out + 5 * i;
+            const bool hitp = h.gid != 0xffffffffu;
+            o[0] = __float_as_uint(hitp ? h.t : b.w);
+            o[1] = hitp ? h.inst : 0xffffffffu;
+            o[2] = hitp ? h.prim : 0xffffffffu;
+            o[3] = __float_as_uint(h.b1);
+            o[4] = __float_as_uint(h.b2);
+        }
+    }
+}
+
+// ---------------------------------------------------------------- test hooks
+
+__global__ void k_halton_eval(HaltonConsts hc, const uint32_t *px, const uint32_t *py, const int32_t *si,
+                              int32_t n, int32_t dims, float *out)
+{
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if(i >= n)
+        return;
+    HaltonState hs = halton_start(hc, px[i], py[i], si[i]);
+    for(int32_t k = 0; k < dims; ++k)
+        out[(size_t)i * dims + k] = halton_get1d(c_primes.p, hs);
+}
+
+__global__ void k_detmath_eval(int32_t fn, const float *x, const float *y, int32_t n, float *out)
+{
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if(i >= n)
+        return;
+    const float a = x[i];
+    float r;
+    switch(fn)
+    {
+    case 0: r = btrc_detmath::sin(a); break;
+    case 1: r = btrc_detmath::cos(a); break;
+    case 2: r = btrc_detmath::log(a); break;
+    case 3: r = btrc_detmath::exp(a); break;
+    case 4: r = btrc_detmath::pow(a, y[i]); break;
+    case 5: r = sqrtf(a); break;
+    default: r = a / y[i]; break;
+    }
+    out[i] = r;
+}
+
+} // namespace
+
+// ---------------------------------------------------------------- launchers
+
+void launch_plan(hipStream_t s, WaveCounters *c, int32_t state_count)
+{
+    hipLaunchKernelGGL(k_plan, dim3(1), dim3(1), 0, s, c, state_count);
+}
+
+void launch_generate(hipStream_t s, int grid, const RenderConsts &rc, const WaveCounters *c, PathSOA path)
+{
+    hipLaunchKernelGGL(k_generate, dim3(grid), dim3(256), 0, s, rc, c, path);
+}
+
+void launch_trace_extension(hipStream_t s, int grid, int stack, const TraceScene &ts, const WaveCounters *c,
+                            PathSOA path, HitSOA hit)
+{
+    hipLaunchKernelGGL(k_trace_extension, dim3(grid), dim3(128), (size_t)stack * 128 * sizeof(int32_t), s,
+                       ts, c, path, hit);
+}
+
+void launch_shade(hipStream_t s, int grid, const RenderConsts &rc, const SceneView &sv, WaveCounters *c,
+                  PathSOA path, HitSOA hit, PathSOA next, ShadowSOA shadow, FilmView film)
+{
+    hipLaunchKernelGGL(k_shade, dim3(grid), dim3(256), 0, s, rc, sv, c, path, hit, next, shadow, film);
+}
+
+void launch_shadow(hipStream_t s, int grid, int stack, const TraceScene &ts, const WaveCounters *c,
+                   ShadowSOA shadow, FilmView film, int32_t)
+{
+    hipLaunchKernelGGL(k_shadow, dim3(grid), dim3(128), (size_t)stack * 128 * sizeof(int32_t), s,
+                       ts, c, shadow, film);
+}
+
+void launch_end_iteration(hipStream_t s, WaveCounters *c)
+{
+    hipLaunchKernelGGL(k_end_iteration, dim3(1), dim3(1), 0, s, c);
+}
+
+void launch_resolve(hipStream_t s, const FilmView &film, int32_t pixels, float inv_weight, float4 *out)
+{
+    hipLaunchKernelGGL(k_resolve, dim3((pixels + 255) / 256), dim3(256), 0, s, film, pixels, inv_weight, out);
+}
+
+void launch_trace_batch(hipStream_t s, int grid, int stack, const TraceScene &ts, int tri_test, bool any_hit,
+                        const float4 *rays, int64_t n, uint32_t *out)
+{
+    const size_t lds = (size_t)stack * 128 * sizeof(int32_t);
+    if(tri_test == TRI_WATERTIGHT)
+    {
+        if(any_hit)
+            hipLaunchKernelGGL((k_trace_batch<TRI_WATERTIGHT, true>), dim3(grid), dim3(128), lds, s, ts, rays, n, out);
+        else
+            hipLaunchKernelGGL((k_trace_batch<TRI_WATERTIGHT, false>), dim3(grid), dim3(128), lds, s, ts, rays, n, out);
+    }
+    else
+    {
+        if(any_hit)
+            hipLaunchKernelGGL((k_trace_batch<TRI_MOLLER_TRUMBORE, true>), dim3(grid), dim3(128), lds, s, ts, rays, n, out);
+        else
+            hipLaunchKernelGGL((k_trace_batch<TRI_MOLLER_TRUMBORE, false>), dim3(grid), dim3(128), lds, s, ts, rays, n, out);
+    }
+}
+
+void launch_halton_eval(hipStream_t s, const HaltonConsts &hc, const uint32_t *px, const uint32_t *py,
+                        const int32_t *sample_index, int32_t n, int32_t dims, float *out)
+{
+    hipLaunchKernelGGL(k_halton_eval, dim3((n + 127) / 128), dim3(128), 0, s, hc, px, py, sample_index, n, dims, out);
+}
+
+void launch_detmath_eval(hipStream_t s, int32_t fn, const float *x, const float *y, int32_t n, float *out)
+{
+    hipLaunchKernelGGL(k_detmath_eval, dim3((n + 255) / 256), dim3(256), 0, s, fn, x, y, n, out);
+}
+
+} // namespace btrc

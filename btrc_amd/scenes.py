@@ -1,0 +1,354 @@
+"""Scene description on the host and the seeded synthetic scenes the tracer
+is measured on.
+
+A `Scene` holds what Btrc's JSON scene factory produces for the render path
+(factory/scene.cpp:5-94): triangle-mesh geometries, materials, entities
+(geometry + material + local_to_world + optional mesh light) and an optional
+environment light. `Scene.to_desc()` packs it into the C structs of
+include/btrc_hip.h.
+
+Every random draw of the synthetic scenes comes from PCG32 (O'Neill 2014,
+XSH-RR output, 64-bit LCG state) with a fixed seed, so a scene is the same
+bits on every host.
+"""
+import ctypes as C
+import math
+
+import numpy as np
+
+from . import _abi
+
+# --------------------------------------------------------------------- PCG32
+
+
+class PCG32:
+    """PCG-XSH-RR 64/32 (M. E. O'Neill, "PCG: A Family of Simple Fast
+    Space-Efficient Statistically Good Algorithms for Random Number
+    Generation", 2014)."""
+
+    MULT = 6364136223846793005
+    MASK = (1 << 64) - 1
+
+    def __init__(self, seed, seq=54):
+        self.inc = ((seq << 1) | 1) & self.MASK
+        self.state = 0
+        self.next_u32()
+        self.state = (self.state + seed) & self.MASK
+        self.next_u32()
+
+    def next_u32(self):
+        old = self.state
+        self.state = (old * self.MULT + self.inc) & self.MASK
+        xorshifted = (((old >> 18) ^ old) >> 27) & 0xFFFFFFFF
+        rot = old >> 59
+        return ((xorshifted >> rot) | (xorshifted << ((-rot) & 31))) & 0xFFFFFFFF
+
+    def uniform(self):
+        """float32 in [0, 1): the top 24 bits of one draw"""
+        return np.float32((self.next_u32() >> 8) * (1.0 / 16777216.0))
+
+    def range(self, lo, hi):
+        return np.float32(lo + (hi - lo) * float(self.uniform()))
+
+
+# ------------------------------------------------------------ transforms
+
+
+def translate(x, y, z):
+    m = np.eye(4)
+    m[:3, 3] = (x, y, z)
+    return m
+
+
+def scale(s):
+    return np.diag([s, s, s, 1.0])
+
+
+def rotate_y(rad):
+    c, s = math.cos(rad), math.sin(rad)
+    m = np.eye(4)
+    m[0, 0], m[0, 2], m[2, 0], m[2, 2] = c, s, -s, c
+    return m
+
+
+def quat_matrix(q):
+    w, x, y, z = q
+    m = np.eye(4)
+    m[:3, :3] = [[1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y)],
+                 [2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x)],
+                 [2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)]]
+    return m
+
+
+def uniform_rotation(rng):
+    """Shoemake's uniform random unit quaternion ("Uniform random rotations",
+    Graphics Gems III, 1992) as a 4x4 matrix."""
+    u1, u2, u3 = float(rng.uniform()), float(rng.uniform()), float(rng.uniform())
+    a, b = math.sqrt(1 - u1), math.sqrt(u1)
+    q = (b * math.cos(2 * math.pi * u3), a * math.sin(2 * math.pi * u2),
+         a * math.cos(2 * math.pi * u2), b * math.sin(2 * math.pi * u3))
+    return quat_matrix(q)
+
+
+# ------------------------------------------------------------ scene model
+
+
+class Scene:
+    def __init__(self):
+        self.geometries = []   # dicts: positions (n,9) f32, normals (n,9) or None, uvs (n,6) or None
+        self.materials = []    # MaterialDesc kwargs
+        self.instances = []    # dicts
+        self.envir = None      # (up, lower, upper)
+
+    def add_geometry(self, positions, normals=None, uvs=None):
+        positions = np.ascontiguousarray(positions, dtype=np.float32).reshape(-1, 9)
+        if normals is not None:
+            normals = np.ascontiguousarray(normals, dtype=np.float32).reshape(-1, 9)
+        if uvs is not None:
+            uvs = np.ascontiguousarray(uvs, dtype=np.float32).reshape(-1, 6)
+        self.geometries.append(dict(positions=positions, normals=normals, uvs=uvs))
+        return len(self.geometries) - 1
+
+    def add_diffuse(self, color, shadow_terminator=True):
+        self.materials.append(dict(type=_abi.MATERIAL_DIFFUSE, color=color,
+                                   shadow_terminator=shadow_terminator))
+        return len(self.materials) - 1
+
+    def add_disney(self, color, metallic, roughness, specular=1.0, specular_tint=0.0, anisotropic=0.0,
+                   sheen=0.0, sheen_tint=0.0, clearcoat=0.0, clearcoat_gloss=1.0, transmission=0.0,
+                   transmission_roughness=None, ior=1.5, shadow_terminator=True):
+        # defaults of DisneyMaterialCreator, builtin/material/disney.cpp:844-894
+        if transmission_roughness is None:
+            transmission_roughness = roughness
+        self.materials.append(dict(type=_abi.MATERIAL_DISNEY, color=color, metallic=metallic,
+                                   roughness=roughness, specular=specular, specular_tint=specular_tint,
+                                   anisotropic=anisotropic, sheen=sheen, sheen_tint=sheen_tint,
+                                   clearcoat=clearcoat, clearcoat_gloss=clearcoat_gloss,
+                                   transmission=transmission, transmission_roughness=transmission_roughness,
+                                   ior=ior, shadow_terminator=shadow_terminator))
+        return len(self.materials) - 1
+
+    def add_instance(self, geometry, material, local_to_world=None, light=None):
+        m = np.eye(4) if local_to_world is None else np.asarray(local_to_world, dtype=np.float64)
+        self.instances.append(dict(geometry=geometry, material=material, mat=m,
+                                   inv=np.linalg.inv(m), light=light))
+        return len(self.instances) - 1
+
+    def set_gradient_sky(self, up, lower, upper):
+        self.envir = (up, lower, upper)
+
+    @property
+    def triangle_count(self):
+        return sum(len(self.geometries[i["geometry"]]["positions"]) for i in self.instances)
+
+    def to_desc(self):
+        """Pack into _abi.SceneDesc; the returned holder keeps the arrays alive."""
+        holder = {}
+        gd = (_abi.GeometryDesc * max(1, len(self.geometries)))()
+        for k, g in enumerate(self.geometries):
+            fp = C.POINTER(C.c_float)
+            gd[k].triangle_count = len(g["positions"])
+            gd[k].positions = g["positions"].ctypes.data_as(fp)
+            gd[k].normals = g["normals"].ctypes.data_as(fp) if g["normals"] is not None else None
+            gd[k].tex_coords = g["uvs"].ctypes.data_as(fp) if g["uvs"] is not None else None
+        md = (_abi.MaterialDesc * max(1, len(self.materials)))()
+        for k, m in enumerate(self.materials):
+            d = md[k]
+            d.type = m["type"]
+            d.shadow_terminator = 1 if m.get("shadow_terminator", True) else 0
+            d.color = _abi.F3(*[float(v) for v in m["color"]])
+            for key in ("metallic", "roughness", "specular", "specular_tint", "anisotropic", "sheen",
+                        "sheen_tint", "clearcoat", "clearcoat_gloss", "transmission",
+                        "transmission_roughness", "ior"):
+                setattr(d, key, float(m.get(key, 0.0)))
+        idesc = (_abi.InstanceDesc * max(1, len(self.instances)))()
+        for k, inst in enumerate(self.instances):
+            d = idesc[k]
+            d.geometry = inst["geometry"]
+            d.material = inst["material"]
+            d.local_to_world = _abi.F16(*inst["mat"].astype(np.float32).reshape(-1).tolist())
+            d.world_to_local = _abi.F16(*inst["inv"].astype(np.float32).reshape(-1).tolist())
+            d.has_light = 1 if inst["light"] is not None else 0
+            if inst["light"] is not None:
+                d.light_intensity = _abi.F3(*[float(v) for v in inst["light"]])
+            d.inner_medium = -1
+            d.outer_medium = -1
+        sd = _abi.SceneDesc()
+        sd.geometry_count = len(self.geometries)
+        sd.geometries = gd
+        sd.material_count = len(self.materials)
+        sd.materials = md
+        sd.instance_count = len(self.instances)
+        sd.instances = idesc
+        if self.envir is not None:
+            up, lower, upper = self.envir
+            sd.envir_light.type = _abi.ENVIR_GRADIENT_SKY
+            sd.envir_light.up = _abi.F3(*map(float, up))
+            sd.envir_light.lower = _abi.F3(*map(float, lower))
+            sd.envir_light.upper = _abi.F3(*map(float, upper))
+        else:
+            sd.envir_light.type = _abi.ENVIR_NONE
+        holder.update(gd=gd, md=md, idesc=idesc, geoms=self.geometries)
+        return sd, holder
+
+
+class Camera:
+    """PinholeCamera parameters (builtin/camera/pinhole.cpp:101-116)."""
+
+    def __init__(self, eye, dst, up, fov_y_deg):
+        self.eye, self.dst, self.up, self.fov_y_deg = eye, dst, up, fov_y_deg
+
+    def to_desc(self, width, height):
+        d = _abi.CameraDesc()
+        d.eye = _abi.F3(*map(float, self.eye))
+        d.dst = _abi.F3(*map(float, self.dst))
+        d.up = _abi.F3(*map(float, self.up))
+        d.fov_y_deg = float(self.fov_y_deg)
+        # main.cpp:52: camera->set_w_over_h(width / height)
+        d.w_over_h = float(np.float32(width) / np.float32(height))
+        return d
+
+
+# ------------------------------------------------------------ meshes
+
+
+def quad(p0, p1, p2, p3):
+    """two triangles (p0, p1, p2), (p0, p2, p3); counter-clockwise seen from the front"""
+    return np.array([[*p0, *p1, *p2], [*p0, *p2, *p3]], dtype=np.float32)
+
+
+def box_faces(sx, sy, sz):
+    """the six faces of an axis-aligned box centred at the origin, outward normals"""
+    x, y, z = sx / 2, sy / 2, sz / 2
+    c = lambda a, b, d: (a * x, b * y, d * z)
+    return [
+        quad(c(-1, -1, 1), c(1, -1, 1), c(1, 1, 1), c(-1, 1, 1)),       # +z
+        quad(c(1, -1, -1), c(-1, -1, -1), c(-1, 1, -1), c(1, 1, -1)),    # -z
+        quad(c(1, -1, 1), c(1, -1, -1), c(1, 1, -1), c(1, 1, 1)),        # +x
+        quad(c(-1, -1, -1), c(-1, -1, 1), c(-1, 1, 1), c(-1, 1, -1)),    # -x
+        quad(c(-1, 1, 1), c(1, 1, 1), c(1, 1, -1), c(-1, 1, -1)),        # +y
+        quad(c(-1, -1, -1), c(1, -1, -1), c(1, -1, 1), c(-1, -1, 1)),    # -y
+    ]
+
+
+def uv_sphere(rng=None, slices=16, stacks=9, displacement=0.0):
+    """UV sphere of 2*slices + 2*slices*(stacks-2) triangles (256 for 16 x 9),
+    radius 1, optionally displaced radially by U[-displacement, displacement]
+    per vertex."""
+    verts = {}
+
+    def v(i, j):
+        key = (i % slices if 0 < j < stacks else 0, j)
+        if key not in verts:
+            th = math.pi * j / stacks
+            ph = 2 * math.pi * key[0] / slices
+            r = 1.0
+            if rng is not None and displacement > 0:
+                r += displacement * (2.0 * float(rng.uniform()) - 1.0)
+            verts[key] = (r * math.sin(th) * math.cos(ph), r * math.cos(th), r * math.sin(th) * math.sin(ph))
+        return verts[key]
+
+    tris = []
+    for j in range(stacks):
+        for i in range(slices):
+            a, b, c, d = v(i, j), v(i + 1, j), v(i + 1, j + 1), v(i, j + 1)
+            # counter-clockwise seen from outside: outward geometric normals
+            if j == 0:
+                tris.append([*a, *c, *d])       # top cap
+            elif j == stacks - 1:
+                tris.append([*a, *b, *d])       # bottom cap
+            else:
+                tris.append([*a, *c, *d])
+                tris.append([*a, *b, *c])
+    return np.array(tris, dtype=np.float32)
+
+
+# ------------------------------------------------------------ synthetic scenes
+
+
+def cornell(seed=1):
+    """Cornell-box-style scene: 5 axis-aligned wall quads + 2 rotated boxes
+    (34 triangles) + 1 quad mesh light (17, 12, 4); Lambert albedo per face
+    from U[0.1, 0.8]^3 (PCG32, seed 1)."""
+    rng = PCG32(seed)
+    s = Scene()
+
+    def albedo():
+        return [float(rng.range(0.1, 0.8)) for _ in range(3)]
+
+    walls = [
+        quad((-1, 0, 1), (1, 0, 1), (1, 0, -1), (-1, 0, -1)),        # floor, +y
+        quad((-1, 2, -1), (1, 2, -1), (1, 2, 1), (-1, 2, 1)),        # ceiling, -y
+        quad((-1, 0, -1), (1, 0, -1), (1, 2, -1), (-1, 2, -1)),      # back, +z
+        quad((-1, 0, 1), (-1, 0, -1), (-1, 2, -1), (-1, 2, 1)),      # left, +x
+        quad((1, 0, -1), (1, 0, 1), (1, 2, 1), (1, 2, -1)),          # right, -x
+    ]
+    for w in walls:
+        s.add_instance(s.add_geometry(w), s.add_diffuse(albedo()))
+    boxes = [((0.6, 0.6, 0.6), translate(0.35, 0.3, 0.3) @ rotate_y(math.radians(-18))),
+             ((0.6, 1.2, 0.6), translate(-0.35, 0.6, -0.3) @ rotate_y(math.radians(15)))]
+    for size, xf in boxes:
+        for f in box_faces(*size):
+            s.add_instance(s.add_geometry(f), s.add_diffuse(albedo()), xf)
+    light = quad((-0.25, 1.99, -0.25), (0.25, 1.99, -0.25), (0.25, 1.99, 0.25), (-0.25, 1.99, 0.25))
+    s.add_instance(s.add_geometry(light), s.add_diffuse([0.0, 0.0, 0.0]), light=(17.0, 12.0, 4.0))
+    cam = Camera((0.0, 1.0, 3.4), (0.0, 1.0, 0.0), (0.0, 1.0, 0.0), 40.0)
+    return s, cam
+
+
+def sphere_field(instances=4096, lights=64, seed=2, extent=100.0, envir=(0.5, 0.5, 0.5)):
+    """Procedural field: `instances` seeded placements (position U[0, extent]^3,
+    uniform rotation, scale U[0.5, 2]) of one 256-triangle displaced sphere,
+    each with its own Disney material (roughness U[0.05, 1], metallic
+    Bernoulli(0.3), specular 0.5, base colour U[0, 1]^3); `lights` sphere mesh
+    lights; a constant environment (gradient sky with upper == lower)."""
+    rng = PCG32(seed)
+    s = Scene()
+    g = s.add_geometry(uv_sphere(rng, displacement=0.15))
+    for _ in range(instances):
+        pos = [float(rng.range(0.0, extent)) for _ in range(3)]
+        rot = uniform_rotation(rng)
+        sc = float(rng.range(0.5, 2.0))
+        mat = s.add_disney(color=[float(rng.uniform()) for _ in range(3)],
+                           metallic=1.0 if float(rng.uniform()) < 0.3 else 0.0,
+                           roughness=float(rng.range(0.05, 1.0)), specular=0.5)
+        s.add_instance(g, mat, translate(*pos) @ rot @ scale(sc))
+    lg = s.add_geometry(uv_sphere())
+    black = s.add_diffuse([0.0, 0.0, 0.0])
+    for _ in range(lights):
+        pos = [float(rng.range(0.0, extent)) for _ in range(3)]
+        power = float(rng.range(5.0, 20.0))
+        tint = [float(rng.range(0.5, 1.0)) for _ in range(3)]
+        s.add_instance(lg, black, translate(*pos) @ scale(1.0), light=tuple(power * t for t in tint))
+    s.set_gradient_sky((0.0, 1.0, 0.0), envir, envir)
+    c = extent / 2
+    cam = Camera((c, c, -0.6 * extent), (c, c, c), (0.0, 1.0, 0.0), 60.0)
+    return s, cam
+
+
+def load_obj_triangles(path):
+    """Triangles of an OBJ file the way lib/bvh's test loader reads it
+    (lib/bvh/test/obj.hpp:62-96: fan triangulation (p0, p1, v), p1 <- v),
+    with strtof-exact vertex parsing."""
+    libc = C.CDLL(None)
+    libc.strtof.restype = C.c_float
+    libc.strtof.argtypes = [C.c_char_p, C.c_void_p]
+    verts, tris = [], []
+    with open(path, "r") as f:
+        for line in f:
+            t = line.strip()
+            if not t or t.startswith("#"):
+                continue
+            parts = t.split()
+            if parts[0] == "v":
+                verts.append([libc.strtof(p.encode(), None) for p in parts[1:4]])
+            elif parts[0] == "f":
+                idx = []
+                for p in parts[1:]:
+                    i = int(p.split("/")[0])
+                    idx.append(len(verts) + i if i < 0 else i - 1)
+                p0, p1 = verts[idx[0]], verts[idx[1]]
+                for k in idx[2:]:
+                    tris.append([*p0, *p1, *verts[k]])
+                    p1 = verts[k]
+    return np.array(tris, dtype=np.float32)

@@ -1,0 +1,212 @@
+/*
+ * btrc_hip.h - C ABI of the MI355X (gfx950) wavefront path tracer.
+ *
+ * This is the drop-in boundary for Btrc's render path. Everything behind it
+ * (device memory, HIP streams, BVH, kernels) is owned by the library; the
+ * caller passes plain host structs and pointers. No HIP or CUDA type appears
+ * here. Every function returns BTRC_OK (0) or a negative status; the message
+ * of the last failure on a context is available from btrc_hip_last_error.
+ *
+ * Reference interfaces replaced (paths relative to the Btrc source tree):
+ *   Renderer (core/renderer.h:13-59), registered as "wfpt"
+ *       (builtin/renderer/wavefront.h:62-69, wavefront.cpp:375-398)
+ *   Scene::commit (core/scene.cpp:92-239) and TriangleMesh::commit
+ *       (builtin/geometry/triangle_mesh.cpp:24-171)
+ *   PinholeCamera::commit (builtin/camera/pinhole.cpp:55-77)
+ *   TracePipeline::trace / ShadowPipeline::test
+ *       (builtin/renderer/wavefront/trace.cpp:127-141, shadow.cpp:143-157)
+ *   Renderer::stop_async (core/renderer.cpp:38-48)
+ */
+#ifndef BTRC_HIP_H
+#define BTRC_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct btrc_ctx btrc_ctx;
+
+enum
+{
+    BTRC_OK = 0,
+    BTRC_ERR_INVALID = -1, /* bad argument                                  */
+    BTRC_ERR_HIP = -2,     /* HIP runtime error (message has the HIP text)   */
+    BTRC_ERR_OOM = -3,     /* device or host allocation failed               */
+    BTRC_ERR_STATE = -4,   /* call out of order (e.g. render before scene)   */
+    BTRC_ERR_STOPPED = -5  /* render ended by btrc_hip_request_stop          */
+};
+
+enum
+{
+    BTRC_MATERIAL_DIFFUSE = 0, /* builtin/material/diffuse.cpp           */
+    BTRC_MATERIAL_DISNEY = 1   /* builtin/material/disney.cpp            */
+};
+
+enum
+{
+    BTRC_ENVIR_NONE = 0,
+    BTRC_ENVIR_GRADIENT_SKY = 1 /* builtin/light/gradient_sky.cpp        */
+};
+
+enum
+{
+    BTRC_TRI_WATERTIGHT = 0,       /* renderer's triangle test             */
+    BTRC_TRI_MOLLER_TRUMBORE = 1   /* lib/bvh Triangle::intersect order    */
+};
+
+/* A triangle mesh in its local space, one record per triangle corner as
+ * TriangleMeshLoader produces (utils/triangle_mesh_loader.cpp:12-146). */
+typedef struct btrc_geometry_desc
+{
+    int32_t triangle_count;
+    const float *positions;  /* 9 floats per triangle: a, b, c                 */
+    const float *normals;    /* 9 floats per triangle or NULL (geometric z)     */
+    const float *tex_coords; /* 6 floats per triangle or NULL (all zero)        */
+} btrc_geometry_desc;
+
+/* Constant-parameter material. Fields unused by a type are ignored.
+ * Disney defaults (disney.cpp:844-894): specular 1, ior 1.5, clearcoat_gloss 1,
+ * transmission_roughness = roughness, the rest 0. */
+typedef struct btrc_material_desc
+{
+    int32_t type;
+    int32_t shadow_terminator; /* default 1 */
+    float color[3];
+    float metallic, roughness, specular, specular_tint, anisotropic;
+    float sheen, sheen_tint, clearcoat, clearcoat_gloss;
+    float transmission, transmission_roughness, ior;
+} btrc_material_desc;
+
+/* Scene::Instance (core/scene.h:71-80) */
+typedef struct btrc_instance_desc
+{
+    int32_t geometry;
+    int32_t material;
+    float local_to_world[16]; /* row-major Transform3D::mat */
+    float world_to_local[16]; /* row-major Transform3D::inv */
+    int32_t has_light;        /* mesh_light on this instance */
+    float light_intensity[3];
+    int32_t inner_medium;     /* -1: none */
+    int32_t outer_medium;     /* -1: none */
+} btrc_instance_desc;
+
+typedef struct btrc_envir_light_desc
+{
+    int32_t type;   /* BTRC_ENVIR_* */
+    float up[3];
+    float lower[3];
+    float upper[3];
+} btrc_envir_light_desc;
+
+typedef struct btrc_scene_desc
+{
+    int32_t geometry_count;
+    const btrc_geometry_desc *geometries;
+    int32_t material_count;
+    const btrc_material_desc *materials;
+    int32_t instance_count;
+    const btrc_instance_desc *instances;
+    btrc_envir_light_desc envir_light;
+} btrc_scene_desc;
+
+/* PinholeCamera (builtin/camera/pinhole.cpp) */
+typedef struct btrc_camera_desc
+{
+    float eye[3];
+    float dst[3];
+    float up[3];
+    float fov_y_deg;
+    float w_over_h;
+} btrc_camera_desc;
+
+/* WavefrontPathTracer::Params (builtin/renderer/wavefront.h:13-27) */
+typedef struct btrc_render_params
+{
+    int32_t width, height;
+    int32_t spp;
+    int32_t min_depth, max_depth;
+    float rr_threshold, rr_cont_prob;
+    int32_t state_count;
+    int32_t film_fp64;   /* also accumulate an FP64 film */
+    int32_t time_stages; /* record per-stage kernel time with HIP events */
+    int32_t reserved[6];
+} btrc_render_params;
+
+typedef struct btrc_render_stats
+{
+    double seconds;          /* wall time of btrc_hip_render */
+    uint64_t extension_rays; /* closest-hit rays traced */
+    uint64_t shadow_rays;    /* visibility rays traced */
+    uint64_t paths;          /* camera paths completed */
+    uint64_t iterations;     /* wavefront bounces executed */
+    double stage_ms[4];      /* generate, trace, shade, shadow (time_stages) */
+    uint64_t stage_launches[4];
+} btrc_render_stats;
+
+/* A ray of a traversal batch and its closest hit */
+typedef struct btrc_ray
+{
+    float o[3];
+    float tmin;
+    float d[3];
+    float tmax;
+} btrc_ray;
+
+typedef struct btrc_hit
+{
+    float t;           /* tmax on a miss */
+    uint32_t instance; /* 0xffffffff on a miss */
+    uint32_t prim;     /* triangle index within the instance's geometry */
+    float u, v;        /* barycentric weights of vertices b and c */
+} btrc_hit;
+
+/* ---- context ---- */
+int btrc_hip_create(int device, btrc_ctx **out);
+void btrc_hip_destroy(btrc_ctx *ctx);
+const char *btrc_hip_last_error(const btrc_ctx *ctx);
+int btrc_hip_device_name(btrc_ctx *ctx, char *buf, size_t len);
+
+/* ---- scene and camera (Scene::commit, PinholeCamera::commit) ---- */
+int btrc_hip_upload_scene(btrc_ctx *ctx, const btrc_scene_desc *scene);
+int btrc_hip_set_camera(btrc_ctx *ctx, const btrc_camera_desc *camera);
+/* statistics of the built BVH: nodes, depth, stack bound, triangles */
+int btrc_hip_scene_info(btrc_ctx *ctx, int64_t *nodes, int32_t *depth, int32_t *stack_bound, int64_t *triangles);
+
+/* ---- rendering (WavefrontPathTracer::render) ---- */
+int btrc_hip_render(btrc_ctx *ctx, const btrc_render_params *params, btrc_render_stats *stats);
+/* value / weight per pixel as RGBA (alpha 1), row 0 at the top; rgb64 (3
+ * doubles per pixel, same normalisation) needs film_fp64, may be NULL */
+int btrc_hip_film_readback(btrc_ctx *ctx, float *rgba, double *rgb64);
+/* thread-safe; the running render returns BTRC_ERR_STOPPED */
+int btrc_hip_request_stop(btrc_ctx *ctx);
+
+/* ---- traversal on ray batches in host memory ---- */
+int btrc_hip_trace_closest(btrc_ctx *ctx, const btrc_ray *rays, int64_t n, int32_t tri_test, btrc_hit *hits);
+int btrc_hip_trace_any(btrc_ctx *ctx, const btrc_ray *rays, int64_t n, int32_t tri_test, uint8_t *occluded);
+
+/* ---- device-resident ray batch (throughput measurement) ---- */
+int btrc_hip_batch_upload(btrc_ctx *ctx, const btrc_ray *rays, int64_t n);
+/* traces the resident batch `repeats` times; *kernel_ms = mean kernel time */
+int btrc_hip_batch_trace(btrc_ctx *ctx, int32_t tri_test, int32_t any_hit, int32_t repeats, double *kernel_ms);
+int btrc_hip_batch_download(btrc_ctx *ctx, btrc_hit *hits);
+
+/* ---- rays of a render, for offline comparison ----
+ * Renders like btrc_hip_render and copies the extension rays of the first
+ * `bounces` bounces of the first pool wave (at most max_rays). */
+int btrc_hip_capture_rays(btrc_ctx *ctx, const btrc_render_params *params, int32_t bounces,
+                          btrc_ray *rays, int64_t max_rays, int64_t *count);
+
+/* ---- test hooks: evaluate device building blocks on host-given inputs ---- */
+int btrc_hip_eval_halton(btrc_ctx *ctx, int32_t width, int32_t height, const uint32_t *px, const uint32_t *py,
+                         const int32_t *sample_index, int32_t n, int32_t dims, float *out);
+/* fn: 0 sin, 1 cos, 2 log, 3 exp, 4 pow(x, y), 5 sqrt, 6 x / y */
+int btrc_hip_eval_detmath(btrc_ctx *ctx, int32_t fn, const float *x, const float *y, int32_t n, float *out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BTRC_HIP_H */

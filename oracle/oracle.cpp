@@ -1,0 +1,1585 @@
+// CPU ORACLE - test infrastructure only.
+//
+// A scalar restatement of the reference's wavefront path tracer, used to
+// check the gfx950 kernels. It is not part of the product: only tests/,
+// __graft_entry__.smoke() and the cpu_baseline leg of bench.py load it.
+//
+// Each path is traced to completion in program order (a "megakernel" loop),
+// which yields the same per-path result as the reference's stage-by-stage
+// wavefront because every random number is a function of (pixel, sample
+// index, dimension) only. Function headers cite the reference file:line they
+// restate. Traversal is brute force over all triangles or a simple
+// median-split binary BVH with a conservative slab test; both return the
+// lexicographic minimum of (t, flattened triangle index).
+//
+// Deterministic FP32 transcendentals come from the product header
+// btrc_amd/csrc/core/detmath.h (pinned on its own against libm): sharing it
+// is what lets host and GPU follow identical sampling decisions.
+//
+// Parity pinning of each part is listed in oracle/README.md.
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <thread>
+#include <vector>
+
+#include "../include/btrc_hip.h"
+#include "../btrc_amd/csrc/core/detmath.h"
+
+#pragma STDC FP_CONTRACT OFF
+
+namespace orc {
+
+const float PI = 3.1415926535f;          // utils/math/scalar.h:8
+const float FMAX = 3.40282347e+38f;      // utils/math/scalar.h:7
+namespace dm = btrc_detmath;
+
+static inline uint32_t fbits(float f) { uint32_t u; std::memcpy(&u, &f, 4); return u; }
+static inline float bitsf(uint32_t u) { float f; std::memcpy(&f, &u, 4); return f; }
+
+struct Vec
+{
+    float x, y, z;
+};
+static inline Vec mk(float x, float y, float z) { return Vec{ x, y, z }; }
+static inline Vec add(Vec a, Vec b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
+static inline Vec sub(Vec a, Vec b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
+static inline Vec mul(Vec a, Vec b) { return mk(a.x * b.x, a.y * b.y, a.z * b.z); }
+static inline Vec scl(Vec a, float s) { return mk(a.x * s, a.y * s, a.z * s); }
+static inline Vec sclf(float s, Vec a) { return mk(s * a.x, s * a.y, s * a.z); }
+static inline Vec dvs(Vec a, float s) { return mk(a.x / s, a.y / s, a.z / s); }
+static inline Vec neg(Vec a) { return mk(-a.x, -a.y, -a.z); }
+static inline float dot(Vec a, Vec b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static inline Vec cross(Vec a, Vec b) { return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
+static inline float len(Vec a) { return std::sqrt(dot(a, a)); }
+static inline Vec nrm(Vec a) { const float inv = 1.0f / std::sqrt(dot(a, a)); return scl(a, inv); }
+static inline bool zero(Vec a) { return a.x == 0.0f && a.y == 0.0f && a.z == 0.0f; }
+static inline float fmx(float a, float b) { return a > b ? a : b; }
+static inline float fmn(float a, float b) { return a < b ? a : b; }
+static inline float sq(float x) { return x * x; }
+static inline float lum(Vec c) { return 0.2126f * c.x + 0.7152f * c.y + 0.0722f * c.z; } // core/spectrum.cpp:84
+static inline Vec lerpv(Vec a, Vec b, float t) { return add(scl(a, 1.0f - t), scl(b, t)); }
+static inline bool finite3(Vec a) { return std::isfinite(a.x) && std::isfinite(a.y) && std::isfinite(a.z); }
+static inline float el(Vec a, int i) { return i == 0 ? a.x : (i == 1 ? a.y : a.z); }
+
+struct Frm
+{
+    Vec x, y, z;
+};
+static inline Frm frm(Vec x, Vec y, Vec z) { return Frm{ nrm(x), nrm(y), nrm(z) }; } // cframe.cpp:18-23
+static inline Vec l2g(const Frm &f, Vec l) { return add(add(scl(f.x, l.x), scl(f.y, l.y)), scl(f.z, l.z)); }
+static inline Vec g2l(const Frm &f, Vec g) { return mk(dot(g, f.x), dot(g, f.y), dot(g, f.z)); }
+static inline Frm rot_z(const Frm &f, Vec z) // cframe.cpp:79-84
+{
+    const Vec nx = cross(f.y, z);
+    const Vec ny = cross(z, nx);
+    return frm(nx, ny, z);
+}
+static inline Frm from_z(Vec z) // utils/math/frame.cpp:45-57
+{
+    const Vec nz = nrm(z);
+    const Vec ny = 1.0f - std::fabs(nz.x) < 0.1f ? cross(nz, mk(0, 1, 0)) : cross(nz, mk(1, 0, 0));
+    return frm(cross(nz, ny), ny, nz);
+}
+
+struct Xf
+{
+    float m[12], n[12];
+};
+static inline Vec xpoint(const Xf &t, Vec p) // ctransform.cpp:31-37
+{
+    return mk(t.m[0] * p.x + t.m[1] * p.y + t.m[2] * p.z + t.m[3],
+              t.m[4] * p.x + t.m[5] * p.y + t.m[6] * p.z + t.m[7],
+              t.m[8] * p.x + t.m[9] * p.y + t.m[10] * p.z + t.m[11]);
+}
+static inline Vec xvec(const Xf &t, Vec v) // ctransform.cpp:39-47
+{
+    return mk(t.m[0] * v.x + t.m[1] * v.y + t.m[2] * v.z,
+              t.m[4] * v.x + t.m[5] * v.y + t.m[6] * v.z,
+              t.m[8] * v.x + t.m[9] * v.y + t.m[10] * v.z);
+}
+static inline Vec xnor(const Xf &t, Vec n) // ctransform.cpp:49-55
+{
+    return mk(t.n[0] * n.x + t.n[4] * n.y + t.n[8] * n.z,
+              t.n[1] * n.x + t.n[5] * n.y + t.n[9] * n.z,
+              t.n[2] * n.x + t.n[6] * n.y + t.n[10] * n.z);
+}
+
+// utils/intersection.cpp:5-37
+static Vec offset_pos(Vec p, Vec n)
+{
+    const float ORIGIN = 1.0f / 32.0f, FS = 1.0f / 65536.0f, IS = 256.0f;
+    const int32_t ox = (int32_t)(2.0f * IS * n.x), oy = (int32_t)(2.0f * IS * n.y), oz = (int32_t)(2.0f * IS * n.z);
+    auto bump = [](float v, int32_t o) {
+        int32_t b; std::memcpy(&b, &v, 4);
+        b += v < 0.0f ? -o : o;
+        float r; std::memcpy(&r, &b, 4);
+        return r;
+    };
+    return mk(std::fabs(p.x) < ORIGIN ? p.x + FS * n.x : bump(p.x, ox),
+              std::fabs(p.y) < ORIGIN ? p.y + FS * n.y : bump(p.y, oy),
+              std::fabs(p.z) < ORIGIN ? p.z + FS * n.z : bump(p.z, oz));
+}
+static Vec offset_dir(Vec p, Vec n, Vec d) { return offset_pos(p, dot(n, d) > 0.0f ? n : neg(n)); }
+
+// utils/cmath/cdistribution.cpp
+static Vec sphere_uniform(float u1, float u2)
+{
+    const float z = 1.0f - (u1 + u1);
+    const float phi = 2.0f * PI * u2;
+    const float r = std::sqrt(fmx(0.0f, 1.0f - z * z));
+    return mk(r * dm::cos(phi), r * dm::sin(phi), z);
+}
+static Vec hemisphere_cos(float u1, float u2)
+{
+    float sx = 0, sy = 0;
+    u1 = 2.0f * u1 - 1.0f;
+    u2 = 2.0f * u2 - 1.0f;
+    if(u1 != 0.0f || u2 != 0.0f)
+    {
+        float th, r;
+        if(std::fabs(u1) > std::fabs(u2)) { r = u1; th = 0.25f * PI * (u2 / u1); }
+        else { r = u2; th = 0.5f * PI - 0.25f * PI * (u1 / u2); }
+        sx = r * dm::cos(th);
+        sy = r * dm::sin(th);
+    }
+    return mk(sx, sy, std::sqrt(fmx(0.0f, 1.0f - (sx * sx + sy * sy))));
+}
+static float hemisphere_cos_pdf(Vec v) { return v.z >= 0.0f ? v.z / PI : 0.0f; }
+
+// ------------------------------------------------------------------ Halton
+
+// first 1000 primes (utils/prime.cpp) by a sieve
+static const std::vector<uint32_t> &primes()
+{
+    static std::vector<uint32_t> p = [] {
+        std::vector<uint32_t> r;
+        std::vector<bool> comp(8000, false);
+        for(uint32_t i = 2; r.size() < 1000; ++i)
+        {
+            if(comp[i])
+                continue;
+            r.push_back(i);
+            for(uint32_t j = i * i; j < 8000; j += i)
+                comp[j] = true;
+        }
+        return r;
+    }();
+    return p;
+}
+
+// utils/low_discrepancy.cpp:49-57
+static uint64_t mixbits(uint64_t v)
+{
+    v ^= v >> 31;
+    v *= 0x7fb5d329728ea185ull;
+    v ^= v >> 27;
+    v *= 0x81dadef4bc2dd44dull;
+    v ^= v >> 33;
+    return v;
+}
+
+// utils/low_discrepancy.cpp:12-45
+static uint32_t perm_elem(uint32_t i, uint32_t l, uint32_t p)
+{
+    uint32_t w = l - 1;
+    w |= w >> 1; w |= w >> 2; w |= w >> 4; w |= w >> 8; w |= w >> 16;
+    do
+    {
+        i ^= p; i *= 0xe170893du; i ^= p >> 16; i ^= (i & w) >> 4;
+        i ^= p >> 8; i *= 0x0929eb3fu; i ^= p >> 23; i ^= (i & w) >> 1;
+        i *= 1u | p >> 27; i *= 0x6935fa69u; i ^= (i & w) >> 11; i *= 0x74dcb303u;
+        i ^= (i & w) >> 2; i *= 0x9e501cc3u; i ^= (i & w) >> 2; i *= 0xc860a3dfu;
+        i &= w; i ^= i >> 5;
+    } while(i >= l);
+    return (i + p) % l;
+}
+
+// utils/low_discrepancy.cpp:83-101
+static float owen_ri(uint32_t base, uint64_t a, uint64_t hash)
+{
+    const float inv_base = 1.0f / (float)base;
+    float inv_base_m = 1.0f;
+    uint64_t rev = 0;
+    while(1.0f - inv_base_m < 1.0f)
+    {
+        const uint64_t next = a / base;
+        uint64_t digit = a - next * base;
+        digit = perm_elem((uint32_t)digit, base, (uint32_t)mixbits(hash ^ rev));
+        rev = rev * base + digit;
+        inv_base_m *= inv_base;
+        a = next;
+    }
+    return fmn(inv_base_m * (float)rev, 0x1.fffffep-1f);
+}
+
+struct Halton
+{
+    int32_t scales[2], exps[2], minv[2];
+    uint32_t dim;
+    uint64_t idx;
+
+    // sampler/halton.cpp:118-133 and 46-66
+    void init(int32_t w, int32_t h, uint32_t px, uint32_t py, int32_t sample)
+    {
+        const int32_t res[2] = { w, h };
+        for(int i = 0; i < 2; ++i)
+        {
+            const int32_t base = i == 0 ? 2 : 3;
+            int32_t s = 1, e = 0;
+            while(s < std::min(res[i], 128)) { s *= base; ++e; }
+            scales[i] = s;
+            exps[i] = e;
+        }
+        auto inv = [](int64_t a, int64_t n) {
+            // extended Euclid (halton.cpp:18-37)
+            int64_t r0 = a, r1 = n, s0 = 1, s1 = 0;
+            while(r1) { const int64_t q = r0 / r1; int64_t t = r0 - q * r1; r0 = r1; r1 = t; t = s0 - q * s1; s0 = s1; s1 = t; }
+            const int64_t m = s0 % n;
+            return m < 0 ? m + n : m;
+        };
+        minv[0] = (int32_t)inv(scales[1], scales[0]);
+        minv[1] = (int32_t)inv(scales[0], scales[1]);
+        idx = 0;
+        const int32_t stride = scales[0] * scales[1];
+        if(stride > 1)
+        {
+            const uint64_t pm[2] = { px % 128u, py % 128u };
+            for(int i = 0; i < 2; ++i)
+            {
+                uint64_t v = pm[i], off = 0;
+                const uint64_t b = i == 0 ? 2 : 3;
+                for(int k = 0; k < exps[i]; ++k) { off = off * b + v % b; v /= b; }
+                idx += off * (uint64_t)(stride / scales[i]) * (uint64_t)minv[i];
+            }
+            idx %= (uint64_t)stride;
+        }
+        idx += (uint64_t)(sample * stride);
+        dim = 2;
+    }
+    float sample(uint32_t d) const
+    {
+        return owen_ri(primes()[d], idx, mixbits((uint64_t)(1u + (d << 4))));
+    }
+    float get1d()
+    {
+        if(dim >= 1000) dim = 2;
+        const float r = sample(dim);
+        dim += 1;
+        return r;
+    }
+    void get3d(float &a, float &b, float &c)
+    {
+        if(dim + 2 >= 1000) dim = 2;
+        a = sample(dim); b = sample(dim + 1); c = sample(dim + 2);
+        dim += 3;
+    }
+};
+
+// ------------------------------------------------------------------ scene
+
+struct Light
+{
+    int type; // 0 mesh, 1 sky
+    int inst, geo;
+    float scale;
+    Vec intensity;
+    Vec up, lower, upper;
+};
+
+struct Geo
+{
+    uint32_t off, n;
+    float total_area;
+    std::vector<float> accept;
+    std::vector<uint32_t> alias;
+};
+
+struct Inst
+{
+    int geo, mat, light;
+    Xf xf;
+};
+
+struct Scene
+{
+    std::vector<Geo> geos;
+    std::vector<btrc_material_desc> mats;
+    std::vector<Inst> insts;
+    std::vector<Light> lights;
+    int env = -1;
+    // per triangle of all geometries: frame records (triangle_mesh.cpp:40-64)
+    std::vector<Vec> gx, gy, gz, sa, sba, sca, la, lba, lca;
+    // flattened world triangles
+    std::vector<Vec> w0, w1, w2;
+    std::vector<uint32_t> tinst, tprim;
+    // binary BVH
+    struct Node { float lo[3], hi[3]; int32_t left, right; uint32_t first, count; };
+    std::vector<Node> nodes;
+    std::vector<uint32_t> order;
+};
+
+static void build_bvh(Scene &s)
+{
+    const uint32_t n = (uint32_t)s.w0.size();
+    s.order.resize(n);
+    for(uint32_t i = 0; i < n; ++i)
+        s.order[i] = i;
+    s.nodes.clear();
+    if(n == 0)
+        return;
+    struct T { int32_t node; uint32_t b, e; };
+    auto bounds = [&](uint32_t b, uint32_t e, Scene::Node &nd) {
+        for(int a = 0; a < 3; ++a) { nd.lo[a] = INFINITY; nd.hi[a] = -INFINITY; }
+        for(uint32_t i = b; i < e; ++i)
+        {
+            const uint32_t t = s.order[i];
+            for(const Vec &v : { s.w0[t], s.w1[t], s.w2[t] })
+                for(int a = 0; a < 3; ++a)
+                {
+                    nd.lo[a] = std::min(nd.lo[a], el(v, a));
+                    nd.hi[a] = std::max(nd.hi[a], el(v, a));
+                }
+        }
+    };
+    s.nodes.push_back(Scene::Node{});
+    std::vector<T> st{ { 0, 0, n } };
+    while(!st.empty())
+    {
+        const T t = st.back();
+        st.pop_back();
+        Scene::Node &nd = s.nodes[t.node];
+        bounds(t.b, t.e, nd);
+        nd.left = nd.right = -1;
+        if(t.e - t.b <= 2)
+        {
+            nd.first = t.b;
+            nd.count = t.e - t.b;
+            continue;
+        }
+        int axis = 0;
+        float ext = -1;
+        for(int a = 0; a < 3; ++a)
+            if(nd.hi[a] - nd.lo[a] > ext) { ext = nd.hi[a] - nd.lo[a]; axis = a; }
+        const uint32_t mid = (t.b + t.e) / 2;
+        auto key = [&](uint32_t i) {
+            return el(s.w0[i], axis) + el(s.w1[i], axis) + el(s.w2[i], axis);
+        };
+        std::nth_element(s.order.begin() + t.b, s.order.begin() + mid, s.order.begin() + t.e,
+                         [&](uint32_t a, uint32_t b) { return key(a) < key(b); });
+        const int32_t l = (int32_t)s.nodes.size();
+        s.nodes.push_back(Scene::Node{});
+        s.nodes.push_back(Scene::Node{});
+        s.nodes[t.node].left = l;
+        s.nodes[t.node].right = l + 1;
+        st.push_back({ l, t.b, mid });
+        st.push_back({ l + 1, mid, t.e });
+    }
+}
+
+static Scene *make_scene(const btrc_scene_desc *sd)
+{
+    Scene *s = new Scene;
+    uint32_t off = 0;
+    for(int gi = 0; gi < sd->geometry_count; ++gi)
+    {
+        const btrc_geometry_desc &g = sd->geometries[gi];
+        Geo geo;
+        geo.off = off;
+        geo.n = (uint32_t)g.triangle_count;
+        std::vector<float> area(g.triangle_count);
+        for(int t = 0; t < g.triangle_count; ++t)
+        {
+            const float *p = g.positions + 9 * (size_t)t;
+            const Vec a = mk(p[0], p[1], p[2]), b = mk(p[3], p[4], p[5]), c = mk(p[6], p[7], p[8]);
+            // utils/triangle_mesh_loader.cpp:84-127
+            const Vec gzv = nrm(cross(sub(b, a), sub(c, a)));
+            Vec nz[3];
+            for(int k = 0; k < 3; ++k)
+            {
+                nz[k] = gzv;
+                if(g.normals)
+                {
+                    const float *q = g.normals + 9 * (size_t)t + 3 * k;
+                    const Vec v = mk(q[0], q[1], q[2]);
+                    if(!zero(v))
+                        nz[k] = v;
+                }
+            }
+            float uv[6] = { 0, 0, 0, 0, 0, 0 };
+            if(g.tex_coords)
+                std::memcpy(uv, g.tex_coords + 6 * (size_t)t, sizeof(uv));
+            // triangle_dpdu, utils/math/triangle.h:8-23
+            Vec ex;
+            {
+                const Vec BA = sub(b, a), CA = sub(c, a);
+                const float m00 = uv[2] - uv[0], m01 = uv[3] - uv[1], m10 = uv[4] - uv[0], m11 = uv[5] - uv[1];
+                const float det = m00 * m11 - m01 * m10;
+                if(det == 0.0f)
+                    ex = from_z(gzv).x;
+                else
+                {
+                    const float id = 1.0f / det;
+                    ex = nrm(sub(sclf(m11 * id, BA), sclf(m01 * id, CA)));
+                }
+            }
+            s->gx.push_back(ex);
+            s->gy.push_back(nrm(cross(gzv, ex)));
+            s->gz.push_back(gzv);
+            s->sa.push_back(nz[0]);
+            s->sba.push_back(sub(nz[1], nz[0]));
+            s->sca.push_back(sub(nz[2], nz[0]));
+            const Vec ba = sub(b, a), ca = sub(c, a);
+            s->la.push_back(a);
+            s->lba.push_back(ba);
+            s->lca.push_back(ca);
+            area[t] = 0.5f * len(cross(ba, ca)); // utils/math/triangle.h:25-28
+        }
+        // AliasTable::initialize, utils/math/alias.cpp:13-58
+        float sum = 0.0f;
+        for(float a : area)
+            sum += a;
+        geo.total_area = sum;
+        const uint32_t n = (uint32_t)area.size();
+        geo.accept.resize(n);
+        geo.alias.resize(n);
+        if(n)
+        {
+            const float ratio = (float)n / sum;
+            std::vector<uint32_t> ov, un;
+            for(uint32_t i = 0; i < n; ++i)
+            {
+                geo.accept[i] = area[i] * ratio;
+                geo.alias[i] = i;
+                if(geo.accept[i] > 1) ov.push_back(i);
+                else if(geo.accept[i] < 1) un.push_back(i);
+            }
+            while(!ov.empty() && !un.empty())
+            {
+                const uint32_t o = ov.back(), u = un.back();
+                ov.pop_back();
+                un.pop_back();
+                geo.accept[o] -= 1 - geo.accept[u];
+                geo.alias[u] = o;
+                if(geo.accept[o] > 1) ov.push_back(o);
+                else if(geo.accept[o] < 1) un.push_back(o);
+            }
+            for(auto i : ov) geo.accept[i] = 1;
+            for(auto i : un) geo.accept[i] = 1;
+        }
+        s->geos.push_back(std::move(geo));
+        off += (uint32_t)g.triangle_count;
+    }
+    s->mats.assign(sd->materials, sd->materials + sd->material_count);
+    for(int i = 0; i < sd->instance_count; ++i)
+    {
+        const btrc_instance_desc &d = sd->instances[i];
+        Inst in;
+        in.geo = d.geometry;
+        in.mat = d.material;
+        in.light = -1;
+        for(int k = 0; k < 12; ++k)
+        {
+            in.xf.m[k] = d.local_to_world[k];
+            in.xf.n[k] = d.world_to_local[k];
+        }
+        if(d.has_light)
+        {
+            // MeshLight::set_geometry, mesh_light.cpp:17-22
+            Light l{};
+            l.type = 0;
+            l.inst = i;
+            l.geo = d.geometry;
+            l.scale = len(sub(xpoint(in.xf, mk(1, 0, 0)), xpoint(in.xf, mk(0, 0, 0))));
+            l.intensity = mk(d.light_intensity[0], d.light_intensity[1], d.light_intensity[2]);
+            in.light = (int)s->lights.size();
+            s->lights.push_back(l);
+        }
+        s->insts.push_back(in);
+        const btrc_geometry_desc &g = sd->geometries[d.geometry];
+        for(int t = 0; t < g.triangle_count; ++t)
+        {
+            const float *p = g.positions + 9 * (size_t)t;
+            s->w0.push_back(xpoint(in.xf, mk(p[0], p[1], p[2])));
+            s->w1.push_back(xpoint(in.xf, mk(p[3], p[4], p[5])));
+            s->w2.push_back(xpoint(in.xf, mk(p[6], p[7], p[8])));
+            s->tinst.push_back((uint32_t)i);
+            s->tprim.push_back((uint32_t)t);
+        }
+    }
+    if(sd->envir_light.type == BTRC_ENVIR_GRADIENT_SKY)
+    {
+        // Scene::commit puts the environment light last (core/scene.cpp:92-99)
+        Light l{};
+        l.type = 1;
+        l.up = nrm(mk(sd->envir_light.up[0], sd->envir_light.up[1], sd->envir_light.up[2]));
+        l.lower = mk(sd->envir_light.lower[0], sd->envir_light.lower[1], sd->envir_light.lower[2]);
+        l.upper = mk(sd->envir_light.upper[0], sd->envir_light.upper[1], sd->envir_light.upper[2]);
+        s->env = (int)s->lights.size();
+        s->lights.push_back(l);
+    }
+    build_bvh(*s);
+    return s;
+}
+
+// ------------------------------------------------------------------ intersection
+
+struct Hit
+{
+    float t, u, v;
+    uint32_t gid; // 0xffffffff = miss
+};
+
+// Watertight test (Woop, Benthin, Wald, JCGT 2013), single precision.
+static bool tri_watertight(Vec o, Vec d, Vec v0, Vec v1, Vec v2, float tmin, float tmax, float &t, float &b1, float &b2)
+{
+    const float ax = std::fabs(d.x), ay = std::fabs(d.y), az = std::fabs(d.z);
+    const int kz = ax > ay ? (ax > az ? 0 : 2) : (ay > az ? 1 : 2);
+    int kx = (kz + 1) % 3, ky = (kx + 1) % 3;
+    if(el(d, kz) < 0.0f)
+        std::swap(kx, ky);
+    const float Sx = el(d, kx) / el(d, kz), Sy = el(d, ky) / el(d, kz), Sz = 1.0f / el(d, kz);
+    const Vec A = sub(v0, o), B = sub(v1, o), C = sub(v2, o);
+    const float Ax = el(A, kx) - Sx * el(A, kz), Ay = el(A, ky) - Sy * el(A, kz);
+    const float Bx = el(B, kx) - Sx * el(B, kz), By = el(B, ky) - Sy * el(B, kz);
+    const float Cx = el(C, kx) - Sx * el(C, kz), Cy = el(C, ky) - Sy * el(C, kz);
+    const float U = Cx * By - Cy * Bx, V = Ax * Cy - Ay * Cx, W = Bx * Ay - By * Ax;
+    if((U < 0 || V < 0 || W < 0) && (U > 0 || V > 0 || W > 0))
+        return false;
+    const float det = U + V + W;
+    if(det == 0.0f)
+        return false;
+    const float T = U * (Sz * el(A, kz)) + V * (Sz * el(B, kz)) + W * (Sz * el(C, kz));
+    const float r = 1.0f / det;
+    const float tt = T * r;
+    if(!(tt >= tmin && tt <= tmax))
+        return false;
+    t = tt;
+    b1 = V * r;
+    b2 = W * r;
+    return true;
+}
+
+// lib/bvh Triangle::intersect, lib/bvh/include/bvh/triangle.hpp:95-115
+static bool tri_mt(Vec o, Vec d, Vec p0, Vec p1, Vec p2, float tmin, float tmax, float &t, float &u, float &v)
+{
+    const Vec e1 = sub(p0, p1), e2 = sub(p2, p0), n = cross(e1, e2);
+    const Vec c = sub(p0, o), r = cross(d, c);
+    const float inv_det = 1.0f / dot(n, d);
+    const float uu = dot(r, e2) * inv_det, vv = dot(r, e1) * inv_det, w = 1.0f - uu - vv;
+    if(uu >= 0 && vv >= 0 && w >= 0)
+    {
+        const float tt = dot(n, c) * inv_det;
+        if(tt >= tmin && tt <= tmax)
+        {
+            t = tt; u = uu; v = vv;
+            return true;
+        }
+    }
+    return false;
+}
+
+static inline bool test_tri(const Scene &s, int mode, uint32_t i, Vec o, Vec d, float tmin, Hit &h)
+{
+    float t, u, v;
+    const bool ok = mode == BTRC_TRI_WATERTIGHT ? tri_watertight(o, d, s.w0[i], s.w1[i], s.w2[i], tmin, h.t, t, u, v)
+                                                : tri_mt(o, d, s.w0[i], s.w1[i], s.w2[i], tmin, h.t, t, u, v);
+    if(ok && (t < h.t || i < h.gid))
+    {
+        h.t = t; h.u = u; h.v = v; h.gid = i;
+        return true;
+    }
+    return false;
+}
+
+// Ize's conservative slab test (JCGT 2013)
+static bool box_hit(const Scene::Node &nd, Vec o, Vec inv, Vec invp, float tmin, float tmax)
+{
+    float te = tmin, tx = tmax;
+    for(int a = 0; a < 3; ++a)
+    {
+        const float ia = el(inv, a);
+        const float lo = nd.lo[a], hi = nd.hi[a];
+        const bool neg = std::signbit(ia);
+        const float nearp = neg ? hi : lo, farp = neg ? lo : hi;
+        const float t0 = (nearp - el(o, a)) * ia;
+        const float t1 = (farp - el(o, a)) * el(invp, a);
+        te = t0 > te ? t0 : te;
+        tx = t1 < tx ? t1 : tx;
+    }
+    return te <= tx;
+}
+
+
+static Hit trace(const Scene &s, int mode, bool any, bool use_bvh, Vec o, Vec d, float tmin, float tmax)
+{
+    Hit h{ tmax, 0.0f, 0.0f, 0xffffffffu };
+    const uint32_t n = (uint32_t)s.w0.size();
+    if(!use_bvh || s.nodes.empty())
+    {
+        for(uint32_t i = 0; i < n; ++i)
+            if(test_tri(s, mode, i, o, d, tmin, h) && any)
+                return h;
+        return h;
+    }
+    const Vec inv = mk(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+    auto pad = [](float x) { return std::isfinite(x) ? bitsf(fbits(x) + 2u) : x; };
+    const Vec invp = mk(pad(inv.x), pad(inv.y), pad(inv.z));
+    std::vector<int32_t> st{ 0 };
+    while(!st.empty())
+    {
+        const Scene::Node &nd = s.nodes[st.back()];
+        st.pop_back();
+        const float cull = h.t < FMAX ? h.t * 1.00000095f : h.t;
+        if(!box_hit(nd, o, inv, invp, tmin, cull))
+            continue;
+        if(nd.left < 0)
+        {
+            for(uint32_t k = nd.first; k < nd.first + nd.count; ++k)
+                if(test_tri(s, mode, s.order[k], o, d, tmin, h) && any)
+                    return h;
+            continue;
+        }
+        st.push_back(nd.right);
+        st.push_back(nd.left);
+    }
+    return h;
+}
+
+// ------------------------------------------------------------------ surfaces
+
+struct SP
+{
+    Vec pos;
+    Frm frame;
+    Vec iz;
+};
+
+// get_hitinfo, core/scene.cpp:9-64
+static SP hitinfo(const Scene &s, Vec o, Vec d, const Inst &in, uint32_t prim, float t, float u, float v)
+{
+    const uint32_t i = s.geos[in.geo].off + prim;
+    SP sp;
+    sp.pos = add(o, sclf(t, d));
+    const Frm gf = frm(s.gx[i], s.gy[i], s.gz[i]);
+    sp.frame.x = nrm(xvec(in.xf, gf.x));
+    sp.frame.y = nrm(xvec(in.xf, gf.y));
+    sp.frame.z = nrm(xnor(in.xf, gf.z));
+    const Vec n = add(add(s.sa[i], scl(s.sba[i], u)), scl(s.sca[i], v));
+    sp.iz = nrm(xnor(in.xf, n));
+    return sp;
+}
+
+// ---------------------------------------------------------- BSDF components
+
+static float cos2sin(float c) { return std::sqrt(fmx(0.0f, 1.0f - c * c)); } // utils/local_angle.h:22-25
+static float tan_th(Vec w) // local_angle.h:28-36
+{
+    const float t = 1.0f - w.z * w.z;
+    return t > 0.0f ? std::sqrt(t) / w.z : 0.0f;
+}
+// sin and cos of local_angle::phi (local_angle.h:55-76) from x / r and y / r
+static void phi_sc(Vec w, float &s, float &c)
+{
+    if(w.x != 0.0f || w.y != 0.0f)
+    {
+        const float r = std::sqrt(w.x * w.x + w.y * w.y);
+        s = w.y / r;
+        c = w.x / r;
+    }
+    else
+    {
+        s = 0.0f;
+        c = 1.0f;
+    }
+}
+
+// material/utils/fresnel.cpp:5-33
+static float fresnel_d(float ei, float eo, float ci)
+{
+    if(ci < 0.0f) { std::swap(ei, eo); ci = -ci; }
+    const float si = std::sqrt(fmx(0.0f, 1.0f - ci * ci));
+    const float st = eo / ei * si;
+    if(st >= 1.0f)
+        return 1.0f;
+    const float ct = std::sqrt(fmx(0.0f, 1.0f - st * st));
+    const float pa = (ei * ci - eo * ct) / (ei * ci + eo * ct);
+    const float pe = (eo * ci - ei * ct) / (eo * ci + ei * ct);
+    return 0.5f * (pa * pa + pe * pe);
+}
+static float schlick(float r0, float c) // fresnel.cpp:35-41
+{
+    const float t = 1.0f - c, t2 = t * t, t5 = t2 * t2 * t;
+    return r0 + (1.0f - r0) * t5;
+}
+static Vec schlick3(Vec r0, float c) // fresnel.cpp:43-49
+{
+    const float t = 1.0f - c, t2 = t * t, t5 = t2 * t2 * t;
+    return add(r0, scl(sub(mk(1, 1, 1), r0), t5));
+}
+
+// material/utils/microfacet.cpp
+static float ca_(float sp, float cp, float ax, float ay) { return sq(cp / ax) + sq(sp / ay); }
+static float smith_gtr2(float tn, float a)
+{
+    if(tn == 0.0f)
+        return 1.0f;
+    const float r = a * tn;
+    return 2.0f / (1.0f + std::sqrt(1.0f + r * r));
+}
+static float aniso_gtr2(float sp, float cp, float st, float ct, float ax, float ay)
+{
+    const float A = ca_(sp, cp, ax, ay);
+    const float RD = sq(st) * A + sq(ct);
+    return 1.0f / (PI * ax * ay * sq(RD));
+}
+static float smith_aniso(float cp, float sp, float ax, float ay, float tn)
+{
+    const float t = sq(ax * cp) + sq(ay * sp);
+    const float lam = -0.5f + 0.5f * std::sqrt(1.0f + t * sq(tn));
+    return 1.0f / (1.0f + lam);
+}
+static Vec sample_aniso_gtr2(float ax, float ay, float s0, float s1)
+{
+    const float ang = 2.0f * PI * s0;
+    float sp = ay * dm::sin(ang), cp = ax * dm::cos(ang);
+    const float nr = 1.0f / std::sqrt(sq(sp) + sq(cp));
+    sp = sp * nr;
+    cp = cp * nr;
+    const float A = ca_(sp, cp, ax, ay);
+    const float ct = std::sqrt(A * (1.0f - s1) / ((1.0f - A) * s1 + A));
+    const float st = cos2sin(ct);
+    return nrm(mk(st * cp, st * sp, ct));
+}
+static float gtr1(float st, float ct, float a)
+{
+    return (sq(a) - 1.0f) / (2.0f * PI * dm::log(a) * (sq(a * ct) + sq(st)));
+}
+static Vec sample_gtr1(float a, float s0, float s1)
+{
+    const float phi = 2.0f * PI * s0;
+    const float ct = std::sqrt((dm::pow(a, 2.0f - 2.0f * s1) - 1.0f) / (sq(a) - 1.0f));
+    const float st = cos2sin(ct);
+    return nrm(mk(st * dm::cos(phi), st * dm::sin(phi), ct));
+}
+static Vec sample_vndf(Vec ve, float ax, float ay, float s0, float s1)
+{
+    const Vec vh = nrm(mk(ax * ve.x, ay * ve.y, ve.z));
+    const float lsq = vh.x * vh.x + vh.y * vh.y;
+    const Vec t1 = lsq > 3e-4f ? dvs(mk(-vh.y, vh.x, 0.0f), std::sqrt(lsq)) : mk(1, 0, 0);
+    const Vec t2 = cross(vh, t1);
+    const float r = std::sqrt(s0);
+    const float phi = 2.0f * PI * s1;
+    const float a1 = r * dm::cos(phi), a2r = r * dm::sin(phi);
+    const float s = 0.5f * (1.0f + vh.z);
+    const float a2 = (1.0f - s) * std::sqrt(1.0f - a1 * a1) + s * a2r;
+    const Vec nh = add(add(sclf(a1, t1), sclf(a2, t2)), sclf(std::sqrt(fmx(0.0f, 1.0f - a1 * a1 - a2 * a2)), vh));
+    return nrm(mk(ax * nh.x, ay * nh.y, fmx(0.0f, nh.z)));
+}
+
+struct BS
+{
+    Vec f, dir;
+    float pdf;
+};
+
+struct Dis
+{
+    Vec C, Ct;
+    float met, rough, spec, stint, sheen, shtint, cc, ccr, trans, ior, tax, tay, ax, ay, wd, ws, wc, wt;
+};
+
+// DisneyBSDF::DisneyBSDF, builtin/material/disney.cpp:434-475
+static Dis dis_make(const btrc_material_desc &m)
+{
+    Dis d;
+    d.C = mk(m.color[0], m.color[1], m.color[2]);
+    const float l = lum(d.C);
+    d.Ct = l > 0.0f ? dvs(d.C, l) : mk(1, 1, 1);
+    d.met = m.metallic;
+    d.rough = m.roughness;
+    d.spec = m.specular;
+    d.stint = m.specular_tint;
+    d.sheen = m.sheen;
+    d.shtint = m.sheen_tint;
+    d.trans = m.transmission;
+    d.ior = fmx(1.01f, m.ior);
+    const float aspect = m.anisotropic > 0.0f ? std::sqrt(1.0f - 0.9f * m.anisotropic) : 1.0f;
+    d.ax = fmx(0.001f, sq(m.roughness) / aspect);
+    d.ay = fmx(0.001f, sq(m.roughness) * aspect);
+    d.tax = fmx(0.001f, sq(m.transmission_roughness) / aspect);
+    d.tay = fmx(0.001f, sq(m.transmission_roughness) * aspect);
+    d.cc = m.clearcoat;
+    d.ccr = fmx(0.0001f, sq(0.1f * (1.0f - m.clearcoat_gloss) + 0.0f * m.clearcoat_gloss));
+    const float A = fmn(fmx(l * (1.0f - m.metallic), 0.3f), 0.7f);
+    const float B = 1.0f - A;
+    d.wd = A * (1.0f - m.transmission);
+    d.wt = A * m.transmission;
+    d.ws = B * 2.0f / (2.0f + m.clearcoat);
+    d.wc = B * m.clearcoat / (2.0f + m.clearcoat);
+    return d;
+}
+
+static float om5(float x) { const float t = 1.0f - x, t2 = t * t; return t2 * t2 * t; }
+
+// D (returned through D) and G of an anisotropic GTR2 lobe
+static float dg(Vec h, Vec i, Vec o, float ax, float ay, float &D)
+{
+    float sph, cph, spi, cpi, spo, cpo;
+    phi_sc(h, sph, cph);
+    D = aniso_gtr2(sph, cph, cos2sin(h.z), h.z, ax, ay);
+    phi_sc(i, spi, cpi);
+    phi_sc(o, spo, cpo);
+    return smith_aniso(cpi, spi, ax, ay, tan_th(i)) * smith_aniso(cpo, spo, ax, ay, tan_th(o));
+}
+
+static Vec dis_eval(const Dis &d, Vec i, Vec o) // disney.cpp:477-558
+{
+    if(std::fabs(i.z) < 1e-4f || std::fabs(o.z) < 1e-4f)
+        return mk(0, 0, 0);
+    if(i.z * o.z < 0.0f)
+    {
+        // eval_trans, disney.cpp:100-145
+        if(d.trans == 0.0f)
+            return mk(0, 0, 0);
+        const float eta = o.z > 0.0f ? d.ior : 1.0f / d.ior;
+        Vec h = nrm(add(o, sclf(eta, i)));
+        if(h.z < 0.0f)
+            h = neg(h);
+        const float cd = dot(o, h);
+        const float F = fresnel_d(d.ior, 1.0f, cd);
+        float D;
+        const float G = dg(h, i, o, d.tax, d.tay, D);
+        const float sdem = cd + eta * dot(i, h);
+        const float corr = 1.0f / eta;
+        const Vec sC = mk(std::sqrt(d.C.x), std::sqrt(d.C.y), std::sqrt(d.C.z));
+        const float val = (1.0f - F) * D * G * eta * eta * dot(i, h) * dot(o, h) * corr * corr
+                        / (i.z * o.z * sdem * sdem);
+        const float tf = o.z > 0.0f ? d.trans : 1.0f;
+        return scl(sclf((1.0f - d.met) * tf, sC), std::fabs(val));
+    }
+    if(i.z < 0.0f && o.z < 0.0f)
+    {
+        // eval_inner_refl, disney.cpp:147-180
+        if(d.trans == 0.0f)
+            return mk(0, 0, 0);
+        const Vec h = neg(nrm(add(i, o)));
+        const float F = fresnel_d(d.ior, 1.0f, dot(o, h));
+        float D;
+        const float G = dg(h, i, o, d.tax, d.tay, D);
+        return scl(sclf(d.trans, d.C), std::fabs(F * D * G / (4.0f * i.z * o.z)));
+    }
+    if(i.z <= 0.0f || o.z <= 0.0f)
+        return mk(0, 0, 0);
+    const Vec h = nrm(add(i, o));
+    const float cd = dot(i, h);
+    Vec diffuse = mk(0, 0, 0), sheen = mk(0, 0, 0);
+    if(d.met < 1.0f)
+    {
+        // eval_diffuse, disney.cpp:76-86
+        const Vec lam = dvs(d.C, PI);
+        const float fl = om5(i.z), fv = om5(o.z);
+        const float rr = 2.0f * d.rough * cd * cd;
+        const Vec retro = scl(scl(dvs(d.C, PI), rr), fl + fv + fl * fv * (rr - 1.0f));
+        diffuse = add(scl(scl(lam, 1.0f - 0.5f * fl), 1.0f - 0.5f * fv), retro);
+        if(d.sheen > 0.0f) // eval_sheen, disney.cpp:88-92
+            sheen = scl(sclf(4.0f * d.sheen, lerpv(mk(1, 1, 1), d.Ct, d.shtint)), om5(cd));
+    }
+    // eval_specular, disney.cpp:182-220
+    Vec spec;
+    {
+        const Vec Cs = lerpv(lerpv(mk(1, 1, 1), d.Ct, d.stint), d.C, d.met);
+        const Vec Fd = scl(Cs, fresnel_d(d.ior, 1.0f, cd));
+        const Vec Fc = schlick3(Cs, cd);
+        const Vec F = lerpv(sclf(d.spec, Fd), Fc, d.met);
+        float D;
+        const float G = dg(h, i, o, d.ax, d.ay, D);
+        spec = dvs(scl(scl(F, D), G), std::fabs(4.0f * i.z * o.z));
+    }
+    Vec cc = mk(0, 0, 0);
+    if(d.cc > 0.0f)
+    {
+        // eval_clearcoat, disney.cpp:94-98
+        const float Dc = gtr1(cos2sin(h.z), h.z, d.ccr);
+        const float Fc = schlick(0.04f, cd);
+        const float Gc = smith_gtr2(tan_th(i), 0.25f) * smith_gtr2(tan_th(o), 0.25f);
+        const float v = d.cc * Dc * Fc * Gc / std::fabs(4.0f * i.z * o.z);
+        cc = mk(v, v, v);
+    }
+    return add(add(sclf((1.0f - d.met) * (1.0f - d.trans), add(diffuse, sheen)), spec), cc);
+}
+
+static float pdf_trans(const Dis &d, Vec i, Vec o) // disney.cpp:369-402
+{
+    const float eta = o.z > 0.0f ? d.ior : 1.0f / d.ior;
+    Vec h = nrm(add(o, sclf(eta, i)));
+    if(h.z < 0.0f)
+        h = neg(h);
+    if(((o.z > 0.0f) != (dot(h, o) > 0.0f)) || ((i.z > 0.0f) != (dot(h, i) > 0.0f)))
+        return 0.0f;
+    const float sdem = dot(o, h) + eta * dot(i, h);
+    const float j = eta * eta * dot(i, h) / (sdem * sdem);
+    float sph, cph;
+    phi_sc(h, sph, cph);
+    const float D = aniso_gtr2(sph, cph, cos2sin(h.z), h.z, d.tax, d.tay);
+    return std::fabs(dot(i, h) * D * j);
+}
+
+static float pdf_inner(const Dis &d, Vec i, Vec o) // disney.cpp:404-420
+{
+    const Vec h = neg(nrm(add(i, o)));
+    float sph, cph;
+    phi_sc(h, sph, cph);
+    const float D = aniso_gtr2(sph, cph, cos2sin(h.z), h.z, d.tax, d.tay);
+    return std::fabs(h.z * D / (4.0f * dot(i, h)));
+}
+
+static float dis_pdf(const Dis &d, Vec i, Vec o) // disney.cpp:682-730
+{
+    if(std::fabs(i.z) < 1e-4f || std::fabs(o.z) < 1e-4f)
+        return 0.0f;
+    if(o.z < 0.0f)
+    {
+        if(d.trans == 0.0f)
+            return 0.0f;
+        const float mF = fmn(fmx(fresnel_d(d.ior, 1.0f, o.z), 0.1f), 0.9f);
+        if(i.z > 0.0f)
+            return (1.0f - mF) * pdf_trans(d, i, o);
+        return mF * pdf_inner(d, i, o);
+    }
+    if(i.z < 0.0f)
+        return d.wt * pdf_trans(d, i, o);
+    const float pd = hemisphere_cos_pdf(i);
+    // pdf_specular_clearcoat, disney.cpp:339-367
+    const Vec h = nrm(add(i, o));
+    float sph, cph, spo_unused, cpo;
+    phi_sc(h, sph, cph);
+    phi_sc(o, spo_unused, cpo);
+    const float spo = cos2sin(cpo);
+    const float Ds = aniso_gtr2(sph, cph, cos2sin(h.z), h.z, d.ax, d.ay);
+    const float ps = smith_aniso(cpo, spo, d.ax, d.ay, tan_th(o)) * Ds / (4.0f * o.z);
+    const float pc = h.z * gtr1(cos2sin(h.z), h.z, d.ccr) / (4.0f * dot(i, h));
+    return d.wd * pd + d.ws * ps + d.wc * pc;
+}
+
+static Vec refl(Vec o, Vec h) { return sub(sclf(2.0f * dot(o, h), h), o); }
+
+static Vec samp_trans(const Dis &d, Vec o, float s0, float s1) // disney.cpp:277-308
+{
+    const Vec h = sample_aniso_gtr2(d.tax, d.tay, s0, s1);
+    if(h.z <= 0.0f || (o.z > 0.0f) != (dot(h, o) > 0.0f))
+        return mk(0, 0, 0);
+    const float eta = o.z > 0.0f ? 1.0f / d.ior : d.ior;
+    const Vec oh = dot(h, o) > 0.0f ? h : neg(h);
+    // refract, disney.cpp:55-74
+    const float ci = dot(o, oh);
+    const float s2t = sq(eta) * (1.0f - sq(ci));
+    const float c2t = 1.0f - s2t;
+    if(c2t <= 0.0f)
+        return mk(0, 0, 0);
+    const float ct = std::sqrt(c2t);
+    const Vec i = nrm(sub(sclf(eta * ci - ct, oh), sclf(eta, o)));
+    if(i.z * o.z > 0.0f || ((i.z > 0.0f) != (dot(h, i) > 0.0f)))
+        return mk(0, 0, 0);
+    return nrm(i);
+}
+
+static BS dis_sample(const Dis &d, Vec o, float s0, float s1, float s2) // disney.cpp:560-658
+{
+    BS r{ mk(0, 0, 0), mk(0, 0, 0), 0.0f };
+    if(std::fabs(o.z) < 1e-4f)
+        return r;
+    Vec i = mk(0, 0, 0);
+    if(o.z < 0.0f)
+    {
+        if(d.trans == 0.0f)
+            return r;
+        const float mF = fmn(fmx(fresnel_d(d.ior, 1.0f, o.z), 0.1f), 0.9f);
+        if(s0 >= mF)
+            i = samp_trans(d, o, s1, s2);
+        else
+        {
+            // sample_inner_refl, disney.cpp:310-331
+            const Vec h = sample_aniso_gtr2(d.tax, d.tay, s1, s2);
+            if(h.z > 0.0f)
+            {
+                const Vec t = refl(o, h);
+                if(!(t.z > 0.0f))
+                    i = nrm(t);
+            }
+        }
+    }
+    else
+    {
+        float sel = s0;
+        if(sel < d.wd)
+            i = hemisphere_cos(s1, s2);
+        else if((sel = sel - d.wd) < d.wt)
+            i = samp_trans(d, o, s1, s2);
+        else if((sel = sel - d.wt) < d.ws)
+        {
+            const Vec h = sample_vndf(o, d.ax, d.ay, s1, s2); // sample_specular, disney.cpp:227-252
+            if(h.z > 0.0f)
+            {
+                const Vec t = refl(o, h);
+                if(t.z > 0.0f)
+                    i = nrm(t);
+            }
+        }
+        else
+        {
+            const Vec h = sample_gtr1(d.ccr, s1, s2); // sample_clearcoat, disney.cpp:254-275
+            if(h.z > 0.0f)
+            {
+                const Vec t = refl(o, h);
+                if(t.z > 0.0f)
+                    i = nrm(t);
+            }
+        }
+    }
+    if(zero(i))
+        return r;
+    r.dir = i;
+    r.f = dis_eval(d, i, o);
+    r.pdf = dis_pdf(d, i, o);
+    const Vec v = dvs(scl(r.f, std::fabs(r.dir.z)), r.pdf); // handle_bad_sample, disney.cpp:566-573
+    if(fmx(fmx(v.x, v.y), v.z) > 10.0f)
+        r = BS{ mk(0, 0, 0), mk(0, 0, 0), 0.0f };
+    return r;
+}
+
+// Lambert component, builtin/material/component/diffuse.cpp
+static BS lam_sample(Vec albedo, Vec o, float s0, float s1)
+{
+    if(o.z <= 0.0f)
+        return BS{ mk(0, 0, 0), mk(0, 0, 0), 0.0f };
+    const Vec i = hemisphere_cos(s0, s1);
+    return BS{ dvs(albedo, PI), i, hemisphere_cos_pdf(i) };
+}
+
+// BSDFAggregate with one component of weight 1 plus ShaderFrame
+// (material/utils/aggregate.cpp:32-229, shader_frame.cpp)
+struct Shd
+{
+    Frm geo, sh;
+    bool st;
+    bool disney;
+    Vec albedo;
+    Dis d;
+};
+
+static bool upper(const Frm &f, Vec w) { return dot(w, f.z) > 0.0f; }
+static bool fringe(const Shd &s, Vec w) { return upper(s.geo, w) != upper(s.sh, w); }
+static Frm flip(const Shd &s, Vec wo) { return fringe(s, wo) ? s.geo : s.sh; }
+static float corr(const Shd &s, const Frm &sh, Vec wi)
+{
+    const float down = std::fabs(dot(s.geo.z, wi));
+    return down > 1e-4f ? std::fabs(dot(sh.z, wi)) / down : 1.0f;
+}
+static float term(const Shd &s, Vec wi)
+{
+    if(!s.st)
+        return 1.0f;
+    const float g = fmn(fmx(dot(s.geo.z, wi) / (dot(s.sh.z, wi) * dot(s.geo.z, s.sh.z)), 0.0f), 1.0f);
+    return -g * g * g + g * g + g;
+}
+
+static BS shd_sample(const Shd &s, Vec wo, float s0, float s1, float s2)
+{
+    const Frm sh = flip(s, wo);
+    const Vec lo = nrm(g2l(sh, wo));
+    const BS c = s.disney ? dis_sample(s.d, lo, s0, s1, s2) : lam_sample(s.albedo, lo, s0, s1);
+    BS r{ mk(0, 0, 0), mk(0, 0, 0), 0.0f };
+    if(c.pdf <= 0.0f)
+        return r;
+    const Vec dir = l2g(sh, c.dir);
+    if(fringe(s, dir))
+        return r;
+    r.dir = dir;
+    r.pdf = c.pdf;
+    r.f = scl(scl(c.f, corr(s, sh, dir)), term(s, dir));
+    return r;
+}
+
+static Vec shd_eval(const Shd &s, Vec wi, Vec wo)
+{
+    const Frm sh = flip(s, wo);
+    const Vec li = nrm(g2l(sh, wi)), lo = nrm(g2l(sh, wo));
+    Vec f;
+    if(s.disney)
+        f = dis_eval(s.d, li, lo);
+    else
+        f = (li.z <= 0.0f || lo.z <= 0.0f) ? mk(0, 0, 0) : dvs(s.albedo, PI);
+    return scl(scl(f, corr(s, sh, wi)), term(s, wi));
+}
+
+static float shd_pdf(const Shd &s, Vec wi, Vec wo)
+{
+    const Frm sh = flip(s, wo);
+    const Vec li = nrm(g2l(sh, wi)), lo = nrm(g2l(sh, wo));
+    if(s.disney)
+        return dis_pdf(s.d, li, lo);
+    return (li.z <= 0.0f || lo.z <= 0.0f) ? 0.0f : hemisphere_cos_pdf(li);
+}
+
+// ------------------------------------------------------------------ lights
+
+static Vec sky_le(const Light &l, Vec to) // gradient_sky.cpp:20-25
+{
+    const float c = dot(l.up, nrm(to));
+    const float s = fmn(fmx(0.5f * (c + 1.0f), 0.0f), 1.0f);
+    return add(scl(l.lower, 1.0f - s), scl(l.upper, s));
+}
+
+struct Shadow
+{
+    bool ok;
+    Vec o, d, li;
+    float t1, pdf;
+};
+
+// sample_light_li_impl + sample_surface_light_li, renderer/wavefront/helper.cpp:10-65,127-142;
+// UniformLightSampler::sample, core/light_sampler.cpp:41-58;
+// MeshLight::sample_li, light/mesh_light.cpp:38-58; TriangleMesh::sample_inline,
+// geometry/triangle_mesh.cpp:188-228; CAliasTable::sample, utils/cmath/calias.cpp:15-21
+static Shadow sample_light(const Scene &s, Vec ref, Vec nor, float l0, float l1, float l2, float lsel)
+{
+    Shadow r{};
+    const int n = (int)s.lights.size();
+    if(n == 0)
+        return r;
+    int idx = (int)(lsel * (float)n);
+    idx = idx < n - 1 ? idx : n - 1;
+    const float sel_pdf = 1.0f / (float)n;
+    const Light &l = s.lights[idx];
+    r.ok = true;
+    if(l.type == 0)
+    {
+        const Geo &g = s.geos[l.geo];
+        const float nu = (float)g.n * l0;
+        uint32_t k = (uint32_t)nu;
+        k = k < g.n - 1 ? k : g.n - 1;
+        const uint32_t prim = (nu - (float)k) <= g.accept[k] ? k : g.alias[k];
+        const float t = std::sqrt(l1);
+        const float b1 = 1.0f - t, b2 = t * l2;
+        const uint32_t i = g.off + prim;
+        const Vec lp = add(add(s.la[i], scl(s.lba[i], b1)), scl(s.lca[i], b2));
+        const Inst &in = s.insts[l.inst];
+        const Vec spos = xpoint(in.xf, lp);
+        const Vec snor = nrm(xnor(in.xf, nrm(s.gz[i])));
+        const Vec p2r = sub(ref, spos);
+        const float dist2 = dot(p2r, p2r);
+        const float pdf = 1.0f / (l.scale * l.scale) * (1.0f / g.total_area) * dist2 / std::fabs(dot(snor, nrm(p2r)));
+        const Vec rad = dot(p2r, snor) > 0.0f ? l.intensity : mk(0, 0, 0);
+        const Vec diff = sub(spos, ref);
+        const Vec dst = offset_dir(spos, snor, neg(diff));
+        r.o = offset_dir(ref, nor, diff);
+        r.d = sub(dst, r.o);
+        r.t1 = 1.0f;
+        r.li = rad;
+        r.pdf = sel_pdf * pdf;
+    }
+    else
+    {
+        const Vec dir = sphere_uniform(l0, l1);
+        r.o = offset_dir(ref, nor, dir);
+        r.d = dir;
+        r.t1 = FMAX;
+        r.li = sky_le(l, dir);
+        r.pdf = sel_pdf * (1.0f / (4.0f * PI));
+    }
+    return r;
+}
+
+// ------------------------------------------------------------------ path tracer
+
+struct Params
+{
+    int32_t w, h, spp, min_depth, max_depth;
+    float rr_thr, rr_p;
+    Vec eye, lbc, fx, fy;
+};
+
+static inline void splat(std::vector<double> &film, uint32_t pixel, Vec v)
+{
+    // Film::splat drops non-finite values (core/film.cpp:145-163)
+    if(!finite3(v))
+        return;
+    film[3 * (size_t)pixel + 0] += (double)v.x;
+    film[3 * (size_t)pixel + 1] += (double)v.y;
+    film[3 * (size_t)pixel + 2] += (double)v.z;
+}
+
+// One camera path: generate_kernel (wavefront/generate.cpp:33-87), then per
+// bounce the trace programs (trace.cpp:38-78), shade_kernel
+// (shade.cpp:29-236) and the shadow programs (shadow.cpp:24-76).
+static void trace_path(const Scene &s, const Params &p, uint64_t path_index, std::vector<double> &film,
+                       uint64_t &ext_rays, uint64_t &shadow_rays)
+{
+    const uint64_t pixels = (uint64_t)p.w * (uint64_t)p.h;
+    const uint32_t pixel = (uint32_t)(path_index % pixels);
+    const int32_t sample = (int32_t)(path_index / pixels);
+    const uint32_t px = pixel % (uint32_t)p.w, py = pixel / (uint32_t)p.w;
+    Halton hs;
+    hs.init(p.w, p.h, px, py, sample);
+    const float fx = hs.get1d() - 0.5f; // BoxFilter::sample, film_filter/box.cpp:5-9
+    const float fy = hs.get1d() - 0.5f;
+    const float film_x = ((float)px + 0.5f + fx) / (float)p.w;
+    const float film_y = ((float)py + 0.5f + fy) / (float)p.h;
+    (void)hs.get1d();
+    // PinholeCamera::generate_ray_inline, camera/pinhole.cpp:84-99
+    const Vec dst = add(add(p.lbc, sclf(film_x, p.fx)), sclf(film_y, p.fy));
+    Vec o = p.eye, d = nrm(sub(dst, p.eye));
+    Vec beta = mk(1, 1, 1), rad = mk(0, 0, 0), beta_le = mk(1, 1, 1);
+    float bsdf_pdf = -1.0f;
+    for(int depth = 0;; ++depth)
+    {
+        ++ext_rays;
+        const Hit h = trace(s, BTRC_TRI_WATERTIGHT, false, true, o, d, 0.0f, FMAX);
+        if(h.gid == 0xffffffffu)
+        {
+            // eval_miss_le, helper.cpp:144-172
+            Vec sp = mk(0, 0, 0);
+            if(s.env >= 0)
+            {
+                const Vec le = sky_le(s.lights[s.env], d);
+                if(bsdf_pdf < 0.0f)
+                    sp = dvs(mul(beta_le, le), -bsdf_pdf);
+                else
+                {
+                    const float lp = (1.0f / (float)s.lights.size()) * (1.0f / (4.0f * PI));
+                    sp = dvs(mul(beta_le, le), bsdf_pdf + lp);
+                }
+            }
+            splat(film, pixel, add(sp, rad));
+            return;
+        }
+        const Inst &in = s.insts[s.tinst[h.gid]];
+        const SP sp = hitinfo(s, o, d, in, s.tprim[h.gid], h.t, h.u, h.v);
+        // handle_intersected_light, helper.cpp:174-209
+        if(in.light >= 0 && s.lights[in.light].type == 0)
+        {
+            const Light &l = s.lights[in.light];
+            const Vec le = dot(sp.frame.z, neg(d)) >= 0.0f ? l.intensity : mk(0, 0, 0);
+            Vec c;
+            if(bsdf_pdf < 0.0f)
+                c = dvs(mul(beta_le, le), -bsdf_pdf);
+            else
+            {
+                float lpdf = 0.0f;
+                const Vec toref = sub(o, sp.pos);
+                if(dot(sp.frame.z, toref) > 0.0f)
+                {
+                    const float d2 = dot(toref, toref);
+                    const float d3 = std::sqrt(d2) * d2;
+                    lpdf = 1.0f / (l.scale * l.scale) * (1.0f / s.geos[l.geo].total_area) * d3
+                         / std::fabs(dot(sp.frame.z, toref));
+                }
+                c = dvs(mul(beta_le, le), bsdf_pdf + (1.0f / (float)s.lights.size()) * lpdf);
+            }
+            rad = add(rad, c);
+        }
+        // simple_russian_roulette, helper.cpp:67-97
+        if(depth >= p.min_depth)
+        {
+            if(depth >= p.max_depth)
+            {
+                splat(film, pixel, rad);
+                return;
+            }
+            const float u = hs.get1d();
+            if(lum(beta) < p.rr_thr)
+            {
+                if(u > p.rr_p)
+                {
+                    splat(film, pixel, rad);
+                    return;
+                }
+                beta = dvs(beta, p.rr_p);
+            }
+        }
+        float l0, l1, l2;
+        hs.get3d(l0, l1, l2);
+        const float lsel = hs.get1d();
+        Shadow sh = sample_light(s, sp.pos, sp.frame.z, l0, l1, l2, lsel);
+
+        const btrc_material_desc &m = s.mats[in.mat];
+        Shd shd;
+        shd.geo = sp.frame;
+        shd.sh = rot_z(sp.frame, sp.iz);
+        shd.st = m.shadow_terminator != 0;
+        shd.disney = m.type == BTRC_MATERIAL_DISNEY;
+        shd.albedo = mk(m.color[0], m.color[1], m.color[2]);
+        if(shd.disney)
+            shd.d = dis_make(m);
+        float s0, s1, s2;
+        hs.get3d(s0, s1, s2);
+        const BS bs = shd_sample(shd, neg(d), s0, s1, s2);
+
+        if(sh.ok)
+        {
+            const Vec f = shd_eval(shd, sh.d, neg(d));
+            if(!zero(f))
+            {
+                const float pb = shd_pdf(shd, sh.d, neg(d));
+                const float c = std::fabs(dot(nrm(sh.d), sp.frame.z));
+                const Vec bl = dvs(scl(mul(mul(sh.li, beta), f), c), pb + sh.pdf);
+                ++shadow_rays;
+                if(trace(s, BTRC_TRI_WATERTIGHT, true, true, sh.o, sh.d, 0.0f, sh.t1).gid == 0xffffffffu)
+                    splat(film, pixel, bl);
+            }
+        }
+        if(zero(bs.f))
+        {
+            splat(film, pixel, rad);
+            return;
+        }
+        const Vec dir = nrm(bs.dir);
+        const float c = std::fabs(dot(dir, sp.frame.z));
+        beta_le = mul(scl(beta, c), bs.f);
+        beta = dvs(beta_le, bs.pdf);
+        bsdf_pdf = bs.pdf;
+        o = offset_dir(sp.pos, sp.frame.z, dir);
+        d = dir;
+    }
+}
+
+} // namespace orc
+
+// ================================================================== C API
+
+extern "C" {
+
+void *oracle_scene_create(const btrc_scene_desc *sd) { return orc::make_scene(sd); }
+void oracle_scene_destroy(void *s) { delete static_cast<orc::Scene *>(s); }
+
+int64_t oracle_scene_bvh_nodes(void *s) { return (int64_t)static_cast<orc::Scene *>(s)->nodes.size(); }
+
+// closest hits as btrc_hit, or occlusion flags when any != 0
+void oracle_trace(void *sp, const btrc_ray *rays, int64_t n, int32_t mode, int32_t any, int32_t use_bvh,
+                  btrc_hit *hits, uint8_t *occluded, int32_t threads)
+{
+    const orc::Scene &s = *static_cast<orc::Scene *>(sp);
+    auto work = [&](int64_t b, int64_t e) {
+        for(int64_t i = b; i < e; ++i)
+        {
+            const btrc_ray &r = rays[i];
+            const orc::Hit h = orc::trace(s, mode, any != 0, use_bvh != 0, orc::mk(r.o[0], r.o[1], r.o[2]),
+                                          orc::mk(r.d[0], r.d[1], r.d[2]), r.tmin, r.tmax);
+            if(any)
+                occluded[i] = h.gid != 0xffffffffu;
+            else
+            {
+                btrc_hit &o = hits[i];
+                const bool hit = h.gid != 0xffffffffu;
+                o.t = hit ? h.t : r.tmax;
+                o.instance = hit ? s.tinst[h.gid] : 0xffffffffu;
+                o.prim = hit ? s.tprim[h.gid] : 0xffffffffu;
+                o.u = h.u;
+                o.v = h.v;
+            }
+        }
+    };
+    threads = std::max(1, threads);
+    std::vector<std::thread> pool;
+    const int64_t chunk = (n + threads - 1) / threads;
+    for(int t = 0; t < threads; ++t)
+    {
+        const int64_t b = t * chunk, e = std::min<int64_t>(n, b + chunk);
+        if(b < e)
+            pool.emplace_back(work, b, e);
+    }
+    for(auto &th : pool)
+        th.join();
+}
+
+// Renders paths [first, first + count) of the uniform path order and adds
+// their radiance to film (3 doubles per pixel, not normalised). Returns the
+// number of extension and shadow rays traced through ext/shadow.
+void oracle_render(void *sp, const btrc_camera_desc *cam, const btrc_render_params *rp, uint64_t first,
+                   uint64_t count, double *film_out, int32_t threads, uint64_t *ext_out, uint64_t *shadow_out)
+{
+    const orc::Scene &s = *static_cast<orc::Scene *>(sp);
+    orc::Params p;
+    p.w = rp->width;
+    p.h = rp->height;
+    p.spp = rp->spp;
+    p.min_depth = rp->min_depth;
+    p.max_depth = rp->max_depth;
+    p.rr_thr = rp->rr_threshold;
+    p.rr_p = rp->rr_cont_prob;
+    // PinholeCamera::commit, camera/pinhole.cpp:55-77
+    {
+        using namespace orc;
+        const Vec eye = mk(cam->eye[0], cam->eye[1], cam->eye[2]);
+        const Vec fwd = nrm(sub(mk(cam->dst[0], cam->dst[1], cam->dst[2]), eye));
+        const Vec ex = nrm(cross(fwd, mk(cam->up[0], cam->up[1], cam->up[2])));
+        const Vec ey = nrm(cross(fwd, ex));
+        const float yl = 2.0f * std::tan(0.5f * cam->fov_y_deg * PI / 180.0f);
+        const float xl = cam->w_over_h * yl;
+        p.eye = eye;
+        p.lbc = sub(add(eye, fwd), sclf(0.5f, add(sclf(xl, ex), sclf(yl, ey))));
+        p.fx = sclf(xl, ex);
+        p.fy = sclf(yl, ey);
+    }
+    const size_t pixels = (size_t)p.w * (size_t)p.h;
+    threads = std::max(1, threads);
+    std::vector<std::vector<double>> films(threads, std::vector<double>(3 * pixels, 0.0));
+    std::vector<uint64_t> ext(threads, 0), shd(threads, 0);
+    std::vector<std::thread> pool;
+    for(int t = 0; t < threads; ++t)
+    {
+        pool.emplace_back([&, t] {
+            for(uint64_t i = (uint64_t)t; i < count; i += (uint64_t)threads)
+                orc::trace_path(s, p, first + i, films[t], ext[t], shd[t]);
+        });
+    }
+    for(auto &th : pool)
+        th.join();
+    uint64_t e = 0, sh = 0;
+    for(int t = 0; t < threads; ++t)
+    {
+        for(size_t k = 0; k < 3 * pixels; ++k)
+            film_out[k] += films[t][k];
+        e += ext[t];
+        sh += shd[t];
+    }
+    if(ext_out) *ext_out = e;
+    if(shadow_out) *shadow_out = sh;
+}
+
+void oracle_halton(int32_t w, int32_t h, const uint32_t *px, const uint32_t *py, const int32_t *si, int32_t n,
+                   int32_t dims, float *out)
+{
+    for(int32_t i = 0; i < n; ++i)
+    {
+        orc::Halton hs;
+        hs.init(w, h, px[i], py[i], si[i]);
+        for(int32_t k = 0; k < dims; ++k)
+            out[(size_t)i * dims + k] = hs.get1d();
+    }
+}
+
+void oracle_detmath(int32_t fn, const float *x, const float *y, int32_t n, float *out)
+{
+    namespace dm = btrc_detmath;
+    for(int32_t i = 0; i < n; ++i)
+    {
+        const float a = x[i];
+        float r;
+        switch(fn)
+        {
+        case 0: r = dm::sin(a); break;
+        case 1: r = dm::cos(a); break;
+        case 2: r = dm::log(a); break;
+        case 3: r = dm::exp(a); break;
+        case 4: r = dm::pow(a, y[i]); break;
+        case 5: r = std::sqrt(a); break;
+        default: r = a / y[i]; break;
+        }
+        out[i] = r;
+    }
+}
+
+// BSDF evaluation hooks for the statistical tests: a material at a surface
+// whose geometric and shading frames are the identity frame.
+void oracle_bsdf_sample(const btrc_material_desc *m, const float *wo, const float *u, int32_t n,
+                        float *f_out, float *dir_out, float *pdf_out)
+{
+    using namespace orc;
+    Shd s;
+    s.geo = Frm{ mk(1, 0, 0), mk(0, 1, 0), mk(0, 0, 1) };
+    s.sh = s.geo;
+    s.st = m->shadow_terminator != 0;
+    s.disney = m->type == BTRC_MATERIAL_DISNEY;
+    s.albedo = mk(m->color[0], m->color[1], m->color[2]);
+    if(s.disney)
+        s.d = dis_make(*m);
+    for(int32_t i = 0; i < n; ++i)
+    {
+        const BS b = shd_sample(s, mk(wo[3 * i], wo[3 * i + 1], wo[3 * i + 2]), u[3 * i], u[3 * i + 1], u[3 * i + 2]);
+        f_out[3 * i] = b.f.x; f_out[3 * i + 1] = b.f.y; f_out[3 * i + 2] = b.f.z;
+        dir_out[3 * i] = b.dir.x; dir_out[3 * i + 1] = b.dir.y; dir_out[3 * i + 2] = b.dir.z;
+        pdf_out[i] = b.pdf;
+    }
+}
+
+void oracle_bsdf_eval(const btrc_material_desc *m, const float *wi, const float *wo, int32_t n,
+                      float *f_out, float *pdf_out)
+{
+    using namespace orc;
+    Shd s;
+    s.geo = Frm{ mk(1, 0, 0), mk(0, 1, 0), mk(0, 0, 1) };
+    s.sh = s.geo;
+    s.st = m->shadow_terminator != 0;
+    s.disney = m->type == BTRC_MATERIAL_DISNEY;
+    s.albedo = mk(m->color[0], m->color[1], m->color[2]);
+    if(s.disney)
+        s.d = dis_make(*m);
+    for(int32_t i = 0; i < n; ++i)
+    {
+        const Vec a = mk(wi[3 * i], wi[3 * i + 1], wi[3 * i + 2]);
+        const Vec b = mk(wo[3 * i], wo[3 * i + 1], wo[3 * i + 2]);
+        const Vec f = shd_eval(s, a, b);
+        f_out[3 * i] = f.x; f_out[3 * i + 1] = f.y; f_out[3 * i + 2] = f.z;
+        pdf_out[i] = shd_pdf(s, a, b);
+    }
+}
+
+} // extern "C"
+
+// The lib/bvh image known-answer test: camera rays of lib/bvh's benchmark
+// renderer (lib/bvh/test/benchmark.cpp:120-145, defaults 1080x720) and its
+// |normal| shading written as 8-bit RGB rows bottom-up (benchmark.cpp:170-176,
+// 253-264).
+extern "C" void oracle_kat_rays(int32_t width, int32_t height, const float *eye, const float *dir, const float *up,
+                                float fov, btrc_ray *out)
+{
+    using namespace orc;
+    const Vec d = nrm(mk(dir[0], dir[1], dir[2]));
+    Vec iu = nrm(cross(d, mk(up[0], up[1], up[2])));
+    Vec iv = nrm(cross(iu, d));
+    const float iw = std::tan(fov * (float)(3.14159265 * (1.0 / 180.0) * 0.5));
+    const float ratio = (float)height / (float)width;
+    iu = scl(iu, iw);
+    iv = scl(scl(iv, iw), ratio);
+    for(int32_t i = 0; i < width; ++i)
+        for(int32_t j = 0; j < height; ++j)
+        {
+            const float u = 2.0f * ((float)i + 0.5f) / (float)width - 1.0f;
+            const float v = 2.0f * ((float)j + 0.5f) / (float)height - 1.0f;
+            const Vec rd = nrm(add(add(scl(iu, u), scl(iv, v)), d));
+            btrc_ray &r = out[(size_t)width * j + i];
+            r.o[0] = eye[0]; r.o[1] = eye[1]; r.o[2] = eye[2];
+            r.d[0] = rd.x; r.d[1] = rd.y; r.d[2] = rd.z;
+            r.tmin = 0.0f;
+            r.tmax = FMAX;
+        }
+}
+
+// tris: 9 floats per triangle; hits in the ray order of oracle_kat_rays;
+// out: width * height * 3 bytes, first row = top of the image
+extern "C" void oracle_kat_pixels(int32_t width, int32_t height, const float *tris, const btrc_hit *hits,
+                                  uint8_t *out)
+{
+    using namespace orc;
+    for(int32_t j = 0; j < height; ++j)
+        for(int32_t i = 0; i < width; ++i)
+        {
+            const btrc_hit &h = hits[(size_t)width * j + i];
+            float px[3] = { 0, 0, 0 };
+            if(h.instance != 0xffffffffu)
+            {
+                const float *p = tris + 9 * (size_t)h.prim;
+                const Vec p0 = mk(p[0], p[1], p[2]), p1 = mk(p[3], p[4], p[5]), p2 = mk(p[6], p[7], p[8]);
+                const Vec n = nrm(cross(sub(p0, p1), sub(p2, p0)));
+                px[0] = std::fabs(n.x);
+                px[1] = std::fabs(n.y);
+                px[2] = std::fabs(n.z);
+            }
+            uint8_t *o = out + 3 * ((size_t)width * (height - 1 - j) + i);
+            for(int c = 0; c < 3; ++c)
+                o[c] = (uint8_t)std::max(std::min(px[c] * 255.0f, 255.0f), 0.0f);
+        }
+}

@@ -1,0 +1,210 @@
+"""Loader of the CPU oracle (oracle/oracle.cpp -> oracle/liboracle.so) and of
+the reference's own CPU traversal built from lib/bvh (oracle/_ref).
+
+Test infrastructure only: imported by tests/, __graft_entry__.smoke() and
+the cpu_baseline leg of bench.py, never by the product package.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from btrc_amd import _abi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(HERE, "liboracle.so")
+REF_SO = os.path.join(HERE, "_ref", "libbvhref.so")
+
+_lib = None
+_ref = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(ORACLE_SO):
+            raise RuntimeError(f"{ORACLE_SO} missing: build it with __graft_entry__.build()")
+        L = C.CDLL(ORACLE_SO)
+        vp, P = C.c_void_p, C.POINTER
+        L.oracle_scene_create.restype = vp
+        L.oracle_scene_create.argtypes = [P(_abi.SceneDesc)]
+        L.oracle_scene_destroy.argtypes = [vp]
+        L.oracle_scene_bvh_nodes.restype = C.c_int64
+        L.oracle_scene_bvh_nodes.argtypes = [vp]
+        L.oracle_trace.argtypes = [vp, vp, C.c_int64, C.c_int32, C.c_int32, C.c_int32, vp, vp, C.c_int32]
+        L.oracle_render.argtypes = [vp, P(_abi.CameraDesc), P(_abi.RenderParams), C.c_uint64, C.c_uint64, vp,
+                                    C.c_int32, P(C.c_uint64), P(C.c_uint64)]
+        L.oracle_halton.argtypes = [C.c_int32, C.c_int32, vp, vp, vp, C.c_int32, C.c_int32, vp]
+        L.oracle_detmath.argtypes = [C.c_int32, vp, vp, C.c_int32, vp]
+        L.oracle_bsdf_sample.argtypes = [P(_abi.MaterialDesc), vp, vp, C.c_int32, vp, vp, vp]
+        L.oracle_bsdf_eval.argtypes = [P(_abi.MaterialDesc), vp, vp, C.c_int32, vp, vp]
+        L.oracle_kat_rays.argtypes = [C.c_int32, C.c_int32, vp, vp, vp, C.c_float, vp]
+        L.oracle_kat_pixels.argtypes = [C.c_int32, C.c_int32, vp, vp, vp]
+        _lib = L
+    return _lib
+
+
+def ref_available():
+    return os.path.exists(REF_SO)
+
+
+def ref():
+    """lib/bvh (the reference's CPU traversal) compiled by oracle/build_ref.sh"""
+    global _ref
+    if _ref is None:
+        if not ref_available():
+            raise RuntimeError(f"{REF_SO} missing: run oracle/build_ref.sh where the reference is present")
+        L = C.CDLL(REF_SO)
+        vp = C.c_void_p
+        L.bvhref_create.restype = vp
+        L.bvhref_create.argtypes = [vp, C.c_int64, C.c_int32]
+        L.bvhref_destroy.argtypes = [vp]
+        L.bvhref_trace.argtypes = [vp, vp, C.c_int64, C.c_int32, vp, vp, vp, vp, vp, vp, C.c_int32]
+        _ref = L
+    return _ref
+
+
+class OracleScene:
+    def __init__(self, scene):
+        self.desc, self.holder = scene.to_desc()
+        self.h = lib().oracle_scene_create(C.byref(self.desc))
+
+    def __del__(self):
+        try:
+            lib().oracle_scene_destroy(self.h)
+        except Exception:
+            pass
+
+    def trace(self, rays, tri_test=_abi.TRI_WATERTIGHT, any_hit=False, use_bvh=True, threads=8):
+        rays = np.ascontiguousarray(rays, dtype=_abi.ray_dtype())
+        n = len(rays)
+        if any_hit:
+            occ = np.empty(n, dtype=np.uint8)
+            lib().oracle_trace(self.h, rays.ctypes.data, n, tri_test, 1, 1 if use_bvh else 0, None,
+                               occ.ctypes.data, threads)
+            return occ.astype(bool)
+        hits = np.empty(n, dtype=_abi.hit_dtype())
+        lib().oracle_trace(self.h, rays.ctypes.data, n, tri_test, 0, 1 if use_bvh else 0, hits.ctypes.data,
+                           None, threads)
+        return hits
+
+    def render(self, camera, params, first=0, count=None, threads=8):
+        """Radiance sums (H, W, 3) float64 of paths [first, first + count) and
+        the (extension, shadow) ray counts."""
+        w, h = params.width, params.height
+        if count is None:
+            count = w * h * params.spp
+        film = np.zeros((h, w, 3), dtype=np.float64)
+        cam = camera.to_desc(w, h)
+        e, s = C.c_uint64(), C.c_uint64()
+        lib().oracle_render(self.h, C.byref(cam), C.byref(params), first, count, film.ctypes.data, threads,
+                            C.byref(e), C.byref(s))
+        return film, e.value, s.value
+
+
+def halton(width, height, px, py, sample_index, dims):
+    px = np.ascontiguousarray(px, dtype=np.uint32)
+    py = np.ascontiguousarray(py, dtype=np.uint32)
+    si = np.ascontiguousarray(sample_index, dtype=np.int32)
+    out = np.empty((len(px), dims), dtype=np.float32)
+    lib().oracle_halton(width, height, px.ctypes.data, py.ctypes.data, si.ctypes.data, len(px), dims,
+                        out.ctypes.data)
+    return out
+
+
+def detmath(fn, x, y=None):
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    y = np.ascontiguousarray(np.ones_like(x) if y is None else y, dtype=np.float32)
+    out = np.empty_like(x)
+    lib().oracle_detmath(fn, x.ctypes.data, y.ctypes.data, len(x), out.ctypes.data)
+    return out
+
+
+def _material(**kw):
+    m = _abi.MaterialDesc()
+    m.type = kw.get("type", _abi.MATERIAL_DISNEY)
+    m.shadow_terminator = kw.get("shadow_terminator", 1)
+    m.color = _abi.F3(*kw.get("color", (0.5, 0.5, 0.5)))
+    defaults = dict(metallic=0.0, roughness=0.5, specular=1.0, specular_tint=0.0, anisotropic=0.0, sheen=0.0,
+                    sheen_tint=0.0, clearcoat=0.0, clearcoat_gloss=1.0, transmission=0.0,
+                    transmission_roughness=None, ior=1.5)
+    for k, v in defaults.items():
+        v = kw.get(k, v)
+        if k == "transmission_roughness" and v is None:
+            v = kw.get("roughness", 0.5)
+        setattr(m, k, float(v))
+    return m
+
+
+def bsdf_sample(material, wo, u):
+    """BSDF samples at a surface with identity frames: (f, dir, pdf)"""
+    m = _material(**material)
+    wo = np.ascontiguousarray(wo, dtype=np.float32)
+    u = np.ascontiguousarray(u, dtype=np.float32)
+    n = len(wo)
+    f = np.empty((n, 3), np.float32)
+    d = np.empty((n, 3), np.float32)
+    p = np.empty(n, np.float32)
+    lib().oracle_bsdf_sample(C.byref(m), wo.ctypes.data, u.ctypes.data, n, f.ctypes.data, d.ctypes.data,
+                             p.ctypes.data)
+    return f, d, p
+
+
+def bsdf_eval(material, wi, wo):
+    m = _material(**material)
+    wi = np.ascontiguousarray(wi, dtype=np.float32)
+    wo = np.ascontiguousarray(wo, dtype=np.float32)
+    n = len(wi)
+    f = np.empty((n, 3), np.float32)
+    p = np.empty(n, np.float32)
+    lib().oracle_bsdf_eval(C.byref(m), wi.ctypes.data, wo.ctypes.data, n, f.ctypes.data, p.ctypes.data)
+    return f, p
+
+
+class RefBvh:
+    """lib/bvh BinnedSahBuilder<16> BVH + SingleRayTraverser over flattened
+    world triangles (9 floats each)."""
+
+    def __init__(self, tris, builder=0):
+        self.tris = np.ascontiguousarray(tris, dtype=np.float32).reshape(-1, 9)
+        self.h = ref().bvhref_create(self.tris.ctypes.data, len(self.tris), builder)
+
+    def __del__(self):
+        try:
+            ref().bvhref_destroy(self.h)
+        except Exception:
+            pass
+
+    def trace(self, rays, any_hit=False, threads=8):
+        """returns prim (uint32, 0xffffffff miss), t, u, v, per-ray traversal
+        steps and primitive intersections"""
+        rays = np.ascontiguousarray(rays, dtype=_abi.ray_dtype())
+        n = len(rays)
+        prim = np.empty(n, np.uint32)
+        t = np.empty(n, np.float32)
+        u = np.empty(n, np.float32)
+        v = np.empty(n, np.float32)
+        steps = np.empty(n, np.uint32)
+        isect = np.empty(n, np.uint32)
+        ref().bvhref_trace(self.h, rays.ctypes.data, n, 1 if any_hit else 0, prim.ctypes.data, t.ctypes.data,
+                           u.ctypes.data, v.ctypes.data, steps.ctypes.data, isect.ctypes.data, threads)
+        return prim, t, u, v, steps, isect
+
+
+def kat_rays(width, height, eye, direction, up, fov):
+    """camera rays of lib/bvh's benchmark renderer (ray index = width * row + column)"""
+    out = np.empty(width * height, dtype=_abi.ray_dtype())
+    e = np.asarray(eye, np.float32)
+    d = np.asarray(direction, np.float32)
+    u = np.asarray(up, np.float32)
+    lib().oracle_kat_rays(width, height, e.ctypes.data, d.ctypes.data, u.ctypes.data, fov, out.ctypes.data)
+    return out
+
+
+def kat_pixels(width, height, tris, hits):
+    """8-bit |normal| image (height, width, 3), top row first"""
+    tris = np.ascontiguousarray(tris, dtype=np.float32)
+    hits = np.ascontiguousarray(hits, dtype=_abi.hit_dtype())
+    out = np.empty((height, width, 3), dtype=np.uint8)
+    lib().oracle_kat_pixels(width, height, tris.ctypes.data, hits.ctypes.data, out.ctypes.data)
+    return out

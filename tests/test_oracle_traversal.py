@@ -1,0 +1,126 @@
+"""Pins the oracle's traversal on the reference's own test vectors:
+lib/bvh's simple_example (lib/bvh/test/simple_example.cpp:63-84), its Cornell
+image known-answer test (lib/bvh/test/CMakeLists.txt:57-82) and lib/bvh's
+closest hits on fixed ray batches (tests/golden/make_golden.py)."""
+import os
+
+import numpy as np
+import pytest
+from PIL import Image
+
+from btrc_amd import _abi, scenes
+from oracle import oracle
+
+from util import random_rays, single_mesh_scene
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+SIMPLE_TRIS = np.array([[1, -1, 1, 1, 1, 1, -1, 1, 1], [1, -1, 1, -1, -1, 1, -1, 1, 1]], np.float32)
+
+
+def simple_ray():
+    r = np.zeros(1, _abi.ray_dtype())
+    r["o"], r["d"], r["tmin"], r["tmax"] = (0, 0, 0), (0, 0, 1), 0.0, 100.0
+    return r
+
+
+@pytest.mark.parametrize("mode", [_abi.TRI_MOLLER_TRUMBORE, _abi.TRI_WATERTIGHT])
+def test_simple_example(mode):
+    # simple_example prints: primitive 0, distance 1, u -0, v 0.5. The ray
+    # passes through the shared edge; the lowest triangle index wins the tie.
+    h = oracle.OracleScene(single_mesh_scene(SIMPLE_TRIS)).trace(simple_ray(), tri_test=mode)[0]
+    assert h["prim"] == 0 and h["t"] == 1.0 and h["v"] == 0.5 and h["u"] == 0.0
+    if mode == _abi.TRI_MOLLER_TRUMBORE:
+        assert np.signbit(h["u"])  # -0 exactly as lib/bvh prints it
+
+
+def _golden_compare(hits, g, tris):
+    prim, t, u, v = g["prim"], g["t"], g["u"], g["v"]
+    hit = prim != 0xFFFFFFFF
+    assert np.array_equal(hits["prim"] != 0xFFFFFFFF, hit)
+    same = hits["prim"] == prim
+    # lib/bvh keeps the last of equal-distance hits, the oracle the lowest
+    # index: where the triangles differ, both must be at the same t
+    tie = hit & ~same
+    assert np.array_equal(hits["t"][tie].view(np.uint32), t[tie].view(np.uint32))
+    # the two tied triangles are hit at the same t by lib/bvh's own test too
+    m = hit & same
+    for k in ("t", "u", "v"):
+        assert np.array_equal(hits[k][m].view(np.uint32), g[k][m].view(np.uint32)), k
+    return tie.sum()
+
+
+def test_mt_matches_lib_bvh_cornell_golden():
+    g = np.load(os.path.join(GOLDEN, "bvhref_cornell.npz"))
+    tris = scenes.load_obj_triangles(os.path.join(GOLDEN, "cornell_box.obj"))
+    hits = oracle.OracleScene(single_mesh_scene(tris)).trace(g["rays"], tri_test=_abi.TRI_MOLLER_TRUMBORE)
+    ties = _golden_compare(hits, g, tris)
+    assert ties < len(hits) // 20
+
+
+def test_mt_matches_lib_bvh_field_golden():
+    g = np.load(os.path.join(GOLDEN, "bvhref_field.npz"))
+    import make_golden_shim as mg
+    scene = mg.field_scene()
+    os_ = oracle.OracleScene(scene)
+    hits = os_.trace(g["rays"], tri_test=_abi.TRI_MOLLER_TRUMBORE)
+    # hits report (instance, prim); lib/bvh the flattened index = 256 * instance + prim
+    flat = hits.copy()
+    ok = hits["prim"] != 0xFFFFFFFF
+    flat["prim"][ok] = hits["instance"][ok] * 256 + hits["prim"][ok]
+    _golden_compare(flat, g, g["world"])
+
+
+def test_watertight_agrees_with_lib_bvh_off_edges():
+    g = np.load(os.path.join(GOLDEN, "bvhref_cornell.npz"))
+    tris = scenes.load_obj_triangles(os.path.join(GOLDEN, "cornell_box.obj"))
+    hits = oracle.OracleScene(single_mesh_scene(tris)).trace(g["rays"], tri_test=_abi.TRI_WATERTIGHT)
+    hit = g["prim"] != 0xFFFFFFFF
+    assert np.array_equal(hits["prim"] != 0xFFFFFFFF, hit)
+    m = hit & (hits["prim"] == g["prim"])
+    assert m.sum() > 0.9 * hit.sum()
+    np.testing.assert_allclose(hits["t"][m], g["t"][m], rtol=1e-5, atol=1e-6)
+
+
+@pytest.mark.parametrize("mode", [_abi.TRI_MOLLER_TRUMBORE, _abi.TRI_WATERTIGHT])
+def test_cornell_image_known_answer(mode):
+    # lib/bvh renders |normal| of the closest hit at 1080x720 from eye
+    # (0, 0.9, 2.5), dir (0, 0.001, -1), fov 60; the image must equal
+    # cornell_box_reference.png exactly
+    tris = scenes.load_obj_triangles(os.path.join(GOLDEN, "cornell_box.obj"))
+    rays = oracle.kat_rays(1080, 720, (0, 0.9, 2.5), (0, 0.001, -1), (0, 1, 0), 60.0)
+    hits = oracle.OracleScene(single_mesh_scene(tris)).trace(rays, tri_test=mode)
+    img = oracle.kat_pixels(1080, 720, tris, hits)
+    ref = np.asarray(Image.open(os.path.join(GOLDEN, "cornell_box_reference.png")))
+    assert (img != ref).any(axis=2).sum() == 0
+
+
+@pytest.mark.parametrize("mode", [_abi.TRI_MOLLER_TRUMBORE, _abi.TRI_WATERTIGHT])
+def test_bvh_equals_brute_force(mode):
+    scene = scenes.sphere_field(instances=32, lights=4, seed=3, extent=12.0)[0]
+    os_ = oracle.OracleScene(scene)
+    rays = random_rays(20000, [0, 0, 0], [12, 12, 12], 5)
+    a = os_.trace(rays, tri_test=mode, use_bvh=True)
+    b = os_.trace(rays, tri_test=mode, use_bvh=False)
+    assert a.tobytes() == b.tobytes()
+    rays["tmax"] = np.float32(3.0)
+    assert np.array_equal(os_.trace(rays, tri_test=mode, any_hit=True, use_bvh=True),
+                          os_.trace(rays, tri_test=mode, any_hit=True, use_bvh=False))
+
+
+def test_any_hit_is_closest_hit_within_tmax():
+    scene = scenes.cornell()[0]
+    os_ = oracle.OracleScene(scene)
+    rays = random_rays(20000, [-0.9, 0.1, -0.9], [0.9, 1.9, 0.9], 9, tmax=np.float32(1.0))
+    occ = os_.trace(rays, any_hit=True)
+    hits = os_.trace(rays)
+    assert np.array_equal(occ, hits["instance"] != 0xFFFFFFFF)
+
+
+@pytest.mark.skipif(not oracle.ref_available(), reason="oracle/_ref (lib/bvh) not built")
+def test_mt_matches_lib_bvh_live():
+    tris = scenes.load_obj_triangles(os.path.join(GOLDEN, "cornell_box.obj"))
+    rays = random_rays(50000, [-1, 0, -1], [1, 2, 1], 21)
+    prim, t, u, v, _, _ = oracle.RefBvh(tris).trace(rays)
+    hits = oracle.OracleScene(single_mesh_scene(tris)).trace(rays, tri_test=_abi.TRI_MOLLER_TRUMBORE)
+    _golden_compare(hits, dict(prim=prim, t=t, u=u, v=v), tris)
