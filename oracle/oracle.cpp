@@ -1583,3 +1583,11 @@ extern "C" void oracle_kat_pixels(int32_t width, int32_t height, const float *tr
                 o[c] = (uint8_t)std::max(std::min(px[c] * 255.0f, 255.0f), 0.0f);
         }
 }
+
+// Halton index of (pixel, sample) for the sampler tests (halton.cpp:46-66)
+extern "C" uint64_t oracle_halton_index(int32_t w, int32_t h, uint32_t px, uint32_t py, int32_t sample)
+{
+    orc::Halton hs;
+    hs.init(w, h, px, py, sample);
+    return hs.idx;
+}

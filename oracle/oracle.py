@@ -40,6 +40,8 @@ def lib():
         L.oracle_bsdf_eval.argtypes = [P(_abi.MaterialDesc), vp, vp, C.c_int32, vp, vp]
         L.oracle_kat_rays.argtypes = [C.c_int32, C.c_int32, vp, vp, vp, C.c_float, vp]
         L.oracle_kat_pixels.argtypes = [C.c_int32, C.c_int32, vp, vp, vp]
+        L.oracle_halton_index.restype = C.c_uint64
+        L.oracle_halton_index.argtypes = [C.c_int32, C.c_int32, C.c_uint32, C.c_uint32, C.c_int32]
         _lib = L
     return _lib
 
@@ -110,6 +112,10 @@ def halton(width, height, px, py, sample_index, dims):
     lib().oracle_halton(width, height, px.ctypes.data, py.ctypes.data, si.ctypes.data, len(px), dims,
                         out.ctypes.data)
     return out
+
+
+def halton_index(width, height, px, py, sample_index):
+    return lib().oracle_halton_index(width, height, px, py, sample_index)
 
 
 def detmath(fn, x, y=None):
