@@ -53,22 +53,23 @@ def read_traffic(path):
 
 
 def cpu_baseline(rays, world, budget_s, threads):
+    """lib/bvh traversal of the captured rays on `threads` host threads,
+    repeated over the sample until about budget_s seconds of work"""
     from oracle import oracle
     if not oracle.ref_available():
         return None, None
     ref = oracle.RefBvh(world)
-    t0 = time.perf_counter()
-    prim, t, u, v, steps, isects = ref.trace(rays[: min(len(rays), 1 << 16)], threads=threads)
-    dt = time.perf_counter() - t0
-    per_ray = dt / min(len(rays), 1 << 16)
-    n = int(min(len(rays), max(1 << 16, budget_s / max(per_ray, 1e-9))))
-    t0 = time.perf_counter()
-    prim, t, u, v, steps, isects = ref.trace(rays[:n], threads=threads)
-    dt = time.perf_counter() - t0
-    return dict(value=n / dt / 1e6, unit="Mrays/s", cores=threads, kind="reference",
-                sample=f"{n} extension rays of the first two bounces of this frame, traced by lib/bvh "
-                       f"(BinnedSAH16 + SingleRayTraverser, closest hit) on {threads} host threads "
-                       f"({dt:.1f} s)"), (steps, isects)
+    prim, t, u, v, steps, isects = ref.trace(rays, threads=threads)  # also the work statistics
+    traced, dt = 0, 0.0
+    while dt < budget_s:
+        t0 = time.perf_counter()
+        ref.trace(rays, threads=threads)
+        dt += time.perf_counter() - t0
+        traced += len(rays)
+    return dict(value=traced / dt / 1e6, unit="Mrays/s", cores=threads, kind="reference",
+                sample=f"{len(rays)} extension rays of the first two bounces of this frame, traced "
+                       f"{traced // len(rays)} times by lib/bvh (BinnedSAH16 + SingleRayTraverser, closest "
+                       f"hit) on {threads} host threads, {dt:.1f} s"), (steps, isects)
 
 
 def main():
@@ -126,17 +127,15 @@ def main():
     samples = totals["paths"] / secs
     trace_ms = totals["stage_ms"][1]
 
-    # rays of this frame for the work model and the CPU baseline
-    log("bench: capturing rays of the first two bounces")
-    pc = _abi.RenderParams.from_buffer_copy(p)
-    pc.time_stages = 0
-    rays = ctx.capture_rays(pc, 2, 1 << 22)
-    rng = np.random.default_rng(0)
-    rays = rays[rng.permutation(len(rays))]
-
     cpu = None
     bytes_per_ray = None
     if not args.no_cpu:
+        # rays of this frame for the work model and the CPU baseline
+        log("bench: capturing rays of the first two bounces")
+        pc = _abi.RenderParams.from_buffer_copy(p)
+        pc.time_stages = 0
+        rays = ctx.capture_rays(pc, 2, 1 << 22)
+        rays = rays[np.random.default_rng(0).permutation(len(rays))]
         # world-space triangles in flattened order, as the GPU traverses them
         world = flatten_instances(scene)
         threads = min(16, os.cpu_count() or 1)
@@ -148,13 +147,19 @@ def main():
 
     roofline = None
     if bytes_per_ray is not None and trace_ms > 0:
-        achieved = totals["ext"] * bytes_per_ray / (trace_ms * 1e-3) / 1e9
+        launches = max(1, int(totals["launches"][1]))
+        rays_per_launch = totals["ext"] / launches
+        launch_s = trace_ms * 1e-3 / launches
+        achieved = bytes_per_ray * rays_per_launch / launch_s / 1e9
         traffic_per_ray = read_traffic(os.path.join(ROOT, "profiles", "trace_extension_traffic.md"))
         roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                         frac=round(achieved / HBM_PEAK_GBS, 4),
-                        traffic=None if traffic_per_ray is None else round(traffic_per_ray, 1),
-                        kernel="k_trace_extension", algorithmic_bytes_per_ray=round(bytes_per_ray, 1),
-                        kernel_ms_per_launch=round(trace_ms / max(1, totals["launches"][1]), 4))
+                        traffic=None if traffic_per_ray is None else round(traffic_per_ray * rays_per_launch),
+                        kernel="k_trace_extension",
+                        algorithmic_bytes_per_ray=round(bytes_per_ray, 1),
+                        algorithmic_bytes_per_launch=round(bytes_per_ray * rays_per_launch),
+                        rays_per_launch=round(rays_per_launch), launch_ms=round(launch_s * 1e3, 4),
+                        traffic_bytes_per_ray=traffic_per_ray)
 
     out = {
         "metric": METRIC,
