@@ -82,6 +82,20 @@ class Context:
         self._check(self.lib.btrc_hip_device_name(self.h, buf, 128))
         return buf.value.decode()
 
+    def get_tuning(self):
+        t = _abi.Tuning()
+        self._check(self.lib.btrc_hip_get_tuning(self.h, C.byref(t)))
+        return dict(closest_postpone_leaves=t.closest_postpone_leaves, any_postpone_leaves=t.any_postpone_leaves,
+                    trace_waves_per_cu=t.trace_waves_per_cu)
+
+    def set_tuning(self, **kw):
+        cur = self.get_tuning()
+        cur.update(kw)
+        t = _abi.Tuning()
+        for k, v in cur.items():
+            setattr(t, k, int(v))
+        self._check(self.lib.btrc_hip_set_tuning(self.h, C.byref(t)))
+
     def upload_scene(self, scene):
         desc, holder = scene.to_desc()
         self._check(self.lib.btrc_hip_upload_scene(self.h, C.byref(desc)))

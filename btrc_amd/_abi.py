@@ -71,6 +71,11 @@ class RenderStats(C.Structure):
                 ("stage_ms", C.c_double * 4), ("stage_launches", C.c_uint64 * 4)]
 
 
+class Tuning(C.Structure):
+    _fields_ = [("closest_postpone_leaves", C.c_int32), ("any_postpone_leaves", C.c_int32),
+                ("trace_waves_per_cu", C.c_int32), ("reserved", C.c_int32 * 5)]
+
+
 class Ray(C.Structure):
     _fields_ = [("o", F3), ("tmin", C.c_float), ("d", F3), ("tmax", C.c_float)]
 
@@ -94,6 +99,7 @@ def hit_dtype():
 # every symbol include/btrc_hip.h declares, in declaration order
 EXPORTS = [
     "btrc_hip_create", "btrc_hip_destroy", "btrc_hip_last_error", "btrc_hip_device_name",
+    "btrc_hip_set_tuning", "btrc_hip_get_tuning",
     "btrc_hip_upload_scene", "btrc_hip_set_camera", "btrc_hip_scene_info",
     "btrc_hip_render", "btrc_hip_film_readback", "btrc_hip_request_stop",
     "btrc_hip_trace_closest", "btrc_hip_trace_any",
@@ -111,6 +117,8 @@ def declare(lib):
         "btrc_hip_destroy": (None, [ctx]),
         "btrc_hip_last_error": (C.c_char_p, [ctx]),
         "btrc_hip_device_name": (C.c_int, [ctx, C.c_char_p, C.c_size_t]),
+        "btrc_hip_set_tuning": (C.c_int, [ctx, P(Tuning)]),
+        "btrc_hip_get_tuning": (C.c_int, [ctx, P(Tuning)]),
         "btrc_hip_upload_scene": (C.c_int, [ctx, P(SceneDesc)]),
         "btrc_hip_set_camera": (C.c_int, [ctx, P(CameraDesc)]),
         "btrc_hip_scene_info": (C.c_int, [ctx, P(C.c_int64), P(C.c_int32), P(C.c_int32), P(C.c_int64)]),

@@ -325,7 +325,8 @@ Bvh4 build_bvh4(const float *vertices, uint32_t triangle_count, const BvhBuildOp
                 n.count[j] = 0;
             }
         }
-        const int pushed_here = it.pushed + std::max(0, inner - 1);
+        // a visit pushes every hit child but the nearest, leaves included
+        const int pushed_here = it.pushed + std::max(0, (int)set.size() - 1);
         max_stack = std::max(max_stack, pushed_here + 1);
         max_depth = std::max(max_depth, it.level);
         for(int j = (int)set.size() - 1; j >= 0; --j)

@@ -206,12 +206,23 @@ struct btrc_ctx
     int32_t film_w = 0, film_h = 0, film_spp = 0;
     bool film_has64 = false;
 
+    // traversal: per-lane stack overflow and the batch fetch counter
+    DevBuf<uint32_t> ovf_code;
+    DevBuf<float> ovf_t;
+    int32_t trace_grid = 0;
+    DevBuf<uint32_t> fetch;
+    btrc_tuning tuning{ 1, 0, TRACE_WAVES_PER_CU, { 0, 0, 0, 0, 0 } };
+
     // resident ray batch
     DevBuf<float4> batch_rays;
     DevBuf<uint32_t> batch_out;
     int64_t batch_n = 0;
 
     TraceScene trace_scene() const { return TraceScene{ nodes.ptr, tris.ptr }; }
+    TraceOverflow trace_overflow() const
+    {
+        return TraceOverflow{ ovf_code.ptr, ovf_t.ptr, trace_grid * TRACE_BLOCK };
+    }
     SceneView scene_view() const
     {
         SceneView v;
@@ -280,6 +291,16 @@ int guarded(btrc_ctx *ctx, F &&f)
         ctx->last_error = e.what();
         return BTRC_ERR_INVALID;
     }
+}
+
+// persistent traversal grid and the per-lane stack overflow it needs
+void size_traversal(btrc_ctx *ctx)
+{
+    ctx->trace_grid = ctx->cu_count * ctx->tuning.trace_waves_per_cu;
+    const size_t deep = ctx->stack_size > SHORT_STACK ? (size_t)(ctx->stack_size - SHORT_STACK) : 0;
+    ctx->ovf_code.alloc(std::max<size_t>(1, deep * ctx->trace_grid * TRACE_BLOCK));
+    ctx->ovf_t.alloc(std::max<size_t>(1, deep * ctx->trace_grid * TRACE_BLOCK));
+    ctx->fetch.alloc(1);
 }
 
 void upload_scene(btrc_ctx *ctx, const btrc_scene_desc *sd)
@@ -480,7 +501,11 @@ void upload_scene(btrc_ctx *ctx, const btrc_scene_desc *sd)
         throw InvalidArg("unknown environment light type");
 
     const uint32_t ntri = prim_base;
-    const Bvh4 bvh = build_bvh4(world.data(), ntri, BvhBuildOptions{});
+    if(ntri > MAX_LEAF_FIRST)
+        throw InvalidArg("too many triangles for the traversal's leaf encoding");
+    BvhBuildOptions opt;
+    opt.max_leaf_size = 4; // leaf codes carry counts 1..4
+    const Bvh4 bvh = build_bvh4(world.data(), ntri, opt);
     std::vector<float4> tri_data(3 * (size_t)ntri);
     for(uint32_t k = 0; k < ntri; ++k)
     {
@@ -515,8 +540,7 @@ void upload_scene(btrc_ctx *ctx, const btrc_scene_desc *sd)
     ctx->node_count = (int64_t)bvh.nodes.size();
     ctx->bvh_depth = bvh.depth;
     ctx->stack_size = bvh.stack_bound + 1;
-    if(ctx->stack_size * 128 * 4 > 64 * 1024)
-        throw InvalidArg("BVH too deep for the LDS traversal stack (bound " + std::to_string(bvh.stack_bound) + ")");
+    size_traversal(ctx);
     ctx->has_scene = true;
 }
 
@@ -670,7 +694,7 @@ void render(btrc_ctx *ctx, const btrc_render_params *p, btrc_render_stats *stats
     ShadowSOA sh{ ctx->sh_o.ptr, ctx->sh_d.ptr, ctx->sh_beta.ptr };
 
     const int grid256 = ctx->persistent_grid(256, (int64_t)P);
-    const int grid128 = ctx->persistent_grid(128, (int64_t)P);
+    const TraceOverflow ovf = ctx->trace_overflow();
 
     StageTimer timer;
     timer.enabled = p->time_stages != 0;
@@ -689,7 +713,7 @@ void render(btrc_ctx *ctx, const btrc_render_params *p, btrc_render_stats *stats
         PathSOA next = ctx->pool[cur ^ 1].view();
         launch_plan(s, ctx->counters.ptr, p->state_count);
         timer.run(STAGE_GENERATE, [&] { launch_generate(s, grid256, rc, ctx->counters.ptr, path); });
-        timer.run(STAGE_TRACE, [&] { launch_trace_extension(s, grid128, ctx->stack_size, ts, ctx->counters.ptr, path, hit); });
+        timer.run(STAGE_TRACE, [&] { launch_trace_extension(s, ctx->trace_grid, ctx->tuning.closest_postpone_leaves != 0, ts, ctx->counters.ptr, path, hit, ovf); });
         if(capture && (int64_t)iter < capture_bounces && captured < capture_max)
         {
             BTRC_HIP_CHECK(hipMemcpyAsync(ctx->host_counters, ctx->counters.ptr, sizeof(WaveCounters), hipMemcpyDeviceToHost, s));
@@ -710,7 +734,7 @@ void render(btrc_ctx *ctx, const btrc_render_params *p, btrc_render_stats *stats
             captured += n;
         }
         timer.run(STAGE_SHADE, [&] { launch_shade(s, grid256, rc, sv, ctx->counters.ptr, path, hit, next, sh, fv); });
-        timer.run(STAGE_SHADOW, [&] { launch_shadow(s, grid128, ctx->stack_size, ts, ctx->counters.ptr, sh, fv, p->width); });
+        timer.run(STAGE_SHADOW, [&] { launch_shadow(s, ctx->trace_grid, ctx->tuning.any_postpone_leaves != 0, ts, ctx->counters.ptr, sh, fv, ovf); });
         launch_end_iteration(s, ctx->counters.ptr);
         BTRC_HIP_CHECK(hipGetLastError());
         cur ^= 1;
@@ -772,10 +796,19 @@ void check_batch_args(btrc_ctx *ctx, const void *rays, int64_t n, int32_t tri_te
 {
     if(!ctx->has_scene)
         throw StateError("trace before btrc_hip_upload_scene");
-    if(n < 0 || (n > 0 && !rays))
+    if(n < 0 || (n > 0 && !rays) || n > 0x7fffffff)
         throw InvalidArg("bad ray batch");
     if(tri_test != BTRC_TRI_WATERTIGHT && tri_test != BTRC_TRI_MOLLER_TRUMBORE)
         throw InvalidArg("unknown triangle test");
+}
+
+void trace_batch(btrc_ctx *ctx, int32_t tri_test, bool any_hit, const float4 *rays, int64_t n, uint32_t *out)
+{
+    BTRC_HIP_CHECK(hipMemsetAsync(ctx->fetch.ptr, 0, sizeof(uint32_t), ctx->stream));
+    const bool postpone = (any_hit ? ctx->tuning.any_postpone_leaves : ctx->tuning.closest_postpone_leaves) != 0;
+    launch_trace_batch(ctx->stream, ctx->trace_grid, postpone, ctx->trace_scene(), tri_test, any_hit, rays,
+                       (int32_t)n, out, ctx->fetch.ptr, ctx->trace_overflow());
+    BTRC_HIP_CHECK(hipGetLastError());
 }
 
 } // namespace
@@ -835,6 +868,27 @@ int btrc_hip_device_name(btrc_ctx *ctx, char *buf, size_t len)
             throw InvalidArg("bad buffer");
         std::strncpy(buf, ctx->device_name.c_str(), len - 1);
         buf[len - 1] = 0;
+    });
+}
+
+int btrc_hip_set_tuning(btrc_ctx *ctx, const btrc_tuning *t)
+{
+    return guarded(ctx, [&] {
+        if(!t || t->trace_waves_per_cu < 1 || t->trace_waves_per_cu > TRACE_WAVES_PER_CU)
+            throw InvalidArg("bad tuning");
+        BTRC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        ctx->tuning = *t;
+        if(ctx->has_scene)
+            size_traversal(ctx);
+    });
+}
+
+int btrc_hip_get_tuning(btrc_ctx *ctx, btrc_tuning *t)
+{
+    return guarded(ctx, [&] {
+        if(!t)
+            throw InvalidArg("tuning is null");
+        *t = ctx->tuning;
     });
 }
 
@@ -900,8 +954,7 @@ int btrc_hip_trace_closest(btrc_ctx *ctx, const btrc_ray *rays, int64_t n, int32
         DevBuf<uint32_t> o;
         r.upload(reinterpret_cast<const float4 *>(rays), 2 * (size_t)n, ctx->stream);
         o.alloc(5 * (size_t)n);
-        launch_trace_batch(ctx->stream, ctx->persistent_grid(128, n), ctx->stack_size, ctx->trace_scene(),
-                           tri_test, false, r.ptr, n, o.ptr);
+        trace_batch(ctx, tri_test, false, r.ptr, n, o.ptr);
         BTRC_HIP_CHECK(hipGetLastError());
         BTRC_HIP_CHECK(hipMemcpyAsync(hits, o.ptr, 5 * (size_t)n * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
         BTRC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
@@ -920,8 +973,7 @@ int btrc_hip_trace_any(btrc_ctx *ctx, const btrc_ray *rays, int64_t n, int32_t t
         DevBuf<uint32_t> o;
         r.upload(reinterpret_cast<const float4 *>(rays), 2 * (size_t)n, ctx->stream);
         o.alloc((size_t)n);
-        launch_trace_batch(ctx->stream, ctx->persistent_grid(128, n), ctx->stack_size, ctx->trace_scene(),
-                           tri_test, true, r.ptr, n, o.ptr);
+        trace_batch(ctx, tri_test, true, r.ptr, n, o.ptr);
         BTRC_HIP_CHECK(hipGetLastError());
         std::vector<uint32_t> tmp(n);
         BTRC_HIP_CHECK(hipMemcpyAsync(tmp.data(), o.ptr, (size_t)n * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
@@ -956,9 +1008,7 @@ int btrc_hip_batch_trace(btrc_ctx *ctx, int32_t tri_test, int32_t any_hit, int32
         for(int32_t k = 0; k < repeats; ++k)
         {
             BTRC_HIP_CHECK(hipEventRecord(a, ctx->stream));
-            launch_trace_batch(ctx->stream, ctx->persistent_grid(128, ctx->batch_n), ctx->stack_size,
-                               ctx->trace_scene(), tri_test, any_hit != 0, ctx->batch_rays.ptr, ctx->batch_n,
-                               ctx->batch_out.ptr);
+            trace_batch(ctx, tri_test, any_hit != 0, ctx->batch_rays.ptr, ctx->batch_n, ctx->batch_out.ptr);
             BTRC_HIP_CHECK(hipEventRecord(b, ctx->stream));
             BTRC_HIP_CHECK(hipEventSynchronize(b));
             float ms = 0;

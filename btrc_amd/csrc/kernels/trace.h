@@ -1,13 +1,23 @@
 // Software BVH traversal for gfx950 (CDNA4 has no ray-tracing instructions).
 //
-// One lane owns one ray. The traversal stack lives in LDS laid out
-// [entry][lane], so pushes and pops of a wavefront touch 64 consecutive
-// dwords and never conflict on banks. Leaves met in a node are intersected
-// at once (eager leaves, as lib/bvh's SingleRayTraverser does); inner
-// children that the ray enters are visited nearest first. Boxes are tested
-// with Ize's conservative slab test so no triangle is culled by rounding,
-// which makes the closest hit the BVH-independent minimum of
-// (t, flattened triangle index).
+// Persistent "while-while" traversal (Aila and Laine, "Understanding the
+// Efficiency of Ray Traversal on GPUs", HPG 2009) laid out for 64-wide
+// wavefronts:
+//  * one wavefront per workgroup, one lane per ray; a lane whose ray has
+//    ended takes a new one from a global counter (one atomic per wavefront
+//    for all idle lanes), so no lane waits for the slowest ray of its batch;
+//  * inner-node steps and leaf (triangle) steps run in separate loops, and
+//    a lane that has found a leaf keeps stepping through inner nodes until
+//    every lane of the wavefront holds a leaf, so leaf tests run together;
+//  * the traversal stack keeps its top SHORT_STACK entries in LDS, laid out
+//    [entry][lane] (a wavefront's pushes hit 64 consecutive dwords, no bank
+//    conflicts), and deeper entries in a per-lane global overflow area sized
+//    from the tree's exact worst case;
+//  * children are visited nearest first; each stack entry keeps its entry
+//    distance so popped subtrees beyond the current hit are skipped.
+// Boxes use Ize's conservative slab test and a 1 + 2^-20 culling margin, so
+// rounding never culls a triangle: the closest hit is the BVH-independent
+// minimum of (t, flattened triangle index).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -16,12 +26,30 @@
 
 namespace btrc {
 
+constexpr int TRACE_BLOCK = 64;   // one wavefront per workgroup
+constexpr int SHORT_STACK = 16;   // LDS stack entries per lane
+constexpr int REFILL_MIN = 16;    // idle lanes that trigger a refill
+constexpr int TRACE_WAVES_PER_CU = 20; // persistent grid: wavefronts per compute unit
+
+// stack entry / node codes
+constexpr uint32_t CODE_LEAF = 0x80000000u;   // | (count - 1) << 28 | first triangle
+constexpr uint32_t CODE_DONE = 0xffffffffu;
+constexpr uint32_t MAX_LEAF_FIRST = 0x0ffffffeu;
+
 struct TraceScene
 {
     const BvhNode4 *nodes;
     // 3 float4 per triangle in leaf order:
     //   (v0.xyz, flattened index), (v1.xyz, instance), (v2.xyz, prim in geometry)
     const float4 *tris;
+};
+
+// per-lane global stack entries below the LDS part: [entry - SHORT_STACK][lane]
+struct TraceOverflow
+{
+    uint32_t *code;
+    float *t;
+    int32_t lanes;   // stride between entries = lanes of the launch grid
 };
 
 struct HitRecord
@@ -32,122 +60,244 @@ struct HitRecord
     uint32_t prim;
 };
 
-// Walk the BVH for one ray. ANY stops at the first hit in [tmin, tmax].
-template<int TRI, bool ANY>
-__device__ __forceinline__ HitRecord trace_ray(
-    const TraceScene &sc, int32_t *stack, int32_t stride, V3 o, V3 d, float tmin, float tmax)
+__device__ __forceinline__ uint32_t leaf_code(int32_t first, int32_t count)
 {
-    HitRecord h;
-    h.t = tmax;
-    h.b1 = h.b2 = 0.0f;
-    h.gid = 0xffffffffu;
-    h.inst = h.prim = 0xffffffffu;
+    return CODE_LEAF | ((uint32_t)(count - 1) << 28) | (uint32_t)first;
+}
 
-    const RayBoxPre rb = raybox_pre(o, d);
+// Traces rays [0, n) handed out through *fetch. Sink provides
+//   void load(uint32_t i, V3 &o, V3 &d, float &tmin, float &tmax)
+//   void store(uint32_t i, const HitRecord &h)
+// ANY ends a ray at its first hit in [tmin, tmax]. POSTPONE selects the
+// while-while schedule (leaves wait until every lane holds one); without it
+// a lane tests a leaf as soon as it meets it, which ends occluded shadow
+// rays earlier.
+template<int TRI, bool ANY, bool POSTPONE, class Sink>
+__device__ __forceinline__ void trace_persistent(
+    const TraceScene &sc, Sink &sink, int32_t n, uint32_t *fetch, const TraceOverflow &ovf)
+{
+    __shared__ uint32_t s_code[SHORT_STACK * TRACE_BLOCK];
+    __shared__ float s_t[SHORT_STACK * TRACE_BLOCK];
+    const int lane = (int)threadIdx.x;
+    const int gl = (int)(blockIdx.x * TRACE_BLOCK + threadIdx.x);
+    const uint64_t lanes_below = (1ull << lane) - 1ull;
+
+    bool active = false;
+    uint32_t ray = 0;
+    V3 o = v3s(0.0f), d = v3s(0.0f);
+    float tmin = 0.0f;
+    RayBoxPre rb;
     WatertightPre wt;
     MTPre mt;
-    if(TRI == TRI_WATERTIGHT)
-        wt = watertight_pre(o, d);
-    else
-        mt = mt_pre(o, d);
-
+    HitRecord h;
+    uint32_t node = CODE_DONE;
     int32_t sp = 0;
-    int32_t node = 0;
+    bool exhausted = false; // wavefront-uniform: the ray counter has passed n
+
+    auto push = [&](uint32_t code, float t) {
+        if(sp < SHORT_STACK)
+        {
+            s_code[sp * TRACE_BLOCK + lane] = code;
+            s_t[sp * TRACE_BLOCK + lane] = t;
+        }
+        else
+        {
+            const size_t k = (size_t)(sp - SHORT_STACK) * ovf.lanes + gl;
+            ovf.code[k] = code;
+            ovf.t[k] = t;
+        }
+        ++sp;
+    };
+    // next entry whose subtree may still hold a closer hit, or CODE_DONE
+    auto pop = [&]() -> uint32_t {
+        const float cull = cull_distance(h.t);
+        while(sp > 0)
+        {
+            --sp;
+            uint32_t code;
+            float t;
+            if(sp < SHORT_STACK)
+            {
+                code = s_code[sp * TRACE_BLOCK + lane];
+                t = s_t[sp * TRACE_BLOCK + lane];
+            }
+            else
+            {
+                const size_t k = (size_t)(sp - SHORT_STACK) * ovf.lanes + gl;
+                code = ovf.code[k];
+                t = ovf.t[k];
+            }
+            if(t <= cull)
+                return code;
+        }
+        return CODE_DONE;
+    };
+
+    // tests the triangles of a leaf code; true on a hit (ANY stops at the first)
+    auto test_leaf = [&](uint32_t code) -> bool {
+        const int32_t first = (int32_t)(code & 0x0fffffffu);
+        const int32_t cnt = (int32_t)((code >> 28) & 3u) + 1;
+        bool any_hit = false;
+        for(int32_t q = 0; q < cnt; ++q)
+        {
+            const float4 a = sc.tris[3 * (first + q) + 0];
+            const float4 b = sc.tris[3 * (first + q) + 1];
+            const float4 cc = sc.tris[3 * (first + q) + 2];
+            const uint32_t gid = __float_as_uint(a.w);
+            float t, u, v;
+            bool ok;
+            if(TRI == TRI_WATERTIGHT)
+                ok = watertight_hit(wt, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), v3(cc.x, cc.y, cc.z), tmin, h.t, &t, &u, &v);
+            else
+                ok = mt_hit(mt, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), v3(cc.x, cc.y, cc.z), tmin, h.t, &t, &u, &v);
+            if(ok && (t < h.t || gid < h.gid))
+            {
+                h.t = t;
+                h.b1 = u;
+                h.b2 = v;
+                h.gid = gid;
+                h.inst = __float_as_uint(b.w);
+                h.prim = __float_as_uint(cc.w);
+                any_hit = true;
+                if(ANY)
+                    break;
+            }
+        }
+        return any_hit;
+    };
+
     for(;;)
     {
-        const float4 *np = reinterpret_cast<const float4 *>(sc.nodes + node);
-        const float4 lox = np[0], hix = np[1], loy = np[2], hiy = np[3], loz = np[4], hiz = np[5];
-        const int4 child = reinterpret_cast<const int4 *>(np)[6];
-        const int4 count = reinterpret_cast<const int4 *>(np)[7];
-        const float blox[4] = { lox.x, lox.y, lox.z, lox.w };
-        const float bhix[4] = { hix.x, hix.y, hix.z, hix.w };
-        const float bloy[4] = { loy.x, loy.y, loy.z, loy.w };
-        const float bhiy[4] = { hiy.x, hiy.y, hiy.z, hiy.w };
-        const float bloz[4] = { loz.x, loz.y, loz.z, loz.w };
-        const float bhiz[4] = { hiz.x, hiz.y, hiz.z, hiz.w };
-        const int32_t ch[4] = { child.x, child.y, child.z, child.w };
-        const int32_t cn[4] = { count.x, count.y, count.z, count.w };
-
-        float te[4];
-        bool hit[4];
+        // ---- hand new rays to idle lanes
+        const uint64_t idle = __ballot(!active);
+        const int nidle = (int)__popcll(idle);
+        if(nidle >= REFILL_MIN && !exhausted)
         {
-            const float cull = cull_distance(h.t);
-#pragma unroll
-            for(int j = 0; j < 4; ++j)
-                hit[j] = raybox(rb, blox[j], bhix[j], bloy[j], bhiy[j], bloz[j], bhiz[j], tmin, cull, &te[j]);
-        }
-
-        // leaves of this node, at once
-#pragma unroll
-        for(int j = 0; j < 4; ++j)
-        {
-            if(hit[j] && ch[j] < 0)
+            const int leader = __ffsll((unsigned long long)idle) - 1;
+            uint32_t base = 0;
+            if(lane == leader)
+                base = atomicAdd(fetch, (uint32_t)nidle);
+            base = __shfl(base, leader);
+            exhausted = base + (uint32_t)nidle >= (uint32_t)n;
+            if(!active)
             {
-                const int32_t first = ~ch[j];
-                const int32_t n = cn[j];
-                for(int32_t k = 0; k < n; ++k)
+                const uint32_t idx = base + (uint32_t)__popcll(idle & lanes_below);
+                if(idx < (uint32_t)n)
                 {
-                    const float4 a = sc.tris[3 * (first + k) + 0];
-                    const float4 b = sc.tris[3 * (first + k) + 1];
-                    const float4 c = sc.tris[3 * (first + k) + 2];
-                    const uint32_t gid = __float_as_uint(a.w);
-                    float t, u, v;
-                    bool ok;
+                    ray = idx;
+                    float tmax;
+                    sink.load(idx, o, d, tmin, tmax);
+                    rb = raybox_pre(o, d);
                     if(TRI == TRI_WATERTIGHT)
-                        ok = watertight_hit(wt, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), v3(c.x, c.y, c.z), tmin, h.t, &t, &u, &v);
+                        wt = watertight_pre(o, d);
                     else
-                        ok = mt_hit(mt, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), v3(c.x, c.y, c.z), tmin, h.t, &t, &u, &v);
-                    if(ok && (t < h.t || gid < h.gid))
-                    {
-                        h.t = t;
-                        h.b1 = u;
-                        h.b2 = v;
-                        h.gid = gid;
-                        h.inst = __float_as_uint(b.w);
-                        h.prim = __float_as_uint(c.w);
-                        if(ANY)
-                            return h;
-                    }
+                        mt = mt_pre(o, d);
+                    h.t = tmax;
+                    h.b1 = h.b2 = 0.0f;
+                    h.gid = h.inst = h.prim = 0xffffffffu;
+                    node = 0;
+                    sp = 0;
+                    active = true;
                 }
             }
         }
-
-        // inner children, nearest first
-        const float cull = cull_distance(h.t);
-        float k[4];
-        int32_t c[4];
-#pragma unroll
-        for(int j = 0; j < 4; ++j)
-        {
-            const bool in = hit[j] && ch[j] >= 0 && te[j] <= cull;
-            k[j] = in ? te[j] : INFINITY;
-            c[j] = in ? ch[j] : -1;
-        }
-#define BTRC_CSWAP(a, b)                                  \
-        if(k[b] < k[a])                                   \
-        {                                                 \
-            const float tk = k[a]; k[a] = k[b]; k[b] = tk; \
-            const int32_t tc = c[a]; c[a] = c[b]; c[b] = tc; \
-        }
-        BTRC_CSWAP(0, 1)
-        BTRC_CSWAP(2, 3)
-        BTRC_CSWAP(0, 2)
-        BTRC_CSWAP(1, 3)
-        BTRC_CSWAP(1, 2)
-#undef BTRC_CSWAP
-        if(c[0] >= 0)
-        {
-            if(c[3] >= 0) stack[(sp++) * stride] = c[3];
-            if(c[2] >= 0) stack[(sp++) * stride] = c[2];
-            if(c[1] >= 0) stack[(sp++) * stride] = c[1];
-            node = c[0];
-            continue;
-        }
-        if(sp == 0)
+        if(!__any(active))
             break;
-        node = stack[(--sp) * stride];
+
+        // ---- inner nodes: step until every lane holds a leaf or has finished
+        uint32_t leaf = 0;
+        for(;;)
+        {
+            const bool step = active && node != CODE_DONE && (leaf == 0 || !(node & CODE_LEAF));
+            if(!__any(step))
+                break;
+            if(step)
+            {
+                if(node & CODE_LEAF)
+                {
+                    if(POSTPONE)
+                    {
+                        leaf = node;
+                        node = pop();
+                    }
+                    else
+                        node = (test_leaf(node) && ANY) ? CODE_DONE : pop();
+                }
+                else
+                {
+                    const float4 *np = reinterpret_cast<const float4 *>(sc.nodes + node);
+                    const float4 lox = np[0], hix = np[1], loy = np[2], hiy = np[3], loz = np[4], hiz = np[5];
+                    const int4 child = reinterpret_cast<const int4 *>(np)[6];
+                    const int4 count = reinterpret_cast<const int4 *>(np)[7];
+                    const float blox[4] = { lox.x, lox.y, lox.z, lox.w };
+                    const float bhix[4] = { hix.x, hix.y, hix.z, hix.w };
+                    const float bloy[4] = { loy.x, loy.y, loy.z, loy.w };
+                    const float bhiy[4] = { hiy.x, hiy.y, hiy.z, hiy.w };
+                    const float bloz[4] = { loz.x, loz.y, loz.z, loz.w };
+                    const float bhiz[4] = { hiz.x, hiz.y, hiz.z, hiz.w };
+                    const int32_t ch[4] = { child.x, child.y, child.z, child.w };
+                    const int32_t cn[4] = { count.x, count.y, count.z, count.w };
+                    const float cull = cull_distance(h.t);
+                    float k[4];
+                    uint32_t c[4];
+#pragma unroll
+                    for(int j = 0; j < 4; ++j)
+                    {
+                        float te;
+                        const bool hit = raybox(rb, blox[j], bhix[j], bloy[j], bhiy[j], bloz[j], bhiz[j],
+                                                tmin, cull, &te);
+                        const bool leafj = ch[j] < 0;
+                        const bool use = hit && (!leafj || cn[j] > 0);
+                        k[j] = use ? te : INFINITY;
+                        c[j] = !use ? CODE_DONE : (leafj ? leaf_code(~ch[j], cn[j]) : (uint32_t)ch[j]);
+                    }
+#define BTRC_CSWAP(a, b)                                                  \
+                    if(k[b] < k[a])                                       \
+                    {                                                     \
+                        const float tk = k[a]; k[a] = k[b]; k[b] = tk;    \
+                        const uint32_t tc = c[a]; c[a] = c[b]; c[b] = tc; \
+                    }
+                    BTRC_CSWAP(0, 1)
+                    BTRC_CSWAP(2, 3)
+                    BTRC_CSWAP(0, 2)
+                    BTRC_CSWAP(1, 3)
+                    BTRC_CSWAP(1, 2)
+#undef BTRC_CSWAP
+                    if(c[0] != CODE_DONE)
+                    {
+                        if(c[3] != CODE_DONE) push(c[3], k[3]);
+                        if(c[2] != CODE_DONE) push(c[2], k[2]);
+                        if(c[1] != CODE_DONE) push(c[1], k[1]);
+                        node = c[0];
+                    }
+                    else
+                        node = pop();
+                }
+            }
+            if(POSTPONE)
+            {
+                if(__all(!active || node == CODE_DONE || leaf != 0))
+                    break;
+            }
+            else if(!exhausted && (int)__popcll(__ballot(!active || node == CODE_DONE)) >= REFILL_MIN)
+                break; // enough lanes to refill
+
+        }
+
+        // ---- leaf: test its triangles
+        if(POSTPONE && active && leaf != 0)
+        {
+            if(test_leaf(leaf) && ANY)
+                node = CODE_DONE;
+        }
+
+        // ---- finished rays leave their result
+        if(active && node == CODE_DONE)
+        {
+            sink.store(ray, h);
+            active = false;
+        }
     }
-    return h;
 }
 
 } // namespace btrc

@@ -48,7 +48,9 @@ struct WaveCounters
     int32_t shadow;       // shadow rays pushed this bounce
     int32_t gen_count;    // states generated this iteration
     int32_t gen_offset;   // first slot of the generated states
-    int32_t pad0[3];
+    uint32_t trace_fetch; // next extension ray handed to a traversal lane
+    uint32_t shadow_fetch;// next shadow ray handed to a traversal lane
+    int32_t pad0;
     uint64_t generated;   // paths generated so far
     uint64_t total;       // spp * width * height
     uint64_t gen_base;    // path index of the first generated state
@@ -95,18 +97,18 @@ enum : int
 
 void launch_plan(hipStream_t s, WaveCounters *c, int32_t state_count);
 void launch_generate(hipStream_t s, int grid, const RenderConsts &rc, const WaveCounters *c, PathSOA path);
-void launch_trace_extension(hipStream_t s, int grid, int stack, const TraceScene &ts, const WaveCounters *c,
-                            PathSOA path, HitSOA hit);
+void launch_trace_extension(hipStream_t s, int grid, bool postpone, const TraceScene &ts, WaveCounters *c,
+                            PathSOA path, HitSOA hit, const TraceOverflow &ovf);
 void launch_shade(hipStream_t s, int grid, const RenderConsts &rc, const SceneView &sv, WaveCounters *c,
                   PathSOA path, HitSOA hit, PathSOA next, ShadowSOA shadow, FilmView film);
-void launch_shadow(hipStream_t s, int grid, int stack, const TraceScene &ts, const WaveCounters *c,
-                   ShadowSOA shadow, FilmView film, int32_t width);
+void launch_shadow(hipStream_t s, int grid, bool postpone, const TraceScene &ts, WaveCounters *c,
+                   ShadowSOA shadow, FilmView film, const TraceOverflow &ovf);
 void launch_end_iteration(hipStream_t s, WaveCounters *c);
 void launch_resolve(hipStream_t s, const FilmView &film, int32_t pixels, float inv_weight, float4 *out);
 
 // ray batches of the traversal entry points: o.xyz/tmin and d.xyz/tmax
-void launch_trace_batch(hipStream_t s, int grid, int stack, const TraceScene &ts, int tri_test, bool any_hit,
-                        const float4 *rays, int64_t n, uint32_t *out);
+void launch_trace_batch(hipStream_t s, int grid, bool postpone, const TraceScene &ts, int tri_test, bool any_hit,
+                        const float4 *rays, int32_t n, uint32_t *out, uint32_t *fetch, const TraceOverflow &ovf);
 
 // test hooks
 void launch_halton_eval(hipStream_t s, const HaltonConsts &hc, const uint32_t *px, const uint32_t *py,

@@ -83,6 +83,8 @@ __global__ void k_plan(WaveCounters *c, int32_t state_count)
         if(!(unfinished > rest && rest + rest < state_count))
             fresh = unfinished < rest ? unfinished : rest;
     }
+    c->trace_fetch = 0;
+    c->shadow_fetch = 0;
     c->gen_offset = active;
     c->gen_count = (int32_t)fresh;
     c->gen_base = c->generated;
@@ -135,16 +137,23 @@ __global__ void __launch_bounds__(256) k_generate(RenderConsts rc, const WaveCou
 
 // ---------------------------------------------------------------- trace
 
-__global__ void __launch_bounds__(128) k_trace_extension(TraceScene ts, const WaveCounters *c, PathSOA path, HitSOA hit)
+// rays of the path pool in, IntersectionSOA-style records out (trace.cpp:38-78)
+struct ExtensionSink
 {
-    extern __shared__ int32_t s_stack[];
-    int32_t *stack = s_stack + threadIdx.x;
-    const int32_t n = c->active;
-    for(int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    PathSOA path;
+    HitSOA hit;
+
+    __device__ __forceinline__ void load(uint32_t i, V3 &o, V3 &d, float &tmin, float &tmax) const
     {
-        const float4 o = path.ray_o[i];
-        const float4 d = path.ray_d[i];
-        const HitRecord h = trace_ray<TRI_WATERTIGHT, false>(ts, stack, blockDim.x, xyz(o), xyz(d), 0.0f, d.w);
+        const float4 a = path.ray_o[i];
+        const float4 b = path.ray_d[i];
+        o = xyz(a);
+        d = xyz(b);
+        tmin = 0.0f;
+        tmax = b.w;
+    }
+    __device__ __forceinline__ void store(uint32_t i, const HitRecord &h) const
+    {
         if(h.gid != 0xffffffffu)
         {
             hit.flag[i] = PATH_FLAG_HAS_INTERSECTION | h.inst;
@@ -153,6 +162,14 @@ __global__ void __launch_bounds__(128) k_trace_extension(TraceScene ts, const Wa
         else
             hit.flag[i] = 0;
     }
+};
+
+template<bool POSTPONE>
+__global__ void __launch_bounds__(TRACE_BLOCK) k_trace_extension(
+    TraceScene ts, WaveCounters *c, PathSOA path, HitSOA hit, TraceOverflow ovf)
+{
+    ExtensionSink sink{ path, hit };
+    trace_persistent<TRI_WATERTIGHT, false, POSTPONE>(ts, sink, c->active, &c->trace_fetch, ovf);
 }
 
 // ---------------------------------------------------------------- shade
@@ -321,23 +338,38 @@ __global__ void __launch_bounds__(256) k_shade(
 
 // ---------------------------------------------------------------- shadow
 
-__global__ void __launch_bounds__(128) k_shadow(TraceScene ts, const WaveCounters *c, ShadowSOA shadow, FilmView film)
+// shadow rays in; on a miss the light sample reaches the film
+// (__miss__shadow, shadow.cpp:43-71; transmittance is 1 in vacuum)
+struct ShadowSink
 {
-    extern __shared__ int32_t s_stack[];
-    int32_t *stack = s_stack + threadIdx.x;
-    const int32_t n = c->shadow;
-    for(int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    ShadowSOA shadow;
+    FilmView film;
+
+    __device__ __forceinline__ void load(uint32_t i, V3 &o, V3 &d, float &tmin, float &tmax) const
     {
-        const float4 o = shadow.o[i];
-        const float4 d = shadow.d[i];
-        const HitRecord h = trace_ray<TRI_WATERTIGHT, true>(ts, stack, blockDim.x, xyz(o), xyz(d), 0.0f, d.w);
+        const float4 a = shadow.o[i];
+        const float4 b = shadow.d[i];
+        o = xyz(a);
+        d = xyz(b);
+        tmin = 0.0f;
+        tmax = b.w;
+    }
+    __device__ __forceinline__ void store(uint32_t i, const HitRecord &h) const
+    {
         if(h.gid == 0xffffffffu)
         {
-            // __miss__shadow, shadow.cpp:43-71; transmittance is 1 in vacuum
             const float4 b = shadow.beta[i];
-            film_splat(film, __float_as_uint(o.w), xyz(b));
+            film_splat(film, __float_as_uint(shadow.o[i].w), xyz(b));
         }
     }
+};
+
+template<bool POSTPONE>
+__global__ void __launch_bounds__(TRACE_BLOCK) k_shadow(
+    TraceScene ts, WaveCounters *c, ShadowSOA shadow, FilmView film, TraceOverflow ovf)
+{
+    ShadowSink sink{ shadow, film };
+    trace_persistent<TRI_WATERTIGHT, true, POSTPONE>(ts, sink, c->shadow, &c->shadow_fetch, ovf);
 }
 
 // ---------------------------------------------------------------- bookkeeping
@@ -363,29 +395,44 @@ __global__ void k_resolve(FilmView film, int32_t pixels, float inv_weight, float
 
 // ---------------------------------------------------------------- ray batches
 
-template<int TRI, bool ANY>
-__global__ void __launch_bounds__(128) k_trace_batch(TraceScene ts, const float4 *rays, int64_t n, uint32_t *out)
+template<bool ANY>
+struct BatchSink
 {
-    extern __shared__ int32_t s_stack[];
-    int32_t *stack = s_stack + threadIdx.x;
-    for(int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    const float4 *rays;
+    uint32_t *out;
+
+    __device__ __forceinline__ void load(uint32_t i, V3 &o, V3 &d, float &tmin, float &tmax) const
     {
-        const float4 a = rays[2 * i + 0];
-        const float4 b = rays[2 * i + 1];
-        const HitRecord h = trace_ray<TRI, ANY>(ts, stack, blockDim.x, xyz(a), xyz(b), a.w, b.w);
+        const float4 a = rays[2 * (size_t)i + 0];
+        const float4 b = rays[2 * (size_t)i + 1];
+        o = xyz(a);
+        d = xyz(b);
+        tmin = a.w;
+        tmax = b.w;
+    }
+    __device__ __forceinline__ void store(uint32_t i, const HitRecord &h) const
+    {
         if(ANY)
             out[i] = h.gid != 0xffffffffu ? 1u : 0u;
         else
         {
-            uint32_t *o = out + 5 * i;
+            uint32_t *o = out + 5 * (size_t)i;
             const bool hitp = h.gid != 0xffffffffu;
-            o[0] = __float_as_uint(hitp ? h.t : b.w);
+            o[0] = __float_as_uint(h.t);
             o[1] = hitp ? h.inst : 0xffffffffu;
             o[2] = hitp ? h.prim : 0xffffffffu;
             o[3] = __float_as_uint(h.b1);
             o[4] = __float_as_uint(h.b2);
         }
     }
+};
+
+template<int TRI, bool ANY, bool POSTPONE>
+__global__ void __launch_bounds__(TRACE_BLOCK) k_trace_batch(
+    TraceScene ts, const float4 *rays, int32_t n, uint32_t *out, uint32_t *fetch, TraceOverflow ovf)
+{
+    BatchSink<ANY> sink{ rays, out };
+    trace_persistent<TRI, ANY, POSTPONE>(ts, sink, n, fetch, ovf);
 }
 
 // ---------------------------------------------------------------- test hooks
@@ -435,11 +482,13 @@ void launch_generate(hipStream_t s, int grid, const RenderConsts &rc, const Wave
     hipLaunchKernelGGL(k_generate, dim3(grid), dim3(256), 0, s, rc, c, path);
 }
 
-void launch_trace_extension(hipStream_t s, int grid, int stack, const TraceScene &ts, const WaveCounters *c,
-                            PathSOA path, HitSOA hit)
+void launch_trace_extension(hipStream_t s, int grid, bool postpone, const TraceScene &ts, WaveCounters *c,
+                            PathSOA path, HitSOA hit, const TraceOverflow &ovf)
 {
-    hipLaunchKernelGGL(k_trace_extension, dim3(grid), dim3(128), (size_t)stack * 128 * sizeof(int32_t), s,
-                       ts, c, path, hit);
+    if(postpone)
+        hipLaunchKernelGGL(k_trace_extension<true>, dim3(grid), dim3(TRACE_BLOCK), 0, s, ts, c, path, hit, ovf);
+    else
+        hipLaunchKernelGGL(k_trace_extension<false>, dim3(grid), dim3(TRACE_BLOCK), 0, s, ts, c, path, hit, ovf);
 }
 
 void launch_shade(hipStream_t s, int grid, const RenderConsts &rc, const SceneView &sv, WaveCounters *c,
@@ -448,11 +497,13 @@ void launch_shade(hipStream_t s, int grid, const RenderConsts &rc, const SceneVi
     hipLaunchKernelGGL(k_shade, dim3(grid), dim3(256), 0, s, rc, sv, c, path, hit, next, shadow, film);
 }
 
-void launch_shadow(hipStream_t s, int grid, int stack, const TraceScene &ts, const WaveCounters *c,
-                   ShadowSOA shadow, FilmView film, int32_t)
+void launch_shadow(hipStream_t s, int grid, bool postpone, const TraceScene &ts, WaveCounters *c,
+                   ShadowSOA shadow, FilmView film, const TraceOverflow &ovf)
 {
-    hipLaunchKernelGGL(k_shadow, dim3(grid), dim3(128), (size_t)stack * 128 * sizeof(int32_t), s,
-                       ts, c, shadow, film);
+    if(postpone)
+        hipLaunchKernelGGL(k_shadow<true>, dim3(grid), dim3(TRACE_BLOCK), 0, s, ts, c, shadow, film, ovf);
+    else
+        hipLaunchKernelGGL(k_shadow<false>, dim3(grid), dim3(TRACE_BLOCK), 0, s, ts, c, shadow, film, ovf);
 }
 
 void launch_end_iteration(hipStream_t s, WaveCounters *c)
@@ -465,23 +516,32 @@ void launch_resolve(hipStream_t s, const FilmView &film, int32_t pixels, float i
     hipLaunchKernelGGL(k_resolve, dim3((pixels + 255) / 256), dim3(256), 0, s, film, pixels, inv_weight, out);
 }
 
-void launch_trace_batch(hipStream_t s, int grid, int stack, const TraceScene &ts, int tri_test, bool any_hit,
-                        const float4 *rays, int64_t n, uint32_t *out)
+template<int TRI, bool ANY>
+static void launch_batch_t(hipStream_t s, int grid, bool postpone, const TraceScene &ts, const float4 *rays,
+                           int32_t n, uint32_t *out, uint32_t *fetch, const TraceOverflow &ovf)
 {
-    const size_t lds = (size_t)stack * 128 * sizeof(int32_t);
+    if(postpone)
+        hipLaunchKernelGGL((k_trace_batch<TRI, ANY, true>), dim3(grid), dim3(TRACE_BLOCK), 0, s, ts, rays, n, out, fetch, ovf);
+    else
+        hipLaunchKernelGGL((k_trace_batch<TRI, ANY, false>), dim3(grid), dim3(TRACE_BLOCK), 0, s, ts, rays, n, out, fetch, ovf);
+}
+
+void launch_trace_batch(hipStream_t s, int grid, bool postpone, const TraceScene &ts, int tri_test, bool any_hit,
+                        const float4 *rays, int32_t n, uint32_t *out, uint32_t *fetch, const TraceOverflow &ovf)
+{
     if(tri_test == TRI_WATERTIGHT)
     {
         if(any_hit)
-            hipLaunchKernelGGL((k_trace_batch<TRI_WATERTIGHT, true>), dim3(grid), dim3(128), lds, s, ts, rays, n, out);
+            launch_batch_t<TRI_WATERTIGHT, true>(s, grid, postpone, ts, rays, n, out, fetch, ovf);
         else
-            hipLaunchKernelGGL((k_trace_batch<TRI_WATERTIGHT, false>), dim3(grid), dim3(128), lds, s, ts, rays, n, out);
+            launch_batch_t<TRI_WATERTIGHT, false>(s, grid, postpone, ts, rays, n, out, fetch, ovf);
     }
     else
     {
         if(any_hit)
-            hipLaunchKernelGGL((k_trace_batch<TRI_MOLLER_TRUMBORE, true>), dim3(grid), dim3(128), lds, s, ts, rays, n, out);
+            launch_batch_t<TRI_MOLLER_TRUMBORE, true>(s, grid, postpone, ts, rays, n, out, fetch, ovf);
         else
-            hipLaunchKernelGGL((k_trace_batch<TRI_MOLLER_TRUMBORE, false>), dim3(grid), dim3(128), lds, s, ts, rays, n, out);
+            launch_batch_t<TRI_MOLLER_TRUMBORE, false>(s, grid, postpone, ts, rays, n, out, fetch, ovf);
     }
 }
 

@@ -163,11 +163,23 @@ typedef struct btrc_hit
     float u, v;        /* barycentric weights of vertices b and c */
 } btrc_hit;
 
+/* Scheduling knobs of the kernels; they change speed, never results.
+ * Picked per device by btrc_amd/autotune.py. */
+typedef struct btrc_tuning
+{
+    int32_t closest_postpone_leaves; /* 1: leaves wait until every lane holds one (default 1) */
+    int32_t any_postpone_leaves;     /* same for visibility rays (default 0) */
+    int32_t trace_waves_per_cu;      /* persistent traversal grid, 1..20 (default 20) */
+    int32_t reserved[5];
+} btrc_tuning;
+
 /* ---- context ---- */
 int btrc_hip_create(int device, btrc_ctx **out);
 void btrc_hip_destroy(btrc_ctx *ctx);
 const char *btrc_hip_last_error(const btrc_ctx *ctx);
 int btrc_hip_device_name(btrc_ctx *ctx, char *buf, size_t len);
+int btrc_hip_set_tuning(btrc_ctx *ctx, const btrc_tuning *tuning);
+int btrc_hip_get_tuning(btrc_ctx *ctx, btrc_tuning *tuning);
 
 /* ---- scene and camera (Scene::commit, PinholeCamera::commit) ---- */
 int btrc_hip_upload_scene(btrc_ctx *ctx, const btrc_scene_desc *scene);

@@ -1,0 +1,58 @@
+"""Every scheduling variant of the traversal returns the same bits."""
+import numpy as np
+import pytest
+
+from btrc_amd import _abi, scenes
+
+from util import assert_hits_equal, random_rays, render_params
+
+pytestmark = pytest.mark.gpu
+
+VARIANTS = [dict(closest_postpone_leaves=c, any_postpone_leaves=a, trace_waves_per_cu=w)
+            for c in (0, 1) for a in (0, 1) for w in (1, 7, 20)]
+
+
+def test_tuning_roundtrip_and_validation(ctx):
+    ctx.set_tuning(trace_waves_per_cu=7)
+    assert ctx.get_tuning()["trace_waves_per_cu"] == 7
+    with pytest.raises(Exception):
+        ctx.set_tuning(trace_waves_per_cu=0)
+    ctx.set_tuning(closest_postpone_leaves=1, any_postpone_leaves=0, trace_waves_per_cu=20)
+
+
+def test_traversal_variants_bitwise_equal(ctx):
+    scene = scenes.sphere_field(instances=1024, lights=8, seed=6, extent=50.0)[0]
+    ctx.upload_scene(scene)
+    rays = random_rays(200000, [0, 0, 0], [50, 50, 50], 31)
+    short = rays.copy()
+    short["tmax"] = np.float32(5.0)
+    ref_h = ref_o = None
+    for v in VARIANTS:
+        ctx.set_tuning(**v)
+        for mode in (_abi.TRI_WATERTIGHT, _abi.TRI_MOLLER_TRUMBORE):
+            h = ctx.trace_closest(rays, mode)
+            o = ctx.trace_any(short, mode)
+            if ref_h is None:
+                ref_h, ref_o = {}, {}
+            if mode not in ref_h:
+                ref_h[mode], ref_o[mode] = h, o
+            else:
+                assert_hits_equal(h, ref_h[mode])
+                assert np.array_equal(o, ref_o[mode])
+    ctx.set_tuning(closest_postpone_leaves=1, any_postpone_leaves=0, trace_waves_per_cu=20)
+
+
+def test_render_variants_equal(ctx):
+    scene, cam = scenes.cornell()
+    ctx.upload_scene(scene)
+    p = render_params(96, 96, 4, state_count=5000)
+    ctx.set_camera(cam, 96, 96)
+    imgs = []
+    for v in VARIANTS[::3]:
+        ctx.set_tuning(**v)
+        ctx.render(p)
+        imgs.append(ctx.film(96, 96)[0])
+    for im in imgs[1:]:
+        # only the FP32 atomic accumulation order may differ
+        np.testing.assert_allclose(im, imgs[0], rtol=1e-5, atol=1e-6)
+    ctx.set_tuning(closest_postpone_leaves=1, any_postpone_leaves=0, trace_waves_per_cu=20)
