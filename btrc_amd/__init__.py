@@ -121,6 +121,25 @@ class Context:
             self.h, rgba.ctypes.data, rgb64.ctypes.data if fp64 else None))
         return rgba, rgb64
 
+    def aov(self, width, height, albedo=True, normal=True):
+        """first-hit albedo and shading-normal images (RGBA), or None"""
+        a = np.empty((height, width, 4), dtype=np.float32) if albedo else None
+        n = np.empty((height, width, 4), dtype=np.float32) if normal else None
+        self._check(self.lib.btrc_hip_aov_readback(self.h, a.ctypes.data if albedo else None,
+                                                   n.ctypes.data if normal else None))
+        return a, n
+
+    def set_filter(self, kind="box", radius=1.3, alpha=1.6):
+        f = _abi.FilterDesc()
+        if kind == "box":
+            f.type = _abi.FILTER_BOX
+        elif kind == "gaussian":
+            f.type = _abi.FILTER_GAUSSIAN
+        else:
+            raise BtrcException(_abi.BTRC_ERR_INVALID, f"unknown film filter {kind!r}")
+        f.radius, f.alpha = float(radius), float(alpha)
+        self._check(self.lib.btrc_hip_set_filter(self.h, C.byref(f)))
+
     def request_stop(self):
         self._check(self.lib.btrc_hip_request_stop(self.h))
 
@@ -170,6 +189,15 @@ class Context:
                                                   si.ctypes.data, len(px), dims, out.ctypes.data))
         return out
 
+    def eval_independent(self, px, py, sample_index, dims):
+        px = np.ascontiguousarray(px, dtype=np.uint32)
+        py = np.ascontiguousarray(py, dtype=np.uint32)
+        si = np.ascontiguousarray(sample_index, dtype=np.int32)
+        out = np.empty((len(px), dims), dtype=np.float32)
+        self._check(self.lib.btrc_hip_eval_independent(self.h, px.ctypes.data, py.ctypes.data, si.ctypes.data,
+                                                       len(px), dims, out.ctypes.data))
+        return out
+
     def eval_detmath(self, fn, x, y=None):
         x = np.ascontiguousarray(x, dtype=np.float32)
         y = np.ascontiguousarray(np.ones_like(x) if y is None else y, dtype=np.float32)
@@ -180,13 +208,16 @@ class Context:
 
 
 class RenderResult:
-    """Renderer::RenderResult (core/renderer.h:19-24): the resolved colour
-    image as host memory, plus the run's statistics."""
+    """Renderer::RenderResult (core/renderer.h:19-24): the resolved colour,
+    albedo and normal images as host memory (albedo / normal None unless
+    requested), plus the run's statistics."""
 
-    def __init__(self, color, color64, stats):
+    def __init__(self, color, color64, stats, albedo=None, normal=None):
         self.color = color
         self.color64 = color64
         self.stats = stats
+        self.albedo = albedo
+        self.normal = normal
 
 
 class WavefrontPathTracer:
@@ -194,11 +225,12 @@ class WavefrontPathTracer:
     WavefrontPathTracer::Params (builtin/renderer/wavefront.h:13-27) and its
     creator (wavefront.cpp:375-398)."""
 
-    DEFAULTS = dict(spp=128, min_depth=5, max_depth=10, rr_threshold=0.1, rr_cont_prob=0.5,
-                    state_count=1000000)
+    DEFAULTS = dict(tile=False, spp=128, min_depth=5, max_depth=10, rr_threshold=0.1, rr_cont_prob=0.5,
+                    state_count=1000000, albedo=False, normal=False)
+    SAMPLERS = {"halton": _abi.SAMPLER_HALTON, "independent": _abi.SAMPLER_INDEPENDENT}
 
     def __init__(self, context=None, **params):
-        unknown = set(params) - set(self.DEFAULTS) - {"film_fp64", "time_stages"}
+        unknown = set(params) - set(self.DEFAULTS) - {"film_fp64", "time_stages", "sampler"}
         if unknown:
             raise BtrcException(_abi.BTRC_ERR_INVALID, f"unknown wfpt parameters: {sorted(unknown)}")
         self.ctx = context or Context()
@@ -206,7 +238,10 @@ class WavefrontPathTracer:
         self.params.update(params)
         self.scene = None
         self.camera = None
+        if self.params.get("sampler", "halton") not in self.SAMPLERS:
+            raise BtrcException(_abi.BTRC_ERR_INVALID, f"unknown sampler {self.params['sampler']!r}")
         self.width, self.height = 512, 512
+        self.filter = ("box", 1.3, 1.6)
         self._committed = False
 
     def set_scene(self, scene):
@@ -215,6 +250,12 @@ class WavefrontPathTracer:
 
     def set_camera(self, camera):
         self.camera = camera
+        self._committed = False
+
+    def set_film_filter(self, kind="box", radius=1.3, alpha=1.6):
+        """WavefrontPathTracer::set_film_filter (wavefront.cpp:72-75): "box"
+        or "gaussian" (GaussianFilterCreator defaults, gaussian.cpp:68-76)"""
+        self.filter = (kind, radius, alpha)
         self._committed = False
 
     def set_film(self, width, height):
@@ -226,6 +267,7 @@ class WavefrontPathTracer:
             raise BtrcException(_abi.BTRC_ERR_STATE, "commit needs a scene and a camera")
         self.ctx.upload_scene(self.scene)
         self.ctx.set_camera(self.camera, self.width, self.height)
+        self.ctx.set_filter(*self.filter)
         self._committed = True
 
     def render_params(self):
@@ -239,6 +281,10 @@ class WavefrontPathTracer:
         p.state_count = int(self.params["state_count"])
         p.film_fp64 = 1 if self.params.get("film_fp64") else 0
         p.time_stages = 1 if self.params.get("time_stages") else 0
+        p.tile = 1 if self.params["tile"] else 0
+        p.sampler = self.SAMPLERS[self.params.get("sampler", "halton")]
+        p.albedo = 1 if self.params["albedo"] else 0
+        p.normal = 1 if self.params["normal"] else 0
         return p
 
     def render(self):
@@ -246,7 +292,11 @@ class WavefrontPathTracer:
             self.commit()
         stats = self.ctx.render(self.render_params())
         color, color64 = self.ctx.film(self.width, self.height, fp64=bool(self.params.get("film_fp64")))
-        return RenderResult(color, color64, stats)
+        albedo = normal = None
+        if self.params["albedo"] or self.params["normal"]:
+            albedo, normal = self.ctx.aov(self.width, self.height, albedo=bool(self.params["albedo"]),
+                                          normal=bool(self.params["normal"]))
+        return RenderResult(color, color64, stats, albedo, normal)
 
     def stop_async(self):
         """Renderer::stop_async (core/renderer.cpp:38-48)"""

@@ -12,6 +12,10 @@ MATERIAL_DIFFUSE = 0
 MATERIAL_DISNEY = 1
 ENVIR_NONE = 0
 ENVIR_GRADIENT_SKY = 1
+SAMPLER_HALTON = 0
+SAMPLER_INDEPENDENT = 1
+FILTER_BOX = 0
+FILTER_GAUSSIAN = 1
 TRI_WATERTIGHT = 0
 TRI_MOLLER_TRUMBORE = 1
 
@@ -62,7 +66,12 @@ class RenderParams(C.Structure):
                 ("min_depth", C.c_int32), ("max_depth", C.c_int32),
                 ("rr_threshold", C.c_float), ("rr_cont_prob", C.c_float),
                 ("state_count", C.c_int32), ("film_fp64", C.c_int32), ("time_stages", C.c_int32),
-                ("reserved", C.c_int32 * 6)]
+                ("tile", C.c_int32), ("sampler", C.c_int32), ("albedo", C.c_int32), ("normal", C.c_int32),
+                ("reserved", C.c_int32 * 2)]
+
+
+class FilterDesc(C.Structure):
+    _fields_ = [("type", C.c_int32), ("radius", C.c_float), ("alpha", C.c_float)]
 
 
 class RenderStats(C.Structure):
@@ -100,11 +109,11 @@ def hit_dtype():
 EXPORTS = [
     "btrc_hip_create", "btrc_hip_destroy", "btrc_hip_last_error", "btrc_hip_device_name",
     "btrc_hip_set_tuning", "btrc_hip_get_tuning",
-    "btrc_hip_upload_scene", "btrc_hip_set_camera", "btrc_hip_scene_info",
-    "btrc_hip_render", "btrc_hip_film_readback", "btrc_hip_request_stop",
+    "btrc_hip_upload_scene", "btrc_hip_set_camera", "btrc_hip_scene_info", "btrc_hip_set_filter",
+    "btrc_hip_render", "btrc_hip_film_readback", "btrc_hip_aov_readback", "btrc_hip_request_stop",
     "btrc_hip_trace_closest", "btrc_hip_trace_any",
     "btrc_hip_batch_upload", "btrc_hip_batch_trace", "btrc_hip_batch_download",
-    "btrc_hip_capture_rays", "btrc_hip_eval_halton", "btrc_hip_eval_detmath",
+    "btrc_hip_capture_rays", "btrc_hip_eval_halton", "btrc_hip_eval_independent", "btrc_hip_eval_detmath",
 ]
 
 
@@ -122,7 +131,9 @@ def declare(lib):
         "btrc_hip_upload_scene": (C.c_int, [ctx, P(SceneDesc)]),
         "btrc_hip_set_camera": (C.c_int, [ctx, P(CameraDesc)]),
         "btrc_hip_scene_info": (C.c_int, [ctx, P(C.c_int64), P(C.c_int32), P(C.c_int32), P(C.c_int64)]),
+        "btrc_hip_set_filter": (C.c_int, [ctx, P(FilterDesc)]),
         "btrc_hip_render": (C.c_int, [ctx, P(RenderParams), P(RenderStats)]),
+        "btrc_hip_aov_readback": (C.c_int, [ctx, C.c_void_p, C.c_void_p]),
         "btrc_hip_film_readback": (C.c_int, [ctx, C.c_void_p, C.c_void_p]),
         "btrc_hip_request_stop": (C.c_int, [ctx]),
         "btrc_hip_trace_closest": (C.c_int, [ctx, C.c_void_p, C.c_int64, C.c_int32, C.c_void_p]),
@@ -133,6 +144,8 @@ def declare(lib):
         "btrc_hip_capture_rays": (C.c_int, [ctx, P(RenderParams), C.c_int32, C.c_void_p, C.c_int64, P(C.c_int64)]),
         "btrc_hip_eval_halton": (C.c_int, [ctx, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p,
                                            C.c_int32, C.c_int32, C.c_void_p]),
+        "btrc_hip_eval_independent": (C.c_int, [ctx, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_int32,
+                                                C.c_void_p]),
         "btrc_hip_eval_detmath": (C.c_int, [ctx, C.c_int32, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p]),
     }
     for name, (res, args) in sig.items():

@@ -1,10 +1,13 @@
-// Halton sampler with per-digit Owen scrambling and pixel stratification,
-// restating HaltonSampler (builtin/sampler/halton.cpp:39-133) and the
-// low-discrepancy helpers it calls (utils/low_discrepancy.cpp:10-110, after
-// pbrt-v4). The wavefront tracer of the reference aliases its global sampler
-// to an independent PCG stream (renderer/wavefront/common.h:37); this build
-// wires the Halton sampler in its place, so every random number of a path is
-// a pure function of (pixel, sample index, dimension).
+// The two samplers of the wavefront path:
+//  * Halton with per-digit Owen scrambling and pixel stratification,
+//    restating HaltonSampler (builtin/sampler/halton.cpp:39-133) and the
+//    low-discrepancy helpers it calls (utils/low_discrepancy.cpp:10-110,
+//    after pbrt-v4); every number of a path is a pure function of (pixel,
+//    sample index, dimension);
+//  * the independent sampler the reference's wavefront tracer uses by
+//    default (renderer/wavefront/common.h:37, sampler/independent.cpp:6-25):
+//    one PCG32 stream per pixel seeded by MurmurHash64A of the pixel
+//    coordinates, advanced by 65536 numbers per sample index.
 #pragma once
 
 #include "vecmath.h"
@@ -210,6 +213,94 @@ BTRC_HD void halton_get3d(const int *primes, HaltonState &st, float *a, float *b
     *b = sample_halton(primes, st.dimension + 1, st.index);
     *c = sample_halton(primes, st.dimension + 2, st.index);
     st.dimension = st.dimension + 3;
+}
+
+// ---- independent sampler
+
+// MurmurHash64A (Austin Appleby, public domain; the form of utils/hash.h:12-71)
+// of the 8-byte key (x, y) as hash::hash(pixel.x, pixel.y) lays it out
+// (two little-endian uint32, seed 0): one whole block, no tail bytes.
+BTRC_HD uint64_t murmur_hash_pixel(uint32_t x, uint32_t y)
+{
+    const uint64_t m = 0xc6a4a7935bd1e995ull;
+    uint64_t h = 8ull * m;
+    uint64_t k = (uint64_t)x | ((uint64_t)y << 32);
+    k = k * m;
+    k = k ^ (k >> 47);
+    k = k * m;
+    h = (h ^ k) * m;
+    h = h ^ (h >> 47);
+    h = h * m;
+    h = h ^ (h >> 47);
+    return h;
+}
+
+// PCG32 (O'Neill, "PCG: A Family of Simple Fast Space-Efficient Statistically
+// Good Algorithms for Random Number Generation", 2014): 64-bit LCG state,
+// XSH-RR output. Seeding by stream index as pbrt-v4's RNG(seq_index).
+struct Pcg32
+{
+    uint64_t state;
+    uint64_t inc;
+};
+
+constexpr uint64_t PCG32_MULT = 0x5851f42d4c957f2dull;
+
+BTRC_HD uint32_t pcg32_next(Pcg32 &r)
+{
+    const uint64_t old = r.state;
+    r.state = old * PCG32_MULT + r.inc;
+    const uint32_t xs = (uint32_t)(((old >> 18) ^ old) >> 27);
+    const uint32_t rot = (uint32_t)(old >> 59);
+    return (xs >> rot) | (xs << ((0u - rot) & 31u));
+}
+
+// stream `seq`, starting offset `offset` (pcg32_srandom_r)
+BTRC_HD Pcg32 pcg32_seed(uint64_t seq, uint64_t offset)
+{
+    Pcg32 r;
+    r.state = 0;
+    r.inc = (seq << 1) | 1ull;
+    (void)pcg32_next(r);
+    r.state = r.state + offset;
+    (void)pcg32_next(r);
+    return r;
+}
+
+// jump ahead by delta numbers in O(log delta) (Brown, "Random Number
+// Generation with Arbitrary Strides", 1994)
+BTRC_HD void pcg32_advance(Pcg32 &r, uint64_t delta)
+{
+    uint64_t cur_mult = PCG32_MULT, cur_plus = r.inc;
+    uint64_t acc_mult = 1, acc_plus = 0;
+    while(delta > 0)
+    {
+        if(delta & 1ull)
+        {
+            acc_mult = acc_mult * cur_mult;
+            acc_plus = acc_plus * cur_mult + cur_plus;
+        }
+        cur_plus = (cur_mult + 1ull) * cur_plus;
+        cur_mult = cur_mult * cur_mult;
+        delta >>= 1;
+    }
+    r.state = acc_mult * r.state + acc_plus;
+}
+
+// uniform float in [0, 1): 32 random bits times 2^-32, capped below 1
+BTRC_HD float pcg32_uniform_float(Pcg32 &r)
+{
+    const float f = (float)pcg32_next(r) * 2.3283064365386963e-10f;
+    return f < 0x1.fffffep-1f ? f : 0x1.fffffep-1f;
+}
+
+// IndependentSampler(res, pixel, sample_index), independent.cpp:11-15
+BTRC_HD Pcg32 independent_start(uint32_t px, uint32_t py, int32_t sample_index)
+{
+    const uint64_t seq = murmur_hash_pixel(px, py);
+    Pcg32 r = pcg32_seed(seq, mix_bits(seq));
+    pcg32_advance(r, (uint64_t)((int64_t)sample_index * 65536));
+    return r;
 }
 
 } // namespace btrc

@@ -202,9 +202,14 @@ struct btrc_ctx
     WaveCounters *host_counters = nullptr; // pinned
     DevBuf<float> film;
     DevBuf<double> film64;
+    DevBuf<float> film_albedo, film_normal;
     DevBuf<float4> resolved;
     int32_t film_w = 0, film_h = 0, film_spp = 0;
-    bool film_has64 = false;
+    bool film_has64 = false, film_has_albedo = false, film_has_normal = false;
+
+    // film filter
+    FilterConsts filter{ FILTER_BOX, 0.0f, 0.0f, nullptr };
+    DevBuf<AliasUnit> filter_table;
 
     // traversal: per-lane stack overflow and the batch fetch counter
     DevBuf<uint32_t> ovf_code;
@@ -293,6 +298,91 @@ int guarded(btrc_ctx *ctx, F &&f)
     }
 }
 
+// AliasTable::initialize (utils/math/alias.cpp:13-58). The sum runs in index
+// order (the reference's std::reduce leaves the order to the library).
+// Returns the sum of the weights.
+float build_alias_table(const std::vector<float> &w, std::vector<AliasUnit> &units)
+{
+    float sum = 0.0f;
+    for(float a : w)
+        sum += a;
+    const size_t n = w.size();
+    units.assign(n, AliasUnit{});
+    if(n == 0)
+        return sum;
+    const float ratio = (float)n / sum;
+    std::vector<uint32_t> overs, unders;
+    for(uint32_t i = 0; i < (uint32_t)n; ++i)
+    {
+        const float pr = w[i] * ratio;
+        units[i].accept_prob = pr;
+        units[i].another_idx = i;
+        if(pr > 1)
+            overs.push_back(i);
+        else if(pr < 1)
+            unders.push_back(i);
+    }
+    while(!overs.empty() && !unders.empty())
+    {
+        const uint32_t over = overs.back();
+        const uint32_t under = unders.back();
+        overs.pop_back();
+        unders.pop_back();
+        units[over].accept_prob -= 1 - units[under].accept_prob;
+        units[under].another_idx = over;
+        if(units[over].accept_prob > 1)
+            overs.push_back(over);
+        else if(units[over].accept_prob < 1)
+            unders.push_back(over);
+    }
+    for(auto i : overs)
+        units[i].accept_prob = 1;
+    for(auto i : unders)
+        units[i].accept_prob = 1;
+    return sum;
+}
+
+// GaussianFilter::commit (film_filter/gaussian.cpp:27-52): separable
+// truncated Gaussian on a GAUSSIAN_GRID^2 grid over [-r, r]^2, zero outside
+// the disc of radius r, turned into an alias table
+void set_filter(btrc_ctx *ctx, const btrc_filter_desc *f)
+{
+    if(!f)
+        throw InvalidArg("filter is null");
+    if(f->type == BTRC_FILTER_BOX)
+    {
+        ctx->filter = FilterConsts{ FILTER_BOX, 0.0f, 0.0f, nullptr };
+        return;
+    }
+    if(f->type != BTRC_FILTER_GAUSSIAN)
+        throw InvalidArg("unknown filter type");
+    if(!(f->radius > 0.0f) || !std::isfinite(f->radius) || !std::isfinite(f->alpha))
+        throw InvalidArg("gaussian filter needs a finite positive radius and a finite alpha");
+    const float r = f->radius, alpha = f->alpha;
+    const float expv = std::exp(-alpha * r * r);
+    auto g = [&](float d) { return std::max(0.0f, std::exp(-alpha * d * d) - expv); };
+    const int G = GAUSSIAN_GRID;
+    std::vector<float> value((size_t)G * G, 0.0f);
+    for(int y = 0; y < G; ++y)
+    {
+        const float yf = 2 * r * (y + 0.5f) / G - r;
+        for(int x = 0; x < G; ++x)
+        {
+            const float xf = 2 * r * (x + 0.5f) / G - r;
+            if(xf * xf + yf * yf < r * r)
+                value[(size_t)y * G + x] = g(xf) * g(yf);
+        }
+    }
+    std::vector<AliasUnit> units;
+    const float sum = build_alias_table(value, units);
+    if(!(sum > 0.0f))
+        throw InvalidArg("gaussian filter has no weight (alpha too large for the radius)");
+    BTRC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    ctx->filter_table.upload(units.data(), units.size(), ctx->stream);
+    BTRC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    ctx->filter = FilterConsts{ FILTER_GAUSSIAN, r, 2 * r / G, ctx->filter_table.ptr };
+}
+
 // persistent traversal grid and the per-lane stack overflow it needs
 void size_traversal(btrc_ctx *ctx)
 {
@@ -365,45 +455,8 @@ void upload_scene(btrc_ctx *ctx, const btrc_scene_desc *sd)
             lpos.push_back(make_float4(ca.x, ca.y, ca.z, 0.0f));
             areas[t] = 0.5f * length(cross(ba, ca));
         }
-        // AliasTable::initialize (utils/math/alias.cpp:13-58), sum in index order
-        float sum = 0.0f;
-        for(float a : areas)
-            sum += a;
-        geos[gi].total_area = sum;
-        const size_t n = areas.size();
-        std::vector<AliasUnit> units(n);
-        if(n)
-        {
-            const float ratio = (float)n / sum;
-            std::vector<uint32_t> overs, unders;
-            for(uint32_t i = 0; i < (uint32_t)n; ++i)
-            {
-                const float pr = areas[i] * ratio;
-                units[i].accept_prob = pr;
-                units[i].another_idx = i;
-                if(pr > 1)
-                    overs.push_back(i);
-                else if(pr < 1)
-                    unders.push_back(i);
-            }
-            while(!overs.empty() && !unders.empty())
-            {
-                const uint32_t over = overs.back();
-                const uint32_t under = unders.back();
-                overs.pop_back();
-                unders.pop_back();
-                units[over].accept_prob -= 1 - units[under].accept_prob;
-                units[under].another_idx = over;
-                if(units[over].accept_prob > 1)
-                    overs.push_back(over);
-                else if(units[over].accept_prob < 1)
-                    unders.push_back(over);
-            }
-            for(auto i : overs)
-                units[i].accept_prob = 1;
-            for(auto i : unders)
-                units[i].accept_prob = 1;
-        }
+        std::vector<AliasUnit> units;
+        geos[gi].total_area = build_alias_table(areas, units);
         alias_units.insert(alias_units.end(), units.begin(), units.end());
         prim_offset += (uint32_t)g.triangle_count;
     }
@@ -636,6 +689,8 @@ void render(btrc_ctx *ctx, const btrc_render_params *p, btrc_render_stats *stats
         throw InvalidArg("width, height, spp and state_count must be positive");
     if(p->min_depth < 0 || p->max_depth < 0)
         throw InvalidArg("depths must be non-negative");
+    if(p->sampler != BTRC_SAMPLER_HALTON && p->sampler != BTRC_SAMPLER_INDEPENDENT)
+        throw InvalidArg("unknown sampler");
     const uint64_t pixels = (uint64_t)p->width * (uint64_t)p->height;
     const uint64_t total = pixels * (uint64_t)p->spp;
     if(pixels > 0x7fffffffull || (uint64_t)p->spp * 31104ull > 0x7fffffffull)
@@ -663,6 +718,18 @@ void render(btrc_ctx *ctx, const btrc_render_params *p, btrc_render_stats *stats
     BTRC_HIP_CHECK(hipMemsetAsync(ctx->film.ptr, 0, 4 * pixels * sizeof(float), s));
     if(p->film_fp64)
         BTRC_HIP_CHECK(hipMemsetAsync(ctx->film64.ptr, 0, 3 * pixels * sizeof(double), s));
+    if(p->albedo)
+    {
+        ctx->film_albedo.alloc(4 * pixels);
+        BTRC_HIP_CHECK(hipMemsetAsync(ctx->film_albedo.ptr, 0, 4 * pixels * sizeof(float), s));
+    }
+    if(p->normal)
+    {
+        ctx->film_normal.alloc(4 * pixels);
+        BTRC_HIP_CHECK(hipMemsetAsync(ctx->film_normal.ptr, 0, 4 * pixels * sizeof(float), s));
+    }
+    ctx->film_has_albedo = p->albedo != 0;
+    ctx->film_has_normal = p->normal != 0;
     ctx->film_w = p->width;
     ctx->film_h = p->height;
     ctx->film_spp = p->spp;
@@ -684,12 +751,15 @@ void render(btrc_ctx *ctx, const btrc_render_params *p, btrc_render_stats *stats
     rc.state_count = p->state_count;
     rc.vol_prim_medium = 0;
     rc.film_fp64 = p->film_fp64;
+    rc.tile = p->tile != 0;
     rc.halton = make_halton_consts(p->width, p->height);
     rc.camera = ctx->camera;
+    rc.filter = ctx->filter;
 
     const TraceScene ts = ctx->trace_scene();
     const SceneView sv = ctx->scene_view();
-    FilmView fv{ ctx->film.ptr, p->film_fp64 ? ctx->film64.ptr : nullptr };
+    FilmView fv{ ctx->film.ptr, p->film_fp64 ? ctx->film64.ptr : nullptr, p->albedo ? ctx->film_albedo.ptr : nullptr,
+                 p->normal ? ctx->film_normal.ptr : nullptr };
     HitSOA hit{ ctx->hit_flag.ptr, ctx->hit_tuv.ptr };
     ShadowSOA sh{ ctx->sh_o.ptr, ctx->sh_d.ptr, ctx->sh_beta.ptr };
 
@@ -712,7 +782,7 @@ void render(btrc_ctx *ctx, const btrc_render_params *p, btrc_render_stats *stats
         PathSOA path = ctx->pool[cur].view();
         PathSOA next = ctx->pool[cur ^ 1].view();
         launch_plan(s, ctx->counters.ptr, p->state_count);
-        timer.run(STAGE_GENERATE, [&] { launch_generate(s, grid256, rc, ctx->counters.ptr, path); });
+        timer.run(STAGE_GENERATE, [&] { launch_generate(s, grid256, p->sampler, rc, ctx->counters.ptr, path); });
         timer.run(STAGE_TRACE, [&] { launch_trace_extension(s, ctx->trace_grid, ctx->tuning.closest_postpone_leaves != 0, ts, ctx->counters.ptr, path, hit, ovf); });
         if(capture && (int64_t)iter < capture_bounces && captured < capture_max)
         {
@@ -733,7 +803,7 @@ void render(btrc_ctx *ctx, const btrc_render_params *p, btrc_render_stats *stats
             }
             captured += n;
         }
-        timer.run(STAGE_SHADE, [&] { launch_shade(s, grid256, rc, sv, ctx->counters.ptr, path, hit, next, sh, fv); });
+        timer.run(STAGE_SHADE, [&] { launch_shade(s, grid256, p->sampler, rc, sv, ctx->counters.ptr, path, hit, next, sh, fv); });
         timer.run(STAGE_SHADOW, [&] { launch_shadow(s, ctx->trace_grid, ctx->tuning.any_postpone_leaves != 0, ts, ctx->counters.ptr, sh, fv, ovf); });
         launch_end_iteration(s, ctx->counters.ptr);
         BTRC_HIP_CHECK(hipGetLastError());
@@ -773,7 +843,7 @@ void film_readback(btrc_ctx *ctx, float *rgba, double *rgb64)
     if(rgba)
     {
         ctx->resolved.alloc(pixels);
-        launch_resolve(s, FilmView{ ctx->film.ptr, nullptr }, pixels, 1.0f / (float)ctx->film_spp, ctx->resolved.ptr);
+        launch_resolve(s, RESOLVE_COLOR, ctx->film.ptr, pixels, 1.0f / (float)ctx->film_spp, ctx->resolved.ptr);
         BTRC_HIP_CHECK(hipGetLastError());
         BTRC_HIP_CHECK(hipMemcpyAsync(rgba, ctx->resolved.ptr, pixels * sizeof(float4), hipMemcpyDeviceToHost, s));
     }
@@ -789,6 +859,33 @@ void film_readback(btrc_ctx *ctx, float *rgba, double *rgb64)
         const double inv = 1.0 / (double)ctx->film_spp;
         for(size_t i = 0; i < 3 * (size_t)pixels; ++i)
             rgb64[i] *= inv;
+    }
+}
+
+// preview.cpp generate_albedo / generate_normal
+void aov_readback(btrc_ctx *ctx, float *albedo, float *normal)
+{
+    if(ctx->film_w <= 0)
+        throw StateError("no film: render first");
+    if((albedo && !ctx->film_has_albedo) || (normal && !ctx->film_has_normal))
+        throw StateError("the last render did not record that output");
+    const int32_t pixels = ctx->film_w * ctx->film_h;
+    hipStream_t s = ctx->stream;
+    ctx->resolved.alloc(pixels);
+    const float inv = 1.0f / (float)ctx->film_spp;
+    if(albedo)
+    {
+        launch_resolve(s, RESOLVE_COLOR, ctx->film_albedo.ptr, pixels, inv, ctx->resolved.ptr);
+        BTRC_HIP_CHECK(hipGetLastError());
+        BTRC_HIP_CHECK(hipMemcpyAsync(albedo, ctx->resolved.ptr, pixels * sizeof(float4), hipMemcpyDeviceToHost, s));
+        BTRC_HIP_CHECK(hipStreamSynchronize(s));
+    }
+    if(normal)
+    {
+        launch_resolve(s, RESOLVE_NORMAL, ctx->film_normal.ptr, pixels, inv, ctx->resolved.ptr);
+        BTRC_HIP_CHECK(hipGetLastError());
+        BTRC_HIP_CHECK(hipMemcpyAsync(normal, ctx->resolved.ptr, pixels * sizeof(float4), hipMemcpyDeviceToHost, s));
+        BTRC_HIP_CHECK(hipStreamSynchronize(s));
     }
 }
 
@@ -934,6 +1031,16 @@ int btrc_hip_film_readback(btrc_ctx *ctx, float *rgba, double *rgb64)
     return guarded(ctx, [&] { film_readback(ctx, rgba, rgb64); });
 }
 
+int btrc_hip_aov_readback(btrc_ctx *ctx, float *albedo_rgba, float *normal_rgba)
+{
+    return guarded(ctx, [&] { aov_readback(ctx, albedo_rgba, normal_rgba); });
+}
+
+int btrc_hip_set_filter(btrc_ctx *ctx, const btrc_filter_desc *filter)
+{
+    return guarded(ctx, [&] { set_filter(ctx, filter); });
+}
+
 int btrc_hip_request_stop(btrc_ctx *ctx)
 {
     if(!ctx)
@@ -1048,6 +1155,28 @@ int btrc_hip_eval_halton(btrc_ctx *ctx, int32_t width, int32_t height, const uin
         ds.upload(sample_index, n, ctx->stream);
         dout.alloc((size_t)n * dims);
         launch_halton_eval(ctx->stream, make_halton_consts(width, height), dx.ptr, dy.ptr, ds.ptr, n, dims, dout.ptr);
+        BTRC_HIP_CHECK(hipGetLastError());
+        BTRC_HIP_CHECK(hipMemcpyAsync(out, dout.ptr, (size_t)n * dims * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+        BTRC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int btrc_hip_eval_independent(btrc_ctx *ctx, const uint32_t *px, const uint32_t *py, const int32_t *sample_index,
+                              int32_t n, int32_t dims, float *out)
+{
+    return guarded(ctx, [&] {
+        if(n < 0 || dims <= 0 || (n > 0 && (!px || !py || !sample_index || !out)))
+            throw InvalidArg("bad sampler query");
+        if(n == 0)
+            return;
+        DevBuf<uint32_t> dx, dy;
+        DevBuf<int32_t> ds;
+        DevBuf<float> dout;
+        dx.upload(px, n, ctx->stream);
+        dy.upload(py, n, ctx->stream);
+        ds.upload(sample_index, n, ctx->stream);
+        dout.alloc((size_t)n * dims);
+        launch_independent_eval(ctx->stream, dx.ptr, dy.ptr, ds.ptr, n, dims, dout.ptr);
         BTRC_HIP_CHECK(hipGetLastError());
         BTRC_HIP_CHECK(hipMemcpyAsync(out, dout.ptr, (size_t)n * dims * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
         BTRC_HIP_CHECK(hipStreamSynchronize(ctx->stream));

@@ -21,7 +21,8 @@ struct PathSOA
     float4 *beta;     // throughput.xyz, depth bits
     float4 *rad;      // path radiance.xyz, unused
     float4 *bsdf_le;  // beta_le.xyz, bsdf pdf (negative: delta lobe)
-    uint4 *smp;       // pixel index, Halton dimension, Halton index lo, hi
+    uint4 *smp;       // pixel index, then Halton: dimension, index lo, hi;
+                      // independent: PCG state lo, hi, unused
 };
 
 struct HitSOA
@@ -67,6 +68,25 @@ struct CameraConsts
     float film_y[3];
 };
 
+enum : int32_t
+{
+    SAMPLER_HALTON = 0,
+    SAMPLER_INDEPENDENT = 1,
+    FILTER_BOX = 0,
+    FILTER_GAUSSIAN = 1,
+};
+
+// GaussianFilter's importance table (film_filter/gaussian.cpp:7-52)
+constexpr int32_t GAUSSIAN_GRID = 256;
+
+struct FilterConsts
+{
+    int32_t type;
+    float radius;
+    float cell;               // 2 * radius / GAUSSIAN_GRID
+    const AliasUnit *table;   // GAUSSIAN_GRID^2 cells, row-major (y, x)
+};
+
 struct RenderConsts
 {
     int32_t width, height, spp;
@@ -75,14 +95,24 @@ struct RenderConsts
     int32_t state_count;
     uint32_t vol_prim_medium;
     int32_t film_fp64;
+    int32_t tile;
     HaltonConsts halton;
     CameraConsts camera;
+    FilterConsts filter;
 };
 
 struct FilmView
 {
     float *rgb;      // 4 floats per pixel (radiance xyz, unused)
     double *rgb64;   // 3 doubles per pixel, or null
+    float *albedo;   // 4 floats per pixel (first-hit albedo xyz), or null
+    float *normal;   // 4 floats per pixel (first-hit shading normal xyz), or null
+};
+
+enum : int32_t
+{
+    RESOLVE_COLOR = 0,   // value / weight
+    RESOLVE_NORMAL = 1,  // value / weight, normalised unless shorter than 1e-3
 };
 
 // kernel timing buckets
@@ -96,15 +126,16 @@ enum : int
 };
 
 void launch_plan(hipStream_t s, WaveCounters *c, int32_t state_count);
-void launch_generate(hipStream_t s, int grid, const RenderConsts &rc, const WaveCounters *c, PathSOA path);
+void launch_generate(hipStream_t s, int grid, int sampler, const RenderConsts &rc, const WaveCounters *c,
+                     PathSOA path);
 void launch_trace_extension(hipStream_t s, int grid, bool postpone, const TraceScene &ts, WaveCounters *c,
                             PathSOA path, HitSOA hit, const TraceOverflow &ovf);
-void launch_shade(hipStream_t s, int grid, const RenderConsts &rc, const SceneView &sv, WaveCounters *c,
-                  PathSOA path, HitSOA hit, PathSOA next, ShadowSOA shadow, FilmView film);
+void launch_shade(hipStream_t s, int grid, int sampler, const RenderConsts &rc, const SceneView &sv,
+                  WaveCounters *c, PathSOA path, HitSOA hit, PathSOA next, ShadowSOA shadow, FilmView film);
 void launch_shadow(hipStream_t s, int grid, bool postpone, const TraceScene &ts, WaveCounters *c,
                    ShadowSOA shadow, FilmView film, const TraceOverflow &ovf);
 void launch_end_iteration(hipStream_t s, WaveCounters *c);
-void launch_resolve(hipStream_t s, const FilmView &film, int32_t pixels, float inv_weight, float4 *out);
+void launch_resolve(hipStream_t s, int mode, const float *value, int32_t pixels, float inv_weight, float4 *out);
 
 // ray batches of the traversal entry points: o.xyz/tmin and d.xyz/tmax
 void launch_trace_batch(hipStream_t s, int grid, bool postpone, const TraceScene &ts, int tri_test, bool any_hit,
@@ -113,6 +144,8 @@ void launch_trace_batch(hipStream_t s, int grid, bool postpone, const TraceScene
 // test hooks
 void launch_halton_eval(hipStream_t s, const HaltonConsts &hc, const uint32_t *px, const uint32_t *py,
                         const int32_t *sample_index, int32_t n, int32_t dims, float *out);
+void launch_independent_eval(hipStream_t s, const uint32_t *px, const uint32_t *py, const int32_t *sample_index,
+                             int32_t n, int32_t dims, float *out);
 void launch_detmath_eval(hipStream_t s, int32_t fn, const float *x, const float *y, int32_t n, float *out);
 
 } // namespace btrc

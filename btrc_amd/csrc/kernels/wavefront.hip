@@ -51,21 +51,75 @@ __device__ __forceinline__ void film_splat(const FilmView &film, uint32_t pixel,
     }
 }
 
+__device__ __forceinline__ void aov_splat(float *buf, uint32_t pixel, V3 v)
+{
+    if(!isfinite3(v))
+        return;
+    float *p = buf + 4 * (size_t)pixel;
+    atomicAdd(p + 0, v.x);
+    atomicAdd(p + 1, v.y);
+    atomicAdd(p + 2, v.z);
+}
+
 __device__ __forceinline__ V3 xyz(float4 a) { return v3(a.x, a.y, a.z); }
 __device__ __forceinline__ float4 f4(V3 a, float w) { return make_float4(a.x, a.y, a.z, w); }
 
-__device__ __forceinline__ HaltonState load_sampler(uint4 s)
-{
-    HaltonState h;
-    h.dimension = s.y;
-    h.index = (uint64_t)s.z | ((uint64_t)s.w << 32);
-    return h;
-}
+// The sampler of a path, kept in PathSOA::smp between stages (x = pixel).
+template<int SAMPLER> struct PathSampler;
 
-__device__ __forceinline__ uint4 pack_sampler(uint32_t pixel, const HaltonState &h)
+template<> struct PathSampler<SAMPLER_HALTON>
 {
-    return make_uint4(pixel, h.dimension, (uint32_t)h.index, (uint32_t)(h.index >> 32));
-}
+    HaltonState st;
+
+    __device__ static PathSampler start(const RenderConsts &rc, uint32_t px, uint32_t py, int32_t sample)
+    {
+        return PathSampler{ halton_start(rc.halton, px, py, sample) };
+    }
+    __device__ static PathSampler load(const RenderConsts &, uint4 s)
+    {
+        PathSampler r;
+        r.st.dimension = s.y;
+        r.st.index = (uint64_t)s.z | ((uint64_t)s.w << 32);
+        return r;
+    }
+    __device__ uint4 pack(uint32_t pixel) const
+    {
+        return make_uint4(pixel, st.dimension, (uint32_t)st.index, (uint32_t)(st.index >> 32));
+    }
+    __device__ float get1d() { return halton_get1d(c_primes.p, st); }
+    __device__ void get3d(float *a, float *b, float *c) { halton_get3d(c_primes.p, st, a, b, c); }
+};
+
+// Sampler::get2d/get3d default to consecutive get1d (core/sampler.h:15-28);
+// the stream increment depends only on the pixel, so it is rederived on load
+template<> struct PathSampler<SAMPLER_INDEPENDENT>
+{
+    Pcg32 rng;
+
+    __device__ static PathSampler start(const RenderConsts &, uint32_t px, uint32_t py, int32_t sample)
+    {
+        return PathSampler{ independent_start(px, py, sample) };
+    }
+    __device__ static PathSampler load(const RenderConsts &rc, uint4 s)
+    {
+        const uint32_t px = s.x % (uint32_t)rc.width, py = s.x / (uint32_t)rc.width;
+        PathSampler r;
+        r.rng.inc = (murmur_hash_pixel(px, py) << 1) | 1ull;
+        r.rng.state = (uint64_t)s.y | ((uint64_t)s.z << 32);
+        return r;
+    }
+    __device__ uint4 pack(uint32_t pixel) const
+    {
+        return make_uint4(pixel, (uint32_t)rng.state, (uint32_t)(rng.state >> 32), 0u);
+    }
+    __device__ float get1d() { return pcg32_uniform_float(rng); }
+    __device__ void get3d(float *a, float *b, float *c)
+    {
+        *a = get1d();
+        *b = get1d();
+        *c = get1d();
+    }
+};
 
 // ---------------------------------------------------------------- plan
 
@@ -95,6 +149,7 @@ __global__ void k_plan(WaveCounters *c, int32_t state_count)
 
 // ---------------------------------------------------------------- generate
 
+template<int SAMPLER>
 __global__ void __launch_bounds__(256) k_generate(RenderConsts rc, const WaveCounters *c, PathSOA path)
 {
     const int32_t n = c->gen_count;
@@ -103,19 +158,46 @@ __global__ void __launch_bounds__(256) k_generate(RenderConsts rc, const WaveCou
     const uint64_t pixels = (uint64_t)rc.width * (uint64_t)rc.height;
     for(int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
     {
+        // generate.cpp:46-62: uniform order walks all pixels once per sample
+        // index; tile order finishes a pixel's samples before the next pixel
         const uint64_t path_index = base + (uint64_t)i;
-        const uint32_t pixel_index = (uint32_t)(path_index % pixels);
-        const int32_t sample_index = (int32_t)(path_index / pixels);
+        uint32_t pixel_index;
+        int32_t sample_index;
+        if(rc.tile)
+        {
+            pixel_index = (uint32_t)(path_index / (uint64_t)rc.spp);
+            sample_index = (int32_t)(path_index % (uint64_t)rc.spp);
+        }
+        else
+        {
+            pixel_index = (uint32_t)(path_index % pixels);
+            sample_index = (int32_t)(path_index / pixels);
+        }
         const uint32_t px = pixel_index % (uint32_t)rc.width;
         const uint32_t py = pixel_index / (uint32_t)rc.width;
 
-        HaltonState hs = halton_start(rc.halton, px, py, sample_index);
-        // BoxFilter::sample, film_filter/box.cpp:5-9
-        const float fx = halton_get1d(c_primes.p, hs) - 0.5f;
-        const float fy = halton_get1d(c_primes.p, hs) - 0.5f;
+        PathSampler<SAMPLER> smp = PathSampler<SAMPLER>::start(rc, px, py, sample_index);
+        float fx, fy;
+        if(rc.filter.type == FILTER_GAUSSIAN)
+        {
+            // GaussianFilter::sample, gaussian.cpp:54-66
+            const uint32_t cell = alias_sample(rc.filter.table, (uint32_t)(GAUSSIAN_GRID * GAUSSIAN_GRID), smp.get1d());
+            const uint32_t gy = cell / (uint32_t)GAUSSIAN_GRID, gx = cell % (uint32_t)GAUSSIAN_GRID;
+            const float r = rc.filter.radius, w = rc.filter.cell;
+            const float x0 = -r + w * (float)gx, x1 = -r + w * (float)(gx + 1);
+            const float y0 = -r + w * (float)gy, y1 = -r + w * (float)(gy + 1);
+            fx = lerpf(x0, x1, smp.get1d());
+            fy = lerpf(y0, y1, smp.get1d());
+        }
+        else
+        {
+            // BoxFilter::sample, film_filter/box.cpp:5-9
+            fx = smp.get1d() - 0.5f;
+            fy = smp.get1d() - 0.5f;
+        }
         const float film_x = ((float)px + 0.5f + fx) / (float)rc.width;
         const float film_y = ((float)py + 0.5f + fy) / (float)rc.height;
-        (void)halton_get1d(c_primes.p, hs); // time sample, generate.cpp:70
+        (void)smp.get1d(); // time sample, generate.cpp:70
 
         // PinholeCamera::generate_ray_inline, camera/pinhole.cpp:84-99
         const CameraConsts &cam = rc.camera;
@@ -131,7 +213,7 @@ __global__ void __launch_bounds__(256) k_generate(RenderConsts rc, const WaveCou
         path.beta[s] = make_float4(1.0f, 1.0f, 1.0f, __int_as_float(0));
         path.rad[s] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
         path.bsdf_le[s] = make_float4(1.0f, 1.0f, 1.0f, -1.0f);
-        path.smp[s] = pack_sampler(pixel_index, hs);
+        path.smp[s] = smp.pack(pixel_index);
     }
 }
 
@@ -174,6 +256,7 @@ __global__ void __launch_bounds__(TRACE_BLOCK) k_trace_extension(
 
 // ---------------------------------------------------------------- shade
 
+template<int SAMPLER>
 __global__ void __launch_bounds__(256) k_shade(
     RenderConsts rc, SceneView sv, WaveCounters *c, PathSOA path, HitSOA hit,
     PathSOA next, ShadowSOA shadow, FilmView film)
@@ -198,7 +281,7 @@ __global__ void __launch_bounds__(256) k_shade(
             const float4 le4 = path.bsdf_le[i];
             const uint4 smp = path.smp[i];
             const uint32_t pixel = smp.x;
-            HaltonState hs = load_sampler(smp);
+            PathSampler<SAMPLER> hs = PathSampler<SAMPLER>::load(rc, smp);
 
             const V3 o = xyz(ro), d = xyz(rd);
             V3 beta = xyz(beta_depth);
@@ -239,7 +322,7 @@ __global__ void __launch_bounds__(256) k_shade(
                             rr_exit = true;
                         else
                         {
-                            const float u = halton_get1d(c_primes.p, hs);
+                            const float u = hs.get1d();
                             if(luminance(beta) < rc.rr_threshold)
                             {
                                 if(u > rc.rr_cont_prob)
@@ -258,13 +341,22 @@ __global__ void __launch_bounds__(256) k_shade(
                 if(!done)
                 {
                     float l0, l1, l2;
-                    halton_get3d(c_primes.p, hs, &l0, &l1, &l2);
-                    const float lsel = halton_get1d(c_primes.p, hs);
+                    hs.get3d(&l0, &l1, &l2);
+                    const float lsel = hs.get1d();
                     ShadowSample ss = sample_surface_light_li(sv, sp.position, sp.frame.z, l0, l1, l2, lsel);
 
                     const Shader shader = make_shader(sv.materials[inst.material_id], sp.frame, sp.interp_z);
+                    if(depth == 0)
+                    {
+                        // g-buffer of the first hit: Shader::albedo and the
+                        // shading normal (shade.cpp:133-139,196-204)
+                        if(film.albedo)
+                            aov_splat(film.albedo, pixel, shader.albedo);
+                        if(film.normal)
+                            aov_splat(film.normal, pixel, shader.frame.shading.z);
+                    }
                     float s0, s1, s2;
-                    halton_get3d(c_primes.p, hs, &s0, &s1, &s2);
+                    hs.get3d(&s0, &s1, &s2);
                     BSDFSample bs;
                     shader_sample(shader, -d, s0, s1, s2, bs);
 
@@ -308,7 +400,7 @@ __global__ void __launch_bounds__(256) k_shade(
                         nx_beta = f4(beta, __int_as_float(depth + 1));
                         nx_rad = f4(path_rad, 1.0f);
                         nx_le = f4(new_beta_le, stored_pdf);
-                        nx_smp = pack_sampler(pixel, hs);
+                        nx_smp = hs.pack(pixel);
                     }
                     else
                         film_splat(film, pixel, path_rad);
@@ -383,14 +475,18 @@ __global__ void k_end_iteration(WaveCounters *c)
     c->iterations += 1;
 }
 
-// preview resolve, wavefront/preview.cpp:28-51 (weight = samples per pixel)
-__global__ void k_resolve(FilmView film, int32_t pixels, float inv_weight, float4 *out)
+// preview resolve, wavefront/preview.cpp:28-80: value / weight (weight =
+// samples per pixel); normals are normalised, or zero when shorter than 1e-3
+__global__ void k_resolve(int32_t mode, const float *value, int32_t pixels, float inv_weight, float4 *out)
 {
     const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if(i >= pixels)
         return;
-    const float *p = film.rgb + 4 * (size_t)i;
-    out[i] = make_float4(p[0] * inv_weight, p[1] * inv_weight, p[2] * inv_weight, 1.0f);
+    const float *p = value + 4 * (size_t)i;
+    V3 v = v3(p[0] * inv_weight, p[1] * inv_weight, p[2] * inv_weight);
+    if(mode == RESOLVE_NORMAL)
+        v = length(v) > 1e-3f ? normalize(v) : v3s(0.0f);
+    out[i] = f4(v, 1.0f);
 }
 
 // ---------------------------------------------------------------- ray batches
@@ -437,6 +533,17 @@ __global__ void __launch_bounds__(TRACE_BLOCK) k_trace_batch(
 
 // ---------------------------------------------------------------- test hooks
 
+__global__ void k_independent_eval(const uint32_t *px, const uint32_t *py, const int32_t *si, int32_t n,
+                                   int32_t dims, float *out)
+{
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if(i >= n)
+        return;
+    Pcg32 r = independent_start(px[i], py[i], si[i]);
+    for(int32_t k = 0; k < dims; ++k)
+        out[(size_t)i * dims + k] = pcg32_uniform_float(r);
+}
+
 __global__ void k_halton_eval(HaltonConsts hc, const uint32_t *px, const uint32_t *py, const int32_t *si,
                               int32_t n, int32_t dims, float *out)
 {
@@ -477,9 +584,13 @@ void launch_plan(hipStream_t s, WaveCounters *c, int32_t state_count)
     hipLaunchKernelGGL(k_plan, dim3(1), dim3(1), 0, s, c, state_count);
 }
 
-void launch_generate(hipStream_t s, int grid, const RenderConsts &rc, const WaveCounters *c, PathSOA path)
+void launch_generate(hipStream_t s, int grid, int sampler, const RenderConsts &rc, const WaveCounters *c,
+                     PathSOA path)
 {
-    hipLaunchKernelGGL(k_generate, dim3(grid), dim3(256), 0, s, rc, c, path);
+    if(sampler == SAMPLER_INDEPENDENT)
+        hipLaunchKernelGGL(k_generate<SAMPLER_INDEPENDENT>, dim3(grid), dim3(256), 0, s, rc, c, path);
+    else
+        hipLaunchKernelGGL(k_generate<SAMPLER_HALTON>, dim3(grid), dim3(256), 0, s, rc, c, path);
 }
 
 void launch_trace_extension(hipStream_t s, int grid, bool postpone, const TraceScene &ts, WaveCounters *c,
@@ -491,10 +602,15 @@ void launch_trace_extension(hipStream_t s, int grid, bool postpone, const TraceS
         hipLaunchKernelGGL(k_trace_extension<false>, dim3(grid), dim3(TRACE_BLOCK), 0, s, ts, c, path, hit, ovf);
 }
 
-void launch_shade(hipStream_t s, int grid, const RenderConsts &rc, const SceneView &sv, WaveCounters *c,
-                  PathSOA path, HitSOA hit, PathSOA next, ShadowSOA shadow, FilmView film)
+void launch_shade(hipStream_t s, int grid, int sampler, const RenderConsts &rc, const SceneView &sv,
+                  WaveCounters *c, PathSOA path, HitSOA hit, PathSOA next, ShadowSOA shadow, FilmView film)
 {
-    hipLaunchKernelGGL(k_shade, dim3(grid), dim3(256), 0, s, rc, sv, c, path, hit, next, shadow, film);
+    if(sampler == SAMPLER_INDEPENDENT)
+        hipLaunchKernelGGL(k_shade<SAMPLER_INDEPENDENT>, dim3(grid), dim3(256), 0, s, rc, sv, c, path, hit, next,
+                           shadow, film);
+    else
+        hipLaunchKernelGGL(k_shade<SAMPLER_HALTON>, dim3(grid), dim3(256), 0, s, rc, sv, c, path, hit, next, shadow,
+                           film);
 }
 
 void launch_shadow(hipStream_t s, int grid, bool postpone, const TraceScene &ts, WaveCounters *c,
@@ -511,9 +627,9 @@ void launch_end_iteration(hipStream_t s, WaveCounters *c)
     hipLaunchKernelGGL(k_end_iteration, dim3(1), dim3(1), 0, s, c);
 }
 
-void launch_resolve(hipStream_t s, const FilmView &film, int32_t pixels, float inv_weight, float4 *out)
+void launch_resolve(hipStream_t s, int mode, const float *value, int32_t pixels, float inv_weight, float4 *out)
 {
-    hipLaunchKernelGGL(k_resolve, dim3((pixels + 255) / 256), dim3(256), 0, s, film, pixels, inv_weight, out);
+    hipLaunchKernelGGL(k_resolve, dim3((pixels + 255) / 256), dim3(256), 0, s, mode, value, pixels, inv_weight, out);
 }
 
 template<int TRI, bool ANY>
@@ -549,6 +665,13 @@ void launch_halton_eval(hipStream_t s, const HaltonConsts &hc, const uint32_t *p
                         const int32_t *sample_index, int32_t n, int32_t dims, float *out)
 {
     hipLaunchKernelGGL(k_halton_eval, dim3((n + 127) / 128), dim3(128), 0, s, hc, px, py, sample_index, n, dims, out);
+}
+
+void launch_independent_eval(hipStream_t s, const uint32_t *px, const uint32_t *py, const int32_t *sample_index,
+                             int32_t n, int32_t dims, float *out)
+{
+    hipLaunchKernelGGL(k_independent_eval, dim3((n + 127) / 128), dim3(128), 0, s, px, py, sample_index, n, dims,
+                       out);
 }
 
 void launch_detmath_eval(hipStream_t s, int32_t fn, const float *x, const float *y, int32_t n, float *out)

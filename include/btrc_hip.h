@@ -53,6 +53,18 @@ enum
 
 enum
 {
+    BTRC_SAMPLER_HALTON = 0,      /* builtin/sampler/halton.cpp           */
+    BTRC_SAMPLER_INDEPENDENT = 1  /* builtin/sampler/independent.cpp      */
+};
+
+enum
+{
+    BTRC_FILTER_BOX = 0,      /* builtin/film_filter/box.cpp              */
+    BTRC_FILTER_GAUSSIAN = 1  /* builtin/film_filter/gaussian.cpp         */
+};
+
+enum
+{
     BTRC_TRI_WATERTIGHT = 0,       /* renderer's triangle test             */
     BTRC_TRI_MOLLER_TRUMBORE = 1   /* lib/bvh Triangle::intersect order    */
 };
@@ -132,8 +144,21 @@ typedef struct btrc_render_params
     int32_t state_count;
     int32_t film_fp64;   /* also accumulate an FP64 film */
     int32_t time_stages; /* record per-stage kernel time with HIP events */
-    int32_t reserved[6];
+    int32_t tile;        /* Params::tile: path i is sample i % spp of pixel i / spp */
+    int32_t sampler;     /* BTRC_SAMPLER_*; the reference's wfpt uses INDEPENDENT */
+    int32_t albedo;      /* Params::albedo: first-hit albedo output */
+    int32_t normal;      /* Params::normal: first-hit shading normal output */
+    int32_t reserved[2];
 } btrc_render_params;
+
+/* FilmFilter (core/film_filter.h); GaussianFilter defaults radius 1.3,
+ * alpha 1.6 (gaussian.cpp:68-76) */
+typedef struct btrc_filter_desc
+{
+    int32_t type; /* BTRC_FILTER_* */
+    float radius;
+    float alpha;
+} btrc_filter_desc;
 
 typedef struct btrc_render_stats
 {
@@ -187,11 +212,18 @@ int btrc_hip_set_camera(btrc_ctx *ctx, const btrc_camera_desc *camera);
 /* statistics of the built BVH: nodes, depth, stack bound, triangles */
 int btrc_hip_scene_info(btrc_ctx *ctx, int64_t *nodes, int32_t *depth, int32_t *stack_bound, int64_t *triangles);
 
+/* WavefrontPathTracer::set_film_filter (wavefront.cpp:72-75); box until set */
+int btrc_hip_set_filter(btrc_ctx *ctx, const btrc_filter_desc *filter);
+
 /* ---- rendering (WavefrontPathTracer::render) ---- */
 int btrc_hip_render(btrc_ctx *ctx, const btrc_render_params *params, btrc_render_stats *stats);
 /* value / weight per pixel as RGBA (alpha 1), row 0 at the top; rgb64 (3
  * doubles per pixel, same normalisation) needs film_fp64, may be NULL */
 int btrc_hip_film_readback(btrc_ctx *ctx, float *rgba, double *rgb64);
+/* RenderResult::albedo / normal (wavefront.cpp:320-350, preview.cpp:28-80):
+ * RGBA per pixel, alpha 1; either pointer may be NULL; each needs the
+ * matching flag in the last render's params */
+int btrc_hip_aov_readback(btrc_ctx *ctx, float *albedo_rgba, float *normal_rgba);
 /* thread-safe; the running render returns BTRC_ERR_STOPPED */
 int btrc_hip_request_stop(btrc_ctx *ctx);
 
@@ -214,6 +246,9 @@ int btrc_hip_capture_rays(btrc_ctx *ctx, const btrc_render_params *params, int32
 /* ---- test hooks: evaluate device building blocks on host-given inputs ---- */
 int btrc_hip_eval_halton(btrc_ctx *ctx, int32_t width, int32_t height, const uint32_t *px, const uint32_t *py,
                          const int32_t *sample_index, int32_t n, int32_t dims, float *out);
+/* the independent sampler's first `dims` numbers for each (px, py, sample) */
+int btrc_hip_eval_independent(btrc_ctx *ctx, const uint32_t *px, const uint32_t *py, const int32_t *sample_index,
+                              int32_t n, int32_t dims, float *out);
 /* fn: 0 sin, 1 cos, 2 log, 3 exp, 4 pow(x, y), 5 sqrt, 6 x / y */
 int btrc_hip_eval_detmath(btrc_ctx *ctx, int32_t fn, const float *x, const float *y, int32_t n, float *out);
 

@@ -277,6 +277,132 @@ struct Halton
     }
 };
 
+// MurmurHash64A (Austin Appleby, public domain; utils/hash.h:12-71 follows
+// it) over an arbitrary byte string: 8-byte little-endian blocks, then the
+// 1..7 tail bytes, then the final avalanche.
+static uint64_t murmur64a(const uint8_t *key, size_t len, uint64_t seed)
+{
+    const uint64_t m = 0xc6a4a7935bd1e995ull;
+    const int r = 47;
+    uint64_t h = seed ^ (len * m);
+    const size_t blocks = len / 8;
+    for(size_t b = 0; b < blocks; ++b)
+    {
+        uint64_t k = 0;
+        for(int j = 7; j >= 0; --j)
+            k = (k << 8) | key[8 * b + j];
+        k *= m;
+        k ^= k >> r;
+        k *= m;
+        h ^= k;
+        h *= m;
+    }
+    const uint8_t *tail = key + 8 * blocks;
+    const size_t rest = len & 7;
+    if(rest)
+    {
+        for(size_t j = rest; j-- > 0;)
+            h ^= (uint64_t)tail[j] << (8 * j);
+        h *= m;
+    }
+    h ^= h >> r;
+    h *= m;
+    h ^= h >> r;
+    return h;
+}
+
+// hash::hash(pixel.x, pixel.y) (utils/hash.h:85-97): the two uint32 values'
+// bytes back to back, seed 0
+static uint64_t hash_pixel(uint32_t x, uint32_t y)
+{
+    uint8_t buf[8];
+    std::memcpy(buf, &x, 4);
+    std::memcpy(buf + 4, &y, 4);
+    return murmur64a(buf, 8, 0);
+}
+
+// PCG32 (O'Neill 2014): LCG x -> a*x + inc mod 2^64, output XSH-RR of the
+// old state. Jump-ahead composes the affine step map by binary powering.
+struct Pcg
+{
+    static constexpr uint64_t A = 6364136223846793005ull;
+    uint64_t state = 0, inc = 1;
+
+    uint32_t next()
+    {
+        const uint64_t x = state;
+        state = A * x + inc;
+        const uint32_t v = (uint32_t)((x ^ (x >> 18)) >> 27);
+        const unsigned rot = (unsigned)(x >> 59);
+        return rot ? (v >> rot) | (v << (32 - rot)) : v;
+    }
+    // pcg32_srandom_r(initstate = offset, initseq = seq)
+    void seed(uint64_t seq, uint64_t offset)
+    {
+        state = 0;
+        inc = 2 * seq + 1;
+        next();
+        state += offset;
+        next();
+    }
+    void advance(uint64_t n)
+    {
+        // (a, c) is the map x -> a x + c; result = step^n
+        uint64_t ra = 1, rc = 0, sa = A, sc = inc;
+        for(; n; n >>= 1)
+        {
+            if(n & 1)
+            {
+                rc = sa * rc + sc; // step o result
+                ra = sa * ra;
+            }
+            sc = sa * sc + sc;     // step o step
+            sa = sa * sa;
+        }
+        state = ra * state + rc;
+    }
+    float uniform()
+    {
+        const float f = (float)next() / 4294967296.0f;
+        return std::min(f, 0x1.fffffep-1f);
+    }
+};
+
+// IndependentSampler (sampler/independent.cpp:11-25), pbrt-v4 RNG(seq) seeding
+struct Indep
+{
+    Pcg rng;
+    void init(uint32_t px, uint32_t py, int32_t sample)
+    {
+        const uint64_t seq = hash_pixel(px, py);
+        rng.seed(seq, mixbits(seq));
+        rng.advance((uint64_t)((int64_t)sample * 65536));
+    }
+    float get1d() { return rng.uniform(); }
+    void get3d(float &a, float &b, float &c)
+    {
+        a = get1d();
+        b = get1d();
+        c = get1d();
+    }
+};
+
+// the path's sampler: Halton or independent
+struct Smp
+{
+    bool indep = false;
+    Halton h;
+    Indep p;
+    float get1d() { return indep ? p.get1d() : h.get1d(); }
+    void get3d(float &a, float &b, float &c)
+    {
+        if(indep)
+            p.get3d(a, b, c);
+        else
+            h.get3d(a, b, c);
+    }
+};
+
 // ------------------------------------------------------------------ scene
 
 struct Light
@@ -1199,7 +1325,73 @@ struct Params
     int32_t w, h, spp, min_depth, max_depth;
     float rr_thr, rr_p;
     Vec eye, lbc, fx, fy;
+    bool tile = false, indep = false, gaussian = false;
+    float radius = 0.0f, alpha = 0.0f;
+    std::vector<float> g_accept;   // Gaussian filter table, 256 x 256 cells
+    std::vector<uint32_t> g_alias;
 };
+
+// GaussianFilter::commit (film_filter/gaussian.cpp:27-52) with
+// AliasTable::initialize (utils/math/alias.cpp:13-58, sum in index order)
+static void gaussian_table(Params &p)
+{
+    const int G = 256;
+    const float r = p.radius, al = p.alpha;
+    const float ev = std::exp(-al * r * r);
+    std::vector<float> v((size_t)G * G, 0.0f);
+    for(int y = 0; y < G; ++y)
+        for(int x = 0; x < G; ++x)
+        {
+            const float xf = 2 * r * (x + 0.5f) / G - r, yf = 2 * r * (y + 0.5f) / G - r;
+            if(xf * xf + yf * yf >= r * r)
+                continue;
+            const float gx = std::max(0.0f, std::exp(-al * xf * xf) - ev);
+            const float gy = std::max(0.0f, std::exp(-al * yf * yf) - ev);
+            v[(size_t)y * G + x] = gx * gy;
+        }
+    float sum = 0.0f;
+    for(float a : v)
+        sum += a;
+    const uint32_t n = (uint32_t)v.size();
+    p.g_accept.assign(n, 0.0f);
+    p.g_alias.assign(n, 0);
+    const float ratio = (float)n / sum;
+    std::vector<uint32_t> ov, un;
+    for(uint32_t i = 0; i < n; ++i)
+    {
+        p.g_accept[i] = v[i] * ratio;
+        p.g_alias[i] = i;
+        if(p.g_accept[i] > 1) ov.push_back(i);
+        else if(p.g_accept[i] < 1) un.push_back(i);
+    }
+    while(!ov.empty() && !un.empty())
+    {
+        const uint32_t o = ov.back(), u = un.back();
+        ov.pop_back();
+        un.pop_back();
+        p.g_accept[o] -= 1 - p.g_accept[u];
+        p.g_alias[u] = o;
+        if(p.g_accept[o] > 1) ov.push_back(o);
+        else if(p.g_accept[o] < 1) un.push_back(o);
+    }
+    for(auto i : ov) p.g_accept[i] = 1;
+    for(auto i : un) p.g_accept[i] = 1;
+}
+
+// GaussianFilter::sample (gaussian.cpp:54-66) with CAliasTable::sample
+// (utils/cmath/calias.cpp:15-21): cell by u0, position in the cell by u1, u2
+static void gaussian_sample(const Params &p, float u0, float u1, float u2, float &fx, float &fy)
+{
+    const uint32_t n = 256u * 256u;
+    const float nu = (float)n * u0;
+    const uint32_t k = std::min((uint32_t)nu, n - 1);
+    const uint32_t cell = (nu - (float)k) <= p.g_accept[k] ? k : p.g_alias[k];
+    const float w = 2 * p.radius / 256;
+    const float x0 = -p.radius + w * (float)(cell % 256u), x1 = -p.radius + w * (float)(cell % 256u + 1);
+    const float y0 = -p.radius + w * (float)(cell / 256u), y1 = -p.radius + w * (float)(cell / 256u + 1);
+    fx = x0 * (1.0f - u1) + x1 * u1;
+    fy = y0 * (1.0f - u2) + y1 * u2;
+}
 
 static inline void splat(std::vector<double> &film, uint32_t pixel, Vec v)
 {
@@ -1215,16 +1407,32 @@ static inline void splat(std::vector<double> &film, uint32_t pixel, Vec v)
 // bounce the trace programs (trace.cpp:38-78), shade_kernel
 // (shade.cpp:29-236) and the shadow programs (shadow.cpp:24-76).
 static void trace_path(const Scene &s, const Params &p, uint64_t path_index, std::vector<double> &film,
-                       uint64_t &ext_rays, uint64_t &shadow_rays)
+                       std::vector<double> *aov, uint64_t &ext_rays, uint64_t &shadow_rays)
 {
     const uint64_t pixels = (uint64_t)p.w * (uint64_t)p.h;
-    const uint32_t pixel = (uint32_t)(path_index % pixels);
-    const int32_t sample = (int32_t)(path_index / pixels);
+    // generate.cpp:46-62: uniform or tile path order
+    const uint32_t pixel = p.tile ? (uint32_t)(path_index / (uint64_t)p.spp) : (uint32_t)(path_index % pixels);
+    const int32_t sample = p.tile ? (int32_t)(path_index % (uint64_t)p.spp) : (int32_t)(path_index / pixels);
     const uint32_t px = pixel % (uint32_t)p.w, py = pixel / (uint32_t)p.w;
-    Halton hs;
-    hs.init(p.w, p.h, px, py, sample);
-    const float fx = hs.get1d() - 0.5f; // BoxFilter::sample, film_filter/box.cpp:5-9
-    const float fy = hs.get1d() - 0.5f;
+    Smp hs;
+    hs.indep = p.indep;
+    if(p.indep)
+        hs.p.init(px, py, sample);
+    else
+        hs.h.init(p.w, p.h, px, py, sample);
+    float fx, fy;
+    if(p.gaussian)
+    {
+        const float u0 = hs.get1d();
+        const float u1 = hs.get1d();
+        const float u2 = hs.get1d();
+        gaussian_sample(p, u0, u1, u2, fx, fy);
+    }
+    else
+    {
+        fx = hs.get1d() - 0.5f; // BoxFilter::sample, film_filter/box.cpp:5-9
+        fy = hs.get1d() - 0.5f;
+    }
     const float film_x = ((float)px + 0.5f + fx) / (float)p.w;
     const float film_y = ((float)py + 0.5f + fy) / (float)p.h;
     (void)hs.get1d();
@@ -1313,6 +1521,19 @@ static void trace_path(const Scene &s, const Params &p, uint64_t path_index, std
         shd.albedo = mk(m.color[0], m.color[1], m.color[2]);
         if(shd.disney)
             shd.d = dis_make(m);
+        if(aov && depth == 0)
+        {
+            // first-hit g-buffer: albedo = material colour, normal = shading z
+            // (shade.cpp:133-139,196-204; aggregate.cpp:211-219)
+            const Vec gb[2] = { shd.albedo, shd.sh.z };
+            for(int k = 0; k < 2; ++k)
+                if(finite3(gb[k]))
+                {
+                    (*aov)[6 * (size_t)pixel + 3 * k + 0] += (double)gb[k].x;
+                    (*aov)[6 * (size_t)pixel + 3 * k + 1] += (double)gb[k].y;
+                    (*aov)[6 * (size_t)pixel + 3 * k + 2] += (double)gb[k].z;
+                }
+        }
         float s0, s1, s2;
         hs.get3d(s0, s1, s2);
         const BS bs = shd_sample(shd, neg(d), s0, s1, s2);
@@ -1397,8 +1618,11 @@ void oracle_trace(void *sp, const btrc_ray *rays, int64_t n, int32_t mode, int32
 // Renders paths [first, first + count) of the uniform path order and adds
 // their radiance to film (3 doubles per pixel, not normalised). Returns the
 // number of extension and shadow rays traced through ext/shadow.
-void oracle_render(void *sp, const btrc_camera_desc *cam, const btrc_render_params *rp, uint64_t first,
-                   uint64_t count, double *film_out, int32_t threads, uint64_t *ext_out, uint64_t *shadow_out)
+// filter may be NULL (box); aov_out (6 doubles per pixel: albedo, normal
+// sums of the first hits) may be NULL.
+void oracle_render(void *sp, const btrc_camera_desc *cam, const btrc_render_params *rp,
+                   const btrc_filter_desc *filter, uint64_t first, uint64_t count, double *film_out, double *aov_out,
+                   int32_t threads, uint64_t *ext_out, uint64_t *shadow_out)
 {
     const orc::Scene &s = *static_cast<orc::Scene *>(sp);
     orc::Params p;
@@ -1409,6 +1633,15 @@ void oracle_render(void *sp, const btrc_camera_desc *cam, const btrc_render_para
     p.max_depth = rp->max_depth;
     p.rr_thr = rp->rr_threshold;
     p.rr_p = rp->rr_cont_prob;
+    p.tile = rp->tile != 0;
+    p.indep = rp->sampler == BTRC_SAMPLER_INDEPENDENT;
+    if(filter && filter->type == BTRC_FILTER_GAUSSIAN)
+    {
+        p.gaussian = true;
+        p.radius = filter->radius;
+        p.alpha = filter->alpha;
+        orc::gaussian_table(p);
+    }
     // PinholeCamera::commit, camera/pinhole.cpp:55-77
     {
         using namespace orc;
@@ -1426,13 +1659,14 @@ void oracle_render(void *sp, const btrc_camera_desc *cam, const btrc_render_para
     const size_t pixels = (size_t)p.w * (size_t)p.h;
     threads = std::max(1, threads);
     std::vector<std::vector<double>> films(threads, std::vector<double>(3 * pixels, 0.0));
+    std::vector<std::vector<double>> aovs(aov_out ? threads : 0, std::vector<double>(6 * pixels, 0.0));
     std::vector<uint64_t> ext(threads, 0), shd(threads, 0);
     std::vector<std::thread> pool;
     for(int t = 0; t < threads; ++t)
     {
         pool.emplace_back([&, t] {
             for(uint64_t i = (uint64_t)t; i < count; i += (uint64_t)threads)
-                orc::trace_path(s, p, first + i, films[t], ext[t], shd[t]);
+                orc::trace_path(s, p, first + i, films[t], aov_out ? &aovs[t] : nullptr, ext[t], shd[t]);
         });
     }
     for(auto &th : pool)
@@ -1442,6 +1676,9 @@ void oracle_render(void *sp, const btrc_camera_desc *cam, const btrc_render_para
     {
         for(size_t k = 0; k < 3 * pixels; ++k)
             film_out[k] += films[t][k];
+        if(aov_out)
+            for(size_t k = 0; k < 6 * pixels; ++k)
+                aov_out[k] += aovs[t][k];
         e += ext[t];
         sh += shd[t];
     }
@@ -1459,6 +1696,44 @@ void oracle_halton(int32_t w, int32_t h, const uint32_t *px, const uint32_t *py,
         for(int32_t k = 0; k < dims; ++k)
             out[(size_t)i * dims + k] = hs.get1d();
     }
+}
+
+void oracle_independent(const uint32_t *px, const uint32_t *py, const int32_t *si, int32_t n, int32_t dims,
+                        float *out)
+{
+    for(int32_t i = 0; i < n; ++i)
+    {
+        orc::Indep smp;
+        smp.init(px[i], py[i], si[i]);
+        for(int32_t k = 0; k < dims; ++k)
+            out[(size_t)i * dims + k] = smp.get1d();
+    }
+}
+
+void oracle_gaussian_filter(float radius, float alpha, const float *u, int32_t n, float *out)
+{
+    orc::Params p{};
+    p.gaussian = true;
+    p.radius = radius;
+    p.alpha = alpha;
+    orc::gaussian_table(p);
+    for(int32_t i = 0; i < n; ++i)
+        orc::gaussian_sample(p, u[3 * i], u[3 * i + 1], u[3 * i + 2], out[2 * i], out[2 * i + 1]);
+}
+
+uint64_t oracle_murmur64a(const uint8_t *key, uint64_t len, uint64_t seed)
+{
+    return orc::murmur64a(key, (size_t)len, seed);
+}
+
+// n outputs of PCG32 seeded (seq, offset) after jumping `skip` numbers ahead
+void oracle_pcg32(uint64_t seq, uint64_t offset, uint64_t skip, int32_t n, uint32_t *out)
+{
+    orc::Pcg r;
+    r.seed(seq, offset);
+    r.advance(skip);
+    for(int32_t i = 0; i < n; ++i)
+        out[i] = r.next();
 }
 
 void oracle_detmath(int32_t fn, const float *x, const float *y, int32_t n, float *out)

@@ -32,8 +32,13 @@ def lib():
         L.oracle_scene_bvh_nodes.restype = C.c_int64
         L.oracle_scene_bvh_nodes.argtypes = [vp]
         L.oracle_trace.argtypes = [vp, vp, C.c_int64, C.c_int32, C.c_int32, C.c_int32, vp, vp, C.c_int32]
-        L.oracle_render.argtypes = [vp, P(_abi.CameraDesc), P(_abi.RenderParams), C.c_uint64, C.c_uint64, vp,
-                                    C.c_int32, P(C.c_uint64), P(C.c_uint64)]
+        L.oracle_render.argtypes = [vp, P(_abi.CameraDesc), P(_abi.RenderParams), P(_abi.FilterDesc), C.c_uint64,
+                                    C.c_uint64, vp, vp, C.c_int32, P(C.c_uint64), P(C.c_uint64)]
+        L.oracle_independent.argtypes = [vp, vp, vp, C.c_int32, C.c_int32, vp]
+        L.oracle_gaussian_filter.argtypes = [C.c_float, C.c_float, vp, C.c_int32, vp]
+        L.oracle_murmur64a.restype = C.c_uint64
+        L.oracle_murmur64a.argtypes = [vp, C.c_uint64, C.c_uint64]
+        L.oracle_pcg32.argtypes = [C.c_uint64, C.c_uint64, C.c_uint64, C.c_int32, vp]
         L.oracle_halton.argtypes = [C.c_int32, C.c_int32, vp, vp, vp, C.c_int32, C.c_int32, vp]
         L.oracle_detmath.argtypes = [C.c_int32, vp, vp, C.c_int32, vp]
         L.oracle_bsdf_sample.argtypes = [P(_abi.MaterialDesc), vp, vp, C.c_int32, vp, vp, vp]
@@ -90,18 +95,57 @@ class OracleScene:
                            None, threads)
         return hits
 
-    def render(self, camera, params, first=0, count=None, threads=8):
+    def render(self, camera, params, first=0, count=None, threads=8, filt=None, aov=False):
         """Radiance sums (H, W, 3) float64 of paths [first, first + count) and
-        the (extension, shadow) ray counts."""
+        the (extension, shadow) ray counts; with aov=True also the first-hit
+        (albedo, normal) sums, each (H, W, 3). filt: None (box) or
+        ("gaussian", radius, alpha)."""
         w, h = params.width, params.height
         if count is None:
             count = w * h * params.spp
         film = np.zeros((h, w, 3), dtype=np.float64)
+        aovs = np.zeros((h, w, 6), dtype=np.float64) if aov else None
         cam = camera.to_desc(w, h)
+        fd = None
+        if filt is not None and filt[0] == "gaussian":
+            fd = _abi.FilterDesc(_abi.FILTER_GAUSSIAN, float(filt[1]), float(filt[2]))
         e, s = C.c_uint64(), C.c_uint64()
-        lib().oracle_render(self.h, C.byref(cam), C.byref(params), first, count, film.ctypes.data, threads,
+        lib().oracle_render(self.h, C.byref(cam), C.byref(params), C.byref(fd) if fd is not None else None, first,
+                            count, film.ctypes.data, aovs.ctypes.data if aov else None, threads,
                             C.byref(e), C.byref(s))
+        if aov:
+            return film, e.value, s.value, aovs[..., :3], aovs[..., 3:]
         return film, e.value, s.value
+
+
+def independent(px, py, sample_index, dims):
+    """first `dims` numbers of the independent sampler per (px, py, sample)"""
+    px = np.ascontiguousarray(px, dtype=np.uint32)
+    py = np.ascontiguousarray(py, dtype=np.uint32)
+    si = np.ascontiguousarray(sample_index, dtype=np.int32)
+    out = np.empty((len(px), dims), dtype=np.float32)
+    lib().oracle_independent(px.ctypes.data, py.ctypes.data, si.ctypes.data, len(px), dims, out.ctypes.data)
+    return out
+
+
+def gaussian_filter(radius, alpha, u):
+    """GaussianFilter::sample offsets (n, 2) for uniforms u (n, 3)"""
+    u = np.ascontiguousarray(u, dtype=np.float32)
+    out = np.empty((len(u), 2), dtype=np.float32)
+    lib().oracle_gaussian_filter(radius, alpha, u.ctypes.data, len(u), out.ctypes.data)
+    return out
+
+
+def murmur64a(key, seed):
+    key = np.frombuffer(bytes(key), dtype=np.uint8)
+    buf = np.ascontiguousarray(key) if len(key) else np.zeros(1, np.uint8)
+    return int(lib().oracle_murmur64a(buf.ctypes.data, len(key), seed))
+
+
+def pcg32(seq, offset, n, skip=0):
+    out = np.empty(n, dtype=np.uint32)
+    lib().oracle_pcg32(seq, offset, skip, n, out.ctypes.data)
+    return out
 
 
 def halton(width, height, px, py, sample_index, dims):
