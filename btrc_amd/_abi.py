@@ -10,6 +10,9 @@ BTRC_ERR_STOPPED = -5
 
 MATERIAL_DIFFUSE = 0
 MATERIAL_DISNEY = 1
+MATERIAL_INVISIBLE = 2
+MEDIUM_HOMOGENEOUS = 0
+MEDIUM_HETEROGENEOUS = 1
 ENVIR_NONE = 0
 ENVIR_GRADIENT_SKY = 1
 SAMPLER_HALTON = 0
@@ -46,6 +49,12 @@ class InstanceDesc(C.Structure):
                 ("inner_medium", C.c_int32), ("outer_medium", C.c_int32)]
 
 
+class MediumDesc(C.Structure):
+    _fields_ = [("type", C.c_int32), ("priority", C.c_float), ("sigma_t", C.c_float), ("albedo", F3),
+                ("g", C.c_float), ("grid_size", C.c_int32 * 3), ("sigma_t_grid", C.POINTER(C.c_float)),
+                ("albedo_grid", C.POINTER(C.c_float)), ("has_transform", C.c_int32), ("world_to_local", F16)]
+
+
 class EnvirLightDesc(C.Structure):
     _fields_ = [("type", C.c_int32), ("up", F3), ("lower", F3), ("upper", F3)]
 
@@ -54,7 +63,7 @@ class SceneDesc(C.Structure):
     _fields_ = [("geometry_count", C.c_int32), ("geometries", C.POINTER(GeometryDesc)),
                 ("material_count", C.c_int32), ("materials", C.POINTER(MaterialDesc)),
                 ("instance_count", C.c_int32), ("instances", C.POINTER(InstanceDesc)),
-                ("envir_light", EnvirLightDesc)]
+                ("envir_light", EnvirLightDesc), ("medium_count", C.c_int32), ("media", C.POINTER(MediumDesc))]
 
 
 class CameraDesc(C.Structure):
@@ -77,7 +86,7 @@ class FilterDesc(C.Structure):
 class RenderStats(C.Structure):
     _fields_ = [("seconds", C.c_double), ("extension_rays", C.c_uint64), ("shadow_rays", C.c_uint64),
                 ("paths", C.c_uint64), ("iterations", C.c_uint64),
-                ("stage_ms", C.c_double * 4), ("stage_launches", C.c_uint64 * 4)]
+                ("stage_ms", C.c_double * 5), ("stage_launches", C.c_uint64 * 5)]
 
 
 class Tuning(C.Structure):

@@ -21,6 +21,7 @@ enum : int32_t
 {
     MATERIAL_DIFFUSE = 0,
     MATERIAL_DISNEY = 1,
+    MATERIAL_INVISIBLE = 2, // InvisibleSurface, builtin/material/invisible.cpp:6-57
 };
 
 // Per-material constants. Textures are constant per material in this path
@@ -675,7 +676,7 @@ struct Shader
     ShaderFrame frame;
     int32_t type;
     bool shadow_terminator;
-    V3 albedo;  // Lambert albedo
+    V3 albedo;  // Lambert albedo; Disney base colour; the g-buffer albedo
     Disney disney;
 };
 
@@ -686,7 +687,7 @@ BTRC_HD Shader make_shader(const MaterialRec &m, const Frame &geometry_frame, V3
     s.frame.shading = rotate_to_new_z(geometry_frame, interp_z);
     s.type = m.type;
     s.shadow_terminator = m.shadow_terminator != 0;
-    s.albedo = v3(m.color[0], m.color[1], m.color[2]);
+    s.albedo = m.type == MATERIAL_INVISIBLE ? v3s(1.0f) : v3(m.color[0], m.color[1], m.color[2]);
     if(m.type == MATERIAL_DISNEY)
         s.disney = disney_make(m);
     return s;
@@ -706,9 +707,25 @@ BTRC_HD float component_pdf(const Shader &s, V3 lwi, V3 lwo)
     return lambert_pdf(lwi, lwo);
 }
 
-// BSDFAggregate::sample, aggregate.cpp:32-103
+// g-buffer normal: the shading normal; an invisible surface reports its
+// geometric normal (invisible.cpp:46-55)
+BTRC_HD V3 shader_normal(const Shader &s)
+{
+    return s.type == MATERIAL_INVISIBLE ? s.frame.geometry.z : s.frame.shading.z;
+}
+
+// BSDFAggregate::sample, aggregate.cpp:32-103; InvisibleShader passes the
+// ray straight through as a delta lobe (invisible.cpp:13-26)
 BTRC_HD void shader_sample(const Shader &s, V3 wo, float s0, float s1, float s2, BSDFSample &out)
 {
+    if(s.type == MATERIAL_INVISIBLE)
+    {
+        out.bsdf = v3s(1.0f) / fabs_(dot(s.frame.geometry.z, normalize(wo)));
+        out.dir = -wo;
+        out.pdf = 1.0f;
+        out.is_delta = true;
+        return;
+    }
     const ShaderFrame frame = frame_flip_for_black_fringes(s.frame, wo);
     const V3 lwo = normalize(global_to_local(frame.shading, wo));
     BSDFSample comp;
@@ -733,6 +750,8 @@ BTRC_HD void shader_sample(const Shader &s, V3 wo, float s0, float s1, float s2,
 // BSDFAggregate::eval, aggregate.cpp:189-202
 BTRC_HD V3 shader_eval(const Shader &s, V3 wi, V3 wo)
 {
+    if(s.type == MATERIAL_INVISIBLE)
+        return v3s(0.0f);
     const ShaderFrame frame = frame_flip_for_black_fringes(s.frame, wo);
     const V3 lwi = normalize(global_to_local(frame.shading, wi));
     const V3 lwo = normalize(global_to_local(frame.shading, wo));
@@ -745,6 +764,8 @@ BTRC_HD V3 shader_eval(const Shader &s, V3 wi, V3 wo)
 // BSDFAggregate::pdf, aggregate.cpp:204-214
 BTRC_HD float shader_pdf(const Shader &s, V3 wi, V3 wo)
 {
+    if(s.type == MATERIAL_INVISIBLE)
+        return 0.0f;
     const ShaderFrame frame = frame_flip_for_black_fringes(s.frame, wo);
     const V3 lwi = normalize(global_to_local(frame.shading, wi));
     const V3 lwo = normalize(global_to_local(frame.shading, wo));

@@ -7,6 +7,7 @@
 #pragma once
 
 #include "material.h"
+#include "medium.h"
 
 #if !defined(__HIPCC__)
 struct float4 { float x, y, z, w; };
@@ -83,7 +84,12 @@ struct SceneView
     int32_t light_count;
     int32_t envir_light_index; // -1 without an environment light
     int32_t instance_count;
-    int32_t pad;
+    // media, by descending priority (core/scene.cpp:144-161): ids 0..n-1;
+    // id n is the volume-primitive medium (no volume primitives here, so
+    // vacuum) and id n + 1 stands for "no medium"
+    int32_t medium_count;
+    const MediumRec *media;
+    const float *voxels;
 };
 
 struct SurfacePoint
@@ -223,8 +229,11 @@ struct ShadowSample
     V3 li;
 };
 
-BTRC_HD ShadowSample sample_surface_light_li(
-    const SceneView &sc, V3 ref_pos, V3 ref_nor, float l0, float l1, float l2, float lsel)
+// offset_origin: surface points move the origin off the surface
+// (sample_surface_light_li, helper.cpp:127-142); medium points keep it
+// (sample_medium_light_li, helper.cpp:115-125)
+BTRC_HD ShadowSample sample_light_li(
+    const SceneView &sc, V3 ref_pos, V3 ref_nor, bool offset_origin, float l0, float l1, float l2, float lsel)
 {
     ShadowSample r;
     float select_pdf;
@@ -243,7 +252,7 @@ BTRC_HD ShadowSample sample_surface_light_li(
         const LightSample s = mesh_light_sample_li(sc, l, ref_pos, l0, l1, l2);
         const V3 diff = s.position - ref_pos;
         const V3 shadow_dst = intersection_offset(s.position, s.normal, -diff);
-        r.o = intersection_offset(ref_pos, ref_nor, diff);
+        r.o = offset_origin ? intersection_offset(ref_pos, ref_nor, diff) : ref_pos;
         r.d = shadow_dst - r.o;
         r.t1 = 1.0f;
         r.li = s.radiance;
@@ -253,13 +262,19 @@ BTRC_HD ShadowSample sample_surface_light_li(
     {
         // GradientSky::sample_li, gradient_sky.cpp:27-35
         const V3 dir = sample_sphere_uniform(l0, l1);
-        r.o = intersection_offset(ref_pos, ref_nor, dir);
+        r.o = offset_origin ? intersection_offset(ref_pos, ref_nor, dir) : ref_pos;
         r.d = dir;
         r.t1 = MAX_FLOAT;
         r.li = sky_eval_le(l, dir);
         r.light_pdf = select_pdf * pdf_sample_sphere_uniform();
     }
     return r;
+}
+
+BTRC_HD ShadowSample sample_surface_light_li(
+    const SceneView &sc, V3 ref_pos, V3 ref_nor, float l0, float l1, float l2, float lsel)
+{
+    return sample_light_li(sc, ref_pos, ref_nor, true, l0, l1, l2, lsel);
 }
 
 // eval_miss_le, helper.cpp:144-172

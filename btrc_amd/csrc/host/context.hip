@@ -180,6 +180,9 @@ struct btrc_ctx
     DevBuf<LightRec> lights;
     DevBuf<float4> gx_u_a, gy_u_ba, gz_u_ca, sn_v_a, sn_v_ba, sn_v_ca, light_pos;
     DevBuf<AliasUnit> alias;
+    DevBuf<MediumRec> media;
+    DevBuf<float> voxels;
+    int32_t medium_count = 0;
     int32_t light_count = 0;
     int32_t envir_light_index = -1;
     int32_t instance_count = 0;
@@ -198,6 +201,7 @@ struct btrc_ctx
     DevBuf<uint32_t> hit_flag;
     DevBuf<float4> hit_tuv;
     DevBuf<float4> sh_o, sh_d, sh_beta;
+    DevBuf<uint2> sh_seed;
     DevBuf<WaveCounters> counters;
     WaveCounters *host_counters = nullptr; // pinned
     DevBuf<float> film;
@@ -242,7 +246,9 @@ struct btrc_ctx
         v.light_count = light_count;
         v.envir_light_index = envir_light_index;
         v.instance_count = instance_count;
-        v.pad = 0;
+        v.medium_count = medium_count;
+        v.media = media.ptr;
+        v.voxels = voxels.ptr;
         return v;
     }
     int persistent_grid(int threads_per_block, int64_t work) const
@@ -466,7 +472,7 @@ void upload_scene(btrc_ctx *ctx, const btrc_scene_desc *sd)
     for(int32_t i = 0; i < sd->material_count; ++i)
     {
         const btrc_material_desc &m = sd->materials[i];
-        if(m.type != BTRC_MATERIAL_DIFFUSE && m.type != BTRC_MATERIAL_DISNEY)
+        if(m.type != BTRC_MATERIAL_DIFFUSE && m.type != BTRC_MATERIAL_DISNEY && m.type != BTRC_MATERIAL_INVISIBLE)
             throw InvalidArg("material " + std::to_string(i) + ": unknown type");
         MaterialRec r{};
         r.type = m.type;
@@ -479,6 +485,96 @@ void upload_scene(btrc_ctx *ctx, const btrc_scene_desc *sd)
         r.transmission = m.transmission; r.transmission_roughness = m.transmission_roughness; r.ior = m.ior;
         mats[i] = r;
     }
+
+    // media: ids by descending priority (ties keep the input order); id R is
+    // the volume-primitive medium, id R + 1 means none
+    if(sd->medium_count < 0 || (sd->medium_count > 0 && !sd->media))
+        throw InvalidArg("bad medium list");
+    const int32_t R = sd->medium_count;
+    std::vector<int32_t> order(R);
+    for(int32_t i = 0; i < R; ++i)
+        order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) {
+        return sd->media[a].priority > sd->media[b].priority;
+    });
+    std::vector<uint32_t> medium_id(R);
+    std::vector<MediumRec> media(R);
+    std::vector<float> voxels;
+    for(int32_t rank = 0; rank < R; ++rank)
+    {
+        const int32_t i = order[rank];
+        const btrc_medium_desc &md = sd->media[i];
+        const std::string name = "medium " + std::to_string(i);
+        MediumRec m{};
+        m.type = md.type;
+        m.g = md.g;
+        if(!(std::fabs(md.g) < 1.0f))
+            throw InvalidArg(name + ": g must lie in (-1, 1)");
+        for(int k = 0; k < 3; ++k)
+        {
+            if(!std::isfinite(md.albedo[k]))
+                throw InvalidArg(name + ": albedo must be finite");
+            m.albedo[k] = md.albedo[k];
+        }
+        if(md.type == BTRC_MEDIUM_HOMOGENEOUS)
+        {
+            if(!(md.sigma_t > 0.0f) || !std::isfinite(md.sigma_t))
+                throw InvalidArg(name + ": sigma_t must be positive and finite");
+            m.sigma_t = md.sigma_t;
+        }
+        else if(md.type == BTRC_MEDIUM_HETEROGENEOUS)
+        {
+            const int32_t nx = md.grid_size[0], ny = md.grid_size[1], nz = md.grid_size[2];
+            if(nx <= 0 || ny <= 0 || nz <= 0 || !md.sigma_t_grid)
+                throw InvalidArg(name + ": heterogeneous media need a density grid");
+            const size_t nv = (size_t)nx * ny * nz;
+            if(voxels.size() + nv * (md.albedo_grid ? 4 : 1) > 0xffffffffull)
+                throw InvalidArg(name + ": voxel grids too large");
+            m.nx = nx;
+            m.ny = ny;
+            m.nz = nz;
+            float mx = -MAX_FLOAT;
+            m.density_offset = (uint32_t)voxels.size();
+            for(size_t v = 0; v < nv; ++v)
+            {
+                const float x = md.sigma_t_grid[v];
+                if(!std::isfinite(x))
+                    throw InvalidArg(name + ": density grid holds a non-finite value");
+                mx = std::max(mx, x);
+                voxels.push_back(x);
+            }
+            // HetergeneousMedium::sample, hetergeneous.cpp:34-35
+            m.inv_max_density = 1.0f / std::max(mx, 0.001f);
+            if(md.albedo_grid)
+            {
+                m.albedo_grid = 1;
+                m.albedo_offset = (uint32_t)voxels.size();
+                for(size_t v = 0; v < 3 * nv; ++v)
+                {
+                    if(!std::isfinite(md.albedo_grid[v]))
+                        throw InvalidArg(name + ": albedo grid holds a non-finite value");
+                    voxels.push_back(md.albedo_grid[v]);
+                }
+            }
+        }
+        else
+            throw InvalidArg(name + ": unknown type");
+        if(md.has_transform)
+        {
+            m.has_xform = 1;
+            for(int k = 0; k < 12; ++k)
+                m.world_to_local[k] = md.world_to_local[k];
+        }
+        media[rank] = m;
+        medium_id[i] = (uint32_t)rank;
+    }
+    auto resolve_medium = [&](int32_t idx, int32_t inst) -> uint32_t {
+        if(idx < 0)
+            return (uint32_t)R + 1;
+        if(idx >= R)
+            throw InvalidArg("instance " + std::to_string(inst) + ": bad medium index");
+        return medium_id[idx];
+    };
 
     // instances, lights and the flattened world-space triangle list
     std::vector<InstanceRec> insts(sd->instance_count);
@@ -493,14 +589,12 @@ void upload_scene(btrc_ctx *ctx, const btrc_scene_desc *sd)
             throw InvalidArg("instance " + std::to_string(i) + ": bad geometry index");
         if(d.material < 0 || d.material >= sd->material_count)
             throw InvalidArg("instance " + std::to_string(i) + ": bad material index");
-        if(d.inner_medium >= 0 || d.outer_medium >= 0)
-            throw InvalidArg("instance " + std::to_string(i) + ": participating media are not supported by this build");
         InstanceRec r{};
         r.geometry_id = d.geometry;
         r.material_id = d.material;
         r.light_id = -1;
-        r.inner_medium = 0;
-        r.outer_medium = 0;
+        r.inner_medium = resolve_medium(d.inner_medium, i);
+        r.outer_medium = resolve_medium(d.outer_medium, i);
         r.flag = 0;
         r.prim_base = prim_base;
         r.xf = make_xform(d.local_to_world, d.world_to_local);
@@ -584,8 +678,11 @@ void upload_scene(btrc_ctx *ctx, const btrc_scene_desc *sd)
     ctx->sn_v_ca.upload(sca.data(), sca.size(), s);
     ctx->light_pos.upload(lpos.data(), lpos.size(), s);
     ctx->alias.upload(alias_units.data(), alias_units.size(), s);
+    ctx->media.upload(media.data(), media.size(), s);
+    ctx->voxels.upload(voxels.data(), voxels.size(), s);
     BTRC_HIP_CHECK(hipStreamSynchronize(s));
 
+    ctx->medium_count = R;
     ctx->light_count = (int32_t)lights.size();
     ctx->envir_light_index = envir_index;
     ctx->instance_count = sd->instance_count;
@@ -709,6 +806,9 @@ void render(btrc_ctx *ctx, const btrc_render_params *p, btrc_render_stats *stats
         ctx->sh_beta.alloc(P);
         ctx->pool_size = P;
     }
+    const bool media = ctx->medium_count > 0;
+    if(media)
+        ctx->sh_seed.alloc(P);
     ctx->counters.alloc(1);
     if(!ctx->host_counters)
         BTRC_HIP_CHECK(hipHostMalloc((void **)&ctx->host_counters, sizeof(WaveCounters), hipHostMallocDefault));
@@ -749,7 +849,7 @@ void render(btrc_ctx *ctx, const btrc_render_params *p, btrc_render_stats *stats
     rc.rr_threshold = p->rr_threshold;
     rc.rr_cont_prob = p->rr_cont_prob;
     rc.state_count = p->state_count;
-    rc.vol_prim_medium = 0;
+    rc.vol_prim_medium = (uint32_t)ctx->medium_count;
     rc.film_fp64 = p->film_fp64;
     rc.tile = p->tile != 0;
     rc.halton = make_halton_consts(p->width, p->height);
@@ -761,7 +861,7 @@ void render(btrc_ctx *ctx, const btrc_render_params *p, btrc_render_stats *stats
     FilmView fv{ ctx->film.ptr, p->film_fp64 ? ctx->film64.ptr : nullptr, p->albedo ? ctx->film_albedo.ptr : nullptr,
                  p->normal ? ctx->film_normal.ptr : nullptr };
     HitSOA hit{ ctx->hit_flag.ptr, ctx->hit_tuv.ptr };
-    ShadowSOA sh{ ctx->sh_o.ptr, ctx->sh_d.ptr, ctx->sh_beta.ptr };
+    ShadowSOA sh{ ctx->sh_o.ptr, ctx->sh_d.ptr, ctx->sh_beta.ptr, media ? ctx->sh_seed.ptr : nullptr };
 
     const int grid256 = ctx->persistent_grid(256, (int64_t)P);
     const TraceOverflow ovf = ctx->trace_overflow();
@@ -803,8 +903,10 @@ void render(btrc_ctx *ctx, const btrc_render_params *p, btrc_render_stats *stats
             }
             captured += n;
         }
+        if(media)
+            timer.run(STAGE_MEDIUM, [&] { launch_medium(s, grid256, p->sampler, rc, sv, ctx->counters.ptr, path, hit, next, sh, fv); });
         timer.run(STAGE_SHADE, [&] { launch_shade(s, grid256, p->sampler, rc, sv, ctx->counters.ptr, path, hit, next, sh, fv); });
-        timer.run(STAGE_SHADOW, [&] { launch_shadow(s, ctx->trace_grid, ctx->tuning.any_postpone_leaves != 0, ts, ctx->counters.ptr, sh, fv, ovf); });
+        timer.run(STAGE_SHADOW, [&] { launch_shadow(s, ctx->trace_grid, ctx->tuning.any_postpone_leaves != 0, media, ts, sv, ctx->counters.ptr, sh, fv, ovf); });
         launch_end_iteration(s, ctx->counters.ptr);
         BTRC_HIP_CHECK(hipGetLastError());
         cur ^= 1;

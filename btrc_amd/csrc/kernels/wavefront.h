@@ -36,6 +36,7 @@ struct ShadowSOA
     float4 *o;        // origin.xyz, pixel index bits
     float4 *d;        // direction.xyz, t1
     float4 *beta;     // beta_li.xyz, medium id bits
+    uint2 *seed;      // transmittance stream seed (scenes with media), else null
 };
 
 constexpr uint32_t PATH_FLAG_HAS_INTERSECTION = 1u << 30; // wavefront/common.h:12
@@ -122,7 +123,8 @@ enum : int
     STAGE_TRACE = 1,
     STAGE_SHADE = 2,
     STAGE_SHADOW = 3,
-    STAGE_COUNT = 4,
+    STAGE_MEDIUM = 4,
+    STAGE_COUNT = 5,
 };
 
 void launch_plan(hipStream_t s, WaveCounters *c, int32_t state_count);
@@ -132,8 +134,10 @@ void launch_trace_extension(hipStream_t s, int grid, bool postpone, const TraceS
                             PathSOA path, HitSOA hit, const TraceOverflow &ovf);
 void launch_shade(hipStream_t s, int grid, int sampler, const RenderConsts &rc, const SceneView &sv,
                   WaveCounters *c, PathSOA path, HitSOA hit, PathSOA next, ShadowSOA shadow, FilmView film);
-void launch_shadow(hipStream_t s, int grid, bool postpone, const TraceScene &ts, WaveCounters *c,
-                   ShadowSOA shadow, FilmView film, const TraceOverflow &ovf);
+void launch_medium(hipStream_t s, int grid, int sampler, const RenderConsts &rc, const SceneView &sv,
+                   WaveCounters *c, PathSOA path, HitSOA hit, PathSOA next, ShadowSOA shadow, FilmView film);
+void launch_shadow(hipStream_t s, int grid, bool postpone, bool media, const TraceScene &ts, const SceneView &sv,
+                   WaveCounters *c, ShadowSOA shadow, FilmView film, const TraceOverflow &ovf);
 void launch_end_iteration(hipStream_t s, WaveCounters *c);
 void launch_resolve(hipStream_t s, int mode, const float *value, int32_t pixels, float inv_weight, float4 *out);
 

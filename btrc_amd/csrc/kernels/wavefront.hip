@@ -121,6 +121,27 @@ template<> struct PathSampler<SAMPLER_INDEPENDENT>
     }
 };
 
+// Seed of a shadow ray's transmittance stream: a hash of its path's sampler
+// state when the ray is made, so ratio tracking along the ray is a function
+// of the path alone.
+__device__ __forceinline__ uint2 shadow_seed(uint4 s)
+{
+    const uint64_t h = mix_bits((((uint64_t)s.x << 32) | s.y) ^ mix_bits(((uint64_t)s.z << 32) | s.w));
+    return make_uint2((uint32_t)h, (uint32_t)(h >> 32));
+}
+
+// PCG32 stream of a shadow ray's ratio tracking
+struct TrSampler
+{
+    Pcg32 rng;
+    __device__ static TrSampler start(uint2 seed)
+    {
+        const uint64_t v = (uint64_t)seed.x | ((uint64_t)seed.y << 32);
+        return TrSampler{ pcg32_seed(v, mix_bits(v)) };
+    }
+    __device__ float get1d() { return pcg32_uniform_float(rng); }
+};
+
 // ---------------------------------------------------------------- plan
 
 // GeneratePipeline::generate's refill rule (generate.cpp:140-176), evaluated
@@ -270,6 +291,7 @@ __global__ void __launch_bounds__(256) k_shade(
         // values carried to the pushes
         float4 nx_o, nx_d, nx_beta, nx_rad, nx_le, sh_o, sh_d, sh_beta;
         uint4 nx_smp;
+        uint2 sh_seed;
 
         if(i < n)
         {
@@ -353,7 +375,7 @@ __global__ void __launch_bounds__(256) k_shade(
                         if(film.albedo)
                             aov_splat(film.albedo, pixel, shader.albedo);
                         if(film.normal)
-                            aov_splat(film.normal, pixel, shader.frame.shading.z);
+                            aov_splat(film.normal, pixel, shader_normal(shader));
                     }
                     float s0, s1, s2;
                     hs.get3d(&s0, &s1, &s2);
@@ -382,6 +404,7 @@ __global__ void __launch_bounds__(256) k_shade(
                         sh_o = f4(ss.o, __uint_as_float(pixel));
                         sh_d = f4(ss.d, ss.t1);
                         sh_beta = f4(beta_li, __uint_as_float(med));
+                        sh_seed = shadow_seed(hs.pack(pixel));
                     }
 
                     if(!is_zero(bs.bsdf))
@@ -414,6 +437,8 @@ __global__ void __launch_bounds__(256) k_shade(
             shadow.o[si] = sh_o;
             shadow.d[si] = sh_d;
             shadow.beta[si] = sh_beta;
+            if(shadow.seed)
+                shadow.seed[si] = sh_seed;
         }
         const int32_t ni = wave_push(&c->next_active, push_next);
         if(push_next)
@@ -428,14 +453,172 @@ __global__ void __launch_bounds__(256) k_shade(
     }
 }
 
+// ---------------------------------------------------------------- medium
+
+// sample_medium_scattering_kernel, wavefront/medium.cpp:28-162: for every
+// path whose last segment runs through a medium, delta-track a scattering
+// point along it, weight beta_le by the segment's transmittance, and at a
+// scattering point finish the vertex here (Russian roulette, light sample
+// through the phase function, phase-sampled continuation). The shading
+// stage then only adds the emission the segment ended on.
+template<int SAMPLER>
+__global__ void __launch_bounds__(256) k_medium(
+    RenderConsts rc, SceneView sv, WaveCounters *c, PathSOA path, HitSOA hit,
+    PathSOA next, ShadowSOA shadow, FilmView film)
+{
+    const int32_t n = c->active;
+    const int32_t stride = gridDim.x * blockDim.x;
+    for(int32_t base = blockIdx.x * blockDim.x; base < n; base += stride)
+    {
+        const int32_t i = base + (int32_t)threadIdx.x;
+        bool push_next = false, push_shadow = false;
+        float4 nx_o, nx_d, nx_beta, nx_le, sh_o, sh_d, sh_beta, rad4;
+        uint4 nx_smp;
+        uint2 sh_seed;
+
+        if(i < n)
+        {
+            const uint32_t flag = hit.flag[i];
+            const float4 ro = path.ray_o[i];
+            const float4 rd = path.ray_d[i];
+            const V3 o = xyz(ro), d = xyz(rd);
+
+            // get_medium_segment, medium.cpp:214-245
+            uint32_t med;
+            V3 end;
+            if(flag & PATH_FLAG_HAS_INTERSECTION)
+            {
+                const float4 tuv = hit.tuv[i];
+                const InstanceRec &inst = sv.instances[flag & PATH_FLAG_INSTANCE_ID_MASK];
+                const V3 nor = hit_geometric_normal(sv, inst, __float_as_uint(tuv.w));
+                const uint32_t inct_med = dot(d, nor) < 0.0f ? inst.outer_medium : inst.inner_medium;
+                const uint32_t ray_med = __float_as_uint(ro.w);
+                med = inct_med < ray_med ? inct_med : ray_med;
+                end = intersection_offset(o + tuv.x * d, nor, -d);
+            }
+            else
+            {
+                // the volume-primitive medium: without volume primitives a vacuum
+                med = (uint32_t)sv.medium_count;
+                end = o;
+            }
+            path.ray_o[i] = f4(o, __uint_as_float(med));
+
+            if(med < (uint32_t)sv.medium_count)
+            {
+                const MediumRec &m = sv.media[med];
+                const uint4 smp = path.smp[i];
+                const uint32_t pixel = smp.x;
+                PathSampler<SAMPLER> hs = PathSampler<SAMPLER>::load(rc, smp);
+                const MediumSample ms = medium_sample(m, sv.voxels, o, end, hs);
+                const V3 tr = medium_tr(m, sv.voxels, o, end, hs);
+                const float4 le4 = path.bsdf_le[i];
+                path.bsdf_le[i] = f4(xyz(le4) * tr, le4.w);
+
+                if(!ms.scattered)
+                    path.smp[i] = hs.pack(pixel);
+                else
+                {
+                    hit.flag[i] = flag | PATH_FLAG_HAS_SCATTERING;
+                    const float4 beta_depth = path.beta[i];
+                    V3 beta = xyz(beta_depth); // medium throughput is one
+                    const int32_t depth = __float_as_int(beta_depth.w);
+                    rad4 = path.rad[i];
+                    const V3 path_rad = xyz(rad4);
+
+                    bool rr_exit = false;
+                    if(depth >= rc.min_depth)
+                    {
+                        if(depth >= rc.max_depth)
+                            rr_exit = true;
+                        else
+                        {
+                            const float u = hs.get1d();
+                            if(luminance(beta) < rc.rr_threshold)
+                            {
+                                if(u > rc.rr_cont_prob)
+                                    rr_exit = true;
+                                else
+                                    beta = beta / rc.rr_cont_prob;
+                            }
+                        }
+                    }
+                    if(rr_exit)
+                        film_splat(film, pixel, path_rad);
+                    else
+                    {
+                        float l0, l1, l2;
+                        hs.get3d(&l0, &l1, &l2);
+                        const float lsel = hs.get1d();
+                        ShadowSample ss = sample_light_li(sv, ms.position, v3s(0.0f), false, l0, l1, l2, lsel);
+                        V3 beta_li = v3s(0.0f);
+                        if(ss.success)
+                        {
+                            const float pv = hg_pdf(ms.g, ss.d, -d);
+                            beta_li = ss.li * beta * (ms.color * pv) / (pv + ss.light_pdf);
+                        }
+                        float s0, s1, s2;
+                        hs.get3d(&s0, &s1, &s2);
+                        const PhaseSample ps = hg_sample(ms.g, ms.color, -d, s0, s1);
+                        if(ss.success)
+                        {
+                            push_shadow = true;
+                            const uint32_t smed = ss.t1 > 1.0f ? (uint32_t)sv.medium_count : med;
+                            sh_o = f4(ss.o, __uint_as_float(pixel));
+                            sh_d = f4(ss.d, ss.t1);
+                            sh_beta = f4(beta_li, __uint_as_float(smed));
+                            sh_seed = shadow_seed(hs.pack(pixel));
+                        }
+                        if(!is_zero(ps.phase))
+                        {
+                            const V3 dir = normalize(ps.dir);
+                            const V3 beta_le = beta * ps.phase;
+                            beta = beta_le / ps.pdf;
+                            push_next = true;
+                            nx_o = f4(ms.position, __uint_as_float(med));
+                            nx_d = f4(dir, MAX_FLOAT);
+                            nx_beta = f4(beta, __int_as_float(depth + 1));
+                            nx_le = f4(beta_le, ps.pdf);
+                            nx_smp = hs.pack(pixel);
+                        }
+                        else
+                            film_splat(film, pixel, path_rad);
+                    }
+                }
+            }
+        }
+
+        const int32_t si = wave_push(&c->shadow, push_shadow);
+        if(push_shadow)
+        {
+            shadow.o[si] = sh_o;
+            shadow.d[si] = sh_d;
+            shadow.beta[si] = sh_beta;
+            shadow.seed[si] = sh_seed;
+        }
+        const int32_t ni = wave_push(&c->next_active, push_next);
+        if(push_next)
+        {
+            next.ray_o[ni] = nx_o;
+            next.ray_d[ni] = nx_d;
+            next.beta[ni] = nx_beta;
+            next.rad[ni] = rad4;
+            next.bsdf_le[ni] = nx_le;
+            next.smp[ni] = nx_smp;
+        }
+    }
+}
+
 // ---------------------------------------------------------------- shadow
 
-// shadow rays in; on a miss the light sample reaches the film
-// (__miss__shadow, shadow.cpp:43-71; transmittance is 1 in vacuum)
+// shadow rays in; on a miss the light sample reaches the film, weighted by
+// the transmittance of the ray's medium (__miss__shadow, shadow.cpp:43-71)
+template<bool MEDIA>
 struct ShadowSink
 {
     ShadowSOA shadow;
     FilmView film;
+    SceneView sv;
 
     __device__ __forceinline__ void load(uint32_t i, V3 &o, V3 &d, float &tmin, float &tmax) const
     {
@@ -450,17 +633,32 @@ struct ShadowSink
     {
         if(h.gid == 0xffffffffu)
         {
+            const float4 a = shadow.o[i];
             const float4 b = shadow.beta[i];
-            film_splat(film, __float_as_uint(shadow.o[i].w), xyz(b));
+            V3 beta = xyz(b);
+            if(MEDIA)
+            {
+                // rays to the environment (t1 > 1) cross the volume-primitive
+                // medium, a vacuum here
+                const float4 dd = shadow.d[i];
+                const uint32_t med = __float_as_uint(b.w);
+                if(!(dd.w > 1.0f) && med < (uint32_t)sv.medium_count)
+                {
+                    TrSampler ts = TrSampler::start(shadow.seed[i]);
+                    const V3 o = xyz(a);
+                    beta = beta * medium_tr(sv.media[med], sv.voxels, o, o + xyz(dd) * dd.w, ts);
+                }
+            }
+            film_splat(film, __float_as_uint(a.w), beta);
         }
     }
 };
 
-template<bool POSTPONE>
+template<bool POSTPONE, bool MEDIA>
 __global__ void __launch_bounds__(TRACE_BLOCK) k_shadow(
-    TraceScene ts, WaveCounters *c, ShadowSOA shadow, FilmView film, TraceOverflow ovf)
+    TraceScene ts, SceneView sv, WaveCounters *c, ShadowSOA shadow, FilmView film, TraceOverflow ovf)
 {
-    ShadowSink sink{ shadow, film };
+    ShadowSink<MEDIA> sink{ shadow, film, sv };
     trace_persistent<TRI_WATERTIGHT, true, POSTPONE>(ts, sink, c->shadow, &c->shadow_fetch, ovf);
 }
 
@@ -613,13 +811,29 @@ void launch_shade(hipStream_t s, int grid, int sampler, const RenderConsts &rc, 
                            film);
 }
 
-void launch_shadow(hipStream_t s, int grid, bool postpone, const TraceScene &ts, WaveCounters *c,
-                   ShadowSOA shadow, FilmView film, const TraceOverflow &ovf)
+void launch_medium(hipStream_t s, int grid, int sampler, const RenderConsts &rc, const SceneView &sv,
+                   WaveCounters *c, PathSOA path, HitSOA hit, PathSOA next, ShadowSOA shadow, FilmView film)
 {
-    if(postpone)
-        hipLaunchKernelGGL(k_shadow<true>, dim3(grid), dim3(TRACE_BLOCK), 0, s, ts, c, shadow, film, ovf);
+    if(sampler == SAMPLER_INDEPENDENT)
+        hipLaunchKernelGGL(k_medium<SAMPLER_INDEPENDENT>, dim3(grid), dim3(256), 0, s, rc, sv, c, path, hit, next,
+                           shadow, film);
     else
-        hipLaunchKernelGGL(k_shadow<false>, dim3(grid), dim3(TRACE_BLOCK), 0, s, ts, c, shadow, film, ovf);
+        hipLaunchKernelGGL(k_medium<SAMPLER_HALTON>, dim3(grid), dim3(256), 0, s, rc, sv, c, path, hit, next,
+                           shadow, film);
+}
+
+void launch_shadow(hipStream_t s, int grid, bool postpone, bool media, const TraceScene &ts, const SceneView &sv,
+                   WaveCounters *c, ShadowSOA shadow, FilmView film, const TraceOverflow &ovf)
+{
+    const dim3 g(grid), b(TRACE_BLOCK);
+    if(postpone && media)
+        hipLaunchKernelGGL((k_shadow<true, true>), g, b, 0, s, ts, sv, c, shadow, film, ovf);
+    else if(postpone)
+        hipLaunchKernelGGL((k_shadow<true, false>), g, b, 0, s, ts, sv, c, shadow, film, ovf);
+    else if(media)
+        hipLaunchKernelGGL((k_shadow<false, true>), g, b, 0, s, ts, sv, c, shadow, film, ovf);
+    else
+        hipLaunchKernelGGL((k_shadow<false, false>), g, b, 0, s, ts, sv, c, shadow, film, ovf);
 }
 
 void launch_end_iteration(hipStream_t s, WaveCounters *c)

@@ -41,8 +41,15 @@ enum
 
 enum
 {
-    BTRC_MATERIAL_DIFFUSE = 0, /* builtin/material/diffuse.cpp           */
-    BTRC_MATERIAL_DISNEY = 1   /* builtin/material/disney.cpp            */
+    BTRC_MATERIAL_DIFFUSE = 0,  /* builtin/material/diffuse.cpp          */
+    BTRC_MATERIAL_DISNEY = 1,   /* builtin/material/disney.cpp           */
+    BTRC_MATERIAL_INVISIBLE = 2 /* builtin/material/invisible.cpp        */
+};
+
+enum
+{
+    BTRC_MEDIUM_HOMOGENEOUS = 0,  /* builtin/medium/homogeneous.cpp     */
+    BTRC_MEDIUM_HETEROGENEOUS = 1 /* builtin/medium/hetergeneous.cpp    */
 };
 
 enum
@@ -101,9 +108,30 @@ typedef struct btrc_instance_desc
     float world_to_local[16]; /* row-major Transform3D::inv */
     int32_t has_light;        /* mesh_light on this instance */
     float light_intensity[3];
-    int32_t inner_medium;     /* -1: none */
-    int32_t outer_medium;     /* -1: none */
+    int32_t inner_medium;     /* index into btrc_scene_desc::media, -1: none */
+    int32_t outer_medium;     /* index into btrc_scene_desc::media, -1: none */
 } btrc_instance_desc;
+
+/* A participating medium, referenced by instances' inner/outer_medium.
+ * HomogeneousMedium: sigma_t, albedo, g. HetergeneousMedium: a density grid
+ * (Array3D text layout, utils/cuda/array.cpp:313-385: x fastest, then y,
+ * then z) sampled at 0.5 + 0.5 * uvw with linear filtering and clamping,
+ * albedo from albedo_grid (3 floats per voxel) or the constant albedo, and
+ * a constant g. With has_transform, uvw = world_to_local(p)
+ * (TransformMedium, core/medium.cpp:98-129); else uvw = p. */
+typedef struct btrc_medium_desc
+{
+    int32_t type;             /* BTRC_MEDIUM_* */
+    float priority;           /* where media meet, the higher wins (core/scene.cpp:144-161) */
+    float sigma_t;            /* homogeneous extinction */
+    float albedo[3];
+    float g;                  /* Henyey-Greenstein asymmetry, |g| < 1 */
+    int32_t grid_size[3];     /* heterogeneous: width, height, depth */
+    const float *sigma_t_grid;
+    const float *albedo_grid; /* or NULL */
+    int32_t has_transform;
+    float world_to_local[16]; /* row-major */
+} btrc_medium_desc;
 
 typedef struct btrc_envir_light_desc
 {
@@ -122,6 +150,8 @@ typedef struct btrc_scene_desc
     int32_t instance_count;
     const btrc_instance_desc *instances;
     btrc_envir_light_desc envir_light;
+    int32_t medium_count;
+    const btrc_medium_desc *media;
 } btrc_scene_desc;
 
 /* PinholeCamera (builtin/camera/pinhole.cpp) */
@@ -167,8 +197,8 @@ typedef struct btrc_render_stats
     uint64_t shadow_rays;    /* visibility rays traced */
     uint64_t paths;          /* camera paths completed */
     uint64_t iterations;     /* wavefront bounces executed */
-    double stage_ms[4];      /* generate, trace, shade, shadow (time_stages) */
-    uint64_t stage_launches[4];
+    double stage_ms[5];      /* generate, trace, shade, shadow, medium (time_stages) */
+    uint64_t stage_launches[5];
 } btrc_render_stats;
 
 /* A ray of a traversal batch and its closest hit */

@@ -1,6 +1,7 @@
 """The C-ABI library: it loads, exports every symbol include/btrc_hip.h
 declares, and its struct layouts match the Python mirror. No GPU calls."""
 import ctypes as C
+import subprocess
 import os
 import re
 
@@ -37,7 +38,39 @@ def test_struct_sizes():
     assert C.sizeof(_abi.RenderParams) == 64
     assert C.sizeof(_abi.MaterialDesc) == 68
     assert C.sizeof(_abi.InstanceDesc) == 4 * (2 + 32 + 1 + 3 + 2)
-    assert C.sizeof(_abi.RenderStats) == 8 * (1 + 4 + 4 + 4)
+    assert C.sizeof(_abi.RenderStats) == 8 * (1 + 4 + 5 + 5)
+
+
+STRUCTS = {
+    "btrc_geometry_desc": "GeometryDesc", "btrc_material_desc": "MaterialDesc",
+    "btrc_instance_desc": "InstanceDesc", "btrc_medium_desc": "MediumDesc",
+    "btrc_envir_light_desc": "EnvirLightDesc", "btrc_scene_desc": "SceneDesc",
+    "btrc_camera_desc": "CameraDesc", "btrc_render_params": "RenderParams",
+    "btrc_filter_desc": "FilterDesc", "btrc_render_stats": "RenderStats", "btrc_ray": "Ray",
+    "btrc_hit": "Hit", "btrc_tuning": "Tuning",
+}
+
+
+def test_mirror_layout_matches_header(tmp_path):
+    """every field of every ctypes mirror sits at the offset the C compiler
+    gives the header's struct"""
+    lines = ["#include <stdio.h>", "#include <stddef.h>", f'#include "{os.path.join(ROOT, "include", "btrc_hip.h")}"',
+             "int main(void) {"]
+    for cname, pyname in STRUCTS.items():
+        lines.append(f'printf("{pyname} size %zu\\n", sizeof({cname}));')
+        for field, _ in getattr(_abi, pyname)._fields_:
+            lines.append(f'printf("{pyname} {field} %zu\\n", offsetof({cname}, {field}));')
+    lines += ["return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", "-o", str(exe), str(src)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    for line in filter(None, out):
+        pyname, what, value = line.split()
+        cls = getattr(_abi, pyname)
+        want = C.sizeof(cls) if what == "size" else getattr(cls, what).offset
+        assert int(value) == want, f"{pyname}.{what}: header {value}, mirror {want}"
     assert _abi.ray_dtype().itemsize == 32 and _abi.hit_dtype().itemsize == 20
 
 
