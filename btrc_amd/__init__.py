@@ -198,6 +198,15 @@ class Context:
                                                        len(px), dims, out.ctypes.data))
         return out
 
+    def eval_medium(self, medium, segments, seeds):
+        """(n, 10) float32: scattered, position, albedo, transmittance"""
+        seg = np.ascontiguousarray(segments, dtype=_abi.ray_dtype())
+        seeds = np.ascontiguousarray(seeds, dtype=np.uint64)
+        out = np.empty((len(seg), 10), dtype=np.float32)
+        self._check(self.lib.btrc_hip_eval_medium(self.h, medium, seg.ctypes.data, seeds.ctypes.data, len(seg),
+                                                  out.ctypes.data))
+        return out
+
     def eval_detmath(self, fn, x, y=None):
         x = np.ascontiguousarray(x, dtype=np.float32)
         y = np.ascontiguousarray(np.ones_like(x) if y is None else y, dtype=np.float32)

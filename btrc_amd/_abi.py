@@ -122,7 +122,8 @@ EXPORTS = [
     "btrc_hip_render", "btrc_hip_film_readback", "btrc_hip_aov_readback", "btrc_hip_request_stop",
     "btrc_hip_trace_closest", "btrc_hip_trace_any",
     "btrc_hip_batch_upload", "btrc_hip_batch_trace", "btrc_hip_batch_download",
-    "btrc_hip_capture_rays", "btrc_hip_eval_halton", "btrc_hip_eval_independent", "btrc_hip_eval_detmath",
+    "btrc_hip_capture_rays", "btrc_hip_eval_halton", "btrc_hip_eval_independent", "btrc_hip_eval_medium",
+    "btrc_hip_eval_detmath",
 ]
 
 
@@ -155,6 +156,7 @@ def declare(lib):
                                            C.c_int32, C.c_int32, C.c_void_p]),
         "btrc_hip_eval_independent": (C.c_int, [ctx, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_int32,
                                                 C.c_void_p]),
+        "btrc_hip_eval_medium": (C.c_int, [ctx, C.c_int32, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p]),
         "btrc_hip_eval_detmath": (C.c_int, [ctx, C.c_int32, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p]),
     }
     for name, (res, args) in sig.items():

@@ -1285,6 +1285,29 @@ int btrc_hip_eval_independent(btrc_ctx *ctx, const uint32_t *px, const uint32_t 
     });
 }
 
+int btrc_hip_eval_medium(btrc_ctx *ctx, int32_t medium, const btrc_ray *segments, const uint64_t *seeds, int32_t n,
+                         float *out)
+{
+    return guarded(ctx, [&] {
+        if(!ctx->has_scene)
+            throw StateError("medium query before btrc_hip_upload_scene");
+        if(medium < 0 || medium >= ctx->medium_count || n < 0 || (n > 0 && (!segments || !seeds || !out)))
+            throw InvalidArg("bad medium query");
+        if(n == 0)
+            return;
+        DevBuf<float4> dseg;
+        DevBuf<uint2> dseed;
+        DevBuf<float> dout;
+        dseg.upload(reinterpret_cast<const float4 *>(segments), 2 * (size_t)n, ctx->stream);
+        dseed.upload(reinterpret_cast<const uint2 *>(seeds), n, ctx->stream);
+        dout.alloc(10 * (size_t)n);
+        launch_medium_eval(ctx->stream, ctx->scene_view(), medium, dseg.ptr, dseed.ptr, n, dout.ptr);
+        BTRC_HIP_CHECK(hipGetLastError());
+        BTRC_HIP_CHECK(hipMemcpyAsync(out, dout.ptr, 10 * (size_t)n * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+        BTRC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    });
+}
+
 int btrc_hip_eval_detmath(btrc_ctx *ctx, int32_t fn, const float *x, const float *y, int32_t n, float *out)
 {
     return guarded(ctx, [&] {

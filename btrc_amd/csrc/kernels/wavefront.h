@@ -148,6 +148,8 @@ void launch_trace_batch(hipStream_t s, int grid, bool postpone, const TraceScene
 // test hooks
 void launch_halton_eval(hipStream_t s, const HaltonConsts &hc, const uint32_t *px, const uint32_t *py,
                         const int32_t *sample_index, int32_t n, int32_t dims, float *out);
+void launch_medium_eval(hipStream_t s, const SceneView &sv, int32_t med, const float4 *seg, const uint2 *seed,
+                        int32_t n, float *out);
 void launch_independent_eval(hipStream_t s, const uint32_t *px, const uint32_t *py, const int32_t *sample_index,
                              int32_t n, int32_t dims, float *out);
 void launch_detmath_eval(hipStream_t s, int32_t fn, const float *x, const float *y, int32_t n, float *out);

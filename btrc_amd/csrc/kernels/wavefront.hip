@@ -731,6 +731,26 @@ __global__ void __launch_bounds__(TRACE_BLOCK) k_trace_batch(
 
 // ---------------------------------------------------------------- test hooks
 
+// one medium along segments a -> a + d * tmax with one PCG32 stream per
+// segment: delta tracking, then ratio tracking on the same stream
+__global__ void k_medium_eval(SceneView sv, int32_t med, const float4 *seg, const uint2 *seed, int32_t n, float *out)
+{
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if(i >= n)
+        return;
+    const float4 a4 = seg[2 * i], d4 = seg[2 * i + 1];
+    const V3 a = xyz(a4), b = xyz(a4) + xyz(d4) * d4.w;
+    TrSampler smp = TrSampler::start(seed[i]);
+    const MediumRec &m = sv.media[med];
+    const MediumSample ms = medium_sample(m, sv.voxels, a, b, smp);
+    const V3 tr = medium_tr(m, sv.voxels, a, b, smp);
+    float *o = out + 10 * (size_t)i;
+    o[0] = ms.scattered ? 1.0f : 0.0f;
+    o[1] = ms.position.x; o[2] = ms.position.y; o[3] = ms.position.z;
+    o[4] = ms.color.x; o[5] = ms.color.y; o[6] = ms.color.z;
+    o[7] = tr.x; o[8] = tr.y; o[9] = tr.z;
+}
+
 __global__ void k_independent_eval(const uint32_t *px, const uint32_t *py, const int32_t *si, int32_t n,
                                    int32_t dims, float *out)
 {
@@ -879,6 +899,12 @@ void launch_halton_eval(hipStream_t s, const HaltonConsts &hc, const uint32_t *p
                         const int32_t *sample_index, int32_t n, int32_t dims, float *out)
 {
     hipLaunchKernelGGL(k_halton_eval, dim3((n + 127) / 128), dim3(128), 0, s, hc, px, py, sample_index, n, dims, out);
+}
+
+void launch_medium_eval(hipStream_t s, const SceneView &sv, int32_t med, const float4 *seg, const uint2 *seed,
+                        int32_t n, float *out)
+{
+    hipLaunchKernelGGL(k_medium_eval, dim3((n + 127) / 128), dim3(128), 0, s, sv, med, seg, seed, n, out);
 }
 
 void launch_independent_eval(hipStream_t s, const uint32_t *px, const uint32_t *py, const int32_t *sample_index,

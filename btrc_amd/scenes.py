@@ -98,6 +98,7 @@ class Scene:
         self.geometries = []   # dicts: positions (n,9) f32, normals (n,9) or None, uvs (n,6) or None
         self.materials = []    # MaterialDesc kwargs
         self.instances = []    # dicts
+        self.media = []        # dicts
         self.envir = None      # (up, lower, upper)
 
     def add_geometry(self, positions, normals=None, uvs=None):
@@ -128,10 +129,35 @@ class Scene:
                                    ior=ior, shadow_terminator=shadow_terminator))
         return len(self.materials) - 1
 
-    def add_instance(self, geometry, material, local_to_world=None, light=None):
+    def add_invisible(self):
+        """InvisibleSurface (builtin/material/invisible.cpp): a boundary rays pass through"""
+        self.materials.append(dict(type=_abi.MATERIAL_INVISIBLE, color=(1.0, 1.0, 1.0)))
+        return len(self.materials) - 1
+
+    def add_homogeneous(self, sigma_t, albedo, g=0.0, priority=0.0):
+        """HomogeneousMedium (builtin/medium/homogeneous.cpp:75-88)"""
+        self.media.append(dict(type=_abi.MEDIUM_HOMOGENEOUS, sigma_t=sigma_t, albedo=albedo, g=g,
+                               priority=priority))
+        return len(self.media) - 1
+
+    def add_heterogeneous(self, sigma_t_grid, albedo, g=0.0, priority=0.0, world_to_local=None, albedo_grid=None):
+        """HetergeneousMedium (builtin/medium/hetergeneous.cpp:108-133) over a
+        density grid indexed [z, y, x], sampled at 0.5 + 0.5 * uvw; with
+        world_to_local, uvw = world_to_local(p) (TransformMedium)."""
+        grid = np.ascontiguousarray(sigma_t_grid, dtype=np.float32)
+        assert grid.ndim == 3
+        ag = None if albedo_grid is None else np.ascontiguousarray(albedo_grid, dtype=np.float32)
+        if ag is not None:
+            assert ag.shape == grid.shape + (3,)
+        self.media.append(dict(type=_abi.MEDIUM_HETEROGENEOUS, grid=grid, albedo=albedo, g=g, priority=priority,
+                               xf=None if world_to_local is None else np.asarray(world_to_local, dtype=np.float64),
+                               albedo_grid=ag))
+        return len(self.media) - 1
+
+    def add_instance(self, geometry, material, local_to_world=None, light=None, inner_medium=-1, outer_medium=-1):
         m = np.eye(4) if local_to_world is None else np.asarray(local_to_world, dtype=np.float64)
         self.instances.append(dict(geometry=geometry, material=material, mat=m,
-                                   inv=np.linalg.inv(m), light=light))
+                                   inv=np.linalg.inv(m), light=light, inner=inner_medium, outer=outer_medium))
         return len(self.instances) - 1
 
     def set_gradient_sky(self, up, lower, upper):
@@ -171,8 +197,26 @@ class Scene:
             d.has_light = 1 if inst["light"] is not None else 0
             if inst["light"] is not None:
                 d.light_intensity = _abi.F3(*[float(v) for v in inst["light"]])
-            d.inner_medium = -1
-            d.outer_medium = -1
+            d.inner_medium = inst.get("inner", -1)
+            d.outer_medium = inst.get("outer", -1)
+        med = (_abi.MediumDesc * max(1, len(self.media)))()
+        fp = C.POINTER(C.c_float)
+        for k, m in enumerate(self.media):
+            d = med[k]
+            d.type = m["type"]
+            d.priority = float(m["priority"])
+            d.albedo = _abi.F3(*[float(v) for v in m["albedo"]])
+            d.g = float(m["g"])
+            if m["type"] == _abi.MEDIUM_HOMOGENEOUS:
+                d.sigma_t = float(m["sigma_t"])
+            else:
+                nz, ny, nx = m["grid"].shape
+                d.grid_size = (C.c_int32 * 3)(nx, ny, nz)
+                d.sigma_t_grid = m["grid"].ctypes.data_as(fp)
+                d.albedo_grid = m["albedo_grid"].ctypes.data_as(fp) if m["albedo_grid"] is not None else None
+                if m["xf"] is not None:
+                    d.has_transform = 1
+                    d.world_to_local = _abi.F16(*m["xf"].astype(np.float32).reshape(-1).tolist())
         sd = _abi.SceneDesc()
         sd.geometry_count = len(self.geometries)
         sd.geometries = gd
@@ -180,6 +224,8 @@ class Scene:
         sd.materials = md
         sd.instance_count = len(self.instances)
         sd.instances = idesc
+        sd.medium_count = len(self.media)
+        sd.media = med
         if self.envir is not None:
             up, lower, upper = self.envir
             sd.envir_light.type = _abi.ENVIR_GRADIENT_SKY
@@ -188,7 +234,7 @@ class Scene:
             sd.envir_light.upper = _abi.F3(*map(float, upper))
         else:
             sd.envir_light.type = _abi.ENVIR_NONE
-        holder.update(gd=gd, md=md, idesc=idesc, geoms=self.geometries)
+        holder.update(gd=gd, md=md, idesc=idesc, med=med, geoms=self.geometries, media=self.media)
         return sd, holder
 
 
@@ -294,6 +340,85 @@ def cornell(seed=1):
     s.add_instance(s.add_geometry(light), s.add_diffuse([0.0, 0.0, 0.0]), light=(17.0, 12.0, 4.0))
     cam = Camera((0.0, 1.0, 3.4), (0.0, 1.0, 0.0), (0.0, 1.0, 0.0), 40.0)
     return s, cam
+
+
+def fbm_grid(n=128, octaves=5, seed=3, max_sigma=2.0, base=4):
+    """n^3 density grid [z, y, x] of seeded fBm value noise: octave k has a
+    (base * 2^k + 1)^3 lattice of PCG32 values in [0, 1), smoothstep-weighted
+    trilinear interpolation, amplitude 2^-k; the sum is scaled so that its
+    maximum is max_sigma."""
+    rng = PCG32(seed)
+    c = (np.arange(n) + 0.5) / n
+    total = np.zeros((n, n, n), dtype=np.float64)
+    for k in range(octaves):
+        f = base * 2 ** k
+        lat = np.array([rng.uniform() for _ in range((f + 1) ** 3)], dtype=np.float64).reshape(f + 1, f + 1, f + 1)
+        x = c * f
+        i = np.minimum(np.floor(x).astype(np.int64), f - 1)
+        t = x - i
+        w = t * t * (3 - 2 * t)
+        iz, iy, ix = np.ix_(i, i, i)
+        wz, wy, wx = np.ix_(w, w, w)
+        v = 0.0
+        for dz in (0, 1):
+            for dy in (0, 1):
+                for dx in (0, 1):
+                    v = v + (lat[iz + dz, iy + dy, ix + dx] * (wz if dz else 1 - wz) * (wy if dy else 1 - wy)
+                             * (wx if dx else 1 - wx))
+        total += v * 0.5 ** k
+    return (total * (max_sigma / total.max())).astype(np.float32)
+
+
+def save_grid_text(path, grid):
+    """Array::load_from_text layout (utils/cuda/array.cpp:313-385): header
+    "float C W H D", then the values with x fastest, then y, then z"""
+    g = np.asarray(grid, dtype=np.float32)
+    ch = 1 if g.ndim == 3 else g.shape[3]
+    d, h, w = g.shape[:3]
+    with open(path, "w") as f:
+        f.write(f"float {ch} {w} {h} {d}\n")
+        np.savetxt(f, g.reshape(-1, ch), fmt="%.9g")
+
+
+def load_grid_text(path):
+    """inverse of save_grid_text; also reads "byte" grids (values / 255 as
+    the SNorm8 texture would not: bytes are returned as integers)"""
+    with open(path) as f:
+        comp, ch, w, h, d = f.readline().split()
+        ch, w, h, d = int(ch), int(w), int(h), int(d)
+        vals = np.loadtxt(f, dtype=np.float32 if comp == "float" else np.int64).reshape(-1)
+    if comp not in ("float", "byte"):
+        raise ValueError(f"unknown component type {comp}")
+    if ch not in (1, 3, 4):
+        raise ValueError(f"unsupported channel count {ch}")
+    g = vals.reshape(d, h, w, ch)
+    return g[..., 0] if ch == 1 else g
+
+
+def cornell_media(scale=30.0, grid=None, g=0.6, albedo=0.8, grid_n=128, seed=3):
+    """The Cornell-style scene scaled by `scale` (room 60^3 at 30) holding a
+    40^3 invisible box centred in the room, filled with a heterogeneous
+    medium: `grid` (default: 128^3 seeded 5-octave fBm, max sigma_t 2.0),
+    albedo 0.8, Henyey-Greenstein g = 0.6."""
+    base, cam = cornell()
+    s = Scene()
+    s.geometries = base.geometries
+    s.materials = base.materials
+    sm = np.diag([scale, scale, scale, 1.0])
+    for inst in base.instances:
+        s.add_instance(inst["geometry"], inst["material"], sm @ inst["mat"], light=inst["light"])
+    if grid is None:
+        grid = fbm_grid(grid_n, seed=seed)
+    half = 20.0
+    centre = (0.0, scale, 0.0)
+    w2l = np.diag([1 / half, 1 / half, 1 / half, 1.0]) @ translate(-centre[0], -centre[1], -centre[2])
+    med = s.add_heterogeneous(grid, (albedo, albedo, albedo), g=g, world_to_local=w2l)
+    inv = s.add_invisible()
+    box = np.concatenate(box_faces(2 * half, 2 * half, 2 * half))
+    s.add_instance(s.add_geometry(box), inv, translate(*centre), inner_medium=med)
+    e = np.array(cam.eye) * scale
+    t = np.array(cam.dst) * scale
+    return s, Camera(tuple(e), tuple(t), cam.up, cam.fov_y_deg)
 
 
 def sphere_field(instances=4096, lights=64, seed=2, extent=100.0, envir=(0.5, 0.5, 0.5)):

@@ -279,6 +279,11 @@ int btrc_hip_eval_halton(btrc_ctx *ctx, int32_t width, int32_t height, const uin
 /* the independent sampler's first `dims` numbers for each (px, py, sample) */
 int btrc_hip_eval_independent(btrc_ctx *ctx, const uint32_t *px, const uint32_t *py, const int32_t *sample_index,
                               int32_t n, int32_t dims, float *out);
+/* medium id (rank by descending priority) along segments o -> o + d * tmax,
+ * one PCG32 stream per segment (seeded like a shadow ray's): out holds 10
+ * floats per segment: scattered (0/1), position xyz, albedo xyz, tr xyz */
+int btrc_hip_eval_medium(btrc_ctx *ctx, int32_t medium, const btrc_ray *segments, const uint64_t *seeds, int32_t n,
+                         float *out);
 /* fn: 0 sin, 1 cos, 2 log, 3 exp, 4 pow(x, y), 5 sqrt, 6 x / y */
 int btrc_hip_eval_detmath(btrc_ctx *ctx, int32_t fn, const float *x, const float *y, int32_t n, float *out);
 

@@ -366,6 +366,7 @@ struct Pcg
         const float f = (float)next() / 4294967296.0f;
         return std::min(f, 0x1.fffffep-1f);
     }
+    float get1d() { return uniform(); }
 };
 
 // IndependentSampler (sampler/independent.cpp:11-25), pbrt-v4 RNG(seq) seeding
@@ -401,7 +402,41 @@ struct Smp
         else
             h.get3d(a, b, c);
     }
+    // the four words a path's sampler keeps between stages (pixel first)
+    void pack(uint32_t pixel, uint32_t w[4]) const
+    {
+        w[0] = pixel;
+        if(indep)
+        {
+            w[1] = (uint32_t)p.rng.state;
+            w[2] = (uint32_t)(p.rng.state >> 32);
+            w[3] = 0;
+        }
+        else
+        {
+            w[1] = h.dim;
+            w[2] = (uint32_t)h.idx;
+            w[3] = (uint32_t)(h.idx >> 32);
+        }
+    }
+    struct Packed { uint32_t w[4]; };
+    Packed pack(uint32_t pixel) const
+    {
+        Packed r;
+        pack(pixel, r.w);
+        return r;
+    }
 };
+
+// seed of a shadow ray's transmittance stream: a hash of the path's sampler
+// words when the ray is made (this build's choice; the reference draws from
+// per-slot shadow samplers, shadow.cpp:45-49)
+static uint64_t shadow_seed(const Smp::Packed &p)
+{
+    const uint64_t a = ((uint64_t)p.w[0] << 32) | p.w[1];
+    const uint64_t b = ((uint64_t)p.w[2] << 32) | p.w[3];
+    return mixbits(a ^ mixbits(b));
+}
 
 // ------------------------------------------------------------------ scene
 
@@ -425,7 +460,20 @@ struct Geo
 struct Inst
 {
     int geo, mat, light;
+    uint32_t inner, outer; // medium ids
     Xf xf;
+};
+
+// a participating medium (medium/homogeneous.cpp, medium/hetergeneous.cpp)
+struct Med
+{
+    int type;
+    float sigma_t, inv_maj, g;
+    Vec albedo;
+    int nx, ny, nz;
+    std::vector<float> dens, alb;
+    bool has_xf;
+    float w2l[12];
 };
 
 struct Scene
@@ -435,6 +483,8 @@ struct Scene
     std::vector<Inst> insts;
     std::vector<Light> lights;
     int env = -1;
+    std::vector<Med> media; // by descending priority
+    int R = 0;
     // per triangle of all geometries: frame records (triangle_mesh.cpp:40-64)
     std::vector<Vec> gx, gy, gz, sa, sba, sca, la, lba, lca;
     // flattened world triangles
@@ -598,6 +648,43 @@ static Scene *make_scene(const btrc_scene_desc *sd)
         off += (uint32_t)g.triangle_count;
     }
     s->mats.assign(sd->materials, sd->materials + sd->material_count);
+    // Scene::commit's medium ids (core/scene.cpp:144-161): descending
+    // priority, the volume-primitive medium next, then "none"
+    s->R = sd->medium_count;
+    std::vector<int> by_priority(s->R);
+    for(int i = 0; i < s->R; ++i)
+        by_priority[i] = i;
+    std::stable_sort(by_priority.begin(), by_priority.end(),
+                     [&](int a, int b) { return sd->media[a].priority > sd->media[b].priority; });
+    std::vector<uint32_t> medium_rank(s->R);
+    for(int r = 0; r < s->R; ++r)
+    {
+        const btrc_medium_desc &md = sd->media[by_priority[r]];
+        medium_rank[by_priority[r]] = (uint32_t)r;
+        Med m{};
+        m.type = md.type;
+        m.sigma_t = md.sigma_t;
+        m.g = md.g;
+        m.albedo = mk(md.albedo[0], md.albedo[1], md.albedo[2]);
+        if(md.type == BTRC_MEDIUM_HETEROGENEOUS)
+        {
+            m.nx = md.grid_size[0];
+            m.ny = md.grid_size[1];
+            m.nz = md.grid_size[2];
+            const size_t nv = (size_t)m.nx * m.ny * m.nz;
+            m.dens.assign(md.sigma_t_grid, md.sigma_t_grid + nv);
+            float mx = m.dens[0];
+            for(float v : m.dens)
+                mx = std::max(mx, v);
+            m.inv_maj = 1.0f / std::max(mx, 0.001f); // hetergeneous.cpp:34-35
+            if(md.albedo_grid)
+                m.alb.assign(md.albedo_grid, md.albedo_grid + 3 * nv);
+        }
+        m.has_xf = md.has_transform != 0;
+        for(int k = 0; k < 12; ++k)
+            m.w2l[k] = md.world_to_local[k];
+        s->media.push_back(std::move(m));
+    }
     for(int i = 0; i < sd->instance_count; ++i)
     {
         const btrc_instance_desc &d = sd->instances[i];
@@ -605,6 +692,8 @@ static Scene *make_scene(const btrc_scene_desc *sd)
         in.geo = d.geometry;
         in.mat = d.material;
         in.light = -1;
+        in.inner = d.inner_medium < 0 ? (uint32_t)s->R + 1 : medium_rank[d.inner_medium];
+        in.outer = d.outer_medium < 0 ? (uint32_t)s->R + 1 : medium_rank[d.outer_medium];
         for(int k = 0; k < 12; ++k)
         {
             in.xf.m[k] = d.local_to_world[k];
@@ -908,6 +997,7 @@ struct BS
 {
     Vec f, dir;
     float pdf;
+    bool delta = false;
 };
 
 struct Dis
@@ -1191,6 +1281,7 @@ struct Shd
     Frm geo, sh;
     bool st;
     bool disney;
+    bool invisible = false;
     Vec albedo;
     Dis d;
 };
@@ -1213,6 +1304,12 @@ static float term(const Shd &s, Vec wi)
 
 static BS shd_sample(const Shd &s, Vec wo, float s0, float s1, float s2)
 {
+    if(s.invisible)
+    {
+        // InvisibleShader::sample_bidir, material/invisible.cpp:18-27
+        BS r{ dvs(mk(1, 1, 1), std::fabs(dot(s.geo.z, nrm(wo)))), neg(wo), 1.0f, true };
+        return r;
+    }
     const Frm sh = flip(s, wo);
     const Vec lo = nrm(g2l(sh, wo));
     const BS c = s.disney ? dis_sample(s.d, lo, s0, s1, s2) : lam_sample(s.albedo, lo, s0, s1);
@@ -1230,6 +1327,8 @@ static BS shd_sample(const Shd &s, Vec wo, float s0, float s1, float s2)
 
 static Vec shd_eval(const Shd &s, Vec wi, Vec wo)
 {
+    if(s.invisible)
+        return mk(0, 0, 0);
     const Frm sh = flip(s, wo);
     const Vec li = nrm(g2l(sh, wi)), lo = nrm(g2l(sh, wo));
     Vec f;
@@ -1242,6 +1341,8 @@ static Vec shd_eval(const Shd &s, Vec wi, Vec wo)
 
 static float shd_pdf(const Shd &s, Vec wi, Vec wo)
 {
+    if(s.invisible)
+        return 0.0f;
     const Frm sh = flip(s, wo);
     const Vec li = nrm(g2l(sh, wi)), lo = nrm(g2l(sh, wo));
     if(s.disney)
@@ -1269,7 +1370,7 @@ struct Shadow
 // UniformLightSampler::sample, core/light_sampler.cpp:41-58;
 // MeshLight::sample_li, light/mesh_light.cpp:38-58; TriangleMesh::sample_inline,
 // geometry/triangle_mesh.cpp:188-228; CAliasTable::sample, utils/cmath/calias.cpp:15-21
-static Shadow sample_light(const Scene &s, Vec ref, Vec nor, float l0, float l1, float l2, float lsel)
+static Shadow sample_light(const Scene &s, Vec ref, Vec nor, bool offset, float l0, float l1, float l2, float lsel)
 {
     Shadow r{};
     const int n = (int)s.lights.size();
@@ -1300,7 +1401,7 @@ static Shadow sample_light(const Scene &s, Vec ref, Vec nor, float l0, float l1,
         const Vec rad = dot(p2r, snor) > 0.0f ? l.intensity : mk(0, 0, 0);
         const Vec diff = sub(spos, ref);
         const Vec dst = offset_dir(spos, snor, neg(diff));
-        r.o = offset_dir(ref, nor, diff);
+        r.o = offset ? offset_dir(ref, nor, diff) : ref; // helper.cpp:115-142
         r.d = sub(dst, r.o);
         r.t1 = 1.0f;
         r.li = rad;
@@ -1309,13 +1410,150 @@ static Shadow sample_light(const Scene &s, Vec ref, Vec nor, float l0, float l1,
     else
     {
         const Vec dir = sphere_uniform(l0, l1);
-        r.o = offset_dir(ref, nor, dir);
+        r.o = offset ? offset_dir(ref, nor, dir) : ref;
         r.d = dir;
         r.t1 = FMAX;
         r.li = sky_le(l, dir);
         r.pdf = sel_pdf * (1.0f / (4.0f * PI));
     }
     return r;
+}
+
+// ------------------------------------------------------------------ media
+
+// henyey_greenstein, utils/henyey_greenstein.h:8-16
+static float hg(float g, float u)
+{
+    const float g2 = g * g;
+    const float dem = 1.0f + g2 - 2.0f * (g * u);
+    return (1.0f - g2) / (4.0f * PI * dem * std::sqrt(dem));
+}
+
+// HenyeyGreensteinPhaseShader::sample, core/medium.cpp:36-66; returns the
+// phase value (colour times the phase function)
+static Vec hg_sample(float g, Vec color, Vec wo, float s0, float s1, Vec &dir, float &pdf)
+{
+    const float sv = s0 + s0 - 1.0f;
+    float u = sv;
+    if(!(std::fabs(g) < 0.001f))
+    {
+        const float g2 = g * g;
+        const float q = (1.0f - g2) / (1.0f + g * sv);
+        u = (1.0f + g2 - q * q) / (g + g);
+    }
+    const float ct = -u;
+    const float st = std::sqrt(fmx(0.0f, 1.0f - ct * ct));
+    const float phi = 2.0f * PI * s1;
+    const Vec local = mk(st * dm::sin(phi), st * dm::cos(phi), ct);
+    const Frm f = from_z(wo);
+    dir = add(add(sclf(local.x, f.x), sclf(local.y, f.y)), sclf(local.z, f.z));
+    pdf = hg(g, u);
+    return scl(color, pdf);
+}
+
+// linear filtering with clamp addressing at normalised coordinates: texel i
+// sits at (i + 0.5) / n; lerp along x, then y, then z
+static float tex3(const std::vector<float> &v, int nx, int ny, int nz, int ch, int c, Vec uvw)
+{
+    const float q[3] = { uvw.x * (float)nx - 0.5f, uvw.y * (float)ny - 0.5f, uvw.z * (float)nz - 0.5f };
+    const int n[3] = { nx, ny, nz };
+    int i0[3], i1[3];
+    float t[3];
+    for(int a = 0; a < 3; ++a)
+    {
+        float x = q[a];
+        if(!(x >= -1.0f)) x = -1.0f;
+        if(x > (float)n[a]) x = (float)n[a];
+        const float fl = std::floor(x);
+        t[a] = x - fl;
+        const int i = (int)fl;
+        i0[a] = std::min(std::max(i, 0), n[a] - 1);
+        i1[a] = std::min(std::max(i + 1, 0), n[a] - 1);
+    }
+    auto at = [&](int x, int y, int z) { return v[(((size_t)z * ny + y) * nx + x) * ch + c]; };
+    auto lerp = [](float a, float b, float w) { return a * (1.0f - w) + b * w; };
+    const float c00 = lerp(at(i0[0], i0[1], i0[2]), at(i1[0], i0[1], i0[2]), t[0]);
+    const float c10 = lerp(at(i0[0], i1[1], i0[2]), at(i1[0], i1[1], i0[2]), t[0]);
+    const float c01 = lerp(at(i0[0], i0[1], i1[2]), at(i1[0], i0[1], i1[2]), t[0]);
+    const float c11 = lerp(at(i0[0], i1[1], i1[2]), at(i1[0], i1[1], i1[2]), t[0]);
+    return lerp(lerp(c00, c10, t[1]), lerp(c01, c11, t[1]), t[2]);
+}
+
+static Vec med_uvw(const Med &m, Vec p) // TransformMedium, core/medium.cpp:98-129
+{
+    if(!m.has_xf)
+        return p;
+    const float *w = m.w2l;
+    return mk(w[0] * p.x + w[1] * p.y + w[2] * p.z + w[3], w[4] * p.x + w[5] * p.y + w[6] * p.z + w[7],
+              w[8] * p.x + w[9] * p.y + w[10] * p.z + w[11]);
+}
+
+const int MAX_TRACKING = 1 << 20;
+
+// Medium::sample along a -> b: HomogeneousMedium (homogeneous.cpp:25-58),
+// delta tracking of HetergeneousMedium (hetergeneous.cpp:25-72)
+template<class S>
+static bool med_sample(const Med &m, Vec a, Vec b, S &smp, Vec &pos, Vec &color)
+{
+    const float tmax = len(sub(b, a));
+    if(m.type == BTRC_MEDIUM_HOMOGENEOUS)
+    {
+        const float st = -dm::log(1.0f - smp.get1d()) / m.sigma_t;
+        if(!(st < tmax))
+            return false;
+        const float t = st / tmax;
+        pos = add(scl(a, 1.0f - t), scl(b, t));
+        color = m.albedo;
+        return true;
+    }
+    const Vec la = add(mk(0.5f, 0.5f, 0.5f), sclf(0.5f, med_uvw(m, a)));
+    const Vec lb = add(mk(0.5f, 0.5f, 0.5f), sclf(0.5f, med_uvw(m, b)));
+    const Vec lstep = dvs(sub(lb, la), tmax), wstep = dvs(sub(b, a), tmax);
+    float t = 0.0f;
+    for(int it = 0; it < MAX_TRACKING; ++it)
+    {
+        t = t + -dm::log(1.0f - smp.get1d()) * m.inv_maj;
+        if(t >= tmax)
+            return false;
+        const Vec uvw = add(la, sclf(t, lstep));
+        const float dens = tex3(m.dens, m.nx, m.ny, m.nz, 1, 0, uvw);
+        if(smp.get1d() < dens * m.inv_maj)
+        {
+            pos = add(a, sclf(t, wstep));
+            color = m.alb.empty() ? m.albedo
+                                  : mk(tex3(m.alb, m.nx, m.ny, m.nz, 3, 0, uvw), tex3(m.alb, m.nx, m.ny, m.nz, 3, 1, uvw),
+                                       tex3(m.alb, m.nx, m.ny, m.nz, 3, 2, uvw));
+            return true;
+        }
+    }
+    return false;
+}
+
+// Medium::tr along a -> b: sigma_a-only closed form (homogeneous.cpp:60-73),
+// ratio tracking (hetergeneous.cpp:74-106)
+template<class S>
+static Vec med_tr(const Med &m, Vec a, Vec b, S &smp)
+{
+    if(m.type == BTRC_MEDIUM_HOMOGENEOUS)
+    {
+        const Vec sa = sclf(m.sigma_t, sub(mk(1, 1, 1), m.albedo));
+        const float l = -len(sub(a, b));
+        return mk(dm::exp(l * sa.x), dm::exp(l * sa.y), dm::exp(l * sa.z));
+    }
+    const float tmax = len(sub(b, a));
+    const Vec la = add(mk(0.5f, 0.5f, 0.5f), sclf(0.5f, med_uvw(m, a)));
+    const Vec lb = add(mk(0.5f, 0.5f, 0.5f), sclf(0.5f, med_uvw(m, b)));
+    const Vec lstep = dvs(sub(lb, la), tmax);
+    float tr = 1.0f, t = 0.0f;
+    for(int it = 0; it < MAX_TRACKING; ++it)
+    {
+        t = t + -dm::log(1.0f - smp.get1d()) * m.inv_maj;
+        if(t >= tmax)
+            break;
+        const Vec uvw = add(la, sclf(t, lstep));
+        tr = tr * (1.0f - tex3(m.dens, m.nx, m.ny, m.nz, 1, 0, uvw) * m.inv_maj);
+    }
+    return mk(tr, tr, tr);
 }
 
 // ------------------------------------------------------------------ path tracer
@@ -1441,11 +1679,75 @@ static void trace_path(const Scene &s, const Params &p, uint64_t path_index, std
     Vec o = p.eye, d = nrm(sub(dst, p.eye));
     Vec beta = mk(1, 1, 1), rad = mk(0, 0, 0), beta_le = mk(1, 1, 1);
     float bsdf_pdf = -1.0f;
+    uint32_t ray_med = (uint32_t)s.R; // the volume-primitive medium (generate.cpp:84)
+
+    // shadow ray: occlusion over all geometry, then the transmittance of its
+    // medium on a miss (shadow.cpp:43-71)
+    auto shadow = [&](Vec so, Vec sd, float t1, uint32_t med, Vec bl) {
+        ++shadow_rays;
+        const uint64_t seed = shadow_seed(hs.pack(pixel));
+        if(trace(s, BTRC_TRI_WATERTIGHT, true, true, so, sd, 0.0f, t1).gid != 0xffffffffu)
+            return;
+        if(!(t1 > 1.0f) && med < (uint32_t)s.R)
+        {
+            Pcg tr_rng;
+            tr_rng.seed(seed, mixbits(seed));
+            bl = mul(bl, med_tr(s.media[med], so, add(so, sclf(t1, sd)), tr_rng));
+        }
+        splat(film, pixel, bl);
+    };
+    // simple_russian_roulette, helper.cpp:67-97; true: the path ends
+    auto roulette = [&](int depth) {
+        if(depth < p.min_depth)
+            return false;
+        if(depth >= p.max_depth)
+            return true;
+        const float u = hs.get1d();
+        if(lum(beta) < p.rr_thr)
+        {
+            if(u > p.rr_p)
+                return true;
+            beta = dvs(beta, p.rr_p);
+        }
+        return false;
+    };
+
     for(int depth = 0;; ++depth)
     {
         ++ext_rays;
         const Hit h = trace(s, BTRC_TRI_WATERTIGHT, false, true, o, d, 0.0f, FMAX);
-        if(h.gid == 0xffffffffu)
+        const bool hit = h.gid != 0xffffffffu;
+
+        // medium stage (wavefront/medium.cpp:28-162): segment, delta tracking,
+        // transmittance into beta_le
+        bool scattered = false;
+        Vec mpos{}, mcolor{};
+        float mg = 0.0f;
+        uint32_t med = 0;
+        if(s.R > 0)
+        {
+            Vec end = o;
+            if(hit)
+            {
+                const Inst &in = s.insts[s.tinst[h.gid]];
+                const Vec nor = xnor(in.xf, s.gz[s.geos[in.geo].off + s.tprim[h.gid]]);
+                const uint32_t im = dot(d, nor) < 0.0f ? in.outer : in.inner;
+                med = std::min(im, ray_med);
+                end = offset_dir(add(o, sclf(h.t, d)), nor, neg(d));
+            }
+            else
+                med = (uint32_t)s.R;
+            ray_med = med;
+            if(med < (uint32_t)s.R)
+            {
+                const Med &m = s.media[med];
+                scattered = med_sample(m, o, end, hs, mpos, mcolor);
+                mg = m.g;
+                beta_le = mul(beta_le, med_tr(m, o, end, hs));
+            }
+        }
+
+        if(!hit)
         {
             // eval_miss_le, helper.cpp:144-172
             Vec sp = mk(0, 0, 0);
@@ -1460,57 +1762,96 @@ static void trace_path(const Scene &s, const Params &p, uint64_t path_index, std
                     sp = dvs(mul(beta_le, le), bsdf_pdf + lp);
                 }
             }
-            splat(film, pixel, add(sp, rad));
-            return;
-        }
-        const Inst &in = s.insts[s.tinst[h.gid]];
-        const SP sp = hitinfo(s, o, d, in, s.tprim[h.gid], h.t, h.u, h.v);
-        // handle_intersected_light, helper.cpp:174-209
-        if(in.light >= 0 && s.lights[in.light].type == 0)
-        {
-            const Light &l = s.lights[in.light];
-            const Vec le = dot(sp.frame.z, neg(d)) >= 0.0f ? l.intensity : mk(0, 0, 0);
-            Vec c;
-            if(bsdf_pdf < 0.0f)
-                c = dvs(mul(beta_le, le), -bsdf_pdf);
-            else
+            if(!scattered)
             {
-                float lpdf = 0.0f;
-                const Vec toref = sub(o, sp.pos);
-                if(dot(sp.frame.z, toref) > 0.0f)
-                {
-                    const float d2 = dot(toref, toref);
-                    const float d3 = std::sqrt(d2) * d2;
-                    lpdf = 1.0f / (l.scale * l.scale) * (1.0f / s.geos[l.geo].total_area) * d3
-                         / std::fabs(dot(sp.frame.z, toref));
-                }
-                c = dvs(mul(beta_le, le), bsdf_pdf + (1.0f / (float)s.lights.size()) * lpdf);
+                splat(film, pixel, add(sp, rad));
+                return;
             }
-            rad = add(rad, c);
+            splat(film, pixel, sp);
         }
-        // simple_russian_roulette, helper.cpp:67-97
-        if(depth >= p.min_depth)
+        const Inst *inp = hit ? &s.insts[s.tinst[h.gid]] : nullptr;
+        SP sp{};
+        if(hit)
         {
-            if(depth >= p.max_depth)
+            const Inst &in = *inp;
+            sp = hitinfo(s, o, d, in, s.tprim[h.gid], h.t, h.u, h.v);
+            // handle_intersected_light, helper.cpp:174-209
+            Vec c = mk(0, 0, 0);
+            if(in.light >= 0 && s.lights[in.light].type == 0)
+            {
+                const Light &l = s.lights[in.light];
+                const Vec le = dot(sp.frame.z, neg(d)) >= 0.0f ? l.intensity : mk(0, 0, 0);
+                if(bsdf_pdf < 0.0f)
+                    c = dvs(mul(beta_le, le), -bsdf_pdf);
+                else
+                {
+                    float lpdf = 0.0f;
+                    const Vec toref = sub(o, sp.pos);
+                    if(dot(sp.frame.z, toref) > 0.0f)
+                    {
+                        const float d2 = dot(toref, toref);
+                        const float d3 = std::sqrt(d2) * d2;
+                        lpdf = 1.0f / (l.scale * l.scale) * (1.0f / s.geos[l.geo].total_area) * d3
+                             / std::fabs(dot(sp.frame.z, toref));
+                    }
+                    c = dvs(mul(beta_le, le), bsdf_pdf + (1.0f / (float)s.lights.size()) * lpdf);
+                }
+            }
+            if(scattered)
+                splat(film, pixel, c);
+            else
+                rad = add(rad, c);
+        }
+
+        if(scattered)
+        {
+            // the medium vertex (medium.cpp:86-160)
+            if(roulette(depth))
             {
                 splat(film, pixel, rad);
                 return;
             }
-            const float u = hs.get1d();
-            if(lum(beta) < p.rr_thr)
+            float l0, l1, l2;
+            hs.get3d(l0, l1, l2);
+            const float lsel = hs.get1d();
+            Shadow sh = sample_light(s, mpos, mk(0, 0, 0), false, l0, l1, l2, lsel);
+            Vec bl{};
+            if(sh.ok)
             {
-                if(u > p.rr_p)
-                {
-                    splat(film, pixel, rad);
-                    return;
-                }
-                beta = dvs(beta, p.rr_p);
+                const float pv = hg(mg, -dot(nrm(sh.d), nrm(neg(d))));
+                bl = dvs(mul(mul(sh.li, beta), scl(mcolor, pv)), pv + sh.pdf);
             }
+            float s0, s1, s2;
+            hs.get3d(s0, s1, s2);
+            Vec pdir;
+            float ppdf;
+            const Vec phase = hg_sample(mg, mcolor, neg(d), s0, s1, pdir, ppdf);
+            if(sh.ok)
+                shadow(sh.o, sh.d, sh.t1, sh.t1 > 1.0f ? (uint32_t)s.R : med, bl);
+            if(zero(phase))
+            {
+                splat(film, pixel, rad);
+                return;
+            }
+            d = nrm(pdir);
+            beta_le = mul(beta, phase);
+            beta = dvs(beta_le, ppdf);
+            bsdf_pdf = ppdf;
+            o = mpos;
+            ray_med = med;
+            continue;
+        }
+
+        const Inst &in = *inp;
+        if(roulette(depth))
+        {
+            splat(film, pixel, rad);
+            return;
         }
         float l0, l1, l2;
         hs.get3d(l0, l1, l2);
         const float lsel = hs.get1d();
-        Shadow sh = sample_light(s, sp.pos, sp.frame.z, l0, l1, l2, lsel);
+        Shadow sh = sample_light(s, sp.pos, sp.frame.z, true, l0, l1, l2, lsel);
 
         const btrc_material_desc &m = s.mats[in.mat];
         Shd shd;
@@ -1518,14 +1859,16 @@ static void trace_path(const Scene &s, const Params &p, uint64_t path_index, std
         shd.sh = rot_z(sp.frame, sp.iz);
         shd.st = m.shadow_terminator != 0;
         shd.disney = m.type == BTRC_MATERIAL_DISNEY;
-        shd.albedo = mk(m.color[0], m.color[1], m.color[2]);
+        shd.invisible = m.type == BTRC_MATERIAL_INVISIBLE;
+        shd.albedo = shd.invisible ? mk(1, 1, 1) : mk(m.color[0], m.color[1], m.color[2]);
         if(shd.disney)
             shd.d = dis_make(m);
         if(aov && depth == 0)
         {
             // first-hit g-buffer: albedo = material colour, normal = shading z
-            // (shade.cpp:133-139,196-204; aggregate.cpp:211-219)
-            const Vec gb[2] = { shd.albedo, shd.sh.z };
+            // (shade.cpp:133-139,196-204; aggregate.cpp:211-219); an invisible
+            // surface reports one and its geometric normal (invisible.cpp:41-55)
+            const Vec gb[2] = { shd.albedo, shd.invisible ? shd.geo.z : shd.sh.z };
             for(int k = 0; k < 2; ++k)
                 if(finite3(gb[k]))
                 {
@@ -1546,9 +1889,10 @@ static void trace_path(const Scene &s, const Params &p, uint64_t path_index, std
                 const float pb = shd_pdf(shd, sh.d, neg(d));
                 const float c = std::fabs(dot(nrm(sh.d), sp.frame.z));
                 const Vec bl = dvs(scl(mul(mul(sh.li, beta), f), c), pb + sh.pdf);
-                ++shadow_rays;
-                if(trace(s, BTRC_TRI_WATERTIGHT, true, true, sh.o, sh.d, 0.0f, sh.t1).gid == 0xffffffffu)
-                    splat(film, pixel, bl);
+                // shade.cpp:166-180: the medium on the shadow ray's side
+                const bool sh_out = dot(sp.frame.z, sh.d) > 0.0f, last_out = dot(sp.frame.z, d) < 0.0f;
+                const uint32_t smed = sh_out == last_out ? ray_med : (sh_out ? in.outer : in.inner);
+                shadow(sh.o, sh.d, sh.t1, smed, bl);
             }
         }
         if(zero(bs.f))
@@ -1560,8 +1904,9 @@ static void trace_path(const Scene &s, const Params &p, uint64_t path_index, std
         const float c = std::fabs(dot(dir, sp.frame.z));
         beta_le = mul(scl(beta, c), bs.f);
         beta = dvs(beta_le, bs.pdf);
-        bsdf_pdf = bs.pdf;
+        bsdf_pdf = bs.delta ? -bs.pdf : bs.pdf;
         o = offset_dir(sp.pos, sp.frame.z, dir);
+        ray_med = dot(sp.frame.z, dir) > 0.0f ? in.outer : in.inner;
         d = dir;
     }
 }
@@ -1719,6 +2064,48 @@ void oracle_gaussian_filter(float radius, float alpha, const float *u, int32_t n
     orc::gaussian_table(p);
     for(int32_t i = 0; i < n; ++i)
         orc::gaussian_sample(p, u[3 * i], u[3 * i + 1], u[3 * i + 2], out[2 * i], out[2 * i + 1]);
+}
+
+void oracle_medium_eval(void *sp, int32_t medium, const btrc_ray *seg, const uint64_t *seeds, int32_t n, float *out)
+{
+    const orc::Scene &s = *static_cast<orc::Scene *>(sp);
+    const orc::Med &m = s.media[medium];
+    for(int32_t i = 0; i < n; ++i)
+    {
+        const orc::Vec a = orc::mk(seg[i].o[0], seg[i].o[1], seg[i].o[2]);
+        const orc::Vec d = orc::mk(seg[i].d[0], seg[i].d[1], seg[i].d[2]);
+        const orc::Vec b = orc::add(a, orc::scl(d, seg[i].tmax));
+        orc::Pcg r;
+        r.seed(seeds[i], orc::mixbits(seeds[i]));
+        orc::Vec pos = orc::mk(0, 0, 0), col = orc::mk(0, 0, 0);
+        const bool sc = orc::med_sample(m, a, b, r, pos, col);
+        const orc::Vec tr = orc::med_tr(m, a, b, r);
+        float *o = out + 10 * (size_t)i;
+        o[0] = sc ? 1.0f : 0.0f;
+        o[1] = pos.x; o[2] = pos.y; o[3] = pos.z;
+        o[4] = col.x; o[5] = col.y; o[6] = col.z;
+        o[7] = tr.x; o[8] = tr.y; o[9] = tr.z;
+    }
+}
+
+void oracle_hg(float g, const float *u, int32_t n, float *out)
+{
+    // out: dir xyz and pdf for wo = +z, then the pdf re-evaluated from dir
+    for(int32_t i = 0; i < n; ++i)
+    {
+        orc::Vec dir;
+        float pdf;
+        orc::hg_sample(g, orc::mk(1, 1, 1), orc::mk(0, 0, 1), u[2 * i], u[2 * i + 1], dir, pdf);
+        out[5 * i + 0] = dir.x; out[5 * i + 1] = dir.y; out[5 * i + 2] = dir.z;
+        out[5 * i + 3] = pdf;
+        out[5 * i + 4] = orc::hg(g, -orc::dot(orc::nrm(dir), orc::mk(0, 0, 1)));
+    }
+}
+
+float oracle_tex3(const float *grid, int32_t nx, int32_t ny, int32_t nz, float u, float v, float w)
+{
+    const std::vector<float> g(grid, grid + (size_t)nx * ny * nz);
+    return orc::tex3(g, nx, ny, nz, 1, 0, orc::mk(u, v, w));
 }
 
 uint64_t oracle_murmur64a(const uint8_t *key, uint64_t len, uint64_t seed)

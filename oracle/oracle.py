@@ -36,6 +36,10 @@ def lib():
                                     C.c_uint64, vp, vp, C.c_int32, P(C.c_uint64), P(C.c_uint64)]
         L.oracle_independent.argtypes = [vp, vp, vp, C.c_int32, C.c_int32, vp]
         L.oracle_gaussian_filter.argtypes = [C.c_float, C.c_float, vp, C.c_int32, vp]
+        L.oracle_medium_eval.argtypes = [vp, C.c_int32, vp, vp, C.c_int32, vp]
+        L.oracle_hg.argtypes = [C.c_float, vp, C.c_int32, vp]
+        L.oracle_tex3.restype = C.c_float
+        L.oracle_tex3.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32, C.c_float, C.c_float, C.c_float]
         L.oracle_murmur64a.restype = C.c_uint64
         L.oracle_murmur64a.argtypes = [vp, C.c_uint64, C.c_uint64]
         L.oracle_pcg32.argtypes = [C.c_uint64, C.c_uint64, C.c_uint64, C.c_int32, vp]
@@ -82,6 +86,14 @@ class OracleScene:
         except Exception:
             pass
 
+    def medium_eval(self, medium, segments, seeds):
+        """(n, 10) float32 like Context.eval_medium"""
+        seg = np.ascontiguousarray(segments, dtype=_abi.ray_dtype())
+        seeds = np.ascontiguousarray(seeds, dtype=np.uint64)
+        out = np.empty((len(seg), 10), dtype=np.float32)
+        lib().oracle_medium_eval(self.h, medium, seg.ctypes.data, seeds.ctypes.data, len(seg), out.ctypes.data)
+        return out
+
     def trace(self, rays, tri_test=_abi.TRI_WATERTIGHT, any_hit=False, use_bvh=True, threads=8):
         rays = np.ascontiguousarray(rays, dtype=_abi.ray_dtype())
         n = len(rays)
@@ -116,6 +128,22 @@ class OracleScene:
         if aov:
             return film, e.value, s.value, aovs[..., :3], aovs[..., 3:]
         return film, e.value, s.value
+
+
+def hg_sample(g, u):
+    """Henyey-Greenstein samples for wo = +z: (n, 5) dir xyz, pdf, pdf
+    re-evaluated from the direction"""
+    u = np.ascontiguousarray(u, dtype=np.float32)
+    out = np.empty((len(u), 5), dtype=np.float32)
+    lib().oracle_hg(g, u.ctypes.data, len(u), out.ctypes.data)
+    return out
+
+
+def tex3(grid, uvw):
+    """linear lookup of a [z, y, x] grid at normalised coordinates"""
+    grid = np.ascontiguousarray(grid, dtype=np.float32)
+    nz, ny, nx = grid.shape
+    return np.array([lib().oracle_tex3(grid.ctypes.data, nx, ny, nz, *map(float, p)) for p in uvw], dtype=np.float32)
 
 
 def independent(px, py, sample_index, dims):
