@@ -80,6 +80,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--spp", type=int, default=SPP)
+    ap.add_argument("--workload", choices=["field", "media"], default="field",
+                    help="field: the 1M-triangle scene (default); media: the Cornell-style room with a "
+                         "128^3 heterogeneous medium, max depth 16")
+    ap.add_argument("--gt-spp", type=int, default=1024,
+                    help="samples per pixel of the FP64 ground truth for relMSE (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -89,8 +94,14 @@ def main():
     import btrc_amd
     from btrc_amd import scenes, _abi
 
-    log("bench: building the 1M-triangle scene")
-    scene, cam = scenes.sphere_field()
+    media = args.workload == "media"
+    max_depth = 16 if media else MAX_DEPTH
+    if media:
+        log("bench: building the media scene (128^3 fBm grid)")
+        scene, cam = scenes.cornell_media()
+    else:
+        log("bench: building the 1M-triangle scene")
+        scene, cam = scenes.sphere_field()
     ctx = btrc_amd.Context(0)
     t0 = time.perf_counter()
     ctx.upload_scene(scene)
@@ -101,7 +112,7 @@ def main():
 
     p = _abi.RenderParams()
     p.width, p.height, p.spp = WIDTH, HEIGHT, args.spp
-    p.min_depth, p.max_depth = MIN_DEPTH, MAX_DEPTH
+    p.min_depth, p.max_depth = MIN_DEPTH, max_depth
     p.rr_threshold, p.rr_cont_prob = 0.1, 0.5
     p.state_count = STATE_COUNT
     p.time_stages = 1
@@ -122,6 +133,22 @@ def main():
         log(f"bench: step {i}: {st.seconds:.3f} s, {(st.extension_rays + st.shadow_rays) / st.seconds / 1e6:.1f} "
             f"Mrays/s, stage ms {[round(x, 1) for x in st.stage_ms]}")
 
+    frame = ctx.film(WIDTH, HEIGHT)[0][..., :3].astype(np.float64)
+    quality = None
+    if args.gt_spp > 0:
+        # FP64 ground truth from the independent sampler, so its samples share
+        # nothing with the Halton samples of the timed frame
+        g = _abi.RenderParams.from_buffer_copy(p)
+        g.spp, g.film_fp64, g.time_stages, g.sampler = args.gt_spp, 1, 0, _abi.SAMPLER_INDEPENDENT
+        t0 = time.perf_counter()
+        ctx.render(g)
+        gt = ctx.film(WIDTH, HEIGHT, fp64=True)[1]
+        quality = dict(relmse=float(np.mean((frame - gt) ** 2 / (gt ** 2 + 1e-2))), spp=args.spp,
+                       gt_spp=args.gt_spp, gt_sampler="independent", gt_film="fp64",
+                       gt_seconds=round(time.perf_counter() - t0, 2))
+        log(f"bench: relMSE of the {args.spp} spp frame vs {args.gt_spp} spp FP64 ground truth: "
+            f"{quality['relmse']:.5f}")
+
     secs = totals["seconds"]
     mrays = (totals["ext"] + totals["shadow"]) / secs / 1e6
     samples = totals["paths"] / secs
@@ -129,7 +156,9 @@ def main():
 
     cpu = None
     bytes_per_ray = None
-    if not args.no_cpu:
+    if not args.no_cpu and media:
+        cpu = media_cpu_baseline(scene, cam, p, args.cpu_seconds)
+    elif not args.no_cpu:
         # rays of this frame for the work model and the CPU baseline
         log("bench: capturing rays of the first two bounces")
         pc = _abi.RenderParams.from_buffer_copy(p)
@@ -146,7 +175,10 @@ def main():
             cpu["isects_per_ray"] = float(np.mean(stats[1]))
 
     roofline = None
-    if bytes_per_ray is not None and trace_ms > 0:
+    if media:
+        # the medium stage dominates here; its roofline is not modelled yet
+        pass
+    elif bytes_per_ray is not None and trace_ms > 0:
         launches = max(1, int(totals["launches"][1]))
         rays_per_launch = totals["ext"] / launches
         launch_s = trace_ms * 1e-3 / launches
@@ -174,20 +206,45 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "procedural 1M-triangle field (4096 instances x 256-tri displaced sphere, "
-                               "Disney per instance, 64 sphere lights + constant env), 1920x1080, "
-                               f"{args.spp} spp, max depth {MAX_DEPTH}",
-                   "width": WIDTH, "height": HEIGHT, "spp": args.spp, "max_depth": MAX_DEPTH,
+        "config": {"workload": ("Cornell-style room scaled x30 holding a 40^3 invisible box of heterogeneous "
+                                "medium (128^3 fBm grid, max sigma_t 2, albedo 0.8, HG g 0.6), 1920x1080, "
+                                f"{args.spp} spp, max depth {max_depth}") if media else
+                               ("procedural 1M-triangle field (4096 instances x 256-tri displaced sphere, "
+                                "Disney per instance, 64 sphere lights + constant env), 1920x1080, "
+                                f"{args.spp} spp, max depth {MAX_DEPTH}"),
+                   "width": WIDTH, "height": HEIGHT, "spp": args.spp, "max_depth": max_depth,
                    "min_depth": MIN_DEPTH, "triangles": info["triangles"], "state_count": STATE_COUNT},
         "samples_per_s": round(samples, 1),
         "extension_rays": int(totals["ext"]),
         "shadow_rays": int(totals["shadow"]),
         "stage_ms": {k: round(float(v), 2) for k, v in zip(("generate", "trace", "shade", "shadow", "medium"),
                                                            totals["stage_ms"])},
+        "quality": quality,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+
+
+def media_cpu_baseline(scene, cam, p, budget_s):
+    """the oracle's scalar path tracer (a port of the same algorithm) on 16
+    host threads over the first paths of the frame, about budget_s seconds"""
+    from oracle import oracle
+    os_ = oracle.OracleScene(scene)
+    threads = min(16, os.cpu_count() or 1)
+    q = type(p).from_buffer_copy(p)
+    count, rays, dt = 1 << 14, 0, 0.0
+    done = 0
+    while dt < budget_s and done + count <= q.width * q.height * q.spp:
+        t0 = time.perf_counter()
+        _, e, s = os_.render(cam, q, first=done, count=count, threads=threads)
+        dt += time.perf_counter() - t0
+        rays += e + s
+        done += count
+        count *= 2
+    return dict(value=rays / dt / 1e6, unit="Mrays/s", cores=threads, kind="port",
+                sample=f"first {done} paths of the frame traced by the oracle's scalar path tracer "
+                       f"(oracle/oracle.cpp) on {threads} host threads, {dt:.1f} s")
 
 
 def flatten_instances(scene):
