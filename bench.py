@@ -121,15 +121,15 @@ def main():
         st = ctx.render(p)
         log(f"bench: warmup {i}: {st.seconds:.3f} s")
 
-    totals = dict(seconds=0.0, ext=0, shadow=0, paths=0, stage_ms=np.zeros(5), launches=np.zeros(5))
+    totals = dict(seconds=0.0, ext=0, shadow=0, paths=0, stage_ms=np.zeros(6), launches=np.zeros(6))
     for i in range(args.steps):
         st = ctx.render(p)
         totals["seconds"] += st.seconds
         totals["ext"] += st.extension_rays
         totals["shadow"] += st.shadow_rays
         totals["paths"] += st.paths
-        totals["stage_ms"] += np.array(st.stage_ms[:5])
-        totals["launches"] += np.array(st.stage_launches[:5])
+        totals["stage_ms"] += np.array(st.stage_ms[:6])
+        totals["launches"] += np.array(st.stage_launches[:6])
         log(f"bench: step {i}: {st.seconds:.3f} s, {(st.extension_rays + st.shadow_rays) / st.seconds / 1e6:.1f} "
             f"Mrays/s, stage ms {[round(x, 1) for x in st.stage_ms]}")
 
@@ -217,7 +217,7 @@ def main():
         "samples_per_s": round(samples, 1),
         "extension_rays": int(totals["ext"]),
         "shadow_rays": int(totals["shadow"]),
-        "stage_ms": {k: round(float(v), 2) for k, v in zip(("generate", "trace", "shade", "shadow", "medium"),
+        "stage_ms": {k: round(float(v), 2) for k, v in zip(("generate", "trace", "shade", "shadow", "medium", "cache"),
                                                            totals["stage_ms"])},
         "quality": quality,
         "roofline": roofline,

@@ -76,7 +76,23 @@ class RenderParams(C.Structure):
                 ("rr_threshold", C.c_float), ("rr_cont_prob", C.c_float),
                 ("state_count", C.c_int32), ("film_fp64", C.c_int32), ("time_stages", C.c_int32),
                 ("tile", C.c_int32), ("sampler", C.c_int32), ("albedo", C.c_int32), ("normal", C.c_int32),
-                ("reserved", C.c_int32 * 2)]
+                ("sample_offset", C.c_int32), ("accumulate", C.c_int32)]
+
+
+class NrcParams(C.Structure):
+    _fields_ = [("enabled", C.c_int32), ("train", C.c_int32), ("query_depth", C.c_int32),
+                ("train_stride", C.c_int32), ("steps", C.c_int32), ("learning_rate", C.c_float),
+                ("loss_scale", C.c_float), ("reserved", C.c_int32)]
+
+
+class NrcStats(C.Structure):
+    _fields_ = [("queries", C.c_int64), ("records", C.c_int64), ("steps_total", C.c_int64),
+                ("loss", C.c_double * 4), ("grad_norm", C.c_double * 4), ("skipped", C.c_int32),
+                ("reserved", C.c_int32)]
+
+
+NRC_PARAMS = 22528
+NRC_VERTEX_FLOATS = 16
 
 
 class FilterDesc(C.Structure):
@@ -86,7 +102,7 @@ class FilterDesc(C.Structure):
 class RenderStats(C.Structure):
     _fields_ = [("seconds", C.c_double), ("extension_rays", C.c_uint64), ("shadow_rays", C.c_uint64),
                 ("paths", C.c_uint64), ("iterations", C.c_uint64),
-                ("stage_ms", C.c_double * 5), ("stage_launches", C.c_uint64 * 5)]
+                ("stage_ms", C.c_double * 6), ("stage_launches", C.c_uint64 * 6)]
 
 
 class Tuning(C.Structure):
@@ -120,6 +136,8 @@ EXPORTS = [
     "btrc_hip_set_tuning", "btrc_hip_get_tuning",
     "btrc_hip_upload_scene", "btrc_hip_set_camera", "btrc_hip_scene_info", "btrc_hip_set_filter",
     "btrc_hip_render", "btrc_hip_film_readback", "btrc_hip_aov_readback", "btrc_hip_request_stop",
+    "btrc_hip_nrc_configure", "btrc_hip_nrc_reset", "btrc_hip_nrc_stats", "btrc_hip_nrc_get_weights",
+    "btrc_hip_nrc_set_weights", "btrc_hip_nrc_eval", "btrc_hip_nrc_gradients", "btrc_hip_nrc_fit",
     "btrc_hip_trace_closest", "btrc_hip_trace_any",
     "btrc_hip_batch_upload", "btrc_hip_batch_trace", "btrc_hip_batch_download",
     "btrc_hip_capture_rays", "btrc_hip_eval_halton", "btrc_hip_eval_independent", "btrc_hip_eval_medium",
@@ -146,6 +164,14 @@ def declare(lib):
         "btrc_hip_aov_readback": (C.c_int, [ctx, C.c_void_p, C.c_void_p]),
         "btrc_hip_film_readback": (C.c_int, [ctx, C.c_void_p, C.c_void_p]),
         "btrc_hip_request_stop": (C.c_int, [ctx]),
+        "btrc_hip_nrc_configure": (C.c_int, [ctx, P(NrcParams)]),
+        "btrc_hip_nrc_reset": (C.c_int, [ctx, C.c_uint64]),
+        "btrc_hip_nrc_stats": (C.c_int, [ctx, P(NrcStats)]),
+        "btrc_hip_nrc_get_weights": (C.c_int, [ctx, C.c_void_p]),
+        "btrc_hip_nrc_set_weights": (C.c_int, [ctx, C.c_void_p]),
+        "btrc_hip_nrc_eval": (C.c_int, [ctx, C.c_void_p, C.c_int32, C.c_void_p]),
+        "btrc_hip_nrc_gradients": (C.c_int, [ctx, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, P(C.c_double)]),
+        "btrc_hip_nrc_fit": (C.c_int, [ctx, C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_void_p]),
         "btrc_hip_trace_closest": (C.c_int, [ctx, C.c_void_p, C.c_int64, C.c_int32, C.c_void_p]),
         "btrc_hip_trace_any": (C.c_int, [ctx, C.c_void_p, C.c_int64, C.c_int32, C.c_void_p]),
         "btrc_hip_batch_upload": (C.c_int, [ctx, C.c_void_p, C.c_int64]),

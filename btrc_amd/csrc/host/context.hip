@@ -202,6 +202,7 @@ struct btrc_ctx
     DevBuf<float4> hit_tuv;
     DevBuf<float4> sh_o, sh_d, sh_beta;
     DevBuf<uint2> sh_seed;
+    DevBuf<uint32_t> sh_train;
     DevBuf<WaveCounters> counters;
     WaveCounters *host_counters = nullptr; // pinned
     DevBuf<float> film;
@@ -214,6 +215,27 @@ struct btrc_ctx
     // film filter
     FilterConsts filter{ FILTER_BOX, 0.0f, 0.0f, nullptr };
     DevBuf<AliasUnit> filter_table;
+
+    // world box of the scene's triangles (the cache's position frame)
+    float scene_lo[3] = { 0, 0, 0 }, scene_hi[3] = { 1, 1, 1 };
+
+    // neural radiance cache
+    struct Nrc
+    {
+        btrc_nrc_params params{ 0, 1, 2, 16, 4, 1e-2f, 128.0f, 0 };
+        bool ready = false;             // weights allocated and initialised
+        DevBuf<float> w, m, v, grad;
+        DevBuf<uint16_t> wb, wtb;
+        DevBuf<NrcQuery> queries;
+        DevBuf<NrcRecord> rec;
+        DevBuf<float> acc;
+        DevBuf<int32_t> counts;         // [0] queries, [1] valid records of a batch
+        DevBuf<uint16_t> xt, dzt;
+        DevBuf<double> loss_partial;
+        DevBuf<NrcStepStats> step_stats;
+        int64_t steps_total = 0;
+        btrc_nrc_stats last{};
+    } nrc;
 
     // traversal: per-lane stack overflow and the batch fetch counter
     DevBuf<uint32_t> ovf_code;
@@ -683,6 +705,22 @@ void upload_scene(btrc_ctx *ctx, const btrc_scene_desc *sd)
     BTRC_HIP_CHECK(hipStreamSynchronize(s));
 
     ctx->medium_count = R;
+    for(int a = 0; a < 3; ++a)
+    {
+        float lo = MAX_FLOAT, hi = -MAX_FLOAT;
+        for(size_t k = a; k < world.size(); k += 3)
+        {
+            lo = std::min(lo, world[k]);
+            hi = std::max(hi, world[k]);
+        }
+        if(!(hi > lo))
+        {
+            lo = world.empty() ? 0.0f : lo;
+            hi = lo + 1.0f;
+        }
+        ctx->scene_lo[a] = lo;
+        ctx->scene_hi[a] = hi;
+    }
     ctx->light_count = (int32_t)lights.size();
     ctx->envir_light_index = envir_index;
     ctx->instance_count = sd->instance_count;
@@ -773,6 +811,147 @@ struct StageTimer
     }
 };
 
+// ---- neural radiance cache
+
+constexpr int64_t NRC_RECORDS = 1 << 16;     // training vertices per frame
+constexpr int64_t NRC_BATCH = 1 << 14;       // vertices per optimizer step
+
+NrcWeights nrc_weights(btrc_ctx *ctx)
+{
+    auto &n = ctx->nrc;
+    return NrcWeights{ n.w.ptr, n.m.ptr, n.v.ptr, n.wb.ptr, n.wtb.ptr };
+}
+
+NrcBox nrc_box(btrc_ctx *ctx)
+{
+    NrcBox b;
+    for(int a = 0; a < 3; ++a)
+    {
+        b.lo[a] = ctx->scene_lo[a];
+        b.inv_extent[a] = 1.0f / (ctx->scene_hi[a] - ctx->scene_lo[a]);
+    }
+    return b;
+}
+
+void nrc_alloc(btrc_ctx *ctx)
+{
+    auto &n = ctx->nrc;
+    n.w.alloc(NRC_PARAMS);
+    n.m.alloc(NRC_PARAMS);
+    n.v.alloc(NRC_PARAMS);
+    n.grad.alloc(NRC_PARAMS);
+    n.wb.alloc(NRC_PARAMS);
+    n.wtb.alloc(NRC_PARAMS);
+    n.rec.alloc(NRC_RECORDS);
+    n.acc.alloc(4 * NRC_RECORDS);
+    n.counts.alloc(2);
+    n.xt.alloc((size_t)NRC_LAYERS * NRC_IN * NRC_RECORDS);
+    n.dzt.alloc((size_t)NRC_LAYERS * NRC_IN * NRC_RECORDS);
+    n.loss_partial.alloc(NRC_RECORDS / 32);
+    n.step_stats.alloc(64);
+}
+
+// uniform He initialisation: U(-sqrt(6 / fan_in), sqrt(6 / fan_in)); the
+// output layer uses sqrt(6 / (fan_in + fan_out)); padding rows are zero
+void nrc_reset(btrc_ctx *ctx, uint64_t seed)
+{
+    nrc_alloc(ctx);
+    auto &n = ctx->nrc;
+    std::vector<float> w(NRC_PARAMS, 0.0f);
+    Pcg32 rng = pcg32_seed(seed, mix_bits(seed));
+    for(int l = 0; l < NRC_LAYERS; ++l)
+    {
+        const bool out = l == NRC_LAYERS - 1;
+        const float a = out ? std::sqrt(6.0f / (float)(NRC_IN + NRC_OUT)) : std::sqrt(6.0f / (float)NRC_IN);
+        const int rows = out ? NRC_OUT : NRC_WIDTH;
+        for(int r = 0; r < rows; ++r)
+            for(int c = 0; c < NRC_IN; ++c)
+                w[nrc_layer_offset(l) + r * NRC_IN + c] = a * (2.0f * pcg32_uniform_float(rng) - 1.0f);
+    }
+    hipStream_t s = ctx->stream;
+    n.w.upload(w.data(), w.size(), s);
+    BTRC_HIP_CHECK(hipMemsetAsync(n.m.ptr, 0, NRC_PARAMS * sizeof(float), s));
+    BTRC_HIP_CHECK(hipMemsetAsync(n.v.ptr, 0, NRC_PARAMS * sizeof(float), s));
+    launch_nrc_publish(s, nrc_weights(ctx));
+    BTRC_HIP_CHECK(hipGetLastError());
+    BTRC_HIP_CHECK(hipStreamSynchronize(s));
+    n.steps_total = 0;
+    n.last = btrc_nrc_stats{};
+    n.ready = true;
+}
+
+void nrc_require(btrc_ctx *ctx)
+{
+    if(!ctx->nrc.ready)
+        nrc_reset(ctx, 1);
+}
+
+NrcAdam nrc_adam(btrc_ctx *ctx)
+{
+    // Adam settings of the cache paper: beta1 0.9, beta2 0.99, eps 1e-15
+    auto &n = ctx->nrc;
+    const double t = (double)(n.steps_total + 1);
+    NrcAdam a;
+    a.lr = n.params.learning_rate;
+    a.beta1 = 0.9f;
+    a.beta2 = 0.99f;
+    a.eps = 1e-15f;
+    a.inv_bias1 = (float)(1.0 / (1.0 - std::pow(0.9, t)));
+    a.inv_bias2 = (float)(1.0 / (1.0 - std::pow(0.99, t)));
+    a.loss_scale = n.params.loss_scale;
+    return a;
+}
+
+NrcScratch nrc_scratch(btrc_ctx *ctx, int64_t batch)
+{
+    auto &n = ctx->nrc;
+    (void)batch;
+    return NrcScratch{ n.xt.ptr, n.dzt.ptr, n.loss_partial.ptr };
+}
+
+// `steps` optimizer steps over the frame's records, NRC_BATCH at a time
+void nrc_train(btrc_ctx *ctx, int64_t first_batch, int32_t steps, int64_t batch)
+{
+    auto &n = ctx->nrc;
+    hipStream_t s = ctx->stream;
+    const int64_t batches = NRC_RECORDS / batch;
+    for(int32_t k = 0; k < steps; ++k)
+    {
+        NrcBatch b{ n.rec.ptr, ((first_batch + k) % batches) * batch, batch, n.acc.ptr, n.counts.ptr + 1,
+                    n.params.loss_scale };
+        launch_nrc_train_step(s, nrc_weights(ctx), nrc_box(ctx), b, nrc_scratch(ctx, batch), nrc_adam(ctx),
+                              n.grad.ptr, n.step_stats.ptr + k);
+        BTRC_HIP_CHECK(hipGetLastError());
+        ++n.steps_total;
+    }
+}
+
+void nrc_collect(btrc_ctx *ctx, int32_t steps)
+{
+    auto &n = ctx->nrc;
+    std::vector<NrcStepStats> st(std::max(steps, 0));
+    int32_t counts[2] = { 0, 0 };
+    BTRC_HIP_CHECK(hipMemcpyAsync(counts, n.counts.ptr, sizeof(counts), hipMemcpyDeviceToHost, ctx->stream));
+    if(steps > 0)
+        BTRC_HIP_CHECK(hipMemcpyAsync(st.data(), n.step_stats.ptr, steps * sizeof(NrcStepStats), hipMemcpyDeviceToHost,
+                                      ctx->stream));
+    BTRC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    btrc_nrc_stats r{};
+    r.queries = counts[0];
+    r.steps_total = n.steps_total;
+    for(int32_t k = 0; k < steps; ++k)
+    {
+        r.records += st[k].samples;
+        if(k < 4)
+        {
+            r.loss[k] = st[k].loss;
+            r.grad_norm[k] = st[k].grad_norm;
+        }
+        r.skipped += st[k].skipped;
+    }
+    n.last = r;
+}
+
 void render(btrc_ctx *ctx, const btrc_render_params *p, btrc_render_stats *stats, int32_t capture_bounces,
             btrc_ray *capture, int64_t capture_max, int64_t *capture_count)
 {
@@ -790,8 +969,21 @@ void render(btrc_ctx *ctx, const btrc_render_params *p, btrc_render_stats *stats
         throw InvalidArg("unknown sampler");
     const uint64_t pixels = (uint64_t)p->width * (uint64_t)p->height;
     const uint64_t total = pixels * (uint64_t)p->spp;
-    if(pixels > 0x7fffffffull || (uint64_t)p->spp * 31104ull > 0x7fffffffull)
+    if(p->sample_offset < 0)
+        throw InvalidArg("sample_offset must be non-negative");
+    if(pixels > 0x7fffffffull || ((uint64_t)p->spp + (uint64_t)p->sample_offset) * 31104ull > 0x7fffffffull)
         throw InvalidArg("film or spp too large");
+    const bool nrc_on = ctx->nrc.params.enabled != 0;
+    if(nrc_on)
+    {
+        if(p->spp != 1)
+            throw InvalidArg("the radiance cache renders one sample per pixel per frame");
+        if(ctx->medium_count > 0)
+            throw InvalidArg("the radiance cache does not support participating media");
+        if(p->max_depth + 1 > NRC_RECORDS)
+            throw InvalidArg("max_depth too large for the radiance cache");
+        nrc_require(ctx);
+    }
 
     hipStream_t s = ctx->stream;
     const size_t P = (size_t)p->state_count;
@@ -812,27 +1004,32 @@ void render(btrc_ctx *ctx, const btrc_render_params *p, btrc_render_stats *stats
     ctx->counters.alloc(1);
     if(!ctx->host_counters)
         BTRC_HIP_CHECK(hipHostMalloc((void **)&ctx->host_counters, sizeof(WaveCounters), hipHostMallocDefault));
+    // a render adds to the previous film when asked and the film matches
+    const bool keep = p->accumulate && ctx->film_w == p->width && ctx->film_h == p->height &&
+                      ctx->film_has64 == (p->film_fp64 != 0) && ctx->film_has_albedo == (p->albedo != 0) &&
+                      ctx->film_has_normal == (p->normal != 0) && ctx->film_spp > 0;
     ctx->film.alloc(4 * pixels);
     if(p->film_fp64)
         ctx->film64.alloc(3 * pixels);
-    BTRC_HIP_CHECK(hipMemsetAsync(ctx->film.ptr, 0, 4 * pixels * sizeof(float), s));
-    if(p->film_fp64)
-        BTRC_HIP_CHECK(hipMemsetAsync(ctx->film64.ptr, 0, 3 * pixels * sizeof(double), s));
     if(p->albedo)
-    {
         ctx->film_albedo.alloc(4 * pixels);
-        BTRC_HIP_CHECK(hipMemsetAsync(ctx->film_albedo.ptr, 0, 4 * pixels * sizeof(float), s));
-    }
     if(p->normal)
-    {
         ctx->film_normal.alloc(4 * pixels);
-        BTRC_HIP_CHECK(hipMemsetAsync(ctx->film_normal.ptr, 0, 4 * pixels * sizeof(float), s));
+    if(!keep)
+    {
+        BTRC_HIP_CHECK(hipMemsetAsync(ctx->film.ptr, 0, 4 * pixels * sizeof(float), s));
+        if(p->film_fp64)
+            BTRC_HIP_CHECK(hipMemsetAsync(ctx->film64.ptr, 0, 3 * pixels * sizeof(double), s));
+        if(p->albedo)
+            BTRC_HIP_CHECK(hipMemsetAsync(ctx->film_albedo.ptr, 0, 4 * pixels * sizeof(float), s));
+        if(p->normal)
+            BTRC_HIP_CHECK(hipMemsetAsync(ctx->film_normal.ptr, 0, 4 * pixels * sizeof(float), s));
     }
     ctx->film_has_albedo = p->albedo != 0;
     ctx->film_has_normal = p->normal != 0;
     ctx->film_w = p->width;
     ctx->film_h = p->height;
-    ctx->film_spp = p->spp;
+    ctx->film_spp = keep ? ctx->film_spp + p->spp : p->spp;
     ctx->film_has64 = p->film_fp64 != 0;
 
     WaveCounters init{};
@@ -852,6 +1049,22 @@ void render(btrc_ctx *ctx, const btrc_render_params *p, btrc_render_stats *stats
     rc.vol_prim_medium = (uint32_t)ctx->medium_count;
     rc.film_fp64 = p->film_fp64;
     rc.tile = p->tile != 0;
+    rc.sample_offset = p->sample_offset;
+    NrcView nv{ -1, 0, nullptr, nullptr, nullptr, nullptr };
+    if(nrc_on)
+    {
+        auto &n = ctx->nrc;
+        const int32_t slots = p->max_depth + 1;
+        rc.nrc_stride = std::max(1, n.params.train ? n.params.train_stride : 0);
+        rc.nrc_phase = p->sample_offset % rc.nrc_stride;
+        rc.nrc_max_paths = n.params.train ? (int32_t)(NRC_RECORDS / slots) : 0;
+        if(!n.params.train)
+            rc.nrc_stride = 0;
+        n.queries.alloc(std::max<size_t>(1, (size_t)pixels));
+        nv = NrcView{ n.params.query_depth, slots, n.queries.ptr, n.counts.ptr, n.rec.ptr, n.acc.ptr };
+        launch_nrc_clear(s, n.rec.ptr, NRC_RECORDS, n.acc.ptr, NRC_RECORDS);
+        BTRC_HIP_CHECK(hipMemsetAsync(n.counts.ptr, 0, 2 * sizeof(int32_t), s));
+    }
     rc.halton = make_halton_consts(p->width, p->height);
     rc.camera = ctx->camera;
     rc.filter = ctx->filter;
@@ -861,7 +1074,10 @@ void render(btrc_ctx *ctx, const btrc_render_params *p, btrc_render_stats *stats
     FilmView fv{ ctx->film.ptr, p->film_fp64 ? ctx->film64.ptr : nullptr, p->albedo ? ctx->film_albedo.ptr : nullptr,
                  p->normal ? ctx->film_normal.ptr : nullptr };
     HitSOA hit{ ctx->hit_flag.ptr, ctx->hit_tuv.ptr };
-    ShadowSOA sh{ ctx->sh_o.ptr, ctx->sh_d.ptr, ctx->sh_beta.ptr, media ? ctx->sh_seed.ptr : nullptr };
+    if(nrc_on)
+        ctx->sh_train.alloc(P);
+    ShadowSOA sh{ ctx->sh_o.ptr, ctx->sh_d.ptr, ctx->sh_beta.ptr, media ? ctx->sh_seed.ptr : nullptr,
+                  nrc_on ? ctx->sh_train.ptr : nullptr, nrc_on ? ctx->nrc.acc.ptr : nullptr };
 
     const int grid256 = ctx->persistent_grid(256, (int64_t)P);
     const TraceOverflow ovf = ctx->trace_overflow();
@@ -905,7 +1121,7 @@ void render(btrc_ctx *ctx, const btrc_render_params *p, btrc_render_stats *stats
         }
         if(media)
             timer.run(STAGE_MEDIUM, [&] { launch_medium(s, grid256, p->sampler, rc, sv, ctx->counters.ptr, path, hit, next, sh, fv); });
-        timer.run(STAGE_SHADE, [&] { launch_shade(s, grid256, p->sampler, rc, sv, ctx->counters.ptr, path, hit, next, sh, fv); });
+        timer.run(STAGE_SHADE, [&] { launch_shade(s, grid256, p->sampler, rc, sv, ctx->counters.ptr, path, hit, next, sh, fv, nv); });
         timer.run(STAGE_SHADOW, [&] { launch_shadow(s, ctx->trace_grid, ctx->tuning.any_postpone_leaves != 0, media, ts, sv, ctx->counters.ptr, sh, fv, ovf); });
         launch_end_iteration(s, ctx->counters.ptr);
         BTRC_HIP_CHECK(hipGetLastError());
@@ -919,8 +1135,28 @@ void render(btrc_ctx *ctx, const btrc_render_params *p, btrc_render_stats *stats
                 break;
         }
     }
+    int32_t nrc_steps = 0;
+    if(nrc_on)
+    {
+        // the cache answers the queries of this frame, then learns from its
+        // training paths
+        auto &n = ctx->nrc;
+        const int igrid = (int)std::max<int64_t>(1, std::min<int64_t>(2 * ctx->cu_count, ((int64_t)pixels + 127) / 128));
+        timer.run(STAGE_CACHE, [&] {
+            launch_nrc_infer(s, igrid, nrc_weights(ctx), nrc_box(ctx), n.queries.ptr, n.counts.ptr, ctx->film.ptr,
+                             p->film_fp64 ? ctx->film64.ptr : nullptr);
+            if(n.params.train)
+            {
+                nrc_steps = std::min(n.params.steps, 64);
+                nrc_train(ctx, 0, nrc_steps, NRC_BATCH);
+            }
+        });
+        BTRC_HIP_CHECK(hipGetLastError());
+    }
     BTRC_HIP_CHECK(hipStreamSynchronize(s));
     const auto t1 = std::chrono::steady_clock::now();
+    if(nrc_on)
+        nrc_collect(ctx, nrc_steps);
     if(capture_count)
         *capture_count = captured;
     if(stats)
@@ -1141,6 +1377,158 @@ int btrc_hip_aov_readback(btrc_ctx *ctx, float *albedo_rgba, float *normal_rgba)
 int btrc_hip_set_filter(btrc_ctx *ctx, const btrc_filter_desc *filter)
 {
     return guarded(ctx, [&] { set_filter(ctx, filter); });
+}
+
+int btrc_hip_nrc_configure(btrc_ctx *ctx, const btrc_nrc_params *params)
+{
+    return guarded(ctx, [&] {
+        if(!params)
+            throw InvalidArg("cache parameters are null");
+        const btrc_nrc_params &q = *params;
+        if(q.query_depth < 0 || q.train_stride < 1 || q.steps < 0 || q.steps > 64 || !(q.learning_rate > 0.0f) ||
+           !(q.loss_scale > 0.0f) || !std::isfinite(q.learning_rate) || !std::isfinite(q.loss_scale))
+            throw InvalidArg("bad cache parameters");
+        BTRC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        ctx->nrc.params = q;
+        if(q.enabled)
+            nrc_require(ctx);
+    });
+}
+
+int btrc_hip_nrc_reset(btrc_ctx *ctx, uint64_t seed)
+{
+    return guarded(ctx, [&] { nrc_reset(ctx, seed); });
+}
+
+int btrc_hip_nrc_stats(btrc_ctx *ctx, btrc_nrc_stats *stats)
+{
+    return guarded(ctx, [&] {
+        if(!stats)
+            throw InvalidArg("stats is null");
+        *stats = ctx->nrc.last;
+    });
+}
+
+int btrc_hip_nrc_get_weights(btrc_ctx *ctx, float *weights)
+{
+    return guarded(ctx, [&] {
+        if(!weights)
+            throw InvalidArg("weights is null");
+        nrc_require(ctx);
+        BTRC_HIP_CHECK(hipMemcpyAsync(weights, ctx->nrc.w.ptr, NRC_PARAMS * sizeof(float), hipMemcpyDeviceToHost,
+                                      ctx->stream));
+        BTRC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int btrc_hip_nrc_set_weights(btrc_ctx *ctx, const float *weights)
+{
+    return guarded(ctx, [&] {
+        if(!weights)
+            throw InvalidArg("weights is null");
+        nrc_require(ctx);
+        // the output layer's padding rows stay zero
+        std::vector<float> w(weights, weights + NRC_PARAMS);
+        for(int k = nrc_layer_offset(NRC_LAYERS - 1) + NRC_OUT * NRC_IN; k < NRC_PARAMS; ++k)
+            w[k] = 0.0f;
+        ctx->nrc.w.upload(w.data(), w.size(), ctx->stream);
+        launch_nrc_publish(ctx->stream, nrc_weights(ctx));
+        BTRC_HIP_CHECK(hipGetLastError());
+        BTRC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int btrc_hip_nrc_eval(btrc_ctx *ctx, const float *vertices, int32_t n, float *out)
+{
+    return guarded(ctx, [&] {
+        if(n < 0 || (n > 0 && (!vertices || !out)))
+            throw InvalidArg("bad cache query");
+        nrc_require(ctx);
+        if(n == 0)
+            return;
+        DevBuf<NrcVertex> dv;
+        DevBuf<float> dout;
+        dv.upload(reinterpret_cast<const NrcVertex *>(vertices), n, ctx->stream);
+        dout.alloc(3 * (size_t)n);
+        launch_nrc_eval(ctx->stream, nrc_weights(ctx), nrc_box(ctx), dv.ptr, n, dout.ptr);
+        BTRC_HIP_CHECK(hipGetLastError());
+        BTRC_HIP_CHECK(hipMemcpyAsync(out, dout.ptr, 3 * (size_t)n * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+        BTRC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+// records whose target is `targets`: throughput 1, nothing gathered before,
+// one training path per record carrying the target as its radiance sum
+void nrc_load_batch(btrc_ctx *ctx, const float *vertices, const float *targets, int32_t n)
+{
+    auto &nc = ctx->nrc;
+    std::vector<NrcRecord> rec(NRC_RECORDS);
+    std::vector<float> acc(4 * (size_t)NRC_RECORDS, 0.0f);
+    for(int32_t i = 0; i < n; ++i)
+    {
+        NrcRecord &r = rec[i];
+        std::memcpy(&r.v, vertices + 16 * (size_t)i, sizeof(NrcVertex));
+        r.beta[0] = r.beta[1] = r.beta[2] = 1.0f;
+        r.gathered[0] = r.gathered[1] = r.gathered[2] = 0.0f;
+        r.path = (uint32_t)i;
+        r.valid = 1;
+        for(int c = 0; c < 3; ++c)
+            acc[4 * (size_t)i + c] = targets[3 * (size_t)i + c];
+    }
+    nc.rec.upload(rec.data(), rec.size(), ctx->stream);
+    nc.acc.upload(acc.data(), acc.size(), ctx->stream);
+}
+
+int btrc_hip_nrc_gradients(btrc_ctx *ctx, const float *vertices, const float *targets, int32_t n, float *grad,
+                           double *loss)
+{
+    return guarded(ctx, [&] {
+        if(n <= 0 || n > NRC_RECORDS || !vertices || !targets || !grad)
+            throw InvalidArg("bad cache batch");
+        nrc_require(ctx);
+        nrc_load_batch(ctx, vertices, targets, n);
+        auto &nc = ctx->nrc;
+        const int64_t batch = ((int64_t)n + 63) / 64 * 64;
+        NrcBatch b{ nc.rec.ptr, 0, batch, nc.acc.ptr, nc.counts.ptr + 1, nc.params.loss_scale };
+        launch_nrc_gradients(ctx->stream, nrc_weights(ctx), nrc_box(ctx), b, nrc_scratch(ctx, batch), nc.grad.ptr);
+        BTRC_HIP_CHECK(hipGetLastError());
+        std::vector<double> partial(batch / 32);
+        BTRC_HIP_CHECK(hipMemcpyAsync(grad, nc.grad.ptr, NRC_PARAMS * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+        BTRC_HIP_CHECK(hipMemcpyAsync(partial.data(), nc.loss_partial.ptr, partial.size() * sizeof(double),
+                                      hipMemcpyDeviceToHost, ctx->stream));
+        BTRC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        for(int k = 0; k < NRC_PARAMS; ++k)
+            grad[k] /= nc.params.loss_scale;
+        if(loss)
+        {
+            double sum = 0.0;
+            for(double v : partial)
+                sum += v;
+            *loss = sum / (3.0 * (double)n);
+        }
+    });
+}
+
+int btrc_hip_nrc_fit(btrc_ctx *ctx, const float *vertices, const float *targets, int32_t n, int32_t steps,
+                     double *loss)
+{
+    return guarded(ctx, [&] {
+        if(n <= 0 || n > NRC_BATCH || steps < 0 || steps > 64 || !vertices || !targets)
+            throw InvalidArg("bad cache batch");
+        nrc_require(ctx);
+        nrc_load_batch(ctx, vertices, targets, n);
+        nrc_train(ctx, 0, steps, NRC_BATCH);
+        nrc_collect(ctx, steps);
+        if(loss)
+        {
+            std::vector<NrcStepStats> st(steps);
+            if(steps > 0)
+                BTRC_HIP_CHECK(hipMemcpy(st.data(), ctx->nrc.step_stats.ptr, steps * sizeof(NrcStepStats),
+                                         hipMemcpyDeviceToHost));
+            for(int32_t k = 0; k < steps; ++k)
+                loss[k] = st[k].loss;
+        }
+    });
 }
 
 int btrc_hip_request_stop(btrc_ctx *ctx)

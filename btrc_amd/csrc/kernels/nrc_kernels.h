@@ -1,0 +1,93 @@
+// Device records and launchers of the neural radiance cache.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "../core/nrc.h"
+
+namespace btrc {
+
+// network state: FP32 master weights and Adam moments, BF16 copies as
+// [out][in] and (per layer) [in][out]
+struct NrcWeights
+{
+    float *w;
+    float *m;
+    float *v;
+    uint16_t *w_bf16;
+    uint16_t *wt_bf16;
+};
+
+// a cache query made at the vertex that ends a path's prefix
+struct NrcQuery
+{
+    NrcVertex v;
+    float beta[3];   // throughput the vertex was reached with
+    uint32_t pixel;
+};
+
+// a training record: a vertex of an extended (training) path
+struct NrcRecord
+{
+    NrcVertex v;
+    float beta[3];      // throughput the vertex was reached with
+    float gathered[3];  // the path's radiance sum when the vertex was reached (its emission included)
+    uint32_t path;      // training path id: row of the radiance sums
+    uint32_t valid;
+};
+
+// what the shading stages need in cache mode
+struct NrcView
+{
+    int32_t query_depth;     // vertex depth answered by the cache (-1: no queries)
+    int32_t slots_per_path;  // record slots per training path (max depth + 1)
+    NrcQuery *queries;
+    int32_t *query_count;
+    NrcRecord *rec;
+    float *acc;              // radiance sum per training path (4 floats)
+};
+
+struct NrcBatch
+{
+    const NrcRecord *rec;
+    int64_t first;
+    int64_t batch;           // a multiple of 64
+    const float *acc;
+    int32_t *valid_count;
+    float loss_scale;
+};
+
+struct NrcScratch
+{
+    uint16_t *xt;            // [layer][64][batch] layer inputs
+    uint16_t *dzt;           // [layer][64][batch] layer output gradients
+    double *loss_partial;    // one per wavefront
+};
+
+struct NrcAdam
+{
+    float lr, beta1, beta2, eps;
+    float inv_bias1, inv_bias2; // 1 / (1 - beta^t)
+    float loss_scale;
+};
+
+struct NrcStepStats
+{
+    double loss;       // mean relative L2 over the batch's valid samples and channels
+    double grad_norm;  // L2 norm of the unscaled gradient
+    int32_t samples;
+    int32_t skipped;   // 1: non-finite gradient or empty batch, no update
+};
+
+void launch_nrc_infer(hipStream_t s, int grid, const NrcWeights &net, const NrcBox &box, const NrcQuery *q,
+                      const int32_t *count, float *film, double *film64);
+void launch_nrc_eval(hipStream_t s, const NrcWeights &net, const NrcBox &box, const NrcVertex *v, int32_t n,
+                     float *out);
+void launch_nrc_train_step(hipStream_t s, const NrcWeights &net, const NrcBox &box, const NrcBatch &b,
+                           const NrcScratch &sc, const NrcAdam &opt, float *grad, NrcStepStats *stats);
+void launch_nrc_gradients(hipStream_t s, const NrcWeights &net, const NrcBox &box, const NrcBatch &b,
+                          const NrcScratch &sc, float *grad);
+void launch_nrc_clear(hipStream_t s, NrcRecord *rec, int64_t nrec, float *acc, int64_t npaths);
+void launch_nrc_publish(hipStream_t s, const NrcWeights &net);
+
+} // namespace btrc

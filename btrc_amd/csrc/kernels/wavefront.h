@@ -9,6 +9,7 @@
 
 #include "../core/sampler.h"
 #include "../core/scene.h"
+#include "nrc_kernels.h"
 #include "trace.h"
 
 namespace btrc {
@@ -19,7 +20,7 @@ struct PathSOA
     float4 *ray_o;    // origin.xyz, medium id bits
     float4 *ray_d;    // direction.xyz, t1
     float4 *beta;     // throughput.xyz, depth bits
-    float4 *rad;      // path radiance.xyz, unused
+    float4 *rad;      // path radiance.xyz, training path id bits (cache mode) or ~0
     float4 *bsdf_le;  // beta_le.xyz, bsdf pdf (negative: delta lobe)
     uint4 *smp;       // pixel index, then Halton: dimension, index lo, hi;
                       // independent: PCG state lo, hi, unused
@@ -37,7 +38,11 @@ struct ShadowSOA
     float4 *d;        // direction.xyz, t1
     float4 *beta;     // beta_li.xyz, medium id bits
     uint2 *seed;      // transmittance stream seed (scenes with media), else null
+    uint32_t *train;  // training path id or ~0 (cache mode), else null
+    float *train_acc; // radiance sums of the training paths (cache mode)
 };
+
+constexpr uint32_t NO_TRAIN = 0xffffffffu;
 
 constexpr uint32_t PATH_FLAG_HAS_INTERSECTION = 1u << 30; // wavefront/common.h:12
 constexpr uint32_t PATH_FLAG_HAS_SCATTERING = 1u << 31;
@@ -97,6 +102,10 @@ struct RenderConsts
     uint32_t vol_prim_medium;
     int32_t film_fp64;
     int32_t tile;
+    int32_t sample_offset;  // first sample index of this render
+    // cache mode: paths whose pixel % nrc_stride == nrc_phase are training
+    // paths with id pixel / nrc_stride while that is below nrc_max_paths
+    int32_t nrc_stride, nrc_phase, nrc_max_paths;
     HaltonConsts halton;
     CameraConsts camera;
     FilterConsts filter;
@@ -124,7 +133,8 @@ enum : int
     STAGE_SHADE = 2,
     STAGE_SHADOW = 3,
     STAGE_MEDIUM = 4,
-    STAGE_COUNT = 5,
+    STAGE_CACHE = 5,
+    STAGE_COUNT = 6,
 };
 
 void launch_plan(hipStream_t s, WaveCounters *c, int32_t state_count);
@@ -133,7 +143,8 @@ void launch_generate(hipStream_t s, int grid, int sampler, const RenderConsts &r
 void launch_trace_extension(hipStream_t s, int grid, bool postpone, const TraceScene &ts, WaveCounters *c,
                             PathSOA path, HitSOA hit, const TraceOverflow &ovf);
 void launch_shade(hipStream_t s, int grid, int sampler, const RenderConsts &rc, const SceneView &sv,
-                  WaveCounters *c, PathSOA path, HitSOA hit, PathSOA next, ShadowSOA shadow, FilmView film);
+                  WaveCounters *c, PathSOA path, HitSOA hit, PathSOA next, ShadowSOA shadow, FilmView film,
+                  const NrcView &nrc);
 void launch_medium(hipStream_t s, int grid, int sampler, const RenderConsts &rc, const SceneView &sv,
                    WaveCounters *c, PathSOA path, HitSOA hit, PathSOA next, ShadowSOA shadow, FilmView film);
 void launch_shadow(hipStream_t s, int grid, bool postpone, bool media, const TraceScene &ts, const SceneView &sv,

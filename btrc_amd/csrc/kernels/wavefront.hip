@@ -194,6 +194,11 @@ __global__ void __launch_bounds__(256) k_generate(RenderConsts rc, const WaveCou
             pixel_index = (uint32_t)(path_index % pixels);
             sample_index = (int32_t)(path_index / pixels);
         }
+        sample_index += rc.sample_offset;
+        uint32_t train = NO_TRAIN;
+        if(rc.nrc_stride > 0 && pixel_index % (uint32_t)rc.nrc_stride == (uint32_t)rc.nrc_phase &&
+           pixel_index / (uint32_t)rc.nrc_stride < (uint32_t)rc.nrc_max_paths)
+            train = pixel_index / (uint32_t)rc.nrc_stride;
         const uint32_t px = pixel_index % (uint32_t)rc.width;
         const uint32_t py = pixel_index / (uint32_t)rc.width;
 
@@ -232,7 +237,7 @@ __global__ void __launch_bounds__(256) k_generate(RenderConsts rc, const WaveCou
         path.ray_o[s] = f4(eye, __uint_as_float(rc.vol_prim_medium));
         path.ray_d[s] = f4(dir, MAX_FLOAT);
         path.beta[s] = make_float4(1.0f, 1.0f, 1.0f, __int_as_float(0));
-        path.rad[s] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+        path.rad[s] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(train));
         path.bsdf_le[s] = make_float4(1.0f, 1.0f, 1.0f, -1.0f);
         path.smp[s] = smp.pack(pixel_index);
     }
@@ -277,10 +282,40 @@ __global__ void __launch_bounds__(TRACE_BLOCK) k_trace_extension(
 
 // ---------------------------------------------------------------- shade
 
+// the cache's view of a surface vertex
+__device__ __forceinline__ NrcVertex nrc_vertex(const SceneView &sv, const InstanceRec &inst, const SurfacePoint &sp,
+                                                V3 d)
+{
+    NrcVertex v;
+    float rough;
+    V3 dif, spe;
+    nrc_material_inputs(sv.materials[inst.material_id], &rough, &dif, &spe);
+    const V3 wo = -d;
+    v.p[0] = sp.position.x; v.p[1] = sp.position.y; v.p[2] = sp.position.z;
+    v.wo[0] = wo.x; v.wo[1] = wo.y; v.wo[2] = wo.z;
+    v.n[0] = sp.interp_z.x; v.n[1] = sp.interp_z.y; v.n[2] = sp.interp_z.z;
+    v.roughness = rough;
+    v.diffuse[0] = dif.x; v.diffuse[1] = dif.y; v.diffuse[2] = dif.z;
+    v.specular[0] = spe.x; v.specular[1] = spe.y; v.specular[2] = spe.z;
+    return v;
+}
+
+// adds v to a training path's radiance sum; a path has at most one thread in
+// any launch, so no atomics are needed
+__device__ __forceinline__ void train_add(float *acc, uint32_t tid, V3 v)
+{
+    if(!isfinite3(v))
+        return;
+    float *a = acc + 4 * (size_t)tid;
+    a[0] += v.x;
+    a[1] += v.y;
+    a[2] += v.z;
+}
+
 template<int SAMPLER>
 __global__ void __launch_bounds__(256) k_shade(
     RenderConsts rc, SceneView sv, WaveCounters *c, PathSOA path, HitSOA hit,
-    PathSOA next, ShadowSOA shadow, FilmView film)
+    PathSOA next, ShadowSOA shadow, FilmView film, NrcView nrc)
 {
     const int32_t n = c->active;
     const int32_t stride = gridDim.x * blockDim.x;
@@ -292,6 +327,9 @@ __global__ void __launch_bounds__(256) k_shade(
         float4 nx_o, nx_d, nx_beta, nx_rad, nx_le, sh_o, sh_d, sh_beta;
         uint4 nx_smp;
         uint2 sh_seed;
+        uint32_t train = NO_TRAIN;
+        bool push_query = false;
+        NrcQuery query;
 
         if(i < n)
         {
@@ -309,15 +347,19 @@ __global__ void __launch_bounds__(256) k_shade(
             V3 beta = xyz(beta_depth);
             const int32_t depth = __float_as_int(beta_depth.w);
             V3 path_rad = xyz(rad4);
+            train = __float_as_uint(rad4.w);
             const V3 beta_le = xyz(le4);
             const float bsdf_pdf = le4.w;
 
             if(!(flag & PATH_FLAG_HAS_INTERSECTION))
             {
-                V3 splat = eval_miss_le(sv, d, beta_le, bsdf_pdf);
+                const V3 miss = eval_miss_le(sv, d, beta_le, bsdf_pdf);
+                V3 splat = miss;
                 if(!(flag & PATH_FLAG_HAS_SCATTERING))
                     splat = splat + path_rad;
                 film_splat(film, pixel, splat);
+                if(train != NO_TRAIN)
+                    train_add(nrc.acc, train, miss);
             }
             else
             {
@@ -331,6 +373,33 @@ __global__ void __launch_bounds__(256) k_shade(
                 if(flag & PATH_FLAG_HAS_SCATTERING)
                 {
                     film_splat(film, pixel, le_rad);
+                    done = true;
+                }
+                if(!done && train != NO_TRAIN)
+                {
+                    // training path: the radiance sum so far, then the vertex
+                    // record (throughput before Russian roulette, so that the
+                    // target stays unbiased)
+                    train_add(nrc.acc, train, le_rad);
+                    if(depth < nrc.slots_per_path)
+                    {
+                        NrcRecord &r = nrc.rec[(size_t)train * nrc.slots_per_path + depth];
+                        r.v = nrc_vertex(sv, inst, sp, d);
+                        const float *a = nrc.acc + 4 * (size_t)train;
+                        r.beta[0] = beta.x; r.beta[1] = beta.y; r.beta[2] = beta.z;
+                        r.gathered[0] = a[0]; r.gathered[1] = a[1]; r.gathered[2] = a[2];
+                        r.path = train;
+                        r.valid = 1;
+                    }
+                }
+                else if(!done && depth == nrc.query_depth)
+                {
+                    // the prefix ends here: the cache answers for the rest
+                    push_query = true;
+                    query.v = nrc_vertex(sv, inst, sp, d);
+                    query.beta[0] = beta.x; query.beta[1] = beta.y; query.beta[2] = beta.z;
+                    query.pixel = pixel;
+                    film_splat(film, pixel, path_rad + le_rad);
                     done = true;
                 }
                 if(!done)
@@ -421,7 +490,7 @@ __global__ void __launch_bounds__(256) k_shade(
                         nx_o = f4(next_o, __uint_as_float(med));
                         nx_d = f4(dir, MAX_FLOAT);
                         nx_beta = f4(beta, __int_as_float(depth + 1));
-                        nx_rad = f4(path_rad, 1.0f);
+                        nx_rad = f4(path_rad, __uint_as_float(train));
                         nx_le = f4(new_beta_le, stored_pdf);
                         nx_smp = hs.pack(pixel);
                     }
@@ -439,6 +508,14 @@ __global__ void __launch_bounds__(256) k_shade(
             shadow.beta[si] = sh_beta;
             if(shadow.seed)
                 shadow.seed[si] = sh_seed;
+            if(shadow.train)
+                shadow.train[si] = train;
+        }
+        if(nrc.queries)
+        {
+            const int32_t qi = wave_push(nrc.query_count, push_query);
+            if(push_query)
+                nrc.queries[qi] = query;
         }
         const int32_t ni = wave_push(&c->next_active, push_next);
         if(push_next)
@@ -595,6 +672,8 @@ __global__ void __launch_bounds__(256) k_medium(
             shadow.d[si] = sh_d;
             shadow.beta[si] = sh_beta;
             shadow.seed[si] = sh_seed;
+            if(shadow.train)
+                shadow.train[si] = __float_as_uint(rad4.w);
         }
         const int32_t ni = wave_push(&c->next_active, push_next);
         if(push_next)
@@ -650,6 +729,12 @@ struct ShadowSink
                 }
             }
             film_splat(film, __float_as_uint(a.w), beta);
+            if(shadow.train)
+            {
+                const uint32_t tid = shadow.train[i];
+                if(tid != NO_TRAIN)
+                    train_add(shadow.train_acc, tid, beta);
+            }
         }
     }
 };
@@ -821,14 +906,15 @@ void launch_trace_extension(hipStream_t s, int grid, bool postpone, const TraceS
 }
 
 void launch_shade(hipStream_t s, int grid, int sampler, const RenderConsts &rc, const SceneView &sv,
-                  WaveCounters *c, PathSOA path, HitSOA hit, PathSOA next, ShadowSOA shadow, FilmView film)
+                  WaveCounters *c, PathSOA path, HitSOA hit, PathSOA next, ShadowSOA shadow, FilmView film,
+                  const NrcView &nrc)
 {
     if(sampler == SAMPLER_INDEPENDENT)
         hipLaunchKernelGGL(k_shade<SAMPLER_INDEPENDENT>, dim3(grid), dim3(256), 0, s, rc, sv, c, path, hit, next,
-                           shadow, film);
+                           shadow, film, nrc);
     else
         hipLaunchKernelGGL(k_shade<SAMPLER_HALTON>, dim3(grid), dim3(256), 0, s, rc, sv, c, path, hit, next, shadow,
-                           film);
+                           film, nrc);
 }
 
 void launch_medium(hipStream_t s, int grid, int sampler, const RenderConsts &rc, const SceneView &sv,

@@ -178,8 +178,39 @@ typedef struct btrc_render_params
     int32_t sampler;     /* BTRC_SAMPLER_*; the reference's wfpt uses INDEPENDENT */
     int32_t albedo;      /* Params::albedo: first-hit albedo output */
     int32_t normal;      /* Params::normal: first-hit shading normal output */
-    int32_t reserved[2];
+    int32_t sample_offset; /* first sample index (progressive frames) */
+    int32_t accumulate;  /* add to the previous render's film instead of clearing it */
 } btrc_render_params;
+
+/* Neural radiance cache (no counterpart in the reference; README.md:17
+ * lists it as future work). With `enabled`, every render (spp 1, no media)
+ * ends non-training paths at vertex depth `query_depth`, where the cache's
+ * prediction times the path throughput goes to the film; paths of one pixel
+ * in `train_stride` (rotating with sample_offset) are traced to the end and
+ * their vertices (up to 65536) train the cache after the frame: `steps`
+ * Adam steps over batches of 16384 vertices, relative L2 loss. */
+typedef struct btrc_nrc_params
+{
+    int32_t enabled;
+    int32_t train;          /* 1: train after each frame */
+    int32_t query_depth;    /* default 2 */
+    int32_t train_stride;   /* default 16 */
+    int32_t steps;          /* optimizer steps per frame, default 4 */
+    float learning_rate;    /* default 1e-2 */
+    float loss_scale;       /* gradient scale of the BF16 backward pass, default 128 */
+    int32_t reserved;
+} btrc_nrc_params;
+
+typedef struct btrc_nrc_stats
+{
+    int64_t queries;         /* cache queries of the last frame */
+    int64_t records;         /* training vertices of the last frame */
+    int64_t steps_total;     /* optimizer steps so far */
+    double loss[4];          /* per step of the last frame (relative L2, FP64 sum) */
+    double grad_norm[4];
+    int32_t skipped;         /* steps of the last frame skipped (non-finite gradient) */
+    int32_t reserved;
+} btrc_nrc_stats;
 
 /* FilmFilter (core/film_filter.h); GaussianFilter defaults radius 1.3,
  * alpha 1.6 (gaussian.cpp:68-76) */
@@ -197,8 +228,8 @@ typedef struct btrc_render_stats
     uint64_t shadow_rays;    /* visibility rays traced */
     uint64_t paths;          /* camera paths completed */
     uint64_t iterations;     /* wavefront bounces executed */
-    double stage_ms[5];      /* generate, trace, shade, shadow, medium (time_stages) */
-    uint64_t stage_launches[5];
+    double stage_ms[6];      /* generate, trace, shade, shadow, medium, cache (time_stages) */
+    uint64_t stage_launches[6];
 } btrc_render_stats;
 
 /* A ray of a traversal batch and its closest hit */
@@ -256,6 +287,28 @@ int btrc_hip_film_readback(btrc_ctx *ctx, float *rgba, double *rgb64);
 int btrc_hip_aov_readback(btrc_ctx *ctx, float *albedo_rgba, float *normal_rgba);
 /* thread-safe; the running render returns BTRC_ERR_STOPPED */
 int btrc_hip_request_stop(btrc_ctx *ctx);
+
+/* ---- neural radiance cache ---- */
+int btrc_hip_nrc_configure(btrc_ctx *ctx, const btrc_nrc_params *params);
+/* fresh weights (uniform He initialisation from PCG32 stream `seed`), zero
+ * Adam state */
+int btrc_hip_nrc_reset(btrc_ctx *ctx, uint64_t seed);
+int btrc_hip_nrc_stats(btrc_ctx *ctx, btrc_nrc_stats *stats);
+/* the 22528 stored FP32 weights: six layers [out][in], the output layer as
+ * 32 rows of which 3 are used */
+int btrc_hip_nrc_get_weights(btrc_ctx *ctx, float *weights);
+int btrc_hip_nrc_set_weights(btrc_ctx *ctx, const float *weights);
+/* network outputs for n vertex records (16 floats each: position,
+ * direction to the previous vertex, shading normal, roughness, diffuse,
+ * specular); out: 3 floats each */
+int btrc_hip_nrc_eval(btrc_ctx *ctx, const float *vertices, int32_t n, float *out);
+/* gradient of the mean relative L2 loss over n samples (n <= 65536) with
+ * respect to the 22528 weights, and the loss */
+int btrc_hip_nrc_gradients(btrc_ctx *ctx, const float *vertices, const float *targets, int32_t n, float *grad,
+                           double *loss);
+/* `steps` Adam steps on one batch (n <= 16384); per-step loss out (may be NULL) */
+int btrc_hip_nrc_fit(btrc_ctx *ctx, const float *vertices, const float *targets, int32_t n, int32_t steps,
+                     double *loss);
 
 /* ---- traversal on ray batches in host memory ---- */
 int btrc_hip_trace_closest(btrc_ctx *ctx, const btrc_ray *rays, int64_t n, int32_t tri_test, btrc_hit *hits);

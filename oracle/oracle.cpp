@@ -1911,6 +1911,101 @@ static void trace_path(const Scene &s, const Params &p, uint64_t path_index, std
     }
 }
 
+// ------------------------------------------------------------------ radiance cache
+
+// The cache network in FP64 (no reference counterpart): 64 inputs, five
+// hidden layers of 64 ReLU units, 3 linear outputs, no biases; weights as
+// six [rows][64] blocks (the output block has 32 rows, 3 used). With
+// `bf16`, the device's rounding points are reproduced: weights, inputs and
+// hidden activations, and the scaled backward gradients, rounded to BF16
+// (round to nearest even); the sums stay FP64.
+namespace nrc {
+
+const int IN = 64, W = 64, L = 6, OUT = 3, OUT_ROWS = 32;
+const int PARAMS = 5 * W * IN + OUT_ROWS * IN;
+
+static double bf16(double x)
+{
+    const float f = (float)x;
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    if((u & 0x7fffffffu) > 0x7f800000u)
+        return (double)f;
+    u = (u + 0x7fffu + ((u >> 16) & 1u)) & 0xffff0000u;
+    float r;
+    std::memcpy(&r, &u, 4);
+    return (double)r;
+}
+
+static int rows(int l) { return l == L - 1 ? OUT_ROWS : W; }
+static int offset(int l) { return l * W * IN; }
+
+// input encoding: 8 frequencies of the box-normalised position, direction,
+// normal, roughness, diffuse, specular, then constant ones up to 64
+static void encode(const float *v, const float *box, double *x)
+{
+    int k = 0;
+    float q[3];
+    for(int a = 0; a < 3; ++a)
+        q[a] = (v[a] - box[a]) * box[3 + a];
+    float sc = PI;
+    for(int f = 0; f < 8; ++f)
+    {
+        for(int a = 0; a < 3; ++a)
+        {
+            const float t = sc * q[a];
+            x[k++] = dm::sin(t);
+            x[k++] = dm::cos(t);
+        }
+        sc = sc * 2.0f;
+    }
+    for(int j = 3; j < 16; ++j)
+        x[k++] = v[j];
+    while(k < IN)
+        x[k++] = 1.0;
+}
+
+struct Act
+{
+    double x[L][IN];   // input of each layer
+    double y[OUT];
+};
+
+static void forward(const std::vector<double> &w, const float *v, const float *box, bool b16, Act &a)
+{
+    encode(v, box, a.x[0]);
+    if(b16)
+        for(int k = 0; k < IN; ++k)
+            a.x[0][k] = bf16(a.x[0][k]);
+    for(int l = 0; l < L; ++l)
+    {
+        const int n = l == L - 1 ? OUT : W;
+        for(int r = 0; r < n; ++r)
+        {
+            double z = 0.0;
+            for(int c = 0; c < IN; ++c)
+                z += w[offset(l) + r * IN + c] * a.x[l][c];
+            if(l == L - 1)
+                a.y[r] = z;
+            else
+            {
+                const double h = z > 0.0 ? z : 0.0;
+                a.x[l + 1][r] = b16 ? bf16(h) : h;
+            }
+        }
+    }
+}
+
+static std::vector<double> weights(const float *wf, bool b16)
+{
+    std::vector<double> w(PARAMS);
+    for(int k = 0; k < PARAMS; ++k)
+        w[k] = b16 ? bf16(wf[k]) : (double)wf[k];
+    return w;
+}
+
+} // namespace nrc
+
 } // namespace orc
 
 // ================================================================== C API
@@ -2106,6 +2201,70 @@ float oracle_tex3(const float *grid, int32_t nx, int32_t ny, int32_t nz, float u
 {
     const std::vector<float> g(grid, grid + (size_t)nx * ny * nz);
     return orc::tex3(g, nx, ny, nz, 1, 0, orc::mk(u, v, w));
+}
+
+void oracle_nrc_forward(const float *w, const float *box, const float *v, int32_t n, int32_t b16, double *out)
+{
+    const std::vector<double> wd = orc::nrc::weights(w, b16 != 0);
+    orc::nrc::Act a;
+    for(int32_t i = 0; i < n; ++i)
+    {
+        orc::nrc::forward(wd, v + 16 * (size_t)i, box, b16 != 0, a);
+        for(int c = 0; c < 3; ++c)
+            out[3 * (size_t)i + c] = a.y[c];
+    }
+}
+
+// gradient of mean_i,c (y - t)^2 / (y^2 + 0.01) (the prediction in the
+// denominator held constant) with respect to the weights, and the loss
+void oracle_nrc_gradients(const float *w, const float *box, const float *v, const float *t, int32_t n, int32_t b16,
+                          float loss_scale, double *grad, double *loss)
+{
+    using namespace orc::nrc;
+    const std::vector<double> wd = weights(w, b16 != 0);
+    std::fill(grad, grad + PARAMS, 0.0);
+    double lsum = 0.0;
+    Act a;
+    for(int32_t i = 0; i < n; ++i)
+    {
+        forward(wd, v + 16 * (size_t)i, box, b16 != 0, a);
+        double dz[IN] = {};
+        for(int c = 0; c < OUT; ++c)
+        {
+            const double y = a.y[c], tc = t[3 * (size_t)i + c];
+            const float yf = (float)y;
+            const double den = (double)(yf * yf + 0.01f);
+            lsum += (y - tc) * (y - tc) / den;
+            double g = 2.0 * (y - tc) / den / (3.0 * n);
+            if(b16)
+                g = bf16(g * loss_scale);
+            dz[c] = g;
+        }
+        for(int l = L - 1; l >= 0; --l)
+        {
+            const int nr = l == L - 1 ? OUT : W;
+            for(int r = 0; r < nr; ++r)
+                for(int c = 0; c < IN; ++c)
+                    grad[offset(l) + r * IN + c] += dz[r] * a.x[l][c];
+            if(l == 0)
+                break;
+            double dx[IN];
+            for(int c = 0; c < IN; ++c)
+            {
+                double sum = 0.0;
+                for(int r = 0; r < nr; ++r)
+                    sum += wd[offset(l) + r * IN + c] * dz[r];
+                const double g = a.x[l][c] > 0.0 ? sum : 0.0;
+                dx[c] = b16 ? bf16(g) : g;
+            }
+            for(int c = 0; c < IN; ++c)
+                dz[c] = dx[c];
+        }
+    }
+    if(b16)
+        for(int k = 0; k < PARAMS; ++k)
+            grad[k] /= loss_scale;
+    *loss = lsum / (3.0 * n);
 }
 
 uint64_t oracle_murmur64a(const uint8_t *key, uint64_t len, uint64_t seed)

@@ -40,6 +40,8 @@ def lib():
         L.oracle_hg.argtypes = [C.c_float, vp, C.c_int32, vp]
         L.oracle_tex3.restype = C.c_float
         L.oracle_tex3.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32, C.c_float, C.c_float, C.c_float]
+        L.oracle_nrc_forward.argtypes = [vp, vp, vp, C.c_int32, C.c_int32, vp]
+        L.oracle_nrc_gradients.argtypes = [vp, vp, vp, vp, C.c_int32, C.c_int32, C.c_float, vp, P(C.c_double)]
         L.oracle_murmur64a.restype = C.c_uint64
         L.oracle_murmur64a.argtypes = [vp, C.c_uint64, C.c_uint64]
         L.oracle_pcg32.argtypes = [C.c_uint64, C.c_uint64, C.c_uint64, C.c_int32, vp]
@@ -162,6 +164,32 @@ def gaussian_filter(radius, alpha, u):
     out = np.empty((len(u), 2), dtype=np.float32)
     lib().oracle_gaussian_filter(radius, alpha, u.ctypes.data, len(u), out.ctypes.data)
     return out
+
+
+NRC_PARAMS = 22528
+
+
+def nrc_forward(weights, box, vertices, bf16=True):
+    """cache network outputs (n, 3) float64; box = (lo xyz, 1 / extent xyz)"""
+    w = np.ascontiguousarray(weights, dtype=np.float32)
+    b = np.ascontiguousarray(box, dtype=np.float32)
+    v = np.ascontiguousarray(vertices, dtype=np.float32).reshape(-1, 16)
+    out = np.empty((len(v), 3), dtype=np.float64)
+    lib().oracle_nrc_forward(w.ctypes.data, b.ctypes.data, v.ctypes.data, len(v), 1 if bf16 else 0, out.ctypes.data)
+    return out
+
+
+def nrc_gradients(weights, box, vertices, targets, bf16=True, loss_scale=128.0):
+    """(gradient (22528,) float64, loss) of the mean relative L2 loss"""
+    w = np.ascontiguousarray(weights, dtype=np.float32)
+    b = np.ascontiguousarray(box, dtype=np.float32)
+    v = np.ascontiguousarray(vertices, dtype=np.float32).reshape(-1, 16)
+    t = np.ascontiguousarray(targets, dtype=np.float32).reshape(-1, 3)
+    g = np.empty(NRC_PARAMS, dtype=np.float64)
+    loss = C.c_double()
+    lib().oracle_nrc_gradients(w.ctypes.data, b.ctypes.data, v.ctypes.data, t.ctypes.data, len(v), 1 if bf16 else 0,
+                               loss_scale, g.ctypes.data, C.byref(loss))
+    return g, loss.value
 
 
 def murmur64a(key, seed):

@@ -38,7 +38,7 @@ def test_struct_sizes():
     assert C.sizeof(_abi.RenderParams) == 64
     assert C.sizeof(_abi.MaterialDesc) == 68
     assert C.sizeof(_abi.InstanceDesc) == 4 * (2 + 32 + 1 + 3 + 2)
-    assert C.sizeof(_abi.RenderStats) == 8 * (1 + 4 + 5 + 5)
+    assert C.sizeof(_abi.RenderStats) == 8 * (1 + 4 + 6 + 6)
 
 
 STRUCTS = {
@@ -47,7 +47,7 @@ STRUCTS = {
     "btrc_envir_light_desc": "EnvirLightDesc", "btrc_scene_desc": "SceneDesc",
     "btrc_camera_desc": "CameraDesc", "btrc_render_params": "RenderParams",
     "btrc_filter_desc": "FilterDesc", "btrc_render_stats": "RenderStats", "btrc_ray": "Ray",
-    "btrc_hit": "Hit", "btrc_tuning": "Tuning",
+    "btrc_hit": "Hit", "btrc_tuning": "Tuning", "btrc_nrc_params": "NrcParams", "btrc_nrc_stats": "NrcStats",
 }
 
 
