@@ -80,9 +80,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--spp", type=int, default=SPP)
-    ap.add_argument("--workload", choices=["field", "media"], default="field",
+    ap.add_argument("--workload", choices=["field", "media", "nrc"], default="field",
                     help="field: the 1M-triangle scene (default); media: the Cornell-style room with a "
-                         "128^3 heterogeneous medium, max depth 16")
+                         "128^3 heterogeneous medium, max depth 16; nrc: the Cornell-style room at 256^2, "
+                         "1 spp per frame for --frames frames, with the radiance cache against plain path "
+                         "tracing at equal time")
+    ap.add_argument("--frames", type=int, default=256)
     ap.add_argument("--gt-spp", type=int, default=1024,
                     help="samples per pixel of the FP64 ground truth for relMSE (0: skip)")
     args = ap.parse_args()
@@ -93,6 +96,10 @@ def main():
 
     import btrc_amd
     from btrc_amd import scenes, _abi
+
+    if args.workload == "nrc":
+        print(json.dumps(nrc_workload(args)), flush=True)
+        return
 
     media = args.workload == "media"
     max_depth = 16 if media else MAX_DEPTH
@@ -121,15 +128,15 @@ def main():
         st = ctx.render(p)
         log(f"bench: warmup {i}: {st.seconds:.3f} s")
 
-    totals = dict(seconds=0.0, ext=0, shadow=0, paths=0, stage_ms=np.zeros(6), launches=np.zeros(6))
+    totals = dict(seconds=0.0, ext=0, shadow=0, paths=0, stage_ms=np.zeros(7), launches=np.zeros(7))
     for i in range(args.steps):
         st = ctx.render(p)
         totals["seconds"] += st.seconds
         totals["ext"] += st.extension_rays
         totals["shadow"] += st.shadow_rays
         totals["paths"] += st.paths
-        totals["stage_ms"] += np.array(st.stage_ms[:6])
-        totals["launches"] += np.array(st.stage_launches[:6])
+        totals["stage_ms"] += np.array(st.stage_ms[:7])
+        totals["launches"] += np.array(st.stage_launches[:7])
         log(f"bench: step {i}: {st.seconds:.3f} s, {(st.extension_rays + st.shadow_rays) / st.seconds / 1e6:.1f} "
             f"Mrays/s, stage ms {[round(x, 1) for x in st.stage_ms]}")
 
@@ -217,13 +224,146 @@ def main():
         "samples_per_s": round(samples, 1),
         "extension_rays": int(totals["ext"]),
         "shadow_rays": int(totals["shadow"]),
-        "stage_ms": {k: round(float(v), 2) for k, v in zip(("generate", "trace", "shade", "shadow", "medium", "cache"),
+        "stage_ms": {k: round(float(v), 2) for k, v in zip(("generate", "trace", "shade", "shadow", "medium", "cache", "train"),
                                                            totals["stage_ms"])},
         "quality": quality,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+
+
+NRC_FLOP_PER_QUERY = 2 * (64 * 64 + 4 * 64 * 64 + 64 * 3)   # 41,344
+NRC_FLOP_PER_TRAIN_SAMPLE = 3 * NRC_FLOP_PER_QUERY            # forward, input gradients, weight gradients
+MFMA_BF16_PEAK_TFLOPS = 2500.0
+
+
+def nrc_workload(args):
+    """Cornell-style room at 256^2, `frames` frames of 1 spp accumulated:
+    plain path tracing, then the radiance cache (queries at depth 2, training
+    on 1/16 of the pixels), each timed per frame and scored by relMSE against
+    an FP64 ground truth of 4096 spp from the independent sampler."""
+    import btrc_amd
+    from btrc_amd import _abi, scenes
+    W = H = 256
+    scene, cam = scenes.cornell()
+    ctx = btrc_amd.Context(0)
+    ctx.upload_scene(scene)
+    ctx.set_camera(cam, W, H)
+
+    def params(spp, fp64=0):
+        p = _abi.RenderParams()
+        p.width, p.height, p.spp = W, H, spp
+        p.min_depth, p.max_depth = 3, 8
+        p.rr_threshold, p.rr_cont_prob = 0.1, 0.5
+        p.state_count = 1 << 17
+        p.film_fp64 = fp64
+        return p
+
+    g = params(4096, 1)
+    g.state_count = 1 << 20
+    g.sampler = _abi.SAMPLER_INDEPENDENT
+    ctx.render(g)
+    gt = ctx.film(W, H, fp64=True)[1]
+
+    def relmse():
+        img = ctx.film(W, H)[0][..., :3].astype(np.float64)
+        return float(np.mean((img - gt) ** 2 / (gt ** 2 + 1e-2)))
+
+    checkpoints = sorted({c for c in (1, 2, 4, 8, 16, 32, 64, 128, 256, 512, 1024) if c <= args.frames} | {args.frames})
+
+    def run(cache):
+        ctx.nrc_configure(enabled=1 if cache else 0, train=1)
+        if cache:
+            ctx.nrc_reset(1)
+        p = params(1)
+        p.time_stages = 1
+        curve, seconds, rays, stage = [], 0.0, 0, np.zeros(7)
+        queries = records = 0
+        losses = []
+        for f in range(args.frames):
+            p.sample_offset, p.accumulate = f, 1 if f else 0
+            st = ctx.render(p)
+            seconds += st.seconds
+            rays += st.extension_rays + st.shadow_rays
+            stage += np.array(st.stage_ms[:7])
+            if cache:
+                ns = ctx.nrc_stats()
+                queries += ns.queries
+                records += ns.records
+                losses.append(float(np.mean(ns.loss[:min(4, max(1, ns.steps_total))])))
+            if f + 1 in checkpoints:
+                curve.append((f + 1, round(seconds, 4), relmse()))
+        return dict(seconds=seconds, rays=rays, stage_ms=stage, curve=curve, queries=queries, records=records,
+                    losses=losses)
+
+    cpu = None if args.no_cpu else media_cpu_baseline(scene, cam, params(16), args.cpu_seconds)
+    log("bench: plain path tracing")
+    plain = run(False)
+    log("bench: path tracing with the radiance cache")
+    cache = run(True)
+    ctx.nrc_configure(enabled=0)
+
+    def time_to(curve, target):
+        for frames, t, e in curve:
+            if e <= target:
+                return t
+        return None
+
+    target = plain["curve"][-1][2]
+    t_cache = time_to(cache["curve"], target)
+    # relMSE of each method at the other's total time, from the checkpoints
+    def at_time(curve, t):
+        best = None
+        for frames, tt, e in curve:
+            if tt <= t:
+                best = e
+        return best
+
+    infer_ms = cache["stage_ms"][5]
+    train_ms = cache["stage_ms"][6]
+    steps = 4 * args.frames
+    infer_tflops = NRC_FLOP_PER_QUERY * cache["queries"] / (infer_ms * 1e-3) / 1e12 if infer_ms > 0 else None
+    train_tflops = NRC_FLOP_PER_TRAIN_SAMPLE * 16384 * steps / (train_ms * 1e-3) / 1e12 if train_ms > 0 else None
+    return {
+        "metric": METRIC,
+        "value": round(cache["rays"] / cache["seconds"] / 1e6, 2),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.frames,
+        "warmup": 0,
+        "ms_per_step": round(cache["seconds"] / args.frames * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 paths, bf16 network (fp32 accumulation)",
+        "data": "synthetic",
+        "config": {"workload": f"Cornell-style room, 256x256, 1 spp per frame x {args.frames} frames, max depth 8, "
+                               "radiance cache queried at depth 2, trained on 1/16 of the pixels (4 Adam steps of "
+                               "16384 records per frame)",
+                   "width": W, "height": H, "frames": args.frames},
+        "equal_time": {
+            "ground_truth": "4096 spp FP64, independent sampler",
+            "plain": {"seconds": round(plain["seconds"], 4), "relmse_curve": plain["curve"],
+                      "mrays_per_s": round(plain["rays"] / plain["seconds"] / 1e6, 2)},
+            "cache": {"seconds": round(cache["seconds"], 4), "relmse_curve": cache["curve"],
+                      "queries_per_frame": round(cache["queries"] / args.frames),
+                      "records_per_frame": round(cache["records"] / args.frames),
+                      "loss_first_last": [round(cache["losses"][0], 4), round(cache["losses"][-1], 4)]},
+            "target_relmse": target,
+            "cache_time_to_target_s": t_cache,
+            "time_ratio": None if t_cache is None else round(t_cache / plain["seconds"], 3),
+            "cache_relmse_at_plain_time": at_time(cache["curve"], plain["seconds"]),
+        },
+        "roofline": {"bound": "mfma", "kernel": "k_nrc_infer", "unit": "TFLOP/s", "peak": MFMA_BF16_PEAK_TFLOPS,
+                     "achieved": None if infer_tflops is None else round(infer_tflops, 4),
+                     "frac": None if infer_tflops is None else round(infer_tflops / MFMA_BF16_PEAK_TFLOPS, 7),
+                     "traffic": None, "flop_per_query": NRC_FLOP_PER_QUERY,
+                     "train_achieved": None if train_tflops is None else round(train_tflops, 4),
+                     "infer_ms_per_frame": round(infer_ms / args.frames, 4),
+                     "train_ms_per_frame": round(train_ms / args.frames, 4)},
+        "cpu_baseline": cpu,
+    }
 
 
 def media_cpu_baseline(scene, cam, p, budget_s):

@@ -100,6 +100,7 @@ class Context:
         desc, holder = scene.to_desc()
         self._check(self.lib.btrc_hip_upload_scene(self.h, C.byref(desc)))
         self._keep = holder
+        self._box = scene.world_box()
 
     def scene_info(self):
         n, d, s, t = C.c_int64(), C.c_int32(), C.c_int32(), C.c_int64()
@@ -142,6 +143,67 @@ class Context:
 
     def request_stop(self):
         self._check(self.lib.btrc_hip_request_stop(self.h))
+
+    # ---- neural radiance cache
+    NRC_DEFAULTS = dict(enabled=0, train=1, query_depth=2, train_stride=16, steps=4, learning_rate=1e-2,
+                        loss_scale=128.0)
+
+    def nrc_configure(self, **kw):
+        unknown = set(kw) - set(self.NRC_DEFAULTS)
+        if unknown:
+            raise BtrcException(_abi.BTRC_ERR_INVALID, f"unknown cache parameters: {sorted(unknown)}")
+        cfg = dict(self.NRC_DEFAULTS)
+        cfg.update(getattr(self, "_nrc_cfg", {}))
+        cfg.update(kw)
+        p = _abi.NrcParams()
+        for k, v in cfg.items():
+            setattr(p, k, v)
+        self._check(self.lib.btrc_hip_nrc_configure(self.h, C.byref(p)))
+        self._nrc_cfg = cfg
+
+    def nrc_reset(self, seed=1):
+        self._check(self.lib.btrc_hip_nrc_reset(self.h, seed))
+
+    def nrc_stats(self):
+        st = _abi.NrcStats()
+        self._check(self.lib.btrc_hip_nrc_stats(self.h, C.byref(st)))
+        return st
+
+    def nrc_weights(self):
+        w = np.empty(_abi.NRC_PARAMS, dtype=np.float32)
+        self._check(self.lib.btrc_hip_nrc_get_weights(self.h, w.ctypes.data))
+        return w
+
+    def nrc_set_weights(self, w):
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        assert w.size == _abi.NRC_PARAMS
+        self._check(self.lib.btrc_hip_nrc_set_weights(self.h, w.ctypes.data))
+
+    def nrc_eval(self, vertices):
+        v = np.ascontiguousarray(vertices, dtype=np.float32).reshape(-1, _abi.NRC_VERTEX_FLOATS)
+        out = np.empty((len(v), 3), dtype=np.float32)
+        self._check(self.lib.btrc_hip_nrc_eval(self.h, v.ctypes.data, len(v), out.ctypes.data))
+        return out
+
+    def nrc_gradients(self, vertices, targets):
+        v = np.ascontiguousarray(vertices, dtype=np.float32).reshape(-1, _abi.NRC_VERTEX_FLOATS)
+        t = np.ascontiguousarray(targets, dtype=np.float32).reshape(-1, 3)
+        g = np.empty(_abi.NRC_PARAMS, dtype=np.float32)
+        loss = C.c_double()
+        self._check(self.lib.btrc_hip_nrc_gradients(self.h, v.ctypes.data, t.ctypes.data, len(v), g.ctypes.data,
+                                                    C.byref(loss)))
+        return g, loss.value
+
+    def nrc_fit(self, vertices, targets, steps):
+        v = np.ascontiguousarray(vertices, dtype=np.float32).reshape(-1, _abi.NRC_VERTEX_FLOATS)
+        t = np.ascontiguousarray(targets, dtype=np.float32).reshape(-1, 3)
+        loss = np.empty(max(1, steps), dtype=np.float64)
+        self._check(self.lib.btrc_hip_nrc_fit(self.h, v.ctypes.data, t.ctypes.data, len(v), steps, loss.ctypes.data))
+        return loss[:steps]
+
+    def scene_box(self):
+        """(lo, 1 / extent) of the scene's triangles: the cache's position frame"""
+        return self._box
 
     # ---- traversal on ray batches
     def trace_closest(self, rays, tri_test=_abi.TRI_WATERTIGHT):

@@ -102,7 +102,7 @@ class FilterDesc(C.Structure):
 class RenderStats(C.Structure):
     _fields_ = [("seconds", C.c_double), ("extension_rays", C.c_uint64), ("shadow_rays", C.c_uint64),
                 ("paths", C.c_uint64), ("iterations", C.c_uint64),
-                ("stage_ms", C.c_double * 6), ("stage_launches", C.c_uint64 * 6)]
+                ("stage_ms", C.c_double * 7), ("stage_launches", C.c_uint64 * 7)]
 
 
 class Tuning(C.Structure):

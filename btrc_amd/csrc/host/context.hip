@@ -909,16 +909,17 @@ NrcScratch nrc_scratch(btrc_ctx *ctx, int64_t batch)
     return NrcScratch{ n.xt.ptr, n.dzt.ptr, n.loss_partial.ptr };
 }
 
-// `steps` optimizer steps over the frame's records, NRC_BATCH at a time
-void nrc_train(btrc_ctx *ctx, int64_t first_batch, int32_t steps, int64_t batch)
+// `steps` optimizer steps over the records, `batch` at a time: step k
+// takes the k-th batch (rotating through all records), or always the first
+void nrc_train(btrc_ctx *ctx, int32_t steps, int64_t batch, bool rotate)
 {
     auto &n = ctx->nrc;
     hipStream_t s = ctx->stream;
     const int64_t batches = NRC_RECORDS / batch;
     for(int32_t k = 0; k < steps; ++k)
     {
-        NrcBatch b{ n.rec.ptr, ((first_batch + k) % batches) * batch, batch, n.acc.ptr, n.counts.ptr + 1,
-                    n.params.loss_scale };
+        NrcBatch b{ n.rec.ptr, rotate ? (k % batches) * batch : 0, batch, n.acc.ptr, n.counts.ptr + 1,
+                    n.params.loss_scale, rotate ? 1 : 0 };
         launch_nrc_train_step(s, nrc_weights(ctx), nrc_box(ctx), b, nrc_scratch(ctx, batch), nrc_adam(ctx),
                               n.grad.ptr, n.step_stats.ptr + k);
         BTRC_HIP_CHECK(hipGetLastError());
@@ -1145,12 +1146,12 @@ void render(btrc_ctx *ctx, const btrc_render_params *p, btrc_render_stats *stats
         timer.run(STAGE_CACHE, [&] {
             launch_nrc_infer(s, igrid, nrc_weights(ctx), nrc_box(ctx), n.queries.ptr, n.counts.ptr, ctx->film.ptr,
                              p->film_fp64 ? ctx->film64.ptr : nullptr);
-            if(n.params.train)
-            {
-                nrc_steps = std::min(n.params.steps, 64);
-                nrc_train(ctx, 0, nrc_steps, NRC_BATCH);
-            }
         });
+        if(n.params.train)
+        {
+            nrc_steps = std::min(n.params.steps, 64);
+            timer.run(STAGE_TRAIN, [&] { nrc_train(ctx, nrc_steps, NRC_BATCH, true); });
+        }
         BTRC_HIP_CHECK(hipGetLastError());
     }
     BTRC_HIP_CHECK(hipStreamSynchronize(s));
@@ -1489,7 +1490,7 @@ int btrc_hip_nrc_gradients(btrc_ctx *ctx, const float *vertices, const float *ta
         nrc_load_batch(ctx, vertices, targets, n);
         auto &nc = ctx->nrc;
         const int64_t batch = ((int64_t)n + 63) / 64 * 64;
-        NrcBatch b{ nc.rec.ptr, 0, batch, nc.acc.ptr, nc.counts.ptr + 1, nc.params.loss_scale };
+        NrcBatch b{ nc.rec.ptr, 0, batch, nc.acc.ptr, nc.counts.ptr + 1, nc.params.loss_scale, 0 };
         launch_nrc_gradients(ctx->stream, nrc_weights(ctx), nrc_box(ctx), b, nrc_scratch(ctx, batch), nc.grad.ptr);
         BTRC_HIP_CHECK(hipGetLastError());
         std::vector<double> partial(batch / 32);
@@ -1517,7 +1518,7 @@ int btrc_hip_nrc_fit(btrc_ctx *ctx, const float *vertices, const float *targets,
             throw InvalidArg("bad cache batch");
         nrc_require(ctx);
         nrc_load_batch(ctx, vertices, targets, n);
-        nrc_train(ctx, 0, steps, NRC_BATCH);
+        nrc_train(ctx, steps, NRC_BATCH, false);
         nrc_collect(ctx, steps);
         if(loss)
         {

@@ -232,7 +232,7 @@ __global__ void __launch_bounds__(128) k_nrc_train_fb(NrcWeights net, NrcBox box
     const NrcVertex *vtx = nullptr;
     if(i < S)
     {
-        const NrcRecord &r = b.rec[b.first + i];
+        const NrcRecord &r = b.rec[b.permute ? nrc_record_slot(b.first + i) : b.first + i];
         valid = r.valid != 0;
         if(valid)
         {
@@ -427,12 +427,13 @@ __global__ void __launch_bounds__(256) k_nrc_adam(NrcWeights net, NrcAdam opt, c
 }
 
 // number of valid records of a batch (one workgroup, fixed order)
-__global__ void __launch_bounds__(256) k_nrc_count(const NrcRecord *rec, int64_t first, int64_t n, int32_t *count)
+__global__ void __launch_bounds__(256) k_nrc_count(const NrcRecord *rec, int64_t first, int64_t n, int32_t permute,
+                                                   int32_t *count)
 {
     __shared__ int32_t s[256];
     int32_t c = 0;
     for(int64_t k = threadIdx.x; k < n; k += 256)
-        c += rec[first + k].valid != 0;
+        c += rec[permute ? nrc_record_slot(first + k) : first + k].valid != 0;
     s[threadIdx.x] = c;
     __syncthreads();
     for(int off = 128; off > 0; off >>= 1)
@@ -487,7 +488,7 @@ void launch_nrc_eval(hipStream_t s, const NrcWeights &net, const NrcBox &box, co
 void launch_nrc_train_step(hipStream_t s, const NrcWeights &net, const NrcBox &box, const NrcBatch &b,
                            const NrcScratch &sc, const NrcAdam &opt, float *grad, NrcStepStats *stats)
 {
-    hipLaunchKernelGGL(k_nrc_count, dim3(1), dim3(256), 0, s, b.rec, b.first, b.batch, b.valid_count);
+    hipLaunchKernelGGL(k_nrc_count, dim3(1), dim3(256), 0, s, b.rec, b.first, b.batch, b.permute, b.valid_count);
     const int blocks = (int)(b.batch / (2 * TILE));
     hipLaunchKernelGGL(k_nrc_train_fb, dim3(blocks), dim3(128), 0, s, net, box, b, sc);
     hipLaunchKernelGGL(k_nrc_wgrad, dim3(22), dim3(64), 0, s, sc, b.batch, grad);
@@ -498,7 +499,7 @@ void launch_nrc_train_step(hipStream_t s, const NrcWeights &net, const NrcBox &b
 void launch_nrc_gradients(hipStream_t s, const NrcWeights &net, const NrcBox &box, const NrcBatch &b,
                           const NrcScratch &sc, float *grad)
 {
-    hipLaunchKernelGGL(k_nrc_count, dim3(1), dim3(256), 0, s, b.rec, b.first, b.batch, b.valid_count);
+    hipLaunchKernelGGL(k_nrc_count, dim3(1), dim3(256), 0, s, b.rec, b.first, b.batch, b.permute, b.valid_count);
     const int blocks = (int)(b.batch / (2 * TILE));
     hipLaunchKernelGGL(k_nrc_train_fb, dim3(blocks), dim3(128), 0, s, net, box, b, sc);
     hipLaunchKernelGGL(k_nrc_wgrad, dim3(22), dim3(64), 0, s, sc, b.batch, grad);

@@ -55,7 +55,16 @@ struct NrcBatch
     const float *acc;
     int32_t *valid_count;
     float loss_scale;
+    int32_t permute;         // 1: batch index i reads record nrc_record_slot(first + i)
 };
+
+// A frame's records sit at (training path, depth); training batches read
+// them through this bijection of [0, 2^16) so that every batch mixes paths
+// and depths.
+__host__ __device__ inline int64_t nrc_record_slot(int64_t i)
+{
+    return (int64_t)(((uint32_t)i * 40503u) & 0xffffu);
+}
 
 struct NrcScratch
 {

@@ -133,8 +133,9 @@ enum : int
     STAGE_SHADE = 2,
     STAGE_SHADOW = 3,
     STAGE_MEDIUM = 4,
-    STAGE_CACHE = 5,
-    STAGE_COUNT = 6,
+    STAGE_CACHE = 5,   // cache inference
+    STAGE_TRAIN = 6,   // cache training
+    STAGE_COUNT = 7,
 };
 
 void launch_plan(hipStream_t s, WaveCounters *c, int32_t state_count);

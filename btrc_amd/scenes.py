@@ -163,6 +163,20 @@ class Scene:
     def set_gradient_sky(self, up, lower, upper):
         self.envir = (up, lower, upper)
 
+    def world_box(self):
+        """(lo, 1 / (hi - lo)) over the world-space triangle vertices in
+        float32, as the device computes it for the radiance cache"""
+        lo = np.full(3, np.float32(3.4028234663852886e38))
+        hi = -lo
+        for inst in self.instances:
+            p = self.geometries[inst["geometry"]]["positions"].reshape(-1, 3).astype(np.float32)
+            m = inst["mat"].astype(np.float32)
+            w = ((m[:, 0] * p[:, :1] + m[:, 1] * p[:, 1:2]) + m[:, 2] * p[:, 2:3]) + m[:, 3]
+            lo = np.minimum(lo, w[:, :3].min(axis=0))
+            hi = np.maximum(hi, w[:, :3].max(axis=0))
+        ext = np.where(hi > lo, hi - lo, np.float32(1.0)).astype(np.float32)
+        return np.concatenate([lo, (np.float32(1.0) / ext).astype(np.float32)]).astype(np.float32)
+
     @property
     def triangle_count(self):
         return sum(len(self.geometries[i["geometry"]]["positions"]) for i in self.instances)

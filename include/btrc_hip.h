@@ -228,8 +228,9 @@ typedef struct btrc_render_stats
     uint64_t shadow_rays;    /* visibility rays traced */
     uint64_t paths;          /* camera paths completed */
     uint64_t iterations;     /* wavefront bounces executed */
-    double stage_ms[6];      /* generate, trace, shade, shadow, medium, cache (time_stages) */
-    uint64_t stage_launches[6];
+    double stage_ms[7];      /* generate, trace, shade, shadow, medium, cache inference,
+                                cache training (time_stages) */
+    uint64_t stage_launches[7];
 } btrc_render_stats;
 
 /* A ray of a traversal batch and its closest hit */

@@ -38,7 +38,7 @@ def test_struct_sizes():
     assert C.sizeof(_abi.RenderParams) == 64
     assert C.sizeof(_abi.MaterialDesc) == 68
     assert C.sizeof(_abi.InstanceDesc) == 4 * (2 + 32 + 1 + 3 + 2)
-    assert C.sizeof(_abi.RenderStats) == 8 * (1 + 4 + 6 + 6)
+    assert C.sizeof(_abi.RenderStats) == 8 * (1 + 4 + 7 + 7)
 
 
 STRUCTS = {
