@@ -232,6 +232,8 @@ struct btrc_ctx
         DevBuf<int32_t> counts;         // [0] queries, [1] valid records of a batch
         DevBuf<uint16_t> xt, dzt;
         DevBuf<double> loss_partial;
+        DevBuf<int32_t> count_partial;
+        DevBuf<float> grad_partial, step_scale;
         DevBuf<NrcStepStats> step_stats;
         int64_t steps_total = 0;
         btrc_nrc_stats last{};
@@ -848,6 +850,9 @@ void nrc_alloc(btrc_ctx *ctx)
     n.xt.alloc((size_t)NRC_LAYERS * NRC_IN * NRC_RECORDS);
     n.dzt.alloc((size_t)NRC_LAYERS * NRC_IN * NRC_RECORDS);
     n.loss_partial.alloc(NRC_RECORDS / 32);
+    n.grad_partial.alloc((size_t)NRC_KSPLIT * NRC_PARAMS);
+    n.count_partial.alloc(NRC_RECORDS / 32);
+    n.step_scale.alloc(1);
     n.step_stats.alloc(64);
 }
 
@@ -906,7 +911,8 @@ NrcScratch nrc_scratch(btrc_ctx *ctx, int64_t batch)
 {
     auto &n = ctx->nrc;
     (void)batch;
-    return NrcScratch{ n.xt.ptr, n.dzt.ptr, n.loss_partial.ptr };
+    return NrcScratch{ n.xt.ptr, n.dzt.ptr, n.loss_partial.ptr, n.count_partial.ptr, n.grad_partial.ptr,
+                       n.step_scale.ptr };
 }
 
 // `steps` optimizer steps over the records, `batch` at a time: step k
@@ -1498,8 +1504,10 @@ int btrc_hip_nrc_gradients(btrc_ctx *ctx, const float *vertices, const float *ta
         BTRC_HIP_CHECK(hipMemcpyAsync(partial.data(), nc.loss_partial.ptr, partial.size() * sizeof(double),
                                       hipMemcpyDeviceToHost, ctx->stream));
         BTRC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        // the kernels sum the loss-scaled per-sample gradients
+        const double scale = 1.0 / (3.0 * (double)n * (double)nc.params.loss_scale);
         for(int k = 0; k < NRC_PARAMS; ++k)
-            grad[k] /= nc.params.loss_scale;
+            grad[k] = (float)((double)grad[k] * scale);
         if(loss)
         {
             double sum = 0.0;

@@ -226,7 +226,6 @@ __global__ void __launch_bounds__(128) k_nrc_train_fb(NrcWeights net, NrcBox box
     const int64_t S = b.batch;
     const int64_t sample0 = (int64_t)(blockIdx.x * 2 + wave) * TILE;
     const int64_t i = sample0 + col;
-    const int32_t valid_count = *b.valid_count;
     bool valid = false;
     float target[3] = { 0.0f, 0.0f, 0.0f };
     const NrcVertex *vtx = nullptr;
@@ -264,18 +263,18 @@ __global__ void __launch_bounds__(128) k_nrc_train_fb(NrcWeights net, NrcBox box
         __syncthreads();
     }
 
-    // loss and output gradient
+    // loss and output gradient: the sum over the batch, times loss_scale (the
+    // 1 / (3 n) of the mean is applied with the update, once n is known)
     double loss = 0.0;
     uint16_t *dz = s_dz[wave][0];
     if(half == 0)
     {
-        const float inv = valid_count > 0 ? b.loss_scale / (3.0f * (float)valid_count) : 0.0f;
         for(int r = 0; r < NRC_OUT_ROWS; ++r)
         {
             float g = 0.0f;
             if(r < NRC_OUT && valid)
             {
-                g = nrc_rel_l2_grad(y[r], target[r]) * inv;
+                g = nrc_rel_l2_grad(y[r], target[r]) * b.loss_scale;
                 loss += (double)nrc_rel_l2(y[r], target[r]);
             }
             dz[col * NRC_IN + r] = to_bf16(g);
@@ -283,8 +282,12 @@ __global__ void __launch_bounds__(128) k_nrc_train_fb(NrcWeights net, NrcBox box
     }
     for(int off = 32; off > 0; off >>= 1)
         loss += __shfl_xor(loss, off);
+    const int32_t nvalid = (int32_t)__popcll(__ballot(valid && half == 0));
     if(lane == 0)
+    {
         sc.loss_partial[blockIdx.x * 2 + wave] = loss;
+        sc.count_partial[blockIdx.x * 2 + wave] = nvalid;
+    }
     __syncthreads();
 
     // backward: dz holds d/d(out) of layer l as [sample][unit]
@@ -322,128 +325,134 @@ __global__ void __launch_bounds__(128) k_nrc_train_fb(NrcWeights net, NrcBox box
     }
 }
 
-// dW_l = sum over samples of dz_l (x) x_l, one 32 x 32 tile per wavefront:
-// tiles 0..19 cover the five 64 x 64 hidden layers, tiles 20..21 the output
-// layer's 32 x 64. Sums run in a fixed order (deterministic).
-__global__ void __launch_bounds__(64) k_nrc_wgrad(NrcScratch sc, int64_t S, float *grad)
+// dW_l = sum over samples of dz_l (x) x_l. One wavefront per 32 x 32 tile
+// and slice of the samples: tiles 0..19 cover the five 64 x 64 hidden
+// layers, tiles 20..21 the output layer's 32 x 64; slice kc of NRC_KSPLIT
+// writes its partial tile to partial[kc]. k_nrc_reduce adds the slices in
+// a fixed order (deterministic).
+__global__ void __launch_bounds__(64) k_nrc_wgrad(NrcScratch sc, int64_t S, float *partial)
 {
-    const int t = blockIdx.x;
+    const int t = blockIdx.x % 22, kc = blockIdx.x / 22;
     const int l = t < 20 ? t / 4 : 5;
     const int mt = t < 20 ? (t % 4) / 2 : 0;
     const int nt = t < 20 ? t % 2 : t - 20;
     const int lane = threadIdx.x, col = lane % 32, half = lane / 32;
+    const int64_t chunk = (S + NRC_KSPLIT * 16 - 1) / (NRC_KSPLIT * 16) * 16;
+    const int64_t k0 = kc * chunk, k1 = k0 + chunk < S ? k0 + chunk : S;
     const uint16_t *dzt = sc.dzt + (int64_t)l * NRC_IN * S;
     const uint16_t *xt = sc.xt + (int64_t)l * NRC_IN * S;
     const uint16_t *arow = dzt + (int64_t)(mt * 32 + col) * S + 8 * half;
     const uint16_t *brow = xt + (int64_t)(nt * 32 + col) * S + 8 * half;
-    f32x16 acc = {};
-    for(int64_t k = 0; k < S; k += 16)
-        acc = mfma(*reinterpret_cast<const u16x8 *>(arow + k), *reinterpret_cast<const u16x8 *>(brow + k), acc);
-    float *g = grad + nrc_layer_offset(l);
+    f32x16 acc0 = {}, acc1 = {};
+    int64_t k = k0;
+    for(; k + 32 <= k1; k += 32)
+    {
+        const u16x8 a0 = *reinterpret_cast<const u16x8 *>(arow + k);
+        const u16x8 b0 = *reinterpret_cast<const u16x8 *>(brow + k);
+        const u16x8 a1 = *reinterpret_cast<const u16x8 *>(arow + k + 16);
+        const u16x8 b1 = *reinterpret_cast<const u16x8 *>(brow + k + 16);
+        acc0 = mfma(a0, b0, acc0);
+        acc1 = mfma(a1, b1, acc1);
+    }
+    for(; k < k1; k += 16)
+        acc0 = mfma(*reinterpret_cast<const u16x8 *>(arow + k), *reinterpret_cast<const u16x8 *>(brow + k), acc0);
+    float *g = partial + (size_t)kc * NRC_PARAMS + nrc_layer_offset(l);
 #pragma unroll
     for(int e = 0; e < 16; ++e)
     {
         const int row = mt * 32 + 8 * (e / 4) + 4 * half + (e % 4);
-        g[row * NRC_IN + nt * 32 + col] = acc[e];
+        g[row * NRC_IN + nt * 32 + col] = acc0[e] + acc1[e];
     }
 }
 
-// Adam (Kingma and Ba 2015) on the FP32 master weights, then the BF16
-// copies. One workgroup; FP64 reductions of the loss partials and of the
-// squared gradient norm. A non-finite gradient skips the step.
-__global__ void __launch_bounds__(256) k_nrc_adam(NrcWeights net, NrcAdam opt, const float *grad,
-                                                  const double *loss_partial, int32_t partials,
-                                                  const int32_t *valid_count, NrcStepStats *stats)
+// grad = sum over the NRC_KSPLIT slices, in slice order
+__global__ void __launch_bounds__(256) k_nrc_reduce(const float *partial, float *grad)
 {
-    __shared__ double s_red[256];
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if(k >= NRC_PARAMS)
+        return;
+    float sum = 0.0f;
+    for(int c = 0; c < NRC_KSPLIT; ++c)
+        sum += partial[(size_t)c * NRC_PARAMS + k];
+    grad[k] = sum;
+}
+
+// Reductions of a step (one workgroup, fixed order, FP64): valid samples n,
+// loss = sum / (3 n), and the norm of the gradient of the mean loss. Sets
+// the scale that turns the summed, loss-scaled gradient into the mean's
+// gradient, or 0 (no update) for a non-finite gradient or an empty batch.
+__global__ void __launch_bounds__(256) k_nrc_finalize(const float *grad, NrcScratch sc, int32_t partials,
+                                                      float loss_scale, int32_t *valid_count, NrcStepStats *stats)
+{
+    __shared__ double s_a[256], s_b[256];
+    __shared__ long long s_c[256];
     const int tid = threadIdx.x;
-    double g2 = 0.0, loss = 0.0;
+    double loss = 0.0;
+    long long cnt = 0;
+    for(int k = tid; k < partials; k += 256)
+    {
+        loss += sc.loss_partial[k];
+        cnt += sc.count_partial[k];
+    }
+    double g2 = 0.0;
     for(int k = tid; k < NRC_PARAMS; k += 256)
     {
-        const double g = (double)grad[k] / (double)opt.loss_scale;
+        const double g = (double)grad[k];
         g2 += g * g;
     }
-    for(int k = tid; k < partials; k += 256)
-        loss += loss_partial[k];
-    s_red[tid] = g2;
+    s_a[tid] = loss;
+    s_b[tid] = g2;
+    s_c[tid] = cnt;
     __syncthreads();
     for(int off = 128; off > 0; off >>= 1)
     {
         if(tid < off)
-            s_red[tid] += s_red[tid + off];
-        __syncthreads();
-    }
-    const double gnorm2 = s_red[0];
-    __syncthreads();
-    s_red[tid] = loss;
-    __syncthreads();
-    for(int off = 128; off > 0; off >>= 1)
-    {
-        if(tid < off)
-            s_red[tid] += s_red[tid + off];
-        __syncthreads();
-    }
-    const double loss_sum = s_red[0];
-    const int32_t nv = *valid_count;
-    const bool finite = isfinite(gnorm2);
-    const bool step = finite && nv > 0;
-    if(step)
-    {
-        for(int k = tid; k < NRC_PARAMS; k += 256)
         {
-            const float g = grad[k] / opt.loss_scale;
-            const float m = opt.beta1 * net.m[k] + (1.0f - opt.beta1) * g;
-            const float v = opt.beta2 * net.v[k] + (1.0f - opt.beta2) * g * g;
-            net.m[k] = m;
-            net.v[k] = v;
-            const float mhat = m * opt.inv_bias1;
-            const float vhat = v * opt.inv_bias2;
-            const float w = net.w[k] - opt.lr * mhat / (sqrtf(vhat) + opt.eps);
-            net.w[k] = w;
+            s_a[tid] += s_a[tid + off];
+            s_b[tid] += s_b[tid + off];
+            s_c[tid] += s_c[tid + off];
         }
-    }
-    __syncthreads();
-    if(step)
-    {
-        // BF16 copies: [out][in] and per layer [in][out]
-        for(int k = tid; k < NRC_PARAMS; k += 256)
-        {
-            const int l = k / (NRC_WIDTH * NRC_IN) < NRC_LAYERS ? k / (NRC_WIDTH * NRC_IN) : NRC_LAYERS - 1;
-            const int off = nrc_layer_offset(l);
-            const int rows = nrc_layer_rows(l);
-            const int r = (k - off) / NRC_IN, c = (k - off) % NRC_IN;
-            const uint16_t h = to_bf16(net.w[k]);
-            net.w_bf16[k] = h;
-            net.wt_bf16[off + c * rows + r] = h;
-        }
+        __syncthreads();
     }
     if(tid == 0)
     {
-        stats->loss = nv > 0 ? loss_sum / (3.0 * (double)nv) : 0.0;
-        stats->grad_norm = sqrt(gnorm2);
-        stats->samples = nv;
+        const long long n = s_c[0];
+        const double scale = n > 0 ? 1.0 / (3.0 * (double)n * (double)loss_scale) : 0.0;
+        const double gnorm = sqrt(s_b[0]) * scale;
+        const bool step = n > 0 && isfinite(gnorm);
+        *sc.step_scale = step ? (float)scale : 0.0f;
+        *valid_count = (int32_t)n;
+        stats->loss = n > 0 ? s_a[0] / (3.0 * (double)n) : 0.0;
+        stats->grad_norm = gnorm;
+        stats->samples = (int32_t)n;
         stats->skipped = step ? 0 : 1;
     }
 }
 
-// number of valid records of a batch (one workgroup, fixed order)
-__global__ void __launch_bounds__(256) k_nrc_count(const NrcRecord *rec, int64_t first, int64_t n, int32_t permute,
-                                                   int32_t *count)
+// Adam (Kingma and Ba 2015) on the FP32 master weights, one weight per
+// thread, then its BF16 copies ([out][in] and per layer [in][out])
+__global__ void __launch_bounds__(256) k_nrc_update(NrcWeights net, NrcAdam opt, const float *grad, NrcScratch sc)
 {
-    __shared__ int32_t s[256];
-    int32_t c = 0;
-    for(int64_t k = threadIdx.x; k < n; k += 256)
-        c += rec[permute ? nrc_record_slot(first + k) : first + k].valid != 0;
-    s[threadIdx.x] = c;
-    __syncthreads();
-    for(int off = 128; off > 0; off >>= 1)
-    {
-        if((int)threadIdx.x < off)
-            s[threadIdx.x] += s[threadIdx.x + off];
-        __syncthreads();
-    }
-    if(threadIdx.x == 0)
-        *count = s[0];
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const float scale = *sc.step_scale;
+    if(k >= NRC_PARAMS || scale == 0.0f)
+        return;
+    const float g = grad[k] * scale;
+    const float m = opt.beta1 * net.m[k] + (1.0f - opt.beta1) * g;
+    const float v = opt.beta2 * net.v[k] + (1.0f - opt.beta2) * g * g;
+    net.m[k] = m;
+    net.v[k] = v;
+    const float mhat = m * opt.inv_bias1;
+    const float vhat = v * opt.inv_bias2;
+    const float w = net.w[k] - opt.lr * mhat / (sqrtf(vhat) + opt.eps);
+    net.w[k] = w;
+    const int l = k / (NRC_WIDTH * NRC_IN) < NRC_LAYERS ? k / (NRC_WIDTH * NRC_IN) : NRC_LAYERS - 1;
+    const int off = nrc_layer_offset(l);
+    const int rows = nrc_layer_rows(l);
+    const int r = (k - off) / NRC_IN, c = (k - off) % NRC_IN;
+    const uint16_t h = to_bf16(w);
+    net.w_bf16[k] = h;
+    net.wt_bf16[off + c * rows + r] = h;
 }
 
 // clears the record flags and the training paths' radiance sums of a frame
@@ -488,21 +497,20 @@ void launch_nrc_eval(hipStream_t s, const NrcWeights &net, const NrcBox &box, co
 void launch_nrc_train_step(hipStream_t s, const NrcWeights &net, const NrcBox &box, const NrcBatch &b,
                            const NrcScratch &sc, const NrcAdam &opt, float *grad, NrcStepStats *stats)
 {
-    hipLaunchKernelGGL(k_nrc_count, dim3(1), dim3(256), 0, s, b.rec, b.first, b.batch, b.permute, b.valid_count);
+    launch_nrc_gradients(s, net, box, b, sc, grad);
     const int blocks = (int)(b.batch / (2 * TILE));
-    hipLaunchKernelGGL(k_nrc_train_fb, dim3(blocks), dim3(128), 0, s, net, box, b, sc);
-    hipLaunchKernelGGL(k_nrc_wgrad, dim3(22), dim3(64), 0, s, sc, b.batch, grad);
-    hipLaunchKernelGGL(k_nrc_adam, dim3(1), dim3(256), 0, s, net, opt, grad, sc.loss_partial, blocks * 2,
-                       b.valid_count, stats);
+    hipLaunchKernelGGL(k_nrc_finalize, dim3(1), dim3(256), 0, s, grad, sc, blocks * 2, b.loss_scale, b.valid_count,
+                       stats);
+    hipLaunchKernelGGL(k_nrc_update, dim3((NRC_PARAMS + 255) / 256), dim3(256), 0, s, net, opt, grad, sc);
 }
 
 void launch_nrc_gradients(hipStream_t s, const NrcWeights &net, const NrcBox &box, const NrcBatch &b,
                           const NrcScratch &sc, float *grad)
 {
-    hipLaunchKernelGGL(k_nrc_count, dim3(1), dim3(256), 0, s, b.rec, b.first, b.batch, b.permute, b.valid_count);
     const int blocks = (int)(b.batch / (2 * TILE));
     hipLaunchKernelGGL(k_nrc_train_fb, dim3(blocks), dim3(128), 0, s, net, box, b, sc);
-    hipLaunchKernelGGL(k_nrc_wgrad, dim3(22), dim3(64), 0, s, sc, b.batch, grad);
+    hipLaunchKernelGGL(k_nrc_wgrad, dim3(22 * NRC_KSPLIT), dim3(64), 0, s, sc, b.batch, sc.grad_partial);
+    hipLaunchKernelGGL(k_nrc_reduce, dim3((NRC_PARAMS + 255) / 256), dim3(256), 0, s, sc.grad_partial, grad);
 }
 
 void launch_nrc_clear(hipStream_t s, NrcRecord *rec, int64_t nrec, float *acc, int64_t npaths)

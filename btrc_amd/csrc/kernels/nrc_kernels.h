@@ -53,7 +53,7 @@ struct NrcBatch
     int64_t first;
     int64_t batch;           // a multiple of 64
     const float *acc;
-    int32_t *valid_count;
+    int32_t *valid_count;    // set by a training step
     float loss_scale;
     int32_t permute;         // 1: batch index i reads record nrc_record_slot(first + i)
 };
@@ -66,11 +66,16 @@ __host__ __device__ inline int64_t nrc_record_slot(int64_t i)
     return (int64_t)(((uint32_t)i * 40503u) & 0xffffu);
 }
 
+constexpr int NRC_KSPLIT = 32;  // sample slices of the weight-gradient products
+
 struct NrcScratch
 {
     uint16_t *xt;            // [layer][64][batch] layer inputs
     uint16_t *dzt;           // [layer][64][batch] layer output gradients
     double *loss_partial;    // one per wavefront
+    int32_t *count_partial;  // valid samples, one per wavefront
+    float *grad_partial;     // [NRC_KSPLIT][NRC_PARAMS]
+    float *step_scale;       // summed gradient -> gradient of the mean loss; 0: skip the update
 };
 
 struct NrcAdam

@@ -2235,7 +2235,7 @@ void oracle_nrc_gradients(const float *w, const float *box, const float *v, cons
             const float yf = (float)y;
             const double den = (double)(yf * yf + 0.01f);
             lsum += (y - tc) * (y - tc) / den;
-            double g = 2.0 * (y - tc) / den / (3.0 * n);
+            double g = 2.0 * (y - tc) / den;
             if(b16)
                 g = bf16(g * loss_scale);
             dz[c] = g;
@@ -2261,9 +2261,10 @@ void oracle_nrc_gradients(const float *w, const float *box, const float *v, cons
                 dz[c] = dx[c];
         }
     }
-    if(b16)
-        for(int k = 0; k < PARAMS; ++k)
-            grad[k] /= loss_scale;
+    // per-sample gradients were summed (and scaled by loss_scale with bf16)
+    const double norm = (b16 ? (double)loss_scale : 1.0) * 3.0 * n;
+    for(int k = 0; k < PARAMS; ++k)
+        grad[k] /= norm;
     *loss = lsum / (3.0 * n);
 }
 
