@@ -11,7 +11,7 @@ import itertools
 
 from . import _abi
 
-WAVES = (10, 14, 20)
+WAVES = (16, 20, 24, 28)
 
 
 def autotune(ctx, params, spp=4, waves=WAVES):

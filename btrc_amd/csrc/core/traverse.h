@@ -32,6 +32,9 @@ struct alignas(16) BvhNode4
     float lo_x[4], hi_x[4];
     float lo_y[4], hi_y[4];
     float lo_z[4], hi_z[4];
+    // as built: inner child index, or ~first triangle of a leaf with
+    // count[j] triangles (count 0: empty slot); the device copy holds the
+    // traversal codes instead (inner index, leaf code, or all ones)
     int32_t child[4];
     int32_t count[4];
 };
@@ -67,6 +70,23 @@ BTRC_HD RayBoxPre raybox_pre(V3 o, V3 d)
 BTRC_HD float rmax(float a, float b) { return a > b ? a : b; }
 BTRC_HD float rmin(float a, float b) { return a < b ? a : b; }
 
+// The slab test with the near and far planes already chosen by the ray's
+// direction signs; same arithmetic as raybox.
+BTRC_HD bool raybox_nf(const RayBoxPre &p, float nx, float fx, float ny, float fy, float nz, float fz,
+                       float tmin, float tmax, float *tentry)
+{
+    const float t0x = (nx - p.o.x) * p.inv.x;
+    const float t0y = (ny - p.o.y) * p.inv.y;
+    const float t0z = (nz - p.o.z) * p.inv.z;
+    const float t1x = (fx - p.o.x) * p.inv_pad.x;
+    const float t1y = (fy - p.o.y) * p.inv_pad.y;
+    const float t1z = (fz - p.o.z) * p.inv_pad.z;
+    const float te = fmaxf(t0x, fmaxf(t0y, fmaxf(t0z, tmin)));
+    const float tx = fminf(t1x, fminf(t1y, fminf(t1z, tmax)));
+    *tentry = te;
+    return te <= tx;
+}
+
 // Conservative slab test of one box; returns the entry distance through *tentry.
 BTRC_HD bool raybox(const RayBoxPre &p, float lox, float hix, float loy, float hiy, float loz, float hiz,
                     float tmin, float tmax, float *tentry)
@@ -80,8 +100,11 @@ BTRC_HD bool raybox(const RayBoxPre &p, float lox, float hix, float loy, float h
     const float t1x = (fx - p.o.x) * p.inv_pad.x;
     const float t1y = (fy - p.o.y) * p.inv_pad.y;
     const float t1z = (fz - p.o.z) * p.inv_pad.z;
-    const float te = rmax(t0x, rmax(t0y, rmax(t0z, tmin)));
-    const float tx = rmin(t1x, rmin(t1y, rmin(t1z, tmax)));
+    // robust max / min: tmin and tmax are never NaN, so every right operand
+    // of the chains is a number and IEEE maxNum / minNum (which drop a NaN
+    // operand) give exactly rmax / rmin, in fewer instructions
+    const float te = fmaxf(t0x, fmaxf(t0y, fmaxf(t0z, tmin)));
+    const float tx = fminf(t1x, fminf(t1y, fminf(t1z, tmax)));
     *tentry = te;
     return te <= tx;
 }

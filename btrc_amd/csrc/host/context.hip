@@ -676,7 +676,7 @@ void upload_scene(btrc_ctx *ctx, const btrc_scene_desc *sd)
         throw InvalidArg("too many triangles for the traversal's leaf encoding");
     BvhBuildOptions opt;
     opt.max_leaf_size = 4; // leaf codes carry counts 1..4
-    const Bvh4 bvh = build_bvh4(world.data(), ntri, opt);
+    Bvh4 bvh = build_bvh4(world.data(), ntri, opt);
     std::vector<float4> tri_data(3 * (size_t)ntri);
     for(uint32_t k = 0; k < ntri; ++k)
     {
@@ -688,6 +688,16 @@ void upload_scene(btrc_ctx *ctx, const btrc_scene_desc *sd)
     }
 
     ctx->has_scene = false;
+    // children as traversal codes: inner index, leaf code, or CODE_DONE
+    for(BvhNode4 &nd : bvh.nodes)
+        for(int j = 0; j < 4; ++j)
+        {
+            uint32_t code = (uint32_t)nd.child[j];
+            if(nd.child[j] < 0)
+                code = nd.count[j] > 0 ? (CODE_LEAF | ((uint32_t)(nd.count[j] - 1) << 28) | (uint32_t)~nd.child[j])
+                                       : CODE_DONE;
+            nd.child[j] = (int32_t)code;
+        }
     ctx->nodes.upload(bvh.nodes.data(), bvh.nodes.size(), s);
     ctx->tris.upload(tri_data.data(), tri_data.size(), s);
     ctx->instances.upload(insts.data(), insts.size(), s);
@@ -1316,7 +1326,7 @@ int btrc_hip_device_name(btrc_ctx *ctx, char *buf, size_t len)
 int btrc_hip_set_tuning(btrc_ctx *ctx, const btrc_tuning *t)
 {
     return guarded(ctx, [&] {
-        if(!t || t->trace_waves_per_cu < 1 || t->trace_waves_per_cu > TRACE_WAVES_PER_CU)
+        if(!t || t->trace_waves_per_cu < 1 || t->trace_waves_per_cu > TRACE_WAVES_PER_CU_MAX)
             throw InvalidArg("bad tuning");
         BTRC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
         ctx->tuning = *t;

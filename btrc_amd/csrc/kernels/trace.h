@@ -10,7 +10,7 @@
 //    a lane that has found a leaf keeps stepping through inner nodes until
 //    every lane of the wavefront holds a leaf, so leaf tests run together;
 //  * the traversal stack keeps its top SHORT_STACK entries in LDS, laid out
-//    [entry][lane] (a wavefront's pushes hit 64 consecutive dwords, no bank
+//    [entry][lane] (a wavefront's pushes hit consecutive words, no bank
 //    conflicts), and deeper entries in a per-lane global overflow area sized
 //    from the tree's exact worst case;
 //  * children are visited nearest first; each stack entry keeps its entry
@@ -29,7 +29,8 @@ namespace btrc {
 constexpr int TRACE_BLOCK = 64;   // one wavefront per workgroup
 constexpr int SHORT_STACK = 16;   // LDS stack entries per lane
 constexpr int REFILL_MIN = 16;    // idle lanes that trigger a refill
-constexpr int TRACE_WAVES_PER_CU = 20; // persistent grid: wavefronts per compute unit
+constexpr int TRACE_WAVES_PER_CU = 20; // persistent grid: wavefronts per compute unit (default)
+constexpr int TRACE_WAVES_PER_CU_MAX = 32;
 
 // stack entry / node codes
 constexpr uint32_t CODE_LEAF = 0x80000000u;   // | (count - 1) << 28 | first triangle
@@ -65,6 +66,8 @@ __device__ __forceinline__ uint32_t leaf_code(int32_t first, int32_t count)
     return CODE_LEAF | ((uint32_t)(count - 1) << 28) | (uint32_t)first;
 }
 
+
+
 // Traces rays [0, n) handed out through *fetch. Sink provides
 //   void load(uint32_t i, V3 &o, V3 &d, float &tmin, float &tmax)
 //   void store(uint32_t i, const HitRecord &h)
@@ -87,6 +90,7 @@ __device__ __forceinline__ void trace_persistent(
     V3 o = v3s(0.0f), d = v3s(0.0f);
     float tmin = 0.0f;
     RayBoxPre rb;
+    uint32_t near_x = 0, near_y = 32, near_z = 64; // byte offsets of the near planes in a node
     WatertightPre wt;
     MTPre mt;
     HitRecord h;
@@ -188,6 +192,9 @@ __device__ __forceinline__ void trace_persistent(
                     float tmax;
                     sink.load(idx, o, d, tmin, tmax);
                     rb = raybox_pre(o, d);
+                    near_x = rb.neg_x ? 16u : 0u;   // hi_x sits 16 bytes after lo_x
+                    near_y = rb.neg_y ? 48u : 32u;
+                    near_z = rb.neg_z ? 80u : 64u;
                     if(TRI == TRI_WATERTIGHT)
                         wt = watertight_pre(o, d);
                     else
@@ -225,18 +232,25 @@ __device__ __forceinline__ void trace_persistent(
                 }
                 else
                 {
-                    const float4 *np = reinterpret_cast<const float4 *>(sc.nodes + node);
-                    const float4 lox = np[0], hix = np[1], loy = np[2], hiy = np[3], loz = np[4], hiz = np[5];
-                    const int4 child = reinterpret_cast<const int4 *>(np)[6];
-                    const int4 count = reinterpret_cast<const int4 *>(np)[7];
-                    const float blox[4] = { lox.x, lox.y, lox.z, lox.w };
-                    const float bhix[4] = { hix.x, hix.y, hix.z, hix.w };
-                    const float bloy[4] = { loy.x, loy.y, loy.z, loy.w };
-                    const float bhiy[4] = { hiy.x, hiy.y, hiy.z, hiy.w };
-                    const float bloz[4] = { loz.x, loz.y, loz.z, loz.w };
-                    const float bhiz[4] = { hiz.x, hiz.y, hiz.z, hiz.w };
-                    const int32_t ch[4] = { child.x, child.y, child.z, child.w };
-                    const int32_t cn[4] = { count.x, count.y, count.z, count.w };
+                    // the node stores lo then hi (16 bytes each) per axis: the
+                    // near and far planes are picked by address from the
+                    // ray's direction signs, not by selects after the load
+                    const char *nb = reinterpret_cast<const char *>(sc.nodes);
+                    const uint32_t off = node * (uint32_t)sizeof(BvhNode4);
+                    const float4 nxv = *reinterpret_cast<const float4 *>(nb + (off + near_x));
+                    const float4 fxv = *reinterpret_cast<const float4 *>(nb + (off + (near_x ^ 16u)));
+                    const float4 nyv = *reinterpret_cast<const float4 *>(nb + (off + near_y));
+                    const float4 fyv = *reinterpret_cast<const float4 *>(nb + (off + (near_y ^ 16u)));
+                    const float4 nzv = *reinterpret_cast<const float4 *>(nb + (off + near_z));
+                    const float4 fzv = *reinterpret_cast<const float4 *>(nb + (off + (near_z ^ 16u)));
+                    const uint4 code = *reinterpret_cast<const uint4 *>(nb + (off + 96u));
+                    const float bnx[4] = { nxv.x, nxv.y, nxv.z, nxv.w };
+                    const float bfx[4] = { fxv.x, fxv.y, fxv.z, fxv.w };
+                    const float bny[4] = { nyv.x, nyv.y, nyv.z, nyv.w };
+                    const float bfy[4] = { fyv.x, fyv.y, fyv.z, fyv.w };
+                    const float bnz[4] = { nzv.x, nzv.y, nzv.z, nzv.w };
+                    const float bfz[4] = { fzv.x, fzv.y, fzv.z, fzv.w };
+                    const uint32_t cc[4] = { code.x, code.y, code.z, code.w };
                     const float cull = cull_distance(h.t);
                     float k[4];
                     uint32_t c[4];
@@ -244,12 +258,10 @@ __device__ __forceinline__ void trace_persistent(
                     for(int j = 0; j < 4; ++j)
                     {
                         float te;
-                        const bool hit = raybox(rb, blox[j], bhix[j], bloy[j], bhiy[j], bloz[j], bhiz[j],
-                                                tmin, cull, &te);
-                        const bool leafj = ch[j] < 0;
-                        const bool use = hit && (!leafj || cn[j] > 0);
-                        k[j] = use ? te : INFINITY;
-                        c[j] = !use ? CODE_DONE : (leafj ? leaf_code(~ch[j], cn[j]) : (uint32_t)ch[j]);
+                        const bool hit = raybox_nf(rb, bnx[j], bfx[j], bny[j], bfy[j], bnz[j], bfz[j],
+                                                   tmin, cull, &te) && cc[j] != CODE_DONE;
+                        k[j] = hit ? te : INFINITY;
+                        c[j] = hit ? cc[j] : CODE_DONE;
                     }
 #define BTRC_CSWAP(a, b)                                                  \
                     if(k[b] < k[a])                                       \

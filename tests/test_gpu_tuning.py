@@ -9,7 +9,7 @@ from util import assert_hits_equal, random_rays, render_params
 pytestmark = pytest.mark.gpu
 
 VARIANTS = [dict(closest_postpone_leaves=c, any_postpone_leaves=a, trace_waves_per_cu=w)
-            for c in (0, 1) for a in (0, 1) for w in (1, 7, 20)]
+            for c in (0, 1) for a in (0, 1) for w in (1, 7, 24, 32)]
 
 
 def test_tuning_roundtrip_and_validation(ctx):
@@ -17,7 +17,7 @@ def test_tuning_roundtrip_and_validation(ctx):
     assert ctx.get_tuning()["trace_waves_per_cu"] == 7
     with pytest.raises(Exception):
         ctx.set_tuning(trace_waves_per_cu=0)
-    ctx.set_tuning(closest_postpone_leaves=1, any_postpone_leaves=0, trace_waves_per_cu=20)
+    ctx.set_tuning(closest_postpone_leaves=1, any_postpone_leaves=0, trace_waves_per_cu=24)
 
 
 def test_traversal_variants_bitwise_equal(ctx):
@@ -39,7 +39,7 @@ def test_traversal_variants_bitwise_equal(ctx):
             else:
                 assert_hits_equal(h, ref_h[mode])
                 assert np.array_equal(o, ref_o[mode])
-    ctx.set_tuning(closest_postpone_leaves=1, any_postpone_leaves=0, trace_waves_per_cu=20)
+    ctx.set_tuning(closest_postpone_leaves=1, any_postpone_leaves=0, trace_waves_per_cu=24)
 
 
 def test_render_variants_equal(ctx):
@@ -55,4 +55,4 @@ def test_render_variants_equal(ctx):
     for im in imgs[1:]:
         # only the FP32 atomic accumulation order may differ
         np.testing.assert_allclose(im, imgs[0], rtol=1e-5, atol=1e-6)
-    ctx.set_tuning(closest_postpone_leaves=1, any_postpone_leaves=0, trace_waves_per_cu=20)
+    ctx.set_tuning(closest_postpone_leaves=1, any_postpone_leaves=0, trace_waves_per_cu=24)
