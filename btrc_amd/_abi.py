@@ -13,6 +13,8 @@ MATERIAL_DISNEY = 1
 MATERIAL_INVISIBLE = 2
 MEDIUM_HOMOGENEOUS = 0
 MEDIUM_HETEROGENEOUS = 1
+ACCEL_FLAT = 0
+ACCEL_TWO_LEVEL = 1
 ENVIR_NONE = 0
 ENVIR_GRADIENT_SKY = 1
 SAMPLER_HALTON = 0
@@ -63,7 +65,8 @@ class SceneDesc(C.Structure):
     _fields_ = [("geometry_count", C.c_int32), ("geometries", C.POINTER(GeometryDesc)),
                 ("material_count", C.c_int32), ("materials", C.POINTER(MaterialDesc)),
                 ("instance_count", C.c_int32), ("instances", C.POINTER(InstanceDesc)),
-                ("envir_light", EnvirLightDesc), ("medium_count", C.c_int32), ("media", C.POINTER(MediumDesc))]
+                ("envir_light", EnvirLightDesc), ("medium_count", C.c_int32), ("media", C.POINTER(MediumDesc)),
+                ("accel", C.c_int32)]
 
 
 class CameraDesc(C.Structure):

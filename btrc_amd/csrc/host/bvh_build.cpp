@@ -215,8 +215,11 @@ void set_slot(BvhNode4 &n, int j, const Box &b)
 
 } // namespace
 
-Bvh4 build_bvh4(const float *vertices, uint32_t triangle_count, const BvhBuildOptions &opt)
+namespace {
+
+Bvh4 build_from_boxes(std::vector<Box> &&boxes, const BvhBuildOptions &opt)
 {
+    const uint32_t triangle_count = (uint32_t)boxes.size();
     Bvh4 out;
     if(triangle_count == 0)
     {
@@ -233,21 +236,17 @@ Bvh4 build_bvh4(const float *vertices, uint32_t triangle_count, const BvhBuildOp
         return out;
     }
     if(triangle_count >= (1u << 30))
-        throw std::runtime_error("too many triangles for the BVH leaf encoding");
+        throw std::runtime_error("too many primitives for the BVH leaf encoding");
 
     Builder b(opt);
-    b.prim_box.resize(triangle_count);
+    b.prim_box = std::move(boxes);
     b.centroid.resize(3 * (size_t)triangle_count);
     b.order.resize(triangle_count);
     for(uint32_t i = 0; i < triangle_count; ++i)
     {
-        Box box;
-        box.grow(vertices + 9 * (size_t)i + 0);
-        box.grow(vertices + 9 * (size_t)i + 3);
-        box.grow(vertices + 9 * (size_t)i + 6);
-        b.prim_box[i] = box;
+        const Box &box = b.prim_box[i];
         for(int a = 0; a < 3; ++a)
-            b.centroid[3 * (size_t)i + a] = 0.5f * (box.lo[a] + box.hi[a]);
+            b.centroid[3 * (size_t)i + a] = box.empty() ? 0.0f : 0.5f * (box.lo[a] + box.hi[a]);
         b.order[i] = i;
     }
     b.build(triangle_count);
@@ -339,6 +338,36 @@ Bvh4 build_bvh4(const float *vertices, uint32_t triangle_count, const BvhBuildOp
     out.stack_bound = max_stack;
     out.root_is_leaf = bn[0].left < 0;
     return out;
+}
+
+} // namespace
+
+Bvh4 build_bvh4(const float *vertices, uint32_t triangle_count, const BvhBuildOptions &opt)
+{
+    std::vector<Box> boxes(triangle_count);
+    for(uint32_t i = 0; i < triangle_count; ++i)
+    {
+        boxes[i].grow(vertices + 9 * (size_t)i + 0);
+        boxes[i].grow(vertices + 9 * (size_t)i + 3);
+        boxes[i].grow(vertices + 9 * (size_t)i + 6);
+    }
+    return build_from_boxes(std::move(boxes), opt);
+}
+
+Bvh4 build_bvh4_boxes(const float *boxes, uint32_t count, const BvhBuildOptions &opt)
+{
+    std::vector<Box> bx(count);
+    for(uint32_t i = 0; i < count; ++i)
+    {
+        const float *b = boxes + 6 * (size_t)i;
+        if(b[0] <= b[3] && b[1] <= b[4] && b[2] <= b[5])
+            for(int a = 0; a < 3; ++a)
+            {
+                bx[i].lo[a] = b[a];
+                bx[i].hi[a] = b[3 + a];
+            }
+    }
+    return build_from_boxes(std::move(bx), opt);
 }
 
 } // namespace btrc

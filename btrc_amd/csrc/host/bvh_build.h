@@ -28,7 +28,11 @@ struct Bvh4
     bool root_is_leaf = false;           // true when the whole scene is one leaf
 };
 
-// vertices: 9 floats (v0, v1, v2) per triangle, world space
+// vertices: 9 floats (v0, v1, v2) per triangle
 Bvh4 build_bvh4(const float *vertices, uint32_t triangle_count, const BvhBuildOptions &opt);
+
+// boxes: 6 floats (lo xyz, hi xyz) per primitive; an empty box (lo > hi)
+// is allowed and never hit
+Bvh4 build_bvh4_boxes(const float *boxes, uint32_t count, const BvhBuildOptions &opt);
 
 } // namespace btrc

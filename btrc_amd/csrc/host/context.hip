@@ -174,6 +174,8 @@ struct btrc_ctx
     bool has_scene = false;
     DevBuf<BvhNode4> nodes;
     DevBuf<float4> tris;
+    DevBuf<TraceInstance> trace_inst;
+    bool two_level = false;
     DevBuf<InstanceRec> instances;
     DevBuf<GeometryRec> geometries;
     DevBuf<MaterialRec> materials;
@@ -251,7 +253,14 @@ struct btrc_ctx
     DevBuf<uint32_t> batch_out;
     int64_t batch_n = 0;
 
-    TraceScene trace_scene() const { return TraceScene{ nodes.ptr, tris.ptr }; }
+    TraceScene trace_scene() const { return TraceScene{ nodes.ptr, tris.ptr, trace_inst.ptr }; }
+    // traversal schedule of closest-hit (any = false) or any-hit rays
+    int trace_sched(bool any) const
+    {
+        if(two_level)
+            return SCHED_TWO_LEVEL;
+        return (any ? tuning.any_postpone_leaves : tuning.closest_postpone_leaves) != 0 ? SCHED_POSTPONE : SCHED_EAGER;
+    }
     TraceOverflow trace_overflow() const
     {
         return TraceOverflow{ ovf_code.ptr, ovf_t.ptr, trace_grid * TRACE_BLOCK };
@@ -421,6 +430,207 @@ void size_traversal(btrc_ctx *ctx)
     ctx->ovf_code.alloc(std::max<size_t>(1, deep * ctx->trace_grid * TRACE_BLOCK));
     ctx->ovf_t.alloc(std::max<size_t>(1, deep * ctx->trace_grid * TRACE_BLOCK));
     ctx->fetch.alloc(1);
+}
+
+// device layout of the traversal structure
+struct Accel
+{
+    std::vector<BvhNode4> nodes;     // child slots hold traversal codes
+    std::vector<float4> tris;        // 3 per triangle in leaf order
+    std::vector<TraceInstance> inst; // two levels only
+    int depth = 0;
+    int stack_bound = 0;
+    float lo[3] = { MAX_FLOAT, MAX_FLOAT, MAX_FLOAT }, hi[3] = { -MAX_FLOAT, -MAX_FLOAT, -MAX_FLOAT };
+};
+
+// child slots as traversal codes: inner index (+ node_base), leaf code of
+// triangles from tri_base, or CODE_DONE
+void append_nodes(Accel &acc, const Bvh4 &bvh, uint32_t tri_base)
+{
+    const uint32_t node_base = (uint32_t)acc.nodes.size();
+    for(BvhNode4 nd : bvh.nodes)
+    {
+        for(int j = 0; j < 4; ++j)
+        {
+            uint32_t code;
+            if(nd.child[j] >= 0)
+                code = node_base + (uint32_t)nd.child[j];
+            else if(nd.count[j] > 0)
+                code = CODE_LEAF | ((uint32_t)(nd.count[j] - 1) << 28) | (tri_base + (uint32_t)~nd.child[j]);
+            else
+                code = CODE_DONE;
+            nd.child[j] = (int32_t)code;
+        }
+        acc.nodes.push_back(nd);
+    }
+}
+
+// One tree over the world-space triangles; a triangle's record carries its
+// flattened index, instance and primitive.
+Accel build_flat(const std::vector<float> &world, const std::vector<uint32_t> &tri_inst,
+                 const std::vector<uint32_t> &tri_prim, uint32_t ntri)
+{
+    if(ntri > MAX_LEAF_FIRST)
+        throw InvalidArg("too many triangles for the traversal's leaf encoding");
+    BvhBuildOptions opt;
+    opt.max_leaf_size = 4; // leaf codes carry counts 1..4
+    const Bvh4 bvh = build_bvh4(world.data(), ntri, opt);
+    Accel acc;
+    acc.tris.resize(3 * (size_t)ntri);
+    for(uint32_t k = 0; k < ntri; ++k)
+    {
+        const uint32_t gid = bvh.prim_order[k];
+        const float *w = world.data() + 9 * (size_t)gid;
+        acc.tris[3 * k + 0] = make_float4(w[0], w[1], w[2], u2f(gid));
+        acc.tris[3 * k + 1] = make_float4(w[3], w[4], w[5], u2f(tri_inst[gid]));
+        acc.tris[3 * k + 2] = make_float4(w[6], w[7], w[8], u2f(tri_prim[gid]));
+    }
+    append_nodes(acc, bvh, 0);
+    acc.depth = bvh.depth;
+    acc.stack_bound = bvh.stack_bound;
+    for(size_t k = 0; k < world.size(); ++k)
+    {
+        acc.lo[k % 3] = std::min(acc.lo[k % 3], world[k]);
+        acc.hi[k % 3] = std::max(acc.hi[k % 3], world[k]);
+    }
+    return acc;
+}
+
+// Relative padding of the instance boxes: a ray moved into an instance's
+// local space carries rounding of order 2^-24 times the transform's
+// condition and the coordinates' magnitude; 2^-14 of the largest world
+// coordinate keeps the instance tree conservative for transforms with a
+// condition up to about 2^8 and ray origins within a few scene extents.
+constexpr float INSTANCE_BOX_PAD = 1.0f / 16384.0f;
+
+// The instance tree (node 0) over one tree per geometry: geometry trees
+// hold local-space triangles tagged with their primitive index, instance
+// leaves name a TraceInstance (world_to_local rows, geometry root, first
+// flattened index). The stack bound adds the instance tree's, the exit
+// marker and the deepest geometry tree's.
+Accel build_two_level(const btrc_scene_desc *sd, const std::vector<InstanceRec> &insts)
+{
+    const int32_t G = sd->geometry_count;
+    const int32_t I = sd->instance_count;
+    if((uint32_t)I > 0x0fffffffu)
+        throw InvalidArg("too many instances for the instance tree's leaf encoding");
+    uint64_t total = 0;
+    for(int32_t gi = 0; gi < G; ++gi)
+        total += (uint64_t)sd->geometries[gi].triangle_count;
+    if(total > MAX_LEAF_FIRST)
+        throw InvalidArg("too many geometry triangles for the traversal's leaf encoding");
+
+    // local bounds of each geometry
+    std::vector<float> gbox(6 * (size_t)G);
+    for(int32_t gi = 0; gi < G; ++gi)
+    {
+        const btrc_geometry_desc &g = sd->geometries[gi];
+        float *b = &gbox[6 * (size_t)gi];
+        b[0] = b[1] = b[2] = MAX_FLOAT;
+        b[3] = b[4] = b[5] = -MAX_FLOAT;
+        for(size_t k = 0; k < 9 * (size_t)g.triangle_count; ++k)
+        {
+            b[k % 3] = std::min(b[k % 3], g.positions[k]);
+            b[3 + k % 3] = std::max(b[3 + k % 3], g.positions[k]);
+        }
+    }
+    // world bounds of each instance: its geometry box's corners, transformed
+    Accel acc;
+    std::vector<float> ibox(6 * (size_t)I);
+    float mag = 0.0f;
+    for(int32_t i = 0; i < I; ++i)
+    {
+        const float *b = &gbox[6 * (size_t)insts[i].geometry_id];
+        float *w = &ibox[6 * (size_t)i];
+        w[0] = w[1] = w[2] = MAX_FLOAT;
+        w[3] = w[4] = w[5] = -MAX_FLOAT;
+        if(!(b[0] <= b[3]))
+            continue; // no triangles: never hit
+        for(int c = 0; c < 8; ++c)
+        {
+            const V3 p = xf_point(insts[i].xf, v3(b[(c & 1) ? 3 : 0], b[(c & 2) ? 4 : 1], b[(c & 4) ? 5 : 2]));
+            const float pc[3] = { p.x, p.y, p.z };
+            for(int a = 0; a < 3; ++a)
+            {
+                w[a] = std::min(w[a], pc[a]);
+                w[3 + a] = std::max(w[3 + a], pc[a]);
+            }
+        }
+        for(int a = 0; a < 6; ++a)
+            mag = std::max(mag, std::fabs(w[a]));
+        // the scene box over the transformed vertices, as one level has it
+        const btrc_geometry_desc &g = sd->geometries[insts[i].geometry_id];
+        for(size_t k = 0; k < 3 * (size_t)g.triangle_count; ++k)
+        {
+            const float *q = g.positions + 3 * k;
+            const V3 p = xf_point(insts[i].xf, v3(q[0], q[1], q[2]));
+            acc.lo[0] = std::min(acc.lo[0], p.x); acc.hi[0] = std::max(acc.hi[0], p.x);
+            acc.lo[1] = std::min(acc.lo[1], p.y); acc.hi[1] = std::max(acc.hi[1], p.y);
+            acc.lo[2] = std::min(acc.lo[2], p.z); acc.hi[2] = std::max(acc.hi[2], p.z);
+        }
+    }
+    const float pad = mag * INSTANCE_BOX_PAD;
+    for(int32_t i = 0; i < I; ++i)
+    {
+        float *w = &ibox[6 * (size_t)i];
+        if(w[0] <= w[3])
+            for(int a = 0; a < 3; ++a)
+            {
+                w[a] -= pad;
+                w[3 + a] += pad;
+            }
+    }
+
+    BvhBuildOptions topt;
+    topt.max_leaf_size = 1; // a leaf is one instance
+    const Bvh4 tlas = build_bvh4_boxes(ibox.data(), (uint32_t)I, topt);
+    append_nodes(acc, tlas, 0);
+    for(BvhNode4 &nd : acc.nodes)
+        for(int j = 0; j < 4; ++j)
+        {
+            const uint32_t code = (uint32_t)nd.child[j];
+            if(code != CODE_DONE && (code & CODE_LEAF))
+                nd.child[j] = (int32_t)(CODE_LEAF | CODE_INST | tlas.prim_order[code & 0x0fffffffu]);
+        }
+
+    // one tree per geometry, local-space triangles
+    std::vector<uint32_t> groot(G);
+    int blas_stack = 0, blas_depth = 0;
+    BvhBuildOptions bopt;
+    bopt.max_leaf_size = 4;
+    for(int32_t gi = 0; gi < G; ++gi)
+    {
+        const btrc_geometry_desc &g = sd->geometries[gi];
+        const uint32_t n = (uint32_t)g.triangle_count;
+        const Bvh4 blas = build_bvh4(g.positions, n, bopt);
+        const uint32_t tri_base = (uint32_t)(acc.tris.size() / 3);
+        groot[gi] = (uint32_t)acc.nodes.size();
+        append_nodes(acc, blas, tri_base);
+        for(uint32_t k = 0; k < n; ++k)
+        {
+            const uint32_t prim = blas.prim_order[k];
+            const float *p = g.positions + 9 * (size_t)prim;
+            acc.tris.push_back(make_float4(p[0], p[1], p[2], u2f(prim)));
+            acc.tris.push_back(make_float4(p[3], p[4], p[5], 0.0f));
+            acc.tris.push_back(make_float4(p[6], p[7], p[8], 0.0f));
+        }
+        blas_stack = std::max(blas_stack, blas.stack_bound);
+        blas_depth = std::max(blas_depth, blas.depth);
+    }
+    if(acc.nodes.size() >= (size_t)CODE_INST)
+        throw InvalidArg("too many tree nodes for the traversal's node encoding");
+    acc.inst.resize(I);
+    for(int32_t i = 0; i < I; ++i)
+    {
+        TraceInstance &t = acc.inst[i];
+        std::memcpy(t.w2l, insts[i].xf.n, sizeof(t.w2l));
+        t.blas_root = groot[insts[i].geometry_id];
+        t.prim_base = insts[i].prim_base;
+        t.pad[0] = t.pad[1] = 0;
+    }
+    acc.depth = tlas.depth + blas_depth;
+    acc.stack_bound = tlas.stack_bound + 1 + blas_stack;
+    return acc;
 }
 
 void upload_scene(btrc_ctx *ctx, const btrc_scene_desc *sd)
@@ -600,7 +810,11 @@ void upload_scene(btrc_ctx *ctx, const btrc_scene_desc *sd)
         return medium_id[idx];
     };
 
-    // instances, lights and the flattened world-space triangle list
+    if(sd->accel != BTRC_ACCEL_FLAT && sd->accel != BTRC_ACCEL_TWO_LEVEL)
+        throw InvalidArg("unknown acceleration structure type");
+    const bool two_level = sd->accel == BTRC_ACCEL_TWO_LEVEL;
+
+    // instances, lights and (one level) the flattened world-space triangle list
     std::vector<InstanceRec> insts(sd->instance_count);
     std::vector<LightRec> lights;
     std::vector<float> world;
@@ -636,7 +850,7 @@ void upload_scene(btrc_ctx *ctx, const btrc_scene_desc *sd)
         }
         insts[i] = r;
         const btrc_geometry_desc &g = sd->geometries[d.geometry];
-        for(int32_t t = 0; t < g.triangle_count; ++t)
+        for(int32_t t = 0; t < (two_level ? 0 : g.triangle_count); ++t)
         {
             const float *p = g.positions + 9 * (size_t)t;
             for(int k = 0; k < 3; ++k)
@@ -672,34 +886,12 @@ void upload_scene(btrc_ctx *ctx, const btrc_scene_desc *sd)
         throw InvalidArg("unknown environment light type");
 
     const uint32_t ntri = prim_base;
-    if(ntri > MAX_LEAF_FIRST)
-        throw InvalidArg("too many triangles for the traversal's leaf encoding");
-    BvhBuildOptions opt;
-    opt.max_leaf_size = 4; // leaf codes carry counts 1..4
-    Bvh4 bvh = build_bvh4(world.data(), ntri, opt);
-    std::vector<float4> tri_data(3 * (size_t)ntri);
-    for(uint32_t k = 0; k < ntri; ++k)
-    {
-        const uint32_t gid = bvh.prim_order[k];
-        const float *w = world.data() + 9 * (size_t)gid;
-        tri_data[3 * k + 0] = make_float4(w[0], w[1], w[2], u2f(gid));
-        tri_data[3 * k + 1] = make_float4(w[3], w[4], w[5], u2f(tri_inst[gid]));
-        tri_data[3 * k + 2] = make_float4(w[6], w[7], w[8], u2f(tri_prim[gid]));
-    }
+    Accel acc = two_level ? build_two_level(sd, insts) : build_flat(world, tri_inst, tri_prim, ntri);
 
     ctx->has_scene = false;
-    // children as traversal codes: inner index, leaf code, or CODE_DONE
-    for(BvhNode4 &nd : bvh.nodes)
-        for(int j = 0; j < 4; ++j)
-        {
-            uint32_t code = (uint32_t)nd.child[j];
-            if(nd.child[j] < 0)
-                code = nd.count[j] > 0 ? (CODE_LEAF | ((uint32_t)(nd.count[j] - 1) << 28) | (uint32_t)~nd.child[j])
-                                       : CODE_DONE;
-            nd.child[j] = (int32_t)code;
-        }
-    ctx->nodes.upload(bvh.nodes.data(), bvh.nodes.size(), s);
-    ctx->tris.upload(tri_data.data(), tri_data.size(), s);
+    ctx->nodes.upload(acc.nodes.data(), acc.nodes.size(), s);
+    ctx->tris.upload(acc.tris.data(), acc.tris.size(), s);
+    ctx->trace_inst.upload(acc.inst.data(), acc.inst.size(), s);
     ctx->instances.upload(insts.data(), insts.size(), s);
     ctx->geometries.upload(geos.data(), geos.size(), s);
     ctx->materials.upload(mats.data(), mats.size(), s);
@@ -719,15 +911,10 @@ void upload_scene(btrc_ctx *ctx, const btrc_scene_desc *sd)
     ctx->medium_count = R;
     for(int a = 0; a < 3; ++a)
     {
-        float lo = MAX_FLOAT, hi = -MAX_FLOAT;
-        for(size_t k = a; k < world.size(); k += 3)
-        {
-            lo = std::min(lo, world[k]);
-            hi = std::max(hi, world[k]);
-        }
+        float lo = acc.lo[a], hi = acc.hi[a];
         if(!(hi > lo))
         {
-            lo = world.empty() ? 0.0f : lo;
+            lo = lo <= hi ? lo : 0.0f;
             hi = lo + 1.0f;
         }
         ctx->scene_lo[a] = lo;
@@ -737,9 +924,10 @@ void upload_scene(btrc_ctx *ctx, const btrc_scene_desc *sd)
     ctx->envir_light_index = envir_index;
     ctx->instance_count = sd->instance_count;
     ctx->triangle_count = ntri;
-    ctx->node_count = (int64_t)bvh.nodes.size();
-    ctx->bvh_depth = bvh.depth;
-    ctx->stack_size = bvh.stack_bound + 1;
+    ctx->node_count = (int64_t)acc.nodes.size();
+    ctx->bvh_depth = acc.depth;
+    ctx->stack_size = acc.stack_bound + 1;
+    ctx->two_level = two_level;
     size_traversal(ctx);
     ctx->has_scene = true;
 }
@@ -1116,7 +1304,7 @@ void render(btrc_ctx *ctx, const btrc_render_params *p, btrc_render_stats *stats
         PathSOA next = ctx->pool[cur ^ 1].view();
         launch_plan(s, ctx->counters.ptr, p->state_count);
         timer.run(STAGE_GENERATE, [&] { launch_generate(s, grid256, p->sampler, rc, ctx->counters.ptr, path); });
-        timer.run(STAGE_TRACE, [&] { launch_trace_extension(s, ctx->trace_grid, ctx->tuning.closest_postpone_leaves != 0, ts, ctx->counters.ptr, path, hit, ovf); });
+        timer.run(STAGE_TRACE, [&] { launch_trace_extension(s, ctx->trace_grid, ctx->trace_sched(false), ts, ctx->counters.ptr, path, hit, ovf); });
         if(capture && (int64_t)iter < capture_bounces && captured < capture_max)
         {
             BTRC_HIP_CHECK(hipMemcpyAsync(ctx->host_counters, ctx->counters.ptr, sizeof(WaveCounters), hipMemcpyDeviceToHost, s));
@@ -1139,7 +1327,7 @@ void render(btrc_ctx *ctx, const btrc_render_params *p, btrc_render_stats *stats
         if(media)
             timer.run(STAGE_MEDIUM, [&] { launch_medium(s, grid256, p->sampler, rc, sv, ctx->counters.ptr, path, hit, next, sh, fv); });
         timer.run(STAGE_SHADE, [&] { launch_shade(s, grid256, p->sampler, rc, sv, ctx->counters.ptr, path, hit, next, sh, fv, nv); });
-        timer.run(STAGE_SHADOW, [&] { launch_shadow(s, ctx->trace_grid, ctx->tuning.any_postpone_leaves != 0, media, ts, sv, ctx->counters.ptr, sh, fv, ovf); });
+        timer.run(STAGE_SHADOW, [&] { launch_shadow(s, ctx->trace_grid, ctx->trace_sched(true), media, ts, sv, ctx->counters.ptr, sh, fv, ovf); });
         launch_end_iteration(s, ctx->counters.ptr);
         BTRC_HIP_CHECK(hipGetLastError());
         cur ^= 1;
@@ -1257,8 +1445,7 @@ void check_batch_args(btrc_ctx *ctx, const void *rays, int64_t n, int32_t tri_te
 void trace_batch(btrc_ctx *ctx, int32_t tri_test, bool any_hit, const float4 *rays, int64_t n, uint32_t *out)
 {
     BTRC_HIP_CHECK(hipMemsetAsync(ctx->fetch.ptr, 0, sizeof(uint32_t), ctx->stream));
-    const bool postpone = (any_hit ? ctx->tuning.any_postpone_leaves : ctx->tuning.closest_postpone_leaves) != 0;
-    launch_trace_batch(ctx->stream, ctx->trace_grid, postpone, ctx->trace_scene(), tri_test, any_hit, rays,
+    launch_trace_batch(ctx->stream, ctx->trace_grid, ctx->trace_sched(any_hit != 0), ctx->trace_scene(), tri_test, any_hit, rays,
                        (int32_t)n, out, ctx->fetch.ptr, ctx->trace_overflow());
     BTRC_HIP_CHECK(hipGetLastError());
 }

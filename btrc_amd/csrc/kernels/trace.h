@@ -32,17 +32,39 @@ constexpr int REFILL_MIN = 16;    // idle lanes that trigger a refill
 constexpr int TRACE_WAVES_PER_CU = 20; // persistent grid: wavefronts per compute unit (default)
 constexpr int TRACE_WAVES_PER_CU_MAX = 32;
 
+// traversal schedules: leaves tested when reached, leaves postponed until
+// every lane holds one, or the two-level (instance) structure
+enum : int
+{
+    SCHED_EAGER = 0,
+    SCHED_POSTPONE = 1,
+    SCHED_TWO_LEVEL = 2,
+};
+
 // stack entry / node codes
 constexpr uint32_t CODE_LEAF = 0x80000000u;   // | (count - 1) << 28 | first triangle
+constexpr uint32_t CODE_INST = 0x40000000u;   // with CODE_LEAF: | instance (top-level leaf)
 constexpr uint32_t CODE_DONE = 0xffffffffu;
+constexpr uint32_t CODE_EXIT = 0xfffffffeu;   // stack marker: back to the top level
 constexpr uint32_t MAX_LEAF_FIRST = 0x0ffffffeu;
+
+// an instance of the two-level structure: the top level's leaves point here
+struct TraceInstance
+{
+    float w2l[12];       // world_to_local rows
+    uint32_t blas_root;  // node index of its geometry's tree
+    uint32_t prim_base;  // flattened index of its first triangle
+    uint32_t pad[2];
+};
 
 struct TraceScene
 {
     const BvhNode4 *nodes;
-    // 3 float4 per triangle in leaf order:
-    //   (v0.xyz, flattened index), (v1.xyz, instance), (v2.xyz, prim in geometry)
+    // 3 float4 per triangle in leaf order. One level (world space):
+    //   (v0.xyz, flattened index), (v1.xyz, instance), (v2.xyz, prim in geometry);
+    // two levels (local space): (v0.xyz, prim in geometry), (v1.xyz, 0), (v2.xyz, 0)
     const float4 *tris;
+    const TraceInstance *inst; // two levels only
 };
 
 // per-lane global stack entries below the LDS part: [entry - SHORT_STACK][lane]
@@ -68,6 +90,12 @@ __device__ __forceinline__ uint32_t leaf_code(int32_t first, int32_t count)
 
 
 
+// The two-level variant (TWO): node 0 roots the instance tree, whose leaves
+// are instance codes; entering one pushes CODE_EXIT, moves the ray into the
+// instance's local space (same parametric t) and continues at its
+// geometry's tree; popping CODE_EXIT restores the world ray. Leaves are then
+// tested as soon as they are reached.
+//
 // Traces rays [0, n) handed out through *fetch. Sink provides
 //   void load(uint32_t i, V3 &o, V3 &d, float &tmin, float &tmax)
 //   void store(uint32_t i, const HitRecord &h)
@@ -75,10 +103,11 @@ __device__ __forceinline__ uint32_t leaf_code(int32_t first, int32_t count)
 // while-while schedule (leaves wait until every lane holds one); without it
 // a lane tests a leaf as soon as it meets it, which ends occluded shadow
 // rays earlier.
-template<int TRI, bool ANY, bool POSTPONE, class Sink>
+template<int TRI, bool ANY, bool POSTPONE, bool TWO, class Sink>
 __device__ __forceinline__ void trace_persistent(
     const TraceScene &sc, Sink &sink, int32_t n, uint32_t *fetch, const TraceOverflow &ovf)
 {
+    static_assert(!(TWO && POSTPONE), "two-level traversal tests leaves when it reaches them");
     __shared__ uint32_t s_code[SHORT_STACK * TRACE_BLOCK];
     __shared__ float s_t[SHORT_STACK * TRACE_BLOCK];
     const int lane = (int)threadIdx.x;
@@ -97,6 +126,18 @@ __device__ __forceinline__ void trace_persistent(
     uint32_t node = CODE_DONE;
     int32_t sp = 0;
     bool exhausted = false; // wavefront-uniform: the ray counter has passed n
+    uint32_t cur_inst = 0xffffffffu, cur_base = 0; // two levels: the instance being traversed
+
+    auto set_ray = [&](V3 ro, V3 rd) {
+        rb = raybox_pre(ro, rd);
+        near_x = rb.neg_x ? 16u : 0u;   // hi_x sits 16 bytes after lo_x
+        near_y = rb.neg_y ? 48u : 32u;
+        near_z = rb.neg_z ? 80u : 64u;
+        if(TRI == TRI_WATERTIGHT)
+            wt = watertight_pre(ro, rd);
+        else
+            mt = mt_pre(ro, rd);
+    };
 
     auto push = [&](uint32_t code, float t) {
         if(sp < SHORT_STACK)
@@ -131,6 +172,12 @@ __device__ __forceinline__ void trace_persistent(
                 code = ovf.code[k];
                 t = ovf.t[k];
             }
+            if(TWO && code == CODE_EXIT)
+            {
+                set_ray(o, d); // back to the world ray
+                cur_inst = 0xffffffffu;
+                continue;
+            }
             if(t <= cull)
                 return code;
         }
@@ -147,7 +194,7 @@ __device__ __forceinline__ void trace_persistent(
             const float4 a = sc.tris[3 * (first + q) + 0];
             const float4 b = sc.tris[3 * (first + q) + 1];
             const float4 cc = sc.tris[3 * (first + q) + 2];
-            const uint32_t gid = __float_as_uint(a.w);
+            const uint32_t gid = TWO ? cur_base + __float_as_uint(a.w) : __float_as_uint(a.w);
             float t, u, v;
             bool ok;
             if(TRI == TRI_WATERTIGHT)
@@ -160,8 +207,8 @@ __device__ __forceinline__ void trace_persistent(
                 h.b1 = u;
                 h.b2 = v;
                 h.gid = gid;
-                h.inst = __float_as_uint(b.w);
-                h.prim = __float_as_uint(cc.w);
+                h.inst = TWO ? cur_inst : __float_as_uint(b.w);
+                h.prim = TWO ? __float_as_uint(a.w) : __float_as_uint(cc.w);
                 any_hit = true;
                 if(ANY)
                     break;
@@ -191,14 +238,8 @@ __device__ __forceinline__ void trace_persistent(
                     ray = idx;
                     float tmax;
                     sink.load(idx, o, d, tmin, tmax);
-                    rb = raybox_pre(o, d);
-                    near_x = rb.neg_x ? 16u : 0u;   // hi_x sits 16 bytes after lo_x
-                    near_y = rb.neg_y ? 48u : 32u;
-                    near_z = rb.neg_z ? 80u : 64u;
-                    if(TRI == TRI_WATERTIGHT)
-                        wt = watertight_pre(o, d);
-                    else
-                        mt = mt_pre(o, d);
+                    set_ray(o, d);
+                    cur_inst = 0xffffffffu;
                     h.t = tmax;
                     h.b1 = h.b2 = 0.0f;
                     h.gid = h.inst = h.prim = 0xffffffffu;
@@ -220,7 +261,24 @@ __device__ __forceinline__ void trace_persistent(
                 break;
             if(step)
             {
-                if(node & CODE_LEAF)
+                if(TWO && (node & CODE_LEAF) && (node & CODE_INST))
+                {
+                    // enter an instance: world -> local, continue in its tree
+                    const TraceInstance &ti = sc.inst[node & 0x0fffffffu];
+                    push(CODE_EXIT, -INFINITY);
+                    const float *r = ti.w2l;
+                    const V3 lo = v3(r[0] * o.x + r[1] * o.y + r[2] * o.z + r[3],
+                                     r[4] * o.x + r[5] * o.y + r[6] * o.z + r[7],
+                                     r[8] * o.x + r[9] * o.y + r[10] * o.z + r[11]);
+                    const V3 ld = v3(r[0] * d.x + r[1] * d.y + r[2] * d.z,
+                                     r[4] * d.x + r[5] * d.y + r[6] * d.z,
+                                     r[8] * d.x + r[9] * d.y + r[10] * d.z);
+                    set_ray(lo, ld);
+                    cur_inst = node & 0x0fffffffu;
+                    cur_base = ti.prim_base;
+                    node = ti.blas_root;
+                }
+                else if(node & CODE_LEAF)
                 {
                     if(POSTPONE)
                     {

@@ -141,20 +141,20 @@ enum : int
 void launch_plan(hipStream_t s, WaveCounters *c, int32_t state_count);
 void launch_generate(hipStream_t s, int grid, int sampler, const RenderConsts &rc, const WaveCounters *c,
                      PathSOA path);
-void launch_trace_extension(hipStream_t s, int grid, bool postpone, const TraceScene &ts, WaveCounters *c,
+void launch_trace_extension(hipStream_t s, int grid, int sched, const TraceScene &ts, WaveCounters *c,
                             PathSOA path, HitSOA hit, const TraceOverflow &ovf);
 void launch_shade(hipStream_t s, int grid, int sampler, const RenderConsts &rc, const SceneView &sv,
                   WaveCounters *c, PathSOA path, HitSOA hit, PathSOA next, ShadowSOA shadow, FilmView film,
                   const NrcView &nrc);
 void launch_medium(hipStream_t s, int grid, int sampler, const RenderConsts &rc, const SceneView &sv,
                    WaveCounters *c, PathSOA path, HitSOA hit, PathSOA next, ShadowSOA shadow, FilmView film);
-void launch_shadow(hipStream_t s, int grid, bool postpone, bool media, const TraceScene &ts, const SceneView &sv,
+void launch_shadow(hipStream_t s, int grid, int sched, bool media, const TraceScene &ts, const SceneView &sv,
                    WaveCounters *c, ShadowSOA shadow, FilmView film, const TraceOverflow &ovf);
 void launch_end_iteration(hipStream_t s, WaveCounters *c);
 void launch_resolve(hipStream_t s, int mode, const float *value, int32_t pixels, float inv_weight, float4 *out);
 
 // ray batches of the traversal entry points: o.xyz/tmin and d.xyz/tmax
-void launch_trace_batch(hipStream_t s, int grid, bool postpone, const TraceScene &ts, int tri_test, bool any_hit,
+void launch_trace_batch(hipStream_t s, int grid, int sched, const TraceScene &ts, int tri_test, bool any_hit,
                         const float4 *rays, int32_t n, uint32_t *out, uint32_t *fetch, const TraceOverflow &ovf);
 
 // test hooks

@@ -272,12 +272,13 @@ struct ExtensionSink
     }
 };
 
-template<bool POSTPONE>
+template<int SCHED>
 __global__ void __launch_bounds__(TRACE_BLOCK) k_trace_extension(
     TraceScene ts, WaveCounters *c, PathSOA path, HitSOA hit, TraceOverflow ovf)
 {
     ExtensionSink sink{ path, hit };
-    trace_persistent<TRI_WATERTIGHT, false, POSTPONE>(ts, sink, c->active, &c->trace_fetch, ovf);
+    trace_persistent<TRI_WATERTIGHT, false, SCHED == SCHED_POSTPONE, SCHED == SCHED_TWO_LEVEL>(
+        ts, sink, c->active, &c->trace_fetch, ovf);
 }
 
 // ---------------------------------------------------------------- shade
@@ -739,12 +740,13 @@ struct ShadowSink
     }
 };
 
-template<bool POSTPONE, bool MEDIA>
+template<int SCHED, bool MEDIA>
 __global__ void __launch_bounds__(TRACE_BLOCK) k_shadow(
     TraceScene ts, SceneView sv, WaveCounters *c, ShadowSOA shadow, FilmView film, TraceOverflow ovf)
 {
     ShadowSink<MEDIA> sink{ shadow, film, sv };
-    trace_persistent<TRI_WATERTIGHT, true, POSTPONE>(ts, sink, c->shadow, &c->shadow_fetch, ovf);
+    trace_persistent<TRI_WATERTIGHT, true, SCHED == SCHED_POSTPONE, SCHED == SCHED_TWO_LEVEL>(
+        ts, sink, c->shadow, &c->shadow_fetch, ovf);
 }
 
 // ---------------------------------------------------------------- bookkeeping
@@ -806,12 +808,12 @@ struct BatchSink
     }
 };
 
-template<int TRI, bool ANY, bool POSTPONE>
+template<int TRI, bool ANY, int SCHED>
 __global__ void __launch_bounds__(TRACE_BLOCK) k_trace_batch(
     TraceScene ts, const float4 *rays, int32_t n, uint32_t *out, uint32_t *fetch, TraceOverflow ovf)
 {
     BatchSink<ANY> sink{ rays, out };
-    trace_persistent<TRI, ANY, POSTPONE>(ts, sink, n, fetch, ovf);
+    trace_persistent<TRI, ANY, SCHED == SCHED_POSTPONE, SCHED == SCHED_TWO_LEVEL>(ts, sink, n, fetch, ovf);
 }
 
 // ---------------------------------------------------------------- test hooks
@@ -896,13 +898,16 @@ void launch_generate(hipStream_t s, int grid, int sampler, const RenderConsts &r
         hipLaunchKernelGGL(k_generate<SAMPLER_HALTON>, dim3(grid), dim3(256), 0, s, rc, c, path);
 }
 
-void launch_trace_extension(hipStream_t s, int grid, bool postpone, const TraceScene &ts, WaveCounters *c,
+void launch_trace_extension(hipStream_t s, int grid, int sched, const TraceScene &ts, WaveCounters *c,
                             PathSOA path, HitSOA hit, const TraceOverflow &ovf)
 {
-    if(postpone)
-        hipLaunchKernelGGL(k_trace_extension<true>, dim3(grid), dim3(TRACE_BLOCK), 0, s, ts, c, path, hit, ovf);
+    const dim3 g(grid), b(TRACE_BLOCK);
+    if(sched == SCHED_TWO_LEVEL)
+        hipLaunchKernelGGL(k_trace_extension<SCHED_TWO_LEVEL>, g, b, 0, s, ts, c, path, hit, ovf);
+    else if(sched == SCHED_POSTPONE)
+        hipLaunchKernelGGL(k_trace_extension<SCHED_POSTPONE>, g, b, 0, s, ts, c, path, hit, ovf);
     else
-        hipLaunchKernelGGL(k_trace_extension<false>, dim3(grid), dim3(TRACE_BLOCK), 0, s, ts, c, path, hit, ovf);
+        hipLaunchKernelGGL(k_trace_extension<SCHED_EAGER>, g, b, 0, s, ts, c, path, hit, ovf);
 }
 
 void launch_shade(hipStream_t s, int grid, int sampler, const RenderConsts &rc, const SceneView &sv,
@@ -928,18 +933,26 @@ void launch_medium(hipStream_t s, int grid, int sampler, const RenderConsts &rc,
                            shadow, film);
 }
 
-void launch_shadow(hipStream_t s, int grid, bool postpone, bool media, const TraceScene &ts, const SceneView &sv,
-                   WaveCounters *c, ShadowSOA shadow, FilmView film, const TraceOverflow &ovf)
+template<int SCHED>
+static void launch_shadow_t(hipStream_t s, int grid, bool media, const TraceScene &ts, const SceneView &sv,
+                            WaveCounters *c, ShadowSOA shadow, FilmView film, const TraceOverflow &ovf)
 {
     const dim3 g(grid), b(TRACE_BLOCK);
-    if(postpone && media)
-        hipLaunchKernelGGL((k_shadow<true, true>), g, b, 0, s, ts, sv, c, shadow, film, ovf);
-    else if(postpone)
-        hipLaunchKernelGGL((k_shadow<true, false>), g, b, 0, s, ts, sv, c, shadow, film, ovf);
-    else if(media)
-        hipLaunchKernelGGL((k_shadow<false, true>), g, b, 0, s, ts, sv, c, shadow, film, ovf);
+    if(media)
+        hipLaunchKernelGGL((k_shadow<SCHED, true>), g, b, 0, s, ts, sv, c, shadow, film, ovf);
     else
-        hipLaunchKernelGGL((k_shadow<false, false>), g, b, 0, s, ts, sv, c, shadow, film, ovf);
+        hipLaunchKernelGGL((k_shadow<SCHED, false>), g, b, 0, s, ts, sv, c, shadow, film, ovf);
+}
+
+void launch_shadow(hipStream_t s, int grid, int sched, bool media, const TraceScene &ts, const SceneView &sv,
+                   WaveCounters *c, ShadowSOA shadow, FilmView film, const TraceOverflow &ovf)
+{
+    if(sched == SCHED_TWO_LEVEL)
+        launch_shadow_t<SCHED_TWO_LEVEL>(s, grid, media, ts, sv, c, shadow, film, ovf);
+    else if(sched == SCHED_POSTPONE)
+        launch_shadow_t<SCHED_POSTPONE>(s, grid, media, ts, sv, c, shadow, film, ovf);
+    else
+        launch_shadow_t<SCHED_EAGER>(s, grid, media, ts, sv, c, shadow, film, ovf);
 }
 
 void launch_end_iteration(hipStream_t s, WaveCounters *c)
@@ -953,31 +966,34 @@ void launch_resolve(hipStream_t s, int mode, const float *value, int32_t pixels,
 }
 
 template<int TRI, bool ANY>
-static void launch_batch_t(hipStream_t s, int grid, bool postpone, const TraceScene &ts, const float4 *rays,
+static void launch_batch_t(hipStream_t s, int grid, int sched, const TraceScene &ts, const float4 *rays,
                            int32_t n, uint32_t *out, uint32_t *fetch, const TraceOverflow &ovf)
 {
-    if(postpone)
-        hipLaunchKernelGGL((k_trace_batch<TRI, ANY, true>), dim3(grid), dim3(TRACE_BLOCK), 0, s, ts, rays, n, out, fetch, ovf);
+    const dim3 g(grid), b(TRACE_BLOCK);
+    if(sched == SCHED_TWO_LEVEL)
+        hipLaunchKernelGGL((k_trace_batch<TRI, ANY, SCHED_TWO_LEVEL>), g, b, 0, s, ts, rays, n, out, fetch, ovf);
+    else if(sched == SCHED_POSTPONE)
+        hipLaunchKernelGGL((k_trace_batch<TRI, ANY, SCHED_POSTPONE>), g, b, 0, s, ts, rays, n, out, fetch, ovf);
     else
-        hipLaunchKernelGGL((k_trace_batch<TRI, ANY, false>), dim3(grid), dim3(TRACE_BLOCK), 0, s, ts, rays, n, out, fetch, ovf);
+        hipLaunchKernelGGL((k_trace_batch<TRI, ANY, SCHED_EAGER>), g, b, 0, s, ts, rays, n, out, fetch, ovf);
 }
 
-void launch_trace_batch(hipStream_t s, int grid, bool postpone, const TraceScene &ts, int tri_test, bool any_hit,
+void launch_trace_batch(hipStream_t s, int grid, int sched, const TraceScene &ts, int tri_test, bool any_hit,
                         const float4 *rays, int32_t n, uint32_t *out, uint32_t *fetch, const TraceOverflow &ovf)
 {
     if(tri_test == TRI_WATERTIGHT)
     {
         if(any_hit)
-            launch_batch_t<TRI_WATERTIGHT, true>(s, grid, postpone, ts, rays, n, out, fetch, ovf);
+            launch_batch_t<TRI_WATERTIGHT, true>(s, grid, sched, ts, rays, n, out, fetch, ovf);
         else
-            launch_batch_t<TRI_WATERTIGHT, false>(s, grid, postpone, ts, rays, n, out, fetch, ovf);
+            launch_batch_t<TRI_WATERTIGHT, false>(s, grid, sched, ts, rays, n, out, fetch, ovf);
     }
     else
     {
         if(any_hit)
-            launch_batch_t<TRI_MOLLER_TRUMBORE, true>(s, grid, postpone, ts, rays, n, out, fetch, ovf);
+            launch_batch_t<TRI_MOLLER_TRUMBORE, true>(s, grid, sched, ts, rays, n, out, fetch, ovf);
         else
-            launch_batch_t<TRI_MOLLER_TRUMBORE, false>(s, grid, postpone, ts, rays, n, out, fetch, ovf);
+            launch_batch_t<TRI_MOLLER_TRUMBORE, false>(s, grid, sched, ts, rays, n, out, fetch, ovf);
     }
 }
 

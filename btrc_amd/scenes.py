@@ -100,6 +100,7 @@ class Scene:
         self.instances = []    # dicts
         self.media = []        # dicts
         self.envir = None      # (up, lower, upper)
+        self.accel = _abi.ACCEL_FLAT
 
     def add_geometry(self, positions, normals=None, uvs=None):
         positions = np.ascontiguousarray(positions, dtype=np.float32).reshape(-1, 9)
@@ -240,6 +241,7 @@ class Scene:
         sd.instances = idesc
         sd.medium_count = len(self.media)
         sd.media = med
+        sd.accel = self.accel
         if self.envir is not None:
             up, lower, upper = self.envir
             sd.envir_light.type = _abi.ENVIR_GRADIENT_SKY

@@ -54,6 +54,12 @@ enum
 
 enum
 {
+    BTRC_ACCEL_FLAT = 0,      /* one tree over world-space triangles   */
+    BTRC_ACCEL_TWO_LEVEL = 1  /* instance tree over per-geometry trees */
+};
+
+enum
+{
     BTRC_ENVIR_NONE = 0,
     BTRC_ENVIR_GRADIENT_SKY = 1 /* builtin/light/gradient_sky.cpp        */
 };
@@ -152,6 +158,11 @@ typedef struct btrc_scene_desc
     btrc_envir_light_desc envir_light;
     int32_t medium_count;
     const btrc_medium_desc *media;
+    /* BTRC_ACCEL_*: FLAT traces one tree over the world-space triangles;
+     * TWO_LEVEL traces an instance tree over one tree per geometry (the
+     * TLAS/BLAS split of Scene::commit, core/scene.cpp:92-239), transforming
+     * the ray into each instance's local space. */
+    int32_t accel;
 } btrc_scene_desc;
 
 /* PinholeCamera (builtin/camera/pinhole.cpp) */

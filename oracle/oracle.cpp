@@ -490,6 +490,12 @@ struct Scene
     // flattened world triangles
     std::vector<Vec> w0, w1, w2;
     std::vector<uint32_t> tinst, tprim;
+    // two levels: local triangles per geometry (off + prim), and per
+    // instance its first flattened index and padded world box
+    bool two_level = false;
+    std::vector<Vec> l0, l1, l2;
+    std::vector<uint32_t> ibase;
+    std::vector<float> ibox; // 6 per instance
     // binary BVH
     struct Node { float lo[3], hi[3]; int32_t left, right; uint32_t first, count; };
     std::vector<Node> nodes;
@@ -607,6 +613,9 @@ static Scene *make_scene(const btrc_scene_desc *sd)
             s->sba.push_back(sub(nz[1], nz[0]));
             s->sca.push_back(sub(nz[2], nz[0]));
             const Vec ba = sub(b, a), ca = sub(c, a);
+            s->l0.push_back(a);
+            s->l1.push_back(b);
+            s->l2.push_back(c);
             s->la.push_back(a);
             s->lba.push_back(ba);
             s->lca.push_back(ca);
@@ -712,6 +721,7 @@ static Scene *make_scene(const btrc_scene_desc *sd)
             s->lights.push_back(l);
         }
         s->insts.push_back(in);
+        s->ibase.push_back((uint32_t)s->w0.size());
         const btrc_geometry_desc &g = sd->geometries[d.geometry];
         for(int t = 0; t < g.triangle_count; ++t)
         {
@@ -735,6 +745,43 @@ static Scene *make_scene(const btrc_scene_desc *sd)
         s->lights.push_back(l);
     }
     build_bvh(*s);
+    s->two_level = sd->accel == BTRC_ACCEL_TWO_LEVEL;
+    if(s->two_level)
+    {
+        // instance world boxes: the geometry box's transformed corners,
+        // padded by 2^-14 of the largest coordinate (a culling aid only:
+        // the result is the minimum over all instances either way)
+        float mag = 0.0f;
+        s->ibox.assign(6 * s->insts.size(), 0.0f);
+        for(size_t i = 0; i < s->insts.size(); ++i)
+        {
+            const Inst &in = s->insts[i];
+            const Geo &g = s->geos[in.geo];
+            float *w = &s->ibox[6 * i];
+            w[0] = w[1] = w[2] = FMAX;
+            w[3] = w[4] = w[5] = -FMAX;
+            for(uint32_t t = g.off; t < g.off + g.n; ++t)
+                for(const Vec &v : { s->l0[t], s->l1[t], s->l2[t] })
+                {
+                    const Vec p = xpoint(in.xf, v);
+                    for(int a = 0; a < 3; ++a)
+                    {
+                        w[a] = std::min(w[a], el(p, a));
+                        w[3 + a] = std::max(w[3 + a], el(p, a));
+                    }
+                }
+            if(g.n)
+                for(int a = 0; a < 6; ++a)
+                    mag = std::max(mag, std::fabs(w[a]));
+        }
+        for(size_t i = 0; i < s->insts.size(); ++i)
+            if(s->geos[s->insts[i].geo].n)
+                for(int a = 0; a < 3; ++a)
+                {
+                    s->ibox[6 * i + a] -= mag / 16384.0f;
+                    s->ibox[6 * i + 3 + a] += mag / 16384.0f;
+                }
+    }
     return s;
 }
 
@@ -827,8 +874,68 @@ static bool box_hit(const Scene::Node &nd, Vec o, Vec inv, Vec invp, float tmin,
 }
 
 
+// world_to_local rows applied to a point / vector, in the operation order of
+// the traversal kernel
+static inline Vec xinv_point(const Xf &t, Vec p)
+{
+    return mk(t.n[0] * p.x + t.n[1] * p.y + t.n[2] * p.z + t.n[3],
+              t.n[4] * p.x + t.n[5] * p.y + t.n[6] * p.z + t.n[7],
+              t.n[8] * p.x + t.n[9] * p.y + t.n[10] * p.z + t.n[11]);
+}
+static inline Vec xinv_vec(const Xf &t, Vec v)
+{
+    return mk(t.n[0] * v.x + t.n[1] * v.y + t.n[2] * v.z,
+              t.n[4] * v.x + t.n[5] * v.y + t.n[6] * v.z,
+              t.n[8] * v.x + t.n[9] * v.y + t.n[10] * v.z);
+}
+
+// Two levels (Scene::commit's instance structure, core/scene.cpp:92-239):
+// every instance's triangles are tested in its local space against the ray
+// moved there; the closest hit is still the minimum of (t, flattened index).
+static Hit trace_two_level(const Scene &s, int mode, bool any, bool use_bvh, Vec o, Vec d, float tmin, float tmax)
+{
+    Hit h{ tmax, 0.0f, 0.0f, 0xffffffffu };
+    const Vec inv = mk(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+    auto pad = [](float x) { return std::isfinite(x) ? bitsf(fbits(x) + 2u) : x; };
+    const Vec invp = mk(pad(inv.x), pad(inv.y), pad(inv.z));
+    for(size_t i = 0; i < s.insts.size(); ++i)
+    {
+        const Inst &in = s.insts[i];
+        const Geo &g = s.geos[in.geo];
+        if(use_bvh)
+        {
+            Scene::Node nd{};
+            for(int a = 0; a < 3; ++a)
+            {
+                nd.lo[a] = s.ibox[6 * i + a];
+                nd.hi[a] = s.ibox[6 * i + 3 + a];
+            }
+            const float cull = h.t < FMAX ? h.t * 1.00000095f : h.t;
+            if(!g.n || !box_hit(nd, o, inv, invp, tmin, cull))
+                continue;
+        }
+        const Vec lo = xinv_point(in.xf, o), ld = xinv_vec(in.xf, d);
+        for(uint32_t p = 0; p < g.n; ++p)
+        {
+            const uint32_t t = g.off + p, gid = s.ibase[i] + p;
+            float tt, u, v;
+            const bool ok = mode == BTRC_TRI_WATERTIGHT ? tri_watertight(lo, ld, s.l0[t], s.l1[t], s.l2[t], tmin, h.t, tt, u, v)
+                                                        : tri_mt(lo, ld, s.l0[t], s.l1[t], s.l2[t], tmin, h.t, tt, u, v);
+            if(ok && (tt < h.t || gid < h.gid))
+            {
+                h.t = tt; h.u = u; h.v = v; h.gid = gid;
+                if(any)
+                    return h;
+            }
+        }
+    }
+    return h;
+}
+
 static Hit trace(const Scene &s, int mode, bool any, bool use_bvh, Vec o, Vec d, float tmin, float tmax)
 {
+    if(s.two_level)
+        return trace_two_level(s, mode, any, use_bvh, o, d, tmin, tmax);
     Hit h{ tmax, 0.0f, 0.0f, 0xffffffffu };
     const uint32_t n = (uint32_t)s.w0.size();
     if(!use_bvh || s.nodes.empty())
