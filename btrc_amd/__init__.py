@@ -260,6 +260,13 @@ class Context:
                                                        len(px), dims, out.ctypes.data))
         return out
 
+    def eval_texture(self, texture, uv):
+        """(n, 4) float32: texture `texture` of the current scene at (n, 2) uv"""
+        uv = np.ascontiguousarray(uv, dtype=np.float32).reshape(-1, 2)
+        out = np.empty((len(uv), 4), dtype=np.float32)
+        self._check(self.lib.btrc_hip_eval_texture(self.h, texture, uv.ctypes.data, len(uv), out.ctypes.data))
+        return out
+
     def eval_medium(self, medium, segments, seeds):
         """(n, 10) float32: scattered, position, albedo, transmittance"""
         seg = np.ascontiguousarray(segments, dtype=_abi.ray_dtype())

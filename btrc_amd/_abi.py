@@ -13,6 +13,14 @@ MATERIAL_DISNEY = 1
 MATERIAL_INVISIBLE = 2
 MEDIUM_HOMOGENEOUS = 0
 MEDIUM_HETEROGENEOUS = 1
+TEXTURE_RGBA8 = 0
+TEXTURE_RGBA32F = 1
+ADDRESS_CLAMP = 0
+ADDRESS_WRAP = 1
+ADDRESS_MIRROR = 2
+ADDRESS_BORDER = 3
+TEX_FILTER_LINEAR = 0
+TEX_FILTER_POINT = 1
 ACCEL_FLAT = 0
 ACCEL_TWO_LEVEL = 1
 ENVIR_NONE = 0
@@ -41,7 +49,14 @@ class MaterialDesc(C.Structure):
                 ("specular_tint", C.c_float), ("anisotropic", C.c_float),
                 ("sheen", C.c_float), ("sheen_tint", C.c_float), ("clearcoat", C.c_float),
                 ("clearcoat_gloss", C.c_float), ("transmission", C.c_float),
-                ("transmission_roughness", C.c_float), ("ior", C.c_float)]
+                ("transmission_roughness", C.c_float), ("ior", C.c_float),
+                ("color_texture", C.c_int32), ("metallic_texture", C.c_int32), ("roughness_texture", C.c_int32)]
+
+
+class TextureDesc(C.Structure):
+    _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("format", C.c_int32), ("texels", C.c_void_p),
+                ("address_u", C.c_int32), ("address_v", C.c_int32), ("filter", C.c_int32),
+                ("srgb_to_linear", C.c_int32), ("border", C.c_float * 4)]
 
 
 class InstanceDesc(C.Structure):
@@ -66,7 +81,7 @@ class SceneDesc(C.Structure):
                 ("material_count", C.c_int32), ("materials", C.POINTER(MaterialDesc)),
                 ("instance_count", C.c_int32), ("instances", C.POINTER(InstanceDesc)),
                 ("envir_light", EnvirLightDesc), ("medium_count", C.c_int32), ("media", C.POINTER(MediumDesc)),
-                ("accel", C.c_int32)]
+                ("accel", C.c_int32), ("texture_count", C.c_int32), ("textures", C.POINTER(TextureDesc))]
 
 
 class CameraDesc(C.Structure):
@@ -143,7 +158,8 @@ EXPORTS = [
     "btrc_hip_nrc_set_weights", "btrc_hip_nrc_eval", "btrc_hip_nrc_gradients", "btrc_hip_nrc_fit",
     "btrc_hip_trace_closest", "btrc_hip_trace_any",
     "btrc_hip_batch_upload", "btrc_hip_batch_trace", "btrc_hip_batch_download",
-    "btrc_hip_capture_rays", "btrc_hip_eval_halton", "btrc_hip_eval_independent", "btrc_hip_eval_medium",
+    "btrc_hip_capture_rays", "btrc_hip_eval_halton", "btrc_hip_eval_independent", "btrc_hip_eval_texture",
+    "btrc_hip_eval_medium",
     "btrc_hip_eval_detmath",
 ]
 
@@ -185,6 +201,7 @@ def declare(lib):
                                            C.c_int32, C.c_int32, C.c_void_p]),
         "btrc_hip_eval_independent": (C.c_int, [ctx, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_int32,
                                                 C.c_void_p]),
+        "btrc_hip_eval_texture": (C.c_int, [ctx, C.c_int32, C.c_void_p, C.c_int32, C.c_void_p]),
         "btrc_hip_eval_medium": (C.c_int, [ctx, C.c_int32, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p]),
         "btrc_hip_eval_detmath": (C.c_int, [ctx, C.c_int32, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p]),
     }

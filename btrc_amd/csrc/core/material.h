@@ -43,7 +43,8 @@ struct MaterialRec
     float transmission;
     float transmission_roughness;
     float ior;
-    float pad[2];
+    // textures in place of color / metallic / roughness, -1: the constant
+    int32_t color_tex, metallic_tex, roughness_tex;
 };
 
 struct BSDFSample

@@ -8,6 +8,7 @@
 
 #include "material.h"
 #include "medium.h"
+#include "texture.h"
 
 #if !defined(__HIPCC__)
 struct float4 { float x, y, z, w; };
@@ -90,6 +91,7 @@ struct SceneView
     int32_t medium_count;
     const MediumRec *media;
     const float *voxels;
+    TextureView tex;
 };
 
 struct SurfacePoint
@@ -98,6 +100,7 @@ struct SurfacePoint
     Frame frame;     // geometric frame, world space
     V3 interp_z;     // interpolated shading normal, world space
     float u, v;      // barycentrics of the hit
+    float tu, tv;    // texture coordinate
 };
 
 BTRC_HD V3 f4xyz(float4 a) { return v3(a.x, a.y, a.z); }
@@ -118,7 +121,30 @@ BTRC_HD SurfacePoint get_hitinfo(
     sp.interp_z = normalize(xf_normal(inst.xf, n));
     sp.u = u;
     sp.v = v;
+    sp.tu = sc.gx_u_a[i].w + sc.gy_u_ba[i].w * u + sc.gz_u_ca[i].w * v;
+    sp.tv = sc.sn_v_a[i].w + sc.sn_v_ba[i].w * u + sc.sn_v_ca[i].w * v;
     return sp;
+}
+
+// The material at a surface point: textured parameters sampled at its
+// texture coordinate (DisneyMaterial::create_shader, disney.cpp:813-842:
+// Texture2D::sample_spectrum / sample_float; sample_float reads the first
+// channel).
+BTRC_HD MaterialRec material_at(const SceneView &sc, int32_t id, const SurfacePoint &sp)
+{
+    MaterialRec m = sc.materials[id];
+    if(m.color_tex >= 0)
+    {
+        const float4 c = tex_sample(sc.tex, m.color_tex, sp.tu, sp.tv);
+        m.color[0] = c.x;
+        m.color[1] = c.y;
+        m.color[2] = c.z;
+    }
+    if(m.metallic_tex >= 0)
+        m.metallic = tex_sample(sc.tex, m.metallic_tex, sp.tu, sp.tv).x;
+    if(m.roughness_tex >= 0)
+        m.roughness = tex_sample(sc.tex, m.roughness_tex, sp.tu, sp.tv).x;
+    return m;
 }
 
 // geometric normal only, as the medium stage reads it (wavefront/medium.cpp:230-234)

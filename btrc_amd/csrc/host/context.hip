@@ -184,6 +184,11 @@ struct btrc_ctx
     DevBuf<AliasUnit> alias;
     DevBuf<MediumRec> media;
     DevBuf<float> voxels;
+    DevBuf<TextureRec> textures;
+    DevBuf<uint32_t> texels8;
+    DevBuf<float4> texels32;
+    DevBuf<float> tex_decode;
+    int32_t texture_count = 0;
     int32_t medium_count = 0;
     int32_t light_count = 0;
     int32_t envir_light_index = -1;
@@ -282,6 +287,7 @@ struct btrc_ctx
         v.medium_count = medium_count;
         v.media = media.ptr;
         v.voxels = voxels.ptr;
+        v.tex = TextureView{ textures.ptr, texels8.ptr, texels32.ptr, tex_decode.ptr };
         return v;
     }
     int persistent_grid(int threads_per_block, int64_t work) const
@@ -701,6 +707,71 @@ void upload_scene(btrc_ctx *ctx, const btrc_scene_desc *sd)
         prim_offset += (uint32_t)g.triangle_count;
     }
 
+    // textures: RGBA8 texels as one uint32 each, RGBA32F as float4
+    if(sd->texture_count < 0 || (sd->texture_count > 0 && !sd->textures))
+        throw InvalidArg("bad texture list");
+    std::vector<TextureRec> texs(sd->texture_count);
+    std::vector<uint32_t> tex8;
+    std::vector<float4> tex32;
+    for(int32_t i = 0; i < sd->texture_count; ++i)
+    {
+        const btrc_texture_desc &t = sd->textures[i];
+        const std::string name = "texture " + std::to_string(i);
+        if(t.width < 1 || t.height < 1 || t.width > 65536 || t.height > 65536 || !t.texels)
+            throw InvalidArg(name + ": size must lie in 1..65536 with texels given");
+        if(t.address_u < BTRC_ADDRESS_CLAMP || t.address_u > BTRC_ADDRESS_BORDER || t.address_v < BTRC_ADDRESS_CLAMP ||
+           t.address_v > BTRC_ADDRESS_BORDER)
+            throw InvalidArg(name + ": unknown address mode");
+        if(t.filter != BTRC_TEX_FILTER_LINEAR && t.filter != BTRC_TEX_FILTER_POINT)
+            throw InvalidArg(name + ": unknown filter");
+        const size_t n = (size_t)t.width * (size_t)t.height;
+        TextureRec r{};
+        r.width = t.width;
+        r.height = t.height;
+        r.format = t.format;
+        r.address_u = t.address_u;
+        r.address_v = t.address_v;
+        r.filter = t.filter;
+        r.srgb = t.srgb_to_linear ? 1 : 0;
+        for(int k = 0; k < 4; ++k)
+            r.border[k] = t.border[k];
+        if(t.format == BTRC_TEXTURE_RGBA8)
+        {
+            if(tex8.size() + n > 0xffffffffull)
+                throw InvalidArg(name + ": texel arrays too large");
+            r.offset = (uint32_t)tex8.size();
+            const uint8_t *p = static_cast<const uint8_t *>(t.texels);
+            for(size_t k = 0; k < n; ++k)
+                tex8.push_back((uint32_t)p[4 * k] | ((uint32_t)p[4 * k + 1] << 8) | ((uint32_t)p[4 * k + 2] << 16) |
+                               ((uint32_t)p[4 * k + 3] << 24));
+        }
+        else if(t.format == BTRC_TEXTURE_RGBA32F)
+        {
+            if(tex32.size() + n > 0xffffffffull)
+                throw InvalidArg(name + ": texel arrays too large");
+            r.offset = (uint32_t)tex32.size();
+            const float *p = static_cast<const float *>(t.texels);
+            for(size_t k = 0; k < n; ++k)
+                tex32.push_back(make_float4(p[4 * k], p[4 * k + 1], p[4 * k + 2], p[4 * k + 3]));
+        }
+        else
+            throw InvalidArg(name + ": unknown format");
+        texs[i] = r;
+    }
+    // 8-bit decode: value / 255, then the sRGB transfer function inverted
+    std::vector<float> decode(512);
+    for(int v = 0; v < 256; ++v)
+    {
+        const double c = v / 255.0;
+        decode[v] = (float)c;
+        decode[256 + v] = (float)(c <= 0.04045 ? c / 12.92 : std::pow((c + 0.055) / 1.055, 2.4));
+    }
+    auto texture_ref = [&](int32_t ref, int32_t mat) -> int32_t {
+        if(ref < 0 || ref > sd->texture_count)
+            throw InvalidArg("material " + std::to_string(mat) + ": bad texture index");
+        return ref - 1;
+    };
+
     // materials
     std::vector<MaterialRec> mats(sd->material_count);
     for(int32_t i = 0; i < sd->material_count; ++i)
@@ -717,6 +788,9 @@ void upload_scene(btrc_ctx *ctx, const btrc_scene_desc *sd)
         r.specular_tint = m.specular_tint; r.anisotropic = m.anisotropic; r.sheen = m.sheen;
         r.sheen_tint = m.sheen_tint; r.clearcoat = m.clearcoat; r.clearcoat_gloss = m.clearcoat_gloss;
         r.transmission = m.transmission; r.transmission_roughness = m.transmission_roughness; r.ior = m.ior;
+        r.color_tex = texture_ref(m.color_texture, i);
+        r.metallic_tex = texture_ref(m.metallic_texture, i);
+        r.roughness_tex = texture_ref(m.roughness_texture, i);
         mats[i] = r;
     }
 
@@ -906,7 +980,12 @@ void upload_scene(btrc_ctx *ctx, const btrc_scene_desc *sd)
     ctx->alias.upload(alias_units.data(), alias_units.size(), s);
     ctx->media.upload(media.data(), media.size(), s);
     ctx->voxels.upload(voxels.data(), voxels.size(), s);
+    ctx->textures.upload(texs.data(), texs.size(), s);
+    ctx->texels8.upload(tex8.data(), tex8.size(), s);
+    ctx->texels32.upload(tex32.data(), tex32.size(), s);
+    ctx->tex_decode.upload(decode.data(), decode.size(), s);
     BTRC_HIP_CHECK(hipStreamSynchronize(s));
+    ctx->texture_count = sd->texture_count;
 
     ctx->medium_count = R;
     for(int a = 0; a < 3; ++a)
@@ -1875,6 +1954,26 @@ int btrc_hip_eval_independent(btrc_ctx *ctx, const uint32_t *px, const uint32_t 
         launch_independent_eval(ctx->stream, dx.ptr, dy.ptr, ds.ptr, n, dims, dout.ptr);
         BTRC_HIP_CHECK(hipGetLastError());
         BTRC_HIP_CHECK(hipMemcpyAsync(out, dout.ptr, (size_t)n * dims * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+        BTRC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int btrc_hip_eval_texture(btrc_ctx *ctx, int32_t texture, const float *uv, int32_t n, float *out)
+{
+    return guarded(ctx, [&] {
+        if(!ctx->has_scene)
+            throw StateError("texture query before btrc_hip_upload_scene");
+        if(texture < 0 || texture >= ctx->texture_count || n < 0 || (n > 0 && (!uv || !out)))
+            throw InvalidArg("bad texture query");
+        if(n == 0)
+            return;
+        DevBuf<float2> duv;
+        DevBuf<float4> dout;
+        duv.upload(reinterpret_cast<const float2 *>(uv), n, ctx->stream);
+        dout.alloc(n);
+        launch_texture_eval(ctx->stream, ctx->scene_view().tex, texture, duv.ptr, n, dout.ptr);
+        BTRC_HIP_CHECK(hipGetLastError());
+        BTRC_HIP_CHECK(hipMemcpyAsync(out, dout.ptr, 4 * (size_t)n * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
         BTRC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
     });
 }

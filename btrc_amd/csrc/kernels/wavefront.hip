@@ -290,7 +290,7 @@ __device__ __forceinline__ NrcVertex nrc_vertex(const SceneView &sv, const Insta
     NrcVertex v;
     float rough;
     V3 dif, spe;
-    nrc_material_inputs(sv.materials[inst.material_id], &rough, &dif, &spe);
+    nrc_material_inputs(material_at(sv, inst.material_id, sp), &rough, &dif, &spe);
     const V3 wo = -d;
     v.p[0] = sp.position.x; v.p[1] = sp.position.y; v.p[2] = sp.position.z;
     v.wo[0] = wo.x; v.wo[1] = wo.y; v.wo[2] = wo.z;
@@ -437,7 +437,7 @@ __global__ void __launch_bounds__(256) k_shade(
                     const float lsel = hs.get1d();
                     ShadowSample ss = sample_surface_light_li(sv, sp.position, sp.frame.z, l0, l1, l2, lsel);
 
-                    const Shader shader = make_shader(sv.materials[inst.material_id], sp.frame, sp.interp_z);
+                    const Shader shader = make_shader(material_at(sv, inst.material_id, sp), sp.frame, sp.interp_z);
                     if(depth == 0)
                     {
                         // g-buffer of the first hit: Shader::albedo and the
@@ -820,6 +820,13 @@ __global__ void __launch_bounds__(TRACE_BLOCK) k_trace_batch(
 
 // one medium along segments a -> a + d * tmax with one PCG32 stream per
 // segment: delta tracking, then ratio tracking on the same stream
+__global__ void k_texture_eval(TextureView tv, int32_t id, const float2 *uv, int32_t n, float4 *out)
+{
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if(i < n)
+        out[i] = tex_sample(tv, id, uv[i].x, uv[i].y);
+}
+
 __global__ void k_medium_eval(SceneView sv, int32_t med, const float4 *seg, const uint2 *seed, int32_t n, float *out)
 {
     const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1001,6 +1008,11 @@ void launch_halton_eval(hipStream_t s, const HaltonConsts &hc, const uint32_t *p
                         const int32_t *sample_index, int32_t n, int32_t dims, float *out)
 {
     hipLaunchKernelGGL(k_halton_eval, dim3((n + 127) / 128), dim3(128), 0, s, hc, px, py, sample_index, n, dims, out);
+}
+
+void launch_texture_eval(hipStream_t s, const TextureView &tv, int32_t id, const float2 *uv, int32_t n, float4 *out)
+{
+    hipLaunchKernelGGL(k_texture_eval, dim3((n + 127) / 128), dim3(128), 0, s, tv, id, uv, n, out);
 }
 
 void launch_medium_eval(hipStream_t s, const SceneView &sv, int32_t med, const float4 *seg, const uint2 *seed,

@@ -101,6 +101,7 @@ class Scene:
         self.media = []        # dicts
         self.envir = None      # (up, lower, upper)
         self.accel = _abi.ACCEL_FLAT
+        self.textures = []     # dicts
 
     def add_geometry(self, positions, normals=None, uvs=None):
         positions = np.ascontiguousarray(positions, dtype=np.float32).reshape(-1, 9)
@@ -111,14 +112,31 @@ class Scene:
         self.geometries.append(dict(positions=positions, normals=normals, uvs=uvs))
         return len(self.geometries) - 1
 
-    def add_diffuse(self, color, shadow_terminator=True):
+    def add_texture(self, texels, address_u="clamp", address_v="clamp", filter="linear", srgb_to_linear=False,
+                    border=(0.0, 0.0, 0.0, 0.0)):
+        """Array2D texture (texture2d/array2d.cpp, description.cpp:32-93);
+        texels (H, W, 4) uint8 (RGBA8) or float32 (RGBA32F), row 0 at v = 0"""
+        texels = np.ascontiguousarray(texels)
+        assert texels.ndim == 3 and texels.shape[2] == 4
+        fmt = _abi.TEXTURE_RGBA8 if texels.dtype == np.uint8 else _abi.TEXTURE_RGBA32F
+        if fmt == _abi.TEXTURE_RGBA32F:
+            texels = np.ascontiguousarray(texels, dtype=np.float32)
+        modes = dict(clamp=_abi.ADDRESS_CLAMP, wrap=_abi.ADDRESS_WRAP, mirror=_abi.ADDRESS_MIRROR,
+                     border=_abi.ADDRESS_BORDER)
+        filters = dict(linear=_abi.TEX_FILTER_LINEAR, point=_abi.TEX_FILTER_POINT)
+        self.textures.append(dict(texels=texels, format=fmt, address_u=modes[address_u], address_v=modes[address_v],
+                                  filter=filters[filter], srgb=bool(srgb_to_linear), border=tuple(border)))
+        return len(self.textures) - 1
+
+    def add_diffuse(self, color, shadow_terminator=True, color_texture=None):
         self.materials.append(dict(type=_abi.MATERIAL_DIFFUSE, color=color,
-                                   shadow_terminator=shadow_terminator))
+                                   shadow_terminator=shadow_terminator, color_texture=color_texture))
         return len(self.materials) - 1
 
     def add_disney(self, color, metallic, roughness, specular=1.0, specular_tint=0.0, anisotropic=0.0,
                    sheen=0.0, sheen_tint=0.0, clearcoat=0.0, clearcoat_gloss=1.0, transmission=0.0,
-                   transmission_roughness=None, ior=1.5, shadow_terminator=True):
+                   transmission_roughness=None, ior=1.5, shadow_terminator=True, color_texture=None,
+                   metallic_texture=None, roughness_texture=None):
         # defaults of DisneyMaterialCreator, builtin/material/disney.cpp:844-894
         if transmission_roughness is None:
             transmission_roughness = roughness
@@ -127,7 +145,8 @@ class Scene:
                                    anisotropic=anisotropic, sheen=sheen, sheen_tint=sheen_tint,
                                    clearcoat=clearcoat, clearcoat_gloss=clearcoat_gloss,
                                    transmission=transmission, transmission_roughness=transmission_roughness,
-                                   ior=ior, shadow_terminator=shadow_terminator))
+                                   ior=ior, shadow_terminator=shadow_terminator, color_texture=color_texture,
+                                   metallic_texture=metallic_texture, roughness_texture=roughness_texture))
         return len(self.materials) - 1
 
     def add_invisible(self):
@@ -202,6 +221,9 @@ class Scene:
                         "sheen_tint", "clearcoat", "clearcoat_gloss", "transmission",
                         "transmission_roughness", "ior"):
                 setattr(d, key, float(m.get(key, 0.0)))
+            for key in ("color_texture", "metallic_texture", "roughness_texture"):
+                t = m.get(key)
+                setattr(d, key, 0 if t is None else int(t) + 1)
         idesc = (_abi.InstanceDesc * max(1, len(self.instances)))()
         for k, inst in enumerate(self.instances):
             d = idesc[k]
@@ -242,6 +264,17 @@ class Scene:
         sd.medium_count = len(self.media)
         sd.media = med
         sd.accel = self.accel
+        td = (_abi.TextureDesc * max(1, len(self.textures)))()
+        for k, t in enumerate(self.textures):
+            h, w = t["texels"].shape[:2]
+            td[k].width, td[k].height, td[k].format = w, h, t["format"]
+            td[k].texels = t["texels"].ctypes.data
+            td[k].address_u, td[k].address_v = t["address_u"], t["address_v"]
+            td[k].filter = t["filter"]
+            td[k].srgb_to_linear = 1 if t["srgb"] else 0
+            td[k].border = (C.c_float * 4)(*map(float, t["border"]))
+        sd.texture_count = len(self.textures)
+        sd.textures = td
         if self.envir is not None:
             up, lower, upper = self.envir
             sd.envir_light.type = _abi.ENVIR_GRADIENT_SKY
@@ -250,7 +283,8 @@ class Scene:
             sd.envir_light.upper = _abi.F3(*map(float, upper))
         else:
             sd.envir_light.type = _abi.ENVIR_NONE
-        holder.update(gd=gd, md=md, idesc=idesc, med=med, geoms=self.geometries, media=self.media)
+        holder.update(gd=gd, md=md, idesc=idesc, med=med, td=td, geoms=self.geometries, media=self.media,
+                      textures=self.textures)
         return sd, holder
 
 
@@ -323,6 +357,53 @@ def uv_sphere(rng=None, slices=16, stacks=9, displacement=0.0):
                 tris.append([*a, *c, *d])
                 tris.append([*a, *b, *c])
     return np.array(tris, dtype=np.float32)
+
+
+def uv_sphere_uvs(slices=16, stacks=9):
+    """texture coordinates of uv_sphere's triangles, in its triangle order:
+    u = slice / slices, v = stack / stacks at each corner"""
+    def t(i, j):
+        return (i / slices, j / stacks)
+
+    uvs = []
+    for j in range(stacks):
+        for i in range(slices):
+            a, b, c, d = t(i, j), t(i + 1, j), t(i + 1, j + 1), t(i, j + 1)
+            if j == 0:
+                uvs.append([*a, *c, *d])
+            elif j == stacks - 1:
+                uvs.append([*a, *b, *d])
+            else:
+                uvs.append([*a, *c, *d])
+                uvs.append([*a, *b, *c])
+    return np.array(uvs, dtype=np.float32)
+
+
+def noise_texture(n=1024, seed=5, octaves=4, base=8):
+    """(n, n, 4) RGBA8 texture of seeded value noise: per channel, octave k
+    has a (base * 2^k + 1)^2 lattice of PCG32 values, smoothstep-weighted
+    bilinear interpolation, amplitude 2^-k, normalised to [0, 255]; alpha 255"""
+    rng = PCG32(seed)
+    c = (np.arange(n) + 0.5) / n
+    out = np.empty((n, n, 4), dtype=np.uint8)
+    out[..., 3] = 255
+    for ch in range(3):
+        total = np.zeros((n, n), dtype=np.float64)
+        for k in range(octaves):
+            f = base * 2 ** k
+            lat = np.array([rng.uniform() for _ in range((f + 1) ** 2)], dtype=np.float64).reshape(f + 1, f + 1)
+            x = c * f
+            i = np.minimum(np.floor(x).astype(np.int64), f - 1)
+            t = x - i
+            w = t * t * (3 - 2 * t)
+            iy, ix = np.ix_(i, i)
+            wy, wx = np.ix_(w, w)
+            v = (lat[iy, ix] * (1 - wy) * (1 - wx) + lat[iy, ix + 1] * (1 - wy) * wx
+                 + lat[iy + 1, ix] * wy * (1 - wx) + lat[iy + 1, ix + 1] * wy * wx)
+            total += v * 0.5 ** k
+        lo, hi = total.min(), total.max()
+        out[..., ch] = np.clip(np.round((total - lo) / max(hi - lo, 1e-12) * 255.0), 0, 255).astype(np.uint8)
+    return out
 
 
 # ------------------------------------------------------------ synthetic scenes

@@ -54,6 +54,26 @@ enum
 
 enum
 {
+    BTRC_TEXTURE_RGBA8 = 0,   /* UNorm8x4 */
+    BTRC_TEXTURE_RGBA32F = 1  /* F32x4    */
+};
+
+enum
+{
+    BTRC_ADDRESS_CLAMP = 0,
+    BTRC_ADDRESS_WRAP = 1,
+    BTRC_ADDRESS_MIRROR = 2,
+    BTRC_ADDRESS_BORDER = 3
+};
+
+enum
+{
+    BTRC_TEX_FILTER_LINEAR = 0,
+    BTRC_TEX_FILTER_POINT = 1
+};
+
+enum
+{
     BTRC_ACCEL_FLAT = 0,      /* one tree over world-space triangles   */
     BTRC_ACCEL_TWO_LEVEL = 1  /* instance tree over per-geometry trees */
 };
@@ -103,7 +123,32 @@ typedef struct btrc_material_desc
     float metallic, roughness, specular, specular_tint, anisotropic;
     float sheen, sheen_tint, clearcoat, clearcoat_gloss;
     float transmission, transmission_roughness, ior;
+    /* Array2D textures in place of the constants (DisneyMaterialCreator,
+     * material/disney.cpp:844-894; DiffuseCreator's albedo): 1 + index into
+     * btrc_scene_desc::textures, 0 for the constant. Colour reads RGB,
+     * metallic and roughness read the first channel, at the hit's texture
+     * coordinate. */
+    int32_t color_texture;
+    int32_t metallic_texture;
+    int32_t roughness_texture;
 } btrc_material_desc;
+
+/* An Array2D texture (texture2d/array2d.cpp, utils/cuda/texture.cpp:63-105,
+ * texture2d/description.cpp:32-93). Texels row-major, row 0 at v = 0.
+ * RGBA8 texels read as value / 255 (sRGB-decoded when srgb_to_linear);
+ * linear filtering blends the four texels around (u W - 0.5, v H - 0.5) with
+ * FP32 weights; address modes act on texel indices. */
+typedef struct btrc_texture_desc
+{
+    int32_t width, height;     /* 1 .. 65536 each */
+    int32_t format;            /* BTRC_TEXTURE_* */
+    const void *texels;        /* RGBA8: 4 bytes per texel; RGBA32F: 4 floats */
+    int32_t address_u;         /* BTRC_ADDRESS_*, default clamp */
+    int32_t address_v;
+    int32_t filter;            /* BTRC_TEX_FILTER_*, default linear */
+    int32_t srgb_to_linear;
+    float border[4];
+} btrc_texture_desc;
 
 /* Scene::Instance (core/scene.h:71-80) */
 typedef struct btrc_instance_desc
@@ -163,6 +208,8 @@ typedef struct btrc_scene_desc
      * TLAS/BLAS split of Scene::commit, core/scene.cpp:92-239), transforming
      * the ray into each instance's local space. */
     int32_t accel;
+    int32_t texture_count;
+    const btrc_texture_desc *textures;
 } btrc_scene_desc;
 
 /* PinholeCamera (builtin/camera/pinhole.cpp) */
@@ -344,9 +391,13 @@ int btrc_hip_eval_halton(btrc_ctx *ctx, int32_t width, int32_t height, const uin
 /* the independent sampler's first `dims` numbers for each (px, py, sample) */
 int btrc_hip_eval_independent(btrc_ctx *ctx, const uint32_t *px, const uint32_t *py, const int32_t *sample_index,
                               int32_t n, int32_t dims, float *out);
+/* texture `texture` of the current scene sampled at n (u, v) pairs; out
+ * gets 4 floats per pair */
+int btrc_hip_eval_texture(btrc_ctx *ctx, int32_t texture, const float *uv, int32_t n, float *out);
 /* medium id (rank by descending priority) along segments o -> o + d * tmax,
  * one PCG32 stream per segment (seeded like a shadow ray's): out holds 10
  * floats per segment: scattered (0/1), position xyz, albedo xyz, tr xyz */
+
 int btrc_hip_eval_medium(btrc_ctx *ctx, int32_t medium, const btrc_ray *segments, const uint64_t *seeds, int32_t n,
                          float *out);
 /* fn: 0 sin, 1 cos, 2 log, 3 exp, 4 pow(x, y), 5 sqrt, 6 x / y */

@@ -476,6 +476,15 @@ struct Med
     float w2l[12];
 };
 
+// an Array2D texture (texture2d/array2d.cpp:19-30 over utils/cuda/texture.cpp:63-105)
+struct Tex
+{
+    int w, h, format, au, av, filter, srgb;
+    float border[4];
+    std::vector<uint32_t> t8;
+    std::vector<float> t32;
+};
+
 struct Scene
 {
     std::vector<Geo> geos;
@@ -487,6 +496,10 @@ struct Scene
     int R = 0;
     // per triangle of all geometries: frame records (triangle_mesh.cpp:40-64)
     std::vector<Vec> gx, gy, gz, sa, sba, sca, la, lba, lca;
+    // texture coordinates (a, b - a, c - a) per triangle, u then v
+    std::vector<float> tuv;
+    std::vector<Tex> texs;
+    float decode[512];
     // flattened world triangles
     std::vector<Vec> w0, w1, w2;
     std::vector<uint32_t> tinst, tprim;
@@ -606,6 +619,8 @@ static Scene *make_scene(const btrc_scene_desc *sd)
                     ex = nrm(sub(sclf(m11 * id, BA), sclf(m01 * id, CA)));
                 }
             }
+            for(float x : { uv[0], uv[2] - uv[0], uv[4] - uv[0], uv[1], uv[3] - uv[1], uv[5] - uv[1] })
+                s->tuv.push_back(x);
             s->gx.push_back(ex);
             s->gy.push_back(nrm(cross(gzv, ex)));
             s->gz.push_back(gzv);
@@ -657,6 +672,33 @@ static Scene *make_scene(const btrc_scene_desc *sd)
         off += (uint32_t)g.triangle_count;
     }
     s->mats.assign(sd->materials, sd->materials + sd->material_count);
+    for(int i = 0; i < sd->texture_count; ++i)
+    {
+        const btrc_texture_desc &t = sd->textures[i];
+        Tex x;
+        x.w = t.width; x.h = t.height; x.format = t.format;
+        x.au = t.address_u; x.av = t.address_v; x.filter = t.filter; x.srgb = t.srgb_to_linear != 0;
+        for(int k = 0; k < 4; ++k)
+            x.border[k] = t.border[k];
+        const size_t n = (size_t)t.width * t.height;
+        if(t.format == BTRC_TEXTURE_RGBA8)
+        {
+            const uint8_t *p = static_cast<const uint8_t *>(t.texels);
+            for(size_t k = 0; k < n; ++k)
+                x.t8.push_back((uint32_t)p[4 * k] | ((uint32_t)p[4 * k + 1] << 8) | ((uint32_t)p[4 * k + 2] << 16) |
+                               ((uint32_t)p[4 * k + 3] << 24));
+        }
+        else
+            x.t32.assign(static_cast<const float *>(t.texels), static_cast<const float *>(t.texels) + 4 * n);
+        s->texs.push_back(std::move(x));
+    }
+    // UNorm8 value / 255 and the inverse sRGB transfer function (IEC 61966-2-1)
+    for(int v = 0; v < 256; ++v)
+    {
+        const double c = v / 255.0;
+        s->decode[v] = (float)c;
+        s->decode[256 + v] = (float)(c <= 0.04045 ? c / 12.92 : std::pow((c + 0.055) / 1.055, 2.4));
+    }
     // Scene::commit's medium ids (core/scene.cpp:144-161): descending
     // priority, the volume-primitive medium next, then "none"
     s->R = sd->medium_count;
@@ -969,6 +1011,78 @@ static Hit trace(const Scene &s, int mode, bool any, bool use_bvh, Vec o, Vec d,
     return h;
 }
 
+// ------------------------------------------------------------------ textures
+
+struct F4 { float x, y, z, w; };
+
+// texel index under an address mode (clamp to the edge, wrap modulo the
+// size, mirror with period 2n, border: -1 outside)
+static int tex_addr(int i, int n, int mode)
+{
+    if(mode == BTRC_ADDRESS_WRAP)
+    {
+        const int m = i % n;
+        return m < 0 ? m + n : m;
+    }
+    if(mode == BTRC_ADDRESS_MIRROR)
+    {
+        int m = i % (2 * n);
+        if(m < 0)
+            m += 2 * n;
+        return m < n ? m : 2 * n - 1 - m;
+    }
+    if(mode == BTRC_ADDRESS_BORDER)
+        return (i < 0 || i >= n) ? -1 : i;
+    return std::min(std::max(i, 0), n - 1);
+}
+
+static F4 tex_texel(const Scene &s, const Tex &t, int x, int y)
+{
+    if(x < 0 || y < 0)
+        return F4{ t.border[0], t.border[1], t.border[2], t.border[3] };
+    const size_t i = (size_t)y * t.w + x;
+    if(t.format == BTRC_TEXTURE_RGBA32F)
+        return F4{ t.t32[4 * i], t.t32[4 * i + 1], t.t32[4 * i + 2], t.t32[4 * i + 3] };
+    const uint32_t p = t.t8[i];
+    const float *lut = s.decode + (t.srgb ? 256 : 0);
+    return F4{ lut[p & 255u], lut[(p >> 8) & 255u], lut[(p >> 16) & 255u], s.decode[p >> 24] };
+}
+
+static int tex_floor(float x, float &frac)
+{
+    const float f = std::floor(x);
+    frac = x - f;
+    return (int)std::fmin(std::fmax(f, -1073741824.0f), 1073741824.0f);
+}
+
+// a normalised-coordinate texture fetch with point or bilinear filtering
+// (CUDA texture semantics restated with FP32 weights)
+static F4 tex_sample(const Scene &s, int id, float u, float v)
+{
+    const Tex &t = s.texs[id];
+    if(t.filter == BTRC_TEX_FILTER_POINT)
+    {
+        float fa, fb;
+        const int x = tex_addr(tex_floor(u * (float)t.w, fa), t.w, t.au);
+        const int y = tex_addr(tex_floor(v * (float)t.h, fb), t.h, t.av);
+        return tex_texel(s, t, x, y);
+    }
+    float a, b;
+    const int x0 = tex_floor(u * (float)t.w - 0.5f, a);
+    const int y0 = tex_floor(v * (float)t.h - 0.5f, b);
+    if(!(a >= 0.0f && a < 1.0f))
+        a = 0.0f;
+    if(!(b >= 0.0f && b < 1.0f))
+        b = 0.0f;
+    const int xa = tex_addr(x0, t.w, t.au), xb = tex_addr(x0 + 1, t.w, t.au);
+    const int ya = tex_addr(y0, t.h, t.av), yb = tex_addr(y0 + 1, t.h, t.av);
+    const F4 p = tex_texel(s, t, xa, ya), q = tex_texel(s, t, xb, ya);
+    const F4 r = tex_texel(s, t, xa, yb), w = tex_texel(s, t, xb, yb);
+    const float w00 = (1.0f - a) * (1.0f - b), w10 = a * (1.0f - b), w01 = (1.0f - a) * b, w11 = a * b;
+    return F4{ ((w00 * p.x + w10 * q.x) + w01 * r.x) + w11 * w.x, ((w00 * p.y + w10 * q.y) + w01 * r.y) + w11 * w.y,
+               ((w00 * p.z + w10 * q.z) + w01 * r.z) + w11 * w.z, ((w00 * p.w + w10 * q.w) + w01 * r.w) + w11 * w.w };
+}
+
 // ------------------------------------------------------------------ surfaces
 
 struct SP
@@ -976,7 +1090,27 @@ struct SP
     Vec pos;
     Frm frame;
     Vec iz;
+    float tu, tv; // texture coordinate
 };
+
+// the material at a surface point: textured colour / metallic / roughness
+// sampled at its texture coordinate (disney.cpp:813-842)
+static btrc_material_desc material_at(const Scene &s, int id, const SP &sp)
+{
+    btrc_material_desc m = s.mats[id];
+    if(m.color_texture > 0)
+    {
+        const F4 c = tex_sample(s, m.color_texture - 1, sp.tu, sp.tv);
+        m.color[0] = c.x;
+        m.color[1] = c.y;
+        m.color[2] = c.z;
+    }
+    if(m.metallic_texture > 0)
+        m.metallic = tex_sample(s, m.metallic_texture - 1, sp.tu, sp.tv).x;
+    if(m.roughness_texture > 0)
+        m.roughness = tex_sample(s, m.roughness_texture - 1, sp.tu, sp.tv).x;
+    return m;
+}
 
 // get_hitinfo, core/scene.cpp:9-64
 static SP hitinfo(const Scene &s, Vec o, Vec d, const Inst &in, uint32_t prim, float t, float u, float v)
@@ -990,6 +1124,9 @@ static SP hitinfo(const Scene &s, Vec o, Vec d, const Inst &in, uint32_t prim, f
     sp.frame.z = nrm(xnor(in.xf, gf.z));
     const Vec n = add(add(s.sa[i], scl(s.sba[i], u)), scl(s.sca[i], v));
     sp.iz = nrm(xnor(in.xf, n));
+    const float *c = &s.tuv[6 * (size_t)i];
+    sp.tu = (c[0] + c[1] * u) + c[2] * v;
+    sp.tv = (c[3] + c[4] * u) + c[5] * v;
     return sp;
 }
 
@@ -1960,7 +2097,7 @@ static void trace_path(const Scene &s, const Params &p, uint64_t path_index, std
         const float lsel = hs.get1d();
         Shadow sh = sample_light(s, sp.pos, sp.frame.z, true, l0, l1, l2, lsel);
 
-        const btrc_material_desc &m = s.mats[in.mat];
+        const btrc_material_desc m = material_at(s, in.mat, sp);
         Shd shd;
         shd.geo = sp.frame;
         shd.sh = rot_z(sp.frame, sp.iz);
@@ -2266,6 +2403,19 @@ void oracle_gaussian_filter(float radius, float alpha, const float *u, int32_t n
     orc::gaussian_table(p);
     for(int32_t i = 0; i < n; ++i)
         orc::gaussian_sample(p, u[3 * i], u[3 * i + 1], u[3 * i + 2], out[2 * i], out[2 * i + 1]);
+}
+
+void oracle_texture_eval(void *sp, int32_t texture, const float *uv, int32_t n, float *out)
+{
+    const orc::Scene &s = *static_cast<orc::Scene *>(sp);
+    for(int32_t i = 0; i < n; ++i)
+    {
+        const orc::F4 r = orc::tex_sample(s, texture, uv[2 * i], uv[2 * i + 1]);
+        out[4 * i] = r.x;
+        out[4 * i + 1] = r.y;
+        out[4 * i + 2] = r.z;
+        out[4 * i + 3] = r.w;
+    }
 }
 
 void oracle_medium_eval(void *sp, int32_t medium, const btrc_ray *seg, const uint64_t *seeds, int32_t n, float *out)

@@ -37,6 +37,7 @@ def lib():
         L.oracle_independent.argtypes = [vp, vp, vp, C.c_int32, C.c_int32, vp]
         L.oracle_gaussian_filter.argtypes = [C.c_float, C.c_float, vp, C.c_int32, vp]
         L.oracle_medium_eval.argtypes = [vp, C.c_int32, vp, vp, C.c_int32, vp]
+        L.oracle_texture_eval.argtypes = [vp, C.c_int32, vp, C.c_int32, vp]
         L.oracle_hg.argtypes = [C.c_float, vp, C.c_int32, vp]
         L.oracle_tex3.restype = C.c_float
         L.oracle_tex3.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32, C.c_float, C.c_float, C.c_float]
@@ -87,6 +88,13 @@ class OracleScene:
             lib().oracle_scene_destroy(self.h)
         except Exception:
             pass
+
+    def texture_eval(self, texture, uv):
+        """(n, 4) float32: the scene's texture `texture` at (n, 2) uv"""
+        uv = np.ascontiguousarray(uv, dtype=np.float32).reshape(-1, 2)
+        out = np.empty((len(uv), 4), dtype=np.float32)
+        lib().oracle_texture_eval(self.h, texture, uv.ctypes.data, len(uv), out.ctypes.data)
+        return out
 
     def medium_eval(self, medium, segments, seeds):
         """(n, 10) float32 like Context.eval_medium"""

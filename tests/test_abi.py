@@ -36,7 +36,8 @@ def test_struct_sizes():
     assert C.sizeof(_abi.Hit) == 20
     assert C.sizeof(_abi.CameraDesc) == 44
     assert C.sizeof(_abi.RenderParams) == 64
-    assert C.sizeof(_abi.MaterialDesc) == 68
+    assert C.sizeof(_abi.MaterialDesc) == 80
+    assert C.sizeof(_abi.TextureDesc) == 4 * 4 + 8 + 4 * 4 + 16
     assert C.sizeof(_abi.InstanceDesc) == 4 * (2 + 32 + 1 + 3 + 2)
     assert C.sizeof(_abi.RenderStats) == 8 * (1 + 4 + 7 + 7)
 
@@ -48,6 +49,7 @@ STRUCTS = {
     "btrc_camera_desc": "CameraDesc", "btrc_render_params": "RenderParams",
     "btrc_filter_desc": "FilterDesc", "btrc_render_stats": "RenderStats", "btrc_ray": "Ray",
     "btrc_hit": "Hit", "btrc_tuning": "Tuning", "btrc_nrc_params": "NrcParams", "btrc_nrc_stats": "NrcStats",
+    "btrc_texture_desc": "TextureDesc",
 }
 
 

@@ -1,0 +1,71 @@
+"""Array2D textures on the GPU against the oracle, bit for bit, and a
+textured render per pixel."""
+import numpy as np
+import pytest
+
+from btrc_amd import _abi, scenes
+from oracle import oracle
+
+from test_oracle_texture import MODES, rgba8, tex_scene
+from util import render_params
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("mode", MODES)
+@pytest.mark.parametrize("filt", ["point", "linear"])
+@pytest.mark.parametrize("srgb", [False, True])
+def test_sample_bitwise_vs_oracle(ctx, mode, filt, srgb):
+    scene = tex_scene(rgba8(37, 53, 7), address_u=mode, address_v="clamp" if mode == "wrap" else mode, filter=filt,
+                      srgb_to_linear=srgb, border=(0.5, 0.25, 0.125, 1.0))
+    ctx.upload_scene(scene)
+    rng = np.random.default_rng(8)
+    uv = rng.uniform(-3.0, 4.0, (50000, 2)).astype(np.float32)
+    uv[:4] = [[np.nan, 0.5], [0.5, np.inf], [-1e30, 1e30], [1e9, -7e8]]  # non-finite and huge coordinates
+    got = ctx.eval_texture(0, uv)
+    want = oracle.OracleScene(scene).texture_eval(0, uv)
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+
+
+def test_float_texture_bitwise_vs_oracle(ctx):
+    t = np.random.default_rng(9).normal(size=(64, 32, 4)).astype(np.float32)
+    scene = tex_scene(t, address_u="wrap", address_v="mirror")
+    ctx.upload_scene(scene)
+    uv = np.random.default_rng(10).uniform(-2, 2, (20000, 2)).astype(np.float32)
+    assert np.array_equal(ctx.eval_texture(0, uv).view(np.uint32),
+                          oracle.OracleScene(scene).texture_eval(0, uv).view(np.uint32))
+
+
+def test_bad_texture_arguments(ctx):
+    import btrc_amd
+    s = tex_scene(rgba8(4, 4, 1))
+    ctx.upload_scene(s)
+    with pytest.raises(btrc_amd.BtrcException):
+        ctx.eval_texture(1, np.zeros((1, 2), np.float32))
+    s.materials[0]["color_texture"] = 5   # no such texture
+    with pytest.raises(btrc_amd.BtrcException):
+        ctx.upload_scene(s)
+
+
+def test_textured_render_per_pixel_parity(ctx):
+    scene, cam = scenes.sphere_field(instances=64, lights=8, seed=11, extent=16.0)
+    uvs = scenes.uv_sphere_uvs()
+    scene.geometries[0]["uvs"] = uvs
+    texs = [scene.add_texture(scenes.noise_texture(64, seed=k), address_u="wrap", srgb_to_linear=(k == 1))
+            for k in range(3)]
+    for k, m in enumerate(scene.materials):
+        if m["type"] == _abi.MATERIAL_DISNEY:
+            m["color_texture"] = texs[k % 3]
+            if k % 2:
+                m["roughness_texture"] = texs[(k + 1) % 3]
+    scene.accel = _abi.ACCEL_TWO_LEVEL
+    p = render_params(96, 64, 8, state_count=4096)
+    ctx.upload_scene(scene)
+    ctx.set_camera(cam, p.width, p.height)
+    stats = ctx.render(p)
+    rgba, _ = ctx.film(p.width, p.height)
+    ref, e, s = oracle.OracleScene(scene).render(cam, p)
+    ref /= p.spp
+    rel = np.abs(rgba[..., :3] - ref) / np.maximum(np.abs(ref), 1e-2)
+    assert rel.max() <= 1e-4, f"max rel err {rel.max()}"
+    assert stats.extension_rays == e and stats.shadow_rays == s
