@@ -115,6 +115,21 @@ class Context:
         self._check(self.lib.btrc_hip_render(self.h, C.byref(params), C.byref(stats)))
         return stats
 
+    def render_views(self, params, views):
+        """views: list of (Camera, width, height); one wavefront over all"""
+        vd = (_abi.ViewDesc * len(views))()
+        for k, (cam, w, h) in enumerate(views):
+            vd[k].camera = cam.to_desc(w, h)
+            vd[k].width, vd[k].height = w, h
+        stats = _abi.RenderStats()
+        self._check(self.lib.btrc_hip_render_views(self.h, C.byref(params), len(views), vd, C.byref(stats)))
+        return stats
+
+    def view_film(self, view, width, height):
+        rgba = np.empty((height, width, 4), dtype=np.float32)
+        self._check(self.lib.btrc_hip_view_readback(self.h, view, rgba.ctypes.data))
+        return rgba
+
     def film(self, width, height, fp64=False):
         rgba = np.empty((height, width, 4), dtype=np.float32)
         rgb64 = np.empty((height, width, 3), dtype=np.float64) if fp64 else None

@@ -88,6 +88,10 @@ class CameraDesc(C.Structure):
     _fields_ = [("eye", F3), ("dst", F3), ("up", F3), ("fov_y_deg", C.c_float), ("w_over_h", C.c_float)]
 
 
+class ViewDesc(C.Structure):
+    _fields_ = [("camera", CameraDesc), ("width", C.c_int32), ("height", C.c_int32)]
+
+
 class RenderParams(C.Structure):
     _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("spp", C.c_int32),
                 ("min_depth", C.c_int32), ("max_depth", C.c_int32),
@@ -153,7 +157,8 @@ EXPORTS = [
     "btrc_hip_create", "btrc_hip_destroy", "btrc_hip_last_error", "btrc_hip_device_name",
     "btrc_hip_set_tuning", "btrc_hip_get_tuning",
     "btrc_hip_upload_scene", "btrc_hip_set_camera", "btrc_hip_scene_info", "btrc_hip_set_filter",
-    "btrc_hip_render", "btrc_hip_film_readback", "btrc_hip_aov_readback", "btrc_hip_request_stop",
+    "btrc_hip_render", "btrc_hip_render_views", "btrc_hip_film_readback", "btrc_hip_view_readback",
+    "btrc_hip_aov_readback", "btrc_hip_request_stop",
     "btrc_hip_nrc_configure", "btrc_hip_nrc_reset", "btrc_hip_nrc_stats", "btrc_hip_nrc_get_weights",
     "btrc_hip_nrc_set_weights", "btrc_hip_nrc_eval", "btrc_hip_nrc_gradients", "btrc_hip_nrc_fit",
     "btrc_hip_trace_closest", "btrc_hip_trace_any",
@@ -180,6 +185,8 @@ def declare(lib):
         "btrc_hip_scene_info": (C.c_int, [ctx, P(C.c_int64), P(C.c_int32), P(C.c_int32), P(C.c_int64)]),
         "btrc_hip_set_filter": (C.c_int, [ctx, P(FilterDesc)]),
         "btrc_hip_render": (C.c_int, [ctx, P(RenderParams), P(RenderStats)]),
+        "btrc_hip_render_views": (C.c_int, [ctx, P(RenderParams), C.c_int32, P(ViewDesc), P(RenderStats)]),
+        "btrc_hip_view_readback": (C.c_int, [ctx, C.c_int32, C.c_void_p]),
         "btrc_hip_aov_readback": (C.c_int, [ctx, C.c_void_p, C.c_void_p]),
         "btrc_hip_film_readback": (C.c_int, [ctx, C.c_void_p, C.c_void_p]),
         "btrc_hip_request_stop": (C.c_int, [ctx]),

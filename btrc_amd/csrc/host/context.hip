@@ -216,7 +216,10 @@ struct btrc_ctx
     DevBuf<double> film64;
     DevBuf<float> film_albedo, film_normal;
     DevBuf<float4> resolved;
-    int32_t film_w = 0, film_h = 0, film_spp = 0;
+    int64_t film_pixels = 0;            // all views of the last render
+    std::vector<ViewRec> film_views;    // their layout
+    DevBuf<ViewRec> views;
+    int32_t film_spp = 0;
     bool film_has64 = false, film_has_albedo = false, film_has_normal = false;
 
     // film filter
@@ -1012,10 +1015,8 @@ void upload_scene(btrc_ctx *ctx, const btrc_scene_desc *sd)
 }
 
 // PinholeCamera::commit, camera/pinhole.cpp:55-77
-void set_camera(btrc_ctx *ctx, const btrc_camera_desc *cd)
+CameraConsts camera_consts(const btrc_camera_desc *cd)
 {
-    if(!cd)
-        throw InvalidArg("camera is null");
     const V3 eye = v3(cd->eye[0], cd->eye[1], cd->eye[2]);
     const V3 dst = v3(cd->dst[0], cd->dst[1], cd->dst[2]);
     const V3 up = v3(cd->up[0], cd->up[1], cd->up[2]);
@@ -1032,7 +1033,14 @@ void set_camera(btrc_ctx *ctx, const btrc_camera_desc *cd)
     c.lbc[0] = lbc.x; c.lbc[1] = lbc.y; c.lbc[2] = lbc.z;
     c.film_x[0] = fx.x; c.film_x[1] = fx.y; c.film_x[2] = fx.z;
     c.film_y[0] = fy.x; c.film_y[1] = fy.y; c.film_y[2] = fy.z;
-    ctx->camera = c;
+    return c;
+}
+
+void set_camera(btrc_ctx *ctx, const btrc_camera_desc *cd)
+{
+    if(!cd)
+        throw InvalidArg("camera is null");
+    ctx->camera = camera_consts(cd);
     ctx->has_camera = true;
 }
 
@@ -1236,27 +1244,73 @@ void nrc_collect(btrc_ctx *ctx, int32_t steps)
     n.last = r;
 }
 
-void render(btrc_ctx *ctx, const btrc_render_params *p, btrc_render_stats *stats, int32_t capture_bounces,
-            btrc_ray *capture, int64_t capture_max, int64_t *capture_count)
+// The views of a render: `n` cameras with film sizes, or (views null) the
+// context's camera at params' size. Films are laid out view after view.
+std::vector<ViewRec> make_views(btrc_ctx *ctx, const btrc_render_params *p, int32_t n, const btrc_view_desc *views)
+{
+    std::vector<ViewRec> out;
+    if(!views)
+    {
+        if(!ctx->has_camera)
+            throw StateError("render before btrc_hip_set_camera");
+        ViewRec v{};
+        v.camera = ctx->camera;
+        v.width = p->width;
+        v.height = p->height;
+        out.push_back(v);
+    }
+    else
+    {
+        if(n < 1 || n > MAX_VIEWS)
+            throw InvalidArg("view count must lie in 1.." + std::to_string(MAX_VIEWS));
+        for(int32_t i = 0; i < n; ++i)
+        {
+            ViewRec v{};
+            v.camera = camera_consts(&views[i].camera);
+            v.width = views[i].width;
+            v.height = views[i].height;
+            out.push_back(v);
+        }
+    }
+    uint64_t pixel_base = 0, path_base = 0;
+    for(ViewRec &v : out)
+    {
+        if(v.width <= 0 || v.height <= 0)
+            throw InvalidArg("width and height must be positive");
+        v.halton = make_halton_consts(v.width, v.height);
+        v.pixel_base = (uint32_t)pixel_base;
+        v.path_base = path_base;
+        const uint64_t pixels = (uint64_t)v.width * (uint64_t)v.height;
+        pixel_base += pixels;
+        path_base += pixels * (uint64_t)std::max(p->spp, 0);
+        if(pixel_base > 0x7fffffffull)
+            throw InvalidArg("film too large");
+    }
+    return out;
+}
+
+void render(btrc_ctx *ctx, const btrc_render_params *p, int32_t view_count, const btrc_view_desc *view_desc,
+            btrc_render_stats *stats, int32_t capture_bounces, btrc_ray *capture, int64_t capture_max,
+            int64_t *capture_count)
 {
     if(!p)
         throw InvalidArg("params is null");
     if(!ctx->has_scene)
         throw StateError("render before btrc_hip_upload_scene");
-    if(!ctx->has_camera)
-        throw StateError("render before btrc_hip_set_camera");
-    if(p->width <= 0 || p->height <= 0 || p->spp <= 0 || p->state_count <= 0)
-        throw InvalidArg("width, height, spp and state_count must be positive");
+    if(p->spp <= 0 || p->state_count <= 0)
+        throw InvalidArg("spp and state_count must be positive");
     if(p->min_depth < 0 || p->max_depth < 0)
         throw InvalidArg("depths must be non-negative");
     if(p->sampler != BTRC_SAMPLER_HALTON && p->sampler != BTRC_SAMPLER_INDEPENDENT)
         throw InvalidArg("unknown sampler");
-    const uint64_t pixels = (uint64_t)p->width * (uint64_t)p->height;
-    const uint64_t total = pixels * (uint64_t)p->spp;
     if(p->sample_offset < 0)
         throw InvalidArg("sample_offset must be non-negative");
-    if(pixels > 0x7fffffffull || ((uint64_t)p->spp + (uint64_t)p->sample_offset) * 31104ull > 0x7fffffffull)
-        throw InvalidArg("film or spp too large");
+    if(((uint64_t)p->spp + (uint64_t)p->sample_offset) * 31104ull > 0x7fffffffull)
+        throw InvalidArg("spp too large");
+    const std::vector<ViewRec> views = make_views(ctx, p, view_count, view_desc);
+    const ViewRec &last = views.back();
+    const uint64_t pixels = (uint64_t)last.pixel_base + (uint64_t)last.width * (uint64_t)last.height;
+    const uint64_t total = last.path_base + (uint64_t)last.width * (uint64_t)last.height * (uint64_t)p->spp;
     const bool nrc_on = ctx->nrc.params.enabled != 0;
     if(nrc_on)
     {
@@ -1289,9 +1343,12 @@ void render(btrc_ctx *ctx, const btrc_render_params *p, btrc_render_stats *stats
     if(!ctx->host_counters)
         BTRC_HIP_CHECK(hipHostMalloc((void **)&ctx->host_counters, sizeof(WaveCounters), hipHostMallocDefault));
     // a render adds to the previous film when asked and the film matches
-    const bool keep = p->accumulate && ctx->film_w == p->width && ctx->film_h == p->height &&
-                      ctx->film_has64 == (p->film_fp64 != 0) && ctx->film_has_albedo == (p->albedo != 0) &&
-                      ctx->film_has_normal == (p->normal != 0) && ctx->film_spp > 0;
+    bool same_views = ctx->film_views.size() == views.size();
+    for(size_t v = 0; same_views && v < views.size(); ++v)
+        same_views = ctx->film_views[v].width == views[v].width && ctx->film_views[v].height == views[v].height;
+    const bool keep = p->accumulate && same_views && ctx->film_has64 == (p->film_fp64 != 0) &&
+                      ctx->film_has_albedo == (p->albedo != 0) && ctx->film_has_normal == (p->normal != 0) &&
+                      ctx->film_spp > 0;
     ctx->film.alloc(4 * pixels);
     if(p->film_fp64)
         ctx->film64.alloc(3 * pixels);
@@ -1311,8 +1368,9 @@ void render(btrc_ctx *ctx, const btrc_render_params *p, btrc_render_stats *stats
     }
     ctx->film_has_albedo = p->albedo != 0;
     ctx->film_has_normal = p->normal != 0;
-    ctx->film_w = p->width;
-    ctx->film_h = p->height;
+    ctx->film_views = views;
+    ctx->film_pixels = (int64_t)pixels;
+    ctx->views.upload(views.data(), views.size(), s);
     ctx->film_spp = keep ? ctx->film_spp + p->spp : p->spp;
     ctx->film_has64 = p->film_fp64 != 0;
 
@@ -1322,8 +1380,8 @@ void render(btrc_ctx *ctx, const btrc_render_params *p, btrc_render_stats *stats
     BTRC_HIP_CHECK(hipMemcpyAsync(ctx->counters.ptr, ctx->host_counters, sizeof(WaveCounters), hipMemcpyHostToDevice, s));
 
     RenderConsts rc{};
-    rc.width = p->width;
-    rc.height = p->height;
+    rc.views = ctx->views.ptr;
+    rc.view_count = (int32_t)views.size();
     rc.spp = p->spp;
     rc.min_depth = p->min_depth;
     rc.max_depth = p->max_depth;
@@ -1349,8 +1407,6 @@ void render(btrc_ctx *ctx, const btrc_render_params *p, btrc_render_stats *stats
         launch_nrc_clear(s, n.rec.ptr, NRC_RECORDS, n.acc.ptr, NRC_RECORDS);
         BTRC_HIP_CHECK(hipMemsetAsync(n.counts.ptr, 0, 2 * sizeof(int32_t), s));
     }
-    rc.halton = make_halton_consts(p->width, p->height);
-    rc.camera = ctx->camera;
     rc.filter = ctx->filter;
 
     const TraceScene ts = ctx->trace_scene();
@@ -1458,9 +1514,9 @@ void render(btrc_ctx *ctx, const btrc_render_params *p, btrc_render_stats *stats
 
 void film_readback(btrc_ctx *ctx, float *rgba, double *rgb64)
 {
-    if(ctx->film_w <= 0)
+    if(ctx->film_pixels <= 0)
         throw StateError("no film: render first");
-    const int32_t pixels = ctx->film_w * ctx->film_h;
+    const int32_t pixels = (int32_t)ctx->film_pixels;
     hipStream_t s = ctx->stream;
     if(rgba)
     {
@@ -1487,11 +1543,11 @@ void film_readback(btrc_ctx *ctx, float *rgba, double *rgb64)
 // preview.cpp generate_albedo / generate_normal
 void aov_readback(btrc_ctx *ctx, float *albedo, float *normal)
 {
-    if(ctx->film_w <= 0)
+    if(ctx->film_pixels <= 0)
         throw StateError("no film: render first");
     if((albedo && !ctx->film_has_albedo) || (normal && !ctx->film_has_normal))
         throw StateError("the last render did not record that output");
-    const int32_t pixels = ctx->film_w * ctx->film_h;
+    const int32_t pixels = (int32_t)ctx->film_pixels;
     hipStream_t s = ctx->stream;
     ctx->resolved.alloc(pixels);
     const float inv = 1.0f / (float)ctx->film_spp;
@@ -1634,7 +1690,34 @@ int btrc_hip_scene_info(btrc_ctx *ctx, int64_t *nodes, int32_t *depth, int32_t *
 
 int btrc_hip_render(btrc_ctx *ctx, const btrc_render_params *params, btrc_render_stats *stats)
 {
-    return guarded(ctx, [&] { render(ctx, params, stats, 0, nullptr, 0, nullptr); });
+    return guarded(ctx, [&] { render(ctx, params, 0, nullptr, stats, 0, nullptr, 0, nullptr); });
+}
+
+int btrc_hip_render_views(btrc_ctx *ctx, const btrc_render_params *params, int32_t view_count,
+                          const btrc_view_desc *views, btrc_render_stats *stats)
+{
+    return guarded(ctx, [&] {
+        if(!views)
+            throw InvalidArg("views is null");
+        render(ctx, params, view_count, views, stats, 0, nullptr, 0, nullptr);
+    });
+}
+
+int btrc_hip_view_readback(btrc_ctx *ctx, int32_t view, float *rgba)
+{
+    return guarded(ctx, [&] {
+        if(view < 0 || view >= (int32_t)ctx->film_views.size() || !rgba)
+            throw InvalidArg("bad view readback");
+        const ViewRec &v = ctx->film_views[view];
+        const int32_t pixels = v.width * v.height;
+        hipStream_t s = ctx->stream;
+        ctx->resolved.alloc(pixels);
+        launch_resolve(s, RESOLVE_COLOR, ctx->film.ptr + 4 * (size_t)v.pixel_base, pixels,
+                       1.0f / (float)ctx->film_spp, ctx->resolved.ptr);
+        BTRC_HIP_CHECK(hipGetLastError());
+        BTRC_HIP_CHECK(hipMemcpyAsync(rgba, ctx->resolved.ptr, pixels * sizeof(float4), hipMemcpyDeviceToHost, s));
+        BTRC_HIP_CHECK(hipStreamSynchronize(s));
+    });
 }
 
 int btrc_hip_capture_rays(btrc_ctx *ctx, const btrc_render_params *params, int32_t bounces,
@@ -1643,7 +1726,7 @@ int btrc_hip_capture_rays(btrc_ctx *ctx, const btrc_render_params *params, int32
     return guarded(ctx, [&] {
         if(!rays || max_rays < 0 || bounces < 0)
             throw InvalidArg("bad capture buffer");
-        render(ctx, params, nullptr, bounces, rays, max_rays, count);
+        render(ctx, params, 0, nullptr, nullptr, bounces, rays, max_rays, count);
     });
 }
 

@@ -93,9 +93,26 @@ struct FilterConsts
     const AliasUnit *table;   // GAUSSIAN_GRID^2 cells, row-major (y, x)
 };
 
+// One view of a render: its camera and film. A render walks the views'
+// paths in order: view v owns path indices [path_base, path_base + width *
+// height * spp) and film pixels [pixel_base, pixel_base + width * height).
+struct ViewRec
+{
+    CameraConsts camera;
+    HaltonConsts halton;
+    int32_t width, height;
+    uint32_t pixel_base;
+    uint32_t pad;
+    uint64_t path_base;
+};
+
+constexpr int32_t MAX_VIEWS = 64;
+
 struct RenderConsts
 {
-    int32_t width, height, spp;
+    const ViewRec *views;
+    int32_t view_count;
+    int32_t spp;
     int32_t min_depth, max_depth;
     float rr_threshold, rr_cont_prob;
     int32_t state_count;
@@ -106,10 +123,17 @@ struct RenderConsts
     // cache mode: paths whose pixel % nrc_stride == nrc_phase are training
     // paths with id pixel / nrc_stride while that is below nrc_max_paths
     int32_t nrc_stride, nrc_phase, nrc_max_paths;
-    HaltonConsts halton;
-    CameraConsts camera;
     FilterConsts filter;
 };
+
+// the view holding film pixel `pixel`
+__device__ __forceinline__ const ViewRec &view_of_pixel(const RenderConsts &rc, uint32_t pixel)
+{
+    int32_t v = rc.view_count - 1;
+    while(v > 0 && pixel < rc.views[v].pixel_base)
+        --v;
+    return rc.views[v];
+}
 
 struct FilmView
 {

@@ -71,9 +71,9 @@ template<> struct PathSampler<SAMPLER_HALTON>
 {
     HaltonState st;
 
-    __device__ static PathSampler start(const RenderConsts &rc, uint32_t px, uint32_t py, int32_t sample)
+    __device__ static PathSampler start(const ViewRec &view, uint32_t px, uint32_t py, int32_t sample)
     {
-        return PathSampler{ halton_start(rc.halton, px, py, sample) };
+        return PathSampler{ halton_start(view.halton, px, py, sample) };
     }
     __device__ static PathSampler load(const RenderConsts &, uint4 s)
     {
@@ -96,13 +96,15 @@ template<> struct PathSampler<SAMPLER_INDEPENDENT>
 {
     Pcg32 rng;
 
-    __device__ static PathSampler start(const RenderConsts &, uint32_t px, uint32_t py, int32_t sample)
+    __device__ static PathSampler start(const ViewRec &, uint32_t px, uint32_t py, int32_t sample)
     {
         return PathSampler{ independent_start(px, py, sample) };
     }
     __device__ static PathSampler load(const RenderConsts &rc, uint4 s)
     {
-        const uint32_t px = s.x % (uint32_t)rc.width, py = s.x / (uint32_t)rc.width;
+        const ViewRec &view = view_of_pixel(rc, s.x);
+        const uint32_t local = s.x - view.pixel_base;
+        const uint32_t px = local % (uint32_t)view.width, py = local / (uint32_t)view.width;
         PathSampler r;
         r.rng.inc = (murmur_hash_pixel(px, py) << 1) | 1ull;
         r.rng.state = (uint64_t)s.y | ((uint64_t)s.z << 32);
@@ -176,33 +178,39 @@ __global__ void __launch_bounds__(256) k_generate(RenderConsts rc, const WaveCou
     const int32_t n = c->gen_count;
     const int32_t offset = c->gen_offset;
     const uint64_t base = c->gen_base;
-    const uint64_t pixels = (uint64_t)rc.width * (uint64_t)rc.height;
     for(int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
     {
+        const uint64_t path_index = base + (uint64_t)i;
+        int32_t v = rc.view_count - 1;
+        while(v > 0 && path_index < rc.views[v].path_base)
+            --v;
+        const ViewRec &view = rc.views[v];
+        const uint64_t local = path_index - view.path_base;
+        const uint64_t pixels = (uint64_t)view.width * (uint64_t)view.height;
         // generate.cpp:46-62: uniform order walks all pixels once per sample
         // index; tile order finishes a pixel's samples before the next pixel
-        const uint64_t path_index = base + (uint64_t)i;
-        uint32_t pixel_index;
+        uint32_t local_pixel;
         int32_t sample_index;
         if(rc.tile)
         {
-            pixel_index = (uint32_t)(path_index / (uint64_t)rc.spp);
-            sample_index = (int32_t)(path_index % (uint64_t)rc.spp);
+            local_pixel = (uint32_t)(local / (uint64_t)rc.spp);
+            sample_index = (int32_t)(local % (uint64_t)rc.spp);
         }
         else
         {
-            pixel_index = (uint32_t)(path_index % pixels);
-            sample_index = (int32_t)(path_index / pixels);
+            local_pixel = (uint32_t)(local % pixels);
+            sample_index = (int32_t)(local / pixels);
         }
         sample_index += rc.sample_offset;
+        const uint32_t pixel_index = view.pixel_base + local_pixel; // film pixel
         uint32_t train = NO_TRAIN;
         if(rc.nrc_stride > 0 && pixel_index % (uint32_t)rc.nrc_stride == (uint32_t)rc.nrc_phase &&
            pixel_index / (uint32_t)rc.nrc_stride < (uint32_t)rc.nrc_max_paths)
             train = pixel_index / (uint32_t)rc.nrc_stride;
-        const uint32_t px = pixel_index % (uint32_t)rc.width;
-        const uint32_t py = pixel_index / (uint32_t)rc.width;
+        const uint32_t px = local_pixel % (uint32_t)view.width;
+        const uint32_t py = local_pixel / (uint32_t)view.width;
 
-        PathSampler<SAMPLER> smp = PathSampler<SAMPLER>::start(rc, px, py, sample_index);
+        PathSampler<SAMPLER> smp = PathSampler<SAMPLER>::start(view, px, py, sample_index);
         float fx, fy;
         if(rc.filter.type == FILTER_GAUSSIAN)
         {
@@ -221,12 +229,12 @@ __global__ void __launch_bounds__(256) k_generate(RenderConsts rc, const WaveCou
             fx = smp.get1d() - 0.5f;
             fy = smp.get1d() - 0.5f;
         }
-        const float film_x = ((float)px + 0.5f + fx) / (float)rc.width;
-        const float film_y = ((float)py + 0.5f + fy) / (float)rc.height;
+        const float film_x = ((float)px + 0.5f + fx) / (float)view.width;
+        const float film_y = ((float)py + 0.5f + fy) / (float)view.height;
         (void)smp.get1d(); // time sample, generate.cpp:70
 
         // PinholeCamera::generate_ray_inline, camera/pinhole.cpp:84-99
-        const CameraConsts &cam = rc.camera;
+        const CameraConsts &cam = view.camera;
         const V3 x = film_x * v3(cam.film_x[0], cam.film_x[1], cam.film_x[2]);
         const V3 y = film_y * v3(cam.film_y[0], cam.film_y[1], cam.film_y[2]);
         const V3 dst = v3(cam.lbc[0], cam.lbc[1], cam.lbc[2]) + x + y;

@@ -337,9 +337,24 @@ int btrc_hip_set_filter(btrc_ctx *ctx, const btrc_filter_desc *filter);
 
 /* ---- rendering (WavefrontPathTracer::render) ---- */
 int btrc_hip_render(btrc_ctx *ctx, const btrc_render_params *params, btrc_render_stats *stats);
-/* value / weight per pixel as RGBA (alpha 1), row 0 at the top; rgb64 (3
- * doubles per pixel, same normalisation) needs film_fp64, may be NULL */
+/* Several views in one wavefront (no counterpart in the reference, which
+ * renders one camera per Renderer): the views' paths share the pool and
+ * every stage launch, so small films still fill the GPU. View v gets the
+ * samples btrc_hip_render would give it with its camera and size;
+ * params->width and height are ignored. 1 .. 64 views. */
+typedef struct btrc_view_desc
+{
+    btrc_camera_desc camera;
+    int32_t width, height;
+} btrc_view_desc;
+int btrc_hip_render_views(btrc_ctx *ctx, const btrc_render_params *params, int32_t view_count,
+                          const btrc_view_desc *views, btrc_render_stats *stats);
+/* value / weight per pixel as RGBA (alpha 1), row 0 at the top, all views of
+ * the last render one after another; rgb64 (3 doubles per pixel, same
+ * normalisation) needs film_fp64, may be NULL */
 int btrc_hip_film_readback(btrc_ctx *ctx, float *rgba, double *rgb64);
+/* RGBA of one view of the last render */
+int btrc_hip_view_readback(btrc_ctx *ctx, int32_t view, float *rgba);
 /* RenderResult::albedo / normal (wavefront.cpp:320-350, preview.cpp:28-80):
  * RGBA per pixel, alpha 1; either pointer may be NULL; each needs the
  * matching flag in the last render's params */
