@@ -80,12 +80,14 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--spp", type=int, default=SPP)
-    ap.add_argument("--workload", choices=["field", "media", "nrc"], default="field",
+    ap.add_argument("--workload", choices=["field", "media", "nrc", "instanced"], default="field",
                     help="field: the 1M-triangle scene (default); media: the Cornell-style room with a "
                          "128^3 heterogeneous medium, max depth 16; nrc: the Cornell-style room at 256^2, "
                          "1 spp per frame for --frames frames, with the radiance cache against plain path "
-                         "tracing at equal time")
-    ap.add_argument("--frames", type=int, default=256)
+                         "tracing at equal time; instanced: the 16.8M-triangle textured two-level field, eight "
+                         "views rendered together, 1 spp per frame with the radiance cache for --frames frames "
+                         "(64 when not given)")
+    ap.add_argument("--frames", type=int, default=None)
     ap.add_argument("--gt-spp", type=int, default=1024,
                     help="samples per pixel of the FP64 ground truth for relMSE (0: skip)")
     args = ap.parse_args()
@@ -98,7 +100,12 @@ def main():
     from btrc_amd import scenes, _abi
 
     if args.workload == "nrc":
+        args.frames = 256 if args.frames is None else args.frames
         print(json.dumps(nrc_workload(args)), flush=True)
+        return
+    if args.workload == "instanced":
+        args.frames = 64 if args.frames is None else args.frames
+        print(json.dumps(instanced_workload(args)), flush=True)
         return
 
     media = args.workload == "media"
@@ -122,25 +129,31 @@ def main():
     p.min_depth, p.max_depth = MIN_DEPTH, max_depth
     p.rr_threshold, p.rr_cont_prob = 0.1, 0.5
     p.state_count = STATE_COUNT
-    p.time_stages = 1
+    p.time_stages = 0   # timed frames replay the captured bounce loop
 
     for i in range(args.warmup):
         st = ctx.render(p)
         log(f"bench: warmup {i}: {st.seconds:.3f} s")
 
-    totals = dict(seconds=0.0, ext=0, shadow=0, paths=0, stage_ms=np.zeros(7), launches=np.zeros(7))
+    totals = dict(seconds=0.0, ext=0, shadow=0, paths=0)
     for i in range(args.steps):
         st = ctx.render(p)
         totals["seconds"] += st.seconds
         totals["ext"] += st.extension_rays
         totals["shadow"] += st.shadow_rays
         totals["paths"] += st.paths
-        totals["stage_ms"] += np.array(st.stage_ms[:7])
-        totals["launches"] += np.array(st.stage_launches[:7])
         log(f"bench: step {i}: {st.seconds:.3f} s, {(st.extension_rays + st.shadow_rays) / st.seconds / 1e6:.1f} "
-            f"Mrays/s, stage ms {[round(x, 1) for x in st.stage_ms]}")
-
+            f"Mrays/s")
     frame = ctx.film(WIDTH, HEIGHT)[0][..., :3].astype(np.float64)
+
+    # one more frame with HIP events around every stage launch (direct
+    # launches, not the graph): per-stage times and the traversal launch time
+    ps = _abi.RenderParams.from_buffer_copy(p)
+    ps.time_stages = 1
+    st = ctx.render(ps)
+    staged = dict(seconds=st.seconds, ext=st.extension_rays, stage_ms=np.array(st.stage_ms[:7]),
+                  launches=np.array(st.stage_launches[:7]))
+    log(f"bench: stage-timed frame: {st.seconds:.3f} s, stage ms {[round(x, 1) for x in st.stage_ms]}")
     quality = None
     if args.gt_spp > 0:
         # FP64 ground truth from the independent sampler, so its samples share
@@ -159,7 +172,7 @@ def main():
     secs = totals["seconds"]
     mrays = (totals["ext"] + totals["shadow"]) / secs / 1e6
     samples = totals["paths"] / secs
-    trace_ms = totals["stage_ms"][1]
+    trace_ms = staged["stage_ms"][1]
 
     cpu = None
     bytes_per_ray = None
@@ -186,8 +199,8 @@ def main():
         # the medium stage dominates here; its roofline is not modelled yet
         pass
     elif bytes_per_ray is not None and trace_ms > 0:
-        launches = max(1, int(totals["launches"][1]))
-        rays_per_launch = totals["ext"] / launches
+        launches = max(1, int(staged["launches"][1]))
+        rays_per_launch = staged["ext"] / launches
         launch_s = trace_ms * 1e-3 / launches
         achieved = bytes_per_ray * rays_per_launch / launch_s / 1e9
         traffic_per_ray = read_traffic(os.path.join(ROOT, "profiles", "trace_extension_traffic.md"))
@@ -224,8 +237,10 @@ def main():
         "samples_per_s": round(samples, 1),
         "extension_rays": int(totals["ext"]),
         "shadow_rays": int(totals["shadow"]),
-        "stage_ms": {k: round(float(v), 2) for k, v in zip(("generate", "trace", "shade", "shadow", "medium", "cache", "train"),
-                                                           totals["stage_ms"])},
+        "stage_ms_one_frame": {k: round(float(v), 2) for k, v in
+                               zip(("generate", "trace", "shade", "shadow", "medium", "cache", "train"),
+                                   staged["stage_ms"])},
+        "stage_timed_frame_s": round(staged["seconds"], 4),
         "quality": quality,
         "roofline": roofline,
         "cpu_baseline": cpu,
@@ -363,6 +378,110 @@ def nrc_workload(args):
                      "infer_ms_per_frame": round(infer_ms / args.frames, 4),
                      "train_ms_per_frame": round(train_ms / args.frames, 4)},
         "cpu_baseline": cpu,
+    }
+
+
+def instanced_workload(args):
+    """The 16.8M-triangle textured instanced field (65,536 instances of four
+    256-triangle spheres, 1024^2 RGBA8 base-colour textures, two-level
+    traversal), eight views from 512x512 to 1920x1080 rendered together in
+    one wavefront, one sample per pixel per frame for `frames` frames,
+    accumulated, with the radiance cache queried at depth 2 and trained on
+    1/16 of the pixels every frame. Frames replay the captured bounce loop."""
+    import btrc_amd
+    from btrc_amd import _abi, scenes
+    log("bench: building the 16.8M-triangle instanced field")
+    t0 = time.perf_counter()
+    scene, views = scenes.instanced_field()
+    log(f"bench: scene built in {time.perf_counter() - t0:.1f} s")
+    ctx = btrc_amd.Context(0)
+    t0 = time.perf_counter()
+    ctx.upload_scene(scene)
+    info = ctx.scene_info()
+    log(f"bench: scene uploaded ({info['triangles']} tris, {info['nodes']} BVH4 nodes, depth {info['depth']}, "
+        f"stack {info['stack']}) in {time.perf_counter() - t0:.1f} s on {ctx.device_name()}")
+    pixels = sum(w * h for _, w, h in views)
+
+    p = _abi.RenderParams()
+    p.spp = 1
+    p.min_depth, p.max_depth = MIN_DEPTH, MAX_DEPTH
+    p.rr_threshold, p.rr_cont_prob = 0.1, 0.5
+    p.state_count = STATE_COUNT
+
+    ctx.nrc_configure(enabled=1, train=1)
+    for f in range(args.warmup):
+        p.sample_offset, p.accumulate = f, 1 if f else 0
+        st = ctx.render_views(p, views)
+        log(f"bench: warmup frame {f}: {st.seconds * 1e3:.1f} ms")
+    ctx.nrc_reset(1)
+    seconds, ext, shadow, paths = 0.0, 0, 0, 0
+    wall0 = time.perf_counter()
+    for f in range(args.frames):
+        p.sample_offset, p.accumulate = f, 1 if f else 0
+        st = ctx.render_views(p, views)
+        seconds += st.seconds
+        ext += st.extension_rays
+        shadow += st.shadow_rays
+        paths += st.paths
+        if f % 16 == 0 or f == args.frames - 1:
+            log(f"bench: frame {f}: {st.seconds * 1e3:.1f} ms, "
+                f"{(st.extension_rays + st.shadow_rays) / st.seconds / 1e6:.1f} Mrays/s")
+    wall = time.perf_counter() - wall0
+    ns = ctx.nrc_stats()
+    cam0, w0, h0 = views[0]
+    img0 = ctx.view_film(0, w0, h0)[..., :3].astype(np.float64)
+
+    # one more frame with per-stage timing (events between stages: no graph)
+    p.sample_offset, p.accumulate, p.time_stages = args.frames, 1, 1
+    st = ctx.render_views(p, views)
+    stage = np.array(st.stage_ms[:7])
+    p.time_stages = 0
+
+    quality = None
+    if args.gt_spp > 0:
+        ctx.nrc_configure(enabled=0)
+        ctx.set_camera(cam0, w0, h0)
+        g = _abi.RenderParams.from_buffer_copy(p)
+        g.width, g.height, g.spp, g.film_fp64, g.sampler = w0, h0, args.gt_spp, 1, _abi.SAMPLER_INDEPENDENT
+        g.sample_offset, g.accumulate = 0, 0
+        t0 = time.perf_counter()
+        ctx.render(g)
+        gt = ctx.film(w0, h0, fp64=True)[1]
+        quality = dict(view=0, relmse=float(np.mean((img0 - gt) ** 2 / (gt ** 2 + 1e-2))), frames=args.frames,
+                       gt_spp=args.gt_spp, gt_sampler="independent", gt_film="fp64", cache=True,
+                       gt_seconds=round(time.perf_counter() - t0, 2))
+        log(f"bench: view 0 relMSE after {args.frames} frames vs {args.gt_spp} spp FP64 GT: {quality['relmse']:.5f}")
+
+    return {
+        "metric": METRIC,
+        "value": round((ext + shadow) / seconds / 1e6, 2),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.frames,
+        "warmup": args.warmup,
+        "ms_per_step": round(seconds / args.frames * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 paths, bf16 network (fp32 accumulation)",
+        "data": "synthetic",
+        "config": {"workload": "instanced field: 65,536 instances x 256-tri spheres (16.8M triangles, two-level), "
+                               "1024^2 RGBA8 noise base-colour textures, 64 sphere lights + constant env; 8 views "
+                               "512x512..1920x1080 in one wavefront; 1 spp per frame with the radiance cache, "
+                               f"{args.frames} frames accumulated, max depth {MAX_DEPTH}",
+                   "views": [[w, h] for _, w, h in views], "pixels": pixels, "frames": args.frames,
+                   "triangles": info["triangles"], "state_count": STATE_COUNT, "launch_graph": True},
+        "samples_per_s": round(paths / seconds, 1),
+        "extension_rays": int(ext),
+        "shadow_rays": int(shadow),
+        "wall_seconds": round(wall, 3),
+        "stage_ms_one_frame": {k: round(float(v), 3) for k, v in
+                               zip(("generate", "trace", "shade", "shadow", "medium", "cache", "train"), stage)},
+        "nrc_last_frame": {"queries": int(ns.queries), "records": int(ns.records),
+                           "loss": round(float(ns.loss[0]), 5)},
+        "quality": quality,
+        "roofline": None,
+        "cpu_baseline": None,
     }
 
 

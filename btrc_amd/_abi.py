@@ -129,7 +129,7 @@ class RenderStats(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("closest_postpone_leaves", C.c_int32), ("any_postpone_leaves", C.c_int32),
-                ("trace_waves_per_cu", C.c_int32), ("reserved", C.c_int32 * 5)]
+                ("trace_waves_per_cu", C.c_int32), ("launch_graph", C.c_int32), ("reserved", C.c_int32 * 4)]
 
 
 class Ray(C.Structure):

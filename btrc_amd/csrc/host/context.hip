@@ -254,7 +254,20 @@ struct btrc_ctx
     DevBuf<float> ovf_t;
     int32_t trace_grid = 0;
     DevBuf<uint32_t> fetch;
-    btrc_tuning tuning{ 1, 0, TRACE_WAVES_PER_CU, { 0, 0, 0, 0, 0 } };
+    btrc_tuning tuning{ 1, 0, TRACE_WAVES_PER_CU, 1, { 0, 0, 0, 0 } };
+
+    // the bounce loop as a captured graph, kept while its launches stay the same
+    RenderConsts *rc_host = nullptr;  // pinned
+    DevBuf<RenderConsts> rc_dev;
+    hipGraphExec_t loop_exec = nullptr;
+    std::vector<uint8_t> loop_key;
+    void drop_loop_graph()
+    {
+        if(loop_exec)
+            (void)hipGraphExecDestroy(loop_exec);
+        loop_exec = nullptr;
+        loop_key.clear();
+    }
 
     // resident ray batch
     DevBuf<float4> batch_rays;
@@ -276,6 +289,7 @@ struct btrc_ctx
     SceneView scene_view() const
     {
         SceneView v;
+        std::memset(&v, 0, sizeof(v)); // padding too: the view is compared bytewise
         v.instances = instances.ptr;
         v.geometries = geometries.ptr;
         v.materials = materials.ptr;
@@ -1422,6 +1436,15 @@ void render(btrc_ctx *ctx, const btrc_render_params *p, int32_t view_count, cons
     const int grid256 = ctx->persistent_grid(256, (int64_t)P);
     const TraceOverflow ovf = ctx->trace_overflow();
 
+    // the constants live in device memory so that a captured loop stays
+    // valid from one render to the next
+    if(!ctx->rc_host)
+        BTRC_HIP_CHECK(hipHostMalloc((void **)&ctx->rc_host, sizeof(RenderConsts), hipHostMallocDefault));
+    ctx->rc_dev.alloc(1);
+    *ctx->rc_host = rc;
+    BTRC_HIP_CHECK(hipMemcpyAsync(ctx->rc_dev.ptr, ctx->rc_host, sizeof(RenderConsts), hipMemcpyHostToDevice, s));
+    const RenderConsts *rcd = ctx->rc_dev.ptr;
+
     StageTimer timer;
     timer.enabled = p->time_stages != 0;
     timer.stream = s;
@@ -1430,7 +1453,73 @@ void render(btrc_ctx *ctx, const btrc_render_params *p, int32_t view_count, cons
     int64_t captured = 0;
     const auto t0 = std::chrono::steady_clock::now();
     int cur = 0;
-    const int check_every = 4;
+    const int check_every = 4; // even: a block of bounces ends on the pool side it began with
+
+    // one bounce: plan, generate, trace, [medium], shade, shadow
+    auto enqueue_bounce = [&](int side, bool timed) {
+        PathSOA path = ctx->pool[side].view();
+        PathSOA next = ctx->pool[side ^ 1].view();
+        launch_plan(s, ctx->counters.ptr, p->state_count);
+        auto run = [&](int stage, auto &&fn) {
+            if(timed)
+                timer.run(stage, fn);
+            else
+                fn();
+        };
+        run(STAGE_GENERATE, [&] { launch_generate(s, grid256, p->sampler, rcd, ctx->counters.ptr, path); });
+        run(STAGE_TRACE, [&] { launch_trace_extension(s, ctx->trace_grid, ctx->trace_sched(false), ts, ctx->counters.ptr, path, hit, ovf); });
+        if(media)
+            run(STAGE_MEDIUM, [&] { launch_medium(s, grid256, p->sampler, rcd, sv, ctx->counters.ptr, path, hit, next, sh, fv); });
+        run(STAGE_SHADE, [&] { launch_shade(s, grid256, p->sampler, rcd, sv, ctx->counters.ptr, path, hit, next, sh, fv, nv); });
+        run(STAGE_SHADOW, [&] { launch_shadow(s, ctx->trace_grid, ctx->trace_sched(true), media, ts, sv, ctx->counters.ptr, sh, fv, ovf); });
+        launch_end_iteration(s, ctx->counters.ptr);
+    };
+
+    if(ctx->tuning.launch_graph && !timer.enabled && !capture)
+    {
+        // blocks of check_every bounces from a captured graph; recaptured
+        // when any launch argument differs from the cached graph's
+        std::vector<uint8_t> key;
+        auto add = [&](const auto &v) {
+            const uint8_t *b = reinterpret_cast<const uint8_t *>(&v);
+            key.insert(key.end(), b, b + sizeof(v));
+        };
+        add(ts); add(sv); add(hit); add(sh); add(fv); add(nv); add(rcd);
+        add(ovf.code); add(ovf.t); add(ovf.lanes);
+        add(ctx->pool[0].view()); add(ctx->pool[1].view()); add(ctx->counters.ptr);
+        const int32_t ints[8] = { grid256, ctx->trace_grid, ctx->trace_sched(false), ctx->trace_sched(true),
+                                  p->sampler, p->state_count, media ? 1 : 0, check_every };
+        add(ints);
+        if(!ctx->loop_exec || key != ctx->loop_key)
+        {
+            ctx->drop_loop_graph();
+            hipGraph_t graph = nullptr;
+            BTRC_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            for(int k = 0; k < check_every; ++k)
+                enqueue_bounce(k & 1, false);
+            BTRC_HIP_CHECK(hipStreamEndCapture(s, &graph));
+            const hipError_t e = hipGraphInstantiate(&ctx->loop_exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if(e != hipSuccess)
+            {
+                ctx->loop_exec = nullptr;
+                throw HipError(std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+            }
+            ctx->loop_key = key;
+        }
+        for(;;)
+        {
+            if(ctx->stop.load())
+                throw Stopped("render stopped");
+            BTRC_HIP_CHECK(hipGraphLaunch(ctx->loop_exec, s));
+            BTRC_HIP_CHECK(hipMemcpyAsync(ctx->host_counters, ctx->counters.ptr, sizeof(WaveCounters), hipMemcpyDeviceToHost, s));
+            BTRC_HIP_CHECK(hipStreamSynchronize(s));
+            const WaveCounters &hc = *ctx->host_counters;
+            if(hc.generated == hc.total && hc.active == 0)
+                break;
+        }
+    }
+    else
     for(uint64_t iter = 0;; ++iter)
     {
         if(ctx->stop.load())
@@ -1438,7 +1527,7 @@ void render(btrc_ctx *ctx, const btrc_render_params *p, int32_t view_count, cons
         PathSOA path = ctx->pool[cur].view();
         PathSOA next = ctx->pool[cur ^ 1].view();
         launch_plan(s, ctx->counters.ptr, p->state_count);
-        timer.run(STAGE_GENERATE, [&] { launch_generate(s, grid256, p->sampler, rc, ctx->counters.ptr, path); });
+        timer.run(STAGE_GENERATE, [&] { launch_generate(s, grid256, p->sampler, rcd, ctx->counters.ptr, path); });
         timer.run(STAGE_TRACE, [&] { launch_trace_extension(s, ctx->trace_grid, ctx->trace_sched(false), ts, ctx->counters.ptr, path, hit, ovf); });
         if(capture && (int64_t)iter < capture_bounces && captured < capture_max)
         {
@@ -1460,8 +1549,8 @@ void render(btrc_ctx *ctx, const btrc_render_params *p, int32_t view_count, cons
             captured += n;
         }
         if(media)
-            timer.run(STAGE_MEDIUM, [&] { launch_medium(s, grid256, p->sampler, rc, sv, ctx->counters.ptr, path, hit, next, sh, fv); });
-        timer.run(STAGE_SHADE, [&] { launch_shade(s, grid256, p->sampler, rc, sv, ctx->counters.ptr, path, hit, next, sh, fv, nv); });
+            timer.run(STAGE_MEDIUM, [&] { launch_medium(s, grid256, p->sampler, rcd, sv, ctx->counters.ptr, path, hit, next, sh, fv); });
+        timer.run(STAGE_SHADE, [&] { launch_shade(s, grid256, p->sampler, rcd, sv, ctx->counters.ptr, path, hit, next, sh, fv, nv); });
         timer.run(STAGE_SHADOW, [&] { launch_shadow(s, ctx->trace_grid, ctx->trace_sched(true), media, ts, sv, ctx->counters.ptr, sh, fv, ovf); });
         launch_end_iteration(s, ctx->counters.ptr);
         BTRC_HIP_CHECK(hipGetLastError());
@@ -1625,6 +1714,9 @@ void btrc_hip_destroy(btrc_ctx *ctx)
         (void)hipStreamSynchronize(ctx->stream);
     if(ctx->host_counters)
         (void)hipHostFree(ctx->host_counters);
+    if(ctx->rc_host)
+        (void)hipHostFree(ctx->rc_host);
+    ctx->drop_loop_graph();
     if(ctx->stream)
         (void)hipStreamDestroy(ctx->stream);
     delete ctx;

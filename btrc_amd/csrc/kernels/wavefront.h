@@ -163,14 +163,14 @@ enum : int
 };
 
 void launch_plan(hipStream_t s, WaveCounters *c, int32_t state_count);
-void launch_generate(hipStream_t s, int grid, int sampler, const RenderConsts &rc, const WaveCounters *c,
+void launch_generate(hipStream_t s, int grid, int sampler, const RenderConsts *rc, const WaveCounters *c,
                      PathSOA path);
 void launch_trace_extension(hipStream_t s, int grid, int sched, const TraceScene &ts, WaveCounters *c,
                             PathSOA path, HitSOA hit, const TraceOverflow &ovf);
-void launch_shade(hipStream_t s, int grid, int sampler, const RenderConsts &rc, const SceneView &sv,
+void launch_shade(hipStream_t s, int grid, int sampler, const RenderConsts *rc, const SceneView &sv,
                   WaveCounters *c, PathSOA path, HitSOA hit, PathSOA next, ShadowSOA shadow, FilmView film,
                   const NrcView &nrc);
-void launch_medium(hipStream_t s, int grid, int sampler, const RenderConsts &rc, const SceneView &sv,
+void launch_medium(hipStream_t s, int grid, int sampler, const RenderConsts *rc, const SceneView &sv,
                    WaveCounters *c, PathSOA path, HitSOA hit, PathSOA next, ShadowSOA shadow, FilmView film);
 void launch_shadow(hipStream_t s, int grid, int sched, bool media, const TraceScene &ts, const SceneView &sv,
                    WaveCounters *c, ShadowSOA shadow, FilmView film, const TraceOverflow &ovf);

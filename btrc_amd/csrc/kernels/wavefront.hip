@@ -173,8 +173,10 @@ __global__ void k_plan(WaveCounters *c, int32_t state_count)
 // ---------------------------------------------------------------- generate
 
 template<int SAMPLER>
-__global__ void __launch_bounds__(256) k_generate(RenderConsts rc, const WaveCounters *c, PathSOA path)
+__global__ void __launch_bounds__(256) k_generate(const RenderConsts *__restrict__ rcp, const WaveCounters *c,
+                                                  PathSOA path)
 {
+    const RenderConsts &rc = *rcp;
     const int32_t n = c->gen_count;
     const int32_t offset = c->gen_offset;
     const uint64_t base = c->gen_base;
@@ -323,9 +325,10 @@ __device__ __forceinline__ void train_add(float *acc, uint32_t tid, V3 v)
 
 template<int SAMPLER>
 __global__ void __launch_bounds__(256) k_shade(
-    RenderConsts rc, SceneView sv, WaveCounters *c, PathSOA path, HitSOA hit,
+    const RenderConsts *__restrict__ rcp, SceneView sv, WaveCounters *c, PathSOA path, HitSOA hit,
     PathSOA next, ShadowSOA shadow, FilmView film, NrcView nrc)
 {
+    const RenderConsts &rc = *rcp;
     const int32_t n = c->active;
     const int32_t stride = gridDim.x * blockDim.x;
     for(int32_t base = blockIdx.x * blockDim.x; base < n; base += stride)
@@ -549,9 +552,10 @@ __global__ void __launch_bounds__(256) k_shade(
 // stage then only adds the emission the segment ended on.
 template<int SAMPLER>
 __global__ void __launch_bounds__(256) k_medium(
-    RenderConsts rc, SceneView sv, WaveCounters *c, PathSOA path, HitSOA hit,
+    const RenderConsts *__restrict__ rcp, SceneView sv, WaveCounters *c, PathSOA path, HitSOA hit,
     PathSOA next, ShadowSOA shadow, FilmView film)
 {
+    const RenderConsts &rc = *rcp;
     const int32_t n = c->active;
     const int32_t stride = gridDim.x * blockDim.x;
     for(int32_t base = blockIdx.x * blockDim.x; base < n; base += stride)
@@ -904,7 +908,7 @@ void launch_plan(hipStream_t s, WaveCounters *c, int32_t state_count)
     hipLaunchKernelGGL(k_plan, dim3(1), dim3(1), 0, s, c, state_count);
 }
 
-void launch_generate(hipStream_t s, int grid, int sampler, const RenderConsts &rc, const WaveCounters *c,
+void launch_generate(hipStream_t s, int grid, int sampler, const RenderConsts *rc, const WaveCounters *c,
                      PathSOA path)
 {
     if(sampler == SAMPLER_INDEPENDENT)
@@ -925,7 +929,7 @@ void launch_trace_extension(hipStream_t s, int grid, int sched, const TraceScene
         hipLaunchKernelGGL(k_trace_extension<SCHED_EAGER>, g, b, 0, s, ts, c, path, hit, ovf);
 }
 
-void launch_shade(hipStream_t s, int grid, int sampler, const RenderConsts &rc, const SceneView &sv,
+void launch_shade(hipStream_t s, int grid, int sampler, const RenderConsts *rc, const SceneView &sv,
                   WaveCounters *c, PathSOA path, HitSOA hit, PathSOA next, ShadowSOA shadow, FilmView film,
                   const NrcView &nrc)
 {
@@ -937,7 +941,7 @@ void launch_shade(hipStream_t s, int grid, int sampler, const RenderConsts &rc, 
                            film, nrc);
 }
 
-void launch_medium(hipStream_t s, int grid, int sampler, const RenderConsts &rc, const SceneView &sv,
+void launch_medium(hipStream_t s, int grid, int sampler, const RenderConsts *rc, const SceneView &sv,
                    WaveCounters *c, PathSOA path, HitSOA hit, PathSOA next, ShadowSOA shadow, FilmView film)
 {
     if(sampler == SAMPLER_INDEPENDENT)

@@ -190,6 +190,8 @@ class Scene:
         hi = -lo
         for inst in self.instances:
             p = self.geometries[inst["geometry"]]["positions"].reshape(-1, 3).astype(np.float32)
+            if len(p) == 0:
+                continue
             m = inst["mat"].astype(np.float32)
             w = ((m[:, 0] * p[:, :1] + m[:, 1] * p[:, 1:2]) + m[:, 2] * p[:, 2:3]) + m[:, 3]
             lo = np.minimum(lo, w[:, :3].min(axis=0))
@@ -546,6 +548,47 @@ def sphere_field(instances=4096, lights=64, seed=2, extent=100.0, envir=(0.5, 0.
     c = extent / 2
     cam = Camera((c, c, -0.6 * extent), (c, c, c), (0.0, 1.0, 0.0), 60.0)
     return s, cam
+
+
+def instanced_field(instances=65536, lights=64, seed=2, extent=250.0, geometries=4, textures=8, texture_size=1024,
+                    envir=(0.5, 0.5, 0.5)):
+    """Textured instanced field: `geometries` 256-triangle displaced spheres
+    with texture coordinates, `instances` seeded placements (position
+    U[0, extent]^3, uniform rotation, scale U[0.5, 2]) each with its own
+    Disney material (roughness U[0.05, 1], metallic Bernoulli(0.3), specular
+    0.5) whose base colour is one of `textures` seeded RGBA8 noise textures
+    (texture_size^2, wrapped in u); `lights` sphere mesh lights; a constant
+    environment. Traced through the two-level structure. Returns the scene
+    and eight views of 512x512 up to 1920x1080 looking into the field."""
+    rng = PCG32(seed)
+    s = Scene()
+    s.accel = _abi.ACCEL_TWO_LEVEL
+    uvs = uv_sphere_uvs()
+    geos = [s.add_geometry(uv_sphere(rng, displacement=0.15), uvs=uvs) for _ in range(geometries)]
+    texs = [s.add_texture(noise_texture(texture_size, seed=seed * 100 + k), address_u="wrap") for k in range(textures)]
+    for k in range(instances):
+        pos = [float(rng.range(0.0, extent)) for _ in range(3)]
+        rot = uniform_rotation(rng)
+        sc = float(rng.range(0.5, 2.0))
+        mat = s.add_disney(color=[1.0, 1.0, 1.0], metallic=1.0 if float(rng.uniform()) < 0.3 else 0.0,
+                           roughness=float(rng.range(0.05, 1.0)), specular=0.5, color_texture=texs[k % textures])
+        s.add_instance(geos[k % geometries], mat, translate(*pos) @ rot @ scale(sc))
+    lg = s.add_geometry(uv_sphere())
+    black = s.add_diffuse([0.0, 0.0, 0.0])
+    for _ in range(lights):
+        pos = [float(rng.range(0.0, extent)) for _ in range(3)]
+        power = float(rng.range(5.0, 20.0))
+        tint = [float(rng.range(0.5, 1.0)) for _ in range(3)]
+        s.add_instance(lg, black, translate(*pos) @ scale(1.0), light=tuple(power * t for t in tint))
+    s.set_gradient_sky((0.0, 1.0, 0.0), envir, envir)
+    c = extent / 2
+    sizes = [(512, 512), (640, 480), (800, 600), (1024, 768), (1280, 720), (1280, 1024), (1600, 900), (1920, 1080)]
+    views = []
+    for k, (w, h) in enumerate(sizes):
+        a = 2 * math.pi * k / len(sizes)
+        eye = (c + 0.8 * extent * math.cos(a), c + 0.1 * extent * (k % 3 - 1), c + 0.8 * extent * math.sin(a))
+        views.append((Camera(eye, (c, c, c), (0.0, 1.0, 0.0), 60.0), w, h))
+    return s, views
 
 
 def load_obj_triangles(path):

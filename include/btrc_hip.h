@@ -314,8 +314,9 @@ typedef struct btrc_tuning
 {
     int32_t closest_postpone_leaves; /* 1: leaves wait until every lane holds one (default 1) */
     int32_t any_postpone_leaves;     /* same for visibility rays (default 0) */
-    int32_t trace_waves_per_cu;      /* persistent traversal grid, 1..32 (default 24) */
-    int32_t reserved[5];
+    int32_t trace_waves_per_cu;      /* persistent traversal grid, 1..32 (default 20) */
+    int32_t launch_graph;            /* 1: the bounce loop replays a captured HIP graph (default 1) */
+    int32_t reserved[4];
 } btrc_tuning;
 
 /* ---- context ---- */

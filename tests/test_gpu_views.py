@@ -31,10 +31,10 @@ def test_views_equal_separate_renders(ctx, sampler, tile):
     total = sum(w * h for _, w, h in views)
     rgba_all, rgb64_all = ctx.film(total, 1, fp64=True)
     rgb64_all = rgb64_all.reshape(-1, 3)
+    batches = [ctx.view_film(k, w, h) for k, (_, w, h) in enumerate(views)]
     base, ext, shadow = 0, 0, 0
     for k, (cam, w, h) in enumerate(views):
-        batch = ctx.view_film(k, w, h)
-        assert np.array_equal(batch, rgba_all.reshape(-1, 4)[base:base + w * h].reshape(h, w, 4))
+        assert np.array_equal(batches[k], rgba_all.reshape(-1, 4)[base:base + w * h].reshape(h, w, 4))
         batch64 = rgb64_all[base:base + w * h].reshape(h, w, 3)
         q = render_params(w, h, 4, state_count=3000, film_fp64=1)
         q.sampler, q.tile = sampler, tile
