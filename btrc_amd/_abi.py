@@ -129,7 +129,8 @@ class RenderStats(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("closest_postpone_leaves", C.c_int32), ("any_postpone_leaves", C.c_int32),
-                ("trace_waves_per_cu", C.c_int32), ("launch_graph", C.c_int32), ("reserved", C.c_int32 * 4)]
+                ("trace_waves_per_cu", C.c_int32), ("launch_graph", C.c_int32),
+                ("two_level_postpone", C.c_int32), ("reserved", C.c_int32 * 3)]
 
 
 class Ray(C.Structure):

@@ -254,7 +254,7 @@ struct btrc_ctx
     DevBuf<float> ovf_t;
     int32_t trace_grid = 0;
     DevBuf<uint32_t> fetch;
-    btrc_tuning tuning{ 1, 0, TRACE_WAVES_PER_CU, 1, { 0, 0, 0, 0 } };
+    btrc_tuning tuning{ 1, 0, TRACE_WAVES_PER_CU, 1, 0, { 0, 0, 0 } };
 
     // the bounce loop as a captured graph, kept while its launches stay the same
     RenderConsts *rc_host = nullptr;  // pinned
@@ -279,8 +279,9 @@ struct btrc_ctx
     int trace_sched(bool any) const
     {
         if(two_level)
-            return SCHED_TWO_LEVEL;
-        return (any ? tuning.any_postpone_leaves : tuning.closest_postpone_leaves) != 0 ? SCHED_POSTPONE : SCHED_EAGER;
+            return SCHED_TWO_LEVEL | ((tuning.two_level_postpone >> (any ? 1 : 0)) & 1 ? SCHED_POSTPONE : 0);
+        const bool postpone = (any ? tuning.any_postpone_leaves : tuning.closest_postpone_leaves) != 0;
+        return postpone ? SCHED_POSTPONE : SCHED_EAGER;
     }
     TraceOverflow trace_overflow() const
     {

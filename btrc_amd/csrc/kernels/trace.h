@@ -38,7 +38,8 @@ enum : int
 {
     SCHED_EAGER = 0,
     SCHED_POSTPONE = 1,
-    SCHED_TWO_LEVEL = 2,
+    SCHED_TWO_LEVEL = 2,          // | SCHED_POSTPONE for postponed leaves
+    SCHED_TWO_LEVEL_POSTPONE = 3,
 };
 
 // stack entry / node codes
@@ -46,6 +47,7 @@ constexpr uint32_t CODE_LEAF = 0x80000000u;   // | (count - 1) << 28 | first tri
 constexpr uint32_t CODE_INST = 0x40000000u;   // with CODE_LEAF: | instance (top-level leaf)
 constexpr uint32_t CODE_DONE = 0xffffffffu;
 constexpr uint32_t CODE_EXIT = 0xfffffffeu;   // stack marker: back to the top level
+constexpr uint32_t CODE_EXIT_NEXT = 0xfffffffdu; // node: the stack top is CODE_EXIT (leaf postponed)
 constexpr uint32_t MAX_LEAF_FIRST = 0x0ffffffeu;
 
 // an instance of the two-level structure: the top level's leaves point here
@@ -93,8 +95,10 @@ __device__ __forceinline__ uint32_t leaf_code(int32_t first, int32_t count)
 // The two-level variant (TWO): node 0 roots the instance tree, whose leaves
 // are instance codes; entering one pushes CODE_EXIT, moves the ray into the
 // instance's local space (same parametric t) and continues at its
-// geometry's tree; popping CODE_EXIT restores the world ray. Leaves are then
-// tested as soon as they are reached.
+// geometry's tree; popping CODE_EXIT restores the world ray. A lane holding
+// a postponed leaf keeps descending only inside the current instance: it
+// stops at an instance entry or before popping CODE_EXIT (node =
+// CODE_EXIT_NEXT), so the leaf is tested with its own instance's ray.
 //
 // Traces rays [0, n) handed out through *fetch. Sink provides
 //   void load(uint32_t i, V3 &o, V3 &d, float &tmin, float &tmax)
@@ -107,7 +111,6 @@ template<int TRI, bool ANY, bool POSTPONE, bool TWO, class Sink>
 __device__ __forceinline__ void trace_persistent(
     const TraceScene &sc, Sink &sink, int32_t n, uint32_t *fetch, const TraceOverflow &ovf)
 {
-    static_assert(!(TWO && POSTPONE), "two-level traversal tests leaves when it reaches them");
     __shared__ uint32_t s_code[SHORT_STACK * TRACE_BLOCK];
     __shared__ float s_t[SHORT_STACK * TRACE_BLOCK];
     const int lane = (int)threadIdx.x;
@@ -153,8 +156,9 @@ __device__ __forceinline__ void trace_persistent(
         }
         ++sp;
     };
-    // next entry whose subtree may still hold a closer hit, or CODE_DONE
-    auto pop = [&]() -> uint32_t {
+    // next entry whose subtree may still hold a closer hit, or CODE_DONE;
+    // stay_inside: stop before leaving the current instance
+    auto pop = [&](bool stay_inside) -> uint32_t {
         const float cull = cull_distance(h.t);
         while(sp > 0)
         {
@@ -174,6 +178,11 @@ __device__ __forceinline__ void trace_persistent(
             }
             if(TWO && code == CODE_EXIT)
             {
+                if(stay_inside)
+                {
+                    ++sp; // left for the lane to pop once its leaf is tested
+                    return CODE_EXIT_NEXT;
+                }
                 set_ray(o, d); // back to the world ray
                 cur_inst = 0xffffffffu;
                 continue;
@@ -261,7 +270,9 @@ __device__ __forceinline__ void trace_persistent(
                 break;
             if(step)
             {
-                if(TWO && (node & CODE_LEAF) && (node & CODE_INST))
+                if(TWO && node == CODE_EXIT_NEXT)
+                    node = pop(false);
+                else if(TWO && (node & CODE_LEAF) && (node & CODE_INST))
                 {
                     // enter an instance: world -> local, continue in its tree
                     const TraceInstance &ti = sc.inst[node & 0x0fffffffu];
@@ -283,10 +294,10 @@ __device__ __forceinline__ void trace_persistent(
                     if(POSTPONE)
                     {
                         leaf = node;
-                        node = pop();
+                        node = pop(TWO);
                     }
                     else
-                        node = (test_leaf(node) && ANY) ? CODE_DONE : pop();
+                        node = (test_leaf(node) && ANY) ? CODE_DONE : pop(false);
                 }
                 else
                 {
@@ -341,7 +352,7 @@ __device__ __forceinline__ void trace_persistent(
                         node = c[0];
                     }
                     else
-                        node = pop();
+                        node = pop(TWO && leaf != 0);
                 }
             }
             if(POSTPONE)

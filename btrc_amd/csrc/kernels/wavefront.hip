@@ -287,7 +287,7 @@ __global__ void __launch_bounds__(TRACE_BLOCK) k_trace_extension(
     TraceScene ts, WaveCounters *c, PathSOA path, HitSOA hit, TraceOverflow ovf)
 {
     ExtensionSink sink{ path, hit };
-    trace_persistent<TRI_WATERTIGHT, false, SCHED == SCHED_POSTPONE, SCHED == SCHED_TWO_LEVEL>(
+    trace_persistent<TRI_WATERTIGHT, false, (SCHED & SCHED_POSTPONE) != 0, (SCHED & SCHED_TWO_LEVEL) != 0>(
         ts, sink, c->active, &c->trace_fetch, ovf);
 }
 
@@ -324,7 +324,7 @@ __device__ __forceinline__ void train_add(float *acc, uint32_t tid, V3 v)
 }
 
 template<int SAMPLER>
-__global__ void __launch_bounds__(256) k_shade(
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) k_shade(
     const RenderConsts *__restrict__ rcp, SceneView sv, WaveCounters *c, PathSOA path, HitSOA hit,
     PathSOA next, ShadowSOA shadow, FilmView film, NrcView nrc)
 {
@@ -334,15 +334,9 @@ __global__ void __launch_bounds__(256) k_shade(
     for(int32_t base = blockIdx.x * blockDim.x; base < n; base += stride)
     {
         const int32_t i = base + (int32_t)threadIdx.x;
-        bool push_next = false, push_shadow = false;
-        // values carried to the pushes
-        float4 nx_o, nx_d, nx_beta, nx_rad, nx_le, sh_o, sh_d, sh_beta;
-        uint4 nx_smp;
-        uint2 sh_seed;
-        uint32_t train = NO_TRAIN;
-        bool push_query = false;
-        NrcQuery query;
-
+        // Queue pushes happen where their payload is made: a wave-aggregated
+        // atomic among the lanes that reach that point together, so no
+        // payload stays live across the rest of the shading code.
         if(i < n)
         {
             const uint32_t flag = hit.flag[i];
@@ -359,7 +353,7 @@ __global__ void __launch_bounds__(256) k_shade(
             V3 beta = xyz(beta_depth);
             const int32_t depth = __float_as_int(beta_depth.w);
             V3 path_rad = xyz(rad4);
-            train = __float_as_uint(rad4.w);
+            const uint32_t train = __float_as_uint(rad4.w);
             const V3 beta_le = xyz(le4);
             const float bsdf_pdf = le4.w;
 
@@ -407,10 +401,14 @@ __global__ void __launch_bounds__(256) k_shade(
                 else if(!done && depth == nrc.query_depth)
                 {
                     // the prefix ends here: the cache answers for the rest
-                    push_query = true;
-                    query.v = nrc_vertex(sv, inst, sp, d);
-                    query.beta[0] = beta.x; query.beta[1] = beta.y; query.beta[2] = beta.z;
-                    query.pixel = pixel;
+                    if(nrc.queries)
+                    {
+                        NrcQuery query;
+                        query.v = nrc_vertex(sv, inst, sp, d);
+                        query.beta[0] = beta.x; query.beta[1] = beta.y; query.beta[2] = beta.z;
+                        query.pixel = pixel;
+                        nrc.queries[wave_push(nrc.query_count, true)] = query;
+                    }
                     film_splat(film, pixel, path_rad + le_rad);
                     done = true;
                 }
@@ -458,11 +456,11 @@ __global__ void __launch_bounds__(256) k_shade(
                         if(film.normal)
                             aov_splat(film.normal, pixel, shader_normal(shader));
                     }
+                    // draws in the reference's order; the light connection
+                    // is finished (its shadow ray pushed) before the BSDF is
+                    // sampled, then the continuation
                     float s0, s1, s2;
                     hs.get3d(&s0, &s1, &s2);
-                    BSDFSample bs;
-                    shader_sample(shader, -d, s0, s1, s2, bs);
-
                     V3 shadow_f = v3s(0.0f);
                     float shadow_pdf = 0.0f;
                     if(ss.success)
@@ -481,13 +479,18 @@ __global__ void __launch_bounds__(256) k_shade(
                                            : (shadow_out ? inst.outer_medium : inst.inner_medium);
                         const float cs = fabs_(dot(normalize(ss.d), sp.frame.z));
                         const V3 beta_li = ss.li * beta * shadow_f * cs / (shadow_pdf + ss.light_pdf);
-                        push_shadow = true;
-                        sh_o = f4(ss.o, __uint_as_float(pixel));
-                        sh_d = f4(ss.d, ss.t1);
-                        sh_beta = f4(beta_li, __uint_as_float(med));
-                        sh_seed = shadow_seed(hs.pack(pixel));
+                        const int32_t si = wave_push(&c->shadow, true);
+                        shadow.o[si] = f4(ss.o, __uint_as_float(pixel));
+                        shadow.d[si] = f4(ss.d, ss.t1);
+                        shadow.beta[si] = f4(beta_li, __uint_as_float(med));
+                        if(shadow.seed)
+                            shadow.seed[si] = shadow_seed(hs.pack(pixel));
+                        if(shadow.train)
+                            shadow.train[si] = train;
                     }
 
+                    BSDFSample bs;
+                    shader_sample(shader, -d, s0, s1, s2, bs);
                     if(!is_zero(bs.bsdf))
                     {
                         const V3 dir = normalize(bs.dir);
@@ -498,46 +501,18 @@ __global__ void __launch_bounds__(256) k_shade(
                         const bool next_out = dot(sp.frame.z, dir) > 0.0f;
                         const uint32_t med = next_out ? inst.outer_medium : inst.inner_medium;
                         const float stored_pdf = bs.is_delta ? -bs.pdf : bs.pdf;
-                        push_next = true;
-                        nx_o = f4(next_o, __uint_as_float(med));
-                        nx_d = f4(dir, MAX_FLOAT);
-                        nx_beta = f4(beta, __int_as_float(depth + 1));
-                        nx_rad = f4(path_rad, __uint_as_float(train));
-                        nx_le = f4(new_beta_le, stored_pdf);
-                        nx_smp = hs.pack(pixel);
+                        const int32_t ni = wave_push(&c->next_active, true);
+                        next.ray_o[ni] = f4(next_o, __uint_as_float(med));
+                        next.ray_d[ni] = f4(dir, MAX_FLOAT);
+                        next.beta[ni] = f4(beta, __int_as_float(depth + 1));
+                        next.rad[ni] = f4(path_rad, __uint_as_float(train));
+                        next.bsdf_le[ni] = f4(new_beta_le, stored_pdf);
+                        next.smp[ni] = hs.pack(pixel);
                     }
                     else
                         film_splat(film, pixel, path_rad);
                 }
             }
-        }
-
-        const int32_t si = wave_push(&c->shadow, push_shadow);
-        if(push_shadow)
-        {
-            shadow.o[si] = sh_o;
-            shadow.d[si] = sh_d;
-            shadow.beta[si] = sh_beta;
-            if(shadow.seed)
-                shadow.seed[si] = sh_seed;
-            if(shadow.train)
-                shadow.train[si] = train;
-        }
-        if(nrc.queries)
-        {
-            const int32_t qi = wave_push(nrc.query_count, push_query);
-            if(push_query)
-                nrc.queries[qi] = query;
-        }
-        const int32_t ni = wave_push(&c->next_active, push_next);
-        if(push_next)
-        {
-            next.ray_o[ni] = nx_o;
-            next.ray_d[ni] = nx_d;
-            next.beta[ni] = nx_beta;
-            next.rad[ni] = nx_rad;
-            next.bsdf_le[ni] = nx_le;
-            next.smp[ni] = nx_smp;
         }
     }
 }
@@ -757,7 +732,7 @@ __global__ void __launch_bounds__(TRACE_BLOCK) k_shadow(
     TraceScene ts, SceneView sv, WaveCounters *c, ShadowSOA shadow, FilmView film, TraceOverflow ovf)
 {
     ShadowSink<MEDIA> sink{ shadow, film, sv };
-    trace_persistent<TRI_WATERTIGHT, true, SCHED == SCHED_POSTPONE, SCHED == SCHED_TWO_LEVEL>(
+    trace_persistent<TRI_WATERTIGHT, true, (SCHED & SCHED_POSTPONE) != 0, (SCHED & SCHED_TWO_LEVEL) != 0>(
         ts, sink, c->shadow, &c->shadow_fetch, ovf);
 }
 
@@ -825,7 +800,7 @@ __global__ void __launch_bounds__(TRACE_BLOCK) k_trace_batch(
     TraceScene ts, const float4 *rays, int32_t n, uint32_t *out, uint32_t *fetch, TraceOverflow ovf)
 {
     BatchSink<ANY> sink{ rays, out };
-    trace_persistent<TRI, ANY, SCHED == SCHED_POSTPONE, SCHED == SCHED_TWO_LEVEL>(ts, sink, n, fetch, ovf);
+    trace_persistent<TRI, ANY, (SCHED & SCHED_POSTPONE) != 0, (SCHED & SCHED_TWO_LEVEL) != 0>(ts, sink, n, fetch, ovf);
 }
 
 // ---------------------------------------------------------------- test hooks
@@ -921,7 +896,9 @@ void launch_trace_extension(hipStream_t s, int grid, int sched, const TraceScene
                             PathSOA path, HitSOA hit, const TraceOverflow &ovf)
 {
     const dim3 g(grid), b(TRACE_BLOCK);
-    if(sched == SCHED_TWO_LEVEL)
+    if(sched == SCHED_TWO_LEVEL_POSTPONE)
+        hipLaunchKernelGGL(k_trace_extension<SCHED_TWO_LEVEL_POSTPONE>, g, b, 0, s, ts, c, path, hit, ovf);
+    else if(sched == SCHED_TWO_LEVEL)
         hipLaunchKernelGGL(k_trace_extension<SCHED_TWO_LEVEL>, g, b, 0, s, ts, c, path, hit, ovf);
     else if(sched == SCHED_POSTPONE)
         hipLaunchKernelGGL(k_trace_extension<SCHED_POSTPONE>, g, b, 0, s, ts, c, path, hit, ovf);
@@ -966,7 +943,9 @@ static void launch_shadow_t(hipStream_t s, int grid, bool media, const TraceScen
 void launch_shadow(hipStream_t s, int grid, int sched, bool media, const TraceScene &ts, const SceneView &sv,
                    WaveCounters *c, ShadowSOA shadow, FilmView film, const TraceOverflow &ovf)
 {
-    if(sched == SCHED_TWO_LEVEL)
+    if(sched == SCHED_TWO_LEVEL_POSTPONE)
+        launch_shadow_t<SCHED_TWO_LEVEL_POSTPONE>(s, grid, media, ts, sv, c, shadow, film, ovf);
+    else if(sched == SCHED_TWO_LEVEL)
         launch_shadow_t<SCHED_TWO_LEVEL>(s, grid, media, ts, sv, c, shadow, film, ovf);
     else if(sched == SCHED_POSTPONE)
         launch_shadow_t<SCHED_POSTPONE>(s, grid, media, ts, sv, c, shadow, film, ovf);
@@ -989,7 +968,9 @@ static void launch_batch_t(hipStream_t s, int grid, int sched, const TraceScene 
                            int32_t n, uint32_t *out, uint32_t *fetch, const TraceOverflow &ovf)
 {
     const dim3 g(grid), b(TRACE_BLOCK);
-    if(sched == SCHED_TWO_LEVEL)
+    if(sched == SCHED_TWO_LEVEL_POSTPONE)
+        hipLaunchKernelGGL((k_trace_batch<TRI, ANY, SCHED_TWO_LEVEL_POSTPONE>), g, b, 0, s, ts, rays, n, out, fetch, ovf);
+    else if(sched == SCHED_TWO_LEVEL)
         hipLaunchKernelGGL((k_trace_batch<TRI, ANY, SCHED_TWO_LEVEL>), g, b, 0, s, ts, rays, n, out, fetch, ovf);
     else if(sched == SCHED_POSTPONE)
         hipLaunchKernelGGL((k_trace_batch<TRI, ANY, SCHED_POSTPONE>), g, b, 0, s, ts, rays, n, out, fetch, ovf);

@@ -312,11 +312,13 @@ typedef struct btrc_hit
  * Picked per device by btrc_amd/autotune.py. */
 typedef struct btrc_tuning
 {
-    int32_t closest_postpone_leaves; /* 1: leaves wait until every lane holds one (default 1) */
+    int32_t closest_postpone_leaves; /* one-level scenes, 1: leaves wait until every lane holds one (default 1) */
     int32_t any_postpone_leaves;     /* same for visibility rays (default 0) */
     int32_t trace_waves_per_cu;      /* persistent traversal grid, 1..32 (default 20) */
     int32_t launch_graph;            /* 1: the bounce loop replays a captured HIP graph (default 1) */
-    int32_t reserved[4];
+    int32_t two_level_postpone;      /* two-level scenes: bit 0 closest-hit, bit 1 visibility rays
+                                        postpone leaves (default 0: both test leaves when reached) */
+    int32_t reserved[3];
 } btrc_tuning;
 
 /* ---- context ---- */

@@ -89,3 +89,18 @@ def test_render_per_pixel_parity(ctx):
     rel = np.abs(rgba[..., :3] - ref) / np.maximum(np.abs(ref), 1e-2)
     assert rel.max() <= 1e-4, f"max rel err {rel.max()}"
     assert stats.extension_rays == e and stats.shadow_rays == s
+
+
+def test_schedules_bitwise_equal(ctx):
+    # leaves tested when reached or postponed (inside the current instance)
+    scene = two_level(scenes.sphere_field(instances=768, lights=8, seed=13, extent=30.0)[0])
+    os_ = upload(ctx, scene)
+    rays = random_rays(60000, [0, 0, 0], [30, 30, 30], 47)
+    short = rays.copy()
+    short["tmax"] = np.float32(4.0)
+    want_h, want_o = os_.trace(rays), os_.trace(short, any_hit=True)
+    for bits in range(4):
+        ctx.set_tuning(two_level_postpone=bits)
+        assert_hits_equal(ctx.trace_closest(rays), want_h)
+        assert np.array_equal(ctx.trace_any(short), want_o)
+    ctx.set_tuning(two_level_postpone=0)
