@@ -536,11 +536,7 @@ __global__ void __launch_bounds__(256) k_medium(
     for(int32_t base = blockIdx.x * blockDim.x; base < n; base += stride)
     {
         const int32_t i = base + (int32_t)threadIdx.x;
-        bool push_next = false, push_shadow = false;
-        float4 nx_o, nx_d, nx_beta, nx_le, sh_o, sh_d, sh_beta, rad4;
-        uint4 nx_smp;
-        uint2 sh_seed;
-
+        // queue pushes where their payload is made, as in k_shade
         if(i < n)
         {
             const uint32_t flag = hit.flag[i];
@@ -588,7 +584,7 @@ __global__ void __launch_bounds__(256) k_medium(
                     const float4 beta_depth = path.beta[i];
                     V3 beta = xyz(beta_depth); // medium throughput is one
                     const int32_t depth = __float_as_int(beta_depth.w);
-                    rad4 = path.rad[i];
+                    const float4 rad4 = path.rad[i];
                     const V3 path_rad = xyz(rad4);
 
                     bool rr_exit = false;
@@ -627,51 +623,33 @@ __global__ void __launch_bounds__(256) k_medium(
                         const PhaseSample ps = hg_sample(ms.g, ms.color, -d, s0, s1);
                         if(ss.success)
                         {
-                            push_shadow = true;
                             const uint32_t smed = ss.t1 > 1.0f ? (uint32_t)sv.medium_count : med;
-                            sh_o = f4(ss.o, __uint_as_float(pixel));
-                            sh_d = f4(ss.d, ss.t1);
-                            sh_beta = f4(beta_li, __uint_as_float(smed));
-                            sh_seed = shadow_seed(hs.pack(pixel));
+                            const int32_t si = wave_push(&c->shadow, true);
+                            shadow.o[si] = f4(ss.o, __uint_as_float(pixel));
+                            shadow.d[si] = f4(ss.d, ss.t1);
+                            shadow.beta[si] = f4(beta_li, __uint_as_float(smed));
+                            shadow.seed[si] = shadow_seed(hs.pack(pixel));
+                            if(shadow.train)
+                                shadow.train[si] = __float_as_uint(rad4.w);
                         }
                         if(!is_zero(ps.phase))
                         {
                             const V3 dir = normalize(ps.dir);
                             const V3 beta_le = beta * ps.phase;
                             beta = beta_le / ps.pdf;
-                            push_next = true;
-                            nx_o = f4(ms.position, __uint_as_float(med));
-                            nx_d = f4(dir, MAX_FLOAT);
-                            nx_beta = f4(beta, __int_as_float(depth + 1));
-                            nx_le = f4(beta_le, ps.pdf);
-                            nx_smp = hs.pack(pixel);
+                            const int32_t ni = wave_push(&c->next_active, true);
+                            next.ray_o[ni] = f4(ms.position, __uint_as_float(med));
+                            next.ray_d[ni] = f4(dir, MAX_FLOAT);
+                            next.beta[ni] = f4(beta, __int_as_float(depth + 1));
+                            next.rad[ni] = rad4;
+                            next.bsdf_le[ni] = f4(beta_le, ps.pdf);
+                            next.smp[ni] = hs.pack(pixel);
                         }
                         else
                             film_splat(film, pixel, path_rad);
                     }
                 }
             }
-        }
-
-        const int32_t si = wave_push(&c->shadow, push_shadow);
-        if(push_shadow)
-        {
-            shadow.o[si] = sh_o;
-            shadow.d[si] = sh_d;
-            shadow.beta[si] = sh_beta;
-            shadow.seed[si] = sh_seed;
-            if(shadow.train)
-                shadow.train[si] = __float_as_uint(rad4.w);
-        }
-        const int32_t ni = wave_push(&c->next_active, push_next);
-        if(push_next)
-        {
-            next.ray_o[ni] = nx_o;
-            next.ray_d[ni] = nx_d;
-            next.beta[ni] = nx_beta;
-            next.rad[ni] = rad4;
-            next.bsdf_le[ni] = nx_le;
-            next.smp[ni] = nx_smp;
         }
     }
 }
