@@ -109,12 +109,25 @@ BTRC_HD bool raybox(const RayBoxPre &p, float lox, float hix, float loy, float h
     return te <= tx;
 }
 
-// Box culling distance for the current closest hit: slightly above best_t so
-// that a triangle lying on a box face is never culled by rounding in the box
-// test.
-BTRC_HD float cull_distance(float best_t)
+// Box culling distance for the current closest hit. A box's entry distance
+// and a triangle's t come from different arithmetic: the slab entry carries
+// ~2 ulps of rounding, the watertight t an error of order 2^-24 times the
+// distance from the ray origin to the triangle's vertices (it works on
+// v - o), which for a ray starting close to a large triangle is many ulps of
+// t (11 ulps were seen for a hit at t = 0.037 on a 2-unit floor). Culling
+// therefore keeps every box whose entry lies within best_t (1 + 2^-20) +
+// margin, with margin = 2^-18 (scene extent + |o|) from cull_margin: a hit
+// no farther than the current one (a tie included) is never culled, so the
+// (t, index) minimum does not depend on the visit order.
+BTRC_HD float cull_margin(float scene_extent, V3 o)
 {
-    return best_t < MAX_FLOAT ? best_t * 1.00000095f : best_t;
+    const float m = fmaxf(fabs_(o.x), fmaxf(fabs_(o.y), fabs_(o.z)));
+    return (scene_extent + m) * (1.0f / 262144.0f);
+}
+
+BTRC_HD float cull_distance(float best_t, float margin)
+{
+    return best_t < MAX_FLOAT ? best_t * 1.00000095f + margin : best_t;
 }
 
 // ---- watertight test ----

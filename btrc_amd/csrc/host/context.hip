@@ -274,7 +274,8 @@ struct btrc_ctx
     DevBuf<uint32_t> batch_out;
     int64_t batch_n = 0;
 
-    TraceScene trace_scene() const { return TraceScene{ nodes.ptr, tris.ptr, trace_inst.ptr }; }
+    float scene_extent = 0.0f; // largest |coordinate| of the world-space triangles
+    TraceScene trace_scene() const { return TraceScene{ nodes.ptr, tris.ptr, trace_inst.ptr, scene_extent }; }
     // traversal schedule of closest-hit (any = false) or any-hit rays
     int trace_sched(bool any) const
     {
@@ -1006,6 +1007,10 @@ void upload_scene(btrc_ctx *ctx, const btrc_scene_desc *sd)
     ctx->texture_count = sd->texture_count;
 
     ctx->medium_count = R;
+    ctx->scene_extent = 0.0f;
+    for(int a = 0; a < 3; ++a)
+        if(acc.lo[a] <= acc.hi[a])
+            ctx->scene_extent = std::max(ctx->scene_extent, std::max(std::fabs(acc.lo[a]), std::fabs(acc.hi[a])));
     for(int a = 0; a < 3; ++a)
     {
         float lo = acc.lo[a], hi = acc.hi[a];

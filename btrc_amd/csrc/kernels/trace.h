@@ -67,6 +67,7 @@ struct TraceScene
     // two levels (local space): (v0.xyz, prim in geometry), (v1.xyz, 0), (v2.xyz, 0)
     const float4 *tris;
     const TraceInstance *inst; // two levels only
+    float extent;              // largest world coordinate magnitude of the triangles
 };
 
 // per-lane global stack entries below the LDS part: [entry - SHORT_STACK][lane]
@@ -121,6 +122,7 @@ __device__ __forceinline__ void trace_persistent(
     uint32_t ray = 0;
     V3 o = v3s(0.0f), d = v3s(0.0f);
     float tmin = 0.0f;
+    float margin = 0.0f; // culling margin of the ray (cull_margin)
     RayBoxPre rb;
     uint32_t near_x = 0, near_y = 32, near_z = 64; // byte offsets of the near planes in a node
     WatertightPre wt;
@@ -159,7 +161,7 @@ __device__ __forceinline__ void trace_persistent(
     // next entry whose subtree may still hold a closer hit, or CODE_DONE;
     // stay_inside: stop before leaving the current instance
     auto pop = [&](bool stay_inside) -> uint32_t {
-        const float cull = cull_distance(h.t);
+        const float cull = cull_distance(h.t, margin);
         while(sp > 0)
         {
             --sp;
@@ -248,6 +250,7 @@ __device__ __forceinline__ void trace_persistent(
                     float tmax;
                     sink.load(idx, o, d, tmin, tmax);
                     set_ray(o, d);
+                    margin = cull_margin(sc.extent, o);
                     cur_inst = 0xffffffffu;
                     h.t = tmax;
                     h.b1 = h.b2 = 0.0f;
@@ -320,7 +323,7 @@ __device__ __forceinline__ void trace_persistent(
                     const float bnz[4] = { nzv.x, nzv.y, nzv.z, nzv.w };
                     const float bfz[4] = { fzv.x, fzv.y, fzv.z, fzv.w };
                     const uint32_t cc[4] = { code.x, code.y, code.z, code.w };
-                    const float cull = cull_distance(h.t);
+                    const float cull = cull_distance(h.t, margin);
                     float k[4];
                     uint32_t c[4];
 #pragma unroll

@@ -506,6 +506,7 @@ struct Scene
     // two levels: local triangles per geometry (off + prim), and per
     // instance its first flattened index and padded world box
     bool two_level = false;
+    float extent = 0.0f; // largest |coordinate| of the world-space triangles
     std::vector<Vec> l0, l1, l2;
     std::vector<uint32_t> ibase;
     std::vector<float> ibox; // 6 per instance
@@ -787,6 +788,9 @@ static Scene *make_scene(const btrc_scene_desc *sd)
         s->lights.push_back(l);
     }
     build_bvh(*s);
+    for(const std::vector<Vec> *w : { &s->w0, &s->w1, &s->w2 })
+        for(const Vec &v : *w)
+            s->extent = std::max(s->extent, std::max(std::fabs(v.x), std::max(std::fabs(v.y), std::fabs(v.z))));
     s->two_level = sd->accel == BTRC_ACCEL_TWO_LEVEL;
     if(s->two_level)
     {
@@ -916,6 +920,18 @@ static bool box_hit(const Scene::Node &nd, Vec o, Vec inv, Vec invp, float tmin,
 }
 
 
+// Box culling keeps every box entered within t (1 + 2^-20) + margin of the
+// current hit: the slab entry and the triangle's t are rounded differently
+// (the watertight t by up to ~2^-24 times the origin-to-vertex distance),
+// and without the scene-scaled margin a triangle at the same t as the
+// current hit can be culled, making the (t, index) minimum depend on visit
+// order.
+static inline float cull_margin(const Scene &s, Vec o)
+{
+    return (s.extent + fmx(std::fabs(o.x), fmx(std::fabs(o.y), std::fabs(o.z)))) * (1.0f / 262144.0f);
+}
+static inline float cull_dist(float t, float margin) { return t < FMAX ? t * 1.00000095f + margin : t; }
+
 // world_to_local rows applied to a point / vector, in the operation order of
 // the traversal kernel
 static inline Vec xinv_point(const Xf &t, Vec p)
@@ -952,7 +968,7 @@ static Hit trace_two_level(const Scene &s, int mode, bool any, bool use_bvh, Vec
                 nd.lo[a] = s.ibox[6 * i + a];
                 nd.hi[a] = s.ibox[6 * i + 3 + a];
             }
-            const float cull = h.t < FMAX ? h.t * 1.00000095f : h.t;
+            const float cull = cull_dist(h.t, cull_margin(s, o));
             if(!g.n || !box_hit(nd, o, inv, invp, tmin, cull))
                 continue;
         }
@@ -995,7 +1011,7 @@ static Hit trace(const Scene &s, int mode, bool any, bool use_bvh, Vec o, Vec d,
     {
         const Scene::Node &nd = s.nodes[st.back()];
         st.pop_back();
-        const float cull = h.t < FMAX ? h.t * 1.00000095f : h.t;
+        const float cull = cull_dist(h.t, cull_margin(s, o));
         if(!box_hit(nd, o, inv, invp, tmin, cull))
             continue;
         if(nd.left < 0)

@@ -17,7 +17,7 @@ def test_tuning_roundtrip_and_validation(ctx):
     assert ctx.get_tuning()["trace_waves_per_cu"] == 7
     with pytest.raises(Exception):
         ctx.set_tuning(trace_waves_per_cu=0)
-    ctx.set_tuning(closest_postpone_leaves=1, any_postpone_leaves=0, trace_waves_per_cu=24)
+    ctx.set_tuning(closest_postpone_leaves=1, any_postpone_leaves=0, trace_waves_per_cu=20)
 
 
 def test_traversal_variants_bitwise_equal(ctx):
@@ -39,7 +39,7 @@ def test_traversal_variants_bitwise_equal(ctx):
             else:
                 assert_hits_equal(h, ref_h[mode])
                 assert np.array_equal(o, ref_o[mode])
-    ctx.set_tuning(closest_postpone_leaves=1, any_postpone_leaves=0, trace_waves_per_cu=24)
+    ctx.set_tuning(closest_postpone_leaves=1, any_postpone_leaves=0, trace_waves_per_cu=20)
 
 
 def test_render_variants_equal(ctx):
@@ -55,7 +55,7 @@ def test_render_variants_equal(ctx):
     for im in imgs[1:]:
         # only the FP32 atomic accumulation order may differ
         np.testing.assert_allclose(im, imgs[0], rtol=1e-5, atol=1e-6)
-    ctx.set_tuning(closest_postpone_leaves=1, any_postpone_leaves=0, trace_waves_per_cu=24)
+    ctx.set_tuning(closest_postpone_leaves=1, any_postpone_leaves=0, trace_waves_per_cu=20)
 
 
 def test_graph_loop_equals_direct_launches(ctx):
@@ -78,3 +78,21 @@ def test_graph_loop_equals_direct_launches(ctx):
         assert ea == eb and sa == sb
         assert np.abs(a - b).max() <= 1e-12 * max(1.0, np.abs(b).max())
     ctx.set_tuning(launch_graph=1)
+
+
+def test_coplanar_hits_every_schedule(ctx):
+    # the floor and coplanar tiles: box culling must keep equal-distance
+    # candidates whatever the visit order, in both structures
+    from oracle import oracle
+    from test_oracle_traversal import coplanar_rays, coplanar_scene
+    rays = coplanar_rays(50000, 5)
+    for accel in (_abi.ACCEL_FLAT, _abi.ACCEL_TWO_LEVEL):
+        scene = coplanar_scene()
+        scene.accel = accel
+        ctx.upload_scene(scene)
+        want = oracle.OracleScene(scene).trace(rays)
+        for c in (0, 1):
+            for w in (1, 20):
+                ctx.set_tuning(closest_postpone_leaves=c, two_level_postpone=c, trace_waves_per_cu=w)
+                assert_hits_equal(ctx.trace_closest(rays), want)
+    ctx.set_tuning(closest_postpone_leaves=1, two_level_postpone=0, trace_waves_per_cu=20)

@@ -124,3 +124,46 @@ def test_mt_matches_lib_bvh_live():
     prim, t, u, v, _, _ = oracle.RefBvh(tris).trace(rays)
     hits = oracle.OracleScene(single_mesh_scene(tris)).trace(rays, tri_test=_abi.TRI_MOLLER_TRUMBORE)
     _golden_compare(hits, dict(prim=prim, t=t, u=u, v=v), tris)
+
+
+def coplanar_scene():
+    """a 2x2 floor at y = 0 (triangles 0, 1) under 64 coplanar tiles: every
+    hit is a tie between the floor and a tile, decided by the lowest index"""
+    s = scenes.Scene()
+    m = s.add_diffuse([0.5, 0.5, 0.5])
+    s.add_instance(s.add_geometry(scenes.quad((-1, 0, 1), (1, 0, 1), (1, 0, -1), (-1, 0, -1))), m)
+    tiles = []
+    for i in range(8):
+        for j in range(8):
+            x0, z0 = -1 + 0.25 * i, -1 + 0.25 * j
+            tiles.append(scenes.quad((x0, 0, z0 + 0.25), (x0 + 0.25, 0, z0 + 0.25), (x0 + 0.25, 0, z0), (x0, 0, z0)))
+    s.add_instance(s.add_geometry(np.concatenate(tiles)), m)
+    return s
+
+
+def coplanar_rays(n, seed):
+    # origins just above the plane, directions downward at all angles: hits at
+    # small t, where the triangle's t and a box's entry differ by many ulps
+    rng = np.random.default_rng(seed)
+    rays = np.zeros(n, dtype=_abi.ray_dtype())
+    rays["o"][:, 0] = rng.uniform(-0.9, 0.9, n)
+    rays["o"][:, 1] = 10.0 ** rng.uniform(-3, -1, n)
+    rays["o"][:, 2] = rng.uniform(-0.9, 0.9, n)
+    d = rng.normal(size=(n, 3))
+    d[:, 1] = -np.abs(d[:, 1])
+    rays["d"] = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
+    rays["tmax"] = np.float32(3.4028234663852886e38)
+    return rays
+
+
+def test_coplanar_hits_do_not_depend_on_the_bvh():
+    # the floor and a tile give equal or 1-ulp-apart t; the BVH's box culling
+    # must keep both candidates, so the result is the brute-force minimum
+    os_ = oracle.OracleScene(coplanar_scene())
+    rays = coplanar_rays(20000, 3)
+    hits = os_.trace(rays)
+    inside = (np.abs(rays["o"][:, 0] + hits["t"] * rays["d"][:, 0]) < 1) & \
+             (np.abs(rays["o"][:, 2] + hits["t"] * rays["d"][:, 2]) < 1)
+    assert inside.mean() > 0.5
+    assert set(np.unique(hits["instance"][inside])) == {0, 1}
+    assert hits.tobytes() == os_.trace(rays, use_bvh=False).tobytes()
