@@ -276,6 +276,15 @@ class Context:
                                                        len(px), dims, out.ctypes.data))
         return out
 
+    def eval_volumes(self, segments, seeds):
+        """(n, 10) float32 of the volume-primitive medium: scattered, position, albedo, transmittance"""
+        seg = np.ascontiguousarray(segments, dtype=_abi.ray_dtype())
+        seeds = np.ascontiguousarray(seeds, dtype=np.uint64)
+        out = np.empty((len(seg), 10), dtype=np.float32)
+        self._check(self.lib.btrc_hip_eval_volumes(self.h, seg.ctypes.data, seeds.ctypes.data, len(seg),
+                                                   out.ctypes.data))
+        return out
+
     def eval_texture(self, texture, uv):
         """(n, 4) float32: texture `texture` of the current scene at (n, 2) uv"""
         uv = np.ascontiguousarray(uv, dtype=np.float32).reshape(-1, 2)

@@ -76,12 +76,19 @@ class EnvirLightDesc(C.Structure):
     _fields_ = [("type", C.c_int32), ("up", F3), ("lower", F3), ("upper", F3)]
 
 
+class VolumeDesc(C.Structure):
+    _fields_ = [("o", F3), ("x", F3), ("y", F3), ("z", F3), ("sigma_t_size", C.c_int32 * 3),
+                ("sigma_t_grid", C.POINTER(C.c_float)), ("sigma_t", C.c_float), ("albedo_size", C.c_int32 * 3),
+                ("albedo_grid", C.POINTER(C.c_float)), ("albedo", F3)]
+
+
 class SceneDesc(C.Structure):
     _fields_ = [("geometry_count", C.c_int32), ("geometries", C.POINTER(GeometryDesc)),
                 ("material_count", C.c_int32), ("materials", C.POINTER(MaterialDesc)),
                 ("instance_count", C.c_int32), ("instances", C.POINTER(InstanceDesc)),
                 ("envir_light", EnvirLightDesc), ("medium_count", C.c_int32), ("media", C.POINTER(MediumDesc)),
-                ("accel", C.c_int32), ("texture_count", C.c_int32), ("textures", C.POINTER(TextureDesc))]
+                ("accel", C.c_int32), ("texture_count", C.c_int32), ("textures", C.POINTER(TextureDesc)),
+                ("volume_count", C.c_int32), ("volumes", C.POINTER(VolumeDesc))]
 
 
 class CameraDesc(C.Structure):
@@ -164,7 +171,7 @@ EXPORTS = [
     "btrc_hip_nrc_set_weights", "btrc_hip_nrc_eval", "btrc_hip_nrc_gradients", "btrc_hip_nrc_fit",
     "btrc_hip_trace_closest", "btrc_hip_trace_any",
     "btrc_hip_batch_upload", "btrc_hip_batch_trace", "btrc_hip_batch_download",
-    "btrc_hip_capture_rays", "btrc_hip_eval_halton", "btrc_hip_eval_independent", "btrc_hip_eval_texture",
+    "btrc_hip_capture_rays", "btrc_hip_eval_halton", "btrc_hip_eval_independent", "btrc_hip_eval_volumes", "btrc_hip_eval_texture",
     "btrc_hip_eval_medium",
     "btrc_hip_eval_detmath",
 ]
@@ -209,6 +216,7 @@ def declare(lib):
                                            C.c_int32, C.c_int32, C.c_void_p]),
         "btrc_hip_eval_independent": (C.c_int, [ctx, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_int32,
                                                 C.c_void_p]),
+        "btrc_hip_eval_volumes": (C.c_int, [ctx, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p]),
         "btrc_hip_eval_texture": (C.c_int, [ctx, C.c_int32, C.c_void_p, C.c_int32, C.c_void_p]),
         "btrc_hip_eval_medium": (C.c_int, [ctx, C.c_int32, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p]),
         "btrc_hip_eval_detmath": (C.c_int, [ctx, C.c_int32, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p]),

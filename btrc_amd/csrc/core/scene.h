@@ -9,6 +9,7 @@
 #include "material.h"
 #include "medium.h"
 #include "texture.h"
+#include "volume.h"
 
 #if !defined(__HIPCC__)
 struct float4 { float x, y, z, w; };
@@ -92,6 +93,8 @@ struct SceneView
     const MediumRec *media;
     const float *voxels;
     TextureView tex;
+    VolumeView vol;           // the volume-primitive medium (id medium_count)
+    float world_diagonal;     // 1.2 |bbox of the scene and the cameras| (wavefront.cpp:138-139)
 };
 
 struct SurfacePoint

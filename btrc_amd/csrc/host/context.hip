@@ -184,6 +184,9 @@ struct btrc_ctx
     DevBuf<AliasUnit> alias;
     DevBuf<MediumRec> media;
     DevBuf<float> voxels;
+    DevBuf<VolumeRec> volumes;
+    int32_t volume_count = 0;
+    float world_lo[3] = { 0, 0, 0 }, world_hi[3] = { 0, 0, 0 }; // triangles and volume primitives
     DevBuf<TextureRec> textures;
     DevBuf<uint32_t> texels8;
     DevBuf<float4> texels32;
@@ -307,6 +310,8 @@ struct btrc_ctx
         v.media = media.ptr;
         v.voxels = voxels.ptr;
         v.tex = TextureView{ textures.ptr, texels8.ptr, texels32.ptr, tex_decode.ptr };
+        v.vol = VolumeView{ volumes.ptr, volume_count, voxels.ptr };
+        v.world_diagonal = 0.0f; // set per render from the cameras
         return v;
     }
     int persistent_grid(int threads_per_block, int64_t work) const
@@ -895,6 +900,77 @@ void upload_scene(btrc_ctx *ctx, const btrc_scene_desc *sd)
         media[rank] = m;
         medium_id[i] = (uint32_t)rank;
     }
+    // volume primitives (core/volume.cpp:21-79): grids appended to the voxels
+    if(sd->volume_count < 0 || (sd->volume_count > 0 && !sd->volumes))
+        throw InvalidArg("bad volume list");
+    std::vector<VolumeRec> vols(sd->volume_count);
+    for(int32_t i = 0; i < sd->volume_count; ++i)
+    {
+        const btrc_volume_desc &vd = sd->volumes[i];
+        const std::string name = "volume " + std::to_string(i);
+        VolumeRec v{};
+        const float *e[3] = { vd.x, vd.y, vd.z };
+        float *xd[3] = { v.xd, v.yd, v.zd };
+        float *dst[3] = { v.x, v.y, v.z };
+        float *inv[3] = { &v.inv_x2, &v.inv_y2, &v.inv_z2 };
+        for(int k = 0; k < 3; ++k)
+        {
+            v.o[k] = vd.o[k];
+            const float l2 = e[k][0] * e[k][0] + e[k][1] * e[k][1] + e[k][2] * e[k][2];
+            if(!(l2 > 0.0f) || !std::isfinite(l2) || !std::isfinite(vd.o[k]))
+                throw InvalidArg(name + ": edges must be finite and non-zero");
+            *inv[k] = 1.0f / l2;
+            for(int c = 0; c < 3; ++c)
+            {
+                dst[k][c] = e[k][c];
+                xd[k][c] = e[k][c] / l2;
+            }
+        }
+        if(vd.sigma_t_size[0] > 0)
+        {
+            const int32_t nx = vd.sigma_t_size[0], ny = vd.sigma_t_size[1], nz = vd.sigma_t_size[2];
+            if(ny <= 0 || nz <= 0 || !vd.sigma_t_grid)
+                throw InvalidArg(name + ": bad density grid");
+            const size_t nv = (size_t)nx * ny * nz;
+            if(voxels.size() + nv > 0xffffffffull)
+                throw InvalidArg(name + ": voxel grids too large");
+            v.nx = nx; v.ny = ny; v.nz = nz;
+            v.density_offset = (uint32_t)voxels.size();
+            float mx = vd.sigma_t_grid[0];
+            for(size_t k = 0; k < nv; ++k)
+            {
+                const float x = vd.sigma_t_grid[k];
+                if(!std::isfinite(x) || x < 0.0f)
+                    throw InvalidArg(name + ": densities must be finite and non-negative");
+                mx = std::max(mx, x);
+                voxels.push_back(x);
+            }
+            v.max_density = mx;
+        }
+        else
+        {
+            if(!std::isfinite(vd.sigma_t) || vd.sigma_t < 0.0f)
+                throw InvalidArg(name + ": sigma_t must be finite and non-negative");
+            v.sigma_t = vd.sigma_t;
+            v.max_density = vd.sigma_t;
+        }
+        for(int k = 0; k < 3; ++k)
+            v.albedo[k] = vd.albedo[k];
+        if(vd.albedo_size[0] > 0)
+        {
+            const int32_t ax = vd.albedo_size[0], ay = vd.albedo_size[1], az = vd.albedo_size[2];
+            if(ay <= 0 || az <= 0 || !vd.albedo_grid)
+                throw InvalidArg(name + ": bad albedo grid");
+            const size_t nv = 3 * (size_t)ax * ay * az;
+            if(voxels.size() + nv > 0xffffffffull)
+                throw InvalidArg(name + ": voxel grids too large");
+            v.ax = ax; v.ay = ay; v.az = az;
+            v.albedo_offset = (uint32_t)voxels.size();
+            voxels.insert(voxels.end(), vd.albedo_grid, vd.albedo_grid + nv);
+        }
+        vols[i] = v;
+    }
+
     auto resolve_medium = [&](int32_t idx, int32_t inst) -> uint32_t {
         if(idx < 0)
             return (uint32_t)R + 1;
@@ -999,6 +1075,7 @@ void upload_scene(btrc_ctx *ctx, const btrc_scene_desc *sd)
     ctx->alias.upload(alias_units.data(), alias_units.size(), s);
     ctx->media.upload(media.data(), media.size(), s);
     ctx->voxels.upload(voxels.data(), voxels.size(), s);
+    ctx->volumes.upload(vols.data(), vols.size(), s);
     ctx->textures.upload(texs.data(), texs.size(), s);
     ctx->texels8.upload(tex8.data(), tex8.size(), s);
     ctx->texels32.upload(tex32.data(), tex32.size(), s);
@@ -1007,6 +1084,22 @@ void upload_scene(btrc_ctx *ctx, const btrc_scene_desc *sd)
     ctx->texture_count = sd->texture_count;
 
     ctx->medium_count = R;
+    ctx->volume_count = sd->volume_count;
+    // Scene::commit's bounding box (core/scene.cpp:225-237): triangles and
+    // each volume's o, o + x, o + y, o + z (VolumePrimitive::get_bounding_box)
+    for(int a = 0; a < 3; ++a)
+    {
+        ctx->world_lo[a] = acc.lo[a];
+        ctx->world_hi[a] = acc.hi[a];
+    }
+    for(const VolumeRec &v : vols)
+        for(int c = 0; c < 4; ++c)
+            for(int a = 0; a < 3; ++a)
+            {
+                const float p = c == 0 ? v.o[a] : v.o[a] + (c == 1 ? v.x[a] : c == 2 ? v.y[a] : v.z[a]);
+                ctx->world_lo[a] = std::min(ctx->world_lo[a], p);
+                ctx->world_hi[a] = std::max(ctx->world_hi[a], p);
+            }
     ctx->scene_extent = 0.0f;
     for(int a = 0; a < 3; ++a)
         if(acc.lo[a] <= acc.hi[a])
@@ -1336,7 +1429,7 @@ void render(btrc_ctx *ctx, const btrc_render_params *p, int32_t view_count, cons
     {
         if(p->spp != 1)
             throw InvalidArg("the radiance cache renders one sample per pixel per frame");
-        if(ctx->medium_count > 0)
+        if(ctx->medium_count > 0 || ctx->volume_count > 0)
             throw InvalidArg("the radiance cache does not support participating media");
         if(p->max_depth + 1 > NRC_RECORDS)
             throw InvalidArg("max_depth too large for the radiance cache");
@@ -1356,7 +1449,7 @@ void render(btrc_ctx *ctx, const btrc_render_params *p, int32_t view_count, cons
         ctx->sh_beta.alloc(P);
         ctx->pool_size = P;
     }
-    const bool media = ctx->medium_count > 0;
+    const bool media = ctx->medium_count > 0 || ctx->volume_count > 0;
     if(media)
         ctx->sh_seed.alloc(P);
     ctx->counters.alloc(1);
@@ -1430,7 +1523,19 @@ void render(btrc_ctx *ctx, const btrc_render_params *p, int32_t view_count, cons
     rc.filter = ctx->filter;
 
     const TraceScene ts = ctx->trace_scene();
-    const SceneView sv = ctx->scene_view();
+    SceneView sv = ctx->scene_view();
+    {
+        // 1.2 |bbox of the scene and the cameras' eyes| (wavefront.cpp:138-139)
+        V3 lo = v3(ctx->world_lo[0], ctx->world_lo[1], ctx->world_lo[2]);
+        V3 hi = v3(ctx->world_hi[0], ctx->world_hi[1], ctx->world_hi[2]);
+        for(const ViewRec &v : views)
+        {
+            const V3 e = v3(v.camera.eye[0], v.camera.eye[1], v.camera.eye[2]);
+            lo = v3(std::min(lo.x, e.x), std::min(lo.y, e.y), std::min(lo.z, e.z));
+            hi = v3(std::max(hi.x, e.x), std::max(hi.y, e.y), std::max(hi.z, e.z));
+        }
+        sv.world_diagonal = 1.2f * length(hi - lo);
+    }
     FilmView fv{ ctx->film.ptr, p->film_fp64 ? ctx->film64.ptr : nullptr, p->albedo ? ctx->film_albedo.ptr : nullptr,
                  p->normal ? ctx->film_normal.ptr : nullptr };
     HitSOA hit{ ctx->hit_flag.ptr, ctx->hit_tuv.ptr };
@@ -2135,6 +2240,28 @@ int btrc_hip_eval_independent(btrc_ctx *ctx, const uint32_t *px, const uint32_t 
         launch_independent_eval(ctx->stream, dx.ptr, dy.ptr, ds.ptr, n, dims, dout.ptr);
         BTRC_HIP_CHECK(hipGetLastError());
         BTRC_HIP_CHECK(hipMemcpyAsync(out, dout.ptr, (size_t)n * dims * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+        BTRC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int btrc_hip_eval_volumes(btrc_ctx *ctx, const btrc_ray *segments, const uint64_t *seeds, int32_t n, float *out)
+{
+    return guarded(ctx, [&] {
+        if(!ctx->has_scene)
+            throw StateError("volume query before btrc_hip_upload_scene");
+        if(n < 0 || (n > 0 && (!segments || !seeds || !out)))
+            throw InvalidArg("bad volume query");
+        if(n == 0)
+            return;
+        DevBuf<float4> dseg;
+        DevBuf<uint2> dseed;
+        DevBuf<float> dout;
+        dseg.upload(reinterpret_cast<const float4 *>(segments), 2 * (size_t)n, ctx->stream);
+        dseed.upload(reinterpret_cast<const uint2 *>(seeds), n, ctx->stream);
+        dout.alloc(10 * (size_t)n);
+        launch_volume_eval(ctx->stream, ctx->scene_view().vol, dseg.ptr, dseed.ptr, n, dout.ptr);
+        BTRC_HIP_CHECK(hipGetLastError());
+        BTRC_HIP_CHECK(hipMemcpyAsync(out, dout.ptr, 10 * (size_t)n * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
         BTRC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
     });
 }

@@ -185,6 +185,8 @@ void launch_trace_batch(hipStream_t s, int grid, int sched, const TraceScene &ts
 void launch_halton_eval(hipStream_t s, const HaltonConsts &hc, const uint32_t *px, const uint32_t *py,
                         const int32_t *sample_index, int32_t n, int32_t dims, float *out);
 void launch_texture_eval(hipStream_t s, const TextureView &tv, int32_t id, const float2 *uv, int32_t n, float4 *out);
+void launch_volume_eval(hipStream_t s, const VolumeView &vv, const float4 *seg, const uint2 *seed, int32_t n,
+                        float *out);
 void launch_medium_eval(hipStream_t s, const SceneView &sv, int32_t med, const float4 *seg, const uint2 *seed,
                         int32_t n, float *out);
 void launch_independent_eval(hipStream_t s, const uint32_t *px, const uint32_t *py, const int32_t *sample_index,

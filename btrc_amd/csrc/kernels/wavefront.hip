@@ -559,20 +559,31 @@ __global__ void __launch_bounds__(256) k_medium(
             }
             else
             {
-                // the volume-primitive medium: without volume primitives a vacuum
+                // the volume-primitive medium up to the world's diagonal
                 med = (uint32_t)sv.medium_count;
-                end = o;
+                end = o + normalize(d) * sv.world_diagonal;
             }
             path.ray_o[i] = f4(o, __uint_as_float(med));
 
-            if(med < (uint32_t)sv.medium_count)
+            const bool vol_prim = med == (uint32_t)sv.medium_count && sv.vol.count > 0;
+            if(med < (uint32_t)sv.medium_count || vol_prim)
             {
-                const MediumRec &m = sv.media[med];
                 const uint4 smp = path.smp[i];
                 const uint32_t pixel = smp.x;
                 PathSampler<SAMPLER> hs = PathSampler<SAMPLER>::load(rc, smp);
-                const MediumSample ms = medium_sample(m, sv.voxels, o, end, hs);
-                const V3 tr = medium_tr(m, sv.voxels, o, end, hs);
+                MediumSample ms;
+                V3 tr;
+                if(vol_prim)
+                {
+                    ms = volume_sample(sv.vol, o, end, hs);
+                    tr = volume_tr(sv.vol, o, end, hs);
+                }
+                else
+                {
+                    const MediumRec &m = sv.media[med];
+                    ms = medium_sample(m, sv.voxels, o, end, hs);
+                    tr = medium_tr(m, sv.voxels, o, end, hs);
+                }
                 const float4 le4 = path.bsdf_le[i];
                 path.bsdf_le[i] = f4(xyz(le4) * tr, le4.w);
 
@@ -684,14 +695,27 @@ struct ShadowSink
             if(MEDIA)
             {
                 // rays to the environment (t1 > 1) cross the volume-primitive
-                // medium, a vacuum here
+                // medium over the world's diagonal (shadow.cpp:56-66)
                 const float4 dd = shadow.d[i];
                 const uint32_t med = __float_as_uint(b.w);
-                if(!(dd.w > 1.0f) && med < (uint32_t)sv.medium_count)
+                const V3 o = xyz(a);
+                if(dd.w > 1.0f)
+                {
+                    if(sv.vol.count > 0)
+                    {
+                        TrSampler ts = TrSampler::start(shadow.seed[i]);
+                        beta = beta * volume_tr(sv.vol, o, o + normalize(xyz(dd)) * sv.world_diagonal, ts);
+                    }
+                }
+                else if(med < (uint32_t)sv.medium_count)
                 {
                     TrSampler ts = TrSampler::start(shadow.seed[i]);
-                    const V3 o = xyz(a);
                     beta = beta * medium_tr(sv.media[med], sv.voxels, o, o + xyz(dd) * dd.w, ts);
+                }
+                else if(med == (uint32_t)sv.medium_count && sv.vol.count > 0)
+                {
+                    TrSampler ts = TrSampler::start(shadow.seed[i]);
+                    beta = beta * volume_tr(sv.vol, o, o + xyz(dd) * dd.w, ts);
                 }
             }
             film_splat(film, __float_as_uint(a.w), beta);
@@ -785,6 +809,23 @@ __global__ void __launch_bounds__(TRACE_BLOCK) k_trace_batch(
 
 // one medium along segments a -> a + d * tmax with one PCG32 stream per
 // segment: delta tracking, then ratio tracking on the same stream
+__global__ void k_volume_eval(VolumeView vv, const float4 *seg, const uint2 *seed, int32_t n, float *out)
+{
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if(i >= n)
+        return;
+    const float4 a4 = seg[2 * i], d4 = seg[2 * i + 1];
+    const V3 a = xyz(a4), b = xyz(a4) + xyz(d4) * d4.w;
+    TrSampler smp = TrSampler::start(seed[i]);
+    const MediumSample ms = volume_sample(vv, a, b, smp);
+    const V3 tr = volume_tr(vv, a, b, smp);
+    float *o = out + 10 * (size_t)i;
+    o[0] = ms.scattered ? 1.0f : 0.0f;
+    o[1] = ms.position.x; o[2] = ms.position.y; o[3] = ms.position.z;
+    o[4] = ms.color.x; o[5] = ms.color.y; o[6] = ms.color.z;
+    o[7] = tr.x; o[8] = tr.y; o[9] = tr.z;
+}
+
 __global__ void k_texture_eval(TextureView tv, int32_t id, const float2 *uv, int32_t n, float4 *out)
 {
     const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -979,6 +1020,12 @@ void launch_halton_eval(hipStream_t s, const HaltonConsts &hc, const uint32_t *p
                         const int32_t *sample_index, int32_t n, int32_t dims, float *out)
 {
     hipLaunchKernelGGL(k_halton_eval, dim3((n + 127) / 128), dim3(128), 0, s, hc, px, py, sample_index, n, dims, out);
+}
+
+void launch_volume_eval(hipStream_t s, const VolumeView &vv, const float4 *seg, const uint2 *seed, int32_t n,
+                        float *out)
+{
+    hipLaunchKernelGGL(k_volume_eval, dim3((n + 127) / 128), dim3(128), 0, s, vv, seg, seed, n, out);
 }
 
 void launch_texture_eval(hipStream_t s, const TextureView &tv, int32_t id, const float2 *uv, int32_t n, float4 *out)

@@ -102,6 +102,7 @@ class Scene:
         self.envir = None      # (up, lower, upper)
         self.accel = _abi.ACCEL_FLAT
         self.textures = []     # dicts
+        self.volumes = []      # dicts
 
     def add_geometry(self, positions, normals=None, uvs=None):
         positions = np.ascontiguousarray(positions, dtype=np.float32).reshape(-1, 9)
@@ -173,6 +174,22 @@ class Scene:
                                xf=None if world_to_local is None else np.asarray(world_to_local, dtype=np.float64),
                                albedo_grid=ag))
         return len(self.media) - 1
+
+    def add_volume(self, o, x, y, z, sigma_t, albedo):
+        """VolumePrimitive (core/volume.cpp:21-79): the box o + u x + v y + w z;
+        sigma_t a float or a (D, H, W) grid, albedo an RGB triple or a
+        (D, H, W, 3) grid; all volumes form the volume-primitive medium"""
+        v = dict(o=tuple(map(float, o)), x=tuple(map(float, x)), y=tuple(map(float, y)), z=tuple(map(float, z)))
+        if np.ndim(sigma_t) == 0:
+            v.update(sigma_t=float(sigma_t), sigma_grid=None)
+        else:
+            v.update(sigma_t=0.0, sigma_grid=np.ascontiguousarray(sigma_t, dtype=np.float32))
+        if np.ndim(albedo) == 1:
+            v.update(albedo=tuple(map(float, albedo)), albedo_grid=None)
+        else:
+            v.update(albedo=(0.0, 0.0, 0.0), albedo_grid=np.ascontiguousarray(albedo, dtype=np.float32))
+        self.volumes.append(v)
+        return len(self.volumes) - 1
 
     def add_instance(self, geometry, material, local_to_world=None, light=None, inner_medium=-1, outer_medium=-1):
         m = np.eye(4) if local_to_world is None else np.asarray(local_to_world, dtype=np.float64)
@@ -277,6 +294,22 @@ class Scene:
             td[k].border = (C.c_float * 4)(*map(float, t["border"]))
         sd.texture_count = len(self.textures)
         sd.textures = td
+        vd = (_abi.VolumeDesc * max(1, len(self.volumes)))()
+        for k, v in enumerate(self.volumes):
+            d = vd[k]
+            d.o, d.x, d.y, d.z = (_abi.F3(*v[key]) for key in ("o", "x", "y", "z"))
+            d.sigma_t = v["sigma_t"]
+            if v["sigma_grid"] is not None:
+                nz, ny, nx = v["sigma_grid"].shape
+                d.sigma_t_size = (C.c_int32 * 3)(nx, ny, nz)
+                d.sigma_t_grid = v["sigma_grid"].ctypes.data_as(fp)
+            d.albedo = _abi.F3(*v["albedo"])
+            if v["albedo_grid"] is not None:
+                nz, ny, nx, _ = v["albedo_grid"].shape
+                d.albedo_size = (C.c_int32 * 3)(nx, ny, nz)
+                d.albedo_grid = v["albedo_grid"].ctypes.data_as(fp)
+        sd.volume_count = len(self.volumes)
+        sd.volumes = vd
         if self.envir is not None:
             up, lower, upper = self.envir
             sd.envir_light.type = _abi.ENVIR_GRADIENT_SKY
@@ -285,8 +318,8 @@ class Scene:
             sd.envir_light.upper = _abi.F3(*map(float, upper))
         else:
             sd.envir_light.type = _abi.ENVIR_NONE
-        holder.update(gd=gd, md=md, idesc=idesc, med=med, td=td, geoms=self.geometries, media=self.media,
-                      textures=self.textures)
+        holder.update(gd=gd, md=md, idesc=idesc, med=med, td=td, vd=vd, geoms=self.geometries, media=self.media,
+                      textures=self.textures, volumes=self.volumes)
         return sd, holder
 
 

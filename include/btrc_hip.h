@@ -184,6 +184,26 @@ typedef struct btrc_medium_desc
     float world_to_local[16]; /* row-major */
 } btrc_medium_desc;
 
+/* A volume primitive (VolumePrimitive, core/volume.cpp:21-79; the scene's
+ * "volumes" array, factory/scene.cpp:57-79): the parallelepiped
+ * o + u x + v y + w z, u, v, w in [0, 1], with a density sigma_t(u, v, w)
+ * and an albedo(u, v, w), each a grid (Array3D layout, x fastest, linear
+ * filtering with clamping at texel centres (i + 0.5) / n) or a constant.
+ * All volume primitives form the scene's volume-primitive medium, the
+ * lowest-priority medium that camera rays start in and that fills
+ * everything outside the instances' media; where volumes overlap their
+ * densities add, and scattering in them is isotropic. */
+typedef struct btrc_volume_desc
+{
+    float o[3], x[3], y[3], z[3];
+    int32_t sigma_t_size[3];     /* grid width, height, depth; 0, 0, 0: constant sigma_t */
+    const float *sigma_t_grid;
+    float sigma_t;
+    int32_t albedo_size[3];      /* 0, 0, 0: constant albedo */
+    const float *albedo_grid;    /* 3 floats per voxel */
+    float albedo[3];
+} btrc_volume_desc;
+
 typedef struct btrc_envir_light_desc
 {
     int32_t type;   /* BTRC_ENVIR_* */
@@ -210,6 +230,8 @@ typedef struct btrc_scene_desc
     int32_t accel;
     int32_t texture_count;
     const btrc_texture_desc *textures;
+    int32_t volume_count;
+    const btrc_volume_desc *volumes;
 } btrc_scene_desc;
 
 /* PinholeCamera (builtin/camera/pinhole.cpp) */
@@ -409,6 +431,10 @@ int btrc_hip_eval_halton(btrc_ctx *ctx, int32_t width, int32_t height, const uin
 /* the independent sampler's first `dims` numbers for each (px, py, sample) */
 int btrc_hip_eval_independent(btrc_ctx *ctx, const uint32_t *px, const uint32_t *py, const int32_t *sample_index,
                               int32_t n, int32_t dims, float *out);
+/* the volume-primitive medium along segments o -> o + d * tmax, one PCG32
+ * stream per segment as in btrc_hip_eval_medium: out holds 10 floats per
+ * segment (scattered, position xyz, albedo xyz, tr xyz) */
+int btrc_hip_eval_volumes(btrc_ctx *ctx, const btrc_ray *segments, const uint64_t *seeds, int32_t n, float *out);
 /* texture `texture` of the current scene sampled at n (u, v) pairs; out
  * gets 4 floats per pair */
 int btrc_hip_eval_texture(btrc_ctx *ctx, int32_t texture, const float *uv, int32_t n, float *out);

@@ -494,6 +494,18 @@ struct Scene
     int env = -1;
     std::vector<Med> media; // by descending priority
     int R = 0;
+    struct Vol
+    {
+        Vec o, x, y, z;
+        float ix2, iy2, iz2;  // 1 / |edge|^2
+        Vec xd, yd, zd;       // edge / |edge|^2
+        float maxd, sigma;
+        int nx = 0, ny = 0, nz = 0, ax = 0, ay = 0, az = 0;
+        std::vector<float> dens, alb;
+        Vec albedo;
+    };
+    std::vector<Vol> vols;    // the volume-primitive medium
+    Vec bb_lo{ FMAX, FMAX, FMAX }, bb_hi{ -FMAX, -FMAX, -FMAX }; // Scene::commit's bbox
     // per triangle of all geometries: frame records (triangle_mesh.cpp:40-64)
     std::vector<Vec> gx, gy, gz, sa, sba, sca, la, lba, lca;
     // texture coordinates (a, b - a, c - a) per triangle, u then v
@@ -792,6 +804,52 @@ static Scene *make_scene(const btrc_scene_desc *sd)
         for(const Vec &v : *w)
             s->extent = std::max(s->extent, std::max(std::fabs(v.x), std::max(std::fabs(v.y), std::fabs(v.z))));
     s->two_level = sd->accel == BTRC_ACCEL_TWO_LEVEL;
+    for(const std::vector<Vec> *w : { &s->w0, &s->w1, &s->w2 })
+        for(const Vec &v : *w)
+        {
+            s->bb_lo = mk(fmn(s->bb_lo.x, v.x), fmn(s->bb_lo.y, v.y), fmn(s->bb_lo.z, v.z));
+            s->bb_hi = mk(fmx(s->bb_hi.x, v.x), fmx(s->bb_hi.y, v.y), fmx(s->bb_hi.z, v.z));
+        }
+    for(int i = 0; i < sd->volume_count; ++i)
+    {
+        // VolumePrimitive (core/volume.cpp:21-79) and the BVH's primitive
+        // record (core/volume/bvh.cpp:83-92)
+        const btrc_volume_desc &vd = sd->volumes[i];
+        Scene::Vol v;
+        v.o = mk(vd.o[0], vd.o[1], vd.o[2]);
+        v.x = mk(vd.x[0], vd.x[1], vd.x[2]);
+        v.y = mk(vd.y[0], vd.y[1], vd.y[2]);
+        v.z = mk(vd.z[0], vd.z[1], vd.z[2]);
+        v.ix2 = 1.0f / dot(v.x, v.x);
+        v.iy2 = 1.0f / dot(v.y, v.y);
+        v.iz2 = 1.0f / dot(v.z, v.z);
+        v.xd = dvs(v.x, dot(v.x, v.x));
+        v.yd = dvs(v.y, dot(v.y, v.y));
+        v.zd = dvs(v.z, dot(v.z, v.z));
+        v.sigma = vd.sigma_t;
+        v.maxd = vd.sigma_t;
+        if(vd.sigma_t_size[0] > 0)
+        {
+            v.nx = vd.sigma_t_size[0]; v.ny = vd.sigma_t_size[1]; v.nz = vd.sigma_t_size[2];
+            v.dens.assign(vd.sigma_t_grid, vd.sigma_t_grid + (size_t)v.nx * v.ny * v.nz);
+            v.maxd = v.dens[0];
+            for(float x : v.dens)
+                v.maxd = fmx(v.maxd, x);
+        }
+        v.albedo = mk(vd.albedo[0], vd.albedo[1], vd.albedo[2]);
+        if(vd.albedo_size[0] > 0)
+        {
+            v.ax = vd.albedo_size[0]; v.ay = vd.albedo_size[1]; v.az = vd.albedo_size[2];
+            v.alb.assign(vd.albedo_grid, vd.albedo_grid + 3 * (size_t)v.ax * v.ay * v.az);
+        }
+        // VolumePrimitive::get_bounding_box: o and the three far corners
+        for(const Vec &c : { v.o, add(v.o, v.x), add(v.o, v.y), add(v.o, v.z) })
+        {
+            s->bb_lo = mk(fmn(s->bb_lo.x, c.x), fmn(s->bb_lo.y, c.y), fmn(s->bb_lo.z, c.z));
+            s->bb_hi = mk(fmx(s->bb_hi.x, c.x), fmx(s->bb_hi.y, c.y), fmx(s->bb_hi.z, c.z));
+        }
+        s->vols.push_back(std::move(v));
+    }
     if(s->two_level)
     {
         // instance world boxes: the geometry box's transformed corners,
@@ -1816,6 +1874,183 @@ static Vec med_tr(const Med &m, Vec a, Vec b, S &smp)
     return mk(tr, tr, tr);
 }
 
+// ------------------------------------------------------------------ volume primitives
+//
+// VolumePrimitiveMedium::sample / tr (core/volume.cpp:108-197) over pieces
+// between primitive boundaries; on each piece volume::Aggregate's delta /
+// ratio tracking over the primitives holding its midpoint
+// (core/volume/aggregate.cpp:128-254). Boundaries and overlap sets by
+// scanning every primitive (the reference's BVH finds the same minimum and
+// the same set; MAX_OVERLAP_COUNT = 4, core/volume/bvh.h:10).
+
+static Vec vol_uvw(const Scene::Vol &v, Vec p)
+{
+    const Vec op = sub(p, v.o);
+    return mk(dot(v.x, op) * v.ix2, dot(v.y, op) * v.iy2, dot(v.z, op) * v.iz2);
+}
+
+static bool vol_inside(const Scene::Vol &v, Vec p)
+{
+    const Vec op = sub(p, v.o);
+    const float a = dot(v.xd, op), b = dot(v.yd, op), c = dot(v.zd, op);
+    return 0.0f <= a && a <= 1.0f && 0.0f <= b && b <= 1.0f && 0.0f <= c && c <= 1.0f;
+}
+
+static float vol_cross(const Scene::Vol &v, Vec p, Vec d, float tmax)
+{
+    const Vec op = sub(p, v.o);
+    float lo = 0.0f, hi = tmax;
+    for(const Vec *ax : { &v.xd, &v.yd, &v.zd })
+    {
+        const float opd = dot(*ax, op), dd = dot(*ax, d);
+        const float a = (0.0f - opd) / dd, b = (1.0f - opd) / dd;
+        lo = std::fmax(lo, std::fmin(a, b));
+        hi = std::fmin(hi, std::fmax(a, b));
+    }
+    if(lo > hi)
+        return -1.0f;
+    if(lo > 0.0f)
+        return lo;
+    return hi < tmax ? hi : -1.0f;
+}
+
+static bool vol_next_boundary(const Scene &s, Vec a, Vec b, Vec &out)
+{
+    const Vec dir = sub(b, a);
+    float best = FMAX, tmax = 1.0f;
+    for(const Scene::Vol &v : s.vols)
+    {
+        const float t = vol_cross(v, a, dir, tmax);
+        if(t >= 0.0f)
+        {
+            best = std::fmin(best, t);
+            tmax = best;
+        }
+    }
+    if(best > 10.0f)
+        return false;
+    out = add(scl(a, 1.0f - best), scl(b, best));
+    return true;
+}
+
+static std::vector<int> vol_set(const Scene &s, Vec p)
+{
+    std::vector<int> r;
+    for(int i = 0; i < (int)s.vols.size() && r.size() < 4; ++i)
+        if(vol_inside(s.vols[i], p))
+            r.push_back(i);
+    return r;
+}
+
+static float vol_density(const Scene::Vol &v, Vec uvw)
+{
+    return v.nx == 0 ? v.sigma : tex3(v.dens, v.nx, v.ny, v.nz, 1, 0, uvw);
+}
+
+static Vec vol_albedo(const Scene::Vol &v, Vec uvw)
+{
+    if(v.ax == 0)
+        return v.albedo;
+    return mk(tex3(v.alb, v.ax, v.ay, v.az, 3, 0, uvw), tex3(v.alb, v.ax, v.ay, v.az, 3, 1, uvw),
+              tex3(v.alb, v.ax, v.ay, v.az, 3, 2, uvw));
+}
+
+// tracking over one piece a -> b with the primitives `set`; sample: true on
+// scattering (pos, color set); otherwise multiplies the piece's
+// transmittance into *tr when tr is given
+template<class S>
+static bool vol_piece(const Scene &s, const std::vector<int> &set, Vec a, Vec b, S &smp, Vec *pos, Vec *color,
+                      float *tr)
+{
+    float maj = 0.0f;
+    for(int i : set)
+        maj = maj + s.vols[i].maxd;
+    const float inv = 1.0f / fmx(maj, 0.01f);
+    std::vector<Vec> ua, ub;
+    for(int i : set)
+    {
+        ua.push_back(vol_uvw(s.vols[i], a));
+        ub.push_back(vol_uvw(s.vols[i], b));
+    }
+    const float len_ab = len(sub(b, a));
+    float t = 0.0f, seg = 1.0f;
+    for(;;)
+    {
+        t = t + -dm::log(1.0f - smp.get1d()) * inv;
+        if(t >= len_ab)
+            break;
+        const float f = t / len_ab;
+        std::vector<float> dv;
+        float sum = 0.0f;
+        for(size_t k = 0; k < set.size(); ++k)
+        {
+            dv.push_back(vol_density(s.vols[set[k]], add(scl(ua[k], 1.0f - f), scl(ub[k], f))));
+            sum = sum + dv.back();
+        }
+        if(!tr)
+        {
+            if(smp.get1d() < sum * inv)
+            {
+                *pos = add(scl(a, 1.0f - f), scl(b, f));
+                Vec c = mk(0, 0, 0);
+                for(size_t k = 0; k < set.size(); ++k)
+                    c = add(c, sclf(dv[k] / sum, vol_albedo(s.vols[set[k]], add(scl(ua[k], 1.0f - f), scl(ub[k], f)))));
+                *color = c;
+                return true;
+            }
+        }
+        else
+            seg = seg * (1.0f - sum * inv);
+    }
+    if(tr)
+        *tr = *tr * seg;
+    return false;
+}
+
+template<class S>
+static bool vol_sample(const Scene &s, Vec a, Vec b, S &smp, Vec &pos, Vec &color)
+{
+    if(s.vols.empty())
+        return false;
+    Vec o = a;
+    const Vec d = nrm(sub(b, a));
+    for(;;)
+    {
+        if(dot(sub(b, o), d) <= 0.0f || dot(sub(b, o), sub(b, o)) < 0.00001f)
+            return false;
+        Vec c;
+        if(!vol_next_boundary(s, o, b, c))
+            c = b;
+        const std::vector<int> set = vol_set(s, scl(add(o, c), 0.5f));
+        if(!set.empty() && vol_piece(s, set, o, c, smp, &pos, &color, nullptr))
+            return true;
+        o = add(c, sclf(0.001f, d));
+    }
+}
+
+template<class S>
+static Vec vol_tr(const Scene &s, Vec a, Vec b, S &smp)
+{
+    float tr = 1.0f;
+    if(s.vols.empty())
+        return mk(tr, tr, tr);
+    Vec o = a;
+    const Vec d = nrm(sub(b, a));
+    for(;;)
+    {
+        if(dot(sub(b, o), d) <= 0.0f || dot(sub(b, o), sub(b, o)) < 0.0001f)
+            break;
+        Vec c;
+        if(!vol_next_boundary(s, o, b, c))
+            c = b;
+        const std::vector<int> set = vol_set(s, scl(add(o, c), 0.5f));
+        if(!set.empty())
+            vol_piece(s, set, o, c, smp, nullptr, nullptr, &tr);
+        o = add(c, sclf(0.001f, d));
+    }
+    return mk(tr, tr, tr);
+}
+
 // ------------------------------------------------------------------ path tracer
 
 struct Params
@@ -1823,6 +2058,7 @@ struct Params
     int32_t w, h, spp, min_depth, max_depth;
     float rr_thr, rr_p;
     Vec eye, lbc, fx, fy;
+    float world_diag = 0.0f; // 1.2 |bbox of scene and eye| (wavefront.cpp:138-139)
     bool tile = false, indep = false, gaussian = false;
     float radius = 0.0f, alpha = 0.0f;
     std::vector<float> g_accept;   // Gaussian filter table, 256 x 256 cells
@@ -1948,12 +2184,19 @@ static void trace_path(const Scene &s, const Params &p, uint64_t path_index, std
         const uint64_t seed = shadow_seed(hs.pack(pixel));
         if(trace(s, BTRC_TRI_WATERTIGHT, true, true, so, sd, 0.0f, t1).gid != 0xffffffffu)
             return;
-        if(!(t1 > 1.0f) && med < (uint32_t)s.R)
+        Pcg tr_rng;
+        tr_rng.seed(seed, mixbits(seed));
+        if(t1 > 1.0f)
         {
-            Pcg tr_rng;
-            tr_rng.seed(seed, mixbits(seed));
-            bl = mul(bl, med_tr(s.media[med], so, add(so, sclf(t1, sd)), tr_rng));
+            // toward the environment: the volume-primitive medium over the
+            // world's diagonal (shadow.cpp:56-60)
+            if(!s.vols.empty())
+                bl = mul(bl, vol_tr(s, so, add(so, sclf(p.world_diag, nrm(sd))), tr_rng));
         }
+        else if(med < (uint32_t)s.R)
+            bl = mul(bl, med_tr(s.media[med], so, add(so, sclf(t1, sd)), tr_rng));
+        else if(med == (uint32_t)s.R && !s.vols.empty())
+            bl = mul(bl, vol_tr(s, so, add(so, sclf(t1, sd)), tr_rng));
         splat(film, pixel, bl);
     };
     // simple_russian_roulette, helper.cpp:67-97; true: the path ends
@@ -1984,9 +2227,9 @@ static void trace_path(const Scene &s, const Params &p, uint64_t path_index, std
         Vec mpos{}, mcolor{};
         float mg = 0.0f;
         uint32_t med = 0;
-        if(s.R > 0)
+        if(s.R > 0 || !s.vols.empty())
         {
-            Vec end = o;
+            Vec end;
             if(hit)
             {
                 const Inst &in = s.insts[s.tinst[h.gid]];
@@ -1996,7 +2239,11 @@ static void trace_path(const Scene &s, const Params &p, uint64_t path_index, std
                 end = offset_dir(add(o, sclf(h.t, d)), nor, neg(d));
             }
             else
+            {
+                // get_medium_segment, medium.cpp:244-248
                 med = (uint32_t)s.R;
+                end = add(o, sclf(p.world_diag, nrm(d)));
+            }
             ray_med = med;
             if(med < (uint32_t)s.R)
             {
@@ -2004,6 +2251,13 @@ static void trace_path(const Scene &s, const Params &p, uint64_t path_index, std
                 scattered = med_sample(m, o, end, hs, mpos, mcolor);
                 mg = m.g;
                 beta_le = mul(beta_le, med_tr(m, o, end, hs));
+            }
+            else if(med == (uint32_t)s.R && !s.vols.empty())
+            {
+                // the volume-primitive medium: isotropic scattering
+                scattered = vol_sample(s, o, end, hs, mpos, mcolor);
+                mg = 0.0f;
+                beta_le = mul(beta_le, vol_tr(s, o, end, hs));
             }
         }
 
@@ -2352,6 +2606,9 @@ void oracle_render(void *sp, const btrc_camera_desc *cam, const btrc_render_para
         const float yl = 2.0f * std::tan(0.5f * cam->fov_y_deg * PI / 180.0f);
         const float xl = cam->w_over_h * yl;
         p.eye = eye;
+        const Vec lo = mk(fmn(s.bb_lo.x, eye.x), fmn(s.bb_lo.y, eye.y), fmn(s.bb_lo.z, eye.z));
+        const Vec hi = mk(fmx(s.bb_hi.x, eye.x), fmx(s.bb_hi.y, eye.y), fmx(s.bb_hi.z, eye.z));
+        p.world_diag = 1.2f * len(sub(hi, lo));
         p.lbc = sub(add(eye, fwd), sclf(0.5f, add(sclf(xl, ex), sclf(yl, ey))));
         p.fx = sclf(xl, ex);
         p.fy = sclf(yl, ey);
@@ -2419,6 +2676,27 @@ void oracle_gaussian_filter(float radius, float alpha, const float *u, int32_t n
     orc::gaussian_table(p);
     for(int32_t i = 0; i < n; ++i)
         orc::gaussian_sample(p, u[3 * i], u[3 * i + 1], u[3 * i + 2], out[2 * i], out[2 * i + 1]);
+}
+
+void oracle_volume_eval(void *sp, const btrc_ray *seg, const uint64_t *seeds, int32_t n, float *out)
+{
+    const orc::Scene &s = *static_cast<orc::Scene *>(sp);
+    for(int32_t i = 0; i < n; ++i)
+    {
+        const orc::Vec a = orc::mk(seg[i].o[0], seg[i].o[1], seg[i].o[2]);
+        const orc::Vec d = orc::mk(seg[i].d[0], seg[i].d[1], seg[i].d[2]);
+        const orc::Vec b = orc::add(a, orc::scl(d, seg[i].tmax));
+        orc::Pcg r;
+        r.seed(seeds[i], orc::mixbits(seeds[i]));
+        orc::Vec pos = orc::mk(0, 0, 0), col = orc::mk(0, 0, 0);
+        const bool sc = orc::vol_sample(s, a, b, r, pos, col);
+        const orc::Vec tr = orc::vol_tr(s, a, b, r);
+        float *o = out + 10 * (size_t)i;
+        o[0] = sc ? 1.0f : 0.0f;
+        o[1] = pos.x; o[2] = pos.y; o[3] = pos.z;
+        o[4] = col.x; o[5] = col.y; o[6] = col.z;
+        o[7] = tr.x; o[8] = tr.y; o[9] = tr.z;
+    }
 }
 
 void oracle_texture_eval(void *sp, int32_t texture, const float *uv, int32_t n, float *out)

@@ -38,6 +38,7 @@ def lib():
         L.oracle_gaussian_filter.argtypes = [C.c_float, C.c_float, vp, C.c_int32, vp]
         L.oracle_medium_eval.argtypes = [vp, C.c_int32, vp, vp, C.c_int32, vp]
         L.oracle_texture_eval.argtypes = [vp, C.c_int32, vp, C.c_int32, vp]
+        L.oracle_volume_eval.argtypes = [vp, vp, vp, C.c_int32, vp]
         L.oracle_hg.argtypes = [C.c_float, vp, C.c_int32, vp]
         L.oracle_tex3.restype = C.c_float
         L.oracle_tex3.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32, C.c_float, C.c_float, C.c_float]
@@ -88,6 +89,14 @@ class OracleScene:
             lib().oracle_scene_destroy(self.h)
         except Exception:
             pass
+
+    def volume_eval(self, segments, seeds):
+        """(n, 10) float32 of the volume-primitive medium: scattered, position, albedo, transmittance"""
+        seg = np.ascontiguousarray(segments, dtype=_abi.ray_dtype())
+        seeds = np.ascontiguousarray(seeds, dtype=np.uint64)
+        out = np.empty((len(seg), 10), dtype=np.float32)
+        lib().oracle_volume_eval(self.h, seg.ctypes.data, seeds.ctypes.data, len(seg), out.ctypes.data)
+        return out
 
     def texture_eval(self, texture, uv):
         """(n, 4) float32: the scene's texture `texture` at (n, 2) uv"""
