@@ -213,7 +213,9 @@ class Scene:
             w = ((m[:, 0] * p[:, :1] + m[:, 1] * p[:, 1:2]) + m[:, 2] * p[:, 2:3]) + m[:, 3]
             lo = np.minimum(lo, w[:, :3].min(axis=0))
             hi = np.maximum(hi, w[:, :3].max(axis=0))
-        ext = np.where(hi > lo, hi - lo, np.float32(1.0)).astype(np.float32)
+        ext = np.ones(3, np.float32)
+        ok = hi > lo
+        ext[ok] = hi[ok] - lo[ok]
         return np.concatenate([lo, (np.float32(1.0) / ext).astype(np.float32)]).astype(np.float32)
 
     @property

@@ -15,11 +15,23 @@ def pytest_configure(config):
 
 
 @pytest.fixture(scope="session")
-def ctx():
+def _session_ctx():
     import btrc_amd
     c = btrc_amd.Context(0)
     yield c
     c.close()
+
+
+@pytest.fixture
+def ctx(_session_ctx):
+    """one context for the whole session; every test starts from the box
+    filter, the default tuning and the cache switched off"""
+    c = _session_ctx
+    c.set_filter("box")
+    c.set_tuning(closest_postpone_leaves=1, any_postpone_leaves=0, trace_waves_per_cu=20, launch_graph=1,
+                 two_level_postpone=0)
+    c.nrc_configure(enabled=0)
+    yield c
 
 
 @pytest.fixture(scope="session")
