@@ -292,6 +292,26 @@ class Context:
         self._check(self.lib.btrc_hip_eval_texture(self.h, texture, uv.ctypes.data, len(uv), out.ctypes.data))
         return out
 
+    def eval_envir(self, dirs, samples):
+        """(n, 12) float32: the environment light's eval_le rgb, pdf_li at
+        world directions `dirs` (n, 3), then sample_li's direction, pdf and
+        radiance for the sample triples `samples` (n, 3), then 0"""
+        d = np.ascontiguousarray(dirs, dtype=np.float32).reshape(-1, 3)
+        sm = np.ascontiguousarray(samples, dtype=np.float32).reshape(-1, 3)
+        if len(d) != len(sm):
+            raise BtrcException("eval_envir: dirs and samples differ in length")
+        out = np.empty((len(d), 12), dtype=np.float32)
+        self._check(self.lib.btrc_hip_eval_envir(self.h, d.ctypes.data, sm.ctypes.data, len(d), out.ctypes.data))
+        return out
+
+    def envir_table(self):
+        """the IBL importance table's tile probabilities (empty without an IBL)"""
+        n = C.c_int32(0)
+        self._check(self.lib.btrc_hip_envir_table(self.h, None, 0, C.byref(n)))
+        out = np.empty(n.value, dtype=np.float32)
+        self._check(self.lib.btrc_hip_envir_table(self.h, out.ctypes.data, n.value, C.byref(n)))
+        return out
+
     def eval_medium(self, medium, segments, seeds):
         """(n, 10) float32: scattered, position, albedo, transmittance"""
         seg = np.ascontiguousarray(segments, dtype=_abi.ray_dtype())

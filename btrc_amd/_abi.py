@@ -25,6 +25,7 @@ ACCEL_FLAT = 0
 ACCEL_TWO_LEVEL = 1
 ENVIR_NONE = 0
 ENVIR_GRADIENT_SKY = 1
+ENVIR_IBL = 2
 SAMPLER_HALTON = 0
 SAMPLER_INDEPENDENT = 1
 FILTER_BOX = 0
@@ -73,7 +74,8 @@ class MediumDesc(C.Structure):
 
 
 class EnvirLightDesc(C.Structure):
-    _fields_ = [("type", C.c_int32), ("up", F3), ("lower", F3), ("upper", F3)]
+    _fields_ = [("type", C.c_int32), ("up", F3), ("lower", F3), ("upper", F3), ("texture", C.c_int32),
+                ("lut_width", C.c_int32), ("lut_height", C.c_int32)]
 
 
 class VolumeDesc(C.Structure):
@@ -172,7 +174,7 @@ EXPORTS = [
     "btrc_hip_trace_closest", "btrc_hip_trace_any",
     "btrc_hip_batch_upload", "btrc_hip_batch_trace", "btrc_hip_batch_download",
     "btrc_hip_capture_rays", "btrc_hip_eval_halton", "btrc_hip_eval_independent", "btrc_hip_eval_volumes", "btrc_hip_eval_texture",
-    "btrc_hip_eval_medium",
+    "btrc_hip_eval_envir", "btrc_hip_envir_table", "btrc_hip_eval_medium",
     "btrc_hip_eval_detmath",
 ]
 
@@ -218,6 +220,8 @@ def declare(lib):
                                                 C.c_void_p]),
         "btrc_hip_eval_volumes": (C.c_int, [ctx, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p]),
         "btrc_hip_eval_texture": (C.c_int, [ctx, C.c_int32, C.c_void_p, C.c_int32, C.c_void_p]),
+        "btrc_hip_eval_envir": (C.c_int, [ctx, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p]),
+        "btrc_hip_envir_table": (C.c_int, [ctx, C.c_void_p, C.c_int32, C.POINTER(C.c_int32)]),
         "btrc_hip_eval_medium": (C.c_int, [ctx, C.c_int32, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p]),
         "btrc_hip_eval_detmath": (C.c_int, [ctx, C.c_int32, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p]),
     }

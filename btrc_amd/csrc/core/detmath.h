@@ -18,6 +18,11 @@
 //             s = (m-1)/(m+1), atanh series to s^11.
 //   exp     : x = k ln2 + r, |r| <= ln2/2, Taylor to r^8, scale by 2^k.
 //   pow     : exp(y * log(x)) for x > 0.
+//   atan2   : octant reduction to t = min/max in [0, 1], then t > tan(pi/8)
+//             maps to pi/4 + atan((t-1)/(t+1)); Taylor series of atan to
+//             s^21 on |s| <= tan(pi/8); pi/4, pi/2, pi added as two-part
+//             splits.
+//   acos    : 2 atan2(sqrt(1-x), sqrt(1+x)).
 #pragma once
 
 #include <cstdint>
@@ -168,6 +173,64 @@ BTRC_DM_FN float pow(float x, float y)
     if(x == 0.0f)
         return y > 0.0f ? 0.0f : u2f(0x7f800000u);
     return exp(y * log(x));
+}
+
+// arctangent of s for |s| <= tan(pi/8):
+// s + s^3 (-1/3 + s^2 (1/5 + s^2 (-1/7 + ... + s^2 (-1/19 + s^2 / 21))))
+BTRC_DM_FN float atan_small(float s)
+{
+    const float s2 = s * s;
+    float p = fmaf(s2, 1.0f / 21.0f, -1.0f / 19.0f);
+    p = fmaf(s2, p, 1.0f / 17.0f);
+    p = fmaf(s2, p, -1.0f / 15.0f);
+    p = fmaf(s2, p, 1.0f / 13.0f);
+    p = fmaf(s2, p, -1.0f / 11.0f);
+    p = fmaf(s2, p, 1.0f / 9.0f);
+    p = fmaf(s2, p, -1.0f / 7.0f);
+    p = fmaf(s2, p, 1.0f / 5.0f);
+    p = fmaf(s2, p, -1.0f / 3.0f);
+    return fmaf(s * s2, p, s);
+}
+
+// atan2(y, x) in [-pi, pi], with the C library's signed-zero and infinity
+// cases; NaN in gives NaN out
+BTRC_DM_FN float atan2(float y, float x)
+{
+    if(x != x || y != y)
+        return x + y;
+    const float INF = u2f(0x7f800000u);
+    const float PI4_HI = 0x1.921fb6p-1f, PI4_LO = -0x1.777a5cp-26f;
+    const float PI2_HI = 0x1.921fb6p+0f, PI2_LO = -0x1.777a5cp-25f;
+    const float PI_HI = 0x1.921fb6p+1f, PI_LO = -0x1.777a5cp-24f;
+    const bool xneg = (f2u(x) >> 31) != 0, yneg = (f2u(y) >> 31) != 0;
+    const float ax = u2f(f2u(x) & 0x7fffffffu), ay = u2f(f2u(y) & 0x7fffffffu);
+    const float mx = ax > ay ? ax : ay, mn = ax > ay ? ay : ax;
+    float a;
+    if(mx == 0.0f)
+        a = 0.0f;
+    else
+    {
+        float t;
+        if(mx == INF)
+            t = mn == INF ? 1.0f : 0.0f;
+        else
+            t = mn / mx;
+        if(t > 0x1.a8279ap-2f) // tan(pi/8)
+            a = (atan_small((t - 1.0f) / (t + 1.0f)) + PI4_LO) + PI4_HI;
+        else
+            a = atan_small(t);
+        if(ay > ax)
+            a = (PI2_LO - a) + PI2_HI;
+    }
+    if(xneg)
+        a = (PI_LO - a) + PI_HI;
+    return yneg ? -a : a;
+}
+
+// arccosine in [0, pi]; NaN outside [-1, 1]
+BTRC_DM_FN float acos(float x)
+{
+    return 2.0f * atan2(sqrtf(1.0f - x), sqrtf(1.0f + x));
 }
 
 } // namespace btrc_detmath

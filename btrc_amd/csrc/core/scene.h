@@ -1,7 +1,8 @@
 // Device-resident scene records and the surface/light queries the shading
 // stages make on them: get_hitinfo (core/scene.cpp:9-64), the uniform light
 // sampler (core/light_sampler.cpp:41-63), MeshLight (light/mesh_light.cpp),
-// GradientSky (light/gradient_sky.cpp), the alias table
+// GradientSky (light/gradient_sky.cpp), IBL (light/ibl.cpp) with its
+// importance table (light/env_sampler.cpp), the alias table
 // (utils/cmath/calias.cpp:10-23) and TriangleMesh::sample_inline
 // (geometry/triangle_mesh.cpp:188-243).
 #pragma once
@@ -21,6 +22,7 @@ enum : int32_t
 {
     LIGHT_MESH = 0,
     LIGHT_GRADIENT_SKY = 1,
+    LIGHT_IBL = 2,
 };
 
 // InstanceInfo, core/scene.h:21-30, with the transform as two 3x4 blocks
@@ -53,10 +55,14 @@ struct LightRec
     int32_t geometry;
     float scale;        // MeshLight::scale_ (mesh_light.cpp:17-22)
     float intensity[3];
-    float up[3];        // GradientSky, normalized (gradient_sky.cpp:15-18)
+    float up[3];        // GradientSky: normalized (gradient_sky.cpp:15-18); IBL: frame z
     float lower[3];
     float upper[3];
-    float pad[2];
+    float frame_x[3];   // IBL: Frame::from_z(up) (ibl.cpp:11-14), axes normalized
+    float frame_y[3];   // once more on the device (CFrame(Frame), cframe.cpp:11-22)
+    int32_t texture;    // IBL radiance texture
+    int32_t lut_w, lut_h; // IBL importance table resolution
+    float pad[3];
 };
 
 struct AliasUnit
@@ -93,6 +99,9 @@ struct SceneView
     const MediumRec *media;
     const float *voxels;
     TextureView tex;
+    // the IBL's tile probabilities and their alias table (lut_w * lut_h each)
+    const float *env_probs;
+    const AliasUnit *env_alias;
     VolumeView vol;           // the volume-primitive medium (id medium_count)
     float world_diagonal;     // 1.2 |bbox of the scene and the cameras| (wavefront.cpp:138-139)
 };
@@ -189,6 +198,181 @@ BTRC_HD V3 sky_eval_le(const LightRec &l, V3 to_light)
     const float cos_theta = dot(v3(l.up[0], l.up[1], l.up[2]), normalize(to_light));
     const float s = saturate(0.5f * (cos_theta + 1.0f));
     return v3(l.lower[0], l.lower[1], l.lower[2]) * (1.0f - s) + v3(l.upper[0], l.upper[1], l.upper[2]) * s;
+}
+
+// local_angle::phi, utils/local_angle.h:63-72: the azimuth in [0, 2 pi)
+BTRC_HD float local_phi(V3 w)
+{
+    float r = 0.0f;
+    if(w.x != 0.0f || w.y != 0.0f)
+    {
+        r = btrc_detmath::atan2(w.y, w.x);
+        r = r < 0.0f ? r + 2.0f * PI : r;
+    }
+    return r;
+}
+
+// local_angle::theta, local_angle.h:45-49
+BTRC_HD float local_theta(V3 w) { return btrc_detmath::acos(fmin_(fmax_(w.z, -1.0f), 1.0f)); }
+
+// hammersley2d, utils/math/hammersley.h:7-21: (i / n, bit-reversed i)
+BTRC_HD void hammersley2d(uint32_t i, uint32_t n, float *x, float *y)
+{
+    uint32_t b = (i << 16u) | (i >> 16u);
+    b = ((b & 0x55555555u) << 1u) | ((b & 0xAAAAAAAAu) >> 1u);
+    b = ((b & 0x33333333u) << 2u) | ((b & 0xCCCCCCCCu) >> 2u);
+    b = ((b & 0x0F0F0F0Fu) << 4u) | ((b & 0xF0F0F0F0u) >> 4u);
+    b = ((b & 0x00FF00FFu) << 8u) | ((b & 0xFF00FF00u) >> 8u);
+    *x = (float)i / (float)n;
+    *y = (float)b * 2.3283064365386963e-10f;
+}
+
+// One tile of EnvirLightSampler's table (env_sampler.cpp:22-57): the mean
+// luminance of the texture at n Hammersley points spread over the tile
+// widened by a tenth of its size on every side, and the tile's solid angle
+// 2 pi du |cos(pi v1) - cos(pi v0)|.
+BTRC_HD void envir_tile_stats(const TextureView &tv, int32_t tex, int32_t w, int32_t h, int32_t xi, int32_t yi,
+                              int32_t n, float *lum, float *area)
+{
+    const float x0 = (float)xi / (float)w;
+    const float x1 = (float)(xi + 1) / (float)w;
+    const float y0 = (float)yi / (float)h;
+    const float y1 = (float)(yi + 1) / (float)h;
+    float sum = 0.0f;
+    for(int32_t i = 0; i < n; ++i)
+    {
+        float hx, hy;
+        hammersley2d((uint32_t)i, (uint32_t)n, &hx, &hy);
+        const float sx = 1.2f * hx - 0.1f;
+        const float sy = 1.2f * hy - 0.1f;
+        const float4 c = tex_sample(tv, tex, lerpf(x0, x1, sx), lerpf(y0, y1, sy));
+        sum = sum + luminance(v3(c.x, c.y, c.z));
+    }
+    *lum = sum / (float)n;
+    *area = fabs_(2.0f * PI * (x1 - x0) * (btrc_detmath::cos(PI * y1) - btrc_detmath::cos(PI * y0)));
+}
+
+// i32(floor(x)) clamped to [0, n - 1]; NaN gives 0, as the device's
+// saturating float-to-int conversion does
+BTRC_HD int32_t envir_tile_index(float x, int32_t n)
+{
+    const float f = floorf(x);
+    if(!(f >= 0.0f))
+        return 0;
+    if(f >= (float)(n - 1))
+        return n - 1;
+    return (int32_t)f;
+}
+
+BTRC_HD V3 ibl_to_local(const LightRec &l, V3 g)
+{
+    return v3(dot(g, v3(l.frame_x[0], l.frame_x[1], l.frame_x[2])), dot(g, v3(l.frame_y[0], l.frame_y[1], l.frame_y[2])),
+              dot(g, v3(l.up[0], l.up[1], l.up[2])));
+}
+
+BTRC_HD V3 ibl_to_global(const LightRec &l, V3 w)
+{
+    return v3(l.frame_x[0], l.frame_x[1], l.frame_x[2]) * w.x + v3(l.frame_y[0], l.frame_y[1], l.frame_y[2]) * w.y +
+           v3(l.up[0], l.up[1], l.up[2]) * w.z;
+}
+
+// IBL::eval_local, ibl.cpp:71-78: the texture at (phi / 2 pi, theta / pi)
+BTRC_HD V3 ibl_eval_local(const SceneView &sc, const LightRec &l, V3 w)
+{
+    const float phi = local_phi(w);
+    const float theta = local_theta(w);
+    const float u = saturate((1.0f / (2.0f * PI)) * phi);
+    const float v = saturate((1.0f / PI) * theta);
+    const float4 c = tex_sample(sc.tex, l.texture, u, v);
+    return v3(c.x, c.y, c.z);
+}
+
+// EnvirLightSampler::sample, env_sampler.cpp:124-159: a tile from the alias
+// table, then a direction uniform in the tile's solid angle
+BTRC_HD void ibl_sample_local(const SceneView &sc, const LightRec &l, float s0, float s1, float s2, V3 *dir, float *pdf)
+{
+    const uint32_t tile = alias_sample(sc.env_alias, (uint32_t)(l.lut_w * l.lut_h), s0);
+    const int32_t ty = (int32_t)tile / l.lut_w;
+    const int32_t tx = (int32_t)tile % l.lut_w;
+    const float tile_pdf = sc.env_probs[tile];
+    const float u0 = (float)tx / (float)l.lut_w;
+    const float u1 = (float)(tx + 1) / (float)l.lut_w;
+    const float v0 = (float)ty / (float)l.lut_h;
+    const float v1 = (float)(ty + 1) / (float)l.lut_h;
+    float cv0 = btrc_detmath::cos(PI * v0);
+    float cv1 = btrc_detmath::cos(PI * v1);
+    if(cv0 > cv1)
+    {
+        const float t = cv0;
+        cv0 = cv1;
+        cv1 = t;
+    }
+    const float cos_theta = cv0 + s2 * (cv1 - cv0);
+    const float sin_theta = sqrtf(fmax_(0.0f, 1.0f - cos_theta * cos_theta));
+    const float phi = 2.0f * PI * lerpf(u0, u1, s1);
+    float sp, cp;
+    btrc_detmath::sincos(phi, &sp, &cp);
+    *dir = v3(sin_theta * cp, sin_theta * sp, cos_theta);
+    const float in_tile_pdf = 1.0f / (2.0f * PI * (u1 - u0) * (cv1 - cv0));
+    *pdf = tile_pdf * in_tile_pdf;
+}
+
+// EnvirLightSampler::pdf, env_sampler.cpp:161-187
+BTRC_HD float ibl_pdf_local(const SceneView &sc, const LightRec &l, V3 to_light)
+{
+    const V3 dir = normalize(to_light);
+    const float theta = btrc_detmath::acos(dir.z);
+    const float phi = local_phi(dir);
+    const float u = phi / (2.0f * PI);
+    const float v = theta / PI;
+    const int32_t tx = envir_tile_index(u * (float)l.lut_w, l.lut_w);
+    const int32_t ty = envir_tile_index(v * (float)l.lut_h, l.lut_h);
+    const float tile_pdf = sc.env_probs[ty * l.lut_w + tx];
+    const float u0 = (float)tx / (float)l.lut_w;
+    const float u1 = (float)(tx + 1) / (float)l.lut_w;
+    const float v0 = (float)ty / (float)l.lut_h;
+    const float v1 = (float)(ty + 1) / (float)l.lut_h;
+    const float c0 = btrc_detmath::cos(PI * v0);
+    const float c1 = btrc_detmath::cos(PI * v1);
+    const float in_tile_pdf = 1.0f / fabs_(2.0f * PI * (u1 - u0) * (c1 - c0));
+    return tile_pdf * in_tile_pdf;
+}
+
+// EnvirLight::eval_le: GradientSky::eval_le or IBL::eval_le_inline
+// (ibl.cpp:27-31, the direction taken into the light's frame and normalized)
+BTRC_HD V3 envir_eval_le(const SceneView &sc, const LightRec &l, V3 to_light)
+{
+    if(l.type == LIGHT_IBL)
+        return ibl_eval_local(sc, l, normalize(ibl_to_local(l, to_light)));
+    return sky_eval_le(l, to_light);
+}
+
+// EnvirLight::pdf_li: GradientSky's uniform sphere (gradient_sky.cpp:37-40)
+// or IBL::pdf_li_inline (ibl.cpp:45-49)
+BTRC_HD float envir_pdf_li(const SceneView &sc, const LightRec &l, V3 to_light)
+{
+    if(l.type == LIGHT_IBL)
+        return ibl_pdf_local(sc, l, ibl_to_local(l, to_light));
+    return pdf_sample_sphere_uniform();
+}
+
+// EnvirLight::sample_li: GradientSky::sample_li (gradient_sky.cpp:27-35) or
+// IBL::sample_li_inline (ibl.cpp:33-43; the radiance is looked up at the
+// local direction as sampled)
+BTRC_HD void envir_sample_li(const SceneView &sc, const LightRec &l, float s0, float s1, float s2, V3 *dir, V3 *li,
+                             float *pdf)
+{
+    if(l.type == LIGHT_IBL)
+    {
+        V3 local;
+        ibl_sample_local(sc, l, s0, s1, s2, &local, pdf);
+        *dir = ibl_to_global(l, local);
+        *li = ibl_eval_local(sc, l, local);
+        return;
+    }
+    *dir = sample_sphere_uniform(s0, s1);
+    *li = sky_eval_le(l, *dir);
+    *pdf = pdf_sample_sphere_uniform();
 }
 
 // MeshLight::eval_le, mesh_light.cpp:24-36
@@ -289,13 +473,14 @@ BTRC_HD ShadowSample sample_light_li(
     }
     else
     {
-        // GradientSky::sample_li, gradient_sky.cpp:27-35
-        const V3 dir = sample_sphere_uniform(l0, l1);
+        V3 dir, li;
+        float pdf;
+        envir_sample_li(sc, l, l0, l1, l2, &dir, &li, &pdf);
         r.o = offset_origin ? intersection_offset(ref_pos, ref_nor, dir) : ref_pos;
         r.d = dir;
         r.t1 = MAX_FLOAT;
-        r.li = sky_eval_le(l, dir);
-        r.light_pdf = select_pdf * pdf_sample_sphere_uniform();
+        r.li = li;
+        r.light_pdf = select_pdf * pdf;
     }
     return r;
 }
@@ -312,11 +497,11 @@ BTRC_HD V3 eval_miss_le(const SceneView &sc, V3 d, V3 beta_le, float bsdf_pdf)
     if(sc.envir_light_index < 0)
         return v3s(0.0f);
     const LightRec &l = sc.lights[sc.envir_light_index];
-    const V3 le = sky_eval_le(l, d);
+    const V3 le = envir_eval_le(sc, l, d);
     if(bsdf_pdf < 0.0f)
         return beta_le * le / -bsdf_pdf;
     const float select_light_pdf = light_select_pdf(sc.light_count);
-    const float light_pdf = select_light_pdf * pdf_sample_sphere_uniform();
+    const float light_pdf = select_light_pdf * envir_pdf_li(sc, l, d);
     return beta_le * le / (bsdf_pdf + light_pdf);
 }
 

@@ -192,6 +192,9 @@ struct btrc_ctx
     DevBuf<float4> texels32;
     DevBuf<float> tex_decode;
     int32_t texture_count = 0;
+    DevBuf<float> env_probs;          // the IBL's tile probabilities
+    DevBuf<AliasUnit> env_alias;      // and their alias table
+    std::vector<float> env_probs_host;
     int32_t medium_count = 0;
     int32_t light_count = 0;
     int32_t envir_light_index = -1;
@@ -310,6 +313,8 @@ struct btrc_ctx
         v.media = media.ptr;
         v.voxels = voxels.ptr;
         v.tex = TextureView{ textures.ptr, texels8.ptr, texels32.ptr, tex_decode.ptr };
+        v.env_probs = env_probs.ptr;
+        v.env_alias = env_alias.ptr;
         v.vol = VolumeView{ volumes.ptr, volume_count, voxels.ptr };
         v.world_diagonal = 0.0f; // set per render from the cameras
         return v;
@@ -409,6 +414,53 @@ float build_alias_table(const std::vector<float> &w, std::vector<AliasUnit> &uni
     for(auto i : unders)
         units[i].accept_prob = 1;
     return sum;
+}
+
+// EnvirLightSampler::preprocess (light/env_sampler.cpp:72-122): per tile the
+// mean luminance L and solid angle A (a kernel over the uploaded texture),
+// then p = max(0, A (L - mean L over the sphere)), normalized when the sum
+// exceeds 1e-3, and an alias table over p. A texture of one luminance leaves
+// every p at zero; the alias table then degenerates as the reference's does
+// (NaN acceptance, so every tile samples itself) and the light's pdf is 0.
+void build_envir_table(btrc_ctx *ctx, const LightRec &l)
+{
+    const int32_t w = l.lut_w, h = l.lut_h, n = w * h;
+    DevBuf<float> dlum, darea;
+    dlum.alloc(n);
+    darea.alloc(n);
+    launch_envir_table(ctx->stream, ctx->scene_view().tex, l.texture, w, h, ENVIR_TABLE_SAMPLES, dlum.ptr, darea.ptr);
+    BTRC_HIP_CHECK(hipGetLastError());
+    std::vector<float> lum(n), area(n);
+    BTRC_HIP_CHECK(hipMemcpyAsync(lum.data(), dlum.ptr, n * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+    BTRC_HIP_CHECK(hipMemcpyAsync(area.data(), darea.ptr, n * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+    BTRC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    float lum_area_sum = 0.0f, area_sum = 0.0f;
+    for(int32_t i = 0; i < n; ++i)
+    {
+        lum_area_sum += lum[i] * area[i];
+        area_sum += area[i];
+    }
+    const float avg_lum = lum_area_sum / area_sum;
+    for(int32_t i = 0; i < n; ++i)
+    {
+        const float p = area[i] * (lum[i] - avg_lum);
+        lum[i] = 0.0f < p ? p : 0.0f;
+    }
+    float table_sum = 0.0f;
+    for(float p : lum)
+        table_sum += p;
+    if(table_sum > 1e-3f)
+    {
+        const float ratio = 1 / table_sum;
+        for(float &p : lum)
+            p *= ratio;
+    }
+    std::vector<AliasUnit> units;
+    build_alias_table(lum, units);
+    ctx->env_probs.upload(lum.data(), lum.size(), ctx->stream);
+    ctx->env_alias.upload(units.data(), units.size(), ctx->stream);
+    BTRC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    ctx->env_probs_host = std::move(lum);
 }
 
 // GaussianFilter::commit (film_filter/gaussian.cpp:27-52): separable
@@ -1051,6 +1103,32 @@ void upload_scene(btrc_ctx *ctx, const btrc_scene_desc *sd)
         envir_index = (int32_t)lights.size();
         lights.push_back(l);
     }
+    else if(sd->envir_light.type == BTRC_ENVIR_IBL)
+    {
+        const btrc_envir_light_desc &e = sd->envir_light;
+        if(e.texture < 0 || e.texture >= sd->texture_count)
+            throw InvalidArg("ibl: bad texture index");
+        if(e.lut_width < 1 || e.lut_height < 1 || (int64_t)e.lut_width * e.lut_height > MAX_ENVIR_TABLE)
+            throw InvalidArg("ibl: importance table size out of range");
+        const V3 up = v3(e.up[0], e.up[1], e.up[2]);
+        if(!(dot(up, up) > 0.0f) || !std::isfinite(dot(up, up)))
+            throw InvalidArg("ibl: bad up direction");
+        LightRec l{};
+        l.type = LIGHT_IBL;
+        l.instance = -1;
+        l.geometry = -1;
+        // Frame::from_z (utils/math/frame.cpp:45-57), normalized once more by CFrame(Frame)
+        const Frame f0 = frame_from_z(up);
+        const Frame f = make_frame(f0.x, f0.y, f0.z);
+        l.frame_x[0] = f.x.x; l.frame_x[1] = f.x.y; l.frame_x[2] = f.x.z;
+        l.frame_y[0] = f.y.x; l.frame_y[1] = f.y.y; l.frame_y[2] = f.y.z;
+        l.up[0] = f.z.x; l.up[1] = f.z.y; l.up[2] = f.z.z;
+        l.texture = e.texture;
+        l.lut_w = e.lut_width;
+        l.lut_h = e.lut_height;
+        envir_index = (int32_t)lights.size();
+        lights.push_back(l);
+    }
     else if(sd->envir_light.type != BTRC_ENVIR_NONE)
         throw InvalidArg("unknown environment light type");
 
@@ -1082,6 +1160,14 @@ void upload_scene(btrc_ctx *ctx, const btrc_scene_desc *sd)
     ctx->tex_decode.upload(decode.data(), decode.size(), s);
     BTRC_HIP_CHECK(hipStreamSynchronize(s));
     ctx->texture_count = sd->texture_count;
+    if(envir_index >= 0 && lights[envir_index].type == LIGHT_IBL)
+        build_envir_table(ctx, lights[envir_index]);
+    else
+    {
+        ctx->env_probs.release();
+        ctx->env_alias.release();
+        ctx->env_probs_host.clear();
+    }
 
     ctx->medium_count = R;
     ctx->volume_count = sd->volume_count;
@@ -2286,6 +2372,42 @@ int btrc_hip_eval_texture(btrc_ctx *ctx, int32_t texture, const float *uv, int32
     });
 }
 
+int btrc_hip_eval_envir(btrc_ctx *ctx, const float *dirs, const float *samples, int32_t n, float *out)
+{
+    return guarded(ctx, [&] {
+        if(!ctx->has_scene)
+            throw StateError("environment query before btrc_hip_upload_scene");
+        if(ctx->envir_light_index < 0)
+            throw InvalidArg("the scene has no environment light");
+        if(n < 0 || (n > 0 && (!dirs || !samples || !out)))
+            throw InvalidArg("bad environment query");
+        if(n == 0)
+            return;
+        DevBuf<float> dd, ds, dout;
+        dd.upload(dirs, 3 * (size_t)n, ctx->stream);
+        ds.upload(samples, 3 * (size_t)n, ctx->stream);
+        dout.alloc(12 * (size_t)n);
+        launch_envir_eval(ctx->stream, ctx->scene_view(), dd.ptr, ds.ptr, n, dout.ptr);
+        BTRC_HIP_CHECK(hipGetLastError());
+        BTRC_HIP_CHECK(hipMemcpyAsync(out, dout.ptr, 12 * (size_t)n * sizeof(float), hipMemcpyDeviceToHost,
+                                      ctx->stream));
+        BTRC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int btrc_hip_envir_table(btrc_ctx *ctx, float *probs, int32_t capacity, int32_t *count)
+{
+    return guarded(ctx, [&] {
+        if(!ctx->has_scene)
+            throw StateError("environment query before btrc_hip_upload_scene");
+        if(!count || capacity < 0 || (capacity > 0 && !probs))
+            throw InvalidArg("bad environment table query");
+        const int32_t n = (int32_t)ctx->env_probs_host.size();
+        *count = n;
+        std::memcpy(probs, ctx->env_probs_host.data(), sizeof(float) * (size_t)std::min(n, capacity));
+    });
+}
+
 int btrc_hip_eval_medium(btrc_ctx *ctx, int32_t medium, const btrc_ray *segments, const uint64_t *seeds, int32_t n,
                          float *out)
 {
@@ -2312,7 +2434,7 @@ int btrc_hip_eval_medium(btrc_ctx *ctx, int32_t medium, const btrc_ray *segments
 int btrc_hip_eval_detmath(btrc_ctx *ctx, int32_t fn, const float *x, const float *y, int32_t n, float *out)
 {
     return guarded(ctx, [&] {
-        if(n < 0 || fn < 0 || fn > 6 || (n > 0 && (!x || !y || !out)))
+        if(n < 0 || fn < 0 || fn > 8 || (n > 0 && (!x || !y || !out)))
             throw InvalidArg("bad detmath query");
         if(n == 0)
             return;

@@ -84,6 +84,8 @@ enum : int32_t
 
 // GaussianFilter's importance table (film_filter/gaussian.cpp:7-52)
 constexpr int32_t GAUSSIAN_GRID = 256;
+constexpr int32_t ENVIR_TABLE_SAMPLES = 256;       // IBL::commit's n_samples (light/ibl.cpp:21-25)
+constexpr int64_t MAX_ENVIR_TABLE = 1 << 24;       // IBL importance table tiles
 
 struct FilterConsts
 {
@@ -184,6 +186,11 @@ void launch_trace_batch(hipStream_t s, int grid, int sched, const TraceScene &ts
 // test hooks
 void launch_halton_eval(hipStream_t s, const HaltonConsts &hc, const uint32_t *px, const uint32_t *py,
                         const int32_t *sample_index, int32_t n, int32_t dims, float *out);
+// EnvirLightSampler::preprocess's table kernel: lum and area per tile (w * h each)
+void launch_envir_table(hipStream_t s, const TextureView &tv, int32_t tex, int32_t w, int32_t h, int32_t n_samples,
+                        float *lum, float *area);
+// the environment light's eval_le, pdf_li and sample_li: 12 floats per query
+void launch_envir_eval(hipStream_t s, const SceneView &sv, const float *dirs, const float *sam, int32_t n, float *out);
 void launch_texture_eval(hipStream_t s, const TextureView &tv, int32_t id, const float2 *uv, int32_t n, float4 *out);
 void launch_volume_eval(hipStream_t s, const VolumeView &vv, const float4 *seg, const uint2 *seed, int32_t n,
                         float *out);

@@ -826,6 +826,32 @@ __global__ void k_volume_eval(VolumeView vv, const float4 *seg, const uint2 *see
     o[7] = tr.x; o[8] = tr.y; o[9] = tr.z;
 }
 
+// the importance table's per-tile luminance and solid angle, one thread per tile
+__global__ void k_envir_table(TextureView tv, int32_t tex, int32_t w, int32_t h, int32_t n_samples, float *lum,
+                              float *area)
+{
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if(i < w * h)
+        envir_tile_stats(tv, tex, w, h, i % w, i / w, n_samples, &lum[i], &area[i]);
+}
+
+__global__ void k_envir_eval(SceneView sv, const float *dirs, const float *sam, int32_t n, float *out)
+{
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if(i >= n)
+        return;
+    const LightRec &l = sv.lights[sv.envir_light_index];
+    const V3 d = v3(dirs[3 * i + 0], dirs[3 * i + 1], dirs[3 * i + 2]);
+    const V3 le = envir_eval_le(sv, l, d);
+    const float pdf = envir_pdf_li(sv, l, d);
+    V3 sd, li;
+    float spdf;
+    envir_sample_li(sv, l, sam[3 * i + 0], sam[3 * i + 1], sam[3 * i + 2], &sd, &li, &spdf);
+    const float r[12] = { le.x, le.y, le.z, pdf, sd.x, sd.y, sd.z, spdf, li.x, li.y, li.z, 0.0f };
+    for(int k = 0; k < 12; ++k)
+        out[12 * i + k] = r[k];
+}
+
 __global__ void k_texture_eval(TextureView tv, int32_t id, const float2 *uv, int32_t n, float4 *out)
 {
     const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -888,6 +914,8 @@ __global__ void k_detmath_eval(int32_t fn, const float *x, const float *y, int32
     case 3: r = btrc_detmath::exp(a); break;
     case 4: r = btrc_detmath::pow(a, y[i]); break;
     case 5: r = sqrtf(a); break;
+    case 7: r = btrc_detmath::atan2(a, y[i]); break;
+    case 8: r = btrc_detmath::acos(a); break;
     default: r = a / y[i]; break;
     }
     out[i] = r;
@@ -1026,6 +1054,17 @@ void launch_volume_eval(hipStream_t s, const VolumeView &vv, const float4 *seg, 
                         float *out)
 {
     hipLaunchKernelGGL(k_volume_eval, dim3((n + 127) / 128), dim3(128), 0, s, vv, seg, seed, n, out);
+}
+
+void launch_envir_table(hipStream_t s, const TextureView &tv, int32_t tex, int32_t w, int32_t h, int32_t n_samples,
+                        float *lum, float *area)
+{
+    hipLaunchKernelGGL(k_envir_table, dim3((w * h + 63) / 64), dim3(64), 0, s, tv, tex, w, h, n_samples, lum, area);
+}
+
+void launch_envir_eval(hipStream_t s, const SceneView &sv, const float *dirs, const float *sam, int32_t n, float *out)
+{
+    hipLaunchKernelGGL(k_envir_eval, dim3((n + 127) / 128), dim3(128), 0, s, sv, dirs, sam, n, out);
 }
 
 void launch_texture_eval(hipStream_t s, const TextureView &tv, int32_t id, const float2 *uv, int32_t n, float4 *out)

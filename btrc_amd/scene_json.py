@@ -11,7 +11,7 @@ vectors and spectra from one or three numbers, angles as {"deg"} or
 scale or arrays of them applied first to last), and the creators of the
 registered object types (builtin/register.cpp) that the wavefront path
 tracer renders: triangle_mesh geometry (OBJ), diffuse / disney / invisible /
-black materials, mesh_light and gradient_sky lights, homogeneous /
+black materials, mesh_light, gradient_sky and ibl lights, homogeneous /
 hetergeneous / transform media, volume primitives, constant and array
 textures, the pinhole camera, box / gaussian film filters, the wfpt and pt
 renderers, and the aces_tonemap / save_to_image post processors. Other
@@ -315,9 +315,56 @@ def load_obj(path, unit_cube=False):
     return pos, nor, uvs
 
 
+def _load_hdr(path):
+    """a Radiance RGBE file as (H, W, 4) float32 with alpha 1: flat or
+    run-length scanlines, each texel m * 2^(e - 136) (utils/image.cpp:150-174
+    over stb_image's HDR reader)"""
+    data = open(path, "rb").read()
+    end = data.find(b"\n\n")
+    if not data.startswith(b"#?") or end < 0:
+        raise SceneFileError(f"{path}: not a Radiance HDR file")
+    eol = data.find(b"\n", end + 2)
+    res = data[end + 2:eol].split()
+    if len(res) != 4 or res[0] != b"-Y" or res[2] != b"+X":
+        raise SceneFileError(f"{path}: unsupported HDR orientation")
+    h, w = int(res[1]), int(res[3])
+    pos = eol + 1
+    out = np.zeros((h, w, 4), np.uint8)
+    for y in range(h):
+        head = data[pos:pos + 4]
+        if 8 <= w < 32768 and len(head) == 4 and head[0] == 2 and head[1] == 2 and (head[2] << 8 | head[3]) == w:
+            pos += 4
+            for c in range(4):
+                x = 0
+                while x < w:
+                    n = data[pos]
+                    pos += 1
+                    if n > 128:
+                        out[y, x:x + n - 128, c] = data[pos]
+                        pos += 1
+                        x += n - 128
+                    else:
+                        out[y, x:x + n, c] = np.frombuffer(data, np.uint8, n, pos)
+                        pos += n
+                        x += n
+        else:
+            out[y] = np.frombuffer(data, np.uint8, 4 * w, pos).reshape(w, 4)
+            pos += 4 * w
+    e = out[..., 3].astype(np.int32)
+    f = np.where(e > 0, np.ldexp(np.float32(1.0), e - 136).astype(np.float32), np.float32(0.0))
+    rgba = np.ones((h, w, 4), np.float32)
+    rgba[..., :3] = out[..., :3].astype(np.float32) * f[..., None]
+    return rgba
+
+
 def load_image(path):
-    """an 8-bit image as (H, W, 4) uint8, the Array2D texels of a PNG / JPG
-    (utils/image.cpp:115-148: RGB gets alpha 255)"""
+    """the Array2D texels of an image file (ImageDynamic::load,
+    utils/image.cpp:299-348): PNG / JPG as (H, W, 4) uint8 (RGB gets alpha
+    255), HDR as (H, W, 4) float32"""
+    if os.path.splitext(path)[1].lower() == ".hdr":
+        return _load_hdr(path)
+    if os.path.splitext(path)[1].lower() == ".exr":
+        raise SceneFileError(f"{path}: EXR images cannot be loaded (as in the reference)")
     from PIL import Image
     im = Image.open(path)
     if im.mode not in ("RGB", "RGBA", "L", "LA", "P"):
@@ -519,9 +566,20 @@ class _Builder:
             self.s.add_volume(vec3(v["o"]), vec3(v["x"]), vec3(v["y"]), vec3(v["z"]), sigma, albedo)
         if "envir_light" in scene:
             el = scene["envir_light"]
-            if el.get("type") != "gradient_sky":
+            if el.get("type") == "gradient_sky":
+                self.s.set_gradient_sky(vec3(el["up"]), spectrum(el["lower"]), spectrum(el["upper"]))
+            elif el.get("type") == "ibl":
+                # IBLCreator, light/ibl.cpp:80-91
+                kind, tex = self.texture2d(el["tex"])
+                if kind == "const":
+                    texel = np.ones((1, 1, 4), np.float32)
+                    texel[0, 0, :3] = tex
+                    tex = self.s.add_texture(texel, filter="point")
+                up = tuple(vec3(el["up"])) if "up" in el else (0.0, 0.0, 1.0)
+                self.s.set_ibl(tex, up=up, lut_width=int(el.get("lut_width", 400)),
+                               lut_height=int(el.get("lut_height", 200)))
+            else:
                 raise SceneFileError(f"environment light type '{el.get('type')}' is not supported by this renderer")
-            self.s.set_gradient_sky(vec3(el["up"]), spectrum(el["lower"]), spectrum(el["upper"]))
 
         out = self.out
         out.width, out.height = int(root["width"]), int(root["height"])

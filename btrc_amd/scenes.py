@@ -99,7 +99,7 @@ class Scene:
         self.materials = []    # MaterialDesc kwargs
         self.instances = []    # dicts
         self.media = []        # dicts
-        self.envir = None      # (up, lower, upper)
+        self.envir = None      # ("sky", up, lower, upper) or ("ibl", texture, up, lut_width, lut_height)
         self.accel = _abi.ACCEL_FLAT
         self.textures = []     # dicts
         self.volumes = []      # dicts
@@ -198,7 +198,15 @@ class Scene:
         return len(self.instances) - 1
 
     def set_gradient_sky(self, up, lower, upper):
-        self.envir = (up, lower, upper)
+        self.envir = ("sky", up, lower, upper)
+
+    def set_ibl(self, texture, up=(0.0, 0.0, 1.0), lut_width=400, lut_height=200):
+        """an image-based environment light (IBLCreator, light/ibl.cpp:80-91):
+        `texture` is an index from add_texture, looked up at (phi / 2pi,
+        theta / pi) of each direction in the frame whose z axis is `up`"""
+        if not 0 <= texture < len(self.textures):
+            raise ValueError("set_ibl: unknown texture")
+        self.envir = ("ibl", int(texture), up, int(lut_width), int(lut_height))
 
     def world_box(self):
         """(lo, 1 / (hi - lo)) over the world-space triangle vertices in
@@ -312,12 +320,19 @@ class Scene:
                 d.albedo_grid = v["albedo_grid"].ctypes.data_as(fp)
         sd.volume_count = len(self.volumes)
         sd.volumes = vd
-        if self.envir is not None:
-            up, lower, upper = self.envir
+        if self.envir is not None and self.envir[0] == "sky":
+            _, up, lower, upper = self.envir
             sd.envir_light.type = _abi.ENVIR_GRADIENT_SKY
             sd.envir_light.up = _abi.F3(*map(float, up))
             sd.envir_light.lower = _abi.F3(*map(float, lower))
             sd.envir_light.upper = _abi.F3(*map(float, upper))
+        elif self.envir is not None:
+            _, tex, up, lw, lh = self.envir
+            sd.envir_light.type = _abi.ENVIR_IBL
+            sd.envir_light.up = _abi.F3(*map(float, up))
+            sd.envir_light.texture = tex
+            sd.envir_light.lut_width = lw
+            sd.envir_light.lut_height = lh
         else:
             sd.envir_light.type = _abi.ENVIR_NONE
         holder.update(gd=gd, md=md, idesc=idesc, med=med, td=td, vd=vd, geoms=self.geometries, media=self.media,
@@ -553,6 +568,27 @@ def cornell_media(scale=30.0, grid=None, g=0.6, albedo=0.8, grid_n=128, seed=3):
     e = np.array(cam.eye) * scale
     t = np.array(cam.dst) * scale
     return s, Camera(tuple(e), tuple(t), cam.up, cam.fov_y_deg)
+
+
+def sky_texture(width=128, height=64, seed=0, sun=(0.3, 0.2), sun_radius=0.04, sun_radiance=40.0):
+    """(height, width, 4) float32 latitude-longitude radiance map for an IBL:
+    a blue-to-white sky over a darker ground, seeded value noise, and a
+    bright disc (the sun) centred at texture coordinate `sun`"""
+    rng = np.random.default_rng(seed)
+    v = (np.arange(height) + 0.5) / height
+    u = (np.arange(width) + 0.5) / width
+    uu, vv = np.meshgrid(u, v)
+    sky = np.where(vv[..., None] < 0.5,
+                   np.array([0.35, 0.55, 1.0]) * (0.6 + 0.8 * vv[..., None]),
+                   np.array([0.25, 0.2, 0.15]) * (1.4 - vv[..., None]))
+    noise = rng.uniform(0.8, 1.2, (height // 4 + 1, width // 4 + 1))
+    sky = sky * np.kron(noise, np.ones((4, 4)))[:height, :width, None]
+    du = np.minimum(np.abs(uu - sun[0]), 1.0 - np.abs(uu - sun[0]))
+    disc = du ** 2 + (vv - sun[1]) ** 2 < sun_radius ** 2
+    sky[disc] = sun_radiance * np.array([1.0, 0.9, 0.7])
+    out = np.ones((height, width, 4), np.float32)
+    out[..., :3] = sky
+    return out
 
 
 def sphere_field(instances=4096, lights=64, seed=2, extent=100.0, envir=(0.5, 0.5, 0.5)):

@@ -81,7 +81,8 @@ enum
 enum
 {
     BTRC_ENVIR_NONE = 0,
-    BTRC_ENVIR_GRADIENT_SKY = 1 /* builtin/light/gradient_sky.cpp        */
+    BTRC_ENVIR_GRADIENT_SKY = 1, /* builtin/light/gradient_sky.cpp       */
+    BTRC_ENVIR_IBL = 2           /* builtin/light/ibl.cpp, env_sampler.cpp */
 };
 
 enum
@@ -207,9 +208,16 @@ typedef struct btrc_volume_desc
 typedef struct btrc_envir_light_desc
 {
     int32_t type;   /* BTRC_ENVIR_* */
-    float up[3];
-    float lower[3];
-    float upper[3];
+    float up[3];    /* both types: the direction that maps to the sky's top */
+    float lower[3]; /* GRADIENT_SKY */
+    float upper[3]; /* GRADIENT_SKY */
+    /* IBL (IBLCreator, light/ibl.cpp:80-91): radiance texture (an index
+     * into btrc_scene_desc.textures) looked up at (phi / 2pi, theta / pi) of
+     * the direction in the frame whose z axis is `up`, and the resolution of
+     * the importance table over that (u, v) square (the creator's defaults
+     * are 400 x 200) */
+    int32_t texture;
+    int32_t lut_width, lut_height;
 } btrc_envir_light_desc;
 
 typedef struct btrc_scene_desc
@@ -438,13 +446,21 @@ int btrc_hip_eval_volumes(btrc_ctx *ctx, const btrc_ray *segments, const uint64_
 /* texture `texture` of the current scene sampled at n (u, v) pairs; out
  * gets 4 floats per pair */
 int btrc_hip_eval_texture(btrc_ctx *ctx, int32_t texture, const float *uv, int32_t n, float *out);
+/* the current scene's environment light at n world directions (3 floats
+ * each) and n sample triples (3 floats each); out gets 12 floats per query:
+ * eval_le rgb, pdf_li, then sample_li's direction xyz, pdf and radiance rgb,
+ * then 0 (EnvirLight::eval_le / pdf_li / sample_li, core/light.h:90-132) */
+int btrc_hip_eval_envir(btrc_ctx *ctx, const float *dirs, const float *samples, int32_t n, float *out);
+/* the IBL's importance table (tile probabilities, row-major lut_height x
+ * lut_width; EnvirLightSampler::preprocess, env_sampler.cpp:72-122); *count
+ * gets the table size, at most `capacity` floats are written */
+int btrc_hip_envir_table(btrc_ctx *ctx, float *probs, int32_t capacity, int32_t *count);
 /* medium id (rank by descending priority) along segments o -> o + d * tmax,
  * one PCG32 stream per segment (seeded like a shadow ray's): out holds 10
  * floats per segment: scattered (0/1), position xyz, albedo xyz, tr xyz */
-
 int btrc_hip_eval_medium(btrc_ctx *ctx, int32_t medium, const btrc_ray *segments, const uint64_t *seeds, int32_t n,
                          float *out);
-/* fn: 0 sin, 1 cos, 2 log, 3 exp, 4 pow(x, y), 5 sqrt, 6 x / y */
+/* fn: 0 sin, 1 cos, 2 log, 3 exp, 4 pow(x, y), 5 sqrt, 6 x / y, 7 atan2(x, y), 8 acos */
 int btrc_hip_eval_detmath(btrc_ctx *ctx, int32_t fn, const float *x, const float *y, int32_t n, float *out);
 
 #ifdef __cplusplus

@@ -442,12 +442,53 @@ static uint64_t shadow_seed(const Smp::Packed &p)
 
 struct Light
 {
-    int type; // 0 mesh, 1 sky
+    int type; // 0 mesh, 1 sky, 2 IBL
     int inst, geo;
     float scale;
     Vec intensity;
     Vec up, lower, upper;
+    // IBL (light/ibl.cpp): frame, texture, importance table
+    Frm fr;
+    int tex = -1, lw = 0, lh = 0;
+    std::vector<float> probs, accept;
+    std::vector<uint32_t> alias;
 };
+
+// AliasTable::initialize, utils/math/alias.cpp:13-58 (sum in index order);
+// returns the sum of the weights
+static float alias_build(const std::vector<float> &w, std::vector<float> &accept, std::vector<uint32_t> &alias)
+{
+    float sum = 0.0f;
+    for(float a : w)
+        sum += a;
+    const uint32_t n = (uint32_t)w.size();
+    accept.resize(n);
+    alias.resize(n);
+    if(n == 0)
+        return sum;
+    const float ratio = (float)n / sum;
+    std::vector<uint32_t> ov, un;
+    for(uint32_t i = 0; i < n; ++i)
+    {
+        accept[i] = w[i] * ratio;
+        alias[i] = i;
+        if(accept[i] > 1) ov.push_back(i);
+        else if(accept[i] < 1) un.push_back(i);
+    }
+    while(!ov.empty() && !un.empty())
+    {
+        const uint32_t o = ov.back(), u = un.back();
+        ov.pop_back();
+        un.pop_back();
+        accept[o] -= 1 - accept[u];
+        alias[u] = o;
+        if(accept[o] > 1) ov.push_back(o);
+        else if(accept[o] < 1) un.push_back(o);
+    }
+    for(auto i : ov) accept[i] = 1;
+    for(auto i : un) accept[i] = 1;
+    return sum;
+}
 
 struct Geo
 {
@@ -586,6 +627,8 @@ static void build_bvh(Scene &s)
     }
 }
 
+static void ibl_table(const Scene &s, Light &l);
+
 static Scene *make_scene(const btrc_scene_desc *sd)
 {
     Scene *s = new Scene;
@@ -649,38 +692,7 @@ static Scene *make_scene(const btrc_scene_desc *sd)
             s->lca.push_back(ca);
             area[t] = 0.5f * len(cross(ba, ca)); // utils/math/triangle.h:25-28
         }
-        // AliasTable::initialize, utils/math/alias.cpp:13-58
-        float sum = 0.0f;
-        for(float a : area)
-            sum += a;
-        geo.total_area = sum;
-        const uint32_t n = (uint32_t)area.size();
-        geo.accept.resize(n);
-        geo.alias.resize(n);
-        if(n)
-        {
-            const float ratio = (float)n / sum;
-            std::vector<uint32_t> ov, un;
-            for(uint32_t i = 0; i < n; ++i)
-            {
-                geo.accept[i] = area[i] * ratio;
-                geo.alias[i] = i;
-                if(geo.accept[i] > 1) ov.push_back(i);
-                else if(geo.accept[i] < 1) un.push_back(i);
-            }
-            while(!ov.empty() && !un.empty())
-            {
-                const uint32_t o = ov.back(), u = un.back();
-                ov.pop_back();
-                un.pop_back();
-                geo.accept[o] -= 1 - geo.accept[u];
-                geo.alias[u] = o;
-                if(geo.accept[o] > 1) ov.push_back(o);
-                else if(geo.accept[o] < 1) un.push_back(o);
-            }
-            for(auto i : ov) geo.accept[i] = 1;
-            for(auto i : un) geo.accept[i] = 1;
-        }
+        geo.total_area = alias_build(area, geo.accept, geo.alias);
         s->geos.push_back(std::move(geo));
         off += (uint32_t)g.triangle_count;
     }
@@ -798,6 +810,21 @@ static Scene *make_scene(const btrc_scene_desc *sd)
         l.upper = mk(sd->envir_light.upper[0], sd->envir_light.upper[1], sd->envir_light.upper[2]);
         s->env = (int)s->lights.size();
         s->lights.push_back(l);
+    }
+    else if(sd->envir_light.type == BTRC_ENVIR_IBL)
+    {
+        const btrc_envir_light_desc &e = sd->envir_light;
+        Light l{};
+        l.type = 2;
+        // Frame::from_z (utils/math/frame.cpp:45-57), normalized again by CFrame(Frame) (cframe.cpp:11-22)
+        const Frm f0 = from_z(mk(e.up[0], e.up[1], e.up[2]));
+        l.fr = frm(f0.x, f0.y, f0.z);
+        l.tex = e.texture;
+        l.lw = e.lut_width;
+        l.lh = e.lut_height;
+        s->env = (int)s->lights.size();
+        s->lights.push_back(l);
+        ibl_table(*s, s->lights.back());
     }
     build_bvh(*s);
     for(const std::vector<Vec> *w : { &s->w0, &s->w1, &s->w2 })
@@ -1155,6 +1182,128 @@ static F4 tex_sample(const Scene &s, int id, float u, float v)
     const float w00 = (1.0f - a) * (1.0f - b), w10 = a * (1.0f - b), w01 = (1.0f - a) * b, w11 = a * b;
     return F4{ ((w00 * p.x + w10 * q.x) + w01 * r.x) + w11 * w.x, ((w00 * p.y + w10 * q.y) + w01 * r.y) + w11 * w.y,
                ((w00 * p.z + w10 * q.z) + w01 * r.z) + w11 * w.z, ((w00 * p.w + w10 * q.w) + w01 * r.w) + w11 * w.w };
+}
+
+// ------------------------------------------------------------------ IBL
+
+static float phi_of(Vec w) // local_angle::phi, utils/local_angle.h:63-72
+{
+    float r = 0.0f;
+    if(w.x != 0.0f || w.y != 0.0f)
+    {
+        r = dm::atan2(w.y, w.x);
+        if(r < 0.0f)
+            r = r + 2.0f * PI;
+    }
+    return r;
+}
+
+static float theta_of(Vec w) { return dm::acos(fmn(fmx(w.z, -1.0f), 1.0f)); } // local_angle.h:45-49
+
+static float hammersley_ri(uint32_t bits) // utils/math/hammersley.h:7-16
+{
+    bits = (bits << 16u) | (bits >> 16u);
+    bits = ((bits & 0x55555555u) << 1u) | ((bits & 0xAAAAAAAAu) >> 1u);
+    bits = ((bits & 0x33333333u) << 2u) | ((bits & 0xCCCCCCCCu) >> 2u);
+    bits = ((bits & 0x0F0F0F0Fu) << 4u) | ((bits & 0xF0F0F0F0u) >> 4u);
+    bits = ((bits & 0x00FF00FFu) << 8u) | ((bits & 0xFF00FF00u) >> 8u);
+    return (float)bits * 2.3283064365386963e-10f;
+}
+
+static float lerpf(float a, float b, float t) { return a * (1.0f - t) + b * t; } // utils/cmath/cscalar.h:41-44
+
+// EnvirLightSampler::preprocess, light/env_sampler.cpp:16-122, with
+// n_samples = 256 (IBL::commit, ibl.cpp:21-25)
+static void ibl_table(const Scene &s, Light &l)
+{
+    const int w = l.lw, h = l.lh, n = w * h, ns = 256;
+    std::vector<float> lm(n), area(n);
+    for(int yi = 0; yi < h; ++yi)
+        for(int xi = 0; xi < w; ++xi)
+        {
+            const float x0 = (float)xi / (float)w, x1 = (float)(xi + 1) / (float)w;
+            const float y0 = (float)yi / (float)h, y1 = (float)(yi + 1) / (float)h;
+            float sum = 0.0f;
+            for(int k = 0; k < ns; ++k)
+            {
+                // hammersley2d(k, ns) stretched to [-0.1, 1.1)
+                const float lx = 1.2f * ((float)k / (float)ns) - 0.1f;
+                const float ly = 1.2f * hammersley_ri((uint32_t)k) - 0.1f;
+                const F4 c = tex_sample(s, l.tex, lerpf(x0, x1, lx), lerpf(y0, y1, ly));
+                sum = sum + lum(mk(c.x, c.y, c.z));
+            }
+            lm[yi * w + xi] = sum / (float)ns;
+            area[yi * w + xi] = std::fabs(2.0f * PI * (x1 - x0) * (dm::cos(PI * y1) - dm::cos(PI * y0)));
+        }
+    float la = 0.0f, as = 0.0f;
+    for(int i = 0; i < n; ++i)
+    {
+        la += lm[i] * area[i];
+        as += area[i];
+    }
+    const float avg = la / as;
+    for(int i = 0; i < n; ++i)
+        lm[i] = std::max(0.0f, area[i] * (lm[i] - avg));
+    float ts = 0.0f;
+    for(float v : lm)
+        ts += v;
+    if(ts > 1e-3f)
+    {
+        const float ratio = 1 / ts;
+        for(float &v : lm)
+            v *= ratio;
+    }
+    alias_build(lm, l.accept, l.alias);
+    l.probs = lm;
+}
+
+static Vec ibl_local_le(const Scene &s, const Light &l, Vec w) // IBL::eval_local, ibl.cpp:71-78
+{
+    const float u = fmn(fmx((1.0f / (2.0f * PI)) * phi_of(w), 0.0f), 1.0f);
+    const float v = fmn(fmx((1.0f / PI) * theta_of(w), 0.0f), 1.0f);
+    const F4 c = tex_sample(s, l.tex, u, v);
+    return mk(c.x, c.y, c.z);
+}
+
+static int tile_of(float x, int n) // i32(floor(x)) with NaN -> 0, clamped to [0, n - 1]
+{
+    const float f = std::floor(x);
+    if(!(f >= 0.0f))
+        return 0;
+    return f >= (float)(n - 1) ? n - 1 : (int)f;
+}
+
+static float ibl_pdf_local(const Light &l, Vec to) // EnvirLightSampler::pdf, env_sampler.cpp:161-187
+{
+    const Vec d = nrm(to);
+    const float theta = dm::acos(d.z);
+    const float u = phi_of(d) / (2.0f * PI), v = theta / PI;
+    const int tx = tile_of(u * (float)l.lw, l.lw), ty = tile_of(v * (float)l.lh, l.lh);
+    const float u0 = (float)tx / (float)l.lw, u1 = (float)(tx + 1) / (float)l.lw;
+    const float v0 = (float)ty / (float)l.lh, v1 = (float)(ty + 1) / (float)l.lh;
+    const float c0 = dm::cos(PI * v0), c1 = dm::cos(PI * v1);
+    return l.probs[ty * l.lw + tx] * (1.0f / std::fabs(2.0f * PI * (u1 - u0) * (c1 - c0)));
+}
+
+// EnvirLightSampler::sample, env_sampler.cpp:124-159
+static void ibl_sample_local(const Light &l, float s0, float s1, float s2, Vec &dir, float &pdf)
+{
+    const uint32_t n = (uint32_t)(l.lw * l.lh);
+    const float nu = (float)n * s0;
+    uint32_t k = (uint32_t)nu;
+    k = k < n - 1 ? k : n - 1;
+    const uint32_t tile = (nu - (float)k) <= l.accept[k] ? k : l.alias[k];
+    const int ty = (int)tile / l.lw, tx = (int)tile % l.lw;
+    const float u0 = (float)tx / (float)l.lw, u1 = (float)(tx + 1) / (float)l.lw;
+    const float v0 = (float)ty / (float)l.lh, v1 = (float)(ty + 1) / (float)l.lh;
+    float cv0 = dm::cos(PI * v0), cv1 = dm::cos(PI * v1);
+    if(cv0 > cv1)
+        std::swap(cv0, cv1);
+    const float ct = cv0 + s2 * (cv1 - cv0);
+    const float st = std::sqrt(fmx(0.0f, 1.0f - ct * ct));
+    const float phi = 2.0f * PI * lerpf(u0, u1, s1);
+    dir = mk(st * dm::cos(phi), st * dm::sin(phi), ct);
+    pdf = l.probs[tile] * (1.0f / (2.0f * PI * (u1 - u0) * (cv1 - cv0)));
 }
 
 // ------------------------------------------------------------------ surfaces
@@ -1677,6 +1826,33 @@ static Vec sky_le(const Light &l, Vec to) // gradient_sky.cpp:20-25
     return add(scl(l.lower, 1.0f - s), scl(l.upper, s));
 }
 
+// EnvirLight::eval_le / pdf_li / sample_li: GradientSky (gradient_sky.cpp:20-40)
+// or IBL (ibl.cpp:27-49)
+static Vec env_le(const Scene &s, const Light &l, Vec to)
+{
+    return l.type == 2 ? ibl_local_le(s, l, nrm(g2l(l.fr, to))) : sky_le(l, to);
+}
+
+static float env_pdf(const Light &l, Vec to)
+{
+    return l.type == 2 ? ibl_pdf_local(l, g2l(l.fr, to)) : 1.0f / (4.0f * PI);
+}
+
+static void env_sample(const Scene &s, const Light &l, float s0, float s1, float s2, Vec &dir, Vec &li, float &pdf)
+{
+    if(l.type == 2)
+    {
+        Vec local;
+        ibl_sample_local(l, s0, s1, s2, local, pdf);
+        dir = l2g(l.fr, local);
+        li = ibl_local_le(s, l, local);
+        return;
+    }
+    dir = sphere_uniform(s0, s1);
+    li = sky_le(l, dir);
+    pdf = 1.0f / (4.0f * PI);
+}
+
 struct Shadow
 {
     bool ok;
@@ -1727,12 +1903,14 @@ static Shadow sample_light(const Scene &s, Vec ref, Vec nor, bool offset, float 
     }
     else
     {
-        const Vec dir = sphere_uniform(l0, l1);
+        Vec dir, li;
+        float pdf;
+        env_sample(s, l, l0, l1, l2, dir, li, pdf);
         r.o = offset ? offset_dir(ref, nor, dir) : ref;
         r.d = dir;
         r.t1 = FMAX;
-        r.li = sky_le(l, dir);
-        r.pdf = sel_pdf * (1.0f / (4.0f * PI));
+        r.li = li;
+        r.pdf = sel_pdf * pdf;
     }
     return r;
 }
@@ -2267,12 +2445,12 @@ static void trace_path(const Scene &s, const Params &p, uint64_t path_index, std
             Vec sp = mk(0, 0, 0);
             if(s.env >= 0)
             {
-                const Vec le = sky_le(s.lights[s.env], d);
+                const Vec le = env_le(s, s.lights[s.env], d);
                 if(bsdf_pdf < 0.0f)
                     sp = dvs(mul(beta_le, le), -bsdf_pdf);
                 else
                 {
-                    const float lp = (1.0f / (float)s.lights.size()) * (1.0f / (4.0f * PI));
+                    const float lp = (1.0f / (float)s.lights.size()) * env_pdf(s.lights[s.env], d);
                     sp = dvs(mul(beta_le, le), bsdf_pdf + lp);
                 }
             }
@@ -2834,6 +3012,38 @@ void oracle_pcg32(uint64_t seq, uint64_t offset, uint64_t skip, int32_t n, uint3
         out[i] = r.next();
 }
 
+// the environment light at n directions / sample triples: 12 floats each
+// (le rgb, pdf_li, sampled direction xyz, pdf, radiance rgb, 0)
+void oracle_envir_eval(void *sp, const float *dirs, const float *sam, int32_t n, float *out)
+{
+    const orc::Scene &s = *static_cast<orc::Scene *>(sp);
+    const orc::Light &l = s.lights[s.env];
+    for(int32_t i = 0; i < n; ++i)
+    {
+        const orc::Vec d = orc::mk(dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]);
+        const orc::Vec le = orc::env_le(s, l, d);
+        const float pdf = orc::env_pdf(l, d);
+        orc::Vec sd, li;
+        float spdf;
+        orc::env_sample(s, l, sam[3 * i], sam[3 * i + 1], sam[3 * i + 2], sd, li, spdf);
+        const float r[12] = { le.x, le.y, le.z, pdf, sd.x, sd.y, sd.z, spdf, li.x, li.y, li.z, 0.0f };
+        for(int k = 0; k < 12; ++k)
+            out[12 * i + k] = r[k];
+    }
+}
+
+// the IBL's tile probabilities; returns the table size
+int32_t oracle_envir_table(void *sp, float *probs, int32_t capacity)
+{
+    const orc::Scene &s = *static_cast<orc::Scene *>(sp);
+    if(s.env < 0)
+        return 0;
+    const std::vector<float> &p = s.lights[s.env].probs;
+    for(int32_t i = 0; i < (int32_t)p.size() && i < capacity; ++i)
+        probs[i] = p[i];
+    return (int32_t)p.size();
+}
+
 void oracle_detmath(int32_t fn, const float *x, const float *y, int32_t n, float *out)
 {
     namespace dm = btrc_detmath;
@@ -2849,6 +3059,8 @@ void oracle_detmath(int32_t fn, const float *x, const float *y, int32_t n, float
         case 3: r = dm::exp(a); break;
         case 4: r = dm::pow(a, y[i]); break;
         case 5: r = std::sqrt(a); break;
+        case 7: r = dm::atan2(a, y[i]); break;
+        case 8: r = dm::acos(a); break;
         default: r = a / y[i]; break;
         }
         out[i] = r;

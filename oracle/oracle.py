@@ -39,6 +39,9 @@ def lib():
         L.oracle_medium_eval.argtypes = [vp, C.c_int32, vp, vp, C.c_int32, vp]
         L.oracle_texture_eval.argtypes = [vp, C.c_int32, vp, C.c_int32, vp]
         L.oracle_volume_eval.argtypes = [vp, vp, vp, C.c_int32, vp]
+        L.oracle_envir_eval.argtypes = [vp, vp, vp, C.c_int32, vp]
+        L.oracle_envir_table.restype = C.c_int32
+        L.oracle_envir_table.argtypes = [vp, vp, C.c_int32]
         L.oracle_hg.argtypes = [C.c_float, vp, C.c_int32, vp]
         L.oracle_tex3.restype = C.c_float
         L.oracle_tex3.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32, C.c_float, C.c_float, C.c_float]
@@ -103,6 +106,21 @@ class OracleScene:
         uv = np.ascontiguousarray(uv, dtype=np.float32).reshape(-1, 2)
         out = np.empty((len(uv), 4), dtype=np.float32)
         lib().oracle_texture_eval(self.h, texture, uv.ctypes.data, len(uv), out.ctypes.data)
+        return out
+
+    def envir_eval(self, dirs, samples):
+        """(n, 12) float32 like Context.eval_envir"""
+        d = np.ascontiguousarray(dirs, dtype=np.float32).reshape(-1, 3)
+        sm = np.ascontiguousarray(samples, dtype=np.float32).reshape(-1, 3)
+        out = np.empty((len(d), 12), dtype=np.float32)
+        lib().oracle_envir_eval(self.h, d.ctypes.data, sm.ctypes.data, len(d), out.ctypes.data)
+        return out
+
+    def envir_table(self):
+        """the IBL's tile probabilities"""
+        n = lib().oracle_envir_table(self.h, None, 0)
+        out = np.empty(n, dtype=np.float32)
+        lib().oracle_envir_table(self.h, out.ctypes.data, n)
         return out
 
     def medium_eval(self, medium, segments, seeds):

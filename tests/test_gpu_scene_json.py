@@ -10,6 +10,8 @@ import btrc_amd
 from btrc_amd import scene_json as sj
 from oracle import oracle
 
+from btrc_amd import scenes
+from test_scene_json import ibl_scene_file
 from util import render_params
 
 pytestmark = pytest.mark.gpu
@@ -19,7 +21,16 @@ SRC = os.path.join(HERE, "golden", "scene_json")
 
 
 def test_scene_file_per_pixel_parity(ctx):
-    sf = sj.load(os.path.join(SRC, "box.json"))
+    check_scene_file(ctx, sj.load(os.path.join(SRC, "box.json")))
+
+
+def test_ibl_scene_file_per_pixel_parity(ctx, tmp_path):
+    sj.save_image(str(tmp_path / "sky.hdr"), scenes.sky_texture(32, 16, seed=2))
+    check_scene_file(ctx, sj.load(ibl_scene_file(tmp_path, '{ "type": "array", "filename": "' +
+                                                 str(tmp_path / "sky.hdr") + '" }')))
+
+
+def check_scene_file(ctx, sf):
     r = btrc_amd.WavefrontPathTracer(ctx, **sf.renderer)
     r.set_scene(sf.scene)
     r.set_camera(sf.camera)

@@ -10,6 +10,7 @@ import numpy as np
 import pytest
 
 from btrc_amd import scene_json as sj
+from btrc_amd import scenes
 from oracle import oracle
 
 from util import render_params
@@ -83,6 +84,40 @@ def test_load_box_scene():
     p = render_params(sf.width, sf.height, 2, min_depth=3, max_depth=8)
     img, e, sh = oracle.OracleScene(s).render(sf.camera, p)
     assert np.isfinite(img).all() and img.sum() > 0 and e > 0 and sh > 0
+
+
+def ibl_scene_file(tmp_path, tex):
+    """the box scene under an image-based environment light"""
+    src = open(BOX).read().replace("$env{scene_directory}", os.path.dirname(BOX))
+    src = src.replace('"volumes": [', '"envir_light": { "type": "ibl", "tex": ' + tex +
+                      ', "up": [ 0, 1, 0 ], "lut_width": 32, "lut_height": 16 },\n    "volumes": [')
+    (tmp_path / "materials.json").write_text(open(os.path.join(os.path.dirname(BOX), "materials.json")).read())
+    path = tmp_path / "ibl.json"
+    path.write_text(src)
+    return str(path)
+
+
+def test_ibl_light(tmp_path):
+    sky = scenes.sky_texture(32, 16, seed=2)
+    sj.save_image(str(tmp_path / "sky.hdr"), sky)
+    sf = sj.load(ibl_scene_file(tmp_path, '{ "type": "array", "filename": "' + str(tmp_path / "sky.hdr") + '" }'))
+    kind, tex, up, lw, lh = sf.scene.envir
+    assert kind == "ibl" and up == (0.0, 1.0, 0.0) and (lw, lh) == (32, 16)
+    t = sf.scene.textures[tex]["texels"]
+    assert t.shape == (16, 32, 4) and t.dtype == np.float32
+    # RGBE keeps 8 bits of mantissa under the texel's largest channel
+    assert (np.abs(t[..., :3] - sky[..., :3]) <= sky[..., :3].max(axis=2, keepdims=True) / 128).all()
+    p = render_params(sf.width, sf.height, 2, min_depth=3, max_depth=8)
+    img, e, sh = oracle.OracleScene(sf.scene).render(sf.camera, p)
+    assert np.isfinite(img).all() and img.sum() > 0
+    # a constant texture becomes a one-texel texture; the defaults are up = +z and a 400 x 200 table
+    src = open(ibl_scene_file(tmp_path, '[ 0.5, 0.25, 1 ]')).read()
+    src = src.replace(', "up": [ 0, 1, 0 ], "lut_width": 32, "lut_height": 16', "")
+    (tmp_path / "const.json").write_text(src)
+    sf = sj.load(str(tmp_path / "const.json"))
+    kind, tex, up, lw, lh = sf.scene.envir
+    assert up == (0.0, 0.0, 1.0) and (lw, lh) == (400, 200)
+    np.testing.assert_array_equal(sf.scene.textures[tex]["texels"], [[[0.5, 0.25, 1.0, 1.0]]])
 
 
 @pytest.mark.parametrize("snippet,name", [
