@@ -213,6 +213,8 @@ def main():
                         rays_per_launch=round(rays_per_launch), launch_ms=round(launch_s * 1e3, 4),
                         traffic_bytes_per_ray=traffic_per_ray)
 
+    bvh = None if media else gpu_bvh_comparison(ctx, scene, ps, trace_ms, staged["seconds"])
+
     out = {
         "metric": METRIC,
         "value": round(mrays, 2),
@@ -244,8 +246,31 @@ def main():
         "quality": quality,
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "bvh_build": bvh,
     }
     print(json.dumps(out), flush=True)
+
+
+def gpu_bvh_comparison(ctx, scene, staged_params, host_trace_ms, host_frame_s):
+    """the same scene with its tree built on the GPU (clustering + capped
+    leaf collapse): build wall time against the host's binned-SAH build of
+    the upload above, and one stage-timed frame traced through it"""
+    from btrc_amd import _abi
+    host_ms = ctx.scene_build_ms()
+    accel = scene.accel
+    scene.accel = _abi.ACCEL_FLAT_GPU
+    ctx.upload_scene(scene)           # first build sizes the builder's buffers
+    ctx.upload_scene(scene)
+    gpu_ms = ctx.scene_build_ms()
+    info = ctx.scene_info()
+    st = ctx.render(staged_params)
+    log(f"bench: GPU-built tree: {gpu_ms:.1f} ms (host SAH build {host_ms:.0f} ms), {info['nodes']} BVH4 nodes, "
+        f"trace {st.stage_ms[1]:.1f} ms vs {host_trace_ms:.1f} ms")
+    scene.accel = accel
+    return dict(host_sah_build_ms=round(host_ms, 1), gpu_build_ms=round(gpu_ms, 2), gpu_bvh4_nodes=info["nodes"],
+                gpu_bvh4_depth=info["depth"], trace_ms_host_sah=round(float(host_trace_ms), 2),
+                trace_ms_gpu_built=round(float(st.stage_ms[1]), 2), frame_s_host_sah=round(host_frame_s, 4),
+                frame_s_gpu_built=round(st.seconds, 4))
 
 
 NRC_FLOP_PER_QUERY = 2 * (64 * 64 + 4 * 64 * 64 + 64 * 3)   # 41,344
@@ -437,6 +462,25 @@ def instanced_workload(args):
     stage = np.array(st.stage_ms[:7])
     p.time_stages = 0
 
+    # the same frame over one tree of all 16.8M world-space triangles built
+    # on the GPU, against the two-level structure above
+    two_level_ms = ctx.scene_build_ms()
+    scene.accel = _abi.ACCEL_FLAT_GPU
+    ctx.upload_scene(scene)
+    flat_ms = ctx.scene_build_ms()
+    finfo = ctx.scene_info()
+    p.time_stages = 1
+    stf = ctx.render_views(p, views)
+    p.time_stages = 0
+    scene.accel = _abi.ACCEL_TWO_LEVEL
+    ctx.upload_scene(scene)
+    log(f"bench: flat tree of {finfo['triangles']} triangles built on the GPU in {flat_ms:.0f} ms "
+        f"({finfo['nodes']} BVH4 nodes): trace {stf.stage_ms[1]:.1f} ms vs two-level {stage[1]:.1f} ms")
+    bvh = dict(two_level_host_build_ms=round(two_level_ms, 1), flat_gpu_build_ms=round(flat_ms, 1),
+               flat_gpu_bvh4_nodes=finfo["nodes"], flat_gpu_bvh4_depth=finfo["depth"],
+               trace_ms_two_level=round(float(stage[1]), 3), trace_ms_flat_gpu_built=round(float(stf.stage_ms[1]), 3),
+               shadow_ms_two_level=round(float(stage[3]), 3), shadow_ms_flat_gpu_built=round(float(stf.stage_ms[3]), 3))
+
     quality = None
     if args.gt_spp > 0:
         ctx.nrc_configure(enabled=0)
@@ -482,6 +526,7 @@ def instanced_workload(args):
         "quality": quality,
         "roofline": None,
         "cpu_baseline": None,
+        "bvh_build": bvh,
     }
 
 

@@ -292,6 +292,29 @@ class Context:
         self._check(self.lib.btrc_hip_eval_texture(self.h, texture, uv.ctypes.data, len(uv), out.ctypes.data))
         return out
 
+    def scene_build_ms(self):
+        """wall time of the last upload's acceleration-structure build"""
+        ms = C.c_double(0)
+        self._check(self.lib.btrc_hip_scene_build_ms(self.h, C.byref(ms)))
+        return ms.value
+
+    def build_bvh(self, boxes, max_leaf_size=0, refit_boxes=None):
+        """GPU binary BVH over (n, 6) boxes (lo xyz, hi xyz): (nodes as
+        _abi.bvh_node_dtype(), primitive indices, build ms)"""
+        boxes = np.ascontiguousarray(boxes, dtype=np.float32).reshape(-1, 6)
+        n = len(boxes)
+        nodes = np.zeros(max(1, 2 * n - 1), dtype=_abi.bvh_node_dtype())
+        prims = np.zeros(max(1, n), np.uint32)
+        rb = None if refit_boxes is None else np.ascontiguousarray(refit_boxes, dtype=np.float32).reshape(-1, 6)
+        if rb is not None and rb.shape != boxes.shape:
+            raise BtrcException("build_bvh: refit boxes differ in shape")
+        count = C.c_int32(0)
+        ms = C.c_double(0)
+        self._check(self.lib.btrc_hip_build_bvh(self.h, boxes.ctypes.data, n, max_leaf_size,
+                                                None if rb is None else rb.ctypes.data, nodes.ctypes.data,
+                                                prims.ctypes.data, C.byref(count), C.byref(ms)))
+        return nodes[:count.value], prims[:n], ms.value
+
     def eval_envir(self, dirs, samples):
         """(n, 12) float32: the environment light's eval_le rgb, pdf_li at
         world directions `dirs` (n, 3), then sample_li's direction, pdf and

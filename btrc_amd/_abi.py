@@ -23,6 +23,7 @@ TEX_FILTER_LINEAR = 0
 TEX_FILTER_POINT = 1
 ACCEL_FLAT = 0
 ACCEL_TWO_LEVEL = 1
+ACCEL_FLAT_GPU = 2
 ENVIR_NONE = 0
 ENVIR_GRADIENT_SKY = 1
 ENVIR_IBL = 2
@@ -142,6 +143,15 @@ class Tuning(C.Structure):
                 ("two_level_postpone", C.c_int32), ("reserved", C.c_int32 * 3)]
 
 
+class BvhNode(C.Structure):
+    _fields_ = [("bounds", C.c_float * 6), ("primitive_count", C.c_uint32), ("first_child_or_primitive", C.c_uint32)]
+
+
+def bvh_node_dtype():
+    import numpy as np
+    return np.dtype([("bounds", np.float32, 6), ("primitive_count", np.uint32), ("first", np.uint32)])
+
+
 class Ray(C.Structure):
     _fields_ = [("o", F3), ("tmin", C.c_float), ("d", F3), ("tmax", C.c_float)]
 
@@ -166,14 +176,15 @@ def hit_dtype():
 EXPORTS = [
     "btrc_hip_create", "btrc_hip_destroy", "btrc_hip_last_error", "btrc_hip_device_name",
     "btrc_hip_set_tuning", "btrc_hip_get_tuning",
-    "btrc_hip_upload_scene", "btrc_hip_set_camera", "btrc_hip_scene_info", "btrc_hip_set_filter",
+    "btrc_hip_upload_scene", "btrc_hip_set_camera", "btrc_hip_scene_info", "btrc_hip_scene_build_ms",
+    "btrc_hip_set_filter",
     "btrc_hip_render", "btrc_hip_render_views", "btrc_hip_film_readback", "btrc_hip_view_readback",
     "btrc_hip_aov_readback", "btrc_hip_request_stop",
     "btrc_hip_nrc_configure", "btrc_hip_nrc_reset", "btrc_hip_nrc_stats", "btrc_hip_nrc_get_weights",
     "btrc_hip_nrc_set_weights", "btrc_hip_nrc_eval", "btrc_hip_nrc_gradients", "btrc_hip_nrc_fit",
     "btrc_hip_trace_closest", "btrc_hip_trace_any",
     "btrc_hip_batch_upload", "btrc_hip_batch_trace", "btrc_hip_batch_download",
-    "btrc_hip_capture_rays", "btrc_hip_eval_halton", "btrc_hip_eval_independent", "btrc_hip_eval_volumes", "btrc_hip_eval_texture",
+    "btrc_hip_capture_rays", "btrc_hip_build_bvh", "btrc_hip_eval_halton", "btrc_hip_eval_independent", "btrc_hip_eval_volumes", "btrc_hip_eval_texture",
     "btrc_hip_eval_envir", "btrc_hip_envir_table", "btrc_hip_eval_medium",
     "btrc_hip_eval_detmath",
 ]
@@ -193,6 +204,9 @@ def declare(lib):
         "btrc_hip_upload_scene": (C.c_int, [ctx, P(SceneDesc)]),
         "btrc_hip_set_camera": (C.c_int, [ctx, P(CameraDesc)]),
         "btrc_hip_scene_info": (C.c_int, [ctx, P(C.c_int64), P(C.c_int32), P(C.c_int32), P(C.c_int64)]),
+        "btrc_hip_scene_build_ms": (C.c_int, [ctx, P(C.c_double)]),
+        "btrc_hip_build_bvh": (C.c_int, [ctx, C.c_void_p, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p,
+                                         P(C.c_int32), P(C.c_double)]),
         "btrc_hip_set_filter": (C.c_int, [ctx, P(FilterDesc)]),
         "btrc_hip_render": (C.c_int, [ctx, P(RenderParams), P(RenderStats)]),
         "btrc_hip_render_views": (C.c_int, [ctx, P(RenderParams), C.c_int32, P(ViewDesc), P(RenderStats)]),

@@ -32,6 +32,7 @@ HOST_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-I" + os.path.
 HIP_SOURCES = [
     os.path.join(CSRC, "kernels", "wavefront.hip"),
     os.path.join(CSRC, "kernels", "nrc.hip"),
+    os.path.join(CSRC, "kernels", "bvh_gpu.hip"),
     os.path.join(CSRC, "host", "context.hip"),
 ]
 HOST_SOURCES = [

@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "../../../include/btrc_hip.h"
+#include "../kernels/bvh_gpu.h"
 #include "../kernels/wavefront.h"
 #include "bvh_build.h"
 
@@ -201,6 +202,8 @@ struct btrc_ctx
     int32_t instance_count = 0;
     int64_t triangle_count = 0;
     int64_t node_count = 0;
+    double build_ms = 0.0;            // the last upload's acceleration-structure build
+    PlocBuffers ploc;                 // GPU builder scratch, kept for rebuilds
     int32_t bvh_depth = 0;
     int32_t stack_size = 0;
 
@@ -520,6 +523,8 @@ struct Accel
     std::vector<BvhNode4> nodes;     // child slots hold traversal codes
     std::vector<float4> tris;        // 3 per triangle in leaf order
     std::vector<TraceInstance> inst; // two levels only
+    bool on_device = false;          // built on the GPU straight into the context's buffers
+    int64_t device_nodes = 0;
     int depth = 0;
     int stack_bound = 0;
     float lo[3] = { MAX_FLOAT, MAX_FLOAT, MAX_FLOAT }, hi[3] = { -MAX_FLOAT, -MAX_FLOAT, -MAX_FLOAT };
@@ -570,6 +575,45 @@ Accel build_flat(const std::vector<float> &world, const std::vector<uint32_t> &t
     append_nodes(acc, bvh, 0);
     acc.depth = bvh.depth;
     acc.stack_bound = bvh.stack_bound;
+    for(size_t k = 0; k < world.size(); ++k)
+    {
+        acc.lo[k % 3] = std::min(acc.lo[k % 3], world[k]);
+        acc.hi[k % 3] = std::max(acc.hi[k % 3], world[k]);
+    }
+    return acc;
+}
+
+// One tree over the world-space triangles built on the GPU: PLOC over the
+// triangle boxes, leaves collapsed by SAH up to four triangles (the leaf
+// code's limit), 4-wide nodes, leaf-ordered triangle records; nodes and
+// records land in the context's buffers directly.
+Accel build_flat_gpu(btrc_ctx *ctx, const std::vector<float> &world, const std::vector<uint32_t> &tri_inst,
+                     const std::vector<uint32_t> &tri_prim, uint32_t ntri)
+{
+    if(ntri > MAX_LEAF_FIRST)
+        throw InvalidArg("too many triangles for the traversal's leaf encoding");
+    hipStream_t s = ctx->stream;
+    DevBuf<float> dv, dbox;
+    DevBuf<uint32_t> dinst, dprim;
+    dv.upload(world.data(), world.size(), s);
+    dinst.upload(tri_inst.data(), tri_inst.size(), s);
+    dprim.upload(tri_prim.data(), tri_prim.size(), s);
+    dbox.alloc(6 * (size_t)ntri);
+    triangle_boxes(dv.ptr, ntri, dbox.ptr, s);
+    const PlocResult r = ploc_build(ctx->ploc, dbox.ptr, ntri, 4, s);
+    const Bvh4Device b4 = ploc_to_bvh4(ctx->ploc, r, s);
+    ctx->nodes.alloc(b4.node_count);
+    BTRC_HIP_CHECK(hipMemcpyAsync(ctx->nodes.ptr, ctx->ploc.nodes4, b4.node_count * sizeof(BvhNode4),
+                                  hipMemcpyDeviceToDevice, s));
+    ctx->tris.alloc(3 * (size_t)ntri);
+    gather_triangles(dv.ptr, dinst.ptr, dprim.ptr, ctx->ploc.prims, ntri, ctx->tris.ptr, s);
+    BTRC_HIP_CHECK(hipGetLastError());
+    BTRC_HIP_CHECK(hipStreamSynchronize(s));
+    Accel acc;
+    acc.on_device = true;
+    acc.device_nodes = b4.node_count;
+    acc.depth = b4.depth;
+    acc.stack_bound = b4.stack_bound;
     for(size_t k = 0; k < world.size(); ++k)
     {
         acc.lo[k % 3] = std::min(acc.lo[k % 3], world[k]);
@@ -1031,7 +1075,7 @@ void upload_scene(btrc_ctx *ctx, const btrc_scene_desc *sd)
         return medium_id[idx];
     };
 
-    if(sd->accel != BTRC_ACCEL_FLAT && sd->accel != BTRC_ACCEL_TWO_LEVEL)
+    if(sd->accel != BTRC_ACCEL_FLAT && sd->accel != BTRC_ACCEL_TWO_LEVEL && sd->accel != BTRC_ACCEL_FLAT_GPU)
         throw InvalidArg("unknown acceleration structure type");
     const bool two_level = sd->accel == BTRC_ACCEL_TWO_LEVEL;
 
@@ -1133,11 +1177,19 @@ void upload_scene(btrc_ctx *ctx, const btrc_scene_desc *sd)
         throw InvalidArg("unknown environment light type");
 
     const uint32_t ntri = prim_base;
-    Accel acc = two_level ? build_two_level(sd, insts) : build_flat(world, tri_inst, tri_prim, ntri);
-
     ctx->has_scene = false;
-    ctx->nodes.upload(acc.nodes.data(), acc.nodes.size(), s);
-    ctx->tris.upload(acc.tris.data(), acc.tris.size(), s);
+    BTRC_HIP_CHECK(hipStreamSynchronize(s));
+    const auto build_start = std::chrono::steady_clock::now();
+    Accel acc = two_level ? build_two_level(sd, insts)
+              : (sd->accel == BTRC_ACCEL_FLAT_GPU && ntri > 0) ? build_flat_gpu(ctx, world, tri_inst, tri_prim, ntri)
+                                                                : build_flat(world, tri_inst, tri_prim, ntri);
+    ctx->build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - build_start).count();
+
+    if(!acc.on_device)
+    {
+        ctx->nodes.upload(acc.nodes.data(), acc.nodes.size(), s);
+        ctx->tris.upload(acc.tris.data(), acc.tris.size(), s);
+    }
     ctx->trace_inst.upload(acc.inst.data(), acc.inst.size(), s);
     ctx->instances.upload(insts.data(), insts.size(), s);
     ctx->geometries.upload(geos.data(), geos.size(), s);
@@ -1205,7 +1257,7 @@ void upload_scene(btrc_ctx *ctx, const btrc_scene_desc *sd)
     ctx->envir_light_index = envir_index;
     ctx->instance_count = sd->instance_count;
     ctx->triangle_count = ntri;
-    ctx->node_count = (int64_t)acc.nodes.size();
+    ctx->node_count = acc.on_device ? acc.device_nodes : (int64_t)acc.nodes.size();
     ctx->bvh_depth = acc.depth;
     ctx->stack_size = acc.stack_bound + 1;
     ctx->two_level = two_level;
@@ -1963,6 +2015,45 @@ int btrc_hip_upload_scene(btrc_ctx *ctx, const btrc_scene_desc *scene)
 int btrc_hip_set_camera(btrc_ctx *ctx, const btrc_camera_desc *camera)
 {
     return guarded(ctx, [&] { set_camera(ctx, camera); });
+}
+
+int btrc_hip_scene_build_ms(btrc_ctx *ctx, double *ms)
+{
+    return guarded(ctx, [&] {
+        if(!ctx->has_scene)
+            throw StateError("no scene");
+        if(!ms)
+            throw InvalidArg("null output");
+        *ms = ctx->build_ms;
+    });
+}
+
+int btrc_hip_build_bvh(btrc_ctx *ctx, const float *boxes, int32_t n, int32_t max_leaf_size, const float *refit_boxes,
+                       btrc_bvh_node *nodes, uint32_t *primitive_indices, int32_t *node_count, double *build_ms)
+{
+    return guarded(ctx, [&] {
+        if(n <= 0 || !boxes || !nodes || !primitive_indices || !node_count || n > (1 << 28))
+            throw InvalidArg("bad bvh build request");
+        hipStream_t s = ctx->stream;
+        DevBuf<float> db, dr;
+        db.upload(boxes, 6 * (size_t)n, s);
+        if(refit_boxes)
+            dr.upload(refit_boxes, 6 * (size_t)n, s);
+        BTRC_HIP_CHECK(hipStreamSynchronize(s));
+        const auto t0 = std::chrono::steady_clock::now();
+        const PlocResult r = ploc_build(ctx->ploc, db.ptr, (uint32_t)n, max_leaf_size, s);
+        BTRC_HIP_CHECK(hipStreamSynchronize(s));
+        if(build_ms)
+            *build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        if(refit_boxes)
+            ploc_refit(ctx->ploc, r, dr.ptr, s);
+        static_assert(sizeof(btrc_bvh_node) == sizeof(PlocNode), "node layouts differ");
+        BTRC_HIP_CHECK(hipMemcpyAsync(nodes, ctx->ploc.nodes, r.node_count * sizeof(PlocNode), hipMemcpyDeviceToHost, s));
+        BTRC_HIP_CHECK(hipMemcpyAsync(primitive_indices, ctx->ploc.prims, (size_t)n * sizeof(uint32_t),
+                                      hipMemcpyDeviceToHost, s));
+        BTRC_HIP_CHECK(hipStreamSynchronize(s));
+        *node_count = (int32_t)r.node_count;
+    });
 }
 
 int btrc_hip_scene_info(btrc_ctx *ctx, int64_t *nodes, int32_t *depth, int32_t *stack_bound, int64_t *triangles)

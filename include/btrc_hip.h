@@ -75,7 +75,8 @@ enum
 enum
 {
     BTRC_ACCEL_FLAT = 0,      /* one tree over world-space triangles   */
-    BTRC_ACCEL_TWO_LEVEL = 1  /* instance tree over per-geometry trees */
+    BTRC_ACCEL_TWO_LEVEL = 1, /* instance tree over per-geometry trees */
+    BTRC_ACCEL_FLAT_GPU = 2   /* one tree over world-space triangles, built on the GPU */
 };
 
 enum
@@ -364,6 +365,9 @@ int btrc_hip_upload_scene(btrc_ctx *ctx, const btrc_scene_desc *scene);
 int btrc_hip_set_camera(btrc_ctx *ctx, const btrc_camera_desc *camera);
 /* statistics of the built BVH: nodes, depth, stack bound, triangles */
 int btrc_hip_scene_info(btrc_ctx *ctx, int64_t *nodes, int32_t *depth, int32_t *stack_bound, int64_t *triangles);
+/* wall time of the last upload's acceleration-structure build, in ms
+ * (the host builder's, or the GPU builder's kernels and read-backs) */
+int btrc_hip_scene_build_ms(btrc_ctx *ctx, double *ms);
 
 /* WavefrontPathTracer::set_film_filter (wavefront.cpp:72-75); box until set */
 int btrc_hip_set_filter(btrc_ctx *ctx, const btrc_filter_desc *filter);
@@ -432,6 +436,33 @@ int btrc_hip_batch_download(btrc_ctx *ctx, btrc_hit *hits);
  * `bounces` bounces of the first pool wave (at most max_rays). */
 int btrc_hip_capture_rays(btrc_ctx *ctx, const btrc_render_params *params, int32_t bounces,
                           btrc_ray *rays, int64_t max_rays, int64_t *count);
+
+/* lib/bvh's Bvh<float>::Node (lib/bvh/include/bvh/bvh.hpp:25-28): bounds
+ * min x, max x, min y, max y, min z, max z; a leaf holds primitive_count
+ * primitives from first_child_or_primitive in the primitive index list, an
+ * inner node (primitive_count 0) its children at first_child_or_primitive
+ * and the next index */
+typedef struct btrc_bvh_node
+{
+    float bounds[6];
+    uint32_t primitive_count;
+    uint32_t first_child_or_primitive;
+} btrc_bvh_node;
+
+/* ---- GPU BVH build on host-given boxes ----
+ * A binary tree over n boxes (6 floats each: lo xyz, hi xyz) built on the
+ * GPU the way lib/bvh's LocallyOrderedClusteringBuilder<Bvh<float>,
+ * uint64_t> builds it (locally_ordered_clustering_builder.hpp; box centres
+ * 0.5 (lo + hi), 63-bit Morton codes over the union box, search radius 14),
+ * then LeafCollapser (leaf_collapser.hpp) unless max_leaf_size < 0
+ * (0: no leaf size limit, as lib/bvh; k > 0: leaves of at most k). With
+ * refit_boxes, HierarchyRefitter (hierarchy_refitter.hpp) then moves the
+ * tree onto those boxes. nodes needs room for 2n - 1, primitive_indices
+ * for n; *node_count gets the nodes written, *build_ms the build's wall
+ * time (may be NULL). Replaces the CPU builders of core/volume/bvh.cpp:40-63
+ * and, for triangles, optixAccelBuild (utils/optix/context.cpp:158-295). */
+int btrc_hip_build_bvh(btrc_ctx *ctx, const float *boxes, int32_t n, int32_t max_leaf_size, const float *refit_boxes,
+                       btrc_bvh_node *nodes, uint32_t *primitive_indices, int32_t *node_count, double *build_ms);
 
 /* ---- test hooks: evaluate device building blocks on host-given inputs ---- */
 int btrc_hip_eval_halton(btrc_ctx *ctx, int32_t width, int32_t height, const uint32_t *px, const uint32_t *py,

@@ -78,6 +78,8 @@ def ref():
         L.bvhref_create.argtypes = [vp, C.c_int64, C.c_int32]
         L.bvhref_destroy.argtypes = [vp]
         L.bvhref_trace.argtypes = [vp, vp, C.c_int64, C.c_int32, vp, vp, vp, vp, vp, vp, C.c_int32]
+        L.bvhref_ploc.restype = C.c_int64
+        L.bvhref_ploc.argtypes = [vp, C.c_int64, C.c_int32, vp, vp, vp]
         _ref = L
     return _ref
 
@@ -330,6 +332,20 @@ class RefBvh:
         ref().bvhref_trace(self.h, rays.ctypes.data, n, 1 if any_hit else 0, prim.ctypes.data, t.ctypes.data,
                            u.ctypes.data, v.ctypes.data, steps.ctypes.data, isect.ctypes.data, threads)
         return prim, t, u, v, steps, isect
+
+
+def ref_ploc(boxes, collapse=True, refit_boxes=None):
+    """lib/bvh's LocallyOrderedClusteringBuilder (+ LeafCollapser, + a
+    HierarchyRefitter onto refit_boxes) over (n, 6) boxes: (nodes as
+    _abi.bvh_node_dtype(), primitive indices)"""
+    boxes = np.ascontiguousarray(boxes, dtype=np.float32).reshape(-1, 6)
+    n = len(boxes)
+    nodes = np.zeros(2 * n - 1, dtype=_abi.bvh_node_dtype())
+    prims = np.zeros(n, np.uint32)
+    rb = None if refit_boxes is None else np.ascontiguousarray(refit_boxes, dtype=np.float32).reshape(-1, 6)
+    count = ref().bvhref_ploc(boxes.ctypes.data, n, 1 if collapse else 0, None if rb is None else rb.ctypes.data,
+                              nodes.ctypes.data, prims.ctypes.data)
+    return nodes[:count], prims
 
 
 def kat_rays(width, height, eye, direction, up, fov):

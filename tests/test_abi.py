@@ -50,6 +50,7 @@ STRUCTS = {
     "btrc_filter_desc": "FilterDesc", "btrc_render_stats": "RenderStats", "btrc_ray": "Ray",
     "btrc_hit": "Hit", "btrc_tuning": "Tuning", "btrc_nrc_params": "NrcParams", "btrc_nrc_stats": "NrcStats",
     "btrc_texture_desc": "TextureDesc", "btrc_view_desc": "ViewDesc", "btrc_volume_desc": "VolumeDesc",
+    "btrc_bvh_node": "BvhNode",
 }
 
 
