@@ -88,6 +88,8 @@ def main():
                          "views rendered together, 1 spp per frame with the radiance cache for --frames frames "
                          "(64 when not given)")
     ap.add_argument("--frames", type=int, default=None)
+    ap.add_argument("--no-bvh-compare", action="store_true",
+                    help="skip the frame traced through a GPU-built tree (field workload)")
     ap.add_argument("--gt-spp", type=int, default=1024,
                     help="samples per pixel of the FP64 ground truth for relMSE (0: skip)")
     args = ap.parse_args()
@@ -213,7 +215,7 @@ def main():
                         rays_per_launch=round(rays_per_launch), launch_ms=round(launch_s * 1e3, 4),
                         traffic_bytes_per_ray=traffic_per_ray)
 
-    bvh = None if media else gpu_bvh_comparison(ctx, scene, ps, trace_ms, staged["seconds"])
+    bvh = None if media or args.no_bvh_compare else gpu_bvh_comparison(ctx, scene, ps, trace_ms, staged["seconds"])
 
     out = {
         "metric": METRIC,
