@@ -298,6 +298,21 @@ class Context:
         self._check(self.lib.btrc_hip_scene_build_ms(self.h, C.byref(ms)))
         return ms.value
 
+    def update_geometry(self, scene, geometry):
+        """re-send geometry `geometry` of `scene` (its arrays changed in
+        place, same triangle count) to the uploaded one-level scene: records
+        rebuilt, tree refitted; returns the update's wall time in ms"""
+        g = scene.geometries[geometry]
+        fp = C.POINTER(C.c_float)
+        d = _abi.GeometryDesc()
+        d.triangle_count = len(g["positions"])
+        d.positions = g["positions"].ctypes.data_as(fp)
+        d.normals = g["normals"].ctypes.data_as(fp) if g["normals"] is not None else None
+        d.tex_coords = g["uvs"].ctypes.data_as(fp) if g["uvs"] is not None else None
+        ms = C.c_double(0)
+        self._check(self.lib.btrc_hip_update_geometry(self.h, geometry, C.byref(d), C.byref(ms)))
+        return ms.value
+
     def build_bvh(self, boxes, max_leaf_size=0, refit_boxes=None):
         """GPU binary BVH over (n, 6) boxes (lo xyz, hi xyz): (nodes as
         _abi.bvh_node_dtype(), primitive indices, build ms)"""

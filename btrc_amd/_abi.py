@@ -177,6 +177,7 @@ EXPORTS = [
     "btrc_hip_create", "btrc_hip_destroy", "btrc_hip_last_error", "btrc_hip_device_name",
     "btrc_hip_set_tuning", "btrc_hip_get_tuning",
     "btrc_hip_upload_scene", "btrc_hip_set_camera", "btrc_hip_scene_info", "btrc_hip_scene_build_ms",
+    "btrc_hip_update_geometry",
     "btrc_hip_set_filter",
     "btrc_hip_render", "btrc_hip_render_views", "btrc_hip_film_readback", "btrc_hip_view_readback",
     "btrc_hip_aov_readback", "btrc_hip_request_stop",
@@ -205,6 +206,7 @@ def declare(lib):
         "btrc_hip_set_camera": (C.c_int, [ctx, P(CameraDesc)]),
         "btrc_hip_scene_info": (C.c_int, [ctx, P(C.c_int64), P(C.c_int32), P(C.c_int32), P(C.c_int64)]),
         "btrc_hip_scene_build_ms": (C.c_int, [ctx, P(C.c_double)]),
+        "btrc_hip_update_geometry": (C.c_int, [ctx, C.c_int32, C.c_void_p, P(C.c_double)]),
         "btrc_hip_build_bvh": (C.c_int, [ctx, C.c_void_p, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p,
                                          P(C.c_int32), P(C.c_double)]),
         "btrc_hip_set_filter": (C.c_int, [ctx, P(FilterDesc)]),

@@ -203,7 +203,16 @@ struct btrc_ctx
     int64_t triangle_count = 0;
     int64_t node_count = 0;
     double build_ms = 0.0;            // the last upload's acceleration-structure build
+    double refit_ms = 0.0;            // the last btrc_hip_update_geometry
     PlocBuffers ploc;                 // GPU builder scratch, kept for rebuilds
+    // geometry updates of one-level structures: per-geometry ranges, the
+    // 4-wide nodes level by level (root level first), the bounds of the
+    // triangles and of the volume primitives
+    std::vector<GeometryRec> geos_host;
+    DevBuf<uint32_t> refit_order;
+    std::vector<uint32_t> refit_levels;
+    float tri_lo[3] = { 0, 0, 0 }, tri_hi[3] = { 0, 0, 0 };
+    float vol_lo[3] = { 0, 0, 0 }, vol_hi[3] = { 0, 0, 0 };
     int32_t bvh_depth = 0;
     int32_t stack_size = 0;
 
@@ -525,6 +534,10 @@ struct Accel
     std::vector<TraceInstance> inst; // two levels only
     bool on_device = false;          // built on the GPU straight into the context's buffers
     int64_t device_nodes = 0;
+    // one-level trees: node indices grouped by depth (root first) and the
+    // group boundaries; empty order: the indices are already in that order
+    std::vector<uint32_t> level_order;
+    std::vector<uint32_t> level_offsets;
     int depth = 0;
     int stack_bound = 0;
     float lo[3] = { MAX_FLOAT, MAX_FLOAT, MAX_FLOAT }, hi[3] = { -MAX_FLOAT, -MAX_FLOAT, -MAX_FLOAT };
@@ -575,6 +588,22 @@ Accel build_flat(const std::vector<float> &world, const std::vector<uint32_t> &t
     append_nodes(acc, bvh, 0);
     acc.depth = bvh.depth;
     acc.stack_bound = bvh.stack_bound;
+    // breadth-first levels of the 4-wide nodes, for refits
+    acc.level_order.push_back(0);
+    acc.level_offsets = { 0, 1 };
+    while(acc.level_offsets.back() > acc.level_offsets[acc.level_offsets.size() - 2])
+    {
+        const uint32_t lo = acc.level_offsets[acc.level_offsets.size() - 2], hi = acc.level_offsets.back();
+        for(uint32_t k = lo; k < hi; ++k)
+            for(int j = 0; j < 4; ++j)
+            {
+                const uint32_t code = (uint32_t)acc.nodes[acc.level_order[k]].child[j];
+                if(code != CODE_DONE && !(code & CODE_LEAF))
+                    acc.level_order.push_back(code);
+            }
+        acc.level_offsets.push_back((uint32_t)acc.level_order.size());
+    }
+    acc.level_offsets.pop_back();
     for(size_t k = 0; k < world.size(); ++k)
     {
         acc.lo[k % 3] = std::min(acc.lo[k % 3], world[k]);
@@ -612,6 +641,7 @@ Accel build_flat_gpu(btrc_ctx *ctx, const std::vector<float> &world, const std::
     Accel acc;
     acc.on_device = true;
     acc.device_nodes = b4.node_count;
+    acc.level_offsets = b4.level_offsets;
     acc.depth = b4.depth;
     acc.stack_bound = b4.stack_bound;
     for(size_t k = 0; k < world.size(); ++k)
@@ -759,6 +789,87 @@ Accel build_two_level(const btrc_scene_desc *sd, const std::vector<InstanceRec> 
     return acc;
 }
 
+// TriangleMesh::commit's per-triangle records of one geometry
+// (triangle_mesh.cpp:24-171), appended to r; its area-weighted alias table
+// and total area as well
+struct GeoRecords
+{
+    std::vector<float4> gx, gy, gz, sa, sba, sca, lpos;
+    std::vector<AliasUnit> alias;
+};
+
+float geometry_records(const btrc_geometry_desc &g, GeoRecords &r)
+{
+    std::vector<float> areas(g.triangle_count);
+    for(int32_t t = 0; t < g.triangle_count; ++t)
+    {
+        const float *p = g.positions + 9 * (size_t)t;
+        const V3 a = v3(p[0], p[1], p[2]), b = v3(p[3], p[4], p[5]), c = v3(p[6], p[7], p[8]);
+        const V3 geo_z = normalize(cross(b - a, c - a));
+        V3 sz[3];
+        for(int k = 0; k < 3; ++k)
+        {
+            if(g.normals)
+            {
+                const float *n = g.normals + 9 * (size_t)t + 3 * k;
+                sz[k] = v3(n[0], n[1], n[2]);
+                if(sz[k].x == 0.0f && sz[k].y == 0.0f && sz[k].z == 0.0f)
+                    sz[k] = geo_z;
+            }
+            else
+                sz[k] = geo_z;
+        }
+        float uv[6] = { 0, 0, 0, 0, 0, 0 };
+        if(g.tex_coords)
+            std::memcpy(uv, g.tex_coords + 6 * (size_t)t, sizeof(uv));
+        const V3 ex = triangle_dpdu(b - a, c - a, uv[2] - uv[0], uv[3] - uv[1], uv[4] - uv[0], uv[5] - uv[1], geo_z);
+        const V3 ey = normalize(cross(geo_z, ex));
+        r.gx.push_back(make_float4(ex.x, ex.y, ex.z, uv[0]));
+        r.gy.push_back(make_float4(ey.x, ey.y, ey.z, uv[2] - uv[0]));
+        r.gz.push_back(make_float4(geo_z.x, geo_z.y, geo_z.z, uv[4] - uv[0]));
+        r.sa.push_back(make_float4(sz[0].x, sz[0].y, sz[0].z, uv[1]));
+        const V3 dba = sz[1] - sz[0], dca = sz[2] - sz[0];
+        r.sba.push_back(make_float4(dba.x, dba.y, dba.z, uv[3] - uv[1]));
+        r.sca.push_back(make_float4(dca.x, dca.y, dca.z, uv[5] - uv[1]));
+        const V3 ba = b - a, ca = c - a;
+        r.lpos.push_back(make_float4(a.x, a.y, a.z, 0.0f));
+        r.lpos.push_back(make_float4(ba.x, ba.y, ba.z, 0.0f));
+        r.lpos.push_back(make_float4(ca.x, ca.y, ca.z, 0.0f));
+        areas[t] = 0.5f * length(cross(ba, ca));
+    }
+    std::vector<AliasUnit> units;
+    const float total_area = build_alias_table(areas, units);
+    r.alias.insert(r.alias.end(), units.begin(), units.end());
+    return total_area;
+}
+
+// Scene::commit's bounding box (core/scene.cpp:225-237) over the triangles
+// and the volume primitives, the triangles' largest coordinate (the
+// traversal's culling margin) and the triangles' box for the radiance cache
+void set_world_bounds(btrc_ctx *ctx)
+{
+    for(int a = 0; a < 3; ++a)
+    {
+        ctx->world_lo[a] = std::min(ctx->tri_lo[a], ctx->vol_lo[a]);
+        ctx->world_hi[a] = std::max(ctx->tri_hi[a], ctx->vol_hi[a]);
+    }
+    ctx->scene_extent = 0.0f;
+    for(int a = 0; a < 3; ++a)
+        if(ctx->tri_lo[a] <= ctx->tri_hi[a])
+            ctx->scene_extent = std::max(ctx->scene_extent, std::max(std::fabs(ctx->tri_lo[a]), std::fabs(ctx->tri_hi[a])));
+    for(int a = 0; a < 3; ++a)
+    {
+        float lo = ctx->tri_lo[a], hi = ctx->tri_hi[a];
+        if(!(hi > lo))
+        {
+            lo = lo <= hi ? lo : 0.0f;
+            hi = lo + 1.0f;
+        }
+        ctx->scene_lo[a] = lo;
+        ctx->scene_hi[a] = hi;
+    }
+}
+
 void upload_scene(btrc_ctx *ctx, const btrc_scene_desc *sd)
 {
     if(!sd)
@@ -774,8 +885,7 @@ void upload_scene(btrc_ctx *ctx, const btrc_scene_desc *sd)
 
     // geometries: TriangleMesh::commit per-triangle records (triangle_mesh.cpp:24-171)
     std::vector<GeometryRec> geos(sd->geometry_count);
-    std::vector<float4> gx, gy, gz, sa, sba, sca, lpos;
-    std::vector<AliasUnit> alias_units;
+    GeoRecords rec;
     uint32_t prim_offset = 0;
     for(int32_t gi = 0; gi < sd->geometry_count; ++gi)
     {
@@ -784,48 +894,12 @@ void upload_scene(btrc_ctx *ctx, const btrc_scene_desc *sd)
             throw InvalidArg("geometry " + std::to_string(gi) + " has no positions");
         geos[gi].prim_offset = prim_offset;
         geos[gi].prim_count = (uint32_t)g.triangle_count;
-        std::vector<float> areas(g.triangle_count);
-        for(int32_t t = 0; t < g.triangle_count; ++t)
-        {
-            const float *p = g.positions + 9 * (size_t)t;
-            const V3 a = v3(p[0], p[1], p[2]), b = v3(p[3], p[4], p[5]), c = v3(p[6], p[7], p[8]);
-            const V3 geo_z = normalize(cross(b - a, c - a));
-            V3 sz[3];
-            for(int k = 0; k < 3; ++k)
-            {
-                if(g.normals)
-                {
-                    const float *n = g.normals + 9 * (size_t)t + 3 * k;
-                    sz[k] = v3(n[0], n[1], n[2]);
-                    if(sz[k].x == 0.0f && sz[k].y == 0.0f && sz[k].z == 0.0f)
-                        sz[k] = geo_z;
-                }
-                else
-                    sz[k] = geo_z;
-            }
-            float uv[6] = { 0, 0, 0, 0, 0, 0 };
-            if(g.tex_coords)
-                std::memcpy(uv, g.tex_coords + 6 * (size_t)t, sizeof(uv));
-            const V3 ex = triangle_dpdu(b - a, c - a, uv[2] - uv[0], uv[3] - uv[1], uv[4] - uv[0], uv[5] - uv[1], geo_z);
-            const V3 ey = normalize(cross(geo_z, ex));
-            gx.push_back(make_float4(ex.x, ex.y, ex.z, uv[0]));
-            gy.push_back(make_float4(ey.x, ey.y, ey.z, uv[2] - uv[0]));
-            gz.push_back(make_float4(geo_z.x, geo_z.y, geo_z.z, uv[4] - uv[0]));
-            sa.push_back(make_float4(sz[0].x, sz[0].y, sz[0].z, uv[1]));
-            const V3 dba = sz[1] - sz[0], dca = sz[2] - sz[0];
-            sba.push_back(make_float4(dba.x, dba.y, dba.z, uv[3] - uv[1]));
-            sca.push_back(make_float4(dca.x, dca.y, dca.z, uv[5] - uv[1]));
-            const V3 ba = b - a, ca = c - a;
-            lpos.push_back(make_float4(a.x, a.y, a.z, 0.0f));
-            lpos.push_back(make_float4(ba.x, ba.y, ba.z, 0.0f));
-            lpos.push_back(make_float4(ca.x, ca.y, ca.z, 0.0f));
-            areas[t] = 0.5f * length(cross(ba, ca));
-        }
-        std::vector<AliasUnit> units;
-        geos[gi].total_area = build_alias_table(areas, units);
-        alias_units.insert(alias_units.end(), units.begin(), units.end());
+        geos[gi].total_area = geometry_records(g, rec);
         prim_offset += (uint32_t)g.triangle_count;
     }
+    std::vector<float4> &gx = rec.gx, &gy = rec.gy, &gz = rec.gz, &sa = rec.sa, &sba = rec.sba, &sca = rec.sca,
+                        &lpos = rec.lpos;
+    std::vector<AliasUnit> &alias_units = rec.alias;
 
     // textures: RGBA8 texels as one uint32 each, RGBA32F as float4
     if(sd->texture_count < 0 || (sd->texture_count > 0 && !sd->textures))
@@ -1223,36 +1297,38 @@ void upload_scene(btrc_ctx *ctx, const btrc_scene_desc *sd)
 
     ctx->medium_count = R;
     ctx->volume_count = sd->volume_count;
-    // Scene::commit's bounding box (core/scene.cpp:225-237): triangles and
     // each volume's o, o + x, o + y, o + z (VolumePrimitive::get_bounding_box)
     for(int a = 0; a < 3; ++a)
     {
-        ctx->world_lo[a] = acc.lo[a];
-        ctx->world_hi[a] = acc.hi[a];
+        ctx->tri_lo[a] = acc.lo[a];
+        ctx->tri_hi[a] = acc.hi[a];
+        ctx->vol_lo[a] = MAX_FLOAT;
+        ctx->vol_hi[a] = -MAX_FLOAT;
     }
     for(const VolumeRec &v : vols)
         for(int c = 0; c < 4; ++c)
             for(int a = 0; a < 3; ++a)
             {
                 const float p = c == 0 ? v.o[a] : v.o[a] + (c == 1 ? v.x[a] : c == 2 ? v.y[a] : v.z[a]);
-                ctx->world_lo[a] = std::min(ctx->world_lo[a], p);
-                ctx->world_hi[a] = std::max(ctx->world_hi[a], p);
+                ctx->vol_lo[a] = std::min(ctx->vol_lo[a], p);
+                ctx->vol_hi[a] = std::max(ctx->vol_hi[a], p);
             }
-    ctx->scene_extent = 0.0f;
-    for(int a = 0; a < 3; ++a)
-        if(acc.lo[a] <= acc.hi[a])
-            ctx->scene_extent = std::max(ctx->scene_extent, std::max(std::fabs(acc.lo[a]), std::fabs(acc.hi[a])));
-    for(int a = 0; a < 3; ++a)
+    set_world_bounds(ctx);
+    ctx->geos_host = geos;
+    ctx->refit_levels = acc.level_offsets;
+    if(!two_level)
     {
-        float lo = acc.lo[a], hi = acc.hi[a];
-        if(!(hi > lo))
+        std::vector<uint32_t> order = acc.level_order;
+        if(order.empty())
         {
-            lo = lo <= hi ? lo : 0.0f;
-            hi = lo + 1.0f;
+            order.resize(acc.on_device ? (size_t)acc.device_nodes : acc.nodes.size());
+            for(size_t k = 0; k < order.size(); ++k)
+                order[k] = (uint32_t)k;
         }
-        ctx->scene_lo[a] = lo;
-        ctx->scene_hi[a] = hi;
+        ctx->refit_order.upload(order.data(), order.size(), s);
     }
+    else
+        ctx->refit_order.release();
     ctx->light_count = (int32_t)lights.size();
     ctx->envir_light_index = envir_index;
     ctx->instance_count = sd->instance_count;
@@ -2015,6 +2091,87 @@ int btrc_hip_upload_scene(btrc_ctx *ctx, const btrc_scene_desc *scene)
 int btrc_hip_set_camera(btrc_ctx *ctx, const btrc_camera_desc *camera)
 {
     return guarded(ctx, [&] { set_camera(ctx, camera); });
+}
+
+// New vertex data for one geometry of a one-level structure: its shading
+// and light-sampling records are rebuilt, the world-space triangle records
+// of its instances recomputed and the tree refitted bottom-up (same
+// topology, boxes the exact unions of what they hold). The reference builds
+// its structures once per commit (Scene::commit, core/scene.cpp:92-239);
+// refitting in place serves callers that animate vertices between frames.
+void update_geometry(btrc_ctx *ctx, int32_t gi, const btrc_geometry_desc *g)
+{
+    if(!ctx->has_scene)
+        throw StateError("geometry update before btrc_hip_upload_scene");
+    if(ctx->two_level)
+        throw InvalidArg("geometry updates need a one-level structure (BTRC_ACCEL_FLAT or BTRC_ACCEL_FLAT_GPU)");
+    if(gi < 0 || gi >= (int32_t)ctx->geos_host.size() || !g || (g->triangle_count > 0 && !g->positions))
+        throw InvalidArg("bad geometry update");
+    GeometryRec &gr = ctx->geos_host[gi];
+    if((uint32_t)g->triangle_count != gr.prim_count)
+        throw InvalidArg("a geometry update must keep the triangle count");
+    hipStream_t s = ctx->stream;
+    BTRC_HIP_CHECK(hipStreamSynchronize(s));
+    const auto t0 = std::chrono::steady_clock::now();
+    GeoRecords rec;
+    gr.total_area = geometry_records(*g, rec);
+    const size_t off = gr.prim_offset, n = gr.prim_count;
+    if(n > 0)
+    {
+        auto put = [&](DevBuf<float4> &dst, const std::vector<float4> &src, size_t scale) {
+            BTRC_HIP_CHECK(hipMemcpyAsync(dst.ptr + scale * off, src.data(), src.size() * sizeof(float4),
+                                          hipMemcpyHostToDevice, s));
+        };
+        put(ctx->gx_u_a, rec.gx, 1);
+        put(ctx->gy_u_ba, rec.gy, 1);
+        put(ctx->gz_u_ca, rec.gz, 1);
+        put(ctx->sn_v_a, rec.sa, 1);
+        put(ctx->sn_v_ba, rec.sba, 1);
+        put(ctx->sn_v_ca, rec.sca, 1);
+        put(ctx->light_pos, rec.lpos, 3);
+        BTRC_HIP_CHECK(hipMemcpyAsync(ctx->alias.ptr + off, rec.alias.data(), n * sizeof(AliasUnit),
+                                      hipMemcpyHostToDevice, s));
+    }
+    BTRC_HIP_CHECK(hipMemcpyAsync(ctx->geometries.ptr + gi, &gr, sizeof(GeometryRec), hipMemcpyHostToDevice, s));
+    if(n > 0 && ctx->triangle_count > 0)
+    {
+        DevBuf<float> dp;
+        dp.upload(g->positions, 9 * n, s);
+        launch_update_world_tris(s, ctx->tris.ptr, (uint32_t)ctx->triangle_count, ctx->instances.ptr, gi, dp.ptr);
+        BTRC_HIP_CHECK(hipGetLastError());
+        refit_bvh4(ctx->nodes.ptr, ctx->refit_order.ptr, ctx->refit_levels, ctx->tris.ptr, s);
+        BvhNode4 root;
+        BTRC_HIP_CHECK(hipMemcpyAsync(&root, ctx->nodes.ptr, sizeof(root), hipMemcpyDeviceToHost, s));
+        BTRC_HIP_CHECK(hipStreamSynchronize(s));
+        for(int a = 0; a < 3; ++a)
+        {
+            ctx->tri_lo[a] = MAX_FLOAT;
+            ctx->tri_hi[a] = -MAX_FLOAT;
+        }
+        for(int j = 0; j < 4; ++j)
+        {
+            const float lo[3] = { root.lo_x[j], root.lo_y[j], root.lo_z[j] };
+            const float hi[3] = { root.hi_x[j], root.hi_y[j], root.hi_z[j] };
+            for(int a = 0; a < 3; ++a)
+            {
+                ctx->tri_lo[a] = std::min(ctx->tri_lo[a], lo[a]);
+                ctx->tri_hi[a] = std::max(ctx->tri_hi[a], hi[a]);
+            }
+        }
+        set_world_bounds(ctx);
+    }
+    BTRC_HIP_CHECK(hipStreamSynchronize(s));
+    ctx->drop_loop_graph();
+    ctx->refit_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+int btrc_hip_update_geometry(btrc_ctx *ctx, int32_t geometry, const btrc_geometry_desc *desc, double *ms)
+{
+    return guarded(ctx, [&] {
+        update_geometry(ctx, geometry, desc);
+        if(ms)
+            *ms = ctx->refit_ms;
+    });
 }
 
 int btrc_hip_scene_build_ms(btrc_ctx *ctx, double *ms)

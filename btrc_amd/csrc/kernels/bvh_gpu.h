@@ -83,6 +83,7 @@ struct Bvh4Device
     uint32_t node_count = 0;
     int depth = 0;
     int stack_bound = 0;
+    std::vector<uint32_t> level_offsets; // nodes of depth d: [offsets[d], offsets[d + 1])
 };
 
 // 4-wide nodes (into buf.nodes4, child slots as traversal codes) from the
@@ -90,6 +91,13 @@ struct Bvh4Device
 // the largest-area inner member until it holds four. Leaves must hold at
 // most four primitives.
 Bvh4Device ploc_to_bvh4(PlocBuffers &buf, const PlocResult &r, hipStream_t s);
+
+// Refit of a one-level 4-wide tree: every slot box becomes the union of
+// what it holds (a leaf's triangles from the leaf-ordered records, an inner
+// slot's child node), level by level from the deepest; `order` lists the
+// nodes grouped by level, `levels` the group boundaries (root level first).
+void refit_bvh4(BvhNode4 *nodes, const uint32_t *order, const std::vector<uint32_t> &levels, const float4 *tris,
+                hipStream_t s);
 
 // The leaf-ordered triangle records of a one-level structure:
 // (v0, flattened index), (v1, instance), (v2, primitive) per leaf slot.

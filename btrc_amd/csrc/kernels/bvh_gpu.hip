@@ -529,6 +529,43 @@ __global__ void k_bvh4_write(const PlocNode *nodes, const uint32_t *frontier, ui
     out[q] = n;
 }
 
+// one level of refit_bvh4
+__global__ void k_refit4(BvhNode4 *nodes, const uint32_t *order, uint32_t lo, uint32_t hi, const float4 *tris)
+{
+    const uint32_t k = lo + blockIdx.x * blockDim.x + threadIdx.x;
+    if(k >= hi)
+        return;
+    BvhNode4 &n = nodes[order[k]];
+    for(int j = 0; j < 4; ++j)
+    {
+        const uint32_t code = (uint32_t)n.child[j];
+        if(code == CODE_DONE)
+            continue;
+        float b[6] = { INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY };
+        if(code & CODE_LEAF)
+        {
+            const uint32_t first = code & 0x0fffffffu, count = ((code >> 28) & 3u) + 1;
+            for(uint32_t t = 0; t < 3 * count; ++t)
+            {
+                const float4 v = tris[3 * (size_t)first + t];
+                b[0] = fminf(b[0], v.x); b[1] = fminf(b[1], v.y); b[2] = fminf(b[2], v.z);
+                b[3] = fmaxf(b[3], v.x); b[4] = fmaxf(b[4], v.y); b[5] = fmaxf(b[5], v.z);
+            }
+        }
+        else
+        {
+            const BvhNode4 &c = nodes[code];
+            for(int q = 0; q < 4; ++q)
+            {
+                b[0] = fminf(b[0], c.lo_x[q]); b[1] = fminf(b[1], c.lo_y[q]); b[2] = fminf(b[2], c.lo_z[q]);
+                b[3] = fmaxf(b[3], c.hi_x[q]); b[4] = fmaxf(b[4], c.hi_y[q]); b[5] = fmaxf(b[5], c.hi_z[q]);
+            }
+        }
+        n.lo_x[j] = b[0]; n.lo_y[j] = b[1]; n.lo_z[j] = b[2];
+        n.hi_x[j] = b[3]; n.hi_y[j] = b[4]; n.hi_z[j] = b[5];
+    }
+}
+
 __global__ void k_gather_tris(const float *v, const uint32_t *tri_inst, const uint32_t *tri_prim, const uint32_t *prims,
                               uint32_t n, float4 *tris)
 {
@@ -799,6 +836,7 @@ Bvh4Device ploc_to_bvh4(PlocBuffers &buf, const PlocResult &r, hipStream_t s)
     hipLaunchKernelGGL(k_iota_frontier, dim3(1), dim3(1), 0, s, buf.frontier);
     uint32_t f = 1, base = 1;
     Bvh4Device out;
+    out.level_offsets.push_back(0);
     uint32_t *cur = buf.frontier, *nxt = buf.frontier_b;
     while(f > 0)
     {
@@ -813,6 +851,7 @@ Bvh4Device ploc_to_bvh4(PlocBuffers &buf, const PlocResult &r, hipStream_t s)
                            nxt, buf.counters);
         PLOC_CHECK(hipGetLastError());
         ++out.depth;
+        out.level_offsets.push_back(base);
         base += total;
         f = total;
         std::swap(cur, nxt);
@@ -825,6 +864,18 @@ Bvh4Device ploc_to_bvh4(PlocBuffers &buf, const PlocResult &r, hipStream_t s)
     out.node_count = base;
     out.stack_bound = counters[0];
     return out;
+}
+
+void refit_bvh4(BvhNode4 *nodes, const uint32_t *order, const std::vector<uint32_t> &levels, const float4 *tris,
+                hipStream_t s)
+{
+    for(size_t d = levels.size() - 1; d-- > 0;)
+    {
+        const uint32_t lo = levels[d], hi = levels[d + 1];
+        if(hi > lo)
+            hipLaunchKernelGGL(k_refit4, grid_for(hi - lo), dim3(BLOCK), 0, s, nodes, order, lo, hi, tris);
+    }
+    PLOC_CHECK(hipGetLastError());
 }
 
 void gather_triangles(const float *vertices, const uint32_t *tri_inst, const uint32_t *tri_prim,

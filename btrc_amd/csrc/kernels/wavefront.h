@@ -186,6 +186,10 @@ void launch_trace_batch(hipStream_t s, int grid, int sched, const TraceScene &ts
 // test hooks
 void launch_halton_eval(hipStream_t s, const HaltonConsts &hc, const uint32_t *px, const uint32_t *py,
                         const int32_t *sample_index, int32_t n, int32_t dims, float *out);
+// one-level structures: world-space triangle records of `geometry`'s
+// instances from its new local positions (9 floats per triangle)
+void launch_update_world_tris(hipStream_t s, float4 *tris, uint32_t n, const InstanceRec *insts, int32_t geometry,
+                              const float *positions);
 // EnvirLightSampler::preprocess's table kernel: lum and area per tile (w * h each)
 void launch_envir_table(hipStream_t s, const TextureView &tv, int32_t tex, int32_t w, int32_t h, int32_t n_samples,
                         float *lum, float *area);

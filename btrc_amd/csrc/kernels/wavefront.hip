@@ -826,6 +826,27 @@ __global__ void k_volume_eval(VolumeView vv, const float4 *seg, const uint2 *see
     o[7] = tr.x; o[8] = tr.y; o[9] = tr.z;
 }
 
+// world-space records of a one-level structure after a geometry's vertices
+// changed: the records of every instance of `geometry` take the new
+// positions through the instance's transform, as the upload computes them
+__global__ void k_update_world_tris(float4 *tris, uint32_t n, const InstanceRec *insts, int32_t geometry,
+                                    const float *positions)
+{
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if(k >= n)
+        return;
+    const uint32_t inst = __float_as_uint(tris[3 * (size_t)k + 1].w);
+    const InstanceRec &r = insts[inst];
+    if(r.geometry_id != geometry)
+        return;
+    const float *p = positions + 9 * (size_t)__float_as_uint(tris[3 * (size_t)k + 2].w);
+    for(int v = 0; v < 3; ++v)
+    {
+        const V3 w = xf_point(r.xf, v3(p[3 * v], p[3 * v + 1], p[3 * v + 2]));
+        tris[3 * (size_t)k + v] = make_float4(w.x, w.y, w.z, tris[3 * (size_t)k + v].w);
+    }
+}
+
 // the importance table's per-tile luminance and solid angle, one thread per tile
 __global__ void k_envir_table(TextureView tv, int32_t tex, int32_t w, int32_t h, int32_t n_samples, float *lum,
                               float *area)
@@ -1054,6 +1075,14 @@ void launch_volume_eval(hipStream_t s, const VolumeView &vv, const float4 *seg, 
                         float *out)
 {
     hipLaunchKernelGGL(k_volume_eval, dim3((n + 127) / 128), dim3(128), 0, s, vv, seg, seed, n, out);
+}
+
+void launch_update_world_tris(hipStream_t s, float4 *tris, uint32_t n, const InstanceRec *insts, int32_t geometry,
+                              const float *positions)
+{
+    if(n)
+        hipLaunchKernelGGL(k_update_world_tris, dim3((n + 255) / 256), dim3(256), 0, s, tris, n, insts, geometry,
+                           positions);
 }
 
 void launch_envir_table(hipStream_t s, const TextureView &tv, int32_t tex, int32_t w, int32_t h, int32_t n_samples,

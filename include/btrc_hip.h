@@ -368,6 +368,15 @@ int btrc_hip_scene_info(btrc_ctx *ctx, int64_t *nodes, int32_t *depth, int32_t *
 /* wall time of the last upload's acceleration-structure build, in ms
  * (the host builder's, or the GPU builder's kernels and read-backs) */
 int btrc_hip_scene_build_ms(btrc_ctx *ctx, double *ms);
+/* New vertex data (same triangle count; positions, optional normals and
+ * texture coordinates as at upload) for geometry `geometry` of a scene
+ * uploaded with BTRC_ACCEL_FLAT or BTRC_ACCEL_FLAT_GPU: the geometry's
+ * shading and light-sampling records are rebuilt and the tree is refitted
+ * in place (same topology, every box the union of what it holds), so the
+ * next render sees the moved triangles without a rebuild. *ms (may be NULL)
+ * gets the update's wall time. No counterpart in the reference, which builds
+ * its structures once per commit (core/scene.cpp:92-239). */
+int btrc_hip_update_geometry(btrc_ctx *ctx, int32_t geometry, const btrc_geometry_desc *desc, double *ms);
 
 /* WavefrontPathTracer::set_film_filter (wavefront.cpp:72-75); box until set */
 int btrc_hip_set_filter(btrc_ctx *ctx, const btrc_filter_desc *filter);

@@ -140,3 +140,70 @@ def test_single_triangle_and_build_time(ctx):
     h = ctx.trace_closest(r)
     assert h["t"][0] == 1.0 and h["prim"][0] == 0
     assert ctx.scene_build_ms() > 0
+
+
+def displace(scene, geometry, seed):
+    g = scene.geometries[geometry]
+    rng = np.random.default_rng(seed)
+    p = g["positions"].reshape(-1, 3, 3)
+    # move every vertex shared by several triangles consistently: by a
+    # function of its position
+    off = 0.15 * np.sin(p * 3.0 + rng.uniform(0, 6.28, 3)).astype(np.float32)
+    g["positions"] = np.ascontiguousarray((p + off).reshape(-1, 9), dtype=np.float32)
+
+
+@pytest.mark.parametrize("accel", [_abi.ACCEL_FLAT, _abi.ACCEL_FLAT_GPU], ids=["host", "gpu"])
+def test_refit_after_update_traces_like_a_fresh_upload(ctx, accel):
+    """moving one geometry's vertices and refitting gives the hits of a
+    fresh upload of the moved scene, bit for bit"""
+    scene, cam = scenes.sphere_field(instances=256, lights=8, seed=5, extent=24.0)
+    scene.accel = accel
+    ctx.upload_scene(scene)
+    displace(scene, 0, 1)
+    ms = ctx.update_geometry(scene, 0)
+    assert ms > 0
+    rays = random_rays(200000, -2.0, 26.0, 8)
+    got = (ctx.trace_closest(rays), ctx.trace_any(rays))
+    ctx.upload_scene(scene)
+    want = (ctx.trace_closest(rays), ctx.trace_any(rays))
+    assert np.array_equal(got[0].view(np.uint8), want[0].view(np.uint8))
+    assert np.array_equal(got[1], want[1])
+
+
+def test_refit_after_update_renders_like_the_oracle(ctx):
+    """a moved emissive geometry: shading records, light sampling tables and
+    the tree all follow"""
+    scene, cam = scenes.sphere_field(instances=64, lights=8, seed=11, extent=16.0)
+    scene.accel = _abi.ACCEL_FLAT_GPU
+    ctx.upload_scene(scene)
+    for gi in range(len(scene.geometries)):
+        displace(scene, gi, 10 + gi)
+        ctx.update_geometry(scene, gi)
+    p = render_params(96, 64, 8, state_count=4096)
+    ctx.set_camera(cam, p.width, p.height)
+    stats = ctx.render(p)
+    img, _ = ctx.film(p.width, p.height)
+    ref, e, s = oracle.OracleScene(scene).render(cam, p)
+    ref /= p.spp
+    rel = np.abs(img[..., :3] - ref) / np.maximum(np.abs(ref), 1e-2)
+    assert rel.max() <= 1e-4, f"max rel err {rel.max()}"
+    assert stats.extension_rays == e and stats.shadow_rays == s
+
+
+def test_update_rejects_bad_requests(ctx):
+    import btrc_amd
+    scene, cam = scenes.sphere_field(instances=16, lights=2, seed=1, extent=8.0)
+    scene.accel = _abi.ACCEL_TWO_LEVEL
+    ctx.upload_scene(scene)
+    with pytest.raises(btrc_amd.BtrcException):
+        ctx.update_geometry(scene, 0)                  # two-level structure
+    scene.accel = _abi.ACCEL_FLAT
+    ctx.upload_scene(scene)
+    g = scene.geometries[0]
+    keep = g["positions"]
+    g["positions"] = keep[:-1].copy()
+    with pytest.raises(btrc_amd.BtrcException):
+        ctx.update_geometry(scene, 0)                  # triangle count changed
+    g["positions"] = keep
+    assert ctx.lib.btrc_hip_update_geometry(ctx.h, len(scene.geometries), None, None) == _abi.BTRC_ERR_INVALID
+    assert ctx.lib.btrc_hip_update_geometry(ctx.h, 0, None, None) == _abi.BTRC_ERR_INVALID
