@@ -470,22 +470,7 @@ def instanced_workload(args):
 
     # the same frame over one tree of all 16.8M world-space triangles built
     # on the GPU, against the two-level structure above
-    two_level_ms = ctx.scene_build_ms()
-    scene.accel = _abi.ACCEL_FLAT_GPU
-    ctx.upload_scene(scene)
-    flat_ms = ctx.scene_build_ms()
-    finfo = ctx.scene_info()
-    p.time_stages = 1
-    stf = ctx.render_views(p, views)
-    p.time_stages = 0
-    scene.accel = _abi.ACCEL_TWO_LEVEL
-    ctx.upload_scene(scene)
-    log(f"bench: flat tree of {finfo['triangles']} triangles built on the GPU in {flat_ms:.0f} ms "
-        f"({finfo['nodes']} BVH4 nodes): trace {stf.stage_ms[1]:.1f} ms vs two-level {stage[1]:.1f} ms")
-    bvh = dict(two_level_host_build_ms=round(two_level_ms, 1), flat_gpu_build_ms=round(flat_ms, 1),
-               flat_gpu_bvh4_nodes=finfo["nodes"], flat_gpu_bvh4_depth=finfo["depth"],
-               trace_ms_two_level=round(float(stage[1]), 3), trace_ms_flat_gpu_built=round(float(stf.stage_ms[1]), 3),
-               shadow_ms_two_level=round(float(stage[3]), 3), shadow_ms_flat_gpu_built=round(float(stf.stage_ms[3]), 3))
+    bvh = None if args.no_bvh_compare else instanced_flat_comparison(ctx, scene, p, views, stage)
 
     quality = None
     if args.gt_spp > 0:
@@ -534,6 +519,29 @@ def instanced_workload(args):
         "cpu_baseline": None,
         "bvh_build": bvh,
     }
+
+
+def instanced_flat_comparison(ctx, scene, p, views, stage):
+    """one stage-timed frame over a single tree of all world-space triangles
+    built on the GPU, against the two-level structure's stage times"""
+    from btrc_amd import _abi
+    two_level_ms = ctx.scene_build_ms()
+    scene.accel = _abi.ACCEL_FLAT_GPU
+    ctx.upload_scene(scene)
+    flat_ms = ctx.scene_build_ms()
+    finfo = ctx.scene_info()
+    p.time_stages = 1
+    stf = ctx.render_views(p, views)
+    p.time_stages = 0
+    scene.accel = _abi.ACCEL_TWO_LEVEL
+    ctx.upload_scene(scene)
+    log(f"bench: flat tree of {finfo['triangles']} triangles built on the GPU in {flat_ms:.0f} ms "
+        f"({finfo['nodes']} BVH4 nodes): trace {stf.stage_ms[1]:.1f} ms vs two-level {stage[1]:.1f} ms")
+    bvh = dict(two_level_host_build_ms=round(two_level_ms, 1), flat_gpu_build_ms=round(flat_ms, 1),
+               flat_gpu_bvh4_nodes=finfo["nodes"], flat_gpu_bvh4_depth=finfo["depth"],
+               trace_ms_two_level=round(float(stage[1]), 3), trace_ms_flat_gpu_built=round(float(stf.stage_ms[1]), 3),
+               shadow_ms_two_level=round(float(stage[3]), 3), shadow_ms_flat_gpu_built=round(float(stf.stage_ms[3]), 3))
+    return bvh
 
 
 def media_cpu_baseline(scene, cam, p, budget_s):
