@@ -96,3 +96,29 @@ def test_coplanar_hits_every_schedule(ctx):
                 ctx.set_tuning(closest_postpone_leaves=c, two_level_postpone=c, trace_waves_per_cu=w)
                 assert_hits_equal(ctx.trace_closest(rays), want)
     ctx.set_tuning(closest_postpone_leaves=1, two_level_postpone=0, trace_waves_per_cu=20)
+
+
+@pytest.mark.parametrize("two_level", [False, True], ids=["one_level", "two_level"])
+def test_autotune_picks_a_candidate_and_keeps_results(ctx, two_level):
+    """the tuner times every candidate, leaves the best one set, and the
+    frame it tunes renders the same paths before and after"""
+    from btrc_amd.autotune import autotune
+    scene, cam = scenes.sphere_field(instances=256, lights=8, seed=4, extent=24.0)
+    if two_level:
+        scene.accel = _abi.ACCEL_TWO_LEVEL
+    ctx.upload_scene(scene)
+    p = render_params(128, 96, 4, state_count=1 << 14)
+    ctx.set_camera(cam, p.width, p.height)
+    before = ctx.render(p)
+    img0, _ = ctx.film(p.width, p.height)
+    best, table = autotune(ctx, p, spp=1, waves=(16, 20), two_level=two_level)
+    assert len(table) == (8 if two_level else 8)
+    assert all(ms > 0 for _, ms, _ in table)
+    t = ctx.get_tuning()
+    for k, v in best.items():
+        assert t[k] == v
+    after = ctx.render(p)
+    img1, _ = ctx.film(p.width, p.height)
+    assert (before.extension_rays, before.shadow_rays) == (after.extension_rays, after.shadow_rays)
+    rel = np.abs(img0 - img1) / np.maximum(np.abs(img0), 1e-2)
+    assert rel.max() <= 1e-5
