@@ -94,8 +94,8 @@ def main():
                     help="samples per pixel of the FP64 ground truth for relMSE (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
-              f"the multi-GPU split of the frame is described in DESIGN.md", file=sys.stderr)
+        print(f"bench.py: the path tracer runs on one GPU only (asked for --gpus {args.gpus}); a frame would "
+              f"shard across GPUs by sample index with one film reduction, which is not built", file=sys.stderr)
         sys.exit(2)
 
     import btrc_amd
