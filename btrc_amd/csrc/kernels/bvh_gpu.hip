@@ -579,7 +579,8 @@ __global__ void k_gather_tris(const float *v, const uint32_t *tri_inst, const ui
     tris[3 * (size_t)k + 2] = make_float4(w[6], w[7], w[8], __uint_as_float(tri_prim[gid]));
 }
 
-__global__ void k_iota_frontier(uint32_t *frontier)
+// the first frontier: binary root, 4-wide node 0, nothing pushed above it
+__global__ void k_root_frontier(uint32_t *frontier)
 {
     frontier[0] = 0;
     frontier[1] = 0;
@@ -833,7 +834,7 @@ Bvh4Device ploc_to_bvh4(PlocBuffers &buf, const PlocResult &r, hipStream_t s)
         buf.nodes4_capacity = cap;
     }
     PLOC_CHECK(hipMemsetAsync(buf.counters, 0, 8 * sizeof(int32_t), s));
-    hipLaunchKernelGGL(k_iota_frontier, dim3(1), dim3(1), 0, s, buf.frontier);
+    hipLaunchKernelGGL(k_root_frontier, dim3(1), dim3(1), 0, s, buf.frontier);
     uint32_t f = 1, base = 1;
     Bvh4Device out;
     out.level_offsets.push_back(0);
