@@ -261,7 +261,7 @@ def gpu_bvh_comparison(ctx, scene, staged_params, host_trace_ms, host_frame_s):
     host_ms = ctx.scene_build_ms()
     accel = scene.accel
     scene.accel = _abi.ACCEL_FLAT_GPU
-    ctx.upload_scene(scene)           # first build sizes the builder's buffers
+    ctx.upload_scene(scene)           # first build loads the builder's kernels
     ctx.upload_scene(scene)
     gpu_ms = ctx.scene_build_ms()
     info = ctx.scene_info()

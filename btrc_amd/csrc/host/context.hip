@@ -204,7 +204,7 @@ struct btrc_ctx
     int64_t node_count = 0;
     double build_ms = 0.0;            // the last upload's acceleration-structure build
     double refit_ms = 0.0;            // the last btrc_hip_update_geometry
-    PlocBuffers ploc;                 // GPU builder scratch, kept for rebuilds
+    PlocBuffers ploc;                 // GPU builder scratch, released after a scene build
     // geometry updates of one-level structures: per-geometry ranges, the
     // 4-wide nodes level by level (root level first), the bounds of the
     // triangles and of the volume primitives
@@ -638,6 +638,7 @@ Accel build_flat_gpu(btrc_ctx *ctx, const std::vector<float> &world, const std::
     gather_triangles(dv.ptr, dinst.ptr, dprim.ptr, ctx->ploc.prims, ntri, ctx->tris.ptr, s);
     BTRC_HIP_CHECK(hipGetLastError());
     BTRC_HIP_CHECK(hipStreamSynchronize(s));
+    ctx->ploc.release(); // the builder's scratch is ~250 bytes per triangle; refits do not need it
     Accel acc;
     acc.on_device = true;
     acc.device_nodes = b4.node_count;
