@@ -88,6 +88,8 @@ def main():
                          "views rendered together, 1 spp per frame with the radiance cache for --frames frames "
                          "(64 when not given)")
     ap.add_argument("--frames", type=int, default=None)
+    ap.add_argument("--nrc-train-depth", type=int, default=0,
+                    help="nrc workload: end training paths at this depth with a cache query (0: trace them out)")
     ap.add_argument("--no-bvh-compare", action="store_true",
                     help="skip the frame traced through a GPU-built tree (field workload)")
     ap.add_argument("--gt-spp", type=int, default=1024,
@@ -319,7 +321,7 @@ def nrc_workload(args):
     checkpoints = sorted({c for c in (1, 2, 4, 8, 16, 32, 64, 128, 256, 512, 1024) if c <= args.frames} | {args.frames})
 
     def run(cache):
-        ctx.nrc_configure(enabled=1 if cache else 0, train=1)
+        ctx.nrc_configure(enabled=1 if cache else 0, train=1, train_depth=args.nrc_train_depth)
         if cache:
             ctx.nrc_reset(1)
         p = params(1)
@@ -387,7 +389,7 @@ def nrc_workload(args):
         "config": {"workload": f"Cornell-style room, 256x256, 1 spp per frame x {args.frames} frames, max depth 8, "
                                "radiance cache queried at depth 2, trained on 1/16 of the pixels (4 Adam steps of "
                                "16384 records per frame)",
-                   "width": W, "height": H, "frames": args.frames},
+                   "width": W, "height": H, "frames": args.frames, "train_depth": args.nrc_train_depth},
         "equal_time": {
             "ground_truth": "4096 spp FP64, independent sampler",
             "plain": {"seconds": round(plain["seconds"], 4), "relmse_curve": plain["curve"],

@@ -162,7 +162,7 @@ class Context:
 
     # ---- neural radiance cache
     NRC_DEFAULTS = dict(enabled=0, train=1, query_depth=2, train_stride=16, steps=4, learning_rate=1e-2,
-                        loss_scale=128.0)
+                        loss_scale=128.0, train_depth=0)
 
     def nrc_configure(self, **kw):
         unknown = set(kw) - set(self.NRC_DEFAULTS)

@@ -114,7 +114,7 @@ class RenderParams(C.Structure):
 class NrcParams(C.Structure):
     _fields_ = [("enabled", C.c_int32), ("train", C.c_int32), ("query_depth", C.c_int32),
                 ("train_stride", C.c_int32), ("steps", C.c_int32), ("learning_rate", C.c_float),
-                ("loss_scale", C.c_float), ("reserved", C.c_int32)]
+                ("loss_scale", C.c_float), ("train_depth", C.c_int32)]
 
 
 class NrcStats(C.Structure):

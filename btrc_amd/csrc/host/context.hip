@@ -1720,7 +1720,7 @@ void render(btrc_ctx *ctx, const btrc_render_params *p, int32_t view_count, cons
     rc.film_fp64 = p->film_fp64;
     rc.tile = p->tile != 0;
     rc.sample_offset = p->sample_offset;
-    NrcView nv{ -1, 0, nullptr, nullptr, nullptr, nullptr };
+    NrcView nv{ -1, 0, -1, nullptr, nullptr, nullptr, nullptr };
     if(nrc_on)
     {
         auto &n = ctx->nrc;
@@ -1731,7 +1731,8 @@ void render(btrc_ctx *ctx, const btrc_render_params *p, int32_t view_count, cons
         if(!n.params.train)
             rc.nrc_stride = 0;
         n.queries.alloc(std::max<size_t>(1, (size_t)pixels));
-        nv = NrcView{ n.params.query_depth, slots, n.queries.ptr, n.counts.ptr, n.rec.ptr, n.acc.ptr };
+        nv = NrcView{ n.params.query_depth, slots, n.params.train_depth > 0 ? n.params.train_depth : -1,
+                      n.queries.ptr, n.counts.ptr, n.rec.ptr, n.acc.ptr };
         launch_nrc_clear(s, n.rec.ptr, NRC_RECORDS, n.acc.ptr, NRC_RECORDS);
         BTRC_HIP_CHECK(hipMemsetAsync(n.counts.ptr, 0, 2 * sizeof(int32_t), s));
     }
@@ -1899,7 +1900,7 @@ void render(btrc_ctx *ctx, const btrc_render_params *p, int32_t view_count, cons
         const int igrid = (int)std::max<int64_t>(1, std::min<int64_t>(2 * ctx->cu_count, ((int64_t)pixels + 127) / 128));
         timer.run(STAGE_CACHE, [&] {
             launch_nrc_infer(s, igrid, nrc_weights(ctx), nrc_box(ctx), n.queries.ptr, n.counts.ptr, ctx->film.ptr,
-                             p->film_fp64 ? ctx->film64.ptr : nullptr);
+                             p->film_fp64 ? ctx->film64.ptr : nullptr, n.acc.ptr);
         });
         if(n.params.train)
         {
@@ -2290,7 +2291,8 @@ int btrc_hip_nrc_configure(btrc_ctx *ctx, const btrc_nrc_params *params)
             throw InvalidArg("cache parameters are null");
         const btrc_nrc_params &q = *params;
         if(q.query_depth < 0 || q.train_stride < 1 || q.steps < 0 || q.steps > 64 || !(q.learning_rate > 0.0f) ||
-           !(q.loss_scale > 0.0f) || !std::isfinite(q.learning_rate) || !std::isfinite(q.loss_scale))
+           !(q.loss_scale > 0.0f) || !std::isfinite(q.learning_rate) || !std::isfinite(q.loss_scale) ||
+           q.train_depth < 0 || (q.train_depth > 0 && q.train_depth <= q.query_depth))
             throw InvalidArg("bad cache parameters");
         BTRC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
         ctx->nrc.params = q;

@@ -123,9 +123,11 @@ __device__ void layer_forward(int l, const uint16_t *s_w, const uint16_t *x, uin
 
 // ---------------------------------------------------------------- inference
 
-// queries [0, n): y = max(net(x), 0); the film gets beta * y at the pixel
+// queries [0, n): y = max(net(x), 0); the film gets beta * y at the pixel,
+// and a training path's radiance sum too when the query is its tail (one
+// query per training path, after every other addition to its sum)
 __global__ void __launch_bounds__(256) k_nrc_infer(NrcWeights net, NrcBox box, const NrcQuery *q, const int32_t *count,
-                                                   float *film, double *film64)
+                                                   float *film, double *film64, float *acc)
 {
     __shared__ __attribute__((aligned(16))) uint16_t s_w[NRC_PARAMS];
     __shared__ __attribute__((aligned(16))) uint16_t s_x[4][2][TILE * NRC_IN];
@@ -167,6 +169,13 @@ __global__ void __launch_bounds__(256) k_nrc_infer(NrcWeights net, NrcBox box, c
                     atomicAdd(d + 0, (double)v[0]);
                     atomicAdd(d + 1, (double)v[1]);
                     atomicAdd(d + 2, (double)v[2]);
+                }
+                if(qi.train != NO_TRAIN)
+                {
+                    float *a = acc + 4 * (size_t)qi.train;
+                    a[0] = a[0] + v[0];
+                    a[1] = a[1] + v[1];
+                    a[2] = a[2] + v[2];
                 }
             }
         }
@@ -483,9 +492,9 @@ __global__ void k_nrc_publish(NrcWeights net)
 // ---------------------------------------------------------------- launchers
 
 void launch_nrc_infer(hipStream_t s, int grid, const NrcWeights &net, const NrcBox &box, const NrcQuery *q,
-                      const int32_t *count, float *film, double *film64)
+                      const int32_t *count, float *film, double *film64, float *acc)
 {
-    hipLaunchKernelGGL(k_nrc_infer, dim3(grid), dim3(256), 0, s, net, box, q, count, film, film64);
+    hipLaunchKernelGGL(k_nrc_infer, dim3(grid), dim3(256), 0, s, net, box, q, count, film, film64, acc);
 }
 
 void launch_nrc_eval(hipStream_t s, const NrcWeights &net, const NrcBox &box, const NrcVertex *v, int32_t n,

@@ -19,11 +19,15 @@ struct NrcWeights
 };
 
 // a cache query made at the vertex that ends a path's prefix
+// the training-path id of a path that is not one
+constexpr uint32_t NO_TRAIN = 0xffffffffu;
+
 struct NrcQuery
 {
     NrcVertex v;
     float beta[3];   // throughput the vertex was reached with
     uint32_t pixel;
+    uint32_t train;  // training path whose tail this is (its sum gets the answer too), or NO_TRAIN
 };
 
 // a training record: a vertex of an extended (training) path
@@ -41,6 +45,7 @@ struct NrcView
 {
     int32_t query_depth;     // vertex depth answered by the cache (-1: no queries)
     int32_t slots_per_path;  // record slots per training path (max depth + 1)
+    int32_t train_depth;     // training paths end here with a query (-1: traced to the end)
     NrcQuery *queries;
     int32_t *query_count;
     NrcRecord *rec;
@@ -94,7 +99,7 @@ struct NrcStepStats
 };
 
 void launch_nrc_infer(hipStream_t s, int grid, const NrcWeights &net, const NrcBox &box, const NrcQuery *q,
-                      const int32_t *count, float *film, double *film64);
+                      const int32_t *count, float *film, double *film64, float *acc);
 void launch_nrc_eval(hipStream_t s, const NrcWeights &net, const NrcBox &box, const NrcVertex *v, int32_t n,
                      float *out);
 void launch_nrc_train_step(hipStream_t s, const NrcWeights &net, const NrcBox &box, const NrcBatch &b,

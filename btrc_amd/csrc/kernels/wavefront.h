@@ -42,7 +42,6 @@ struct ShadowSOA
     float *train_acc; // radiance sums of the training paths (cache mode)
 };
 
-constexpr uint32_t NO_TRAIN = 0xffffffffu;
 
 constexpr uint32_t PATH_FLAG_HAS_INTERSECTION = 1u << 30; // wavefront/common.h:12
 constexpr uint32_t PATH_FLAG_HAS_SCATTERING = 1u << 31;

@@ -387,7 +387,20 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))
                     // record (throughput before Russian roulette, so that the
                     // target stays unbiased)
                     train_add(nrc.acc, train, le_rad);
-                    if(depth < nrc.slots_per_path)
+                    if(depth == nrc.train_depth && nrc.queries)
+                    {
+                        // self-training: the cache answers for the tail of
+                        // the training path; its answer joins the path's sum
+                        NrcQuery query;
+                        query.v = nrc_vertex(sv, inst, sp, d);
+                        query.beta[0] = beta.x; query.beta[1] = beta.y; query.beta[2] = beta.z;
+                        query.pixel = pixel;
+                        query.train = train;
+                        nrc.queries[wave_push(nrc.query_count, true)] = query;
+                        film_splat(film, pixel, path_rad + le_rad);
+                        done = true;
+                    }
+                    else if(depth < nrc.slots_per_path)
                     {
                         NrcRecord &r = nrc.rec[(size_t)train * nrc.slots_per_path + depth];
                         r.v = nrc_vertex(sv, inst, sp, d);
@@ -407,6 +420,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))
                         query.v = nrc_vertex(sv, inst, sp, d);
                         query.beta[0] = beta.x; query.beta[1] = beta.y; query.beta[2] = beta.z;
                         query.pixel = pixel;
+                        query.train = NO_TRAIN;
                         nrc.queries[wave_push(nrc.query_count, true)] = query;
                     }
                     film_splat(film, pixel, path_rad + le_rad);

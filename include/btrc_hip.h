@@ -287,7 +287,12 @@ typedef struct btrc_nrc_params
     int32_t steps;          /* optimizer steps per frame, default 4 */
     float learning_rate;    /* default 1e-2 */
     float loss_scale;       /* gradient scale of the BF16 backward pass, default 128 */
-    int32_t reserved;
+    /* 0 (default): training paths are traced to the end, so targets are
+     * unbiased path estimates. k > query_depth: training paths end at depth
+     * k with a cache query whose answer joins their radiance sums, so the
+     * bounce loop ends after about k bounces (the cache trains on its own
+     * tails, as the radiance cache paper's self-training does) */
+    int32_t train_depth;
 } btrc_nrc_params;
 
 typedef struct btrc_nrc_stats

@@ -149,3 +149,42 @@ def test_trained_cache_beats_truncated_paths(cornell_ctx):
     e_cache, e_trunc, e_plain = relmse(img, gt), relmse(trunc, gt), relmse(plain, gt)
     print(f"relMSE at 32 spp: cache {e_cache:.5f}, truncated {e_trunc:.5f}, plain {e_plain:.5f}")
     assert e_cache < 0.5 * e_trunc
+
+
+def test_self_training_ends_training_paths_early(cornell_ctx):
+    """train_depth k: training paths stop at depth k with a cache query, so
+    the frame's bounce loop is shorter; the cache still learns and still
+    beats the truncated paths"""
+    ctx, _, _ = cornell_ctx
+    w, h = 64, 64
+    ctx.nrc_configure(enabled=1, train=1, train_depth=0)
+    ctx.nrc_reset(9)
+    p = render_params(w, h, 1, min_depth=3, max_depth=8, state_count=1 << 14)
+    full = ctx.render(p)
+    ctx.nrc_configure(enabled=1, train=1, train_depth=4)
+    ctx.nrc_reset(9)
+    short = ctx.render(p)
+    # the loop checks for completion every few bounces, so compare, not count
+    assert short.iterations < full.iterations
+    import btrc_amd
+    with pytest.raises(btrc_amd.BtrcException):
+        ctx.nrc_configure(train_depth=2)        # not beyond the query depth
+    ctx.nrc_configure(train_depth=4)
+    img, stats = cache_frames(ctx, 96, 0, w, h)
+    assert all(math.isfinite(x) for s in stats for x in s.loss)
+    early = np.mean([np.mean(s.loss) for s in stats[:12]])
+    late = np.mean([np.mean(s.loss) for s in stats[-12:]])
+    assert late < early, (early, late)
+    g = render_params(w, h, 1024, min_depth=3, max_depth=8, state_count=1 << 18, film_fp64=1)
+    g.sampler = _abi.SAMPLER_INDEPENDENT
+    ctx.nrc_configure(enabled=0)
+    ctx.render(g)
+    gt = ctx.film(w, h, fp64=True)[1]
+    ctx.nrc_configure(enabled=1, train_depth=4)
+    img, _ = cache_frames(ctx, 32, 96, w, h, reset=False)
+    ctx.nrc_set_weights(np.zeros(_abi.NRC_PARAMS, dtype=np.float32))
+    trunc, _ = cache_frames(ctx, 32, 96, w, h, train=False, reset=False)
+    ctx.nrc_configure(train_depth=0)
+    e_cache, e_trunc = relmse(img, gt), relmse(trunc, gt)
+    print(f"self-training: relMSE at 32 spp: cache {e_cache:.5f}, truncated {e_trunc:.5f}")
+    assert e_cache < 0.5 * e_trunc
