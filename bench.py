@@ -88,8 +88,9 @@ def main():
                          "views rendered together, 1 spp per frame with the radiance cache for --frames frames "
                          "(64 when not given)")
     ap.add_argument("--frames", type=int, default=None)
-    ap.add_argument("--nrc-train-depth", type=int, default=0,
-                    help="nrc workload: end training paths at this depth with a cache query (0: trace them out)")
+    ap.add_argument("--nrc-train-depth", type=int, default=3,
+                    help="nrc workload: end training paths at this depth with a cache query (self-training; "
+                         "0: trace them out)")
     ap.add_argument("--no-bvh-compare", action="store_true",
                     help="skip the frame traced through a GPU-built tree (field workload)")
     ap.add_argument("--gt-spp", type=int, default=1024,
@@ -320,7 +321,9 @@ def nrc_workload(args):
 
     checkpoints = sorted({c for c in (1, 2, 4, 8, 16, 32, 64, 128, 256, 512, 1024) if c <= args.frames} | {args.frames})
 
-    def run(cache):
+    def run(cache, target=None):
+        # with a target, the run goes on past the frame budget (up to 4x,
+        # checkpoints every 16 frames) until its relMSE reaches the target
         ctx.nrc_configure(enabled=1 if cache else 0, train=1, train_depth=args.nrc_train_depth)
         if cache:
             ctx.nrc_reset(1)
@@ -329,7 +332,8 @@ def nrc_workload(args):
         curve, seconds, rays, stage = [], 0.0, 0, np.zeros(7)
         queries = records = 0
         losses = []
-        for f in range(args.frames):
+        f = 0
+        while f < args.frames or (target is not None and f < 4 * args.frames and curve[-1][2] > target):
             p.sample_offset, p.accumulate = f, 1 if f else 0
             st = ctx.render(p)
             seconds += st.seconds
@@ -340,17 +344,19 @@ def nrc_workload(args):
                 queries += ns.queries
                 records += ns.records
                 losses.append(float(np.mean(ns.loss[:min(4, max(1, ns.steps_total))])))
-            if f + 1 in checkpoints:
-                curve.append((f + 1, round(seconds, 4), relmse()))
+            f += 1
+            if f in checkpoints or (f > args.frames and f % 16 == 0):
+                curve.append((f, round(seconds, 4), relmse()))
         return dict(seconds=seconds, rays=rays, stage_ms=stage, curve=curve, queries=queries, records=records,
-                    losses=losses)
+                    losses=losses, frames=f)
 
     cpu = None if args.no_cpu else media_cpu_baseline(scene, cam, params(16), args.cpu_seconds)
     log("bench: plain path tracing")
     plain = run(False)
     log("bench: path tracing with the radiance cache")
-    cache = run(True)
+    cache = run(True, plain["curve"][-1][2])
     ctx.nrc_configure(enabled=0)
+    frames = cache["frames"]
 
     def time_to(curve, target):
         for frames, t, e in curve:
@@ -370,7 +376,19 @@ def nrc_workload(args):
 
     infer_ms = cache["stage_ms"][5]
     train_ms = cache["stage_ms"][6]
-    steps = 4 * args.frames
+    steps = 4 * frames
+
+    # the plain tracer's relMSE at the cache's time for the frame budget,
+    # interpolated on log-log axes between its checkpoints
+    def plain_at(t):
+        pts = plain["curve"]
+        for (f0, t0, e0), (f1, t1, e1) in zip(pts, pts[1:]):
+            if t0 <= t <= t1 and t1 > t0:
+                a = (np.log(t) - np.log(t0)) / (np.log(t1) - np.log(t0))
+                return float(np.exp(np.log(e0) + a * (np.log(e1) - np.log(e0))))
+        return None
+
+    cache_budget = [c for c in cache["curve"] if c[0] <= args.frames][-1]
     infer_tflops = NRC_FLOP_PER_QUERY * cache["queries"] / (infer_ms * 1e-3) / 1e12 if infer_ms > 0 else None
     train_tflops = NRC_FLOP_PER_TRAIN_SAMPLE * 16384 * steps / (train_ms * 1e-3) / 1e12 if train_ms > 0 else None
     return {
@@ -378,9 +396,9 @@ def nrc_workload(args):
         "value": round(cache["rays"] / cache["seconds"] / 1e6, 2),
         "unit": "Mrays/s",
         "n_gpus": 1,
-        "steps": args.frames,
+        "steps": frames,
         "warmup": 0,
-        "ms_per_step": round(cache["seconds"] / args.frames * 1e3, 3),
+        "ms_per_step": round(cache["seconds"] / frames * 1e3, 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -388,28 +406,32 @@ def nrc_workload(args):
         "data": "synthetic",
         "config": {"workload": f"Cornell-style room, 256x256, 1 spp per frame x {args.frames} frames, max depth 8, "
                                "radiance cache queried at depth 2, trained on 1/16 of the pixels (4 Adam steps of "
-                               "16384 records per frame)",
+                               "16384 records per frame); the cache run goes on (up to 4x) until it reaches the "
+                               "plain tracer's final relMSE",
                    "width": W, "height": H, "frames": args.frames, "train_depth": args.nrc_train_depth},
         "equal_time": {
             "ground_truth": "4096 spp FP64, independent sampler",
             "plain": {"seconds": round(plain["seconds"], 4), "relmse_curve": plain["curve"],
                       "mrays_per_s": round(plain["rays"] / plain["seconds"] / 1e6, 2)},
             "cache": {"seconds": round(cache["seconds"], 4), "relmse_curve": cache["curve"],
-                      "queries_per_frame": round(cache["queries"] / args.frames),
-                      "records_per_frame": round(cache["records"] / args.frames),
+                      "frames": frames,
+                      "queries_per_frame": round(cache["queries"] / frames),
+                      "records_per_frame": round(cache["records"] / frames),
                       "loss_first_last": [round(cache["losses"][0], 4), round(cache["losses"][-1], 4)]},
             "target_relmse": target,
             "cache_time_to_target_s": t_cache,
             "time_ratio": None if t_cache is None else round(t_cache / plain["seconds"], 3),
             "cache_relmse_at_plain_time": at_time(cache["curve"], plain["seconds"]),
+            "at_cache_time_for_frame_budget": {"seconds": cache_budget[1], "cache_relmse": cache_budget[2],
+                                               "plain_relmse": plain_at(cache_budget[1])},
         },
         "roofline": {"bound": "mfma", "kernel": "k_nrc_infer", "unit": "TFLOP/s", "peak": MFMA_BF16_PEAK_TFLOPS,
                      "achieved": None if infer_tflops is None else round(infer_tflops, 4),
                      "frac": None if infer_tflops is None else round(infer_tflops / MFMA_BF16_PEAK_TFLOPS, 7),
                      "traffic": None, "flop_per_query": NRC_FLOP_PER_QUERY,
                      "train_achieved": None if train_tflops is None else round(train_tflops, 4),
-                     "infer_ms_per_frame": round(infer_ms / args.frames, 4),
-                     "train_ms_per_frame": round(train_ms / args.frames, 4)},
+                     "infer_ms_per_frame": round(infer_ms / frames, 4),
+                     "train_ms_per_frame": round(train_ms / frames, 4)},
         "cpu_baseline": cpu,
     }
 
