@@ -91,6 +91,10 @@ def main():
     ap.add_argument("--nrc-train-depth", type=int, default=3,
                     help="nrc workload: end training paths at this depth with a cache query (self-training; "
                          "0: trace them out)")
+    ap.add_argument("--nrc-steps", type=int, default=4, help="nrc workload: optimizer steps per frame")
+    ap.add_argument("--nrc-query-depth", type=int, default=2, help="nrc workload: depth of the cache query")
+    ap.add_argument("--nrc-train-stride", type=int, default=16,
+                    help="nrc workload: one pixel in this many traces a training path")
     ap.add_argument("--no-bvh-compare", action="store_true",
                     help="skip the frame traced through a GPU-built tree (field workload)")
     ap.add_argument("--gt-spp", type=int, default=1024,
@@ -324,7 +328,9 @@ def nrc_workload(args):
     def run(cache, target=None):
         # with a target, the run goes on past the frame budget (up to 4x,
         # checkpoints every 16 frames) until its relMSE reaches the target
-        ctx.nrc_configure(enabled=1 if cache else 0, train=1, train_depth=args.nrc_train_depth)
+        ctx.nrc_configure(enabled=1 if cache else 0, train=1, train_depth=args.nrc_train_depth,
+                          steps=args.nrc_steps, train_stride=args.nrc_train_stride,
+                          query_depth=args.nrc_query_depth)
         if cache:
             ctx.nrc_reset(1)
         p = params(1)
@@ -376,7 +382,7 @@ def nrc_workload(args):
 
     infer_ms = cache["stage_ms"][5]
     train_ms = cache["stage_ms"][6]
-    steps = 4 * frames
+    steps = args.nrc_steps * frames
 
     # the plain tracer's relMSE at the cache's time for the frame budget,
     # interpolated on log-log axes between its checkpoints
@@ -405,10 +411,14 @@ def nrc_workload(args):
         "dtype": "f32 paths, bf16 network (fp32 accumulation)",
         "data": "synthetic",
         "config": {"workload": f"Cornell-style room, 256x256, 1 spp per frame x {args.frames} frames, max depth 8, "
-                               "radiance cache queried at depth 2, trained on 1/16 of the pixels (4 Adam steps of "
-                               "16384 records per frame); the cache run goes on (up to 4x) until it reaches the "
+                               f"radiance cache queried at depth {args.nrc_query_depth}, trained on 1/{args.nrc_train_stride} of the "
+                               f"pixels ({args.nrc_steps} Adam steps of 16384 records per frame, training paths "
+                               f"{'traced out' if args.nrc_train_depth == 0 else f'ended at depth {args.nrc_train_depth} with a cache query'}"
+                               "); the cache run goes on (up to 4x) until it reaches the "
                                "plain tracer's final relMSE",
-                   "width": W, "height": H, "frames": args.frames, "train_depth": args.nrc_train_depth},
+                   "width": W, "height": H, "frames": args.frames, "train_depth": args.nrc_train_depth,
+                   "steps_per_frame": args.nrc_steps, "train_stride": args.nrc_train_stride,
+                   "query_depth": args.nrc_query_depth},
         "equal_time": {
             "ground_truth": "4096 spp FP64, independent sampler",
             "plain": {"seconds": round(plain["seconds"], 4), "relmse_curve": plain["curve"],

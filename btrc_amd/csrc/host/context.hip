@@ -262,6 +262,7 @@ struct btrc_ctx
         DevBuf<double> loss_partial;
         DevBuf<int32_t> count_partial;
         DevBuf<float> grad_partial, step_scale;
+        DevBuf<double> g2_partial;
         DevBuf<NrcStepStats> step_stats;
         int64_t steps_total = 0;
         btrc_nrc_stats last{};
@@ -1465,6 +1466,7 @@ void nrc_alloc(btrc_ctx *ctx)
     n.loss_partial.alloc(NRC_RECORDS / 32);
     n.grad_partial.alloc((size_t)NRC_KSPLIT * NRC_PARAMS);
     n.count_partial.alloc(NRC_RECORDS / 32);
+    n.g2_partial.alloc(NRC_REDUCE_BLOCKS);
     n.step_scale.alloc(1);
     n.step_stats.alloc(64);
 }
@@ -1525,7 +1527,7 @@ NrcScratch nrc_scratch(btrc_ctx *ctx, int64_t batch)
     auto &n = ctx->nrc;
     (void)batch;
     return NrcScratch{ n.xt.ptr, n.dzt.ptr, n.loss_partial.ptr, n.count_partial.ptr, n.grad_partial.ptr,
-                       n.step_scale.ptr };
+                       n.step_scale.ptr, n.g2_partial.ptr };
 }
 
 // `steps` optimizer steps over the records, `batch` at a time: step k

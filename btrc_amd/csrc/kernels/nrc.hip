@@ -375,15 +375,31 @@ __global__ void __launch_bounds__(64) k_nrc_wgrad(NrcScratch sc, int64_t S, floa
 }
 
 // grad = sum over the NRC_KSPLIT slices, in slice order
-__global__ void __launch_bounds__(256) k_nrc_reduce(const float *partial, float *grad)
+// Also the workgroup's share of the squared gradient norm (FP64, fixed-order
+// tree), so the step's reductions read one value per workgroup.
+__global__ void __launch_bounds__(256) k_nrc_reduce(const float *partial, float *grad, double *g2_partial)
 {
+    __shared__ double s_g2[256];
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if(k >= NRC_PARAMS)
-        return;
-    float sum = 0.0f;
-    for(int c = 0; c < NRC_KSPLIT; ++c)
-        sum += partial[(size_t)c * NRC_PARAMS + k];
-    grad[k] = sum;
+    double g2 = 0.0;
+    if(k < NRC_PARAMS)
+    {
+        float sum = 0.0f;
+        for(int c = 0; c < NRC_KSPLIT; ++c)
+            sum += partial[(size_t)c * NRC_PARAMS + k];
+        grad[k] = sum;
+        g2 = (double)sum * (double)sum;
+    }
+    s_g2[threadIdx.x] = g2;
+    __syncthreads();
+    for(int off = 128; off > 0; off >>= 1)
+    {
+        if((int)threadIdx.x < off)
+            s_g2[threadIdx.x] += s_g2[threadIdx.x + off];
+        __syncthreads();
+    }
+    if(threadIdx.x == 0)
+        g2_partial[blockIdx.x] = s_g2[0];
 }
 
 // Reductions of a step (one workgroup, fixed order, FP64): valid samples n,
@@ -404,11 +420,8 @@ __global__ void __launch_bounds__(256) k_nrc_finalize(const float *grad, NrcScra
         cnt += sc.count_partial[k];
     }
     double g2 = 0.0;
-    for(int k = tid; k < NRC_PARAMS; k += 256)
-    {
-        const double g = (double)grad[k];
-        g2 += g * g;
-    }
+    for(int k = tid; k < NRC_REDUCE_BLOCKS; k += 256)
+        g2 += sc.g2_partial[k];
     s_a[tid] = loss;
     s_b[tid] = g2;
     s_c[tid] = cnt;
@@ -519,7 +532,7 @@ void launch_nrc_gradients(hipStream_t s, const NrcWeights &net, const NrcBox &bo
     const int blocks = (int)(b.batch / (2 * TILE));
     hipLaunchKernelGGL(k_nrc_train_fb, dim3(blocks), dim3(128), 0, s, net, box, b, sc);
     hipLaunchKernelGGL(k_nrc_wgrad, dim3(22 * NRC_KSPLIT), dim3(64), 0, s, sc, b.batch, sc.grad_partial);
-    hipLaunchKernelGGL(k_nrc_reduce, dim3((NRC_PARAMS + 255) / 256), dim3(256), 0, s, sc.grad_partial, grad);
+    hipLaunchKernelGGL(k_nrc_reduce, dim3(NRC_REDUCE_BLOCKS), dim3(256), 0, s, sc.grad_partial, grad, sc.g2_partial);
 }
 
 void launch_nrc_clear(hipStream_t s, NrcRecord *rec, int64_t nrec, float *acc, int64_t npaths)

@@ -81,7 +81,10 @@ struct NrcScratch
     int32_t *count_partial;  // valid samples, one per wavefront
     float *grad_partial;     // [NRC_KSPLIT][NRC_PARAMS]
     float *step_scale;       // summed gradient -> gradient of the mean loss; 0: skip the update
+    double *g2_partial;      // squared gradient norm, one per reduce workgroup
 };
+
+constexpr int NRC_REDUCE_BLOCKS = (NRC_PARAMS + 255) / 256;
 
 struct NrcAdam
 {
