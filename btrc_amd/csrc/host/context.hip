@@ -2687,7 +2687,7 @@ int btrc_hip_eval_medium(btrc_ctx *ctx, int32_t medium, const btrc_ray *segments
 int btrc_hip_eval_detmath(btrc_ctx *ctx, int32_t fn, const float *x, const float *y, int32_t n, float *out)
 {
     return guarded(ctx, [&] {
-        if(n < 0 || fn < 0 || fn > 8 || (n > 0 && (!x || !y || !out)))
+        if(n < 0 || fn < 0 || fn > 9 || (n > 0 && (!x || !y || !out)))
             throw InvalidArg("bad detmath query");
         if(n == 0)
             return;

@@ -951,6 +951,7 @@ __global__ void k_detmath_eval(int32_t fn, const float *x, const float *y, int32
     case 5: r = sqrtf(a); break;
     case 7: r = btrc_detmath::atan2(a, y[i]); break;
     case 8: r = btrc_detmath::acos(a); break;
+    case 9: r = intersection_offset(v3(a, 0.0f, 0.0f), v3(y[i], 0.0f, 0.0f)).x; break;
     default: r = a / y[i]; break;
     }
     out[i] = r;

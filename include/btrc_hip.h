@@ -505,7 +505,9 @@ int btrc_hip_envir_table(btrc_ctx *ctx, float *probs, int32_t capacity, int32_t 
  * floats per segment: scattered (0/1), position xyz, albedo xyz, tr xyz */
 int btrc_hip_eval_medium(btrc_ctx *ctx, int32_t medium, const btrc_ray *segments, const uint64_t *seeds, int32_t n,
                          float *out);
-/* fn: 0 sin, 1 cos, 2 log, 3 exp, 4 pow(x, y), 5 sqrt, 6 x / y, 7 atan2(x, y), 8 acos */
+/* fn: 0 sin, 1 cos, 2 log, 3 exp, 4 pow(x, y), 5 sqrt, 6 x / y, 7 atan2(x, y), 8 acos,
+ * 9 one axis of the self-intersection offset (x = position, y = normal coordinate;
+ *   intersection_offset, utils/intersection.cpp:5-31, acts on each axis alone) */
 int btrc_hip_eval_detmath(btrc_ctx *ctx, int32_t fn, const float *x, const float *y, int32_t n, float *out);
 
 #ifdef __cplusplus

@@ -3061,6 +3061,7 @@ void oracle_detmath(int32_t fn, const float *x, const float *y, int32_t n, float
         case 5: r = std::sqrt(a); break;
         case 7: r = dm::atan2(a, y[i]); break;
         case 8: r = dm::acos(a); break;
+        case 9: r = orc::offset_pos(orc::mk(a, 0.0f, 0.0f), orc::mk(y[i], 0.0f, 0.0f)).x; break;
         default: r = a / y[i]; break;
         }
         out[i] = r;

@@ -1,6 +1,14 @@
 """Closed-form scenes that pin the estimator (generate -> shade -> shadow, MIS,
 light sampling, Russian roulette) independently of any restatement: the CPU
-oracle and the GPU path tracer must both reproduce them."""
+oracle and the GPU path tracer must both reproduce them.
+
+Russian roulette (simple_russian_roulette, renderer/wavefront/helper.cpp:67-97)
+is pinned by the closed box with roulette on every bounce from depth 1:
+the pixel mean stays Le (1 - a^(D+1)) / (1 - a) only if survivors are divided
+by the continuation probability, and the number of extension rays per path
+follows the truncated geometric law of the survival draws (u <= p continues,
+luminance(beta) < threshold selects the roulette), which a swapped comparison
+or a missing threshold test changes."""
 import numpy as np
 import pytest
 
@@ -48,6 +56,31 @@ def check_box(img, albedo, Le, D):
     assert abs(m - want) < 5 * s + 1e-4 * want, (m, want, s)
 
 
+def rr_ray_law(m, D, p):
+    """mean and variance of extension rays per path in a closed box when
+    every vertex at depth m..D-1 plays roulette with continuation p: the
+    camera ray and depths 1..m always exist, then K survivals with
+    P(K = k) = p^k (1 - p) for k < D - m and P(K = D - m) = p^(D - m)"""
+    ks = np.arange(D - m + 1)
+    prob = p ** ks * (1 - p)
+    prob[-1] = p ** (D - m)
+    rays = m + 1 + ks
+    mean = float((prob * rays).sum())
+    return mean, float((prob * (rays - mean) ** 2).sum())
+
+
+def check_rr_rays(ext_rays, paths, m, D, p, rr_on):
+    if not rr_on:
+        assert ext_rays == paths * (D + 1)   # no roulette: every path reaches depth D
+        return
+    mean, var = rr_ray_law(m, D, p)
+    got = ext_rays / paths
+    assert abs(got - mean) < 5 * np.sqrt(var / paths), (got, mean)
+
+
+RR_CASES = [(0.5, 10.0), (0.8, 10.0), (0.8, 0.0)]  # (continuation probability, luminance threshold)
+
+
 def oracle_image(scene, cam, p):
     film, _, _ = oracle.OracleScene(scene).render(cam, p)
     return film / p.spp
@@ -64,6 +97,32 @@ def test_closed_box_oracle():
     D = 4
     p = render_params(16, 16, 32, min_depth=D, max_depth=D)
     check_box(oracle_image(scene, cam, p), 0.5, 2.0, D)
+
+
+@pytest.mark.parametrize("cont,thr", RR_CASES)
+def test_closed_box_rr_oracle(cont, thr):
+    """roulette at every bounce from depth 1 (threshold above any throughput
+    of the box) or never (threshold 0)"""
+    a, Le, m, D = 0.5, 2.0, 1, 6
+    scene, cam = closed_box_scene(a, Le)
+    p = render_params(16, 16, 32, min_depth=m, max_depth=D, rr_threshold=thr, rr_cont_prob=cont)
+    film, ext, _ = oracle.OracleScene(scene).render(cam, p)
+    check_box(film / p.spp, a, Le, D)
+    check_rr_rays(ext, 16 * 16 * 32, m, D, cont, thr > 0)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("cont,thr", RR_CASES)
+def test_closed_box_rr_gpu(ctx, cont, thr):
+    a, Le, m, D = 0.5, 2.0, 1, 6
+    scene, cam = closed_box_scene(a, Le)
+    p = render_params(128, 128, 64, min_depth=m, max_depth=D, rr_threshold=thr, rr_cont_prob=cont)
+    ctx.upload_scene(scene)
+    ctx.set_camera(cam, p.width, p.height)
+    stats = ctx.render(p)
+    img, _ = ctx.film(p.width, p.height)
+    check_box(img[..., :3].astype(np.float64), a, Le, D)
+    check_rr_rays(stats.extension_rays, 128 * 128 * 64, m, D, cont, thr > 0)
 
 
 @pytest.mark.gpu
