@@ -2,6 +2,7 @@
 1M-triangle scene at 1080p.
 
     python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--no-cpu]
+    python bench.py --probe {copy,frame,batch,nrc}   # short workloads for rocprofv3 counter runs
 
 A step renders one full frame: 1920x1080, 64 samples per pixel, max depth 8
 (Russian roulette from depth 3), 4096 Disney-shaded instances of a 256-triangle
@@ -52,6 +53,27 @@ def read_traffic(path):
     return float(m.group(1)) if m else None
 
 
+def host_threads():
+    """threads for the CPU baselines: the job's CPU share (OMP_NUM_THREADS,
+    16 per GPU on the benchmark hosts), at most the cores this process may
+    run on"""
+    share = int(os.environ.get("OMP_NUM_THREADS", "16") or 16)
+    return max(1, min(share, len(os.sched_getaffinity(0))))
+
+
+def host_info(threads):
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return dict(threads_used=threads, nproc=os.cpu_count(), affinity=len(os.sched_getaffinity(0)),
+                cpu_model=model)
+
+
 def cpu_baseline(rays, world, budget_s, threads):
     """lib/bvh traversal of the captured rays on `threads` host threads,
     repeated over the sample until about budget_s seconds of work"""
@@ -69,7 +91,76 @@ def cpu_baseline(rays, world, budget_s, threads):
     return dict(value=traced / dt / 1e6, unit="Mrays/s", cores=threads, kind="reference",
                 sample=f"{len(rays)} extension rays of the first two bounces of this frame, traced "
                        f"{traced // len(rays)} times by lib/bvh (BinnedSAH16 + SingleRayTraverser, closest "
-                       f"hit) on {threads} host threads, {dt:.1f} s"), (steps, isects)
+                       f"hit) on {threads} host threads, {dt:.1f} s", host=host_info(threads)), (steps, isects)
+
+
+def probe(args):
+    """Short fixed workloads for rocprofv3 runs, each printing the count that
+    its per-dispatch counters divide by:
+      copy  - one device-to-device copy of --mib MiB (torch's copy kernel),
+              to scale FETCH_SIZE / WRITE_SIZE readings to bytes;
+      frame - one 1080p frame of the 1M-triangle scene at --spp samples per
+              pixel (the benchmark frame at 64), with its ray counts;
+      batch - the extension rays of that frame's first two bounces (up to
+              --rays of them) traced --repeats times as one resident batch;
+      nrc   - --frames 1 spp frames of the Cornell-style room at 256^2 with
+              the radiance cache (queries at depth 2, training on 1/16 of the
+              pixels), with query and record counts."""
+    if args.probe == "copy":
+        import torch
+        n = args.mib * 1024 * 1024 // 4
+        x = torch.ones(n, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        torch.cuda.synchronize()
+        y.copy_(x)
+        torch.cuda.synchronize()
+        print(f"copy: {args.mib} MiB read + {args.mib} MiB written", flush=True)
+        return
+    import btrc_amd
+    from btrc_amd import _abi, scenes
+    ctx = btrc_amd.Context(0)
+    if args.probe == "nrc":
+        scene, cam = scenes.cornell()
+        ctx.upload_scene(scene)
+        ctx.set_camera(cam, 256, 256)
+        p = _abi.RenderParams()
+        p.width, p.height, p.spp = 256, 256, 1
+        p.min_depth, p.max_depth = MIN_DEPTH, MAX_DEPTH
+        p.rr_threshold, p.rr_cont_prob = 0.1, 0.5
+        p.state_count = 1 << 17
+        ctx.nrc_configure(enabled=1, train=1, train_depth=args.nrc_train_depth, steps=args.nrc_steps,
+                          train_stride=args.nrc_train_stride, query_depth=args.nrc_query_depth)
+        ctx.nrc_reset(1)
+        queries = records = 0
+        frames = args.frames or 64
+        for f in range(frames):
+            p.sample_offset, p.accumulate = f, 1 if f else 0
+            ctx.render(p)
+            ns = ctx.nrc_stats()
+            queries += ns.queries
+            records += ns.records
+        print(f"nrc: {frames} frames, {queries} queries, {records} training records, "
+              f"{frames * args.nrc_steps} optimizer steps", flush=True)
+        return
+    scene, cam = scenes.sphere_field()
+    ctx.upload_scene(scene)
+    ctx.set_camera(cam, WIDTH, HEIGHT)
+    p = _abi.RenderParams()
+    p.width, p.height, p.spp = WIDTH, HEIGHT, args.spp
+    p.min_depth, p.max_depth = MIN_DEPTH, MAX_DEPTH
+    p.rr_threshold, p.rr_cont_prob = 0.1, 0.5
+    p.state_count = STATE_COUNT
+    if args.probe == "frame":
+        st = ctx.render(p)
+        print(f"frame: {args.spp} spp, extension_rays {st.extension_rays} shadow_rays {st.shadow_rays} "
+              f"iterations {st.iterations}", flush=True)
+        return
+    r = ctx.capture_rays(p, 2, args.rays)
+    r = r[np.random.default_rng(0).permutation(len(r))]
+    ctx.batch_upload(r)
+    for _ in range(args.repeats):
+        ms = ctx.batch_trace(_abi.TRI_WATERTIGHT, False, 1)
+        print(f"batch: {len(r)} rays in {ms:.3f} ms ({len(r) / ms / 1e3:.1f} Mrays/s)", flush=True)
 
 
 def main():
@@ -95,15 +186,29 @@ def main():
     ap.add_argument("--nrc-query-depth", type=int, default=2, help="nrc workload: depth of the cache query")
     ap.add_argument("--nrc-train-stride", type=int, default=16,
                     help="nrc workload: one pixel in this many traces a training path")
+    ap.add_argument("--nrc-gt-spp", type=int, default=65536,
+                    help="samples per pixel of the radiance-cache comparison's FP64 ground truth")
+    ap.add_argument("--no-nrc", action="store_true",
+                    help="field workload: skip the radiance-cache equal-time comparison")
     ap.add_argument("--no-bvh-compare", action="store_true",
                     help="skip the frame traced through a GPU-built tree (field workload)")
-    ap.add_argument("--gt-spp", type=int, default=1024,
-                    help="samples per pixel of the FP64 ground truth for relMSE (0: skip)")
+    ap.add_argument("--gt-spp", type=int, default=None,
+                    help="samples per pixel of the FP64 ground truth for relMSE (0: skip; default 16384 for the "
+                         "field, 256 for media, 1024 for the instanced views)")
+    ap.add_argument("--probe", choices=["copy", "frame", "batch", "nrc"], default=None,
+                    help="run a short fixed workload for rocprofv3 instead of the benchmark")
+    ap.add_argument("--mib", type=int, default=1024, help="probe copy: MiB copied")
+    ap.add_argument("--rays", type=int, default=1 << 22, help="probe batch: rays captured")
+    ap.add_argument("--repeats", type=int, default=3, help="probe batch: traversals of the batch")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: the path tracer runs on one GPU only (asked for --gpus {args.gpus}); a frame would "
               f"shard across GPUs by sample index with one film reduction, which is not built", file=sys.stderr)
         sys.exit(2)
+
+    if args.probe:
+        probe(args)
+        return
 
     import btrc_amd
     from btrc_amd import scenes, _abi
@@ -114,10 +219,13 @@ def main():
         return
     if args.workload == "instanced":
         args.frames = 64 if args.frames is None else args.frames
+        args.gt_spp = 1024 if args.gt_spp is None else args.gt_spp
         print(json.dumps(instanced_workload(args)), flush=True)
         return
 
     media = args.workload == "media"
+    if args.gt_spp is None:
+        args.gt_spp = 256 if media else 16384
     max_depth = 16 if media else MAX_DEPTH
     if media:
         log("bench: building the media scene (128^3 fBm grid)")
@@ -167,13 +275,20 @@ def main():
     if args.gt_spp > 0:
         # FP64 ground truth from the independent sampler, so its samples share
         # nothing with the Halton samples of the timed frame
-        g = _abi.RenderParams.from_buffer_copy(p)
-        g.spp, g.film_fp64, g.time_stages, g.sampler = args.gt_spp, 1, 0, _abi.SAMPLER_INDEPENDENT
+        # rendered as two halves: their mutual relMSE is about four times the
+        # ground truth's own error
         t0 = time.perf_counter()
-        ctx.render(g)
-        gt = ctx.film(WIDTH, HEIGHT, fp64=True)[1]
+        halves = []
+        for k in range(2):
+            g = _abi.RenderParams.from_buffer_copy(p)
+            g.spp, g.film_fp64, g.time_stages, g.sampler = args.gt_spp // 2, 1, 0, _abi.SAMPLER_INDEPENDENT
+            g.sample_offset = k * (args.gt_spp // 2)
+            ctx.render(g)
+            halves.append(ctx.film(WIDTH, HEIGHT, fp64=True)[1])
+        gt = 0.5 * (halves[0] + halves[1])
         quality = dict(relmse=float(np.mean((frame - gt) ** 2 / (gt ** 2 + 1e-2))), spp=args.spp,
                        gt_spp=args.gt_spp, gt_sampler="independent", gt_film="fp64",
+                       gt_split_half_relmse=float(np.mean((halves[0] - halves[1]) ** 2 / (gt ** 2 + 1e-2))),
                        gt_seconds=round(time.perf_counter() - t0, 2))
         log(f"bench: relMSE of the {args.spp} spp frame vs {args.gt_spp} spp FP64 ground truth: "
             f"{quality['relmse']:.5f}")
@@ -196,8 +311,7 @@ def main():
         rays = rays[np.random.default_rng(0).permutation(len(rays))]
         # world-space triangles in flattened order, as the GPU traverses them
         world = flatten_instances(scene)
-        threads = min(16, os.cpu_count() or 1)
-        cpu, stats = cpu_baseline(rays, world, args.cpu_seconds, threads)
+        cpu, stats = cpu_baseline(rays, world, args.cpu_seconds, host_threads())
         if stats is not None:
             bytes_per_ray = algorithmic_bytes_per_ray(*stats)
             cpu["steps_per_ray"] = float(np.mean(stats[0]))
@@ -223,6 +337,11 @@ def main():
                         traffic_bytes_per_ray=traffic_per_ray)
 
     bvh = None if media or args.no_bvh_compare else gpu_bvh_comparison(ctx, scene, ps, trace_ms, staged["seconds"])
+
+    nrc = None
+    if not media and not args.no_nrc:
+        args.frames = 256 if args.frames is None else args.frames
+        nrc = nrc_equal_time(args, sorted({0, args.nrc_train_depth}))
 
     out = {
         "metric": METRIC,
@@ -256,6 +375,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "bvh_build": bvh,
+        "nrc_equal_time": nrc,
     }
     print(json.dumps(out), flush=True)
 
@@ -291,11 +411,18 @@ NRC_FLOP_PER_TRAIN_SAMPLE = 3 * NRC_FLOP_PER_QUERY            # forward, input g
 MFMA_BF16_PEAK_TFLOPS = 2500.0
 
 
-def nrc_workload(args):
-    """Cornell-style room at 256^2, `frames` frames of 1 spp accumulated:
-    plain path tracing, then the radiance cache (queries at depth 2, training
-    on 1/16 of the pixels), each timed per frame and scored by relMSE against
-    an FP64 ground truth of 4096 spp from the independent sampler."""
+def nrc_equal_time(args, train_depths):
+    """Cornell-style room at 256^2, `args.frames` frames of 1 spp accumulated:
+    plain path tracing, then the radiance cache (queries at depth
+    args.nrc_query_depth, training on 1/args.nrc_train_stride of the pixels)
+    once per training depth in `train_depths` (0: training paths traced out,
+    unbiased targets; d > 0: they end at depth d with a cache query,
+    self-training). Timed frames replay the captured bounce loop (no stage
+    timing); each cache run goes on past the frame budget (up to 4x) until
+    it reaches the plain tracer's final relMSE against an FP64 ground truth
+    of args.nrc_gt_spp samples per pixel from the independent sampler (two
+    halves, whose mutual relMSE bounds the ground truth's own error). A
+    separate stage-timed pass of 32 frames gives the cache kernels' times."""
     import btrc_amd
     from btrc_amd import _abi, scenes
     W = H = 256
@@ -307,17 +434,25 @@ def nrc_workload(args):
     def params(spp, fp64=0):
         p = _abi.RenderParams()
         p.width, p.height, p.spp = W, H, spp
-        p.min_depth, p.max_depth = 3, 8
+        p.min_depth, p.max_depth = MIN_DEPTH, MAX_DEPTH
         p.rr_threshold, p.rr_cont_prob = 0.1, 0.5
         p.state_count = 1 << 17
         p.film_fp64 = fp64
         return p
 
-    g = params(4096, 1)
-    g.state_count = 1 << 20
-    g.sampler = _abi.SAMPLER_INDEPENDENT
-    ctx.render(g)
-    gt = ctx.film(W, H, fp64=True)[1]
+    log(f"bench: cache comparison: {args.nrc_gt_spp} spp FP64 ground truth of the 256^2 room")
+    t0 = time.perf_counter()
+    halves = []
+    for k in range(2):
+        g = params(args.nrc_gt_spp // 2, 1)
+        g.state_count = 1 << 20
+        g.sampler = _abi.SAMPLER_INDEPENDENT
+        g.sample_offset = k * (args.nrc_gt_spp // 2)
+        ctx.render(g)
+        halves.append(ctx.film(W, H, fp64=True)[1])
+    gt = 0.5 * (halves[0] + halves[1])
+    gt_split = float(np.mean((halves[0] - halves[1]) ** 2 / (gt ** 2 + 1e-2)))
+    gt_seconds = time.perf_counter() - t0
 
     def relmse():
         img = ctx.film(W, H)[0][..., :3].astype(np.float64)
@@ -325,17 +460,17 @@ def nrc_workload(args):
 
     checkpoints = sorted({c for c in (1, 2, 4, 8, 16, 32, 64, 128, 256, 512, 1024) if c <= args.frames} | {args.frames})
 
-    def run(cache, target=None):
-        # with a target, the run goes on past the frame budget (up to 4x,
-        # checkpoints every 16 frames) until its relMSE reaches the target
-        ctx.nrc_configure(enabled=1 if cache else 0, train=1, train_depth=args.nrc_train_depth,
-                          steps=args.nrc_steps, train_stride=args.nrc_train_stride,
-                          query_depth=args.nrc_query_depth)
+    def configure(cache, train_depth):
+        ctx.nrc_configure(enabled=1 if cache else 0, train=1, train_depth=train_depth, steps=args.nrc_steps,
+                          train_stride=args.nrc_train_stride, query_depth=args.nrc_query_depth)
         if cache:
             ctx.nrc_reset(1)
+
+    def run(cache, train_depth=0, target=None):
+        configure(cache, train_depth)
         p = params(1)
-        p.time_stages = 1
-        curve, seconds, rays, stage = [], 0.0, 0, np.zeros(7)
+        p.time_stages = 0
+        curve, seconds, rays = [], 0.0, 0
         queries = records = 0
         losses = []
         f = 0
@@ -344,25 +479,29 @@ def nrc_workload(args):
             st = ctx.render(p)
             seconds += st.seconds
             rays += st.extension_rays + st.shadow_rays
-            stage += np.array(st.stage_ms[:7])
             if cache:
                 ns = ctx.nrc_stats()
                 queries += ns.queries
                 records += ns.records
-                losses.append(float(np.mean(ns.loss[:min(4, max(1, ns.steps_total))])))
+                losses.append(float(np.mean(ns.loss[:min(args.nrc_steps, max(1, ns.steps_total))])))
             f += 1
             if f in checkpoints or (f > args.frames and f % 16 == 0):
                 curve.append((f, round(seconds, 4), relmse()))
-        return dict(seconds=seconds, rays=rays, stage_ms=stage, curve=curve, queries=queries, records=records,
-                    losses=losses, frames=f)
+        return dict(seconds=seconds, rays=rays, curve=curve, queries=queries, records=records, losses=losses,
+                    frames=f)
 
-    cpu = None if args.no_cpu else media_cpu_baseline(scene, cam, params(16), args.cpu_seconds)
-    log("bench: plain path tracing")
-    plain = run(False)
-    log("bench: path tracing with the radiance cache")
-    cache = run(True, plain["curve"][-1][2])
-    ctx.nrc_configure(enabled=0)
-    frames = cache["frames"]
+    def staged(cache, train_depth=0, frames=32):
+        configure(cache, train_depth)
+        p = params(1)
+        p.time_stages = 1
+        stage, queries = np.zeros(7), 0
+        for f in range(frames):
+            p.sample_offset, p.accumulate = f, 1 if f else 0
+            st = ctx.render(p)
+            stage += np.array(st.stage_ms[:7])
+            if cache:
+                queries += ctx.nrc_stats().queries
+        return stage / frames, queries / frames
 
     def time_to(curve, target):
         for frames, t, e in curve:
@@ -370,9 +509,6 @@ def nrc_workload(args):
                 return t
         return None
 
-    target = plain["curve"][-1][2]
-    t_cache = time_to(cache["curve"], target)
-    # relMSE of each method at the other's total time, from the checkpoints
     def at_time(curve, t):
         best = None
         for frames, tt, e in curve:
@@ -380,13 +516,12 @@ def nrc_workload(args):
                 best = e
         return best
 
-    infer_ms = cache["stage_ms"][5]
-    train_ms = cache["stage_ms"][6]
-    steps = args.nrc_steps * frames
+    log("bench: plain path tracing, 1 spp frames")
+    plain = run(False)
+    target = plain["curve"][-1][2]
 
-    # the plain tracer's relMSE at the cache's time for the frame budget,
-    # interpolated on log-log axes between its checkpoints
     def plain_at(t):
+        """the plain tracer's relMSE at time t, log-log interpolated between checkpoints"""
         pts = plain["curve"]
         for (f0, t0, e0), (f1, t1, e1) in zip(pts, pts[1:]):
             if t0 <= t <= t1 and t1 > t0:
@@ -394,54 +529,88 @@ def nrc_workload(args):
                 return float(np.exp(np.log(e0) + a * (np.log(e1) - np.log(e0))))
         return None
 
-    cache_budget = [c for c in cache["curve"] if c[0] <= args.frames][-1]
-    infer_tflops = NRC_FLOP_PER_QUERY * cache["queries"] / (infer_ms * 1e-3) / 1e12 if infer_ms > 0 else None
-    train_tflops = NRC_FLOP_PER_TRAIN_SAMPLE * 16384 * steps / (train_ms * 1e-3) / 1e12 if train_ms > 0 else None
+    runs = {}
+    for d in train_depths:
+        log(f"bench: path tracing with the radiance cache, training depth {d}")
+        c = run(True, d, target)
+        t_cache = time_to(c["curve"], target)
+        budget = [x for x in c["curve"] if x[0] <= args.frames][-1]
+        stage, qpf = staged(True, d)
+        infer_ms, train_ms = stage[5], stage[6]
+        infer_tflops = NRC_FLOP_PER_QUERY * qpf / (infer_ms * 1e-3) / 1e12 if infer_ms > 0 else None
+        train_tflops = (NRC_FLOP_PER_TRAIN_SAMPLE * 16384 * args.nrc_steps / (train_ms * 1e-3) / 1e12
+                        if train_ms > 0 else None)
+        runs[f"train_depth_{d}"] = {
+            "seconds": round(c["seconds"], 4), "frames": c["frames"], "relmse_curve": c["curve"],
+            "mrays_per_s": round(c["rays"] / c["seconds"] / 1e6, 2),
+            "queries_per_frame": round(c["queries"] / c["frames"]),
+            "records_per_frame": round(c["records"] / c["frames"]),
+            "loss_first_last": [round(c["losses"][0], 4), round(c["losses"][-1], 4)],
+            "time_to_target_s": t_cache,
+            "time_ratio": None if t_cache is None else round(t_cache / plain["seconds"], 3),
+            "relmse_at_plain_time": at_time(c["curve"], plain["seconds"]),
+            "at_frame_budget": {"seconds": budget[1], "cache_relmse": budget[2],
+                                "plain_relmse_at_that_time": plain_at(budget[1])},
+            "stage_ms_per_frame": {k: round(float(v), 4) for k, v in
+                                   zip(("generate", "trace", "shade", "shadow", "medium", "cache", "train"), stage)},
+            "infer_tflops": None if infer_tflops is None else round(infer_tflops, 3),
+            "train_tflops": None if train_tflops is None else round(train_tflops, 3),
+        }
+        log(f"bench: cache (training depth {d}): time ratio {runs[f'train_depth_{d}']['time_ratio']}")
+    ctx.nrc_configure(enabled=0)
+    plain_stage, _ = staged(False)
+    return {
+        "workload": f"Cornell-style room, 256x256, 1 spp per frame x {args.frames} frames, max depth {MAX_DEPTH}, "
+                    f"radiance cache queried at depth {args.nrc_query_depth}, trained on 1/{args.nrc_train_stride} "
+                    f"of the pixels ({args.nrc_steps} Adam steps of 16384 records per frame); timed frames replay "
+                    "the captured bounce loop",
+        "ground_truth": {"spp": args.nrc_gt_spp, "sampler": "independent", "film": "fp64",
+                         "split_half_relmse": gt_split, "seconds": round(gt_seconds, 2)},
+        "target_relmse": target,
+        "plain": {"seconds": round(plain["seconds"], 4), "relmse_curve": plain["curve"],
+                  "mrays_per_s": round(plain["rays"] / plain["seconds"] / 1e6, 2),
+                  "stage_ms_per_frame": {k: round(float(v), 4) for k, v in
+                                         zip(("generate", "trace", "shade", "shadow"), plain_stage[:4])}},
+        "cache": runs,
+    }
+
+
+def nrc_workload(args):
+    """the radiance-cache comparison as its own benchmark line: value = the
+    cache run's Mrays/s at the chosen training depth"""
+    cmp_ = nrc_equal_time(args, sorted({0, args.nrc_train_depth}))
+    main_run = cmp_["cache"][f"train_depth_{args.nrc_train_depth}"]
+    infer = main_run["infer_tflops"]
+    cpu = None
+    if not args.no_cpu:
+        from btrc_amd import _abi, scenes
+        scene, cam = scenes.cornell()
+        p = _abi.RenderParams()
+        p.width, p.height, p.spp = 256, 256, 16
+        p.min_depth, p.max_depth = MIN_DEPTH, MAX_DEPTH
+        p.rr_threshold, p.rr_cont_prob = 0.1, 0.5
+        cpu = media_cpu_baseline(scene, cam, p, args.cpu_seconds)
     return {
         "metric": METRIC,
-        "value": round(cache["rays"] / cache["seconds"] / 1e6, 2),
+        "value": main_run["mrays_per_s"],
         "unit": "Mrays/s",
         "n_gpus": 1,
-        "steps": frames,
+        "steps": main_run["frames"],
         "warmup": 0,
-        "ms_per_step": round(cache["seconds"] / frames * 1e3, 3),
+        "ms_per_step": round(main_run["seconds"] / main_run["frames"] * 1e3, 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32 paths, bf16 network (fp32 accumulation)",
         "data": "synthetic",
-        "config": {"workload": f"Cornell-style room, 256x256, 1 spp per frame x {args.frames} frames, max depth 8, "
-                               f"radiance cache queried at depth {args.nrc_query_depth}, trained on 1/{args.nrc_train_stride} of the "
-                               f"pixels ({args.nrc_steps} Adam steps of 16384 records per frame, training paths "
-                               f"{'traced out' if args.nrc_train_depth == 0 else f'ended at depth {args.nrc_train_depth} with a cache query'}"
-                               "); the cache run goes on (up to 4x) until it reaches the "
-                               "plain tracer's final relMSE",
-                   "width": W, "height": H, "frames": args.frames, "train_depth": args.nrc_train_depth,
-                   "steps_per_frame": args.nrc_steps, "train_stride": args.nrc_train_stride,
-                   "query_depth": args.nrc_query_depth},
-        "equal_time": {
-            "ground_truth": "4096 spp FP64, independent sampler",
-            "plain": {"seconds": round(plain["seconds"], 4), "relmse_curve": plain["curve"],
-                      "mrays_per_s": round(plain["rays"] / plain["seconds"] / 1e6, 2)},
-            "cache": {"seconds": round(cache["seconds"], 4), "relmse_curve": cache["curve"],
-                      "frames": frames,
-                      "queries_per_frame": round(cache["queries"] / frames),
-                      "records_per_frame": round(cache["records"] / frames),
-                      "loss_first_last": [round(cache["losses"][0], 4), round(cache["losses"][-1], 4)]},
-            "target_relmse": target,
-            "cache_time_to_target_s": t_cache,
-            "time_ratio": None if t_cache is None else round(t_cache / plain["seconds"], 3),
-            "cache_relmse_at_plain_time": at_time(cache["curve"], plain["seconds"]),
-            "at_cache_time_for_frame_budget": {"seconds": cache_budget[1], "cache_relmse": cache_budget[2],
-                                               "plain_relmse": plain_at(cache_budget[1])},
-        },
+        "config": {"workload": cmp_["workload"], "width": 256, "height": 256, "frames": args.frames,
+                   "train_depth": args.nrc_train_depth, "steps_per_frame": args.nrc_steps,
+                   "train_stride": args.nrc_train_stride, "query_depth": args.nrc_query_depth},
+        "equal_time": cmp_,
         "roofline": {"bound": "mfma", "kernel": "k_nrc_infer", "unit": "TFLOP/s", "peak": MFMA_BF16_PEAK_TFLOPS,
-                     "achieved": None if infer_tflops is None else round(infer_tflops, 4),
-                     "frac": None if infer_tflops is None else round(infer_tflops / MFMA_BF16_PEAK_TFLOPS, 7),
+                     "achieved": infer, "frac": None if infer is None else round(infer / MFMA_BF16_PEAK_TFLOPS, 6),
                      "traffic": None, "flop_per_query": NRC_FLOP_PER_QUERY,
-                     "train_achieved": None if train_tflops is None else round(train_tflops, 4),
-                     "infer_ms_per_frame": round(infer_ms / frames, 4),
-                     "train_ms_per_frame": round(train_ms / frames, 4)},
+                     "train_achieved": main_run["train_tflops"]},
         "cpu_baseline": cpu,
     }
 
@@ -583,7 +752,7 @@ def media_cpu_baseline(scene, cam, p, budget_s):
     host threads over the first paths of the frame, about budget_s seconds"""
     from oracle import oracle
     os_ = oracle.OracleScene(scene)
-    threads = min(16, os.cpu_count() or 1)
+    threads = host_threads()
     q = type(p).from_buffer_copy(p)
     count, rays, dt = 1 << 14, 0, 0.0
     done = 0
@@ -596,7 +765,7 @@ def media_cpu_baseline(scene, cam, p, budget_s):
         count *= 2
     return dict(value=rays / dt / 1e6, unit="Mrays/s", cores=threads, kind="port",
                 sample=f"first {done} paths of the frame traced by the oracle's scalar path tracer "
-                       f"(oracle/oracle.cpp) on {threads} host threads, {dt:.1f} s")
+                       f"(oracle/oracle.cpp) on {threads} host threads, {dt:.1f} s", host=host_info(threads))
 
 
 def flatten_instances(scene):

@@ -3068,6 +3068,22 @@ void oracle_detmath(int32_t fn, const float *x, const float *y, int32_t n, float
     }
 }
 
+// ACESToneMap, postprocess/tonemap.cpp:14-24,35-51: the Narkowicz fit of
+// exposure-scaled RGB, saturated; alpha unchanged
+void oracle_aces(const float *rgba, int32_t n, float exposure, float *out)
+{
+    for(int32_t i = 0; i < n; ++i)
+    {
+        for(int c = 0; c < 3; ++c)
+        {
+            const float x = rgba[4 * i + c] * exposure;
+            const float y = (x * (2.51f * x + 0.03f)) / (x * (2.43f * x + 0.59f) + 0.14f);
+            out[4 * i + c] = y < 0.0f ? 0.0f : (y > 1.0f ? 1.0f : y);
+        }
+        out[4 * i + 3] = rgba[4 * i + 3];
+    }
+}
+
 // BSDF evaluation hooks for the statistical tests: a material at a surface
 // whose geometric and shading frames are the identity frame.
 void oracle_bsdf_sample(const btrc_material_desc *m, const float *wo, const float *u, int32_t n,

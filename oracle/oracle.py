@@ -56,6 +56,7 @@ def lib():
         L.oracle_bsdf_eval.argtypes = [P(_abi.MaterialDesc), vp, vp, C.c_int32, vp, vp]
         L.oracle_kat_rays.argtypes = [C.c_int32, C.c_int32, vp, vp, vp, C.c_float, vp]
         L.oracle_kat_pixels.argtypes = [C.c_int32, C.c_int32, vp, vp, vp]
+        L.oracle_aces.argtypes = [vp, C.c_int32, C.c_float, vp]
         L.oracle_halton_index.restype = C.c_uint64
         L.oracle_halton_index.argtypes = [C.c_int32, C.c_int32, C.c_uint32, C.c_uint32, C.c_int32]
         _lib = L
@@ -248,6 +249,14 @@ def halton(width, height, px, py, sample_index, dims):
     out = np.empty((len(px), dims), dtype=np.float32)
     lib().oracle_halton(width, height, px.ctypes.data, py.ctypes.data, si.ctypes.data, len(px), dims,
                         out.ctypes.data)
+    return out
+
+
+def aces(rgba, exposure=1.0):
+    """ACES tone map (Narkowicz fit) of (..., 4) float32 RGBA"""
+    a = np.ascontiguousarray(rgba, dtype=np.float32)
+    out = np.empty_like(a)
+    lib().oracle_aces(a.ctypes.data, a.size // 4, exposure, out.ctypes.data)
     return out
 
 
