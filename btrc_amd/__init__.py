@@ -229,6 +229,12 @@ class Context:
                                                     hits.ctypes.data))
         return hits
 
+    def trace_closest_noalloc(self, rays_ptr, n, hits_ptr, tri_test=_abi.TRI_WATERTIGHT):
+        """closest hits of n rays already in device memory (n btrc_ray at
+        device address rays_ptr) into device memory (n btrc_hit at hits_ptr);
+        allocates nothing"""
+        self._check(self.lib.btrc_hip_trace_closest_noalloc(self.h, rays_ptr, n, tri_test, hits_ptr))
+
     def trace_any(self, rays, tri_test=_abi.TRI_WATERTIGHT):
         rays = np.ascontiguousarray(rays, dtype=_abi.ray_dtype())
         occ = np.empty(len(rays), dtype=np.uint8)

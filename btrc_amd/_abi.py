@@ -183,7 +183,7 @@ EXPORTS = [
     "btrc_hip_aov_readback", "btrc_hip_request_stop",
     "btrc_hip_nrc_configure", "btrc_hip_nrc_reset", "btrc_hip_nrc_stats", "btrc_hip_nrc_get_weights",
     "btrc_hip_nrc_set_weights", "btrc_hip_nrc_eval", "btrc_hip_nrc_gradients", "btrc_hip_nrc_fit",
-    "btrc_hip_trace_closest", "btrc_hip_trace_any",
+    "btrc_hip_trace_closest", "btrc_hip_trace_any", "btrc_hip_trace_closest_noalloc",
     "btrc_hip_batch_upload", "btrc_hip_batch_trace", "btrc_hip_batch_download",
     "btrc_hip_capture_rays", "btrc_hip_build_bvh", "btrc_hip_eval_halton", "btrc_hip_eval_independent", "btrc_hip_eval_volumes", "btrc_hip_eval_texture",
     "btrc_hip_eval_envir", "btrc_hip_envir_table", "btrc_hip_eval_medium",
@@ -226,6 +226,7 @@ def declare(lib):
         "btrc_hip_nrc_fit": (C.c_int, [ctx, C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_void_p]),
         "btrc_hip_trace_closest": (C.c_int, [ctx, C.c_void_p, C.c_int64, C.c_int32, C.c_void_p]),
         "btrc_hip_trace_any": (C.c_int, [ctx, C.c_void_p, C.c_int64, C.c_int32, C.c_void_p]),
+        "btrc_hip_trace_closest_noalloc": (C.c_int, [ctx, C.c_void_p, C.c_int64, C.c_int32, C.c_void_p]),
         "btrc_hip_batch_upload": (C.c_int, [ctx, C.c_void_p, C.c_int64]),
         "btrc_hip_batch_trace": (C.c_int, [ctx, C.c_int32, C.c_int32, C.c_int32, P(C.c_double)]),
         "btrc_hip_batch_download": (C.c_int, [ctx, C.c_void_p]),

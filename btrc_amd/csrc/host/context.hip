@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <stdexcept>
 #include <string>
@@ -18,7 +19,65 @@
 
 using namespace btrc;
 
+// Scratch of btrc_hip_trace_closest_noalloc, for callers that cannot
+// allocate: the batch's ray counter in the first 256 bytes, then the
+// traversal's per-lane stack entries beyond the LDS part (codes, then entry
+// distances, each [entry][lane]). Every GPU the library runs on holds one
+// copy until the process ends. Host code never uses the array's name as an
+// address: it resolves the device address once per device
+// (hipGetSymbolAddress, at context creation) and uses that.
+constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)8 << 20;
+constexpr size_t NOALLOC_COUNTER_BYTES = 256;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
 namespace {
+
+// device address of `scratch` per device, resolved once per device with that
+// device current; null with a message when the lookup failed
+struct ScratchAddress
+{
+    std::mutex lock;
+    static constexpr int MAX_DEVICES = 64;
+    bool resolved[MAX_DEVICES] = {};
+    void *ptr[MAX_DEVICES] = {};
+    std::string error[MAX_DEVICES];
+};
+
+ScratchAddress &scratch_address()
+{
+    static ScratchAddress a;
+    return a;
+}
+
+// called by btrc_hip_create with `device` current
+void resolve_scratch(int device, void **ptr, std::string *error)
+{
+    ScratchAddress &a = scratch_address();
+    std::lock_guard<std::mutex> g(a.lock);
+    if(device < 0 || device >= ScratchAddress::MAX_DEVICES)
+    {
+        *ptr = nullptr;
+        *error = "no-allocation scratch: device index beyond the table";
+        return;
+    }
+    if(!a.resolved[device])
+    {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if(e != hipSuccess || !p)
+        {
+            (void)hipGetLastError();
+            a.ptr[device] = nullptr;
+            a.error[device] = std::string("no-allocation scratch: hipGetSymbolAddress failed: ") +
+                              hipGetErrorString(e);
+        }
+        else
+            a.ptr[device] = p;
+        a.resolved[device] = true;
+    }
+    *ptr = a.ptr[device];
+    *error = a.error[device];
+}
 
 struct HipError : std::runtime_error
 {
@@ -170,6 +229,9 @@ struct btrc_ctx
     std::atomic<bool> stop{ false };
     int cu_count = 256;
     std::string device_name;
+    // this device's copy of `scratch` (btrc_hip_trace_closest_noalloc)
+    void *noalloc_scratch = nullptr;
+    std::string noalloc_error;
 
     // scene
     bool has_scene = false;
@@ -2030,6 +2092,7 @@ int btrc_hip_create(int device, btrc_ctx **out)
         ctx->cu_count = prop.multiProcessorCount;
         ctx->device_name = prop.gcnArchName;
     }
+    resolve_scratch(device, &ctx->noalloc_scratch, &ctx->noalloc_error);
     *out = ctx;
     return BTRC_OK;
 }
@@ -2487,6 +2550,64 @@ int btrc_hip_trace_any(btrc_ctx *ctx, const btrc_ray *rays, int64_t n, int32_t t
         BTRC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
         for(int64_t i = 0; i < n; ++i)
             occluded[i] = (uint8_t)tmp[i];
+    });
+}
+
+int btrc_hip_trace_closest_noalloc(btrc_ctx *ctx, const btrc_ray *rays, int64_t n, int32_t tri_test, btrc_hit *hits)
+{
+    return guarded(ctx, [&] {
+        check_batch_args(ctx, rays, n, tri_test);
+        if(n > 0 && !hits)
+            throw InvalidArg("hits is null");
+        if(n == 0)
+            return;
+        if(!ctx->noalloc_scratch)
+            throw StateError(ctx->noalloc_error);
+        // rays and hits must be device allocations that hold n entries
+        auto check_range = [&](const void *p, size_t bytes, const char *what) {
+            hipPointerAttribute_t attr;
+            if(hipPointerGetAttributes(&attr, p) != hipSuccess ||
+               (attr.type != hipMemoryTypeDevice && attr.type != hipMemoryTypeManaged))
+            {
+                (void)hipGetLastError();
+                throw InvalidArg(std::string(what) + " is not device memory");
+            }
+            hipDeviceptr_t base = nullptr;
+            size_t size = 0;
+            if(hipMemGetAddressRange(&base, &size, const_cast<void *>(p)) != hipSuccess)
+            {
+                (void)hipGetLastError();
+                throw InvalidArg(std::string(what) + ": allocation range unknown");
+            }
+            const uintptr_t b = reinterpret_cast<uintptr_t>(base), q = reinterpret_cast<uintptr_t>(p);
+            if(q < b || q + bytes > b + size)
+                throw InvalidArg(std::string(what) + " allocation is smaller than n entries");
+        };
+        check_range(rays, (size_t)n * sizeof(btrc_ray), "rays");
+        check_range(hits, (size_t)n * sizeof(btrc_hit), "hits");
+        // the persistent grid, capped so that its stack entries fit the scratch
+        const size_t deep = ctx->stack_size > SHORT_STACK ? (size_t)(ctx->stack_size - SHORT_STACK) : 0;
+        const size_t per_wave = deep * TRACE_BLOCK * (sizeof(uint32_t) + sizeof(float));
+        const size_t room = NOALLOC_SCRATCH_BYTES - NOALLOC_COUNTER_BYTES;
+        int waves = ctx->trace_grid;
+        if(per_wave > 0 && (size_t)waves * per_wave > room)
+            waves = (int)(room / per_wave);
+        if(waves < 1)
+            throw InvalidArg("scene tree too deep for the no-allocation scratch: " + std::to_string(ctx->stack_size) +
+                             " stack entries need " + std::to_string(per_wave) + " bytes per wavefront beyond the " +
+                             std::to_string(room) + " the scratch holds");
+        uint8_t *base = static_cast<uint8_t *>(ctx->noalloc_scratch);
+        uint32_t *fetch = reinterpret_cast<uint32_t *>(base);
+        const size_t lanes = (size_t)waves * TRACE_BLOCK;
+        uint32_t *ovf_code = reinterpret_cast<uint32_t *>(base + NOALLOC_COUNTER_BYTES);
+        float *ovf_t = reinterpret_cast<float *>(base + NOALLOC_COUNTER_BYTES + deep * lanes * sizeof(uint32_t));
+        launch_fill_u32(ctx->stream, fetch, 1, 0u);
+        BTRC_HIP_CHECK(hipGetLastError());
+        launch_trace_batch(ctx->stream, waves, ctx->trace_sched(false), ctx->trace_scene(), tri_test, false,
+                           reinterpret_cast<const float4 *>(rays), (int32_t)n, reinterpret_cast<uint32_t *>(hits),
+                           fetch, TraceOverflow{ ovf_code, ovf_t, (int32_t)lanes });
+        BTRC_HIP_CHECK(hipGetLastError());
+        BTRC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
     });
 }
 

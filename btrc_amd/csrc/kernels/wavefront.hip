@@ -934,6 +934,13 @@ __global__ void k_halton_eval(HaltonConsts hc, const uint32_t *px, const uint32_
         out[(size_t)i * dims + k] = halton_get1d(c_primes.p, hs);
 }
 
+__global__ void k_fill_u32(uint32_t *p, uint32_t n, uint32_t value)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if(i < n)
+        p[i] = value;
+}
+
 __global__ void k_detmath_eval(int32_t fn, const float *x, const float *y, int32_t n, float *out)
 {
     const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1132,6 +1139,12 @@ void launch_independent_eval(hipStream_t s, const uint32_t *px, const uint32_t *
 void launch_detmath_eval(hipStream_t s, int32_t fn, const float *x, const float *y, int32_t n, float *out)
 {
     hipLaunchKernelGGL(k_detmath_eval, dim3((n + 255) / 256), dim3(256), 0, s, fn, x, y, n, out);
+}
+
+void launch_fill_u32(hipStream_t s, uint32_t *p, uint32_t n, uint32_t value)
+{
+    if(n)
+        hipLaunchKernelGGL(k_fill_u32, dim3((n + 255) / 256), dim3(256), 0, s, p, n, value);
 }
 
 } // namespace btrc

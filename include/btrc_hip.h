@@ -438,6 +438,17 @@ int btrc_hip_nrc_fit(btrc_ctx *ctx, const float *vertices, const float *targets,
 /* ---- traversal on ray batches in host memory ---- */
 int btrc_hip_trace_closest(btrc_ctx *ctx, const btrc_ray *rays, int64_t n, int32_t tri_test, btrc_hit *hits);
 int btrc_hip_trace_any(btrc_ctx *ctx, const btrc_ray *rays, int64_t n, int32_t tri_test, uint8_t *occluded);
+/* No counterpart in the reference. For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): btrc_hip_trace_closest on rays and hits that already
+ * live in device memory (`rays` and `hits` are device or managed
+ * allocations holding n entries each). It allocates nothing: the
+ * traversal's ray counter and its per-lane stack entries beyond the LDS part
+ * live in a static 8 MiB device array per GPU, which caps the persistent
+ * grid and fails with BTRC_ERR_INVALID when the scene's tree needs more
+ * stack than the array holds. Calls that overlap on one GPU race, since they
+ * share that array. Returns once the hits are written. */
+int btrc_hip_trace_closest_noalloc(btrc_ctx *ctx, const btrc_ray *rays, int64_t n, int32_t tri_test, btrc_hit *hits);
 
 /* ---- device-resident ray batch (throughput measurement) ---- */
 int btrc_hip_batch_upload(btrc_ctx *ctx, const btrc_ray *rays, int64_t n);
