@@ -318,9 +318,8 @@ def main():
             cpu["isects_per_ray"] = float(np.mean(stats[1]))
 
     roofline = None
-    if media:
-        # the medium stage dominates here; its roofline is not modelled yet
-        pass
+    if media and staged["stage_ms"][4] > 0:
+        roofline = medium_roofline(scene, cam, p, staged)
     elif bytes_per_ray is not None and trace_ms > 0:
         launches = max(1, int(staged["launches"][1]))
         rays_per_launch = staged["ext"] / launches
@@ -745,6 +744,42 @@ def instanced_flat_comparison(ctx, scene, p, views, stage):
                trace_ms_two_level=round(float(stage[1]), 3), trace_ms_flat_gpu_built=round(float(stf.stage_ms[1]), 3),
                shadow_ms_two_level=round(float(stage[3]), 3), shadow_ms_flat_gpu_built=round(float(stf.stage_ms[3]), 3))
     return bvh
+
+
+MEDIUM_BYTES_PER_STATE = 204.0   # a path state and its hit read by the medium stage (shade-like)
+MEDIUM_BYTES_PER_LOOKUP = 32.0   # one trilinear density fetch: eight FP32 voxels
+
+
+def medium_roofline(scene, cam, p, staged, bands=8, rows=2):
+    """Work model of the medium stage (k_medium): 204 B per path state it
+    processes (one per extension ray) plus 32 B per density lookup of delta
+    tracking. Lookups per path come from the oracle's medium stage on the
+    frame's own paths (`bands` bands of `rows` image rows spread over the
+    frame, sample 0); the stage's time is the stage-timed frame's."""
+    from oracle import oracle
+    os_ = oracle.OracleScene(scene)
+    oracle.tracking_stats()
+    W, H = p.width, p.height
+    paths = 0
+    for k in range(bands):
+        r0 = (2 * k + 1) * H // (2 * bands)
+        os_.render(cam, p, first=r0 * W, count=rows * W, threads=host_threads())
+        paths += rows * W
+    segments, lookups = oracle.tracking_stats()
+    lookups_per_path = lookups / paths
+    frame_paths = W * H * p.spp
+    launches = max(1, int(staged["launches"][4]))
+    total_bytes = MEDIUM_BYTES_PER_STATE * staged["ext"] + MEDIUM_BYTES_PER_LOOKUP * lookups_per_path * frame_paths
+    stage_s = staged["stage_ms"][4] * 1e-3
+    achieved = total_bytes / stage_s / 1e9
+    return dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None, kernel="k_medium",
+                algorithmic_bytes_per_launch=round(total_bytes / launches),
+                launch_ms=round(staged["stage_ms"][4] / launches, 4),
+                bytes_per_state=MEDIUM_BYTES_PER_STATE, bytes_per_lookup=MEDIUM_BYTES_PER_LOOKUP,
+                lookups_per_path=round(lookups_per_path, 3),
+                tracking_segments_per_path=round(segments / paths, 3),
+                sample=f"oracle medium stage on {paths} paths of the frame ({bands} bands of {rows} rows, sample 0)")
 
 
 def media_cpu_baseline(scene, cam, p, budget_s):

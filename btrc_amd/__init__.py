@@ -138,6 +138,34 @@ class Context:
             self.h, rgba.ctypes.data, rgb64.ctypes.data if fp64 else None))
         return rgba, rgb64
 
+    def set_reporter(self, reporter):
+        """progress callbacks of later renders (Reporter, core/reporter.h):
+        `reporter` has new_stage(name), complete_stage() and
+        progress(percentage) methods (any may be missing); None removes it.
+        The library calls them from the rendering thread during a render."""
+        if reporter is None:
+            self._reporter = None
+            self._check(self.lib.btrc_hip_set_reporter(self.h, None))
+            return
+
+        def method(name, cb_type, wrap):
+            f = getattr(reporter, name, None)
+            return cb_type(wrap(f)) if f is not None else cb_type()
+        r = _abi.Reporter()
+        r.user = None
+        r.new_stage = method("new_stage", _abi.REPORTER_STAGE, lambda f: lambda _u, name: f(name.decode()))
+        r.complete_stage = method("complete_stage", _abi.REPORTER_DONE, lambda f: lambda _u: f())
+        r.progress = method("progress", _abi.REPORTER_PROGRESS, lambda f: lambda _u, pct: f(pct))
+        self._reporter = r   # the callbacks must outlive the library's copy
+        self._check(self.lib.btrc_hip_set_reporter(self.h, C.byref(r)))
+
+    def tonemapped(self, width, height, exposure=1.0):
+        """the resolved colour film through the ACES tone map, on the device
+        (ACESToneMap, postprocess/tonemap.cpp)"""
+        rgba = np.empty((height, width, 4), dtype=np.float32)
+        self._check(self.lib.btrc_hip_tonemap_readback(self.h, float(exposure), rgba.ctypes.data))
+        return rgba
+
     def aov(self, width, height, albedo=True, normal=True):
         """first-hit albedo and shading-normal images (RGBA), or None"""
         a = np.empty((height, width, 4), dtype=np.float32) if albedo else None
@@ -428,6 +456,11 @@ class WavefrontPathTracer:
     def set_film(self, width, height):
         self.width, self.height = int(width), int(height)
         self._committed = False
+
+    def set_reporter(self, reporter):
+        """Renderer::set_reporter (core/renderer.h): an object with
+        new_stage / complete_stage / progress methods, or None"""
+        self.ctx.set_reporter(reporter)
 
     def commit(self):
         if self.scene is None or self.camera is None:

@@ -52,7 +52,18 @@ class MaterialDesc(C.Structure):
                 ("sheen", C.c_float), ("sheen_tint", C.c_float), ("clearcoat", C.c_float),
                 ("clearcoat_gloss", C.c_float), ("transmission", C.c_float),
                 ("transmission_roughness", C.c_float), ("ior", C.c_float),
-                ("color_texture", C.c_int32), ("metallic_texture", C.c_int32), ("roughness_texture", C.c_int32)]
+                ("color_texture", C.c_int32), ("metallic_texture", C.c_int32), ("roughness_texture", C.c_int32),
+                ("specular_texture", C.c_int32), ("specular_tint_texture", C.c_int32),
+                ("anisotropic_texture", C.c_int32), ("sheen_texture", C.c_int32), ("sheen_tint_texture", C.c_int32),
+                ("clearcoat_texture", C.c_int32), ("clearcoat_gloss_texture", C.c_int32),
+                ("transmission_texture", C.c_int32), ("transmission_roughness_texture", C.c_int32),
+                ("ior_texture", C.c_int32), ("normal_texture", C.c_int32)]
+
+
+# texture slots of btrc_material_desc, in declaration order (Disney parameters, then the normal map)
+MATERIAL_TEXTURE_SLOTS = ("color", "metallic", "roughness", "specular", "specular_tint", "anisotropic", "sheen",
+                          "sheen_tint", "clearcoat", "clearcoat_gloss", "transmission", "transmission_roughness",
+                          "ior", "normal")
 
 
 class TextureDesc(C.Structure):
@@ -172,6 +183,16 @@ def hit_dtype():
     return np.dtype([("t", "<f4"), ("instance", "<u4"), ("prim", "<u4"), ("u", "<f4"), ("v", "<f4")])
 
 
+REPORTER_STAGE = C.CFUNCTYPE(None, C.c_void_p, C.c_char_p)
+REPORTER_DONE = C.CFUNCTYPE(None, C.c_void_p)
+REPORTER_PROGRESS = C.CFUNCTYPE(None, C.c_void_p, C.c_float)
+
+
+class Reporter(C.Structure):
+    _fields_ = [("user", C.c_void_p), ("new_stage", REPORTER_STAGE), ("complete_stage", REPORTER_DONE),
+                ("progress", REPORTER_PROGRESS)]
+
+
 # every symbol include/btrc_hip.h declares, in declaration order
 EXPORTS = [
     "btrc_hip_create", "btrc_hip_destroy", "btrc_hip_last_error", "btrc_hip_device_name",
@@ -180,7 +201,8 @@ EXPORTS = [
     "btrc_hip_update_geometry",
     "btrc_hip_set_filter",
     "btrc_hip_render", "btrc_hip_render_views", "btrc_hip_film_readback", "btrc_hip_view_readback",
-    "btrc_hip_aov_readback", "btrc_hip_request_stop",
+    "btrc_hip_aov_readback", "btrc_hip_tonemap_readback", "btrc_hip_request_stop",
+    "btrc_hip_set_reporter",
     "btrc_hip_nrc_configure", "btrc_hip_nrc_reset", "btrc_hip_nrc_stats", "btrc_hip_nrc_get_weights",
     "btrc_hip_nrc_set_weights", "btrc_hip_nrc_eval", "btrc_hip_nrc_gradients", "btrc_hip_nrc_fit",
     "btrc_hip_trace_closest", "btrc_hip_trace_any", "btrc_hip_trace_closest_noalloc",
@@ -214,8 +236,10 @@ def declare(lib):
         "btrc_hip_render_views": (C.c_int, [ctx, P(RenderParams), C.c_int32, P(ViewDesc), P(RenderStats)]),
         "btrc_hip_view_readback": (C.c_int, [ctx, C.c_int32, C.c_void_p]),
         "btrc_hip_aov_readback": (C.c_int, [ctx, C.c_void_p, C.c_void_p]),
+        "btrc_hip_tonemap_readback": (C.c_int, [ctx, C.c_float, C.c_void_p]),
         "btrc_hip_film_readback": (C.c_int, [ctx, C.c_void_p, C.c_void_p]),
         "btrc_hip_request_stop": (C.c_int, [ctx]),
+        "btrc_hip_set_reporter": (C.c_int, [ctx, C.POINTER(Reporter)]),
         "btrc_hip_nrc_configure": (C.c_int, [ctx, P(NrcParams)]),
         "btrc_hip_nrc_reset": (C.c_int, [ctx, C.c_uint64]),
         "btrc_hip_nrc_stats": (C.c_int, [ctx, P(NrcStats)]),

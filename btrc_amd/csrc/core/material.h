@@ -43,8 +43,14 @@ struct MaterialRec
     float transmission;
     float transmission_roughness;
     float ior;
-    // textures in place of color / metallic / roughness, -1: the constant
-    int32_t color_tex, metallic_tex, roughness_tex;
+    // textures in place of the parameters above, -1: the constant
+    int32_t color_tex, metallic_tex, roughness_tex, specular_tex, specular_tint_tex, anisotropic_tex;
+    int32_t sheen_tex, sheen_tint_tex, clearcoat_tex, clearcoat_gloss_tex, transmission_tex;
+    int32_t transmission_roughness_tex, ior_tex;
+    // normal map texture (-1: none) and, once sampled at a surface point by
+    // material_at, its RGB
+    int32_t normal_tex;
+    float normal_rgb[3];
 };
 
 struct BSDFSample
@@ -686,6 +692,12 @@ BTRC_HD Shader make_shader(const MaterialRec &m, const Frame &geometry_frame, V3
     Shader s;
     s.frame.geometry = geometry_frame;
     s.frame.shading = rotate_to_new_z(geometry_frame, interp_z);
+    if(m.normal_tex >= 0 && m.type != MATERIAL_INVISIBLE)
+    {
+        // NormalMap::adjust_frame, material/utils/normal_map.cpp:13-21
+        const V3 local = v3(2.0f * m.normal_rgb[0] - 1.0f, 2.0f * m.normal_rgb[1] - 1.0f, 2.0f * m.normal_rgb[2] - 1.0f);
+        s.frame.shading = rotate_to_new_z(s.frame.shading, normalize(local_to_global(s.frame.shading, local)));
+    }
     s.type = m.type;
     s.shadow_terminator = m.shadow_terminator != 0;
     s.albedo = m.type == MATERIAL_INVISIBLE ? v3s(1.0f) : v3(m.color[0], m.color[1], m.color[2]);

@@ -15,7 +15,9 @@
 // Input encoding (64 values) of a vertex record (NrcVertex):
 //   [0, 48)  sin(2^f pi x_a), cos(2^f pi x_a) for f = 0..7 and the three
 //            axes of the position normalised to the scene box, ordered
-//            (f, axis, sin/cos)
+//            (f, axis, sin/cos); f = 0 evaluated directly, each higher
+//            frequency from the one below by the double-angle identities
+//            sin 2t = 2 sin t cos t, cos 2t = (cos t - sin t)(cos t + sin t)
 //   [48, 51) direction toward the previous vertex (unit)
 //   [51, 54) shading normal
 //   [54]     roughness
@@ -89,16 +91,24 @@ BTRC_HD void nrc_encode(const NrcVertex &v, const NrcBox &box, float *x)
     for(int a = 0; a < 3; ++a)
         q[a] = (v.p[a] - box.lo[a]) * box.inv_extent[a];
     int k = 0;
-    float scale = PI;
+    float s[3], c[3];
+    for(int a = 0; a < 3; ++a)
+    {
+        const float t = PI * q[a];
+        s[a] = btrc_detmath::sin(t);
+        c[a] = btrc_detmath::cos(t);
+    }
     for(int f = 0; f < NRC_FREQS; ++f)
     {
         for(int a = 0; a < 3; ++a)
         {
-            const float t = scale * q[a];
-            x[k++] = btrc_detmath::sin(t);
-            x[k++] = btrc_detmath::cos(t);
+            x[k++] = s[a];
+            x[k++] = c[a];
+            const float s2 = 2.0f * s[a] * c[a];
+            const float c2 = (c[a] - s[a]) * (c[a] + s[a]);
+            s[a] = s2;
+            c[a] = c2;
         }
-        scale = scale * 2.0f;
     }
     for(int a = 0; a < 3; ++a)
         x[k++] = v.wo[a];

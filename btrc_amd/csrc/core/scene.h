@@ -141,7 +141,7 @@ BTRC_HD SurfacePoint get_hitinfo(
 // The material at a surface point: textured parameters sampled at its
 // texture coordinate (DisneyMaterial::create_shader, disney.cpp:813-842:
 // Texture2D::sample_spectrum / sample_float; sample_float reads the first
-// channel).
+// channel), and the normal map's RGB (normal_map.cpp:13-21).
 BTRC_HD MaterialRec material_at(const SceneView &sc, int32_t id, const SurfacePoint &sp)
 {
     MaterialRec m = sc.materials[id];
@@ -152,10 +152,29 @@ BTRC_HD MaterialRec material_at(const SceneView &sc, int32_t id, const SurfacePo
         m.color[1] = c.y;
         m.color[2] = c.z;
     }
-    if(m.metallic_tex >= 0)
-        m.metallic = tex_sample(sc.tex, m.metallic_tex, sp.tu, sp.tv).x;
-    if(m.roughness_tex >= 0)
-        m.roughness = tex_sample(sc.tex, m.roughness_tex, sp.tu, sp.tv).x;
+    auto scalar = [&](int32_t tex, float &value) {
+        if(tex >= 0)
+            value = tex_sample(sc.tex, tex, sp.tu, sp.tv).x;
+    };
+    scalar(m.metallic_tex, m.metallic);
+    scalar(m.roughness_tex, m.roughness);
+    scalar(m.specular_tex, m.specular);
+    scalar(m.specular_tint_tex, m.specular_tint);
+    scalar(m.anisotropic_tex, m.anisotropic);
+    scalar(m.sheen_tex, m.sheen);
+    scalar(m.sheen_tint_tex, m.sheen_tint);
+    scalar(m.clearcoat_tex, m.clearcoat);
+    scalar(m.clearcoat_gloss_tex, m.clearcoat_gloss);
+    scalar(m.transmission_tex, m.transmission);
+    scalar(m.transmission_roughness_tex, m.transmission_roughness);
+    scalar(m.ior_tex, m.ior);
+    if(m.normal_tex >= 0)
+    {
+        const float4 c = tex_sample(sc.tex, m.normal_tex, sp.tu, sp.tv);
+        m.normal_rgb[0] = c.x;
+        m.normal_rgb[1] = c.y;
+        m.normal_rgb[2] = c.z;
+    }
     return m;
 }
 

@@ -232,6 +232,8 @@ struct btrc_ctx
     // this device's copy of `scratch` (btrc_hip_trace_closest_noalloc)
     void *noalloc_scratch = nullptr;
     std::string noalloc_error;
+    // progress callbacks of renders (Reporter, core/reporter.h:7-36)
+    btrc_reporter reporter{ nullptr, nullptr, nullptr, nullptr };
 
     // scene
     bool has_scene = false;
@@ -319,7 +321,8 @@ struct btrc_ctx
         DevBuf<NrcQuery> queries;
         DevBuf<NrcRecord> rec;
         DevBuf<float> acc;
-        DevBuf<int32_t> counts;         // [0] queries, [1] valid records of a batch
+        DevBuf<int32_t> counts;         // [0] queries, [1] valid records of a batch, [2] valid records of the frame
+        DevBuf<int32_t> index;          // the frame's valid record slots, compacted
         DevBuf<uint16_t> xt, dzt;
         DevBuf<double> loss_partial;
         DevBuf<int32_t> count_partial;
@@ -1049,6 +1052,17 @@ void upload_scene(btrc_ctx *ctx, const btrc_scene_desc *sd)
         r.color_tex = texture_ref(m.color_texture, i);
         r.metallic_tex = texture_ref(m.metallic_texture, i);
         r.roughness_tex = texture_ref(m.roughness_texture, i);
+        r.specular_tex = texture_ref(m.specular_texture, i);
+        r.specular_tint_tex = texture_ref(m.specular_tint_texture, i);
+        r.anisotropic_tex = texture_ref(m.anisotropic_texture, i);
+        r.sheen_tex = texture_ref(m.sheen_texture, i);
+        r.sheen_tint_tex = texture_ref(m.sheen_tint_texture, i);
+        r.clearcoat_tex = texture_ref(m.clearcoat_texture, i);
+        r.clearcoat_gloss_tex = texture_ref(m.clearcoat_gloss_texture, i);
+        r.transmission_tex = texture_ref(m.transmission_texture, i);
+        r.transmission_roughness_tex = texture_ref(m.transmission_roughness_texture, i);
+        r.ior_tex = texture_ref(m.ior_texture, i);
+        r.normal_tex = texture_ref(m.normal_texture, i);
         mats[i] = r;
     }
 
@@ -1522,7 +1536,8 @@ void nrc_alloc(btrc_ctx *ctx)
     n.wtb.alloc(NRC_PARAMS);
     n.rec.alloc(NRC_RECORDS);
     n.acc.alloc(4 * NRC_RECORDS);
-    n.counts.alloc(2);
+    n.counts.alloc(3);
+    n.index.alloc(NRC_RECORDS);
     n.xt.alloc((size_t)NRC_LAYERS * NRC_IN * NRC_RECORDS);
     n.dzt.alloc((size_t)NRC_LAYERS * NRC_IN * NRC_RECORDS);
     n.loss_partial.alloc(NRC_RECORDS / 32);
@@ -1601,8 +1616,11 @@ void nrc_train(btrc_ctx *ctx, int32_t steps, int64_t batch, bool rotate)
     const int64_t batches = NRC_RECORDS / batch;
     for(int32_t k = 0; k < steps; ++k)
     {
+        // a rendered frame's steps take equal parts of its valid records
+        // (compacted after the bounce loop); fits take the loaded batch
         NrcBatch b{ n.rec.ptr, rotate ? (k % batches) * batch : 0, batch, n.acc.ptr, n.counts.ptr + 1,
-                    n.params.loss_scale, rotate ? 1 : 0 };
+                    n.params.loss_scale, rotate ? 1 : 0, rotate ? n.index.ptr : nullptr,
+                    rotate ? n.counts.ptr + 2 : nullptr, k, steps };
         launch_nrc_train_step(s, nrc_weights(ctx), nrc_box(ctx), b, nrc_scratch(ctx, batch), nrc_adam(ctx),
                               n.grad.ptr, n.step_stats.ptr + k);
         BTRC_HIP_CHECK(hipGetLastError());
@@ -1766,6 +1784,29 @@ void render(btrc_ctx *ctx, const btrc_render_params *p, int32_t view_count, cons
     ctx->film_spp = keep ? ctx->film_spp + p->spp : p->spp;
     ctx->film_has64 = p->film_fp64 != 0;
 
+    // Reporter calls of WavefrontPathTracer::render (wavefront.cpp:221,287,314):
+    // a new stage now, progress at every counter read-back, the stage
+    // completed when the loop ends (also when it stops early)
+    struct ReporterStage
+    {
+        const btrc_reporter &r;
+        explicit ReporterStage(const btrc_reporter &rep) : r(rep)
+        {
+            if(r.new_stage)
+                r.new_stage(r.user, "");
+        }
+        ~ReporterStage()
+        {
+            if(r.complete_stage)
+                r.complete_stage(r.user);
+        }
+        void progress(const WaveCounters &c) const
+        {
+            if(r.progress && c.total > 0)
+                r.progress(r.user, (float)(100.0 * (double)(c.generated - (uint64_t)c.active) / (double)c.total));
+        }
+    } reporter_stage(ctx->reporter);
+
     WaveCounters init{};
     init.total = total;
     *ctx->host_counters = init;
@@ -1788,7 +1829,9 @@ void render(btrc_ctx *ctx, const btrc_render_params *p, int32_t view_count, cons
     if(nrc_on)
     {
         auto &n = ctx->nrc;
-        const int32_t slots = p->max_depth + 1;
+        // record slots per training path: its vertices at depths 0..max_depth,
+        // or 0..train_depth - 1 when the path ends at train_depth with a query
+        const int32_t slots = n.params.train_depth > 0 ? n.params.train_depth : p->max_depth + 1;
         rc.nrc_stride = std::max(1, n.params.train ? n.params.train_stride : 0);
         rc.nrc_phase = p->sample_offset % rc.nrc_stride;
         rc.nrc_max_paths = n.params.train ? (int32_t)(NRC_RECORDS / slots) : 0;
@@ -1906,6 +1949,7 @@ void render(btrc_ctx *ctx, const btrc_render_params *p, int32_t view_count, cons
             BTRC_HIP_CHECK(hipMemcpyAsync(ctx->host_counters, ctx->counters.ptr, sizeof(WaveCounters), hipMemcpyDeviceToHost, s));
             BTRC_HIP_CHECK(hipStreamSynchronize(s));
             const WaveCounters &hc = *ctx->host_counters;
+            reporter_stage.progress(hc);
             if(hc.generated == hc.total && hc.active == 0)
                 break;
         }
@@ -1951,6 +1995,7 @@ void render(btrc_ctx *ctx, const btrc_render_params *p, int32_t view_count, cons
             BTRC_HIP_CHECK(hipMemcpyAsync(ctx->host_counters, ctx->counters.ptr, sizeof(WaveCounters), hipMemcpyDeviceToHost, s));
             BTRC_HIP_CHECK(hipStreamSynchronize(s));
             const WaveCounters &hc = *ctx->host_counters;
+            reporter_stage.progress(hc);
             if(hc.generated == hc.total && hc.active == 0)
                 break;
         }
@@ -1969,7 +2014,10 @@ void render(btrc_ctx *ctx, const btrc_render_params *p, int32_t view_count, cons
         if(n.params.train)
         {
             nrc_steps = std::min(n.params.steps, 64);
-            timer.run(STAGE_TRAIN, [&] { nrc_train(ctx, nrc_steps, NRC_BATCH, true); });
+            timer.run(STAGE_TRAIN, [&] {
+                launch_nrc_compact(s, n.rec.ptr, n.index.ptr, n.counts.ptr + 2);
+                nrc_train(ctx, nrc_steps, NRC_BATCH, true);
+            });
         }
         BTRC_HIP_CHECK(hipGetLastError());
     }
@@ -2018,6 +2066,27 @@ void film_readback(btrc_ctx *ctx, float *rgba, double *rgb64)
         for(size_t i = 0; i < 3 * (size_t)pixels; ++i)
             rgb64[i] *= inv;
     }
+}
+
+// ACESToneMap::process (postprocess/tonemap.cpp:86-97) on the resolved colour
+// film, then the RGBA readback
+void tonemap_readback(btrc_ctx *ctx, float exposure, float *rgba)
+{
+    if(ctx->film_pixels <= 0)
+        throw StateError("no film: render first");
+    if(!rgba)
+        throw InvalidArg("rgba is null");
+    if(!std::isfinite(exposure))
+        throw InvalidArg("exposure is not finite");
+    const int32_t pixels = (int32_t)ctx->film_pixels;
+    hipStream_t s = ctx->stream;
+    ctx->resolved.alloc(pixels);
+    launch_resolve(s, RESOLVE_COLOR, ctx->film.ptr, pixels, 1.0f / (float)ctx->film_spp, ctx->resolved.ptr);
+    BTRC_HIP_CHECK(hipGetLastError());
+    launch_tonemap_aces(s, ctx->resolved.ptr, pixels, exposure);
+    BTRC_HIP_CHECK(hipGetLastError());
+    BTRC_HIP_CHECK(hipMemcpyAsync(rgba, ctx->resolved.ptr, pixels * sizeof(float4), hipMemcpyDeviceToHost, s));
+    BTRC_HIP_CHECK(hipStreamSynchronize(s));
 }
 
 // preview.cpp generate_albedo / generate_normal
@@ -2339,6 +2408,11 @@ int btrc_hip_film_readback(btrc_ctx *ctx, float *rgba, double *rgb64)
     return guarded(ctx, [&] { film_readback(ctx, rgba, rgb64); });
 }
 
+int btrc_hip_tonemap_readback(btrc_ctx *ctx, float exposure, float *rgba)
+{
+    return guarded(ctx, [&] { tonemap_readback(ctx, exposure, rgba); });
+}
+
 int btrc_hip_aov_readback(btrc_ctx *ctx, float *albedo_rgba, float *normal_rgba)
 {
     return guarded(ctx, [&] { aov_readback(ctx, albedo_rgba, normal_rgba); });
@@ -2460,7 +2534,8 @@ int btrc_hip_nrc_gradients(btrc_ctx *ctx, const float *vertices, const float *ta
         nrc_load_batch(ctx, vertices, targets, n);
         auto &nc = ctx->nrc;
         const int64_t batch = ((int64_t)n + 63) / 64 * 64;
-        NrcBatch b{ nc.rec.ptr, 0, batch, nc.acc.ptr, nc.counts.ptr + 1, nc.params.loss_scale, 0 };
+        NrcBatch b{ nc.rec.ptr, 0, batch, nc.acc.ptr, nc.counts.ptr + 1, nc.params.loss_scale, 0, nullptr, nullptr,
+                    0, 1 };
         launch_nrc_gradients(ctx->stream, nrc_weights(ctx), nrc_box(ctx), b, nrc_scratch(ctx, batch), nc.grad.ptr);
         BTRC_HIP_CHECK(hipGetLastError());
         std::vector<double> partial(batch / 32);
@@ -2501,6 +2576,13 @@ int btrc_hip_nrc_fit(btrc_ctx *ctx, const float *vertices, const float *targets,
             for(int32_t k = 0; k < steps; ++k)
                 loss[k] = st[k].loss;
         }
+    });
+}
+
+int btrc_hip_set_reporter(btrc_ctx *ctx, const btrc_reporter *reporter)
+{
+    return guarded(ctx, [&] {
+        ctx->reporter = reporter ? *reporter : btrc_reporter{ nullptr, nullptr, nullptr, nullptr };
     });
 }
 

@@ -4,15 +4,24 @@
 // All matrix products run on MFMA (v_mfma_f32_32x32x16_bf16): BF16 operands,
 // FP32 accumulation. One wavefront owns 32 samples; a layer is
 // Y^T = W X^T with M = output units, N = the 32 samples, K = input units, so
-// the 16 results a lane holds belong to one sample (lane % 32) and land in
-// LDS as that sample's next input row. Weights sit in LDS as [out][in] (A
-// operand) and, for the backward pass, as [in][out].
+// the 16 results a lane holds belong to one sample (lane % 32). Weights sit
+// in LDS as [out][in] (A operand) and, for the backward pass, as [in][out],
+// rows padded to 72 halves.
 //
 // Lane layout of the 32x32x16 MFMA (checked on the device before use):
 //   A (32 x 16): lane l holds row l % 32, k = 8 (l / 32) + 0..7
 //   B (16 x 32): lane l holds column l % 32, k = 8 (l / 32) + 0..7
 //   D (32 x 32): element i of lane l is row 8 (i / 4) + 4 (l / 32) + i % 4,
 //                column l % 32
+//
+// Activations and gradients stay in registers from layer to layer. The A rows
+// of a 32-row output tile read weight rows in the order swap23(r) (bits 2
+// and 3 of r exchanged); D element i of lane l then holds unit
+// 16 (i / 8) + 8 (l / 32) + i % 8 of the tile, which is exactly the next
+// layer's B operand for k slice 2 tile + i / 8. The weights keep their
+// natural layouts; only the order in which rows are read changes. The
+// backward pass reads rows of the [in][out] copy the same way, so each
+// input gradient is the next product's B operand too.
 #include <hip/hip_runtime.h>
 
 #include "../core/nrc.h"
@@ -47,76 +56,122 @@ __device__ __forceinline__ f32x16 mfma(u16x8 a, u16x8 b, f32x16 c)
 
 __device__ __forceinline__ u16x8 lds8(const uint16_t *p) { return *reinterpret_cast<const u16x8 *>(p); }
 
-// weights of all layers into LDS as [out][in] and, optionally, [in][out]
-__device__ void load_weights(const uint16_t *w, const uint16_t *wt, uint16_t *s_w, uint16_t *s_wt)
+// ---- register-resident forward pass (inference)
+
+constexpr int WS = NRC_IN + 8;                  // LDS row stride of the weights, halves (no bank conflicts)
+constexpr int NRC_ROWS = NRC_PARAMS / NRC_IN;   // 352 weight rows
+
+typedef short s16x2 __attribute__((ext_vector_type(2)));
+
+// two FP32 values to packed BF16, round to nearest even (v_cvt_pk_bf16_f32)
+__device__ __forceinline__ uint32_t pk_bf16(float lo, float hi)
 {
-    for(int i = threadIdx.x; i < NRC_PARAMS / 8; i += blockDim.x)
+    uint32_t r;
+    asm("v_cvt_pk_bf16_f32 %0, %1, %2" : "=v"(r) : "v"(lo), "v"(hi));
+    return r;
+}
+
+// ReLU of two packed BF16 values: negative values (sign bit set) are negative
+// as int16, so a signed max with 0 clears them; ReLU and rounding commute
+__device__ __forceinline__ uint32_t relu2(uint32_t x)
+{
+    return __builtin_bit_cast(uint32_t, __builtin_elementwise_max(__builtin_bit_cast(s16x2, x), (s16x2)0));
+}
+
+__device__ __forceinline__ u16x8 words8(uint32_t a, uint32_t b, uint32_t c, uint32_t d)
+{
+    typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+    const u32x4 w = { a, b, c, d };
+    return __builtin_bit_cast(u16x8, w);
+}
+
+__device__ __forceinline__ int swap23(int r) { return (r & ~12) | ((r & 4) << 1) | ((r & 8) >> 1); }
+
+// the B operand of a 64-wide layer input: k slices 0..3, this lane's 8 values each
+struct Frag
+{
+    u16x8 s[4];
+};
+
+// weights of all layers into LDS, rows padded to WS halves
+__device__ void load_weights_padded(const uint16_t *w, uint16_t *s_w)
+{
+    for(int i = threadIdx.x; i < NRC_ROWS * (NRC_IN / 8); i += blockDim.x)
     {
-        reinterpret_cast<uint4 *>(s_w)[i] = reinterpret_cast<const uint4 *>(w)[i];
-        if(s_wt)
-            reinterpret_cast<uint4 *>(s_wt)[i] = reinterpret_cast<const uint4 *>(wt)[i];
+        const int r = i / (NRC_IN / 8), c = i % (NRC_IN / 8);
+        *reinterpret_cast<uint4 *>(s_w + r * WS + 8 * c) = reinterpret_cast<const uint4 *>(w)[i];
     }
 }
 
-// the encoded inputs of sample `col` of this wavefront into x (row-major
-// [sample][64]); lanes 32..63 only wait
-__device__ void encode_tile(const NrcVertex *v, bool valid, const NrcBox &box, uint16_t *x, int col, int half)
+// the encoded inputs of this lane's sample as the first layer's B operand:
+// lane half h holds inputs 16 s + 8 h + 0..7 of slice s
+__device__ __forceinline__ void encode_frag(const NrcVertex &v, bool valid, const NrcBox &box, int h, Frag &x)
 {
-    if(half == 0)
-    {
-        float in[NRC_IN];
-        if(valid)
-            nrc_encode(*v, box, in);
-        else
-            for(int k = 0; k < NRC_IN; ++k)
-                in[k] = 0.0f;
+    float in[NRC_IN];
+    if(valid)
+        nrc_encode(v, box, in);
+    else
+#pragma unroll
         for(int k = 0; k < NRC_IN; ++k)
-            x[col * NRC_IN + k] = to_bf16(in[k]);
+            in[k] = 0.0f;
+#pragma unroll
+    for(int s = 0; s < 4; ++s)
+    {
+        uint32_t w[4];
+#pragma unroll
+        for(int j = 0; j < 4; ++j)
+        {
+            const int k = 16 * s + 2 * j;
+            w[j] = pk_bf16(h ? in[k + 8] : in[k], h ? in[k + 9] : in[k + 1]);
+        }
+        x.s[s] = words8(w[0], w[1], w[2], w[3]);
     }
 }
 
-// One layer forward for the wavefront's 32 samples: out = W_l x (ReLU on
-// hidden layers, BF16 into `next`); the output layer returns y for lanes
-// 0..31 in y[0..2]. With xt, hidden outputs also go to global memory as
-// [unit][sample] (the next layer's input, for the weight gradients).
-__device__ void layer_forward(int l, const uint16_t *s_w, const uint16_t *x, uint16_t *next, float *y,
-                              uint16_t *xt, int64_t stride, int64_t sample0)
+// 8 accumulator values from o on, ReLU, as one B fragment
+__device__ __forceinline__ u16x8 relu_frag(const f32x16 &a, int o)
 {
-    const int lane = threadIdx.x % WAVE, col = lane % 32, half = lane / 32;
-    const uint16_t *w = s_w + nrc_layer_offset(l);
-    const int tiles = nrc_layer_rows(l) / 32;
-    for(int mt = 0; mt < tiles; ++mt)
+    return words8(relu2(pk_bf16(a[o + 0], a[o + 1])), relu2(pk_bf16(a[o + 2], a[o + 3])),
+                  relu2(pk_bf16(a[o + 4], a[o + 5])), relu2(pk_bf16(a[o + 6], a[o + 7])));
+}
+
+// hidden layer l (64 x 64, ReLU) for the wavefront's 32 samples, in registers
+__device__ __forceinline__ void hidden_layer(const uint16_t *s_w, int l, int col, int h, Frag &x)
+{
+    const uint16_t *r0 = s_w + (l * NRC_WIDTH + swap23(col)) * WS + 8 * h;
+    const uint16_t *r1 = r0 + 32 * WS;
+    f32x16 acc0 = {}, acc1 = {};
+#pragma unroll
+    for(int s = 0; s < 4; ++s)
     {
-        f32x16 acc = {};
-#pragma unroll
-        for(int s = 0; s < NRC_IN / 16; ++s)
-        {
-            const u16x8 a = lds8(w + (mt * 32 + col) * NRC_IN + 16 * s + 8 * half);
-            const u16x8 b = lds8(x + col * NRC_IN + 16 * s + 8 * half);
-            acc = mfma(a, b, acc);
-        }
-        if(l == NRC_LAYERS - 1)
-        {
-            if(half == 0)
-            {
-                y[0] = acc[0];
-                y[1] = acc[1];
-                y[2] = acc[2];
-            }
-        }
-        else
-        {
-#pragma unroll
-            for(int i = 0; i < 16; ++i)
-            {
-                const int row = mt * 32 + 8 * (i / 4) + 4 * half + (i % 4);
-                const uint16_t h = to_bf16(fmaxf(acc[i], 0.0f));
-                next[col * NRC_IN + row] = h;
-                if(xt)
-                    xt[(int64_t)row * stride + sample0 + col] = h;
-            }
-        }
+        acc0 = mfma(lds8(r0 + 16 * s), x.s[s], acc0);
+        acc1 = mfma(lds8(r1 + 16 * s), x.s[s], acc1);
     }
+    x.s[0] = relu_frag(acc0, 0);
+    x.s[1] = relu_frag(acc0, 8);
+    x.s[2] = relu_frag(acc1, 0);
+    x.s[3] = relu_frag(acc1, 8);
+}
+
+// the whole network for this lane's sample; y[0..2] valid on lanes 0..31
+__device__ __forceinline__ void forward_regs(const uint16_t *s_w, const NrcVertex &v, bool valid, const NrcBox &box,
+                                             float *y)
+{
+    const int lane = threadIdx.x % WAVE, col = lane % 32, h = lane / 32;
+    Frag x;
+    encode_frag(v, valid, box, h, x);
+#pragma unroll
+    for(int l = 0; l < NRC_LAYERS - 1; ++l)
+        hidden_layer(s_w, l, col, h, x);
+    // output layer: natural row order, rows 3..31 are zero
+    const uint16_t *r = s_w + ((NRC_LAYERS - 1) * NRC_WIDTH + col) * WS + 8 * h;
+    f32x16 acc = {};
+#pragma unroll
+    for(int s = 0; s < 4; ++s)
+        acc = mfma(lds8(r + 16 * s), x.s[s], acc);
+    y[0] = acc[0];
+    y[1] = acc[1];
+    y[2] = acc[2];
 }
 
 } // namespace
@@ -125,13 +180,14 @@ __device__ void layer_forward(int l, const uint16_t *s_w, const uint16_t *x, uin
 
 // queries [0, n): y = max(net(x), 0); the film gets beta * y at the pixel,
 // and a training path's radiance sum too when the query is its tail (one
-// query per training path, after every other addition to its sum)
+// query per training path, after every other addition to its sum).
+// Persistent: each wavefront takes tiles of 32 queries; activations stay in
+// registers, the weights in LDS.
 __global__ void __launch_bounds__(256) k_nrc_infer(NrcWeights net, NrcBox box, const NrcQuery *q, const int32_t *count,
                                                    float *film, double *film64, float *acc)
 {
-    __shared__ __attribute__((aligned(16))) uint16_t s_w[NRC_PARAMS];
-    __shared__ __attribute__((aligned(16))) uint16_t s_x[4][2][TILE * NRC_IN];
-    load_weights(net.w_bf16, nullptr, s_w, nullptr);
+    __shared__ __attribute__((aligned(16))) uint16_t s_w[NRC_ROWS * WS];
+    load_weights_padded(net.w_bf16, s_w);
     __syncthreads();
     const int n = *count;
     const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE, col = lane % 32, half = lane / 32;
@@ -140,72 +196,56 @@ __global__ void __launch_bounds__(256) k_nrc_infer(NrcWeights net, NrcBox box, c
     {
         const int i = base + col;
         const bool valid = i < n;
-        encode_tile(valid ? &q[i].v : nullptr, valid, box, s_x[wave][0], col, half);
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        float y[3] = { 0.0f, 0.0f, 0.0f };
-        for(int l = 0; l < NRC_LAYERS; ++l)
-        {
-            layer_forward(l, s_w, s_x[wave][l & 1], s_x[wave][(l + 1) & 1], y, nullptr, 0, 0);
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        }
+        NrcVertex v{};
+        if(valid)
+            v = q[i].v;
+        float y[3];
+        forward_regs(s_w, v, valid, box, y);
         if(half == 0 && valid)
         {
             const NrcQuery &qi = q[i];
-            const float v[3] = { qi.beta[0] * fmaxf(y[0], 0.0f), qi.beta[1] * fmaxf(y[1], 0.0f),
+            const float r[3] = { qi.beta[0] * fmaxf(y[0], 0.0f), qi.beta[1] * fmaxf(y[1], 0.0f),
                                  qi.beta[2] * fmaxf(y[2], 0.0f) };
-            if(isfinite(v[0]) && isfinite(v[1]) && isfinite(v[2]))
+            if(isfinite(r[0]) && isfinite(r[1]) && isfinite(r[2]))
             {
                 float *p = film + 4 * (size_t)qi.pixel;
-                atomicAdd(p + 0, v[0]);
-                atomicAdd(p + 1, v[1]);
-                atomicAdd(p + 2, v[2]);
+                atomicAdd(p + 0, r[0]);
+                atomicAdd(p + 1, r[1]);
+                atomicAdd(p + 2, r[2]);
                 if(film64)
                 {
                     double *d = film64 + 3 * (size_t)qi.pixel;
-                    atomicAdd(d + 0, (double)v[0]);
-                    atomicAdd(d + 1, (double)v[1]);
-                    atomicAdd(d + 2, (double)v[2]);
+                    atomicAdd(d + 0, (double)r[0]);
+                    atomicAdd(d + 1, (double)r[1]);
+                    atomicAdd(d + 2, (double)r[2]);
                 }
                 if(qi.train != NO_TRAIN)
                 {
                     float *a = acc + 4 * (size_t)qi.train;
-                    a[0] = a[0] + v[0];
-                    a[1] = a[1] + v[1];
-                    a[2] = a[2] + v[2];
+                    a[0] = a[0] + r[0];
+                    a[1] = a[1] + r[1];
+                    a[2] = a[2] + r[2];
                 }
             }
         }
     }
 }
 
-// raw network outputs for test hooks: out[3 i + c]
+// raw network outputs for test hooks: out[3 i + c], the inference kernel's arithmetic
 __global__ void __launch_bounds__(256) k_nrc_eval(NrcWeights net, NrcBox box, const NrcVertex *v, int32_t n,
                                                   float *out)
 {
-    __shared__ __attribute__((aligned(16))) uint16_t s_w[NRC_PARAMS];
-    __shared__ __attribute__((aligned(16))) uint16_t s_x[4][2][TILE * NRC_IN];
-    load_weights(net.w_bf16, nullptr, s_w, nullptr);
+    __shared__ __attribute__((aligned(16))) uint16_t s_w[NRC_ROWS * WS];
+    load_weights_padded(net.w_bf16, s_w);
     __syncthreads();
     const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE, col = lane % 32, half = lane / 32;
-    const int base = (blockIdx.x * 4 + wave) * TILE;
-    const int i = base + col;
+    const int i = (blockIdx.x * 4 + wave) * TILE + col;
     const bool valid = i < n;
-    encode_tile(valid ? &v[i] : nullptr, valid, box, s_x[wave][0], col, half);
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    float y[3] = { 0.0f, 0.0f, 0.0f };
-    for(int l = 0; l < NRC_LAYERS; ++l)
-    {
-        layer_forward(l, s_w, s_x[wave][l & 1], s_x[wave][(l + 1) & 1], y, nullptr, 0, 0);
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    }
+    NrcVertex vi{};
+    if(valid)
+        vi = v[i];
+    float y[3];
+    forward_regs(s_w, vi, valid, box, y);
     if(half == 0 && valid)
     {
         out[3 * (size_t)i + 0] = y[0];
@@ -222,29 +262,56 @@ __global__ void __launch_bounds__(256) k_nrc_eval(NrcWeights net, NrcBox box, co
 // (dzt), both BF16, for k_nrc_wgrad; and one FP64 loss partial per
 // wavefront. The output gradient is d loss / d y times loss_scale, where the
 // loss is the mean relative L2 over the valid samples' three channels.
+// Activations and gradients stay in registers (the swap23 row order makes
+// each product's result the next product's B operand, forward and
+// backward); the weights sit in LDS as [out][in] and [in][out].
 __global__ void __launch_bounds__(128) k_nrc_train_fb(NrcWeights net, NrcBox box, NrcBatch b, NrcScratch sc)
 {
-    __shared__ __attribute__((aligned(16))) uint16_t s_w[NRC_PARAMS];
-    __shared__ __attribute__((aligned(16))) uint16_t s_wt[NRC_PARAMS];
-    __shared__ __attribute__((aligned(16))) uint16_t s_x[2][NRC_LAYERS][TILE * NRC_IN];
-    __shared__ __attribute__((aligned(16))) uint16_t s_dz[2][2][TILE * NRC_IN];
-    load_weights(net.w_bf16, net.wt_bf16, s_w, s_wt);
+    __shared__ __attribute__((aligned(16))) uint16_t s_w[NRC_ROWS * WS];
+    __shared__ __attribute__((aligned(16))) uint16_t s_wt[NRC_LAYERS * NRC_IN * WS];
+    load_weights_padded(net.w_bf16, s_w);
+    // [in][out] copies, per layer 64 rows of `rows` values, padded to WS
+    for(int l = 0; l < NRC_LAYERS; ++l)
+    {
+        const int rows = nrc_layer_rows(l), chunks = rows / 8;
+        const uint16_t *src = net.wt_bf16 + nrc_layer_offset(l);
+        for(int i = threadIdx.x; i < NRC_IN * chunks; i += blockDim.x)
+        {
+            const int r = i / chunks, c = i % chunks;
+            *reinterpret_cast<uint4 *>(s_wt + (l * NRC_IN + r) * WS + 8 * c) =
+                reinterpret_cast<const uint4 *>(src + r * rows)[c];
+        }
+    }
     __syncthreads();
 
-    const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE, col = lane % 32, half = lane / 32;
+    const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE, col = lane % 32, h = lane / 32;
     const int64_t S = b.batch;
+    int64_t first, count;
+    nrc_batch_range(b, &first, &count);
     const int64_t sample0 = (int64_t)(blockIdx.x * 2 + wave) * TILE;
+    if(sample0 >= count)
+    {
+        // beyond the step's samples (k_nrc_wgrad reads none of them)
+        if(lane == 0)
+        {
+            sc.loss_partial[blockIdx.x * 2 + wave] = 0.0;
+            sc.count_partial[blockIdx.x * 2 + wave] = 0;
+        }
+        return;
+    }
     const int64_t i = sample0 + col;
     bool valid = false;
     float target[3] = { 0.0f, 0.0f, 0.0f };
-    const NrcVertex *vtx = nullptr;
-    if(i < S)
+    NrcVertex vtx{};
+    if(i < count)
     {
-        const NrcRecord &r = b.rec[b.permute ? nrc_record_slot(b.first + i) : b.first + i];
+        const int64_t slot = b.index ? (int64_t)b.index[first + i]
+                                     : (b.permute ? nrc_record_slot(first + i) : first + i);
+        const NrcRecord &r = b.rec[slot];
         valid = r.valid != 0;
         if(valid)
         {
-            vtx = &r.v;
+            vtx = r.v;
             // radiance leaving the vertex: what the path gathered after it,
             // divided by the throughput the vertex was reached with
             const float *acc = b.acc + 4 * (size_t)r.path;
@@ -255,82 +322,107 @@ __global__ void __launch_bounds__(128) k_nrc_train_fb(NrcWeights net, NrcBox box
         }
     }
 
-    uint16_t(*x)[TILE * NRC_IN] = s_x[wave];
-    encode_tile(vtx, valid, box, x[0], col, half);
-    __syncthreads();
-    // the encoded inputs also go to global memory for the first layer's gradient
-    if(half == 0)
-        for(int k = 0; k < NRC_IN; ++k)
-            sc.xt[(int64_t)k * S + i] = x[0][col * NRC_IN + k];
-
-    // forward
-    float y[3] = { 0.0f, 0.0f, 0.0f };
-    for(int l = 0; l < NRC_LAYERS; ++l)
-    {
-        uint16_t *xt_next = l + 1 < NRC_LAYERS ? sc.xt + (int64_t)(l + 1) * NRC_IN * S : nullptr;
-        layer_forward(l, s_w, x[l], l + 1 < NRC_LAYERS ? x[l + 1] : nullptr, y, xt_next, S, sample0);
-        __syncthreads();
-    }
-
-    // loss and output gradient: the sum over the batch, times loss_scale (the
-    // 1 / (3 n) of the mean is applied with the update, once n is known)
-    double loss = 0.0;
-    uint16_t *dz = s_dz[wave][0];
-    if(half == 0)
-    {
-        for(int r = 0; r < NRC_OUT_ROWS; ++r)
+    // lane (col, h) holds units 16 s + 8 h + 0..7 of sample col in slice s
+    auto store_units = [&](uint16_t *dst, const Frag &f, int slices) {
+#pragma unroll
+        for(int s = 0; s < 4; ++s)
         {
-            float g = 0.0f;
-            if(r < NRC_OUT && valid)
-            {
-                g = nrc_rel_l2_grad(y[r], target[r]) * b.loss_scale;
-                loss += (double)nrc_rel_l2(y[r], target[r]);
-            }
-            dz[col * NRC_IN + r] = to_bf16(g);
+            if(s >= slices)
+                break;
+#pragma unroll
+            for(int j = 0; j < 8; ++j)
+                dst[(int64_t)(16 * s + 8 * h + j) * S + i] = f.s[s][j];
         }
+    };
+
+    // forward: X[l] is the input of layer l
+    Frag X[NRC_LAYERS];
+    encode_frag(vtx, valid, box, h, X[0]);
+    store_units(sc.xt, X[0], 4);
+#pragma unroll
+    for(int l = 0; l < NRC_LAYERS - 1; ++l)
+    {
+        X[l + 1] = X[l];
+        hidden_layer(s_w, l, col, h, X[l + 1]);
+        store_units(sc.xt + (int64_t)(l + 1) * NRC_IN * S, X[l + 1], 4);
     }
+    f32x16 out = {};
+    {
+        const uint16_t *r = s_w + ((NRC_LAYERS - 1) * NRC_WIDTH + col) * WS + 8 * h;
+#pragma unroll
+        for(int s = 0; s < 4; ++s)
+            out = mfma(lds8(r + 16 * s), X[NRC_LAYERS - 1].s[s], out);
+    }
+
+    // loss and output gradient (lanes of half 0 hold rows 0..2 of their
+    // sample): the sum over the batch, times loss_scale (the 1 / (3 n) of the
+    // mean is applied with the update, once n is known)
+    double loss = 0.0;
+    float g[3] = { 0.0f, 0.0f, 0.0f };
+    if(h == 0 && valid)
+        for(int r = 0; r < NRC_OUT; ++r)
+        {
+            g[r] = nrc_rel_l2_grad(out[r], target[r]) * b.loss_scale;
+            loss += (double)nrc_rel_l2(out[r], target[r]);
+        }
     for(int off = 32; off > 0; off >>= 1)
         loss += __shfl_xor(loss, off);
-    const int32_t nvalid = (int32_t)__popcll(__ballot(valid && half == 0));
+    const int32_t nvalid = (int32_t)__popcll(__ballot(valid && h == 0));
     if(lane == 0)
     {
         sc.loss_partial[blockIdx.x * 2 + wave] = loss;
         sc.count_partial[blockIdx.x * 2 + wave] = nvalid;
     }
-    __syncthreads();
-
-    // backward: dz holds d/d(out) of layer l as [sample][unit]
-    for(int l = NRC_LAYERS - 1; l >= 0; --l)
+    // dz of the output layer as a B operand: rows 0..2 in slice 0, half 0
+    Frag dz;
+    dz.s[0] = words8(h == 0 ? pk_bf16(g[0], g[1]) : 0u, h == 0 ? pk_bf16(g[2], 0.0f) : 0u, 0u, 0u);
+    dz.s[1] = dz.s[2] = dz.s[3] = words8(0u, 0u, 0u, 0u);
     {
-        const int rows = nrc_layer_rows(l);
-        uint16_t *dzt = sc.dzt + (int64_t)l * NRC_IN * S;
-        if(half == 0)
-            for(int r = 0; r < rows; ++r)
-                dzt[(int64_t)r * S + i] = dz[col * NRC_IN + r];
-        if(l == 0)
-            break;
-        // d/d(input of l) = W_l^T dz, masked by the ReLU of layer l - 1
-        uint16_t *dz_next = s_dz[wave][(NRC_LAYERS - l) & 1];
-        const uint16_t *wt = s_wt + nrc_layer_offset(l); // [in][rows]
-        for(int mt = 0; mt < NRC_IN / 32; ++mt)
-        {
-            f32x16 acc = {};
-            for(int s = 0; s < rows / 16; ++s)
-            {
-                const u16x8 a = lds8(wt + (mt * 32 + col) * rows + 16 * s + 8 * half);
-                const u16x8 bb = lds8(dz + col * NRC_IN + 16 * s + 8 * half);
-                acc = mfma(a, bb, acc);
-            }
+        // the output layer's 32 gradient rows: 0..2 from half 0, zeros beyond
+        uint16_t *d5 = sc.dzt + (int64_t)(NRC_LAYERS - 1) * NRC_IN * S;
 #pragma unroll
-            for(int e = 0; e < 16; ++e)
-            {
-                const int unit = mt * 32 + 8 * (e / 4) + 4 * half + (e % 4);
-                const bool on = (x[l][col * NRC_IN + unit] & 0x7fffu) != 0; // ReLU output > 0
-                dz_next[col * NRC_IN + unit] = on ? to_bf16(acc[e]) : (uint16_t)0;
-            }
+        for(int j = 0; j < 16; ++j)
+        {
+            const int r = 16 * h + j;
+            d5[(int64_t)r * S + i] = r < NRC_OUT ? dz.s[0][r] : (uint16_t)0;
         }
-        __syncthreads();
-        dz = dz_next;
+    }
+
+    // backward: dX_l = W_l^T dz_l, then dz_(l-1) = dX_l where X_l > 0
+#pragma unroll
+    for(int l = NRC_LAYERS - 1; l >= 1; --l)
+    {
+        // the output layer's gradient is zero beyond row 2: one k slice
+        const int slices = l == NRC_LAYERS - 1 ? 1 : 4;
+        const uint16_t *r0 = s_wt + (l * NRC_IN + swap23(col)) * WS + 8 * h;
+        const uint16_t *r1 = r0 + 32 * WS;
+        f32x16 acc0 = {}, acc1 = {};
+#pragma unroll
+        for(int s = 0; s < 4; ++s)
+        {
+            if(s >= slices)
+                break;
+            acc0 = mfma(lds8(r0 + 16 * s), dz.s[s], acc0);
+            acc1 = mfma(lds8(r1 + 16 * s), dz.s[s], acc1);
+        }
+#pragma unroll
+        for(int q = 0; q < 4; ++q)
+        {
+            const f32x16 &a = q < 2 ? acc0 : acc1;
+            const int o = 8 * (q & 1);
+            const u16x8 x = X[l].s[q];
+            u16x8 d;
+#pragma unroll
+            for(int j = 0; j < 8; j += 2)
+            {
+                const uint32_t v = pk_bf16(a[o + j], a[o + j + 1]);
+                const uint32_t m = (x[j] ? 0xffffu : 0u) | (x[j + 1] ? 0xffff0000u : 0u);
+                d[j] = (uint16_t)(v & m);
+                d[j + 1] = (uint16_t)((v & m) >> 16);
+            }
+            dz.s[q] = d;
+        }
+        store_units(sc.dzt + (int64_t)(l - 1) * NRC_IN * S, dz, 4);
     }
 }
 
@@ -339,8 +431,14 @@ __global__ void __launch_bounds__(128) k_nrc_train_fb(NrcWeights net, NrcBox box
 // layers, tiles 20..21 the output layer's 32 x 64; slice kc of NRC_KSPLIT
 // writes its partial tile to partial[kc]. k_nrc_reduce adds the slices in
 // a fixed order (deterministic).
-__global__ void __launch_bounds__(64) k_nrc_wgrad(NrcScratch sc, int64_t S, float *partial)
+__global__ void __launch_bounds__(64) k_nrc_wgrad(NrcScratch sc, NrcBatch b, float *partial)
 {
+    // operand rows have stride b.batch; the step's samples, rounded up to a
+    // tile of 32 (the tail samples hold zeros), are the K extent
+    const int64_t stride = b.batch;
+    int64_t first, count;
+    nrc_batch_range(b, &first, &count);
+    const int64_t S = (count + 31) / 32 * 32;
     const int t = blockIdx.x % 22, kc = blockIdx.x / 22;
     const int l = t < 20 ? t / 4 : 5;
     const int mt = t < 20 ? (t % 4) / 2 : 0;
@@ -348,10 +446,10 @@ __global__ void __launch_bounds__(64) k_nrc_wgrad(NrcScratch sc, int64_t S, floa
     const int lane = threadIdx.x, col = lane % 32, half = lane / 32;
     const int64_t chunk = (S + NRC_KSPLIT * 16 - 1) / (NRC_KSPLIT * 16) * 16;
     const int64_t k0 = kc * chunk, k1 = k0 + chunk < S ? k0 + chunk : S;
-    const uint16_t *dzt = sc.dzt + (int64_t)l * NRC_IN * S;
-    const uint16_t *xt = sc.xt + (int64_t)l * NRC_IN * S;
-    const uint16_t *arow = dzt + (int64_t)(mt * 32 + col) * S + 8 * half;
-    const uint16_t *brow = xt + (int64_t)(nt * 32 + col) * S + 8 * half;
+    const uint16_t *dzt = sc.dzt + (int64_t)l * NRC_IN * stride;
+    const uint16_t *xt = sc.xt + (int64_t)l * NRC_IN * stride;
+    const uint16_t *arow = dzt + (int64_t)(mt * 32 + col) * stride + 8 * half;
+    const uint16_t *brow = xt + (int64_t)(nt * 32 + col) * stride + 8 * half;
     f32x16 acc0 = {}, acc1 = {};
     int64_t k = k0;
     for(; k + 32 <= k1; k += 32)
@@ -477,6 +575,37 @@ __global__ void __launch_bounds__(256) k_nrc_update(NrcWeights net, NrcAdam opt,
     net.wt_bf16[off + c * rows + r] = h;
 }
 
+// valid record slots in the order of the slot bijection, one workgroup of
+// 1024 threads, 64 consecutive bijection indices each: count, scan, write
+__global__ void __launch_bounds__(1024) k_nrc_compact(const NrcRecord *rec, int32_t *index, int32_t *total)
+{
+    constexpr int PER = (1 << 16) / 1024;
+    __shared__ int32_t s_sum[1024];
+    const int t = threadIdx.x;
+    int32_t n = 0;
+    for(int j = 0; j < PER; ++j)
+        n += rec[nrc_record_slot(t * PER + j)].valid != 0 ? 1 : 0;
+    s_sum[t] = n;
+    __syncthreads();
+    // inclusive scan (Hillis-Steele)
+    for(int off = 1; off < 1024; off <<= 1)
+    {
+        const int32_t v = t >= off ? s_sum[t - off] : 0;
+        __syncthreads();
+        s_sum[t] += v;
+        __syncthreads();
+    }
+    int32_t o = s_sum[t] - n;
+    for(int j = 0; j < PER; ++j)
+    {
+        const int32_t slot = (int32_t)nrc_record_slot(t * PER + j);
+        if(rec[slot].valid != 0)
+            index[o++] = slot;
+    }
+    if(t == 1023)
+        *total = s_sum[1023];
+}
+
 // clears the record flags and the training paths' radiance sums of a frame
 __global__ void k_nrc_clear(NrcRecord *rec, int64_t nrec, float *acc, int64_t npaths)
 {
@@ -531,13 +660,18 @@ void launch_nrc_gradients(hipStream_t s, const NrcWeights &net, const NrcBox &bo
 {
     const int blocks = (int)(b.batch / (2 * TILE));
     hipLaunchKernelGGL(k_nrc_train_fb, dim3(blocks), dim3(128), 0, s, net, box, b, sc);
-    hipLaunchKernelGGL(k_nrc_wgrad, dim3(22 * NRC_KSPLIT), dim3(64), 0, s, sc, b.batch, sc.grad_partial);
+    hipLaunchKernelGGL(k_nrc_wgrad, dim3(22 * NRC_KSPLIT), dim3(64), 0, s, sc, b, sc.grad_partial);
     hipLaunchKernelGGL(k_nrc_reduce, dim3(NRC_REDUCE_BLOCKS), dim3(256), 0, s, sc.grad_partial, grad, sc.g2_partial);
 }
 
 void launch_nrc_clear(hipStream_t s, NrcRecord *rec, int64_t nrec, float *acc, int64_t npaths)
 {
     hipLaunchKernelGGL(k_nrc_clear, dim3(256), dim3(256), 0, s, rec, nrec, acc, npaths);
+}
+
+void launch_nrc_compact(hipStream_t s, const NrcRecord *rec, int32_t *index, int32_t *total)
+{
+    hipLaunchKernelGGL(k_nrc_compact, dim3(1), dim3(1024), 0, s, rec, index, total);
 }
 
 void launch_nrc_publish(hipStream_t s, const NrcWeights &net)

@@ -44,7 +44,7 @@ struct NrcRecord
 struct NrcView
 {
     int32_t query_depth;     // vertex depth answered by the cache (-1: no queries)
-    int32_t slots_per_path;  // record slots per training path (max depth + 1)
+    int32_t slots_per_path;  // record slots per training path (max depth + 1, or the training depth)
     int32_t train_depth;     // training paths end here with a query (-1: traced to the end)
     NrcQuery *queries;
     int32_t *query_count;
@@ -52,16 +52,43 @@ struct NrcView
     float *acc;              // radiance sum per training path (4 floats)
 };
 
+// A training step's samples. Direct mode (index null): samples
+// [first, first + batch), sample i reading record first + i, or
+// nrc_record_slot(first + i) with permute. Compacted mode (index set): the
+// frame's valid records, index[0, *total), split into `steps` equal parts
+// (multiples of 32, at most `batch`); this step takes part `step`. Either way `batch` is the
+// row stride of the step's operand buffers (a multiple of 64, at least the
+// step's sample count).
 struct NrcBatch
 {
     const NrcRecord *rec;
     int64_t first;
-    int64_t batch;           // a multiple of 64
+    int64_t batch;
     const float *acc;
     int32_t *valid_count;    // set by a training step
     float loss_scale;
-    int32_t permute;         // 1: batch index i reads record nrc_record_slot(first + i)
+    int32_t permute;
+    const int32_t *index;    // compacted mode: record slots of the valid records
+    const int32_t *total;    // compacted mode: how many
+    int32_t step, steps;
 };
+
+// first sample and sample count of a step
+__host__ __device__ inline void nrc_batch_range(const NrcBatch &b, int64_t *first, int64_t *count)
+{
+    if(!b.index)
+    {
+        *first = b.first;
+        *count = b.batch;
+        return;
+    }
+    const int64_t total = *b.total;
+    int64_t per = ((total + b.steps - 1) / b.steps + 31) / 32 * 32;
+    per = per < b.batch ? per : b.batch; // at most one stride of samples per step
+    const int64_t f = (int64_t)b.step * per;
+    *first = f;
+    *count = total - f < 0 ? 0 : (total - f < per ? total - f : per);
+}
 
 // A frame's records sit at (training path, depth); training batches read
 // them through this bijection of [0, 2^16) so that every batch mixes paths
@@ -110,6 +137,9 @@ void launch_nrc_train_step(hipStream_t s, const NrcWeights &net, const NrcBox &b
 void launch_nrc_gradients(hipStream_t s, const NrcWeights &net, const NrcBox &box, const NrcBatch &b,
                           const NrcScratch &sc, float *grad);
 void launch_nrc_clear(hipStream_t s, NrcRecord *rec, int64_t nrec, float *acc, int64_t npaths);
+// index = the valid record slots among rec[0, 2^16) in the order of
+// nrc_record_slot(0, 1, ...), *total = how many (one workgroup, deterministic)
+void launch_nrc_compact(hipStream_t s, const NrcRecord *rec, int32_t *index, int32_t *total);
 void launch_nrc_publish(hipStream_t s, const NrcWeights &net);
 
 } // namespace btrc

@@ -202,6 +202,8 @@ void launch_medium_eval(hipStream_t s, const SceneView &sv, int32_t med, const f
 void launch_independent_eval(hipStream_t s, const uint32_t *px, const uint32_t *py, const int32_t *sample_index,
                              int32_t n, int32_t dims, float *out);
 void launch_detmath_eval(hipStream_t s, int32_t fn, const float *x, const float *y, int32_t n, float *out);
+// ACESToneMap (postprocess/tonemap.cpp:14-24,35-51) in place on resolved RGBA
+void launch_tonemap_aces(hipStream_t s, float4 *rgba, int32_t pixels, float exposure);
 // p[0, n) = value, by a kernel (a memset node misbehaves in captured graphs)
 void launch_fill_u32(hipStream_t s, uint32_t *p, uint32_t n, uint32_t value);
 

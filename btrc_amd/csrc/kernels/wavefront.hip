@@ -934,6 +934,26 @@ __global__ void k_halton_eval(HaltonConsts hc, const uint32_t *px, const uint32_
         out[(size_t)i * dims + k] = halton_get1d(c_primes.p, hs);
 }
 
+// ACESToneMap's kernel, postprocess/tonemap.cpp:14-24,35-51: Narkowicz's
+// fit of the exposure-scaled colour, saturated (NaN to 0); alpha unchanged
+__device__ __forceinline__ float aces_fit(float x)
+{
+    const float y = (x * (2.51f * x + 0.03f)) / (x * (2.43f * x + 0.59f) + 0.14f);
+    return !(y > 0.0f) ? 0.0f : (y > 1.0f ? 1.0f : y);
+}
+
+__global__ void k_tonemap_aces(float4 *rgba, int32_t pixels, float exposure)
+{
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if(i >= pixels)
+        return;
+    float4 c = rgba[i];
+    c.x = aces_fit(c.x * exposure);
+    c.y = aces_fit(c.y * exposure);
+    c.z = aces_fit(c.z * exposure);
+    rgba[i] = c;
+}
+
 __global__ void k_fill_u32(uint32_t *p, uint32_t n, uint32_t value)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1139,6 +1159,12 @@ void launch_independent_eval(hipStream_t s, const uint32_t *px, const uint32_t *
 void launch_detmath_eval(hipStream_t s, int32_t fn, const float *x, const float *y, int32_t n, float *out)
 {
     hipLaunchKernelGGL(k_detmath_eval, dim3((n + 255) / 256), dim3(256), 0, s, fn, x, y, n, out);
+}
+
+void launch_tonemap_aces(hipStream_t s, float4 *rgba, int32_t pixels, float exposure)
+{
+    if(pixels > 0)
+        hipLaunchKernelGGL(k_tonemap_aces, dim3((pixels + 255) / 256), dim3(256), 0, s, rgba, pixels, exposure);
 }
 
 void launch_fill_u32(hipStream_t s, uint32_t *p, uint32_t n, uint32_t value)

@@ -452,26 +452,35 @@ class _Builder:
     def material(self, node):
         return self._cached("mat", node, self._make_material)
 
-    def _scalar(self, node, name, default, allow_texture=False):
+    def _scalar(self, node, name, default):
+        """(constant, None) or (default, texture index) of a Disney parameter"""
         if name not in node:
             return default, None
         kind, v = self.texture2d(node[name])
         if kind == "const":
             return v[0], None
-        if not allow_texture:
-            raise SceneFileError(f"disney: a texture for '{name}' is not supported by this renderer")
         return default, v
+
+    def _normal_map(self, node):
+        """NormalMap::load (material/utils/normal_map.cpp:5-11): the texture
+        under "normal", if any, as a texture index"""
+        if "normal" not in node:
+            return None
+        kind, v = self.texture2d(node["normal"])
+        if kind == "const":
+            # a constant normal map: a one-texel texture of that colour
+            v = self.s.add_texture(np.array([[[*v[:3], 1.0]]], np.float32), filter="point")
+        return v
 
     def _make_material(self, node):
         t = node.get("type")
         st = _bool(node.get("shadow_terminator_term", True))
-        if "normal_map" in node or "normal" in node and isinstance(node.get("normal"), dict):
-            raise SceneFileError("normal maps are not supported by this renderer")
         if t == "diffuse":
             kind, v = self.texture2d(node["color"])
+            nmap = self._normal_map(node)
             if kind == "const":
-                return self.s.add_diffuse(v, shadow_terminator=st)
-            return self.s.add_diffuse([1.0, 1.0, 1.0], shadow_terminator=st, color_texture=v)
+                return self.s.add_diffuse(v, shadow_terminator=st, normal_texture=nmap)
+            return self.s.add_diffuse([1.0, 1.0, 1.0], shadow_terminator=st, color_texture=v, normal_texture=nmap)
         if t == "black":
             return self.s.add_diffuse([0.0, 0.0, 0.0], shadow_terminator=st)
         if t == "invisible":
@@ -479,22 +488,22 @@ class _Builder:
         if t == "disney":
             # DisneyMaterialCreator, material/disney.cpp:844-894
             kind, col = self.texture2d(node["color"])
-            color, color_tex = (col, None) if kind == "const" else ([1.0, 1.0, 1.0], col)
-            metallic, metallic_tex = self._scalar(node, "metallic", 0.0, True)
-            roughness, roughness_tex = self._scalar(node, "roughness", 0.0, True)
+            textures = {}
+            color = col
+            if kind != "const":
+                color, textures["color"] = [1.0, 1.0, 1.0], col
             kw = {}
-            for name, d in (("transmission", 0.0), ("ior", 1.5), ("specular", 1.0), ("specular_tint", 0.0),
-                            ("anisotropic", 0.0), ("sheen", 0.0), ("sheen_tint", 0.0), ("clearcoat", 0.0),
-                            ("clearcoat_gloss", 1.0)):
-                kw[name] = self._scalar(node, name, d)[0]
-            tr = self._scalar(node, "transmission_roughness", None)[0]
-            if tr is None:
-                if roughness_tex is not None:
-                    raise SceneFileError("disney: a textured transmission_roughness is not supported")
-                tr = roughness
-            return self.s.add_disney(color, metallic, roughness, transmission_roughness=tr, shadow_terminator=st,
-                                     color_texture=color_tex, metallic_texture=metallic_tex,
-                                     roughness_texture=roughness_tex, **kw)
+            for name, d in (("metallic", 0.0), ("roughness", 0.0), ("transmission", 0.0), ("ior", 1.5),
+                            ("specular", 1.0), ("specular_tint", 0.0), ("anisotropic", 0.0), ("sheen", 0.0),
+                            ("sheen_tint", 0.0), ("clearcoat", 0.0), ("clearcoat_gloss", 1.0),
+                            ("transmission_roughness", None)):
+                kw[name], tex = self._scalar(node, name, d)
+                if tex is not None:
+                    textures[name] = tex
+            if "transmission_roughness" in textures and kw["transmission_roughness"] is None:
+                kw["transmission_roughness"] = 0.0
+            return self.s.add_disney(color, shadow_terminator=st, textures=textures,
+                                     normal_texture=self._normal_map(node), **kw)
         raise SceneFileError(f"material type '{t}' is not supported by this renderer")
 
     # --- media
@@ -627,16 +636,6 @@ def load(path):
 
 # ------------------------------------------------------------ post processing
 
-def aces(color, exposure=1.0):
-    """ACESToneMap (postprocess/tonemap.cpp:14-24,76-82): the Narkowicz fit
-    of exposure-scaled RGB, saturated"""
-    x = color[..., :3] * np.float32(exposure)
-    y = (x * (2.51 * x + 0.03)) / (x * (2.43 * x + 0.59) + 0.14)
-    out = color.copy()
-    out[..., :3] = np.clip(y, 0.0, 1.0)
-    return out
-
-
 def save_image(path, rgba):
     """Image::save (utils/image.cpp, image.h:495-540) by extension: PNG / JPG
     as bytes min(int(256 x), 255) (negative values as 0), HDR as RGBE, EXR
@@ -712,9 +711,14 @@ def render_file(path, ctx=None, log=print):
     log(f"render {sf.width}x{sf.height}, {sf.renderer['spp']} spp")
     result = r.render()
     color = result.color
+    tonemapped = False
     for pp in sf.post_processors:
         if pp["type"] == "aces_tonemap":
-            color = aces(color, pp["exposure"])
+            # on the device, from the film (ACESToneMap::process)
+            if tonemapped:
+                raise SceneFileError("only one aces_tonemap post processor is supported")
+            color = r.ctx.tonemapped(sf.width, sf.height, pp["exposure"])
+            tonemapped = True
         elif pp["type"] == "save_to_image":
             if "color" in pp:
                 save_image(pp["color"], np.power(np.maximum(color, 0.0), 1.0 / pp["gamma"]))

@@ -129,25 +129,43 @@ class Scene:
                                   filter=filters[filter], srgb=bool(srgb_to_linear), border=tuple(border)))
         return len(self.textures) - 1
 
-    def add_diffuse(self, color, shadow_terminator=True, color_texture=None):
+    def add_diffuse(self, color, shadow_terminator=True, color_texture=None, normal_texture=None):
+        """Diffuse (material/diffuse.cpp): albedo `color` or a texture index,
+        optional normal map texture"""
         self.materials.append(dict(type=_abi.MATERIAL_DIFFUSE, color=color,
-                                   shadow_terminator=shadow_terminator, color_texture=color_texture))
+                                   shadow_terminator=shadow_terminator, color_texture=color_texture,
+                                   normal_texture=normal_texture))
         return len(self.materials) - 1
 
     def add_disney(self, color, metallic, roughness, specular=1.0, specular_tint=0.0, anisotropic=0.0,
                    sheen=0.0, sheen_tint=0.0, clearcoat=0.0, clearcoat_gloss=1.0, transmission=0.0,
                    transmission_roughness=None, ior=1.5, shadow_terminator=True, color_texture=None,
-                   metallic_texture=None, roughness_texture=None):
-        # defaults of DisneyMaterialCreator, builtin/material/disney.cpp:844-894
+                   metallic_texture=None, roughness_texture=None, textures=None, normal_texture=None):
+        """DisneyMaterial with the creator's defaults (builtin/material/
+        disney.cpp:844-894). Any parameter may come from a texture: the
+        `<name>_texture` arguments or `textures` = {name: texture index};
+        transmission_roughness follows roughness (value and texture) unless
+        given. normal_texture: a normal map (normal_map.cpp)."""
+        tex = dict(textures or {})
+        for name, t in (("color", color_texture), ("metallic", metallic_texture), ("roughness", roughness_texture)):
+            if t is not None:
+                tex[name] = t
+        unknown = set(tex) - set(_abi.MATERIAL_TEXTURE_SLOTS[:-1])
+        if unknown:
+            raise ValueError(f"add_disney: no parameter named {sorted(unknown)}")
         if transmission_roughness is None:
             transmission_roughness = roughness
-        self.materials.append(dict(type=_abi.MATERIAL_DISNEY, color=color, metallic=metallic,
-                                   roughness=roughness, specular=specular, specular_tint=specular_tint,
-                                   anisotropic=anisotropic, sheen=sheen, sheen_tint=sheen_tint,
-                                   clearcoat=clearcoat, clearcoat_gloss=clearcoat_gloss,
-                                   transmission=transmission, transmission_roughness=transmission_roughness,
-                                   ior=ior, shadow_terminator=shadow_terminator, color_texture=color_texture,
-                                   metallic_texture=metallic_texture, roughness_texture=roughness_texture))
+            if "roughness" in tex and "transmission_roughness" not in tex:
+                tex["transmission_roughness"] = tex["roughness"]
+        m = dict(type=_abi.MATERIAL_DISNEY, color=color, metallic=metallic,
+                 roughness=roughness, specular=specular, specular_tint=specular_tint,
+                 anisotropic=anisotropic, sheen=sheen, sheen_tint=sheen_tint,
+                 clearcoat=clearcoat, clearcoat_gloss=clearcoat_gloss,
+                 transmission=transmission, transmission_roughness=transmission_roughness,
+                 ior=ior, shadow_terminator=shadow_terminator, normal_texture=normal_texture)
+        for name, t in tex.items():
+            m[name + "_texture"] = t
+        self.materials.append(m)
         return len(self.materials) - 1
 
     def add_invisible(self):
@@ -250,9 +268,9 @@ class Scene:
                         "sheen_tint", "clearcoat", "clearcoat_gloss", "transmission",
                         "transmission_roughness", "ior"):
                 setattr(d, key, float(m.get(key, 0.0)))
-            for key in ("color_texture", "metallic_texture", "roughness_texture"):
-                t = m.get(key)
-                setattr(d, key, 0 if t is None else int(t) + 1)
+            for slot in _abi.MATERIAL_TEXTURE_SLOTS:
+                t = m.get(slot + "_texture")
+                setattr(d, slot + "_texture", 0 if t is None else int(t) + 1)
         idesc = (_abi.InstanceDesc * max(1, len(self.instances)))()
         for k, inst in enumerate(self.instances):
             d = idesc[k]

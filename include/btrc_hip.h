@@ -127,12 +127,21 @@ typedef struct btrc_material_desc
     float transmission, transmission_roughness, ior;
     /* Array2D textures in place of the constants (DisneyMaterialCreator,
      * material/disney.cpp:844-894; DiffuseCreator's albedo): 1 + index into
-     * btrc_scene_desc::textures, 0 for the constant. Colour reads RGB,
-     * metallic and roughness read the first channel, at the hit's texture
-     * coordinate. */
+     * btrc_scene_desc::textures, 0 for the constant. Colour reads RGB, the
+     * other parameters read the first channel, at the hit's texture
+     * coordinate (DisneyMaterial::create_shader, disney.cpp:813-842). The
+     * reference's transmission_roughness defaults to the roughness texture
+     * itself; a caller mirrors that by passing roughness_texture here too. */
     int32_t color_texture;
     int32_t metallic_texture;
     int32_t roughness_texture;
+    int32_t specular_texture, specular_tint_texture, anisotropic_texture;
+    int32_t sheen_texture, sheen_tint_texture, clearcoat_texture, clearcoat_gloss_texture;
+    int32_t transmission_texture, transmission_roughness_texture, ior_texture;
+    /* NormalMap (material/utils/normal_map.cpp:13-21), Disney and diffuse:
+     * the texture's RGB as 2 rgb - 1 in the shading frame becomes the new
+     * shading normal; 1 + texture index, 0 for none */
+    int32_t normal_texture;
 } btrc_material_desc;
 
 /* An Array2D texture (texture2d/array2d.cpp, utils/cuda/texture.cpp:63-105,
@@ -410,8 +419,32 @@ int btrc_hip_view_readback(btrc_ctx *ctx, int32_t view, float *rgba);
  * RGBA per pixel, alpha 1; either pointer may be NULL; each needs the
  * matching flag in the last render's params */
 int btrc_hip_aov_readback(btrc_ctx *ctx, float *albedo_rgba, float *normal_rgba);
+/* The "aces" post processor (ACESToneMap::process, postprocess/tonemap.cpp:
+ * 14-24,35-51,86-97) on the device: the resolved colour film (as
+ * btrc_hip_film_readback returns it) through Narkowicz's ACES fit of
+ * exposure * value per channel, saturated to [0, 1]; alpha unchanged. */
+int btrc_hip_tonemap_readback(btrc_ctx *ctx, float exposure, float *rgba);
 /* thread-safe; the running render returns BTRC_ERR_STOPPED */
 int btrc_hip_request_stop(btrc_ctx *ctx);
+
+/* Reporter (core/reporter.h:7-36; set by Renderer::set_reporter,
+ * core/renderer.h): callbacks made from the thread that calls a render
+ * entry point, during that call. new_stage when a render starts,
+ * progress(percentage of the frame's paths finished) at every read-back of
+ * the device counters (once per block of captured bounces, as the
+ * reference reports once per bounce, wavefront.cpp:287), complete_stage
+ * when the bounce loop ends, also when it stops early. Any callback may be
+ * NULL. The reference's preview images (need_preview / new_preview) are a
+ * GUI feature and are not called. */
+typedef struct btrc_reporter
+{
+    void *user;
+    void (*new_stage)(void *user, const char *name);
+    void (*complete_stage)(void *user);
+    void (*progress)(void *user, float percentage);
+} btrc_reporter;
+/* copies *reporter; NULL removes it */
+int btrc_hip_set_reporter(btrc_ctx *ctx, const btrc_reporter *reporter);
 
 /* ---- neural radiance cache ---- */
 int btrc_hip_nrc_configure(btrc_ctx *ctx, const btrc_nrc_params *params);

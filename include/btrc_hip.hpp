@@ -75,6 +75,17 @@ struct RenderResult
     bool stopped = false;      // ended by stop_async
 };
 
+// Reporter (core/reporter.h:7-36): override what you need; calls come from
+// the rendering thread during WavefrontPathTracer::render
+class Reporter
+{
+public:
+    virtual ~Reporter() = default;
+    virtual void new_stage(const char *name) { (void)name; }
+    virtual void complete_stage() {}
+    virtual void progress(float percentage) { (void)percentage; }
+};
+
 class WavefrontPathTracer
 {
 public:
@@ -101,6 +112,28 @@ public:
     void set_film(int width, int height) { width_ = width; height_ = height; dirty_ = true; }
     void set_film_filter(const btrc_filter_desc &filter) { filter_ = filter; dirty_ = true; }
     const Params &params() const { return params_; }
+
+    // Renderer::set_reporter; the reporter must outlive the renders it sees
+    // (nullptr removes it)
+    void set_reporter(Reporter *reporter)
+    {
+        btrc_reporter r{ reporter, nullptr, nullptr, nullptr };
+        if(reporter)
+        {
+            r.new_stage = [](void *u, const char *name) { static_cast<Reporter *>(u)->new_stage(name); };
+            r.complete_stage = [](void *u) { static_cast<Reporter *>(u)->complete_stage(); };
+            r.progress = [](void *u, float pct) { static_cast<Reporter *>(u)->progress(pct); };
+        }
+        ctx_.check(btrc_hip_set_reporter(ctx_.get(), reporter ? &r : nullptr));
+    }
+
+    // the "aces" post processor on the last render (ACESToneMap::process)
+    std::vector<float> tonemapped(float exposure = 1.0f)
+    {
+        std::vector<float> rgba(static_cast<size_t>(width_) * static_cast<size_t>(height_) * 4);
+        ctx_.check(btrc_hip_tonemap_readback(ctx_.get(), exposure, rgba.data()));
+        return rgba;
+    }
 
     // Renderer::commit: the scene, the camera (its aspect from the film,
     // main.cpp:52) and the filter go to the device

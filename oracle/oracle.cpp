@@ -18,6 +18,7 @@
 //
 // Parity pinning of each part is listed in oracle/README.md.
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -1316,8 +1317,9 @@ struct SP
     float tu, tv; // texture coordinate
 };
 
-// the material at a surface point: textured colour / metallic / roughness
-// sampled at its texture coordinate (disney.cpp:813-842)
+// the material at a surface point: every textured parameter sampled at its
+// texture coordinate, colour as RGB, the others from the first channel
+// (disney.cpp:813-842)
 static btrc_material_desc material_at(const Scene &s, int id, const SP &sp)
 {
     btrc_material_desc m = s.mats[id];
@@ -1328,11 +1330,29 @@ static btrc_material_desc material_at(const Scene &s, int id, const SP &sp)
         m.color[1] = c.y;
         m.color[2] = c.z;
     }
-    if(m.metallic_texture > 0)
-        m.metallic = tex_sample(s, m.metallic_texture - 1, sp.tu, sp.tv).x;
-    if(m.roughness_texture > 0)
-        m.roughness = tex_sample(s, m.roughness_texture - 1, sp.tu, sp.tv).x;
+    const struct { int32_t tex; float *value; } scalars[] = {
+        { m.metallic_texture, &m.metallic }, { m.roughness_texture, &m.roughness },
+        { m.specular_texture, &m.specular }, { m.specular_tint_texture, &m.specular_tint },
+        { m.anisotropic_texture, &m.anisotropic }, { m.sheen_texture, &m.sheen },
+        { m.sheen_tint_texture, &m.sheen_tint }, { m.clearcoat_texture, &m.clearcoat },
+        { m.clearcoat_gloss_texture, &m.clearcoat_gloss }, { m.transmission_texture, &m.transmission },
+        { m.transmission_roughness_texture, &m.transmission_roughness }, { m.ior_texture, &m.ior },
+    };
+    for(const auto &p : scalars)
+        if(p.tex > 0)
+            *p.value = tex_sample(s, p.tex - 1, sp.tu, sp.tv).x;
     return m;
+}
+
+// NormalMap::adjust_frame (material/utils/normal_map.cpp:13-21): the map's
+// RGB as 2 rgb - 1 in the shading frame is the new shading z
+static Frm normal_mapped(const Scene &s, const btrc_material_desc &m, const SP &sp, const Frm &shading)
+{
+    if(m.normal_texture <= 0 || m.type == BTRC_MATERIAL_INVISIBLE)
+        return shading;
+    const F4 c = tex_sample(s, m.normal_texture - 1, sp.tu, sp.tv);
+    const Vec local = mk(2.0f * c.x - 1.0f, 2.0f * c.y - 1.0f, 2.0f * c.z - 1.0f);
+    return rot_z(shading, nrm(l2g(shading, local)));
 }
 
 // get_hitinfo, core/scene.cpp:9-64
@@ -1986,6 +2006,11 @@ static Vec med_uvw(const Med &m, Vec p) // TransformMedium, core/medium.cpp:98-1
 
 const int MAX_TRACKING = 1 << 20;
 
+// work statistics of the medium stage for the benchmark's work model:
+// delta-tracking segments and density lookups (each a trilinear fetch of
+// eight voxels)
+static std::atomic<uint64_t> g_track_segments{ 0 }, g_track_lookups{ 0 };
+
 // Medium::sample along a -> b: HomogeneousMedium (homogeneous.cpp:25-58),
 // delta tracking of HetergeneousMedium (hetergeneous.cpp:25-72)
 template<class S>
@@ -2006,12 +2031,14 @@ static bool med_sample(const Med &m, Vec a, Vec b, S &smp, Vec &pos, Vec &color)
     const Vec lb = add(mk(0.5f, 0.5f, 0.5f), sclf(0.5f, med_uvw(m, b)));
     const Vec lstep = dvs(sub(lb, la), tmax), wstep = dvs(sub(b, a), tmax);
     float t = 0.0f;
+    g_track_segments.fetch_add(1, std::memory_order_relaxed);
     for(int it = 0; it < MAX_TRACKING; ++it)
     {
         t = t + -dm::log(1.0f - smp.get1d()) * m.inv_maj;
         if(t >= tmax)
             return false;
         const Vec uvw = add(la, sclf(t, lstep));
+        g_track_lookups.fetch_add(1, std::memory_order_relaxed);
         const float dens = tex3(m.dens, m.nx, m.ny, m.nz, 1, 0, uvw);
         if(smp.get1d() < dens * m.inv_maj)
         {
@@ -2548,7 +2575,7 @@ static void trace_path(const Scene &s, const Params &p, uint64_t path_index, std
         const btrc_material_desc m = material_at(s, in.mat, sp);
         Shd shd;
         shd.geo = sp.frame;
-        shd.sh = rot_z(sp.frame, sp.iz);
+        shd.sh = normal_mapped(s, m, sp, rot_z(sp.frame, sp.iz));
         shd.st = m.shadow_terminator != 0;
         shd.disney = m.type == BTRC_MATERIAL_DISNEY;
         shd.invisible = m.type == BTRC_MATERIAL_INVISIBLE;
@@ -2632,24 +2659,32 @@ static double bf16(double x)
 static int rows(int l) { return l == L - 1 ? OUT_ROWS : W; }
 static int offset(int l) { return l * W * IN; }
 
-// input encoding: 8 frequencies of the box-normalised position, direction,
-// normal, roughness, diffuse, specular, then constant ones up to 64
+// input encoding: 8 frequencies of the box-normalised position (sin and
+// cos of pi q directly, each doubled frequency by the double-angle
+// identities in FP32), direction, normal, roughness, diffuse, specular, then
+// constant ones up to 64
 static void encode(const float *v, const float *box, double *x)
 {
     int k = 0;
-    float q[3];
+    float q[3], sn[3], cs[3];
     for(int a = 0; a < 3; ++a)
+    {
         q[a] = (v[a] - box[a]) * box[3 + a];
-    float sc = PI;
+        const float t = PI * q[a];
+        sn[a] = dm::sin(t);
+        cs[a] = dm::cos(t);
+    }
     for(int f = 0; f < 8; ++f)
     {
         for(int a = 0; a < 3; ++a)
         {
-            const float t = sc * q[a];
-            x[k++] = dm::sin(t);
-            x[k++] = dm::cos(t);
+            x[k++] = sn[a];
+            x[k++] = cs[a];
+            const float s2 = 2.0f * sn[a] * cs[a];
+            const float c2 = (cs[a] - sn[a]) * (cs[a] + sn[a]);
+            sn[a] = s2;
+            cs[a] = c2;
         }
-        sc = sc * 2.0f;
     }
     for(int j = 3; j < 16; ++j)
         x[k++] = v[j];
@@ -3068,6 +3103,13 @@ void oracle_detmath(int32_t fn, const float *x, const float *y, int32_t n, float
     }
 }
 
+// medium-stage work counters since the last call (segments, lookups); resets them
+void oracle_tracking_stats(uint64_t *segments, uint64_t *lookups)
+{
+    *segments = orc::g_track_segments.exchange(0);
+    *lookups = orc::g_track_lookups.exchange(0);
+}
+
 // ACESToneMap, postprocess/tonemap.cpp:14-24,35-51: the Narkowicz fit of
 // exposure-scaled RGB, saturated; alpha unchanged
 void oracle_aces(const float *rgba, int32_t n, float exposure, float *out)
@@ -3078,7 +3120,7 @@ void oracle_aces(const float *rgba, int32_t n, float exposure, float *out)
         {
             const float x = rgba[4 * i + c] * exposure;
             const float y = (x * (2.51f * x + 0.03f)) / (x * (2.43f * x + 0.59f) + 0.14f);
-            out[4 * i + c] = y < 0.0f ? 0.0f : (y > 1.0f ? 1.0f : y);
+            out[4 * i + c] = !(y > 0.0f) ? 0.0f : (y > 1.0f ? 1.0f : y); // saturate: NaN -> 0
         }
         out[4 * i + 3] = rgba[4 * i + 3];
     }

@@ -57,6 +57,7 @@ def lib():
         L.oracle_kat_rays.argtypes = [C.c_int32, C.c_int32, vp, vp, vp, C.c_float, vp]
         L.oracle_kat_pixels.argtypes = [C.c_int32, C.c_int32, vp, vp, vp]
         L.oracle_aces.argtypes = [vp, C.c_int32, C.c_float, vp]
+        L.oracle_tracking_stats.argtypes = [P(C.c_uint64), P(C.c_uint64)]
         L.oracle_halton_index.restype = C.c_uint64
         L.oracle_halton_index.argtypes = [C.c_int32, C.c_int32, C.c_uint32, C.c_uint32, C.c_int32]
         _lib = L
@@ -250,6 +251,14 @@ def halton(width, height, px, py, sample_index, dims):
     lib().oracle_halton(width, height, px.ctypes.data, py.ctypes.data, si.ctypes.data, len(px), dims,
                         out.ctypes.data)
     return out
+
+
+def tracking_stats():
+    """(delta-tracking segments, density lookups) of the oracle's medium
+    stage since the last call"""
+    a, b = C.c_uint64(), C.c_uint64()
+    lib().oracle_tracking_stats(C.byref(a), C.byref(b))
+    return a.value, b.value
 
 
 def aces(rgba, exposure=1.0):

@@ -6,6 +6,20 @@
 
 #include <cstdio>
 
+// counts the Reporter calls of a render
+struct CountingReporter : btrc_hip::Reporter
+{
+    int stages = 0, completed = 0, progress_calls = 0;
+    float last = -1.0f;
+    void new_stage(const char *) override { ++stages; }
+    void complete_stage() override { ++completed; }
+    void progress(float pct) override
+    {
+        ++progress_calls;
+        last = pct;
+    }
+};
+
 int main(int argc, char **argv)
 {
     if(argc < 2)
@@ -73,7 +87,14 @@ int main(int argc, char **argv)
         tracer.set_scene(scene);
         tracer.set_camera(cam);
         tracer.set_film(48, 32);
+        CountingReporter rep;
+        tracer.set_reporter(&rep);
         const btrc_hip::RenderResult r = tracer.render();
+        tracer.set_reporter(nullptr);
+        const std::vector<float> toned = tracer.tonemapped(1.0f);
+        float toned_max = 0.0f;
+        for(size_t k = 0; k < toned.size(); ++k)
+            toned_max = toned[k] > toned_max ? toned[k] : toned_max;
 
         std::FILE *f = std::fopen(argv[1], "wb");
         if(!f || std::fwrite(r.color.data(), sizeof(float), r.color.size(), f) != r.color.size())
@@ -84,6 +105,8 @@ int main(int argc, char **argv)
         std::fclose(f);
         std::printf("rendered %d x %d, %llu extension rays, %llu shadow rays\n", r.width, r.height,
                     (unsigned long long)r.stats.extension_rays, (unsigned long long)r.stats.shadow_rays);
+        std::printf("reporter: %d stage, %d completed, %d progress calls, last %.1f; tone mapped max %.3f\n",
+                    rep.stages, rep.completed, rep.progress_calls, rep.last, toned_max);
         return 0;
     }
     catch(const btrc_hip::BtrcException &e)

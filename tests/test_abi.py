@@ -36,7 +36,7 @@ def test_struct_sizes():
     assert C.sizeof(_abi.Hit) == 20
     assert C.sizeof(_abi.CameraDesc) == 44
     assert C.sizeof(_abi.RenderParams) == 64
-    assert C.sizeof(_abi.MaterialDesc) == 80
+    assert C.sizeof(_abi.MaterialDesc) == 80 + 11 * 4
     assert C.sizeof(_abi.TextureDesc) == 4 * 4 + 8 + 4 * 4 + 16
     assert C.sizeof(_abi.InstanceDesc) == 4 * (2 + 32 + 1 + 3 + 2)
     assert C.sizeof(_abi.RenderStats) == 8 * (1 + 4 + 7 + 7)

@@ -61,4 +61,7 @@ def test_program_renders_like_the_oracle(tmp_path):
     rel = np.abs(img - ref) / np.maximum(np.abs(ref), 1e-2)
     assert rel.max() <= 1e-4, f"max rel err {rel.max()}"
     assert f"{e} extension rays, {sh} shadow rays" in r.stdout
+    assert "reporter: 1 stage, 1 completed," in r.stdout and "last 100.0;" in r.stdout
+    toned_max = float(r.stdout.split("tone mapped max ")[1].split()[0])
+    assert 0.0 < toned_max <= 1.0
     assert ref.sum() > 0

@@ -60,4 +60,7 @@ def test_cli_run_writes_the_image(ctx, tmp_path):
     png = np.asarray(Image.open(tmp_path / "out" / "color.png"))
     want = np.clip(np.floor(np.maximum(res.color[..., :3], 0) ** (1 / 2.2) * 256), 0, 255).astype(np.uint8)
     assert png.shape == (36, 48, 3) and np.array_equal(png, want)
-    assert (res.color[..., :3] <= 1.0).all()        # tone mapped
+    # tone mapped on the device: the oracle's ACES fit of the film, bit for bit
+    film = ctx.film(48, 36)[0]
+    assert np.array_equal(res.color.view(np.uint32), oracle.aces(film, 1.0).view(np.uint32))
+    assert (res.color[..., :3] <= 1.0).all()

@@ -69,3 +69,53 @@ def test_textured_render_per_pixel_parity(ctx):
     rel = np.abs(rgba[..., :3] - ref) / np.maximum(np.abs(ref), 1e-2)
     assert rel.max() <= 1e-4, f"max rel err {rel.max()}"
     assert stats.extension_rays == e and stats.shadow_rays == s
+
+
+def test_every_parameter_textured_and_normal_maps_per_pixel(ctx):
+    """noise textures on all thirteen Disney parameters and normal maps on
+    Disney and diffuse materials: image and first-hit g-buffer per pixel"""
+    scene, cam = scenes.sphere_field(instances=48, lights=6, seed=21, extent=12.0)
+    scene.geometries[0]["uvs"] = scenes.uv_sphere_uvs()
+    rng = np.random.default_rng(4)
+
+    def float_tex(lo, hi, size=32):
+        t = rng.uniform(lo, hi, (size, size, 4)).astype(np.float32)
+        return scene.add_texture(t, address_u="wrap")
+    unit = [float_tex(0.0, 1.0) for _ in range(4)]
+    ior = float_tex(1.2, 1.8)
+    nm = rng.uniform(0.3, 0.7, (32, 32, 4)).astype(np.float32)
+    nm[..., 2] = rng.uniform(0.85, 1.0, (32, 32))
+    nmap = scene.add_texture(nm, address_u="wrap")
+    names = [n for n in _abi.MATERIAL_TEXTURE_SLOTS if n not in ("color", "normal", "ior")]
+    k = 0
+    for m in scene.materials:
+        if m["type"] != _abi.MATERIAL_DISNEY:
+            continue
+        m["color_texture"] = unit[k % 4]
+        for j, name in enumerate(names):
+            if (k + j) % 3:
+                m[name + "_texture"] = unit[(k + j) % 4]
+        m["ior_texture"] = ior
+        if k % 2:
+            m["normal_texture"] = nmap
+        k += 1
+    for i in range(0, len(scene.instances), 5):
+        if scene.instances[i].get("light") is None:
+            scene.instances[i]["material"] = scene.add_diffuse([0.7, 0.6, 0.5], normal_texture=nmap)
+    p = render_params(96, 64, 8, state_count=4096)
+    p.albedo, p.normal = 1, 1
+    ctx.upload_scene(scene)
+    ctx.set_camera(cam, p.width, p.height)
+    stats = ctx.render(p)
+    rgba, _ = ctx.film(p.width, p.height)
+    alb, nrm = ctx.aov(p.width, p.height)
+    ref, e, s, ralb, rnrm = oracle.OracleScene(scene).render(cam, p, aov=True)
+    ref /= p.spp
+    rel = np.abs(rgba[..., :3] - ref) / np.maximum(np.abs(ref), 1e-2)
+    assert rel.max() <= 1e-4, f"max rel err {rel.max()}"
+    assert stats.extension_rays == e and stats.shadow_rays == s
+    assert np.abs(alb[..., :3] - ralb / p.spp).max() <= 1e-5
+    n = rnrm / p.spp
+    ln = np.linalg.norm(n, axis=2, keepdims=True)
+    n = np.where(ln > 1e-3, n / np.maximum(ln, 1e-30), 0.0)
+    assert np.abs(nrm[..., :3] - n).max() <= 1e-4

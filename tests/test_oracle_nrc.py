@@ -93,3 +93,23 @@ def test_bf16_rounding_is_a_small_perturbation():
     rounded = oracle.nrc_forward(w, BOX, v, bf16=True)
     rel = np.abs(rounded - exact) / (np.abs(exact).mean() + 1e-9)
     assert rel.max() < 0.1 and rel.mean() < 0.02
+
+
+def test_frequency_encoding_order_and_values():
+    """input 6 f + 2 a + c is sin (c = 0) or cos (c = 1) of 2^f pi q_a, q the
+    box-normalised position: read through two ReLU units (+x and -x)"""
+    rng = np.random.default_rng(4)
+    v = random_vertices(64, rng)
+    q = ((v[:, :3] - BOX[:3]) * BOX[3:]).astype(np.float64)
+    for f in range(8):
+        for a in range(3):
+            for c in range(2):
+                k = 6 * f + 2 * a + c
+                w = np.zeros(P, dtype=np.float32)
+                layer(w, 0)[0, k], layer(w, 0)[1, k] = 1.0, -1.0
+                for l in range(1, 5):
+                    layer(w, l)[0, 0] = layer(w, l)[1, 1] = 1.0
+                layer(w, 5)[0, 0], layer(w, 5)[0, 1] = 1.0, -1.0
+                got = oracle.nrc_forward(w, BOX, v, bf16=False)[:, 0]
+                want = (np.cos if c else np.sin)(2.0 ** f * np.pi * q[:, a])
+                assert np.abs(got - want).max() < 2e-4, (f, a, c, np.abs(got - want).max())

@@ -9,8 +9,8 @@ import struct
 import numpy as np
 import pytest
 
+from btrc_amd import _abi, scenes
 from btrc_amd import scene_json as sj
-from btrc_amd import scenes
 from oracle import oracle
 
 from util import render_params
@@ -131,12 +131,7 @@ def test_unsupported_types_are_named(tmp_path, snippet, name):
         sj.load(str(path))
 
 
-def test_aces_and_image_writers(tmp_path):
-    c = np.array([[[0.0, 0.5, 2.0, 1.0]]], np.float32)
-    t = sj.aces(c, exposure=1.0)
-    x = np.array([0.0, 0.5, 2.0])
-    want = np.clip((x * (2.51 * x + 0.03)) / (x * (2.43 * x + 0.59) + 0.14), 0, 1)
-    assert np.allclose(t[0, 0, :3], want, atol=1e-6)
+def test_image_writers(tmp_path):
     from PIL import Image
     img = np.array([[[0.0, 0.5, 0.999], [1.0, -0.2, 0.00390625]]], np.float32)
     sj.save_image(str(tmp_path / "a.png"), np.concatenate([img, np.ones((1, 2, 1), np.float32)], axis=2))
@@ -149,3 +144,32 @@ def test_aces_and_image_writers(tmp_path):
     # the last scanline holds B, G, R of the bottom row
     last = data[-(4 * 5 * 3):]
     assert np.array_equal(np.frombuffer(last, "<f4").reshape(3, 5), rgb[2].T[::-1])
+
+
+def test_textured_parameters_and_normal_maps(tmp_path):
+    """a Disney material with textures on sheen and ior and a normal map, and
+    a diffuse material with a normal map (DisneyMaterialCreator,
+    disney.cpp:844-894; NormalMap::load, normal_map.cpp:5-11)"""
+    sky = scenes.sky_texture(8, 4, seed=1)
+    sj.save_image(str(tmp_path / "t.hdr"), sky)
+    tex = '{ "type": "array", "filename": "' + str(tmp_path / "t.hdr") + '" }'
+    mats = open(os.path.join(os.path.dirname(BOX), "materials.json")).read()
+    mats = mats.replace('"specular": 0.5, "clearcoat": 0.4 }',
+                        f'"specular": 0.5, "clearcoat": 0.4, "sheen": {tex}, "ior": {tex}, "normal": {tex} }}')
+    mats = mats.replace('"white": { "type": "diffuse", "color": [ 0.73, 0.73, 0.73 ] }',
+                        '"white": { "type": "diffuse", "color": [ 0.73, 0.73, 0.73 ], "normal": [ 0.5, 0.5, 1 ] }')
+    (tmp_path / "materials.json").write_text(mats)
+    src = open(BOX).read().replace("$env{scene_directory}", os.path.dirname(BOX))
+    (tmp_path / "scene.json").write_text(src)
+    sf = sj.load(str(tmp_path / "scene.json"))
+    glossy = [m for m in sf.scene.materials if m["type"] == _abi.MATERIAL_DISNEY and m.get("sheen_texture") is not None]
+    assert glossy, "the Disney material lost its sheen texture"
+    g = glossy[0]
+    for slot in ("sheen", "ior", "normal"):      # one texture object per node, as the factory makes them
+        assert sf.scene.textures[g[slot + "_texture"]]["texels"].shape == (4, 8, 4)
+    assert g["specular"] == 0.5 and g.get("specular_texture") is None
+    white = [m for m in sf.scene.materials if m["type"] == _abi.MATERIAL_DIFFUSE and m.get("normal_texture") is not None]
+    assert white and sf.scene.textures[white[0]["normal_texture"]]["texels"].shape == (1, 1, 4)
+    p = render_params(sf.width, sf.height, 1, min_depth=3, max_depth=4)
+    img = oracle.OracleScene(sf.scene).render(sf.camera, p)[0]
+    assert np.isfinite(img).all() and img.sum() > 0

@@ -44,3 +44,20 @@ def test_aces_exposure_is_a_scale():
     scaled = a.copy()
     scaled[:, :3] *= e
     assert np.array_equal(oracle.aces(a, e), oracle.aces(scaled, 1.0))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("exposure", [1.0, 0.37, 4.0])
+def test_device_tonemap_equals_oracle(ctx, exposure):
+    from btrc_amd import scenes
+    from util import render_params
+    scene, cam = scenes.cornell()
+    ctx.upload_scene(scene)
+    ctx.set_camera(cam, 64, 48)
+    ctx.render(render_params(64, 48, 4))
+    film = ctx.film(64, 48)[0]
+    got = ctx.tonemapped(64, 48, exposure)
+    assert np.array_equal(got.view(np.uint32), oracle.aces(film, exposure).view(np.uint32))
+    assert (got[..., :3] >= 0).all() and (got[..., :3] <= 1).all() and (got[..., 3] == 1).all()
+    if exposure >= 1.0:
+        assert 0 < (got[..., :3] == 1).sum() < got[..., :3].size     # the light saturates, the rest does not
