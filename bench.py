@@ -186,6 +186,8 @@ def main():
     ap.add_argument("--nrc-query-depth", type=int, default=2, help="nrc workload: depth of the cache query")
     ap.add_argument("--nrc-train-stride", type=int, default=16,
                     help="nrc workload: one pixel in this many traces a training path")
+    ap.add_argument("--nrc-seeds", default="1,2,3,4",
+                    help="network initialisation seeds of the radiance-cache runs (each trained from scratch)")
     ap.add_argument("--nrc-gt-spp", type=int, default=65536,
                     help="samples per pixel of the radiance-cache comparison's FP64 ground truth")
     ap.add_argument("--no-nrc", action="store_true",
@@ -459,14 +461,14 @@ def nrc_equal_time(args, train_depths):
 
     checkpoints = sorted({c for c in (1, 2, 4, 8, 16, 32, 64, 128, 256, 512, 1024) if c <= args.frames} | {args.frames})
 
-    def configure(cache, train_depth):
+    def configure(cache, train_depth, seed=1):
         ctx.nrc_configure(enabled=1 if cache else 0, train=1, train_depth=train_depth, steps=args.nrc_steps,
                           train_stride=args.nrc_train_stride, query_depth=args.nrc_query_depth)
         if cache:
-            ctx.nrc_reset(1)
+            ctx.nrc_reset(seed)
 
-    def run(cache, train_depth=0, target=None):
-        configure(cache, train_depth)
+    def run(cache, train_depth=0, target=None, seed=1):
+        configure(cache, train_depth, seed)
         p = params(1)
         p.time_stages = 0
         curve, seconds, rays = [], 0.0, 0
@@ -529,17 +531,33 @@ def nrc_equal_time(args, train_depths):
         return None
 
     runs = {}
+    seeds = [int(s) for s in args.nrc_seeds.split(",")]
     for d in train_depths:
-        log(f"bench: path tracing with the radiance cache, training depth {d}")
-        c = run(True, d, target)
+        log(f"bench: path tracing with the radiance cache, training depth {d}, network seeds {seeds}")
+        # each seed trains its own network from scratch; the first seed's run is reported in full
+        per_seed = []
+        for seed in seeds:
+            cs = run(True, d, target, seed)
+            ts = time_to(cs["curve"], target)
+            per_seed.append(dict(seed=seed, frames=cs["frames"], seconds=round(cs["seconds"], 4),
+                                 time_to_target_s=ts,
+                                 time_ratio=None if ts is None else round(ts / plain["seconds"], 3),
+                                 final_relmse=cs["curve"][-1][2]))
+            if seed == seeds[0]:
+                c = cs
+        ratios = [s["time_ratio"] for s in per_seed if s["time_ratio"] is not None]
         t_cache = time_to(c["curve"], target)
         budget = [x for x in c["curve"] if x[0] <= args.frames][-1]
         stage, qpf = staged(True, d)
         infer_ms, train_ms = stage[5], stage[6]
         infer_tflops = NRC_FLOP_PER_QUERY * qpf / (infer_ms * 1e-3) / 1e12 if infer_ms > 0 else None
-        train_tflops = (NRC_FLOP_PER_TRAIN_SAMPLE * 16384 * args.nrc_steps / (train_ms * 1e-3) / 1e12
+        records_pf = c["records"] / c["frames"]
+        train_tflops = (NRC_FLOP_PER_TRAIN_SAMPLE * records_pf / (train_ms * 1e-3) / 1e12
                         if train_ms > 0 else None)
         runs[f"train_depth_{d}"] = {
+            "time_ratio_over_seeds": {"mean": None if len(ratios) < len(seeds) else round(float(np.mean(ratios)), 3),
+                                      "min": min(ratios) if ratios else None,
+                                      "max": max(ratios) if ratios else None, "runs": per_seed},
             "seconds": round(c["seconds"], 4), "frames": c["frames"], "relmse_curve": c["curve"],
             "mrays_per_s": round(c["rays"] / c["seconds"] / 1e6, 2),
             "queries_per_frame": round(c["queries"] / c["frames"]),
@@ -555,14 +573,15 @@ def nrc_equal_time(args, train_depths):
             "infer_tflops": None if infer_tflops is None else round(infer_tflops, 3),
             "train_tflops": None if train_tflops is None else round(train_tflops, 3),
         }
-        log(f"bench: cache (training depth {d}): time ratio {runs[f'train_depth_{d}']['time_ratio']}")
+        log(f"bench: cache (training depth {d}): time ratio {runs[f'train_depth_{d}']['time_ratio']}, "
+            f"over seeds {runs[f'train_depth_{d}']['time_ratio_over_seeds']['mean']}")
     ctx.nrc_configure(enabled=0)
     plain_stage, _ = staged(False)
     return {
         "workload": f"Cornell-style room, 256x256, 1 spp per frame x {args.frames} frames, max depth {MAX_DEPTH}, "
                     f"radiance cache queried at depth {args.nrc_query_depth}, trained on 1/{args.nrc_train_stride} "
-                    f"of the pixels ({args.nrc_steps} Adam steps of 16384 records per frame); timed frames replay "
-                    "the captured bounce loop",
+                    f"of the pixels ({args.nrc_steps} Adam steps per frame over the frame's training records); timed "
+                    "frames replay the captured bounce loop",
         "ground_truth": {"spp": args.nrc_gt_spp, "sampler": "independent", "film": "fp64",
                          "split_half_relmse": gt_split, "seconds": round(gt_seconds, 2)},
         "target_relmse": target,
@@ -668,6 +687,7 @@ def instanced_workload(args):
     p.sample_offset, p.accumulate, p.time_stages = args.frames, 1, 1
     st = ctx.render_views(p, views)
     stage = np.array(st.stage_ms[:7])
+    staged_ext, trace_launches = st.extension_rays, max(1, int(st.stage_launches[1]))
     p.time_stages = 0
 
     # the same frame over one tree of all 16.8M world-space triangles built
@@ -688,6 +708,10 @@ def instanced_workload(args):
                        gt_spp=args.gt_spp, gt_sampler="independent", gt_film="fp64", cache=True,
                        gt_seconds=round(time.perf_counter() - t0, 2))
         log(f"bench: view 0 relMSE after {args.frames} frames vs {args.gt_spp} spp FP64 GT: {quality['relmse']:.5f}")
+
+    cpu, roofline = None, None
+    if not args.no_cpu:
+        cpu, roofline = instanced_baseline(ctx, scene, views, p, stage[1], staged_ext, trace_launches, args)
 
     return {
         "metric": METRIC,
@@ -717,10 +741,50 @@ def instanced_workload(args):
         "nrc_last_frame": {"queries": int(ns.queries), "records": int(ns.records),
                            "loss": round(float(ns.loss[0]), 5)},
         "quality": quality,
-        "roofline": None,
-        "cpu_baseline": None,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
         "bvh_build": bvh,
     }
+
+
+def instanced_baseline(ctx, scene, views, p, trace_ms, ext, launches, args):
+    """CPU baseline and traversal work model of the instanced workload: the
+    extension rays of the first two bounces of its largest view (1 spp),
+    traced by lib/bvh (BinnedSAH16 + SingleRayTraverser) over the 16.8M
+    world-space triangles flattened on the host. lib/bvh has no two-level
+    traversal, so the work model prices each ray by what a one-level
+    traversal of the same triangles reads (32 + 20 + 64 per step + 36 per
+    triangle test, its statistics on these rays); the two-level kernel's
+    time comes from the stage-timed frame."""
+    from btrc_amd import _abi
+    cam, w, h = max(views, key=lambda v: v[1] * v[2])
+    ctx.nrc_configure(enabled=0)
+    ctx.set_camera(cam, w, h)
+    q = _abi.RenderParams.from_buffer_copy(p)
+    q.width, q.height, q.spp, q.sample_offset, q.accumulate, q.time_stages = w, h, 1, 0, 0, 0
+    log("bench: capturing rays of the largest view's first two bounces")
+    rays = ctx.capture_rays(q, 2, 1 << 22)
+    rays = rays[np.random.default_rng(0).permutation(len(rays))]
+    log("bench: flattening the instances for lib/bvh")
+    world = flatten_instances(scene)
+    cpu, stats = cpu_baseline(rays, world, args.cpu_seconds, host_threads())
+    if cpu is None:
+        return None, None
+    cpu["steps_per_ray"] = float(np.mean(stats[0]))
+    cpu["isects_per_ray"] = float(np.mean(stats[1]))
+    cpu["sample"] = cpu["sample"].replace("of this frame", f"of the {w}x{h} view") + \
+        f", over the {len(world)} flattened world triangles"
+    bytes_per_ray = algorithmic_bytes_per_ray(*stats)
+    rays_per_launch = ext / launches
+    launch_s = trace_ms * 1e-3 / launches
+    achieved = bytes_per_ray * rays_per_launch / launch_s / 1e9
+    roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None, kernel="k_trace_extension (two-level)",
+                    algorithmic_bytes_per_ray=round(bytes_per_ray, 1),
+                    algorithmic_bytes_per_launch=round(bytes_per_ray * rays_per_launch),
+                    rays_per_launch=round(rays_per_launch), launch_ms=round(launch_s * 1e3, 4),
+                    model="one-level lib/bvh statistics over the flattened triangles, same rays")
+    return cpu, roofline
 
 
 def instanced_flat_comparison(ctx, scene, p, views, stage):

@@ -87,7 +87,8 @@ class Context:
         self._check(self.lib.btrc_hip_get_tuning(self.h, C.byref(t)))
         return dict(closest_postpone_leaves=t.closest_postpone_leaves, any_postpone_leaves=t.any_postpone_leaves,
                     trace_waves_per_cu=t.trace_waves_per_cu, launch_graph=t.launch_graph,
-                    two_level_postpone=t.two_level_postpone)
+                    two_level_postpone=t.two_level_postpone, stage_blocks_per_cu=t.stage_blocks_per_cu,
+                    graph_bounces=t.graph_bounces)
 
     def set_tuning(self, **kw):
         cur = self.get_tuning()

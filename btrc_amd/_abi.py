@@ -151,7 +151,8 @@ class RenderStats(C.Structure):
 class Tuning(C.Structure):
     _fields_ = [("closest_postpone_leaves", C.c_int32), ("any_postpone_leaves", C.c_int32),
                 ("trace_waves_per_cu", C.c_int32), ("launch_graph", C.c_int32),
-                ("two_level_postpone", C.c_int32), ("reserved", C.c_int32 * 3)]
+                ("two_level_postpone", C.c_int32), ("stage_blocks_per_cu", C.c_int32),
+                ("graph_bounces", C.c_int32), ("reserved", C.c_int32)]
 
 
 class BvhNode(C.Structure):

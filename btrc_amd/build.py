@@ -29,6 +29,12 @@ HIP_FLAGS = [
 ]
 HOST_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-I" + os.path.join(ROOT, "include")]
 
+# per-source extra flags: the cache's kernels keep MFMA results in VGPRs (no
+# accumulator-register read-backs between layers)
+EXTRA_FLAGS = {
+    "nrc.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"],
+}
+
 HIP_SOURCES = [
     os.path.join(CSRC, "kernels", "wavefront.hip"),
     os.path.join(CSRC, "kernels", "nrc.hip"),
@@ -53,7 +59,7 @@ def build_library(verbose=False):
     jobs = []
     for src in HIP_SOURCES:
         obj = os.path.join(BUILD, os.path.basename(src) + ".o")
-        jobs.append([HIPCC, *HIP_FLAGS, "-c", src, "-o", obj])
+        jobs.append([HIPCC, *HIP_FLAGS, *EXTRA_FLAGS.get(os.path.basename(src), []), "-c", src, "-o", obj])
     for src in HOST_SOURCES:
         obj = os.path.join(BUILD, os.path.basename(src) + ".o")
         jobs.append(["g++", *HOST_FLAGS, "-c", src, "-o", obj])

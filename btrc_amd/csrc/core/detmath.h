@@ -80,13 +80,13 @@ BTRC_DM_FN void sincos(float x, float *s, float *c)
     pc = fmaf(r2, pc, -0.5f);
     const float cr = fmaf(r2, pc, 1.0f);
 
-    float so, co;
-    if(q == 0)      { so = sr;  co = cr;  }
-    else if(q == 1) { so = cr;  co = -sr; }
-    else if(q == 2) { so = -sr; co = -cr; }
-    else            { so = -cr; co = sr;  }
-    *s = so;
-    *c = co;
+    // quadrant q: (sin, cos) = (sr, cr), (cr, -sr), (-sr, -cr), (-cr, sr),
+    // chosen without branches (selects and sign flips only)
+    const bool odd = (q & 1) != 0;
+    const float so = odd ? cr : sr;
+    const float co = odd ? sr : cr;
+    *s = (q & 2) ? -so : so;
+    *c = ((q + 1) & 2) ? -co : co;
 }
 
 BTRC_DM_FN float sin(float x) { float s, c; sincos(x, &s, &c); return s; }

@@ -12,18 +12,18 @@
 // FP32 (MFMA). The output layer is stored padded to 32 rows; rows 3..31 stay
 // zero.
 //
-// Input encoding (64 values) of a vertex record (NrcVertex):
-//   [0, 48)  sin(2^f pi x_a), cos(2^f pi x_a) for f = 0..7 and the three
-//            axes of the position normalised to the scene box, ordered
-//            (f, axis, sin/cos); f = 0 evaluated directly, each higher
-//            frequency from the one below by the double-angle identities
-//            sin 2t = 2 sin t cos t, cos 2t = (cos t - sin t)(cos t + sin t)
-//   [48, 51) direction toward the previous vertex (unit)
-//   [51, 54) shading normal
-//   [54]     roughness
-//   [55, 58) diffuse reflectance
-//   [58, 61) specular reflectance
-//   [61, 64) constant 1
+// Input encoding (64 values) of a vertex record (NrcVertex), laid out for
+// the MFMA operand it feeds: input k = 16 s + 8 h + j (slice s = 0..3, half
+// h = 0..1, j = 0..7) is element 8 s + j of half h's 32 values:
+//   e = 0..23  sin (e even) or cos (e odd) of 2^f pi x_a, f = 4 h + e / 6,
+//              axis a = (e % 6) / 2, x the position normalised to the scene
+//              box; each half starts its chain at 2^(4 h) pi x_a directly
+//              and reaches the next three frequencies by the double-angle
+//              identities sin 2t = 2 sin t cos t,
+//              cos 2t = (cos t - sin t)(cos t + sin t)
+//   e = 24..31 half 0: direction toward the previous vertex (3), shading
+//              normal (3), roughness, diffuse red; half 1: diffuse green and
+//              blue, specular reflectance (3), three constant ones
 #pragma once
 
 #include "detmath.h"
@@ -84,21 +84,18 @@ BTRC_HD void nrc_material_inputs(const MaterialRec &m, float *roughness, V3 *dif
     }
 }
 
-// the 64 network inputs of a vertex record
-BTRC_HD void nrc_encode(const NrcVertex &v, const NrcBox &box, float *x)
+// the 32 network inputs of half h of a vertex record, in the order above
+BTRC_HD void nrc_encode_half(const NrcVertex &v, const NrcBox &box, int h, float *x)
 {
-    float q[3];
-    for(int a = 0; a < 3; ++a)
-        q[a] = (v.p[a] - box.lo[a]) * box.inv_extent[a];
-    int k = 0;
+    const float base = h ? 16.0f * PI : PI;
     float s[3], c[3];
     for(int a = 0; a < 3; ++a)
     {
-        const float t = PI * q[a];
-        s[a] = btrc_detmath::sin(t);
-        c[a] = btrc_detmath::cos(t);
+        const float q = (v.p[a] - box.lo[a]) * box.inv_extent[a];
+        btrc_detmath::sincos(base * q, &s[a], &c[a]);
     }
-    for(int f = 0; f < NRC_FREQS; ++f)
+    int k = 0;
+    for(int f = 0; f < 4; ++f)
     {
         for(int a = 0; a < 3; ++a)
         {
@@ -110,17 +107,11 @@ BTRC_HD void nrc_encode(const NrcVertex &v, const NrcBox &box, float *x)
             c[a] = c2;
         }
     }
-    for(int a = 0; a < 3; ++a)
-        x[k++] = v.wo[a];
-    for(int a = 0; a < 3; ++a)
-        x[k++] = v.n[a];
-    x[k++] = v.roughness;
-    for(int a = 0; a < 3; ++a)
-        x[k++] = v.diffuse[a];
-    for(int a = 0; a < 3; ++a)
-        x[k++] = v.specular[a];
-    while(k < NRC_IN)
-        x[k++] = 1.0f;
+    const float tail0[8] = { v.wo[0], v.wo[1], v.wo[2], v.n[0], v.n[1], v.n[2], v.roughness, v.diffuse[0] };
+    const float tail1[8] = { v.diffuse[1], v.diffuse[2], v.specular[0], v.specular[1], v.specular[2],
+                             1.0f, 1.0f, 1.0f };
+    for(int j = 0; j < 8; ++j)
+        x[k++] = h ? tail1[j] : tail0[j];
 }
 
 // relative L2 loss of one output channel (Mueller et al. eq. 3): the

@@ -323,6 +323,7 @@ struct btrc_ctx
         DevBuf<float> acc;
         DevBuf<int32_t> counts;         // [0] queries, [1] valid records of a batch, [2] valid records of the frame
         DevBuf<int32_t> index;          // the frame's valid record slots, compacted
+        DevBuf<uint64_t> valid_mask;    // their flags in bijection order, one bit each
         DevBuf<uint16_t> xt, dzt;
         DevBuf<double> loss_partial;
         DevBuf<int32_t> count_partial;
@@ -338,7 +339,7 @@ struct btrc_ctx
     DevBuf<float> ovf_t;
     int32_t trace_grid = 0;
     DevBuf<uint32_t> fetch;
-    btrc_tuning tuning{ 1, 0, TRACE_WAVES_PER_CU, 1, 0, { 0, 0, 0 } };
+    btrc_tuning tuning{ 1, 0, TRACE_WAVES_PER_CU, 1, 0, 0, 0, 0 };
 
     // the bounce loop as a captured graph, kept while its launches stay the same
     RenderConsts *rc_host = nullptr;  // pinned
@@ -400,7 +401,7 @@ struct btrc_ctx
     int persistent_grid(int threads_per_block, int64_t work) const
     {
         const int64_t want = (work + threads_per_block - 1) / threads_per_block;
-        const int64_t cap = (int64_t)cu_count * 16;
+        const int64_t cap = (int64_t)cu_count * (tuning.stage_blocks_per_cu > 0 ? tuning.stage_blocks_per_cu : 16);
         return (int)std::max<int64_t>(1, std::min(want, cap));
     }
 };
@@ -1538,6 +1539,7 @@ void nrc_alloc(btrc_ctx *ctx)
     n.acc.alloc(4 * NRC_RECORDS);
     n.counts.alloc(3);
     n.index.alloc(NRC_RECORDS);
+    n.valid_mask.alloc(NRC_RECORDS / 64);
     n.xt.alloc((size_t)NRC_LAYERS * NRC_IN * NRC_RECORDS);
     n.dzt.alloc((size_t)NRC_LAYERS * NRC_IN * NRC_RECORDS);
     n.loss_partial.alloc(NRC_RECORDS / 32);
@@ -1887,7 +1889,8 @@ void render(btrc_ctx *ctx, const btrc_render_params *p, int32_t view_count, cons
     int64_t captured = 0;
     const auto t0 = std::chrono::steady_clock::now();
     int cur = 0;
-    const int check_every = 4; // even: a block of bounces ends on the pool side it began with
+    // even: a block of bounces ends on the pool side it began with
+    const int check_every = ctx->tuning.graph_bounces > 0 ? ctx->tuning.graph_bounces : 4;
 
     // one bounce: plan, generate, trace, [medium], shade, shadow
     auto enqueue_bounce = [&](int side, bool timed) {
@@ -2015,7 +2018,7 @@ void render(btrc_ctx *ctx, const btrc_render_params *p, int32_t view_count, cons
         {
             nrc_steps = std::min(n.params.steps, 64);
             timer.run(STAGE_TRAIN, [&] {
-                launch_nrc_compact(s, n.rec.ptr, n.index.ptr, n.counts.ptr + 2);
+                launch_nrc_compact(s, n.rec.ptr, n.valid_mask.ptr, n.index.ptr, n.counts.ptr + 2);
                 nrc_train(ctx, nrc_steps, NRC_BATCH, true);
             });
         }
@@ -2201,7 +2204,9 @@ int btrc_hip_device_name(btrc_ctx *ctx, char *buf, size_t len)
 int btrc_hip_set_tuning(btrc_ctx *ctx, const btrc_tuning *t)
 {
     return guarded(ctx, [&] {
-        if(!t || t->trace_waves_per_cu < 1 || t->trace_waves_per_cu > TRACE_WAVES_PER_CU_MAX)
+        if(!t || t->trace_waves_per_cu < 1 || t->trace_waves_per_cu > TRACE_WAVES_PER_CU_MAX ||
+           t->stage_blocks_per_cu < 0 || t->stage_blocks_per_cu > 64 || t->graph_bounces < 0 ||
+           t->graph_bounces > 16 || (t->graph_bounces & 1))
             throw InvalidArg("bad tuning");
         BTRC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
         ctx->tuning = *t;

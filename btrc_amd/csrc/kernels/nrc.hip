@@ -62,13 +62,16 @@ constexpr int WS = NRC_IN + 8;                  // LDS row stride of the weights
 constexpr int NRC_ROWS = NRC_PARAMS / NRC_IN;   // 352 weight rows
 
 typedef short s16x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 
-// two FP32 values to packed BF16, round to nearest even (v_cvt_pk_bf16_f32)
+// two FP32 values to packed BF16, round to nearest even: one
+// v_cvt_pk_bf16_f32 from the compiler's own lowering (not inline asm, which
+// would read MFMA results without the compiler's hazard waits)
 __device__ __forceinline__ uint32_t pk_bf16(float lo, float hi)
 {
-    uint32_t r;
-    asm("v_cvt_pk_bf16_f32 %0, %1, %2" : "=v"(r) : "v"(lo), "v"(hi));
-    return r;
+    const f32x2 v = { lo, hi };
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector(v, bf16x2));
 }
 
 // ReLU of two packed BF16 values: negative values (sign bit set) are negative
@@ -104,28 +107,17 @@ __device__ void load_weights_padded(const uint16_t *w, uint16_t *s_w)
 }
 
 // the encoded inputs of this lane's sample as the first layer's B operand:
-// lane half h holds inputs 16 s + 8 h + 0..7 of slice s
-__device__ __forceinline__ void encode_frag(const NrcVertex &v, bool valid, const NrcBox &box, int h, Frag &x)
+// lane half h holds inputs 16 s + 8 h + 0..7 of slice s, which is what
+// nrc_encode_half computes for its half (an invalid sample encodes a zero
+// vertex: finite inputs whose gradients are zero)
+__device__ __forceinline__ void encode_frag(const NrcVertex &v, const NrcBox &box, int h, Frag &x)
 {
-    float in[NRC_IN];
-    if(valid)
-        nrc_encode(v, box, in);
-    else
-#pragma unroll
-        for(int k = 0; k < NRC_IN; ++k)
-            in[k] = 0.0f;
+    float in[NRC_IN / 2];
+    nrc_encode_half(v, box, h, in);
 #pragma unroll
     for(int s = 0; s < 4; ++s)
-    {
-        uint32_t w[4];
-#pragma unroll
-        for(int j = 0; j < 4; ++j)
-        {
-            const int k = 16 * s + 2 * j;
-            w[j] = pk_bf16(h ? in[k + 8] : in[k], h ? in[k + 9] : in[k + 1]);
-        }
-        x.s[s] = words8(w[0], w[1], w[2], w[3]);
-    }
+        x.s[s] = words8(pk_bf16(in[8 * s + 0], in[8 * s + 1]), pk_bf16(in[8 * s + 2], in[8 * s + 3]),
+                        pk_bf16(in[8 * s + 4], in[8 * s + 5]), pk_bf16(in[8 * s + 6], in[8 * s + 7]));
 }
 
 // 8 accumulator values from o on, ReLU, as one B fragment
@@ -154,12 +146,11 @@ __device__ __forceinline__ void hidden_layer(const uint16_t *s_w, int l, int col
 }
 
 // the whole network for this lane's sample; y[0..2] valid on lanes 0..31
-__device__ __forceinline__ void forward_regs(const uint16_t *s_w, const NrcVertex &v, bool valid, const NrcBox &box,
-                                             float *y)
+__device__ __forceinline__ void forward_regs(const uint16_t *s_w, const NrcVertex &v, const NrcBox &box, float *y)
 {
     const int lane = threadIdx.x % WAVE, col = lane % 32, h = lane / 32;
     Frag x;
-    encode_frag(v, valid, box, h, x);
+    encode_frag(v, box, h, x);
 #pragma unroll
     for(int l = 0; l < NRC_LAYERS - 1; ++l)
         hidden_layer(s_w, l, col, h, x);
@@ -200,7 +191,7 @@ __global__ void __launch_bounds__(256) k_nrc_infer(NrcWeights net, NrcBox box, c
         if(valid)
             v = q[i].v;
         float y[3];
-        forward_regs(s_w, v, valid, box, y);
+        forward_regs(s_w, v, box, y);
         if(half == 0 && valid)
         {
             const NrcQuery &qi = q[i];
@@ -245,7 +236,7 @@ __global__ void __launch_bounds__(256) k_nrc_eval(NrcWeights net, NrcBox box, co
     if(valid)
         vi = v[i];
     float y[3];
-    forward_regs(s_w, vi, valid, box, y);
+    forward_regs(s_w, vi, box, y);
     if(half == 0 && valid)
     {
         out[3 * (size_t)i + 0] = y[0];
@@ -337,7 +328,7 @@ __global__ void __launch_bounds__(128) k_nrc_train_fb(NrcWeights net, NrcBox box
 
     // forward: X[l] is the input of layer l
     Frag X[NRC_LAYERS];
-    encode_frag(vtx, valid, box, h, X[0]);
+    encode_frag(vtx, box, h, X[0]);
     store_units(sc.xt, X[0], 4);
 #pragma unroll
     for(int l = 0; l < NRC_LAYERS - 1; ++l)
@@ -575,16 +566,24 @@ __global__ void __launch_bounds__(256) k_nrc_update(NrcWeights net, NrcAdam opt,
     net.wt_bf16[off + c * rows + r] = h;
 }
 
-// valid record slots in the order of the slot bijection, one workgroup of
-// 1024 threads, 64 consecutive bijection indices each: count, scan, write
-__global__ void __launch_bounds__(1024) k_nrc_compact(const NrcRecord *rec, int32_t *index, int32_t *total)
+// valid record slots in the order of the slot bijection, in two passes:
+// k_nrc_valid_mask gathers the valid flags of bijection indices 64 w..64 w+63
+// as one 64-bit ballot per wavefront; k_nrc_compact (one workgroup of 1024
+// threads, one mask word each) counts, scans and writes the slots in order
+__global__ void __launch_bounds__(64) k_nrc_valid_mask(const NrcRecord *rec, uint64_t *mask)
 {
-    constexpr int PER = (1 << 16) / 1024;
+    const int i = blockIdx.x * 64 + threadIdx.x;
+    const uint64_t b = __ballot(rec[nrc_record_slot(i)].valid != 0);
+    if(threadIdx.x == 0)
+        mask[blockIdx.x] = b;
+}
+
+__global__ void __launch_bounds__(1024) k_nrc_compact(const uint64_t *mask, int32_t *index, int32_t *total)
+{
     __shared__ int32_t s_sum[1024];
     const int t = threadIdx.x;
-    int32_t n = 0;
-    for(int j = 0; j < PER; ++j)
-        n += rec[nrc_record_slot(t * PER + j)].valid != 0 ? 1 : 0;
+    const uint64_t m = mask[t];
+    const int32_t n = (int32_t)__popcll(m);
     s_sum[t] = n;
     __syncthreads();
     // inclusive scan (Hillis-Steele)
@@ -596,12 +595,8 @@ __global__ void __launch_bounds__(1024) k_nrc_compact(const NrcRecord *rec, int3
         __syncthreads();
     }
     int32_t o = s_sum[t] - n;
-    for(int j = 0; j < PER; ++j)
-    {
-        const int32_t slot = (int32_t)nrc_record_slot(t * PER + j);
-        if(rec[slot].valid != 0)
-            index[o++] = slot;
-    }
+    for(uint64_t r = m; r != 0; r &= r - 1)
+        index[o++] = (int32_t)nrc_record_slot(64 * t + __ffsll((unsigned long long)r) - 1);
     if(t == 1023)
         *total = s_sum[1023];
 }
@@ -669,9 +664,10 @@ void launch_nrc_clear(hipStream_t s, NrcRecord *rec, int64_t nrec, float *acc, i
     hipLaunchKernelGGL(k_nrc_clear, dim3(256), dim3(256), 0, s, rec, nrec, acc, npaths);
 }
 
-void launch_nrc_compact(hipStream_t s, const NrcRecord *rec, int32_t *index, int32_t *total)
+void launch_nrc_compact(hipStream_t s, const NrcRecord *rec, uint64_t *mask, int32_t *index, int32_t *total)
 {
-    hipLaunchKernelGGL(k_nrc_compact, dim3(1), dim3(1024), 0, s, rec, index, total);
+    hipLaunchKernelGGL(k_nrc_valid_mask, dim3((1 << 16) / 64), dim3(64), 0, s, rec, mask);
+    hipLaunchKernelGGL(k_nrc_compact, dim3(1), dim3(1024), 0, s, mask, index, total);
 }
 
 void launch_nrc_publish(hipStream_t s, const NrcWeights &net)

@@ -138,8 +138,9 @@ void launch_nrc_gradients(hipStream_t s, const NrcWeights &net, const NrcBox &bo
                           const NrcScratch &sc, float *grad);
 void launch_nrc_clear(hipStream_t s, NrcRecord *rec, int64_t nrec, float *acc, int64_t npaths);
 // index = the valid record slots among rec[0, 2^16) in the order of
-// nrc_record_slot(0, 1, ...), *total = how many (one workgroup, deterministic)
-void launch_nrc_compact(hipStream_t s, const NrcRecord *rec, int32_t *index, int32_t *total);
+// nrc_record_slot(0, 1, ...), *total = how many (deterministic); mask: 1024
+// words of scratch
+void launch_nrc_compact(hipStream_t s, const NrcRecord *rec, uint64_t *mask, int32_t *index, int32_t *total);
 void launch_nrc_publish(hipStream_t s, const NrcWeights &net);
 
 } // namespace btrc

@@ -363,7 +363,11 @@ typedef struct btrc_tuning
     int32_t launch_graph;            /* 1: the bounce loop replays a captured HIP graph (default 1) */
     int32_t two_level_postpone;      /* two-level scenes: bit 0 closest-hit, bit 1 visibility rays
                                         postpone leaves (default 0: both test leaves when reached) */
-    int32_t reserved[3];
+    int32_t stage_blocks_per_cu;     /* grid of the 256-lane stage kernels (generate, medium, shade): at most
+                                        this many workgroups per CU, 1..64 (0: 16) */
+    int32_t graph_bounces;           /* bounces per captured graph block between counter read-backs,
+                                        even, 2..16 (0: 4) */
+    int32_t reserved;
 } btrc_tuning;
 
 /* ---- context ---- */

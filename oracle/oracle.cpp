@@ -2659,37 +2659,44 @@ static double bf16(double x)
 static int rows(int l) { return l == L - 1 ? OUT_ROWS : W; }
 static int offset(int l) { return l * W * IN; }
 
-// input encoding: 8 frequencies of the box-normalised position (sin and
-// cos of pi q directly, each doubled frequency by the double-angle
-// identities in FP32), direction, normal, roughness, diffuse, specular, then
-// constant ones up to 64
+// input encoding: input k = 16 s + 8 h + j (s slice, h half, j 0..7) is
+// element e = 8 s + j of half h: for e < 24 sin (e even) or cos (e odd) of
+// 2^f pi x_a with f = 4 h + e / 6, a = (e % 6) / 2 (each half's chain starts
+// at 2^(4 h) pi x_a and doubles by the double-angle identities in FP32);
+// then half 0: direction, normal, roughness, diffuse red; half 1: diffuse
+// green, blue, specular, three constant ones
 static void encode(const float *v, const float *box, double *x)
 {
-    int k = 0;
-    float q[3], sn[3], cs[3];
-    for(int a = 0; a < 3; ++a)
+    for(int h = 0; h < 2; ++h)
     {
-        q[a] = (v[a] - box[a]) * box[3 + a];
-        const float t = PI * q[a];
-        sn[a] = dm::sin(t);
-        cs[a] = dm::cos(t);
-    }
-    for(int f = 0; f < 8; ++f)
-    {
+        float e[32];
+        const float base = h ? 16.0f * PI : PI;
+        float sn[3], cs[3];
         for(int a = 0; a < 3; ++a)
         {
-            x[k++] = sn[a];
-            x[k++] = cs[a];
-            const float s2 = 2.0f * sn[a] * cs[a];
-            const float c2 = (cs[a] - sn[a]) * (cs[a] + sn[a]);
-            sn[a] = s2;
-            cs[a] = c2;
+            const float q = (v[a] - box[a]) * box[3 + a];
+            dm::sincos(base * q, &sn[a], &cs[a]);
         }
+        int k = 0;
+        for(int f = 0; f < 4; ++f)
+            for(int a = 0; a < 3; ++a)
+            {
+                e[k++] = sn[a];
+                e[k++] = cs[a];
+                const float s2 = 2.0f * sn[a] * cs[a];
+                const float c2 = (cs[a] - sn[a]) * (cs[a] + sn[a]);
+                sn[a] = s2;
+                cs[a] = c2;
+            }
+        // v = p[3], wo[3], n[3], roughness, diffuse[3], specular[3]
+        const float t0[8] = { v[3], v[4], v[5], v[6], v[7], v[8], v[9], v[10] };
+        const float t1[8] = { v[11], v[12], v[13], v[14], v[15], 1.0f, 1.0f, 1.0f };
+        for(int j = 0; j < 8; ++j)
+            e[k++] = h ? t1[j] : t0[j];
+        for(int s = 0; s < 4; ++s)
+            for(int j = 0; j < 8; ++j)
+                x[16 * s + 8 * h + j] = e[8 * s + j];
     }
-    for(int j = 3; j < 16; ++j)
-        x[k++] = v[j];
-    while(k < IN)
-        x[k++] = 1.0;
 }
 
 struct Act

@@ -80,6 +80,28 @@ def test_graph_loop_equals_direct_launches(ctx):
     ctx.set_tuning(launch_graph=1)
 
 
+def test_stage_grid_and_graph_block_keep_results(ctx):
+    """the stage kernels' grid and the bounces per captured block change
+    speed only"""
+    scene, cam = scenes.sphere_field(instances=128, lights=8, seed=7, extent=20.0)
+    ctx.upload_scene(scene)
+    ctx.set_camera(cam, 64, 48)
+    ref = None
+    for blocks, bounces in ((0, 0), (1, 2), (64, 16), (3, 6)):
+        ctx.set_tuning(stage_blocks_per_cu=blocks, graph_bounces=bounces, launch_graph=1)
+        st = ctx.render(render_params(64, 48, 2, state_count=2500, film_fp64=1))
+        got = (ctx.film(64, 48, fp64=True)[1], st.extension_rays, st.shadow_rays)
+        if ref is None:
+            ref = got
+        assert got[1:] == ref[1:]
+        assert np.abs(got[0] - ref[0]).max() <= 1e-12 * max(1.0, np.abs(ref[0]).max())
+    import btrc_amd
+    for bad in (dict(graph_bounces=3), dict(graph_bounces=18), dict(stage_blocks_per_cu=65)):
+        with pytest.raises(btrc_amd.BtrcException):
+            ctx.set_tuning(**bad)
+    ctx.set_tuning(stage_blocks_per_cu=0, graph_bounces=0)
+
+
 def test_coplanar_hits_every_schedule(ctx):
     # the floor and coplanar tiles: box culling must keep equal-distance
     # candidates whatever the visit order, in both structures
@@ -111,12 +133,23 @@ def test_autotune_picks_a_candidate_and_keeps_results(ctx, two_level):
     ctx.set_camera(cam, p.width, p.height)
     before = ctx.render(p)
     img0, _ = ctx.film(p.width, p.height)
-    best, table = autotune(ctx, p, spp=1, waves=(16, 20), two_level=two_level)
-    assert len(table) == (8 if two_level else 8)
-    assert all(ms > 0 for _, ms, _ in table)
+    best, pool, table = autotune(ctx, p, spp=1, waves=(16, 20), two_level=two_level, stage_blocks=(8, 16),
+                                 graph_bounces=(2, 4), pools=(1 << 13, 1 << 14))
+    assert len(table) == 8 + 2 + 4
+    assert all(ms > 0 for _, _, ms in table)
+    assert pool in (1 << 13, 1 << 14)
     t = ctx.get_tuning()
     for k, v in best.items():
         assert t[k] == v
+    assert best["stage_blocks_per_cu"] in (8, 16) and best["graph_bounces"] in (2, 4)
+    after = ctx.render(p)
+    img1, _ = ctx.film(p.width, p.height)
+    assert (before.extension_rays, before.shadow_rays) == (after.extension_rays, after.shadow_rays)
+    # the pool size the tuner picked renders the same paths too
+    q = render_params(128, 96, 4, state_count=pool)
+    ctx.render(q)
+    img2, _ = ctx.film(p.width, p.height)
+    assert np.abs(img0 - img2).max() / max(1.0, np.abs(img0).max()) <= 1e-5
     after = ctx.render(p)
     img1, _ = ctx.film(p.width, p.height)
     assert (before.extension_rays, before.shadow_rays) == (after.extension_rays, after.shadow_rays)

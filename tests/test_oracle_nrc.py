@@ -95,16 +95,25 @@ def test_bf16_rounding_is_a_small_perturbation():
     assert rel.max() < 0.1 and rel.mean() < 0.02
 
 
+def frequency_input(f, a, c):
+    """the input index of sin (c = 0) or cos (c = 1) of 2^f pi q_a: element
+    e = 6 (f % 4) + 2 a + c of half f // 4, which is input 16 (e // 8) +
+    8 half + e % 8"""
+    h, e = f // 4, 6 * (f % 4) + 2 * a + c
+    return 16 * (e // 8) + 8 * h + e % 8
+
+
 def test_frequency_encoding_order_and_values():
-    """input 6 f + 2 a + c is sin (c = 0) or cos (c = 1) of 2^f pi q_a, q the
-    box-normalised position: read through two ReLU units (+x and -x)"""
+    """each frequency input is sin or cos of 2^f pi q_a, q the box-normalised
+    position, read through two ReLU units (+x and -x); the other inputs are
+    the vertex's features and three constant ones"""
     rng = np.random.default_rng(4)
     v = random_vertices(64, rng)
     q = ((v[:, :3] - BOX[:3]) * BOX[3:]).astype(np.float64)
     for f in range(8):
         for a in range(3):
             for c in range(2):
-                k = 6 * f + 2 * a + c
+                k = frequency_input(f, a, c)
                 w = np.zeros(P, dtype=np.float32)
                 layer(w, 0)[0, k], layer(w, 0)[1, k] = 1.0, -1.0
                 for l in range(1, 5):
@@ -113,3 +122,23 @@ def test_frequency_encoding_order_and_values():
                 got = oracle.nrc_forward(w, BOX, v, bf16=False)[:, 0]
                 want = (np.cos if c else np.sin)(2.0 ** f * np.pi * q[:, a])
                 assert np.abs(got - want).max() < 2e-4, (f, a, c, np.abs(got - want).max())
+
+
+def test_feature_inputs():
+    """inputs 48..55: direction, normal, roughness, diffuse red; 56..60:
+    diffuse green, blue, specular; 61..63: constant ones"""
+    rng = np.random.default_rng(5)
+    v = random_vertices(16, rng)
+    want = {48: v[:, 3], 49: v[:, 4], 50: v[:, 5], 51: v[:, 6], 52: v[:, 7], 53: v[:, 8], 54: v[:, 9],
+            55: v[:, 10], 56: v[:, 11], 57: v[:, 12], 58: v[:, 13], 59: v[:, 14], 60: v[:, 15],
+            61: 1.0, 62: 1.0, 63: 1.0}
+    freq = {frequency_input(f, a, c) for f in range(8) for a in range(3) for c in range(2)}
+    assert freq == set(range(48))
+    for k, x in want.items():
+        w = np.zeros(P, dtype=np.float32)
+        layer(w, 0)[0, k], layer(w, 0)[1, k] = 1.0, -1.0
+        for l in range(1, 5):
+            layer(w, l)[0, 0] = layer(w, l)[1, 1] = 1.0
+        layer(w, 5)[0, 0], layer(w, 5)[0, 1] = 1.0, -1.0
+        got = oracle.nrc_forward(w, BOX, v, bf16=False)[:, 0]
+        assert np.allclose(got, x, atol=1e-7), k
