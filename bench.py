@@ -186,6 +186,8 @@ def main():
     ap.add_argument("--nrc-query-depth", type=int, default=2, help="nrc workload: depth of the cache query")
     ap.add_argument("--nrc-train-stride", type=int, default=16,
                     help="nrc workload: one pixel in this many traces a training path")
+    ap.add_argument("--nrc-ema-decay", type=float, default=0.0,
+                    help="radiance cache: frames query a moving average of the weights with this decay (0: off)")
     ap.add_argument("--nrc-seeds", default="1,2,3,4",
                     help="network initialisation seeds of the radiance-cache runs (each trained from scratch)")
     ap.add_argument("--nrc-gt-spp", type=int, default=65536,
@@ -327,7 +329,7 @@ def main():
         rays_per_launch = staged["ext"] / launches
         launch_s = trace_ms * 1e-3 / launches
         achieved = bytes_per_ray * rays_per_launch / launch_s / 1e9
-        traffic_per_ray = read_traffic(os.path.join(ROOT, "profiles", "trace_extension_traffic.md"))
+        traffic_per_ray = read_traffic(os.path.join(ROOT, "profiles", "trace_extension_sphere_field_1080p_64spp.md"))
         roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                         frac=round(achieved / HBM_PEAK_GBS, 4),
                         traffic=None if traffic_per_ray is None else round(traffic_per_ray * rays_per_launch),
@@ -463,7 +465,8 @@ def nrc_equal_time(args, train_depths):
 
     def configure(cache, train_depth, seed=1):
         ctx.nrc_configure(enabled=1 if cache else 0, train=1, train_depth=train_depth, steps=args.nrc_steps,
-                          train_stride=args.nrc_train_stride, query_depth=args.nrc_query_depth)
+                          train_stride=args.nrc_train_stride, query_depth=args.nrc_query_depth,
+                          ema_decay=args.nrc_ema_decay)
         if cache:
             ctx.nrc_reset(seed)
 

@@ -191,7 +191,7 @@ class Context:
 
     # ---- neural radiance cache
     NRC_DEFAULTS = dict(enabled=0, train=1, query_depth=2, train_stride=16, steps=4, learning_rate=1e-2,
-                        loss_scale=128.0, train_depth=0)
+                        loss_scale=128.0, train_depth=0, ema_decay=0.0)
 
     def nrc_configure(self, **kw):
         unknown = set(kw) - set(self.NRC_DEFAULTS)
@@ -217,6 +217,11 @@ class Context:
     def nrc_weights(self):
         w = np.empty(_abi.NRC_PARAMS, dtype=np.float32)
         self._check(self.lib.btrc_hip_nrc_get_weights(self.h, w.ctypes.data))
+        return w
+
+    def nrc_ema_weights(self):
+        w = np.empty(_abi.NRC_PARAMS, dtype=np.float32)
+        self._check(self.lib.btrc_hip_nrc_get_ema_weights(self.h, w.ctypes.data))
         return w
 
     def nrc_set_weights(self, w):

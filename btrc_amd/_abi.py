@@ -125,7 +125,7 @@ class RenderParams(C.Structure):
 class NrcParams(C.Structure):
     _fields_ = [("enabled", C.c_int32), ("train", C.c_int32), ("query_depth", C.c_int32),
                 ("train_stride", C.c_int32), ("steps", C.c_int32), ("learning_rate", C.c_float),
-                ("loss_scale", C.c_float), ("train_depth", C.c_int32)]
+                ("loss_scale", C.c_float), ("train_depth", C.c_int32), ("ema_decay", C.c_float)]
 
 
 class NrcStats(C.Structure):
@@ -205,7 +205,7 @@ EXPORTS = [
     "btrc_hip_aov_readback", "btrc_hip_tonemap_readback", "btrc_hip_request_stop",
     "btrc_hip_set_reporter",
     "btrc_hip_nrc_configure", "btrc_hip_nrc_reset", "btrc_hip_nrc_stats", "btrc_hip_nrc_get_weights",
-    "btrc_hip_nrc_set_weights", "btrc_hip_nrc_eval", "btrc_hip_nrc_gradients", "btrc_hip_nrc_fit",
+    "btrc_hip_nrc_set_weights", "btrc_hip_nrc_get_ema_weights", "btrc_hip_nrc_eval", "btrc_hip_nrc_gradients", "btrc_hip_nrc_fit",
     "btrc_hip_trace_closest", "btrc_hip_trace_any", "btrc_hip_trace_closest_noalloc",
     "btrc_hip_batch_upload", "btrc_hip_batch_trace", "btrc_hip_batch_download",
     "btrc_hip_capture_rays", "btrc_hip_build_bvh", "btrc_hip_eval_halton", "btrc_hip_eval_independent", "btrc_hip_eval_volumes", "btrc_hip_eval_texture",
@@ -245,6 +245,7 @@ def declare(lib):
         "btrc_hip_nrc_reset": (C.c_int, [ctx, C.c_uint64]),
         "btrc_hip_nrc_stats": (C.c_int, [ctx, P(NrcStats)]),
         "btrc_hip_nrc_get_weights": (C.c_int, [ctx, C.c_void_p]),
+        "btrc_hip_nrc_get_ema_weights": (C.c_int, [ctx, C.c_void_p]),
         "btrc_hip_nrc_set_weights": (C.c_int, [ctx, C.c_void_p]),
         "btrc_hip_nrc_eval": (C.c_int, [ctx, C.c_void_p, C.c_int32, C.c_void_p]),
         "btrc_hip_nrc_gradients": (C.c_int, [ctx, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, P(C.c_double)]),

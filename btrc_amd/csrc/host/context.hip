@@ -314,10 +314,12 @@ struct btrc_ctx
     // neural radiance cache
     struct Nrc
     {
-        btrc_nrc_params params{ 0, 1, 2, 16, 4, 1e-2f, 128.0f, 0 };
+        btrc_nrc_params params{ 0, 1, 2, 16, 4, 1e-2f, 128.0f, 0, 0.0f };
         bool ready = false;             // weights allocated and initialised
         DevBuf<float> w, m, v, grad;
         DevBuf<uint16_t> wb, wtb;
+        DevBuf<float> ema;              // moving average of the weights, and its BF16 copy
+        DevBuf<uint16_t> ema_b;
         DevBuf<NrcQuery> queries;
         DevBuf<NrcRecord> rec;
         DevBuf<float> acc;
@@ -1512,7 +1514,16 @@ constexpr int64_t NRC_BATCH = 1 << 14;       // vertices per optimizer step
 NrcWeights nrc_weights(btrc_ctx *ctx)
 {
     auto &n = ctx->nrc;
-    return NrcWeights{ n.w.ptr, n.m.ptr, n.v.ptr, n.wb.ptr, n.wtb.ptr };
+    return NrcWeights{ n.w.ptr, n.m.ptr, n.v.ptr, n.wb.ptr, n.wtb.ptr, n.ema.ptr, n.ema_b.ptr };
+}
+
+// the weights rendered frames query: the trained ones, or their moving average
+NrcWeights nrc_render_weights(btrc_ctx *ctx)
+{
+    NrcWeights w = nrc_weights(ctx);
+    if(ctx->nrc.params.ema_decay > 0.0f)
+        w.w_bf16 = ctx->nrc.ema_b.ptr;
+    return w;
 }
 
 NrcBox nrc_box(btrc_ctx *ctx)
@@ -1535,6 +1546,8 @@ void nrc_alloc(btrc_ctx *ctx)
     n.grad.alloc(NRC_PARAMS);
     n.wb.alloc(NRC_PARAMS);
     n.wtb.alloc(NRC_PARAMS);
+    n.ema.alloc(NRC_PARAMS);
+    n.ema_b.alloc(NRC_PARAMS);
     n.rec.alloc(NRC_RECORDS);
     n.acc.alloc(4 * NRC_RECORDS);
     n.counts.alloc(3);
@@ -1598,6 +1611,7 @@ NrcAdam nrc_adam(btrc_ctx *ctx)
     a.inv_bias1 = (float)(1.0 / (1.0 - std::pow(0.9, t)));
     a.inv_bias2 = (float)(1.0 / (1.0 - std::pow(0.99, t)));
     a.loss_scale = n.params.loss_scale;
+    a.ema_decay = n.params.ema_decay;
     return a;
 }
 
@@ -2009,10 +2023,11 @@ void render(btrc_ctx *ctx, const btrc_render_params *p, int32_t view_count, cons
         // the cache answers the queries of this frame, then learns from its
         // training paths
         auto &n = ctx->nrc;
-        const int igrid = (int)std::max<int64_t>(1, std::min<int64_t>(2 * ctx->cu_count, ((int64_t)pixels + 127) / 128));
+        // the cache renders one sample per pixel and a path makes at most
+        // one query: no two queries share a pixel
         timer.run(STAGE_CACHE, [&] {
-            launch_nrc_infer(s, igrid, nrc_weights(ctx), nrc_box(ctx), n.queries.ptr, n.counts.ptr, ctx->film.ptr,
-                             p->film_fp64 ? ctx->film64.ptr : nullptr, n.acc.ptr);
+            launch_nrc_infer(s, ctx->cu_count, (int64_t)pixels, nrc_render_weights(ctx), nrc_box(ctx), n.queries.ptr,
+                             n.counts.ptr, ctx->film.ptr, p->film_fp64 ? ctx->film64.ptr : nullptr, n.acc.ptr);
         });
         if(n.params.train)
         {
@@ -2436,7 +2451,8 @@ int btrc_hip_nrc_configure(btrc_ctx *ctx, const btrc_nrc_params *params)
         const btrc_nrc_params &q = *params;
         if(q.query_depth < 0 || q.train_stride < 1 || q.steps < 0 || q.steps > 64 || !(q.learning_rate > 0.0f) ||
            !(q.loss_scale > 0.0f) || !std::isfinite(q.learning_rate) || !std::isfinite(q.loss_scale) ||
-           q.train_depth < 0 || (q.train_depth > 0 && q.train_depth <= q.query_depth))
+           q.train_depth < 0 || (q.train_depth > 0 && q.train_depth <= q.query_depth) || !(q.ema_decay >= 0.0f) ||
+           !(q.ema_decay < 1.0f))
             throw InvalidArg("bad cache parameters");
         BTRC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
         ctx->nrc.params = q;
@@ -2466,6 +2482,18 @@ int btrc_hip_nrc_get_weights(btrc_ctx *ctx, float *weights)
             throw InvalidArg("weights is null");
         nrc_require(ctx);
         BTRC_HIP_CHECK(hipMemcpyAsync(weights, ctx->nrc.w.ptr, NRC_PARAMS * sizeof(float), hipMemcpyDeviceToHost,
+                                      ctx->stream));
+        BTRC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int btrc_hip_nrc_get_ema_weights(btrc_ctx *ctx, float *weights)
+{
+    return guarded(ctx, [&] {
+        if(!weights)
+            throw InvalidArg("weights is null");
+        nrc_require(ctx);
+        BTRC_HIP_CHECK(hipMemcpyAsync(weights, ctx->nrc.ema.ptr, NRC_PARAMS * sizeof(float), hipMemcpyDeviceToHost,
                                       ctx->stream));
         BTRC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
     });

@@ -24,6 +24,8 @@
 // input gradient is the next product's B operand too.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "../core/nrc.h"
 #include "nrc_kernels.h"
 
@@ -127,98 +129,177 @@ __device__ __forceinline__ u16x8 relu_frag(const f32x16 &a, int o)
                   relu2(pk_bf16(a[o + 4], a[o + 5])), relu2(pk_bf16(a[o + 6], a[o + 7])));
 }
 
-// hidden layer l (64 x 64, ReLU) for the wavefront's 32 samples, in registers
-__device__ __forceinline__ void hidden_layer(const uint16_t *s_w, int l, int col, int h, Frag &x)
+// hidden layer l (64 x 64, ReLU) for T tiles of 32 samples, in registers:
+// each weight fragment read from LDS feeds T products
+template <int T>
+__device__ __forceinline__ void hidden_layer_n(const uint16_t *s_w, int l, int col, int h, Frag *x)
 {
+    // keeps the scheduler from hoisting every layer's weight loads to the
+    // top (their registers would lower occupancy)
+    __builtin_amdgcn_sched_barrier(0);
     const uint16_t *r0 = s_w + (l * NRC_WIDTH + swap23(col)) * WS + 8 * h;
     const uint16_t *r1 = r0 + 32 * WS;
-    f32x16 acc0 = {}, acc1 = {};
+    f32x16 acc0[T], acc1[T];
+#pragma unroll
+    for(int t = 0; t < T; ++t)
+        acc0[t] = acc1[t] = f32x16{};
 #pragma unroll
     for(int s = 0; s < 4; ++s)
     {
-        acc0 = mfma(lds8(r0 + 16 * s), x.s[s], acc0);
-        acc1 = mfma(lds8(r1 + 16 * s), x.s[s], acc1);
+        const u16x8 a0 = lds8(r0 + 16 * s), a1 = lds8(r1 + 16 * s);
+#pragma unroll
+        for(int t = 0; t < T; ++t)
+        {
+            acc0[t] = mfma(a0, x[t].s[s], acc0[t]);
+            acc1[t] = mfma(a1, x[t].s[s], acc1[t]);
+        }
     }
-    x.s[0] = relu_frag(acc0, 0);
-    x.s[1] = relu_frag(acc0, 8);
-    x.s[2] = relu_frag(acc1, 0);
-    x.s[3] = relu_frag(acc1, 8);
+#pragma unroll
+    for(int t = 0; t < T; ++t)
+    {
+        x[t].s[0] = relu_frag(acc0[t], 0);
+        x[t].s[1] = relu_frag(acc0[t], 8);
+        x[t].s[2] = relu_frag(acc1[t], 0);
+        x[t].s[3] = relu_frag(acc1[t], 8);
+    }
 }
 
-// the whole network for this lane's sample; y[0..2] valid on lanes 0..31
-__device__ __forceinline__ void forward_regs(const uint16_t *s_w, const NrcVertex &v, const NrcBox &box, float *y)
+__device__ __forceinline__ void hidden_layer(const uint16_t *s_w, int l, int col, int h, Frag &x)
+{
+    hidden_layer_n<1>(s_w, l, col, h, &x);
+}
+
+// the whole network for T tiles: this lane's sample of tile t is v[t];
+// y[t][0..2] valid on lanes 0..31
+template <int T>
+__device__ __forceinline__ void forward_regs_n(const uint16_t *s_w, const NrcVertex *v, const NrcBox &box,
+                                               float (*y)[3])
 {
     const int lane = threadIdx.x % WAVE, col = lane % 32, h = lane / 32;
-    Frag x;
-    encode_frag(v, box, h, x);
+    Frag x[T];
+#pragma unroll
+    for(int t = 0; t < T; ++t)
+        encode_frag(v[t], box, h, x[t]);
 #pragma unroll
     for(int l = 0; l < NRC_LAYERS - 1; ++l)
-        hidden_layer(s_w, l, col, h, x);
+        hidden_layer_n<T>(s_w, l, col, h, x);
     // output layer: natural row order, rows 3..31 are zero
     const uint16_t *r = s_w + ((NRC_LAYERS - 1) * NRC_WIDTH + col) * WS + 8 * h;
-    f32x16 acc = {};
+    f32x16 acc[T];
+#pragma unroll
+    for(int t = 0; t < T; ++t)
+        acc[t] = f32x16{};
 #pragma unroll
     for(int s = 0; s < 4; ++s)
-        acc = mfma(lds8(r + 16 * s), x.s[s], acc);
-    y[0] = acc[0];
-    y[1] = acc[1];
-    y[2] = acc[2];
+    {
+        const u16x8 a = lds8(r + 16 * s);
+#pragma unroll
+        for(int t = 0; t < T; ++t)
+            acc[t] = mfma(a, x[t].s[s], acc[t]);
+    }
+#pragma unroll
+    for(int t = 0; t < T; ++t)
+    {
+        y[t][0] = acc[t][0];
+        y[t][1] = acc[t][1];
+        y[t][2] = acc[t][2];
+    }
+}
+
+__device__ __forceinline__ void forward_regs(const uint16_t *s_w, const NrcVertex &v, const NrcBox &box, float *y)
+{
+    forward_regs_n<1>(s_w, &v, box, reinterpret_cast<float(*)[3]>(y));
+}
+
+// a query's answer: y = max(net(x), 0); the film gets beta * y at the pixel,
+// and a training path's radiance sum too when the query is its tail. No two
+// queries of a launch share a pixel (the cache renders one path per pixel,
+// and a path makes at most one query), so both are plain read-modify-writes.
+__device__ __forceinline__ void emit_answer(const NrcQuery &qi, const float *y, float *film, double *film64,
+                                            float *acc)
+{
+    const float r[3] = { qi.beta[0] * fmaxf(y[0], 0.0f), qi.beta[1] * fmaxf(y[1], 0.0f),
+                         qi.beta[2] * fmaxf(y[2], 0.0f) };
+    if(!(isfinite(r[0]) && isfinite(r[1]) && isfinite(r[2])))
+        return;
+    float4 *p = reinterpret_cast<float4 *>(film) + qi.pixel;
+    float4 f = *p;
+    f.x += r[0];
+    f.y += r[1];
+    f.z += r[2];
+    *p = f;
+    if(film64)
+    {
+        double *d = film64 + 3 * (size_t)qi.pixel;
+        d[0] += (double)r[0];
+        d[1] += (double)r[1];
+        d[2] += (double)r[2];
+    }
+    if(qi.train != NO_TRAIN)
+    {
+        float *a = acc + 4 * (size_t)qi.train;
+        a[0] = a[0] + r[0];
+        a[1] = a[1] + r[1];
+        a[2] = a[2] + r[2];
+    }
 }
 
 } // namespace
 
 // ---------------------------------------------------------------- inference
 
+// inference launch shape: one tile of 32 queries per wavefront at a time
+// (two tiles sharing each weight fragment measured no faster: 170 VGPRs
+// halve the resident waves), four wavefronts per workgroup, three workgroups
+// per CU (three LDS copies of the weights fit)
+constexpr int INFER_TILES = 1;
+constexpr int INFER_THREADS = 256;
+constexpr int INFER_BLOCKS_PER_CU = 3;
+
 // queries [0, n): y = max(net(x), 0); the film gets beta * y at the pixel,
 // and a training path's radiance sum too when the query is its tail (one
 // query per training path, after every other addition to its sum).
-// Persistent: each wavefront takes tiles of 32 queries; activations stay in
-// registers, the weights in LDS.
-__global__ void __launch_bounds__(256) k_nrc_infer(NrcWeights net, NrcBox box, const NrcQuery *q, const int32_t *count,
-                                                   float *film, double *film64, float *acc)
+// Persistent: each wavefront takes INFER_TILES tiles of 32 queries at a
+// time; activations stay in registers, the weights in LDS.
+__global__ void __launch_bounds__(INFER_THREADS) k_nrc_infer(NrcWeights net, NrcBox box, const NrcQuery *q,
+                                                             const int32_t *count, float *film, double *film64,
+                                                             float *acc)
 {
+    constexpr int T = INFER_TILES, WAVES = INFER_THREADS / WAVE, SPAN = T * TILE;
     __shared__ __attribute__((aligned(16))) uint16_t s_w[NRC_ROWS * WS];
     load_weights_padded(net.w_bf16, s_w);
     __syncthreads();
     const int n = *count;
     const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE, col = lane % 32, half = lane / 32;
-    const int waves_total = gridDim.x * 4;
-    for(int base = (blockIdx.x * 4 + wave) * TILE; base < n; base += waves_total * TILE)
+    const int stride = gridDim.x * WAVES * SPAN;
+    // the next span's queries are loaded while the current ones are computed
+    int base = (blockIdx.x * WAVES + wave) * SPAN;
+    NrcQuery next[T];
+#pragma unroll
+    for(int t = 0; t < T; ++t)
     {
-        const int i = base + col;
-        const bool valid = i < n;
-        NrcVertex v{};
-        if(valid)
-            v = q[i].v;
-        float y[3];
-        forward_regs(s_w, v, box, y);
-        if(half == 0 && valid)
+        next[t] = NrcQuery{};
+        if(base + t * TILE + col < n)
+            next[t] = q[base + t * TILE + col];
+    }
+    for(; base < n; base += stride)
+    {
+        NrcQuery qi[T];
+        NrcVertex v[T];
+#pragma unroll
+        for(int t = 0; t < T; ++t)
         {
-            const NrcQuery &qi = q[i];
-            const float r[3] = { qi.beta[0] * fmaxf(y[0], 0.0f), qi.beta[1] * fmaxf(y[1], 0.0f),
-                                 qi.beta[2] * fmaxf(y[2], 0.0f) };
-            if(isfinite(r[0]) && isfinite(r[1]) && isfinite(r[2]))
-            {
-                float *p = film + 4 * (size_t)qi.pixel;
-                atomicAdd(p + 0, r[0]);
-                atomicAdd(p + 1, r[1]);
-                atomicAdd(p + 2, r[2]);
-                if(film64)
-                {
-                    double *d = film64 + 3 * (size_t)qi.pixel;
-                    atomicAdd(d + 0, (double)r[0]);
-                    atomicAdd(d + 1, (double)r[1]);
-                    atomicAdd(d + 2, (double)r[2]);
-                }
-                if(qi.train != NO_TRAIN)
-                {
-                    float *a = acc + 4 * (size_t)qi.train;
-                    a[0] = a[0] + r[0];
-                    a[1] = a[1] + r[1];
-                    a[2] = a[2] + r[2];
-                }
-            }
+            qi[t] = next[t];
+            v[t] = qi[t].v;
+            if(base + stride + t * TILE + col < n)
+                next[t] = q[base + stride + t * TILE + col];
         }
+        float y[T][3];
+        forward_regs_n<T>(s_w, v, box, y);
+#pragma unroll
+        for(int t = 0; t < T; ++t)
+            if(half == 0 && base + t * TILE + col < n)
+                emit_answer(qi[t], y[t], film, film64, acc);
     }
 }
 
@@ -260,6 +341,18 @@ __global__ void __launch_bounds__(128) k_nrc_train_fb(NrcWeights net, NrcBox box
 {
     __shared__ __attribute__((aligned(16))) uint16_t s_w[NRC_ROWS * WS];
     __shared__ __attribute__((aligned(16))) uint16_t s_wt[NRC_LAYERS * NRC_IN * WS];
+    int64_t first, count;
+    nrc_batch_range(b, &first, &count);
+    if((int64_t)blockIdx.x * 2 * TILE >= count)
+    {
+        // the whole workgroup lies beyond the step's samples: no weights to stage
+        if(threadIdx.x % WAVE == 0)
+        {
+            sc.loss_partial[blockIdx.x * 2 + threadIdx.x / WAVE] = 0.0;
+            sc.count_partial[blockIdx.x * 2 + threadIdx.x / WAVE] = 0;
+        }
+        return;
+    }
     load_weights_padded(net.w_bf16, s_w);
     // [in][out] copies, per layer 64 rows of `rows` values, padded to WS
     for(int l = 0; l < NRC_LAYERS; ++l)
@@ -277,8 +370,6 @@ __global__ void __launch_bounds__(128) k_nrc_train_fb(NrcWeights net, NrcBox box
 
     const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE, col = lane % 32, h = lane / 32;
     const int64_t S = b.batch;
-    int64_t first, count;
-    nrc_batch_range(b, &first, &count);
     const int64_t sample0 = (int64_t)(blockIdx.x * 2 + wave) * TILE;
     if(sample0 >= count)
     {
@@ -491,12 +582,15 @@ __global__ void __launch_bounds__(256) k_nrc_reduce(const float *partial, float 
         g2_partial[blockIdx.x] = s_g2[0];
 }
 
-// Reductions of a step (one workgroup, fixed order, FP64): valid samples n,
-// loss = sum / (3 n), and the norm of the gradient of the mean loss. Sets
-// the scale that turns the summed, loss-scaled gradient into the mean's
-// gradient, or 0 (no update) for a non-finite gradient or an empty batch.
-__global__ void __launch_bounds__(256) k_nrc_finalize(const float *grad, NrcScratch sc, int32_t partials,
-                                                      float loss_scale, int32_t *valid_count, NrcStepStats *stats)
+// Adam (Kingma and Ba 2015) on the FP32 master weights, one weight per
+// thread, then its BF16 copies ([out][in] and per layer [in][out]). Every
+// workgroup first forms the step's reductions itself (fixed order, FP64):
+// valid samples n, loss = sum / (3 n), and the norm of the gradient of the
+// mean loss, which give the scale that turns the summed, loss-scaled
+// gradient into the mean's gradient, or 0 (no update) for a non-finite
+// gradient or an empty batch; workgroup 0 records them.
+__global__ void __launch_bounds__(256) k_nrc_update(NrcWeights net, NrcAdam opt, const float *grad, NrcScratch sc,
+                                                    int32_t partials, int32_t *valid_count, NrcStepStats *stats)
 {
     __shared__ double s_a[256], s_b[256];
     __shared__ long long s_c[256];
@@ -525,27 +619,21 @@ __global__ void __launch_bounds__(256) k_nrc_finalize(const float *grad, NrcScra
         }
         __syncthreads();
     }
-    if(tid == 0)
+    const long long n = s_c[0];
+    const double dscale = n > 0 ? 1.0 / (3.0 * (double)n * (double)opt.loss_scale) : 0.0;
+    const double gnorm = sqrt(s_b[0]) * dscale;
+    const bool step = n > 0 && isfinite(gnorm);
+    const float scale = step ? (float)dscale : 0.0f;
+    if(blockIdx.x == 0 && tid == 0)
     {
-        const long long n = s_c[0];
-        const double scale = n > 0 ? 1.0 / (3.0 * (double)n * (double)loss_scale) : 0.0;
-        const double gnorm = sqrt(s_b[0]) * scale;
-        const bool step = n > 0 && isfinite(gnorm);
-        *sc.step_scale = step ? (float)scale : 0.0f;
+        *sc.step_scale = scale;
         *valid_count = (int32_t)n;
         stats->loss = n > 0 ? s_a[0] / (3.0 * (double)n) : 0.0;
         stats->grad_norm = gnorm;
         stats->samples = (int32_t)n;
         stats->skipped = step ? 0 : 1;
     }
-}
-
-// Adam (Kingma and Ba 2015) on the FP32 master weights, one weight per
-// thread, then its BF16 copies ([out][in] and per layer [in][out])
-__global__ void __launch_bounds__(256) k_nrc_update(NrcWeights net, NrcAdam opt, const float *grad, NrcScratch sc)
-{
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    const float scale = *sc.step_scale;
     if(k >= NRC_PARAMS || scale == 0.0f)
         return;
     const float g = grad[k] * scale;
@@ -564,6 +652,12 @@ __global__ void __launch_bounds__(256) k_nrc_update(NrcWeights net, NrcAdam opt,
     const uint16_t h = to_bf16(w);
     net.w_bf16[k] = h;
     net.wt_bf16[off + c * rows + r] = h;
+    if(opt.ema_decay > 0.0f)
+    {
+        const float e = opt.ema_decay * net.ema[k] + (1.0f - opt.ema_decay) * w;
+        net.ema[k] = e;
+        net.ema_bf16[k] = to_bf16(e);
+    }
 }
 
 // valid record slots in the order of the slot bijection, in two passes:
@@ -623,15 +717,24 @@ __global__ void k_nrc_publish(NrcWeights net)
         const uint16_t h = to_bf16(net.w[k]);
         net.w_bf16[k] = h;
         net.wt_bf16[off + c * rows + r] = h;
+        // the moving average restarts at the published weights
+        net.ema[k] = net.w[k];
+        net.ema_bf16[k] = h;
     }
 }
 
 // ---------------------------------------------------------------- launchers
 
-void launch_nrc_infer(hipStream_t s, int grid, const NrcWeights &net, const NrcBox &box, const NrcQuery *q,
-                      const int32_t *count, float *film, double *film64, float *acc)
+void launch_nrc_infer(hipStream_t s, int cu_count, int64_t max_queries, const NrcWeights &net, const NrcBox &box,
+                      const NrcQuery *q, const int32_t *count, float *film, double *film64, float *acc)
 {
-    hipLaunchKernelGGL(k_nrc_infer, dim3(grid), dim3(256), 0, s, net, box, q, count, film, film64, acc);
+    // persistent grid: as many workgroups as the LDS copies of the weights
+    // (and the registers) let stay resident, fewer for a small query bound
+    const int64_t span = (int64_t)INFER_THREADS / WAVE * INFER_TILES * TILE;
+    const int grid = (int)std::max<int64_t>(
+        1, std::min<int64_t>((int64_t)INFER_BLOCKS_PER_CU * cu_count, (max_queries + span - 1) / span));
+    hipLaunchKernelGGL(k_nrc_infer, dim3(grid), dim3(INFER_THREADS), 0, s, net, box, q, count, film, film64,
+                       acc);
 }
 
 void launch_nrc_eval(hipStream_t s, const NrcWeights &net, const NrcBox &box, const NrcVertex *v, int32_t n,
@@ -645,9 +748,8 @@ void launch_nrc_train_step(hipStream_t s, const NrcWeights &net, const NrcBox &b
 {
     launch_nrc_gradients(s, net, box, b, sc, grad);
     const int blocks = (int)(b.batch / (2 * TILE));
-    hipLaunchKernelGGL(k_nrc_finalize, dim3(1), dim3(256), 0, s, grad, sc, blocks * 2, b.loss_scale, b.valid_count,
-                       stats);
-    hipLaunchKernelGGL(k_nrc_update, dim3((NRC_PARAMS + 255) / 256), dim3(256), 0, s, net, opt, grad, sc);
+    hipLaunchKernelGGL(k_nrc_update, dim3((NRC_PARAMS + 255) / 256), dim3(256), 0, s, net, opt, grad, sc, blocks * 2,
+                       b.valid_count, stats);
 }
 
 void launch_nrc_gradients(hipStream_t s, const NrcWeights &net, const NrcBox &box, const NrcBatch &b,

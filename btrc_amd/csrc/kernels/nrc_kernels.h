@@ -16,18 +16,21 @@ struct NrcWeights
     float *v;
     uint16_t *w_bf16;
     uint16_t *wt_bf16;
+    float *ema;          // moving average of w (FP32) and its BF16 copy
+    uint16_t *ema_bf16;
 };
 
 // a cache query made at the vertex that ends a path's prefix
 // the training-path id of a path that is not one
 constexpr uint32_t NO_TRAIN = 0xffffffffu;
 
-struct NrcQuery
+struct __attribute__((aligned(16))) NrcQuery
 {
     NrcVertex v;
     float beta[3];   // throughput the vertex was reached with
     uint32_t pixel;
     uint32_t train;  // training path whose tail this is (its sum gets the answer too), or NO_TRAIN
+    uint32_t pad[3]; // 96 bytes: six 16-byte loads
 };
 
 // a training record: a vertex of an extended (training) path
@@ -118,6 +121,7 @@ struct NrcAdam
     float lr, beta1, beta2, eps;
     float inv_bias1, inv_bias2; // 1 / (1 - beta^t)
     float loss_scale;
+    float ema_decay;            // 0: no moving average
 };
 
 struct NrcStepStats
@@ -128,8 +132,10 @@ struct NrcStepStats
     int32_t skipped;   // 1: non-finite gradient or empty batch, no update
 };
 
-void launch_nrc_infer(hipStream_t s, int grid, const NrcWeights &net, const NrcBox &box, const NrcQuery *q,
-                      const int32_t *count, float *film, double *film64, float *acc);
+// queries [0, *count) with *count <= max_queries (which only sizes the grid),
+// no two of them for the same pixel
+void launch_nrc_infer(hipStream_t s, int cu_count, int64_t max_queries, const NrcWeights &net, const NrcBox &box,
+                      const NrcQuery *q, const int32_t *count, float *film, double *film64, float *acc);
 void launch_nrc_eval(hipStream_t s, const NrcWeights &net, const NrcBox &box, const NrcVertex *v, int32_t n,
                      float *out);
 void launch_nrc_train_step(hipStream_t s, const NrcWeights &net, const NrcBox &box, const NrcBatch &b,

@@ -101,7 +101,8 @@ __device__ __forceinline__ uint32_t leaf_code(int32_t first, int32_t count)
 // stops at an instance entry or before popping CODE_EXIT (node =
 // CODE_EXIT_NEXT), so the leaf is tested with its own instance's ray.
 //
-// Traces rays [0, n) handed out through *fetch. Sink provides
+// Traces rays [0, n): each wavefront's first TRACE_BLOCK by its workgroup
+// index, the rest handed out through *fetch (which starts at 0). Sink provides
 //   void load(uint32_t i, V3 &o, V3 &d, float &tmin, float &tmax)
 //   void store(uint32_t i, const HitRecord &h)
 // ANY ends a ray at its first hit in [tmin, tmax]. POSTPONE selects the
@@ -131,6 +132,12 @@ __device__ __forceinline__ void trace_persistent(
     uint32_t node = CODE_DONE;
     int32_t sp = 0;
     bool exhausted = false; // wavefront-uniform: the ray counter has passed n
+    // rays [0, static_span) go out as one batch of TRACE_BLOCK per wavefront
+    // (the first refill of workgroup b takes rays TRACE_BLOCK b + 0..63), the
+    // rest through *fetch: a small launch takes no atomics, a large one
+    // gridDim.x fewer
+    const uint32_t static_span = gridDim.x * (uint32_t)TRACE_BLOCK;
+    bool first = true;
     uint32_t cur_inst = 0xffffffffu, cur_base = 0; // two levels: the instance being traversed
 
     auto set_ray = [&](V3 ro, V3 rd) {
@@ -235,12 +242,23 @@ __device__ __forceinline__ void trace_persistent(
         const int nidle = (int)__popcll(idle);
         if(nidle >= REFILL_MIN && !exhausted)
         {
-            const int leader = __ffsll((unsigned long long)idle) - 1;
             uint32_t base = 0;
-            if(lane == leader)
-                base = atomicAdd(fetch, (uint32_t)nidle);
-            base = __shfl(base, leader);
-            exhausted = base + (uint32_t)nidle >= (uint32_t)n;
+            if(first)
+            {
+                // the wavefront's own first rays, no atomic; the counter
+                // hands out the rays past the grid's first batches
+                first = false;
+                base = blockIdx.x * TRACE_BLOCK;
+                exhausted = static_span >= (uint32_t)n;
+            }
+            else
+            {
+                const int leader = __ffsll((unsigned long long)idle) - 1;
+                if(lane == leader)
+                    base = static_span + atomicAdd(fetch, (uint32_t)nidle);
+                base = __shfl(base, leader);
+                exhausted = base + (uint32_t)nidle >= (uint32_t)n;
+            }
             if(!active)
             {
                 const uint32_t idx = base + (uint32_t)__popcll(idle & lanes_below);

@@ -54,8 +54,8 @@ struct WaveCounters
     int32_t shadow;       // shadow rays pushed this bounce
     int32_t gen_count;    // states generated this iteration
     int32_t gen_offset;   // first slot of the generated states
-    uint32_t trace_fetch; // next extension ray handed to a traversal lane
-    uint32_t shadow_fetch;// next shadow ray handed to a traversal lane
+    uint32_t trace_fetch; // extension rays handed out past the traversal grid's first batches
+    uint32_t shadow_fetch;// shadow rays handed out past the traversal grid's first batches
     int32_t pad0;
     uint64_t generated;   // paths generated so far
     uint64_t total;       // spp * width * height

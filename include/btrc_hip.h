@@ -302,6 +302,10 @@ typedef struct btrc_nrc_params
      * bounce loop ends after about k bounces (the cache trains on its own
      * tails, as the radiance cache paper's self-training does) */
     int32_t train_depth;
+    /* 0 (default): frames query the trained weights. d in (0, 1): they
+     * query an exponential moving average e <- d e + (1 - d) w, updated after
+     * every optimizer step (the radiance cache paper's averaged weights) */
+    float ema_decay;
 } btrc_nrc_params;
 
 typedef struct btrc_nrc_stats
@@ -460,6 +464,9 @@ int btrc_hip_nrc_stats(btrc_ctx *ctx, btrc_nrc_stats *stats);
  * 32 rows of which 3 are used */
 int btrc_hip_nrc_get_weights(btrc_ctx *ctx, float *weights);
 int btrc_hip_nrc_set_weights(btrc_ctx *ctx, const float *weights);
+/* the moving average of the weights (btrc_nrc_params.ema_decay), same layout;
+ * it restarts at the weights on every reset and set */
+int btrc_hip_nrc_get_ema_weights(btrc_ctx *ctx, float *weights);
 /* network outputs for n vertex records (16 floats each: position,
  * direction to the previous vertex, shading normal, roughness, diffuse,
  * specular); out: 3 floats each */

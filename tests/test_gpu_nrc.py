@@ -188,3 +188,27 @@ def test_self_training_ends_training_paths_early(cornell_ctx):
     e_cache, e_trunc = relmse(img, gt), relmse(trunc, gt)
     print(f"self-training: relMSE at 32 spp: cache {e_cache:.5f}, truncated {e_trunc:.5f}")
     assert e_cache < 0.5 * e_trunc
+
+
+def test_weight_moving_average(cornell_ctx):
+    """with ema_decay d every optimizer step sets e <- d e + (1 - d) w, starting
+    at the weights of the reset; without it the average stays there"""
+    ctx, _, _ = cornell_ctx
+    v = scene_vertices(4096, ctx.scene_box(), 9)
+    t = np.random.default_rng(10).uniform(0, 1, (4096, 3)).astype(np.float32)
+    d = 0.9
+    ctx.nrc_configure(learning_rate=1e-2, ema_decay=d)
+    ctx.nrc_reset(11)
+    e = ctx.nrc_weights().astype(np.float64)
+    assert np.array_equal(ctx.nrc_ema_weights(), ctx.nrc_weights())
+    for _ in range(3):
+        ctx.nrc_fit(v, t, 1)
+        e = d * e + (1 - d) * ctx.nrc_weights().astype(np.float64)
+        assert np.allclose(ctx.nrc_ema_weights(), e, rtol=1e-5, atol=1e-7)
+    ctx.nrc_configure(ema_decay=0.0)
+    before = ctx.nrc_ema_weights()
+    ctx.nrc_fit(v, t, 1)
+    assert np.array_equal(ctx.nrc_ema_weights(), before)
+    import btrc_amd
+    with pytest.raises(btrc_amd.BtrcException):
+        ctx.nrc_configure(ema_decay=1.0)
