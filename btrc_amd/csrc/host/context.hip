@@ -20,14 +20,14 @@
 using namespace btrc;
 
 // Scratch of btrc_hip_trace_closest_noalloc, for callers that cannot
-// allocate: the batch's ray counter in the first 256 bytes, then the
+// allocate: the batch's ray counters in the first NOALLOC_COUNTER_BYTES, then the
 // traversal's per-lane stack entries beyond the LDS part (codes, then entry
 // distances, each [entry][lane]). Every GPU the library runs on holds one
 // copy until the process ends. Host code never uses the array's name as an
 // address: it resolves the device address once per device
 // (hipGetSymbolAddress, at context creation) and uses that.
 constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)8 << 20;
-constexpr size_t NOALLOC_COUNTER_BYTES = 256;
+constexpr size_t NOALLOC_COUNTER_BYTES = FETCH_WORDS * sizeof(uint32_t);
 static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
 
 namespace {
@@ -292,7 +292,8 @@ struct btrc_ctx
     DevBuf<float4> sh_o, sh_d, sh_beta;
     DevBuf<uint2> sh_seed;
     DevBuf<uint32_t> sh_train;
-    DevBuf<WaveCounters> counters;
+    DevBuf<WaveState> counters;       // the counters (read back) and the queue bookkeeping
+    WaveCounters *wc() const { return &counters.ptr->c; }
     WaveCounters *host_counters = nullptr; // pinned
     DevBuf<float> film;
     DevBuf<double> film64;
@@ -592,7 +593,7 @@ void size_traversal(btrc_ctx *ctx)
     const size_t deep = ctx->stack_size > SHORT_STACK ? (size_t)(ctx->stack_size - SHORT_STACK) : 0;
     ctx->ovf_code.alloc(std::max<size_t>(1, deep * ctx->trace_grid * TRACE_BLOCK));
     ctx->ovf_t.alloc(std::max<size_t>(1, deep * ctx->trace_grid * TRACE_BLOCK));
-    ctx->fetch.alloc(1);
+    ctx->fetch.alloc(FETCH_WORDS);
 }
 
 // device layout of the traversal structure
@@ -1725,6 +1726,8 @@ void render(btrc_ctx *ctx, const btrc_render_params *p, int32_t view_count, cons
         throw StateError("render before btrc_hip_upload_scene");
     if(p->spp <= 0 || p->state_count <= 0)
         throw InvalidArg("spp and state_count must be positive");
+    if(p->state_count > (1 << 28))
+        throw InvalidArg("state_count above 2^28");
     if(p->min_depth < 0 || p->max_depth < 0)
         throw InvalidArg("depths must be non-negative");
     if(p->sampler != BTRC_SAMPLER_HALTON && p->sampler != BTRC_SAMPLER_INDEPENDENT)
@@ -1751,20 +1754,23 @@ void render(btrc_ctx *ctx, const btrc_render_params *p, int32_t view_count, cons
 
     hipStream_t s = ctx->stream;
     const size_t P = (size_t)p->state_count;
+    // pool sides: the last bounce's stripes, then the generated states;
+    // shadow queue: one stripe per COUNTER_STRIPES-th of the inputs
+    const size_t PS = pool_capacity(P), SS = shadow_capacity(P);
     if(ctx->pool_size != P)
     {
-        ctx->pool[0].alloc(P);
-        ctx->pool[1].alloc(P);
+        ctx->pool[0].alloc(PS);
+        ctx->pool[1].alloc(PS);
         ctx->hit_flag.alloc(P);
         ctx->hit_tuv.alloc(P);
-        ctx->sh_o.alloc(P);
-        ctx->sh_d.alloc(P);
-        ctx->sh_beta.alloc(P);
+        ctx->sh_o.alloc(SS);
+        ctx->sh_d.alloc(SS);
+        ctx->sh_beta.alloc(SS);
         ctx->pool_size = P;
     }
     const bool media = ctx->medium_count > 0 || ctx->volume_count > 0;
     if(media)
-        ctx->sh_seed.alloc(P);
+        ctx->sh_seed.alloc(SS);
     ctx->counters.alloc(1);
     if(!ctx->host_counters)
         BTRC_HIP_CHECK(hipHostMalloc((void **)&ctx->host_counters, sizeof(WaveCounters), hipHostMallocDefault));
@@ -1826,7 +1832,8 @@ void render(btrc_ctx *ctx, const btrc_render_params *p, int32_t view_count, cons
     WaveCounters init{};
     init.total = total;
     *ctx->host_counters = init;
-    BTRC_HIP_CHECK(hipMemcpyAsync(ctx->counters.ptr, ctx->host_counters, sizeof(WaveCounters), hipMemcpyHostToDevice, s));
+    BTRC_HIP_CHECK(hipMemsetAsync(ctx->counters.ptr, 0, sizeof(WaveState), s));
+    BTRC_HIP_CHECK(hipMemcpyAsync(ctx->wc(), ctx->host_counters, sizeof(WaveCounters), hipMemcpyHostToDevice, s));
 
     RenderConsts rc{};
     rc.views = ctx->views.ptr;
@@ -1879,7 +1886,7 @@ void render(btrc_ctx *ctx, const btrc_render_params *p, int32_t view_count, cons
                  p->normal ? ctx->film_normal.ptr : nullptr };
     HitSOA hit{ ctx->hit_flag.ptr, ctx->hit_tuv.ptr };
     if(nrc_on)
-        ctx->sh_train.alloc(P);
+        ctx->sh_train.alloc(SS);
     ShadowSOA sh{ ctx->sh_o.ptr, ctx->sh_d.ptr, ctx->sh_beta.ptr, media ? ctx->sh_seed.ptr : nullptr,
                   nrc_on ? ctx->sh_train.ptr : nullptr, nrc_on ? ctx->nrc.acc.ptr : nullptr };
 
@@ -1910,20 +1917,20 @@ void render(btrc_ctx *ctx, const btrc_render_params *p, int32_t view_count, cons
     auto enqueue_bounce = [&](int side, bool timed) {
         PathSOA path = ctx->pool[side].view();
         PathSOA next = ctx->pool[side ^ 1].view();
-        launch_plan(s, ctx->counters.ptr, p->state_count);
+        launch_plan(s, ctx->wc(), p->state_count);
         auto run = [&](int stage, auto &&fn) {
             if(timed)
                 timer.run(stage, fn);
             else
                 fn();
         };
-        run(STAGE_GENERATE, [&] { launch_generate(s, grid256, p->sampler, rcd, ctx->counters.ptr, path); });
-        run(STAGE_TRACE, [&] { launch_trace_extension(s, ctx->trace_grid, ctx->trace_sched(false), ts, ctx->counters.ptr, path, hit, ovf); });
+        run(STAGE_GENERATE, [&] { launch_generate(s, grid256, p->sampler, rcd, ctx->wc(), path); });
+        run(STAGE_TRACE, [&] { launch_trace_extension(s, ctx->trace_grid, ctx->trace_sched(false), ts, ctx->wc(), path, hit, ovf); });
         if(media)
-            run(STAGE_MEDIUM, [&] { launch_medium(s, grid256, p->sampler, rcd, sv, ctx->counters.ptr, path, hit, next, sh, fv); });
-        run(STAGE_SHADE, [&] { launch_shade(s, grid256, p->sampler, rcd, sv, ctx->counters.ptr, path, hit, next, sh, fv, nv); });
-        run(STAGE_SHADOW, [&] { launch_shadow(s, ctx->trace_grid, ctx->trace_sched(true), media, ts, sv, ctx->counters.ptr, sh, fv, ovf); });
-        launch_end_iteration(s, ctx->counters.ptr);
+            run(STAGE_MEDIUM, [&] { launch_medium(s, grid256, p->sampler, rcd, sv, ctx->wc(), path, hit, next, sh, fv); });
+        run(STAGE_SHADE, [&] { launch_shade(s, grid256, p->sampler, rcd, sv, ctx->wc(), path, hit, next, sh, fv, nv); });
+        run(STAGE_SHADOW, [&] { launch_shadow(s, ctx->trace_grid, ctx->trace_sched(true), media, ts, sv, ctx->wc(), sh, fv, ovf); });
+        launch_end_iteration(s, ctx->wc());
     };
 
     if(ctx->tuning.launch_graph && !timer.enabled && !capture)
@@ -1937,7 +1944,7 @@ void render(btrc_ctx *ctx, const btrc_render_params *p, int32_t view_count, cons
         };
         add(ts); add(sv); add(hit); add(sh); add(fv); add(nv); add(rcd);
         add(ovf.code); add(ovf.t); add(ovf.lanes);
-        add(ctx->pool[0].view()); add(ctx->pool[1].view()); add(ctx->counters.ptr);
+        add(ctx->pool[0].view()); add(ctx->pool[1].view()); add(ctx->wc());
         const int32_t ints[8] = { grid256, ctx->trace_grid, ctx->trace_sched(false), ctx->trace_sched(true),
                                   p->sampler, p->state_count, media ? 1 : 0, check_every };
         add(ints);
@@ -1963,7 +1970,7 @@ void render(btrc_ctx *ctx, const btrc_render_params *p, int32_t view_count, cons
             if(ctx->stop.load())
                 throw Stopped("render stopped");
             BTRC_HIP_CHECK(hipGraphLaunch(ctx->loop_exec, s));
-            BTRC_HIP_CHECK(hipMemcpyAsync(ctx->host_counters, ctx->counters.ptr, sizeof(WaveCounters), hipMemcpyDeviceToHost, s));
+            BTRC_HIP_CHECK(hipMemcpyAsync(ctx->host_counters, ctx->wc(), sizeof(WaveCounters), hipMemcpyDeviceToHost, s));
             BTRC_HIP_CHECK(hipStreamSynchronize(s));
             const WaveCounters &hc = *ctx->host_counters;
             reporter_stage.progress(hc);
@@ -1978,19 +1985,31 @@ void render(btrc_ctx *ctx, const btrc_render_params *p, int32_t view_count, cons
             throw Stopped("render stopped");
         PathSOA path = ctx->pool[cur].view();
         PathSOA next = ctx->pool[cur ^ 1].view();
-        launch_plan(s, ctx->counters.ptr, p->state_count);
-        timer.run(STAGE_GENERATE, [&] { launch_generate(s, grid256, p->sampler, rcd, ctx->counters.ptr, path); });
-        timer.run(STAGE_TRACE, [&] { launch_trace_extension(s, ctx->trace_grid, ctx->trace_sched(false), ts, ctx->counters.ptr, path, hit, ovf); });
+        launch_plan(s, ctx->wc(), p->state_count);
+        timer.run(STAGE_GENERATE, [&] { launch_generate(s, grid256, p->sampler, rcd, ctx->wc(), path); });
+        timer.run(STAGE_TRACE, [&] { launch_trace_extension(s, ctx->trace_grid, ctx->trace_sched(false), ts, ctx->wc(), path, hit, ovf); });
         if(capture && (int64_t)iter < capture_bounces && captured < capture_max)
         {
-            BTRC_HIP_CHECK(hipMemcpyAsync(ctx->host_counters, ctx->counters.ptr, sizeof(WaveCounters), hipMemcpyDeviceToHost, s));
+            BTRC_HIP_CHECK(hipMemcpyAsync(ctx->host_counters, ctx->wc(), sizeof(WaveCounters), hipMemcpyDeviceToHost, s));
             BTRC_HIP_CHECK(hipStreamSynchronize(s));
             const int64_t n = std::min<int64_t>(ctx->host_counters->active, capture_max - captured);
             std::vector<float4> o(n), d(n);
             if(n > 0)
             {
-                BTRC_HIP_CHECK(hipMemcpy(o.data(), path.ray_o, n * sizeof(float4), hipMemcpyDeviceToHost));
-                BTRC_HIP_CHECK(hipMemcpy(d.data(), path.ray_d, n * sizeof(float4), hipMemcpyDeviceToHost));
+                // the rays in dense order: segment by segment of the pool side
+                QueueState qs;
+                BTRC_HIP_CHECK(hipMemcpy(&qs, queue_state(ctx->wc()), sizeof(QueueState), hipMemcpyDeviceToHost));
+                for(int k = 0; k < POOL_SEGMENTS; ++k)
+                {
+                    const int64_t lo = qs.pool_prefix[k], hi = std::min<int64_t>(qs.pool_prefix[k + 1], n);
+                    if(hi > lo)
+                    {
+                        BTRC_HIP_CHECK(hipMemcpy(o.data() + lo, path.ray_o + qs.pool_first[k], (hi - lo) * sizeof(float4),
+                                                 hipMemcpyDeviceToHost));
+                        BTRC_HIP_CHECK(hipMemcpy(d.data() + lo, path.ray_d + qs.pool_first[k], (hi - lo) * sizeof(float4),
+                                                 hipMemcpyDeviceToHost));
+                    }
+                }
             }
             for(int64_t k = 0; k < n; ++k)
             {
@@ -2001,15 +2020,15 @@ void render(btrc_ctx *ctx, const btrc_render_params *p, int32_t view_count, cons
             captured += n;
         }
         if(media)
-            timer.run(STAGE_MEDIUM, [&] { launch_medium(s, grid256, p->sampler, rcd, sv, ctx->counters.ptr, path, hit, next, sh, fv); });
-        timer.run(STAGE_SHADE, [&] { launch_shade(s, grid256, p->sampler, rcd, sv, ctx->counters.ptr, path, hit, next, sh, fv, nv); });
-        timer.run(STAGE_SHADOW, [&] { launch_shadow(s, ctx->trace_grid, ctx->trace_sched(true), media, ts, sv, ctx->counters.ptr, sh, fv, ovf); });
-        launch_end_iteration(s, ctx->counters.ptr);
+            timer.run(STAGE_MEDIUM, [&] { launch_medium(s, grid256, p->sampler, rcd, sv, ctx->wc(), path, hit, next, sh, fv); });
+        timer.run(STAGE_SHADE, [&] { launch_shade(s, grid256, p->sampler, rcd, sv, ctx->wc(), path, hit, next, sh, fv, nv); });
+        timer.run(STAGE_SHADOW, [&] { launch_shadow(s, ctx->trace_grid, ctx->trace_sched(true), media, ts, sv, ctx->wc(), sh, fv, ovf); });
+        launch_end_iteration(s, ctx->wc());
         BTRC_HIP_CHECK(hipGetLastError());
         cur ^= 1;
         if((iter + 1) % check_every == 0)
         {
-            BTRC_HIP_CHECK(hipMemcpyAsync(ctx->host_counters, ctx->counters.ptr, sizeof(WaveCounters), hipMemcpyDeviceToHost, s));
+            BTRC_HIP_CHECK(hipMemcpyAsync(ctx->host_counters, ctx->wc(), sizeof(WaveCounters), hipMemcpyDeviceToHost, s));
             BTRC_HIP_CHECK(hipStreamSynchronize(s));
             const WaveCounters &hc = *ctx->host_counters;
             reporter_stage.progress(hc);
@@ -2146,7 +2165,7 @@ void check_batch_args(btrc_ctx *ctx, const void *rays, int64_t n, int32_t tri_te
 
 void trace_batch(btrc_ctx *ctx, int32_t tri_test, bool any_hit, const float4 *rays, int64_t n, uint32_t *out)
 {
-    BTRC_HIP_CHECK(hipMemsetAsync(ctx->fetch.ptr, 0, sizeof(uint32_t), ctx->stream));
+    BTRC_HIP_CHECK(hipMemsetAsync(ctx->fetch.ptr, 0, FETCH_WORDS * sizeof(uint32_t), ctx->stream));
     launch_trace_batch(ctx->stream, ctx->trace_grid, ctx->trace_sched(any_hit != 0), ctx->trace_scene(), tri_test, any_hit, rays,
                        (int32_t)n, out, ctx->fetch.ptr, ctx->trace_overflow());
     BTRC_HIP_CHECK(hipGetLastError());
@@ -2716,7 +2735,7 @@ int btrc_hip_trace_closest_noalloc(btrc_ctx *ctx, const btrc_ray *rays, int64_t 
         const size_t lanes = (size_t)waves * TRACE_BLOCK;
         uint32_t *ovf_code = reinterpret_cast<uint32_t *>(base + NOALLOC_COUNTER_BYTES);
         float *ovf_t = reinterpret_cast<float *>(base + NOALLOC_COUNTER_BYTES + deep * lanes * sizeof(uint32_t));
-        launch_fill_u32(ctx->stream, fetch, 1, 0u);
+        launch_fill_u32(ctx->stream, fetch, FETCH_WORDS, 0u);
         BTRC_HIP_CHECK(hipGetLastError());
         launch_trace_batch(ctx->stream, waves, ctx->trace_sched(false), ctx->trace_scene(), tri_test, false,
                            reinterpret_cast<const float4 *>(rays), (int32_t)n, reinterpret_cast<uint32_t *>(hits),

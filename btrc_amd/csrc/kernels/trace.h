@@ -32,6 +32,15 @@ constexpr int REFILL_MIN = 32;    // idle lanes that trigger a refill
 constexpr int TRACE_WAVES_PER_CU = 20; // persistent grid: wavefronts per compute unit (default)
 constexpr int TRACE_WAVES_PER_CU_MAX = 32;
 
+// Device-scope atomics on one address serialise at about 11 ns each on
+// MI355X (a wavefront-per-atomic microbenchmark: 65,536 atomics on one word
+// take 746 us, spread over 64 words 22 us). Work counters are therefore
+// striped: COUNTER_STRIPES counters, each on its own 128-byte line
+// (COUNTER_STRIDE 32-bit words apart).
+constexpr int COUNTER_STRIPES = 16;
+constexpr int COUNTER_STRIDE = 32;
+constexpr int FETCH_WORDS = COUNTER_STRIPES * COUNTER_STRIDE; // a traversal's fetch counters
+
 // traversal schedules: leaves tested when reached, leaves postponed until
 // every lane holds one, or the two-level (instance) structure
 enum : int
@@ -101,10 +110,16 @@ __device__ __forceinline__ uint32_t leaf_code(int32_t first, int32_t count)
 // stops at an instance entry or before popping CODE_EXIT (node =
 // CODE_EXIT_NEXT), so the leaf is tested with its own instance's ray.
 //
-// Traces rays [0, n): each wavefront's first TRACE_BLOCK by its workgroup
-// index, the rest handed out through *fetch (which starts at 0). Sink provides
-//   void load(uint32_t i, V3 &o, V3 &d, float &tmin, float &tmax)
-//   void store(uint32_t i, const HitRecord &h)
+// Traces rays [0, n): each wavefront's first TRACE_BLOCK rays by its
+// workgroup index, the rest [gridDim.x * TRACE_BLOCK, n) in granules of
+// TRACE_BLOCK rays dealt round robin to C = min(COUNTER_STRIPES, gridDim.x)
+// chunks, chunk k's granules handed out by counter fetch[k * COUNTER_STRIDE]
+// (all FETCH_WORDS words start at 0) to the wavefronts of workgroups
+// b = k (mod C). Sink provides
+//   uint32_t load(uint32_t base, uint32_t i, V3 &o, V3 &d, float &tmin, float &tmax)
+//       (ray i of a refill that hands out rays base .. base + 63; returns
+//       the key its result is stored under)
+//   void store(uint32_t key, const HitRecord &h)
 // ANY ends a ray at its first hit in [tmin, tmax]. POSTPONE selects the
 // while-while schedule (leaves wait until every lane holds one); without it
 // a lane tests a leaf as soon as it meets it, which ends occluded shadow
@@ -120,7 +135,7 @@ __device__ __forceinline__ void trace_persistent(
     const uint64_t lanes_below = (1ull << lane) - 1ull;
 
     bool active = false;
-    uint32_t ray = 0;
+    uint32_t ray = 0; // the sink's key of the lane's ray
     V3 o = v3s(0.0f), d = v3s(0.0f);
     float tmin = 0.0f;
     float margin = 0.0f; // culling margin of the ray (cull_margin)
@@ -131,13 +146,24 @@ __device__ __forceinline__ void trace_persistent(
     HitRecord h;
     uint32_t node = CODE_DONE;
     int32_t sp = 0;
-    bool exhausted = false; // wavefront-uniform: the ray counter has passed n
-    // rays [0, static_span) go out as one batch of TRACE_BLOCK per wavefront
-    // (the first refill of workgroup b takes rays TRACE_BLOCK b + 0..63), the
-    // rest through *fetch: a small launch takes no atomics, a large one
-    // gridDim.x fewer
+    // Rays [0, static_span) go out as one batch of TRACE_BLOCK per wavefront
+    // (workgroup b's first refill takes rays TRACE_BLOCK b + 0..63), so a
+    // small launch takes no atomics. The rest go out in granules of
+    // TRACE_BLOCK rays: granule g belongs to chunk g % C, and chunk k's
+    // counter hands its granules out in order to the wavefronts of
+    // workgroups b = k (mod C). Every chunk spans the whole queue (balanced
+    // work) and the granules in flight stay close together (coherent rays).
+    // A wavefront keeps the rest of its granule for its next refill.
     const uint32_t static_span = gridDim.x * (uint32_t)TRACE_BLOCK;
-    bool first = true;
+    const uint32_t chunks = min((uint32_t)COUNTER_STRIPES, gridDim.x); // every chunk has wavefronts
+    const uint32_t home = blockIdx.x % chunks;
+    uint32_t *counter = fetch + home * COUNTER_STRIDE;
+    // wavefront-uniform: reserved rays not yet handed out, whether the home
+    // chunk has run dry, and whether no ray is left for this wavefront
+    uint32_t res_lo = min((uint32_t)n, blockIdx.x * (uint32_t)TRACE_BLOCK);
+    uint32_t res_hi = min((uint32_t)n, res_lo + (uint32_t)TRACE_BLOCK);
+    bool dry = static_span >= (uint32_t)n;
+    bool exhausted = dry && res_lo == res_hi;
     uint32_t cur_inst = 0xffffffffu, cur_base = 0; // two levels: the instance being traversed
 
     auto set_ray = [&](V3 ro, V3 rd) {
@@ -242,31 +268,35 @@ __device__ __forceinline__ void trace_persistent(
         const int nidle = (int)__popcll(idle);
         if(nidle >= REFILL_MIN && !exhausted)
         {
-            uint32_t base = 0;
-            if(first)
+            if(res_lo == res_hi && !dry)
             {
-                // the wavefront's own first rays, no atomic; the counter
-                // hands out the rays past the grid's first batches
-                first = false;
-                base = blockIdx.x * TRACE_BLOCK;
-                exhausted = static_span >= (uint32_t)n;
+                // the home chunk's next granule (every lane runs this loop,
+                // so lane 0 takes it for the wavefront)
+                uint32_t g = 0;
+                if(lane == 0)
+                    g = atomicAdd(counter, 1u);
+                g = __builtin_amdgcn_readfirstlane(g);
+                const uint64_t lo = (uint64_t)static_span + ((uint64_t)g * chunks + home) * TRACE_BLOCK;
+                if(lo < (uint64_t)n)
+                {
+                    res_lo = (uint32_t)lo;
+                    res_hi = min((uint32_t)n, res_lo + (uint32_t)TRACE_BLOCK);
+                }
+                else
+                    dry = true;
             }
-            else
-            {
-                const int leader = __ffsll((unsigned long long)idle) - 1;
-                if(lane == leader)
-                    base = static_span + atomicAdd(fetch, (uint32_t)nidle);
-                base = __shfl(base, leader);
-                exhausted = base + (uint32_t)nidle >= (uint32_t)n;
-            }
+            const uint32_t base = res_lo;                          // this refill's rays: base + 0 .. take - 1
+            const uint32_t take = min((uint32_t)nidle, res_hi - res_lo);
+            res_lo += take;
+            exhausted = dry && res_lo == res_hi;
             if(!active)
             {
-                const uint32_t idx = base + (uint32_t)__popcll(idle & lanes_below);
-                if(idx < (uint32_t)n)
+                const uint32_t rank = (uint32_t)__popcll(idle & lanes_below);
+                if(rank < take)
                 {
-                    ray = idx;
+                    const uint32_t idx = base + rank;
                     float tmax;
-                    sink.load(idx, o, d, tmin, tmax);
+                    ray = sink.load(base, idx, o, d, tmin, tmax);
                     set_ray(o, d);
                     margin = cull_margin(sc.extent, o);
                     cur_inst = 0xffffffffu;

@@ -50,12 +50,8 @@ constexpr uint32_t PATH_FLAG_INSTANCE_ID_MASK = ~0u >> 2;
 struct WaveCounters
 {
     int32_t active;       // live states in the current pool side
-    int32_t next_active;  // states pushed into the other side this bounce
-    int32_t shadow;       // shadow rays pushed this bounce
     int32_t gen_count;    // states generated this iteration
-    int32_t gen_offset;   // first slot of the generated states
-    uint32_t trace_fetch; // extension rays handed out past the traversal grid's first batches
-    uint32_t shadow_fetch;// shadow rays handed out past the traversal grid's first batches
+    int32_t gen_offset;   // dense index of the first generated state
     int32_t pad0;
     uint64_t generated;   // paths generated so far
     uint64_t total;       // spp * width * height
@@ -64,6 +60,90 @@ struct WaveCounters
     uint64_t shadow_rays; // shadow rays traced
     uint64_t iterations;
 };
+
+// Queues without a shared counter. A bounce's stage kernels take their
+// inputs by dense index j (0 <= j < n) and push at most one state and one
+// shadow ray per input; input j pushes into stripe k = j / stripe (stripe:
+// a multiple of 64, so a wavefront's inputs share one stripe, and
+// COUNTER_STRIPES * stripe >= n) at slots k * stripe + 0, 1, ... from its
+// own counter. A pool side therefore holds the states of the last bounce in
+// COUNTER_STRIPES segments and, after them, the states generated for this
+// bounce at slot pool_gen_slot(state_count); a SegmentMap turns dense
+// indices into slots.
+constexpr int POOL_SEGMENTS = COUNTER_STRIPES + 1;
+
+struct QueueState
+{
+    int32_t next_n[COUNTER_STRIPES * COUNTER_STRIDE];   // states pushed into the other pool side, per stripe
+    int32_t shadow_n[COUNTER_STRIPES * COUNTER_STRIDE]; // shadow rays pushed, per stripe
+    uint32_t trace_fetch[FETCH_WORDS];                  // extension-ray chunk counters
+    uint32_t shadow_fetch[FETCH_WORDS];                 // shadow-ray chunk counters
+    int32_t stripe;                                     // inputs per stripe of this bounce
+    int32_t pool_first[POOL_SEGMENTS];                  // slot of each segment's first state
+    int32_t pool_prefix[POOL_SEGMENTS + 1];             // dense index of each segment's first state
+};
+
+// the counters the host reads back, then the queue bookkeeping (device only)
+struct WaveState
+{
+    WaveCounters c;
+    alignas(128) QueueState q;
+};
+
+__host__ __device__ inline QueueState *queue_state(WaveCounters *c) { return &reinterpret_cast<WaveState *>(c)->q; }
+__host__ __device__ inline const QueueState *queue_state(const WaveCounters *c)
+{
+    return &reinterpret_cast<const WaveState *>(c)->q;
+}
+
+// slots of a pool side and of the shadow queue for a path pool of state_count
+__host__ __device__ inline int32_t pool_gen_slot(int32_t state_count) { return state_count + 64 * COUNTER_STRIPES; }
+inline size_t pool_capacity(size_t state_count) { return 2 * state_count + 64 * COUNTER_STRIPES; }
+inline size_t shadow_capacity(size_t state_count) { return state_count + 64 * COUNTER_STRIPES; }
+
+// Slot of the state with dense index j of the current pool side, for lanes
+// of one wavefront whose indices lie in [j0, j0 + 63] (j0 wave-uniform):
+// the segment of j0 is found with scalar loads and compares, then the lanes
+// past its end (segments that start inside the range: rare) are moved on.
+__device__ __forceinline__ int32_t pool_slot(const QueueState *q, int32_t j0, int32_t j)
+{
+    j0 = __builtin_amdgcn_readfirstlane(j0);
+    int32_t s = 0;
+    for(int k = 1; k < POOL_SEGMENTS; ++k)
+        if(q->pool_prefix[k] <= j0)
+            s = k;
+    int32_t slot = j - q->pool_prefix[s] + q->pool_first[s];
+    for(int t = s + 1; t < POOL_SEGMENTS && q->pool_prefix[t] <= j0 + 63; ++t)
+        if(j >= q->pool_prefix[t])
+            slot = j - q->pool_prefix[t] + q->pool_first[t];
+    return slot;
+}
+
+// the same for this bounce's shadow rays, whose segments (stripe k at slot
+// k * stripe, in stripe order) follow from the stripe counters
+__device__ __forceinline__ int32_t shadow_slot(const QueueState *q, int32_t j0, int32_t j)
+{
+    j0 = __builtin_amdgcn_readfirstlane(j0);
+    const int32_t stripe = q->stripe;
+    int32_t slot = j, first = 0;
+    for(int k = 0; k < COUNTER_STRIPES && first <= j0 + 63; ++k)
+    {
+        const int32_t count = q->shadow_n[k * COUNTER_STRIDE];
+        if(first + count > j0 && j >= first)
+            slot = j - first + k * stripe;
+        first += count;
+    }
+    return slot;
+}
+
+// the number of shadow rays pushed this bounce
+__device__ __forceinline__ int32_t shadow_count(const QueueState *q)
+{
+    int32_t n = 0;
+    for(int k = 0; k < COUNTER_STRIPES; ++k)
+        n += q->shadow_n[k * COUNTER_STRIDE];
+    return n;
+}
 
 struct CameraConsts
 {

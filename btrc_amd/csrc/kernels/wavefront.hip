@@ -33,6 +33,23 @@ __device__ __forceinline__ int32_t wave_push(int32_t *counter, bool pred)
     return pred ? base + (int32_t)__popcll(below) : -1;
 }
 
+// Wave-aggregated push into stripe k of a striped queue (counters k *
+// COUNTER_STRIDE apart): lanes with pred receive consecutive slots from
+// k * stripe on.
+__device__ __forceinline__ int32_t stripe_push(int32_t *counters, int32_t k, int32_t stripe, bool pred)
+{
+    const uint64_t mask = __ballot(pred);
+    if(mask == 0)
+        return -1;
+    const int leader = __ffsll((unsigned long long)mask) - 1;
+    int32_t base = 0;
+    if(lane_id() == leader)
+        base = atomicAdd(counters + k * COUNTER_STRIDE, (int32_t)__popcll(mask));
+    base = __shfl(base, leader);
+    const uint64_t below = mask & ((1ull << lane_id()) - 1ull);
+    return pred ? k * stripe + base + (int32_t)__popcll(below) : -1;
+}
+
 __device__ __forceinline__ void film_splat(const FilmView &film, uint32_t pixel, V3 v)
 {
     // Film::splat_atomic drops non-finite values (core/film.cpp:145-163)
@@ -151,6 +168,7 @@ struct TrSampler
 // remaining paths fit.
 __global__ void k_plan(WaveCounters *c, int32_t state_count)
 {
+    QueueState *q = queue_state(c);
     const int32_t active = c->active;
     const int64_t rest = (int64_t)state_count - active;
     int64_t fresh = 0;
@@ -160,14 +178,25 @@ __global__ void k_plan(WaveCounters *c, int32_t state_count)
         if(!(unfinished > rest && rest + rest < state_count))
             fresh = unfinished < rest ? unfinished : rest;
     }
-    c->trace_fetch = 0;
-    c->shadow_fetch = 0;
+    for(int k = 0; k < FETCH_WORDS; k += COUNTER_STRIDE)
+    {
+        q->trace_fetch[k] = 0;
+        q->shadow_fetch[k] = 0;
+    }
+    // the generated states form the last segment of the current side
+    const int32_t n = active + (int32_t)fresh;
+    q->pool_first[COUNTER_STRIPES] = pool_gen_slot(state_count);
+    q->pool_prefix[COUNTER_STRIPES] = active;
+    q->pool_prefix[COUNTER_STRIPES + 1] = n;
+    // this bounce's pushes: COUNTER_STRIPES stripes over its n inputs
+    const int32_t per = (n + COUNTER_STRIPES - 1) / COUNTER_STRIPES;
+    q->stripe = per > 0 ? (per + 63) / 64 * 64 : 64;
     c->gen_offset = active;
     c->gen_count = (int32_t)fresh;
     c->gen_base = c->generated;
     c->generated += (uint64_t)fresh;
-    c->active = active + (int32_t)fresh;
-    c->ext_rays += (uint64_t)(active + fresh);
+    c->active = n;
+    c->ext_rays += (uint64_t)n;
 }
 
 // ---------------------------------------------------------------- generate
@@ -178,7 +207,7 @@ __global__ void __launch_bounds__(256) k_generate(const RenderConsts *__restrict
 {
     const RenderConsts &rc = *rcp;
     const int32_t n = c->gen_count;
-    const int32_t offset = c->gen_offset;
+    const int32_t gen_slot = queue_state(c)->pool_first[COUNTER_STRIPES]; // the generated segment
     const uint64_t base = c->gen_base;
     for(int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
     {
@@ -243,7 +272,7 @@ __global__ void __launch_bounds__(256) k_generate(const RenderConsts *__restrict
         const V3 eye = v3(cam.eye[0], cam.eye[1], cam.eye[2]);
         const V3 dir = normalize(dst - eye);
 
-        const int32_t s = offset + i;
+        const int32_t s = gen_slot + i;
         path.ray_o[s] = f4(eye, __uint_as_float(rc.vol_prim_medium));
         path.ray_d[s] = f4(dir, MAX_FLOAT);
         path.beta[s] = make_float4(1.0f, 1.0f, 1.0f, __int_as_float(0));
@@ -260,15 +289,19 @@ struct ExtensionSink
 {
     PathSOA path;
     HitSOA hit;
+    const QueueState *q;
 
-    __device__ __forceinline__ void load(uint32_t i, V3 &o, V3 &d, float &tmin, float &tmax) const
+    // the state's ray from its pool slot; the hit goes to dense index i
+    __device__ __forceinline__ uint32_t load(uint32_t base, uint32_t i, V3 &o, V3 &d, float &tmin, float &tmax) const
     {
-        const float4 a = path.ray_o[i];
-        const float4 b = path.ray_d[i];
+        const int32_t slot = pool_slot(q, (int32_t)base, (int32_t)i);
+        const float4 a = path.ray_o[slot];
+        const float4 b = path.ray_d[slot];
         o = xyz(a);
         d = xyz(b);
         tmin = 0.0f;
         tmax = b.w;
+        return i;
     }
     __device__ __forceinline__ void store(uint32_t i, const HitRecord &h) const
     {
@@ -286,9 +319,10 @@ template<int SCHED>
 __global__ void __launch_bounds__(TRACE_BLOCK) k_trace_extension(
     TraceScene ts, WaveCounters *c, PathSOA path, HitSOA hit, TraceOverflow ovf)
 {
-    ExtensionSink sink{ path, hit };
+    QueueState *q = queue_state(c);
+    ExtensionSink sink{ path, hit, q };
     trace_persistent<TRI_WATERTIGHT, false, (SCHED & SCHED_POSTPONE) != 0, (SCHED & SCHED_TWO_LEVEL) != 0>(
-        ts, sink, c->active, &c->trace_fetch, ovf);
+        ts, sink, c->active, q->trace_fetch, ovf);
 }
 
 // ---------------------------------------------------------------- shade
@@ -329,23 +363,28 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))
     PathSOA next, ShadowSOA shadow, FilmView film, NrcView nrc)
 {
     const RenderConsts &rc = *rcp;
+    QueueState *q = queue_state(c);
     const int32_t n = c->active;
+    const int32_t stripe = q->stripe;
     const int32_t stride = gridDim.x * blockDim.x;
     for(int32_t base = blockIdx.x * blockDim.x; base < n; base += stride)
     {
         const int32_t i = base + (int32_t)threadIdx.x;
+        const int32_t w0 = base + (int32_t)(threadIdx.x & ~63u); // this wavefront's first input
+        const int32_t k = w0 / stripe;                            // its stripe
         // Queue pushes happen where their payload is made: a wave-aggregated
         // atomic among the lanes that reach that point together, so no
         // payload stays live across the rest of the shading code.
         if(i < n)
         {
+            const int32_t ps = pool_slot(q, w0, i); // the state's pool slot
             const uint32_t flag = hit.flag[i];
-            const float4 ro = path.ray_o[i];
-            const float4 rd = path.ray_d[i];
-            const float4 beta_depth = path.beta[i];
-            const float4 rad4 = path.rad[i];
-            const float4 le4 = path.bsdf_le[i];
-            const uint4 smp = path.smp[i];
+            const float4 ro = path.ray_o[ps];
+            const float4 rd = path.ray_d[ps];
+            const float4 beta_depth = path.beta[ps];
+            const float4 rad4 = path.rad[ps];
+            const float4 le4 = path.bsdf_le[ps];
+            const uint4 smp = path.smp[ps];
             const uint32_t pixel = smp.x;
             PathSampler<SAMPLER> hs = PathSampler<SAMPLER>::load(rc, smp);
 
@@ -493,7 +532,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))
                                            : (shadow_out ? inst.outer_medium : inst.inner_medium);
                         const float cs = fabs_(dot(normalize(ss.d), sp.frame.z));
                         const V3 beta_li = ss.li * beta * shadow_f * cs / (shadow_pdf + ss.light_pdf);
-                        const int32_t si = wave_push(&c->shadow, true);
+                        const int32_t si = stripe_push(q->shadow_n, k, stripe, true);
                         shadow.o[si] = f4(ss.o, __uint_as_float(pixel));
                         shadow.d[si] = f4(ss.d, ss.t1);
                         shadow.beta[si] = f4(beta_li, __uint_as_float(med));
@@ -515,7 +554,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))
                         const bool next_out = dot(sp.frame.z, dir) > 0.0f;
                         const uint32_t med = next_out ? inst.outer_medium : inst.inner_medium;
                         const float stored_pdf = bs.is_delta ? -bs.pdf : bs.pdf;
-                        const int32_t ni = wave_push(&c->next_active, true);
+                        const int32_t ni = stripe_push(q->next_n, k, stripe, true);
                         next.ray_o[ni] = f4(next_o, __uint_as_float(med));
                         next.ray_d[ni] = f4(dir, MAX_FLOAT);
                         next.beta[ni] = f4(beta, __int_as_float(depth + 1));
@@ -540,22 +579,27 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))
 // through the phase function, phase-sampled continuation). The shading
 // stage then only adds the emission the segment ended on.
 template<int SAMPLER>
-__global__ void __launch_bounds__(256) k_medium(
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) k_medium(
     const RenderConsts *__restrict__ rcp, SceneView sv, WaveCounters *c, PathSOA path, HitSOA hit,
     PathSOA next, ShadowSOA shadow, FilmView film)
 {
     const RenderConsts &rc = *rcp;
+    QueueState *q = queue_state(c);
     const int32_t n = c->active;
+    const int32_t stripe = q->stripe;
     const int32_t stride = gridDim.x * blockDim.x;
     for(int32_t base = blockIdx.x * blockDim.x; base < n; base += stride)
     {
         const int32_t i = base + (int32_t)threadIdx.x;
+        const int32_t w0 = base + (int32_t)(threadIdx.x & ~63u); // this wavefront's first input
+        const int32_t k = w0 / stripe;                            // its stripe
         // queue pushes where their payload is made, as in k_shade
         if(i < n)
         {
+            const int32_t ps = pool_slot(q, w0, i); // the state's pool slot
             const uint32_t flag = hit.flag[i];
-            const float4 ro = path.ray_o[i];
-            const float4 rd = path.ray_d[i];
+            const float4 ro = path.ray_o[ps];
+            const float4 rd = path.ray_d[ps];
             const V3 o = xyz(ro), d = xyz(rd);
 
             // get_medium_segment, medium.cpp:214-245
@@ -577,12 +621,12 @@ __global__ void __launch_bounds__(256) k_medium(
                 med = (uint32_t)sv.medium_count;
                 end = o + normalize(d) * sv.world_diagonal;
             }
-            path.ray_o[i] = f4(o, __uint_as_float(med));
+            path.ray_o[ps] = f4(o, __uint_as_float(med));
 
             const bool vol_prim = med == (uint32_t)sv.medium_count && sv.vol.count > 0;
             if(med < (uint32_t)sv.medium_count || vol_prim)
             {
-                const uint4 smp = path.smp[i];
+                const uint4 smp = path.smp[ps];
                 const uint32_t pixel = smp.x;
                 PathSampler<SAMPLER> hs = PathSampler<SAMPLER>::load(rc, smp);
                 MediumSample ms;
@@ -598,18 +642,18 @@ __global__ void __launch_bounds__(256) k_medium(
                     ms = medium_sample(m, sv.voxels, o, end, hs);
                     tr = medium_tr(m, sv.voxels, o, end, hs);
                 }
-                const float4 le4 = path.bsdf_le[i];
-                path.bsdf_le[i] = f4(xyz(le4) * tr, le4.w);
+                const float4 le4 = path.bsdf_le[ps];
+                path.bsdf_le[ps] = f4(xyz(le4) * tr, le4.w);
 
                 if(!ms.scattered)
-                    path.smp[i] = hs.pack(pixel);
+                    path.smp[ps] = hs.pack(pixel);
                 else
                 {
                     hit.flag[i] = flag | PATH_FLAG_HAS_SCATTERING;
-                    const float4 beta_depth = path.beta[i];
+                    const float4 beta_depth = path.beta[ps];
                     V3 beta = xyz(beta_depth); // medium throughput is one
                     const int32_t depth = __float_as_int(beta_depth.w);
-                    const float4 rad4 = path.rad[i];
+                    const float4 rad4 = path.rad[ps];
                     const V3 path_rad = xyz(rad4);
 
                     bool rr_exit = false;
@@ -649,7 +693,7 @@ __global__ void __launch_bounds__(256) k_medium(
                         if(ss.success)
                         {
                             const uint32_t smed = ss.t1 > 1.0f ? (uint32_t)sv.medium_count : med;
-                            const int32_t si = wave_push(&c->shadow, true);
+                            const int32_t si = stripe_push(q->shadow_n, k, stripe, true);
                             shadow.o[si] = f4(ss.o, __uint_as_float(pixel));
                             shadow.d[si] = f4(ss.d, ss.t1);
                             shadow.beta[si] = f4(beta_li, __uint_as_float(smed));
@@ -662,7 +706,7 @@ __global__ void __launch_bounds__(256) k_medium(
                             const V3 dir = normalize(ps.dir);
                             const V3 beta_le = beta * ps.phase;
                             beta = beta_le / ps.pdf;
-                            const int32_t ni = wave_push(&c->next_active, true);
+                            const int32_t ni = stripe_push(q->next_n, k, stripe, true);
                             next.ray_o[ni] = f4(ms.position, __uint_as_float(med));
                             next.ray_d[ni] = f4(dir, MAX_FLOAT);
                             next.beta[ni] = f4(beta, __int_as_float(depth + 1));
@@ -689,15 +733,19 @@ struct ShadowSink
     ShadowSOA shadow;
     FilmView film;
     SceneView sv;
+    const QueueState *q;
 
-    __device__ __forceinline__ void load(uint32_t i, V3 &o, V3 &d, float &tmin, float &tmax) const
+    // the ray from its shadow queue slot, which is also its key
+    __device__ __forceinline__ uint32_t load(uint32_t base, uint32_t j, V3 &o, V3 &d, float &tmin, float &tmax) const
     {
+        const int32_t i = shadow_slot(q, (int32_t)base, (int32_t)j);
         const float4 a = shadow.o[i];
         const float4 b = shadow.d[i];
         o = xyz(a);
         d = xyz(b);
         tmin = 0.0f;
         tmax = b.w;
+        return (uint32_t)i;
     }
     __device__ __forceinline__ void store(uint32_t i, const HitRecord &h) const
     {
@@ -747,19 +795,34 @@ template<int SCHED, bool MEDIA>
 __global__ void __launch_bounds__(TRACE_BLOCK) k_shadow(
     TraceScene ts, SceneView sv, WaveCounters *c, ShadowSOA shadow, FilmView film, TraceOverflow ovf)
 {
-    ShadowSink<MEDIA> sink{ shadow, film, sv };
+    QueueState *q = queue_state(c);
+    ShadowSink<MEDIA> sink{ shadow, film, sv, q };
     trace_persistent<TRI_WATERTIGHT, true, (SCHED & SCHED_POSTPONE) != 0, (SCHED & SCHED_TWO_LEVEL) != 0>(
-        ts, sink, c->shadow, &c->shadow_fetch, ovf);
+        ts, sink, shadow_count(q), q->shadow_fetch, ovf);
 }
 
 // ---------------------------------------------------------------- bookkeeping
 
+// the states pushed this bounce become the next bounce's pool side: one
+// segment per stripe, then the (empty) generated segment; counters restart
 __global__ void k_end_iteration(WaveCounters *c)
 {
-    c->shadow_rays += (uint64_t)c->shadow;
-    c->active = c->next_active;
-    c->next_active = 0;
-    c->shadow = 0;
+    QueueState *q = queue_state(c);
+    const int32_t stripe = q->stripe;
+    int32_t states = 0, shadows = 0;
+    for(int k = 0; k < COUNTER_STRIPES; ++k)
+    {
+        q->pool_first[k] = k * stripe;
+        q->pool_prefix[k] = states;
+        states += q->next_n[k * COUNTER_STRIDE];
+        shadows += q->shadow_n[k * COUNTER_STRIDE];
+        q->next_n[k * COUNTER_STRIDE] = 0;
+        q->shadow_n[k * COUNTER_STRIDE] = 0;
+    }
+    q->pool_prefix[COUNTER_STRIPES] = states;
+    q->pool_prefix[COUNTER_STRIPES + 1] = states;
+    c->shadow_rays += (uint64_t)shadows;
+    c->active = states;
     c->iterations += 1;
 }
 
@@ -785,7 +848,7 @@ struct BatchSink
     const float4 *rays;
     uint32_t *out;
 
-    __device__ __forceinline__ void load(uint32_t i, V3 &o, V3 &d, float &tmin, float &tmax) const
+    __device__ __forceinline__ uint32_t load(uint32_t, uint32_t i, V3 &o, V3 &d, float &tmin, float &tmax) const
     {
         const float4 a = rays[2 * (size_t)i + 0];
         const float4 b = rays[2 * (size_t)i + 1];
@@ -793,6 +856,7 @@ struct BatchSink
         d = xyz(b);
         tmin = a.w;
         tmax = b.w;
+        return i;
     }
     __device__ __forceinline__ void store(uint32_t i, const HitRecord &h) const
     {
