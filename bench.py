@@ -182,7 +182,10 @@ def main():
     ap.add_argument("--nrc-train-depth", type=int, default=3,
                     help="nrc workload: end training paths at this depth with a cache query (self-training; "
                          "0: trace them out)")
-    ap.add_argument("--nrc-steps", type=int, default=4, help="nrc workload: optimizer steps per frame")
+    ap.add_argument("--nrc-steps", type=int, default=1, help="nrc workload: optimizer steps per frame")
+    ap.add_argument("--nrc-warmup", type=int, default=8,
+                    help="nrc workload: the cache runs' image restarts after this many frames, which only train "
+                         "the cache (their time counts)")
     ap.add_argument("--nrc-query-depth", type=int, default=2, help="nrc workload: depth of the cache query")
     ap.add_argument("--nrc-train-stride", type=int, default=16,
                     help="nrc workload: one pixel in this many traces a training path")
@@ -477,9 +480,11 @@ def nrc_equal_time(args, train_depths):
         curve, seconds, rays = [], 0.0, 0
         queries = records = 0
         losses = []
+        # the cache's first frames only train it: the image restarts after them
+        warm = args.nrc_warmup if cache else 0
         f = 0
         while f < args.frames or (target is not None and f < 4 * args.frames and curve[-1][2] > target):
-            p.sample_offset, p.accumulate = f, 1 if f else 0
+            p.sample_offset, p.accumulate = f, 1 if (f and f != warm) else 0
             st = ctx.render(p)
             seconds += st.seconds
             rays += st.extension_rays + st.shadow_rays
@@ -489,7 +494,7 @@ def nrc_equal_time(args, train_depths):
                 records += ns.records
                 losses.append(float(np.mean(ns.loss[:min(args.nrc_steps, max(1, ns.steps_total))])))
             f += 1
-            if f in checkpoints or (f > args.frames and f % 16 == 0):
+            if f in checkpoints or (f >= 32 and f % 8 == 0):
                 curve.append((f, round(seconds, 4), relmse()))
         return dict(seconds=seconds, rays=rays, curve=curve, queries=queries, records=records, losses=losses,
                     frames=f)
@@ -583,8 +588,9 @@ def nrc_equal_time(args, train_depths):
     return {
         "workload": f"Cornell-style room, 256x256, 1 spp per frame x {args.frames} frames, max depth {MAX_DEPTH}, "
                     f"radiance cache queried at depth {args.nrc_query_depth}, trained on 1/{args.nrc_train_stride} "
-                    f"of the pixels ({args.nrc_steps} Adam steps per frame over the frame's training records); timed "
-                    "frames replay the captured bounce loop",
+                    f"of the pixels ({args.nrc_steps} Adam steps per frame over the frame's training records); the "
+                    f"cache runs' image restarts after {args.nrc_warmup} frames that only train the cache (their "
+                    "time counts); timed frames replay the captured bounce loop",
         "ground_truth": {"spp": args.nrc_gt_spp, "sampler": "independent", "film": "fp64",
                          "split_half_relmse": gt_split, "seconds": round(gt_seconds, 2)},
         "target_relmse": target,

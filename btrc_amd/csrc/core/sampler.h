@@ -57,7 +57,8 @@ BTRC_HD uint64_t mix_bits(uint64_t v)
 }
 
 // Kensler's hashed permutation, low_discrepancy.cpp:12-45
-BTRC_HD uint32_t permutation_element(uint32_t i, uint32_t l, uint32_t p)
+// the mask Kensler's permutation works in: l - 1 with every lower bit set
+BTRC_HD uint32_t permutation_mask(uint32_t l)
 {
     uint32_t w = l - 1;
     w = w | (w >> 1);
@@ -65,6 +66,32 @@ BTRC_HD uint32_t permutation_element(uint32_t i, uint32_t l, uint32_t p)
     w = w | (w >> 4);
     w = w | (w >> 8);
     w = w | (w >> 16);
+    return w;
+}
+
+BTRC_HD uint32_t mulhi32(uint32_t a, uint32_t b)
+{
+#ifdef __HIP_DEVICE_COMPILE__
+    return __umulhi(a, b);
+#else
+    return (uint32_t)(((uint64_t)a * (uint64_t)b) >> 32);
+#endif
+}
+
+// x mod l from m = floor((2^32 - 1) / l): mulhi(x, m) > x / l - 1, so one
+// correction step gives the exact remainder
+BTRC_HD uint32_t mod_recip(uint32_t x, uint32_t l, uint32_t m)
+{
+    uint32_t r = x - mulhi32(x, m) * l;
+    if(r >= l)
+        r = r - l;
+    return r;
+}
+
+// Kensler's permutation of i over [0, l) with w = permutation_mask(l) and
+// m = floor((2^32 - 1) / l)
+BTRC_HD uint32_t permutation_element_masked(uint32_t i, uint32_t l, uint32_t w, uint32_t m, uint32_t p)
+{
     for(;;)
     {
         i = i ^ p;
@@ -88,19 +115,32 @@ BTRC_HD uint32_t permutation_element(uint32_t i, uint32_t l, uint32_t p)
         if(i < l)
             break;
     }
-    return (i + p) % l;
+    return mod_recip(i + p, l, m);
+}
+
+BTRC_HD uint32_t permutation_element(uint32_t i, uint32_t l, uint32_t p)
+{
+    return permutation_element_masked(i, l, permutation_mask(l), 0xffffffffu / l, p);
 }
 
 // Quotient and remainder of a by a small base; a stays below 2^32 for every
-// image this tracer renders, which keeps the division in 32 bits.
-BTRC_HD void divmod_base(uint64_t a, uint32_t base, uint64_t *q, uint64_t *r)
+// image this tracer renders, which keeps the division in 32 bits. There it
+// multiplies by m = floor((2^32 - 1) / base): mulhi(a, m) > a / base - 1, so
+// one correction step gives the exact quotient.
+BTRC_HD void divmod_base(uint64_t a, uint32_t base, uint32_t m, uint64_t *q, uint64_t *r)
 {
     if(a < 0x100000000ull)
     {
         const uint32_t a32 = (uint32_t)a;
-        const uint32_t q32 = a32 / base;
+        uint32_t q32 = mulhi32(a32, m);
+        uint32_t r32 = a32 - q32 * base;
+        if(r32 >= base)
+        {
+            q32 = q32 + 1;
+            r32 = r32 - base;
+        }
         *q = q32;
-        *r = a32 - q32 * base;
+        *r = r32;
     }
     else
     {
@@ -110,18 +150,21 @@ BTRC_HD void divmod_base(uint64_t a, uint32_t base, uint64_t *q, uint64_t *r)
     }
 }
 
-// owen_scrambled_radical_inverse, low_discrepancy.cpp:83-101
+// owen_scrambled_radical_inverse, low_discrepancy.cpp:83-101 (the digit
+// split's reciprocal and the permutation's mask computed once per call)
 BTRC_HD float owen_scrambled_radical_inverse(uint32_t base, uint64_t a, uint64_t hash)
 {
     const float inv_base = 1.0f / (float)base;
+    const uint32_t m = 0xffffffffu / base;
+    const uint32_t w = permutation_mask(base);
     float inv_base_m = 1.0f;
     uint64_t reversed_digits = 0;
     while(1.0f - inv_base_m < 1.0f)
     {
         uint64_t next, digit_value;
-        divmod_base(a, base, &next, &digit_value);
+        divmod_base(a, base, m, &next, &digit_value);
         const uint64_t digit_hash = mix_bits(hash ^ reversed_digits);
-        digit_value = permutation_element((uint32_t)digit_value, base, (uint32_t)digit_hash);
+        digit_value = permutation_element_masked((uint32_t)digit_value, base, w, m, (uint32_t)digit_hash);
         reversed_digits = reversed_digits * base + digit_value;
         inv_base_m = inv_base_m * inv_base;
         a = next;

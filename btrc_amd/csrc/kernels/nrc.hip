@@ -98,14 +98,36 @@ struct Frag
     u16x8 s[4];
 };
 
-// weights of all layers into LDS, rows padded to WS halves
-__device__ void load_weights_padded(const uint16_t *w, uint16_t *s_w)
+// Copies N 16-byte chunks from global memory into LDS with THREADS threads,
+// chunk i to halves s + dst(i): each thread issues BATCH loads before its
+// LDS writes (one memory latency per BATCH chunks, not per chunk)
+template <int N, int THREADS, int BATCH, class Dst>
+__device__ __forceinline__ void stage_lds(const uint4 *src, uint16_t *s, Dst dst)
 {
-    for(int i = threadIdx.x; i < NRC_ROWS * (NRC_IN / 8); i += blockDim.x)
+    static_assert(N % (THREADS * BATCH) == 0, "whole batches only");
+#pragma unroll
+    for(int i0 = 0; i0 < N; i0 += THREADS * BATCH)
     {
-        const int r = i / (NRC_IN / 8), c = i % (NRC_IN / 8);
-        *reinterpret_cast<uint4 *>(s_w + r * WS + 8 * c) = reinterpret_cast<const uint4 *>(w)[i];
+        uint4 v[BATCH];
+#pragma unroll
+        for(int k = 0; k < BATCH; ++k)
+            v[k] = src[i0 + k * THREADS + (int)threadIdx.x];
+#pragma unroll
+        for(int k = 0; k < BATCH; ++k)
+            *reinterpret_cast<uint4 *>(s + dst(i0 + k * THREADS + (int)threadIdx.x)) = v[k];
     }
+}
+
+constexpr int WEIGHT_CHUNKS = NRC_PARAMS / 8; // 16-byte chunks of one weight copy
+
+// weights of all layers into LDS as [out][in], rows padded to WS halves, by
+// a workgroup of THREADS threads
+template <int THREADS>
+__device__ __forceinline__ void load_weights_padded(const uint16_t *w, uint16_t *s_w)
+{
+    stage_lds<WEIGHT_CHUNKS, THREADS, 11>(
+        reinterpret_cast<const uint4 *>(w), s_w,
+        [](int i) { return (i / (NRC_IN / 8)) * WS + 8 * (i % (NRC_IN / 8)); });
 }
 
 // the encoded inputs of this lane's sample as the first layer's B operand:
@@ -267,7 +289,7 @@ __global__ void __launch_bounds__(INFER_THREADS) k_nrc_infer(NrcWeights net, Nrc
 {
     constexpr int T = INFER_TILES, WAVES = INFER_THREADS / WAVE, SPAN = T * TILE;
     __shared__ __attribute__((aligned(16))) uint16_t s_w[NRC_ROWS * WS];
-    load_weights_padded(net.w_bf16, s_w);
+    load_weights_padded<INFER_THREADS>(net.w_bf16, s_w);
     __syncthreads();
     const int n = *count;
     const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE, col = lane % 32, half = lane / 32;
@@ -308,7 +330,7 @@ __global__ void __launch_bounds__(256) k_nrc_eval(NrcWeights net, NrcBox box, co
                                                   float *out)
 {
     __shared__ __attribute__((aligned(16))) uint16_t s_w[NRC_ROWS * WS];
-    load_weights_padded(net.w_bf16, s_w);
+    load_weights_padded<256>(net.w_bf16, s_w);
     __syncthreads();
     const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE, col = lane % 32, half = lane / 32;
     const int i = (blockIdx.x * 4 + wave) * TILE + col;
@@ -353,19 +375,17 @@ __global__ void __launch_bounds__(128) k_nrc_train_fb(NrcWeights net, NrcBox box
         }
         return;
     }
-    load_weights_padded(net.w_bf16, s_w);
-    // [in][out] copies, per layer 64 rows of `rows` values, padded to WS
-    for(int l = 0; l < NRC_LAYERS; ++l)
-    {
-        const int rows = nrc_layer_rows(l), chunks = rows / 8;
-        const uint16_t *src = net.wt_bf16 + nrc_layer_offset(l);
-        for(int i = threadIdx.x; i < NRC_IN * chunks; i += blockDim.x)
-        {
-            const int r = i / chunks, c = i % chunks;
-            *reinterpret_cast<uint4 *>(s_wt + (l * NRC_IN + r) * WS + 8 * c) =
-                reinterpret_cast<const uint4 *>(src + r * rows)[c];
-        }
-    }
+    load_weights_padded<128>(net.w_bf16, s_w);
+    // [in][out] copies, per layer 64 rows of `rows` values (64, or the
+    // output layer's NRC_OUT_ROWS), padded to WS; the source is contiguous
+    stage_lds<WEIGHT_CHUNKS, 128, 11>(reinterpret_cast<const uint4 *>(net.wt_bf16), s_wt, [](int i) {
+        constexpr int HIDDEN = (NRC_LAYERS - 1) * NRC_IN * (NRC_WIDTH / 8); // chunks of the hidden layers
+        constexpr int OUT_CHUNKS = NRC_OUT_ROWS / 8;
+        if(i < HIDDEN)
+            return (i / (NRC_WIDTH / 8)) * WS + 8 * (i % (NRC_WIDTH / 8));
+        const int j = i - HIDDEN;
+        return ((NRC_LAYERS - 1) * NRC_IN + j / OUT_CHUNKS) * WS + 8 * (j % OUT_CHUNKS);
+    });
     __syncthreads();
 
     const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE, col = lane % 32, h = lane / 32;
