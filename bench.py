@@ -179,7 +179,7 @@ def main():
                          "views rendered together, 1 spp per frame with the radiance cache for --frames frames "
                          "(64 when not given)")
     ap.add_argument("--frames", type=int, default=None)
-    ap.add_argument("--nrc-train-depth", type=int, default=3,
+    ap.add_argument("--nrc-train-depth", type=int, default=2,
                     help="nrc workload: end training paths at this depth with a cache query (self-training; "
                          "0: trace them out)")
     ap.add_argument("--nrc-steps", type=int, default=1, help="nrc workload: optimizer steps per frame")
@@ -478,39 +478,46 @@ def nrc_equal_time(args, train_depths):
         p = params(1)
         p.time_stages = 0
         curve, seconds, rays = [], 0.0, 0
-        queries = records = 0
         losses = []
         # the cache's first frames only train it: the image restarts after them
         warm = args.nrc_warmup if cache else 0
+
+        def loss():
+            ns = ctx.nrc_stats()
+            return float(np.mean(ns.loss[:min(args.nrc_steps, max(1, ns.steps_total))]))
         f = 0
         while f < args.frames or (target is not None and f < 4 * args.frames and curve[-1][2] > target):
             p.sample_offset, p.accumulate = f, 1 if (f and f != warm) else 0
             st = ctx.render(p)
             seconds += st.seconds
             rays += st.extension_rays + st.shadow_rays
-            if cache:
-                ns = ctx.nrc_stats()
-                queries += ns.queries
-                records += ns.records
-                losses.append(float(np.mean(ns.loss[:min(args.nrc_steps, max(1, ns.steps_total))])))
+            if cache and f == 0:
+                # a frame's training overlaps the next frame's bounces; the
+                # statistics wait for it, so this wait counts as render time
+                t0 = time.perf_counter()
+                losses.append(loss())
+                seconds += time.perf_counter() - t0
             f += 1
             if f in checkpoints or (f >= 32 and f % 8 == 0):
                 curve.append((f, round(seconds, 4), relmse()))
-        return dict(seconds=seconds, rays=rays, curve=curve, queries=queries, records=records, losses=losses,
-                    frames=f)
+        if cache:
+            losses.append(loss())  # the last frame's training is not part of the image
+        return dict(seconds=seconds, rays=rays, curve=curve, losses=losses, frames=f)
 
     def staged(cache, train_depth=0, frames=32):
         configure(cache, train_depth)
         p = params(1)
         p.time_stages = 1
-        stage, queries = np.zeros(7), 0
+        stage, queries, records = np.zeros(7), 0, 0
         for f in range(frames):
             p.sample_offset, p.accumulate = f, 1 if f else 0
             st = ctx.render(p)
             stage += np.array(st.stage_ms[:7])
             if cache:
-                queries += ctx.nrc_stats().queries
-        return stage / frames, queries / frames
+                ns = ctx.nrc_stats()
+                queries += ns.queries
+                records += ns.records
+        return stage / frames, queries / frames, records / frames
 
     def time_to(curve, target):
         for frames, t, e in curve:
@@ -556,10 +563,9 @@ def nrc_equal_time(args, train_depths):
         ratios = [s["time_ratio"] for s in per_seed if s["time_ratio"] is not None]
         t_cache = time_to(c["curve"], target)
         budget = [x for x in c["curve"] if x[0] <= args.frames][-1]
-        stage, qpf = staged(True, d)
+        stage, qpf, records_pf = staged(True, d)
         infer_ms, train_ms = stage[5], stage[6]
         infer_tflops = NRC_FLOP_PER_QUERY * qpf / (infer_ms * 1e-3) / 1e12 if infer_ms > 0 else None
-        records_pf = c["records"] / c["frames"]
         train_tflops = (NRC_FLOP_PER_TRAIN_SAMPLE * records_pf / (train_ms * 1e-3) / 1e12
                         if train_ms > 0 else None)
         runs[f"train_depth_{d}"] = {
@@ -568,8 +574,8 @@ def nrc_equal_time(args, train_depths):
                                       "max": max(ratios) if ratios else None, "runs": per_seed},
             "seconds": round(c["seconds"], 4), "frames": c["frames"], "relmse_curve": c["curve"],
             "mrays_per_s": round(c["rays"] / c["seconds"] / 1e6, 2),
-            "queries_per_frame": round(c["queries"] / c["frames"]),
-            "records_per_frame": round(c["records"] / c["frames"]),
+            "queries_per_frame": round(qpf),
+            "records_per_frame": round(records_pf),
             "loss_first_last": [round(c["losses"][0], 4), round(c["losses"][-1], 4)],
             "time_to_target_s": t_cache,
             "time_ratio": None if t_cache is None else round(t_cache / plain["seconds"], 3),
@@ -584,7 +590,7 @@ def nrc_equal_time(args, train_depths):
         log(f"bench: cache (training depth {d}): time ratio {runs[f'train_depth_{d}']['time_ratio']}, "
             f"over seeds {runs[f'train_depth_{d}']['time_ratio_over_seeds']['mean']}")
     ctx.nrc_configure(enabled=0)
-    plain_stage, _ = staged(False)
+    plain_stage, _, _ = staged(False)
     return {
         "workload": f"Cornell-style room, 256x256, 1 spp per frame x {args.frames} frames, max depth {MAX_DEPTH}, "
                     f"radiance cache queried at depth {args.nrc_query_depth}, trained on 1/{args.nrc_train_stride} "

@@ -225,6 +225,11 @@ struct btrc_ctx
 {
     int device = 0;
     hipStream_t stream = nullptr;
+    // the cache's training runs on its own stream, overlapping the next
+    // frame's bounces; the next inference waits for it (train_done)
+    hipStream_t train_stream = nullptr;
+    hipEvent_t infer_done = nullptr, train_done = nullptr;
+    bool train_pending = false;       // training enqueued and not yet waited for on the host
     std::string last_error;
     std::atomic<bool> stop{ false };
     int cu_count = 256;
@@ -322,8 +327,14 @@ struct btrc_ctx
         DevBuf<float> ema;              // moving average of the weights, and its BF16 copy
         DevBuf<uint16_t> ema_b;
         DevBuf<NrcQuery> queries;
-        DevBuf<NrcRecord> rec;
-        DevBuf<float> acc;
+        // training records and path radiance sums, two sets: a frame fills
+        // one while the previous frame's training reads the other
+        DevBuf<NrcRecord> rec[2];
+        DevBuf<float> acc[2];
+        int set = 0;                    // the set of the last frame
+        int pending_set = -1;           // the set the pending training reads
+        int32_t pending_steps = 0;      // steps whose statistics are still to be read
+        bool stats_pending = false;
         DevBuf<int32_t> counts;         // [0] queries, [1] valid records of a batch, [2] valid records of the frame
         DevBuf<int32_t> index;          // the frame's valid record slots, compacted
         DevBuf<uint64_t> valid_mask;    // their flags in bijection order, one bit each
@@ -347,14 +358,19 @@ struct btrc_ctx
     // the bounce loop as a captured graph, kept while its launches stay the same
     RenderConsts *rc_host = nullptr;  // pinned
     DevBuf<RenderConsts> rc_dev;
-    hipGraphExec_t loop_exec = nullptr;
-    std::vector<uint8_t> loop_key;
+    // (two: cache frames alternate between two record sets)
+    hipGraphExec_t loop_exec[2] = { nullptr, nullptr };
+    std::vector<uint8_t> loop_key[2];
+    int loop_next = 0;                // the entry the next capture replaces
     void drop_loop_graph()
     {
-        if(loop_exec)
-            (void)hipGraphExecDestroy(loop_exec);
-        loop_exec = nullptr;
-        loop_key.clear();
+        for(int k = 0; k < 2; ++k)
+        {
+            if(loop_exec[k])
+                (void)hipGraphExecDestroy(loop_exec[k]);
+            loop_exec[k] = nullptr;
+            loop_key[k].clear();
+        }
     }
 
     // resident ray batch
@@ -1538,6 +1554,8 @@ NrcBox nrc_box(btrc_ctx *ctx)
     return b;
 }
 
+void nrc_sync(btrc_ctx *ctx);
+
 void nrc_alloc(btrc_ctx *ctx)
 {
     auto &n = ctx->nrc;
@@ -1549,8 +1567,11 @@ void nrc_alloc(btrc_ctx *ctx)
     n.wtb.alloc(NRC_PARAMS);
     n.ema.alloc(NRC_PARAMS);
     n.ema_b.alloc(NRC_PARAMS);
-    n.rec.alloc(NRC_RECORDS);
-    n.acc.alloc(4 * NRC_RECORDS);
+    for(int k = 0; k < 2; ++k)
+    {
+        n.rec[k].alloc(NRC_RECORDS);
+        n.acc[k].alloc(4 * NRC_RECORDS);
+    }
     n.counts.alloc(3);
     n.index.alloc(NRC_RECORDS);
     n.valid_mask.alloc(NRC_RECORDS / 64);
@@ -1568,6 +1589,7 @@ void nrc_alloc(btrc_ctx *ctx)
 // output layer uses sqrt(6 / (fan_in + fan_out)); padding rows are zero
 void nrc_reset(btrc_ctx *ctx, uint64_t seed)
 {
+    nrc_sync(ctx);
     nrc_alloc(ctx);
     auto &n = ctx->nrc;
     std::vector<float> w(NRC_PARAMS, 0.0f);
@@ -1626,16 +1648,15 @@ NrcScratch nrc_scratch(btrc_ctx *ctx, int64_t batch)
 
 // `steps` optimizer steps over the records, `batch` at a time: step k
 // takes the k-th batch (rotating through all records), or always the first
-void nrc_train(btrc_ctx *ctx, int32_t steps, int64_t batch, bool rotate)
+void nrc_train(btrc_ctx *ctx, hipStream_t s, int set, int32_t steps, int64_t batch, bool rotate)
 {
     auto &n = ctx->nrc;
-    hipStream_t s = ctx->stream;
     const int64_t batches = NRC_RECORDS / batch;
     for(int32_t k = 0; k < steps; ++k)
     {
         // a rendered frame's steps take equal parts of its valid records
         // (compacted after the bounce loop); fits take the loaded batch
-        NrcBatch b{ n.rec.ptr, rotate ? (k % batches) * batch : 0, batch, n.acc.ptr, n.counts.ptr + 1,
+        NrcBatch b{ n.rec[set].ptr, rotate ? (k % batches) * batch : 0, batch, n.acc[set].ptr, n.counts.ptr + 1,
                     n.params.loss_scale, rotate ? 1 : 0, rotate ? n.index.ptr : nullptr,
                     rotate ? n.counts.ptr + 2 : nullptr, k, steps };
         launch_nrc_train_step(s, nrc_weights(ctx), nrc_box(ctx), b, nrc_scratch(ctx, batch), nrc_adam(ctx),
@@ -1669,6 +1690,24 @@ void nrc_collect(btrc_ctx *ctx, int32_t steps)
         r.skipped += st[k].skipped;
     }
     n.last = r;
+}
+
+// Waits for the cache training still running on the training stream and
+// reads the last frame's statistics; every cache entry point calls it first
+void nrc_sync(btrc_ctx *ctx)
+{
+    auto &n = ctx->nrc;
+    if(ctx->train_pending)
+    {
+        BTRC_HIP_CHECK(hipStreamSynchronize(ctx->train_stream));
+        ctx->train_pending = false;
+        n.pending_set = -1;
+    }
+    if(n.stats_pending)
+    {
+        n.stats_pending = false;
+        nrc_collect(ctx, n.pending_steps);
+    }
 }
 
 // The views of a render: `n` cameras with film sizes, or (views null) the
@@ -1861,10 +1900,14 @@ void render(btrc_ctx *ctx, const btrc_render_params *p, int32_t view_count, cons
         if(!n.params.train)
             rc.nrc_stride = 0;
         n.queries.alloc(std::max<size_t>(1, (size_t)pixels));
+        // this frame's record set; a training still reading it finishes first
+        n.set ^= 1;
+        if(ctx->train_pending && n.pending_set == n.set)
+            BTRC_HIP_CHECK(hipStreamWaitEvent(s, ctx->train_done, 0));
         nv = NrcView{ n.params.query_depth, slots, n.params.train_depth > 0 ? n.params.train_depth : -1,
-                      n.queries.ptr, n.counts.ptr, n.rec.ptr, n.acc.ptr };
-        launch_nrc_clear(s, n.rec.ptr, NRC_RECORDS, n.acc.ptr, NRC_RECORDS);
-        BTRC_HIP_CHECK(hipMemsetAsync(n.counts.ptr, 0, 2 * sizeof(int32_t), s));
+                      n.queries.ptr, n.counts.ptr, n.rec[n.set].ptr, n.acc[n.set].ptr };
+        launch_nrc_clear(s, n.rec[n.set].ptr, NRC_RECORDS, n.acc[n.set].ptr, NRC_RECORDS);
+        BTRC_HIP_CHECK(hipMemsetAsync(n.counts.ptr, 0, sizeof(int32_t), s)); // the query count
     }
     rc.filter = ctx->filter;
 
@@ -1888,7 +1931,7 @@ void render(btrc_ctx *ctx, const btrc_render_params *p, int32_t view_count, cons
     if(nrc_on)
         ctx->sh_train.alloc(SS);
     ShadowSOA sh{ ctx->sh_o.ptr, ctx->sh_d.ptr, ctx->sh_beta.ptr, media ? ctx->sh_seed.ptr : nullptr,
-                  nrc_on ? ctx->sh_train.ptr : nullptr, nrc_on ? ctx->nrc.acc.ptr : nullptr };
+                  nrc_on ? ctx->sh_train.ptr : nullptr, nrc_on ? ctx->nrc.acc[ctx->nrc.set].ptr : nullptr };
 
     const int grid256 = ctx->persistent_grid(256, (int64_t)P);
     const TraceOverflow ovf = ctx->trace_overflow();
@@ -1948,28 +1991,37 @@ void render(btrc_ctx *ctx, const btrc_render_params *p, int32_t view_count, cons
         const int32_t ints[8] = { grid256, ctx->trace_grid, ctx->trace_sched(false), ctx->trace_sched(true),
                                   p->sampler, p->state_count, media ? 1 : 0, check_every };
         add(ints);
-        if(!ctx->loop_exec || key != ctx->loop_key)
+        int g = -1;
+        for(int k = 0; k < 2; ++k)
+            if(ctx->loop_exec[k] && key == ctx->loop_key[k])
+                g = k;
+        if(g < 0)
         {
-            ctx->drop_loop_graph();
+            g = ctx->loop_next;
+            ctx->loop_next ^= 1;
+            if(ctx->loop_exec[g])
+                (void)hipGraphExecDestroy(ctx->loop_exec[g]);
+            ctx->loop_exec[g] = nullptr;
+            ctx->loop_key[g].clear();
             hipGraph_t graph = nullptr;
             BTRC_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
             for(int k = 0; k < check_every; ++k)
                 enqueue_bounce(k & 1, false);
             BTRC_HIP_CHECK(hipStreamEndCapture(s, &graph));
-            const hipError_t e = hipGraphInstantiate(&ctx->loop_exec, graph, nullptr, nullptr, 0);
+            const hipError_t e = hipGraphInstantiate(&ctx->loop_exec[g], graph, nullptr, nullptr, 0);
             (void)hipGraphDestroy(graph);
             if(e != hipSuccess)
             {
-                ctx->loop_exec = nullptr;
+                ctx->loop_exec[g] = nullptr;
                 throw HipError(std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
             }
-            ctx->loop_key = key;
+            ctx->loop_key[g] = key;
         }
         for(;;)
         {
             if(ctx->stop.load())
                 throw Stopped("render stopped");
-            BTRC_HIP_CHECK(hipGraphLaunch(ctx->loop_exec, s));
+            BTRC_HIP_CHECK(hipGraphLaunch(ctx->loop_exec[g], s));
             BTRC_HIP_CHECK(hipMemcpyAsync(ctx->host_counters, ctx->wc(), sizeof(WaveCounters), hipMemcpyDeviceToHost, s));
             BTRC_HIP_CHECK(hipStreamSynchronize(s));
             const WaveCounters &hc = *ctx->host_counters;
@@ -2044,24 +2096,42 @@ void render(btrc_ctx *ctx, const btrc_render_params *p, int32_t view_count, cons
         auto &n = ctx->nrc;
         // the cache renders one sample per pixel and a path makes at most
         // one query: no two queries share a pixel
+        // the weights the previous frame's training writes
+        if(ctx->train_pending)
+            BTRC_HIP_CHECK(hipStreamWaitEvent(s, ctx->train_done, 0));
         timer.run(STAGE_CACHE, [&] {
             launch_nrc_infer(s, ctx->cu_count, (int64_t)pixels, nrc_render_weights(ctx), nrc_box(ctx), n.queries.ptr,
-                             n.counts.ptr, ctx->film.ptr, p->film_fp64 ? ctx->film64.ptr : nullptr, n.acc.ptr);
+                             n.counts.ptr, ctx->film.ptr, p->film_fp64 ? ctx->film64.ptr : nullptr,
+                             n.acc[n.set].ptr);
         });
         if(n.params.train)
         {
             nrc_steps = std::min(n.params.steps, 64);
-            timer.run(STAGE_TRAIN, [&] {
-                launch_nrc_compact(s, n.rec.ptr, n.valid_mask.ptr, n.index.ptr, n.counts.ptr + 2);
-                nrc_train(ctx, nrc_steps, NRC_BATCH, true);
-            });
+            const int set = n.set;
+            auto train = [&](hipStream_t ts) {
+                launch_nrc_compact(ts, n.rec[set].ptr, n.valid_mask.ptr, n.index.ptr, n.counts.ptr + 2);
+                nrc_train(ctx, ts, set, nrc_steps, NRC_BATCH, true);
+            };
+            if(timer.enabled)
+                timer.run(STAGE_TRAIN, [&] { train(s); });
+            else
+            {
+                // on the training stream, after this frame's inference; the
+                // render returns without waiting for it
+                BTRC_HIP_CHECK(hipEventRecord(ctx->infer_done, s));
+                BTRC_HIP_CHECK(hipStreamWaitEvent(ctx->train_stream, ctx->infer_done, 0));
+                train(ctx->train_stream);
+                BTRC_HIP_CHECK(hipEventRecord(ctx->train_done, ctx->train_stream));
+                ctx->train_pending = true;
+                n.pending_set = set;
+            }
         }
+        n.pending_steps = nrc_steps;
+        n.stats_pending = true;
         BTRC_HIP_CHECK(hipGetLastError());
     }
     BTRC_HIP_CHECK(hipStreamSynchronize(s));
     const auto t1 = std::chrono::steady_clock::now();
-    if(nrc_on)
-        nrc_collect(ctx, nrc_steps);
     if(capture_count)
         *capture_count = captured;
     if(stats)
@@ -2187,9 +2257,12 @@ int btrc_hip_create(int device, btrc_ctx **out)
     if(!ctx)
         return BTRC_ERR_OOM;
     ctx->device = device;
-    if(hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess)
+    if(hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
+       hipStreamCreateWithFlags(&ctx->train_stream, hipStreamNonBlocking) != hipSuccess ||
+       hipEventCreateWithFlags(&ctx->infer_done, hipEventDisableTiming) != hipSuccess ||
+       hipEventCreateWithFlags(&ctx->train_done, hipEventDisableTiming) != hipSuccess)
     {
-        delete ctx;
+        btrc_hip_destroy(ctx);
         return BTRC_ERR_HIP;
     }
     hipDeviceProp_t prop;
@@ -2210,11 +2283,19 @@ void btrc_hip_destroy(btrc_ctx *ctx)
     (void)hipSetDevice(ctx->device);
     if(ctx->stream)
         (void)hipStreamSynchronize(ctx->stream);
+    if(ctx->train_stream)
+        (void)hipStreamSynchronize(ctx->train_stream);
     if(ctx->host_counters)
         (void)hipHostFree(ctx->host_counters);
     if(ctx->rc_host)
         (void)hipHostFree(ctx->rc_host);
     ctx->drop_loop_graph();
+    if(ctx->infer_done)
+        (void)hipEventDestroy(ctx->infer_done);
+    if(ctx->train_done)
+        (void)hipEventDestroy(ctx->train_done);
+    if(ctx->train_stream)
+        (void)hipStreamDestroy(ctx->train_stream);
     if(ctx->stream)
         (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -2470,9 +2551,10 @@ int btrc_hip_nrc_configure(btrc_ctx *ctx, const btrc_nrc_params *params)
         const btrc_nrc_params &q = *params;
         if(q.query_depth < 0 || q.train_stride < 1 || q.steps < 0 || q.steps > 64 || !(q.learning_rate > 0.0f) ||
            !(q.loss_scale > 0.0f) || !std::isfinite(q.learning_rate) || !std::isfinite(q.loss_scale) ||
-           q.train_depth < 0 || (q.train_depth > 0 && q.train_depth <= q.query_depth) || !(q.ema_decay >= 0.0f) ||
+           q.train_depth < 0 || (q.train_depth > 0 && q.train_depth < q.query_depth) || !(q.ema_decay >= 0.0f) ||
            !(q.ema_decay < 1.0f))
             throw InvalidArg("bad cache parameters");
+        nrc_sync(ctx);
         BTRC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
         ctx->nrc.params = q;
         if(q.enabled)
@@ -2490,6 +2572,7 @@ int btrc_hip_nrc_stats(btrc_ctx *ctx, btrc_nrc_stats *stats)
     return guarded(ctx, [&] {
         if(!stats)
             throw InvalidArg("stats is null");
+        nrc_sync(ctx);
         *stats = ctx->nrc.last;
     });
 }
@@ -2500,6 +2583,7 @@ int btrc_hip_nrc_get_weights(btrc_ctx *ctx, float *weights)
         if(!weights)
             throw InvalidArg("weights is null");
         nrc_require(ctx);
+        nrc_sync(ctx);
         BTRC_HIP_CHECK(hipMemcpyAsync(weights, ctx->nrc.w.ptr, NRC_PARAMS * sizeof(float), hipMemcpyDeviceToHost,
                                       ctx->stream));
         BTRC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
@@ -2512,6 +2596,7 @@ int btrc_hip_nrc_get_ema_weights(btrc_ctx *ctx, float *weights)
         if(!weights)
             throw InvalidArg("weights is null");
         nrc_require(ctx);
+        nrc_sync(ctx);
         BTRC_HIP_CHECK(hipMemcpyAsync(weights, ctx->nrc.ema.ptr, NRC_PARAMS * sizeof(float), hipMemcpyDeviceToHost,
                                       ctx->stream));
         BTRC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
@@ -2524,6 +2609,7 @@ int btrc_hip_nrc_set_weights(btrc_ctx *ctx, const float *weights)
         if(!weights)
             throw InvalidArg("weights is null");
         nrc_require(ctx);
+        nrc_sync(ctx);
         // the output layer's padding rows stay zero
         std::vector<float> w(weights, weights + NRC_PARAMS);
         for(int k = nrc_layer_offset(NRC_LAYERS - 1) + NRC_OUT * NRC_IN; k < NRC_PARAMS; ++k)
@@ -2541,6 +2627,7 @@ int btrc_hip_nrc_eval(btrc_ctx *ctx, const float *vertices, int32_t n, float *ou
         if(n < 0 || (n > 0 && (!vertices || !out)))
             throw InvalidArg("bad cache query");
         nrc_require(ctx);
+        nrc_sync(ctx);
         if(n == 0)
             return;
         DevBuf<NrcVertex> dv;
@@ -2572,8 +2659,8 @@ void nrc_load_batch(btrc_ctx *ctx, const float *vertices, const float *targets, 
         for(int c = 0; c < 3; ++c)
             acc[4 * (size_t)i + c] = targets[3 * (size_t)i + c];
     }
-    nc.rec.upload(rec.data(), rec.size(), ctx->stream);
-    nc.acc.upload(acc.data(), acc.size(), ctx->stream);
+    nc.rec[0].upload(rec.data(), rec.size(), ctx->stream);
+    nc.acc[0].upload(acc.data(), acc.size(), ctx->stream);
 }
 
 int btrc_hip_nrc_gradients(btrc_ctx *ctx, const float *vertices, const float *targets, int32_t n, float *grad,
@@ -2583,10 +2670,11 @@ int btrc_hip_nrc_gradients(btrc_ctx *ctx, const float *vertices, const float *ta
         if(n <= 0 || n > NRC_RECORDS || !vertices || !targets || !grad)
             throw InvalidArg("bad cache batch");
         nrc_require(ctx);
+        nrc_sync(ctx);
         nrc_load_batch(ctx, vertices, targets, n);
         auto &nc = ctx->nrc;
         const int64_t batch = ((int64_t)n + 63) / 64 * 64;
-        NrcBatch b{ nc.rec.ptr, 0, batch, nc.acc.ptr, nc.counts.ptr + 1, nc.params.loss_scale, 0, nullptr, nullptr,
+        NrcBatch b{ nc.rec[0].ptr, 0, batch, nc.acc[0].ptr, nc.counts.ptr + 1, nc.params.loss_scale, 0, nullptr, nullptr,
                     0, 1 };
         launch_nrc_gradients(ctx->stream, nrc_weights(ctx), nrc_box(ctx), b, nrc_scratch(ctx, batch), nc.grad.ptr);
         BTRC_HIP_CHECK(hipGetLastError());
@@ -2616,8 +2704,9 @@ int btrc_hip_nrc_fit(btrc_ctx *ctx, const float *vertices, const float *targets,
         if(n <= 0 || n > NRC_BATCH || steps < 0 || steps > 64 || !vertices || !targets)
             throw InvalidArg("bad cache batch");
         nrc_require(ctx);
+        nrc_sync(ctx);
         nrc_load_batch(ctx, vertices, targets, n);
-        nrc_train(ctx, steps, NRC_BATCH, false);
+        nrc_train(ctx, ctx->stream, 0, steps, NRC_BATCH, false);
         nrc_collect(ctx, steps);
         if(loss)
         {

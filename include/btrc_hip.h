@@ -297,10 +297,12 @@ typedef struct btrc_nrc_params
     float learning_rate;    /* default 1e-2 */
     float loss_scale;       /* gradient scale of the BF16 backward pass, default 128 */
     /* 0 (default): training paths are traced to the end, so targets are
-     * unbiased path estimates. k > query_depth: training paths end at depth
+     * unbiased path estimates. k >= query_depth: training paths end at depth
      * k with a cache query whose answer joins their radiance sums, so the
      * bounce loop ends after about k bounces (the cache trains on its own
-     * tails, as the radiance cache paper's self-training does) */
+     * tails, as the radiance cache paper's self-training does); their
+     * vertices at depths 0..k-1 are the training records. k = query_depth
+     * adds no bounce beyond the rendered paths' own */
     int32_t train_depth;
     /* 0 (default): frames query the trained weights. d in (0, 1): they
      * query an exponential moving average e <- d e + (1 - d) w, updated after

@@ -168,7 +168,15 @@ def test_self_training_ends_training_paths_early(cornell_ctx):
     assert short.iterations < full.iterations
     import btrc_amd
     with pytest.raises(btrc_amd.BtrcException):
-        ctx.nrc_configure(train_depth=2)        # not beyond the query depth
+        ctx.nrc_configure(train_depth=1)        # before the query depth
+    # at the query depth: training paths end with the rendered paths, no
+    # bounce beyond theirs
+    ctx.nrc_configure(enabled=1, train=1, train_depth=2)
+    ctx.nrc_reset(9)
+    same = ctx.render(p)
+    assert same.iterations <= short.iterations
+    ns = ctx.nrc_stats()
+    assert ns.records > 0 and all(math.isfinite(x) for x in ns.loss[:4])
     ctx.nrc_configure(train_depth=4)
     img, stats = cache_frames(ctx, 96, 0, w, h)
     assert all(math.isfinite(x) for s in stats for x in s.loss)
