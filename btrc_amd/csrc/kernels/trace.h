@@ -40,6 +40,7 @@ constexpr int TRACE_WAVES_PER_CU_MAX = 32;
 constexpr int COUNTER_STRIPES = 16;
 constexpr int COUNTER_STRIDE = 32;
 constexpr int FETCH_WORDS = COUNTER_STRIPES * COUNTER_STRIDE; // a traversal's fetch counters
+constexpr int GRANULE = 64; // rays a traversal wavefront takes from a chunk counter at a time
 
 // traversal schedules: leaves tested when reached, leaves postponed until
 // every lane holds one, or the two-level (instance) structure
@@ -112,7 +113,7 @@ __device__ __forceinline__ uint32_t leaf_code(int32_t first, int32_t count)
 //
 // Traces rays [0, n): each wavefront's first TRACE_BLOCK rays by its
 // workgroup index, the rest [gridDim.x * TRACE_BLOCK, n) in granules of
-// TRACE_BLOCK rays dealt round robin to C = min(COUNTER_STRIPES, gridDim.x)
+// GRANULE rays dealt round robin to C = min(COUNTER_STRIPES, gridDim.x)
 // chunks, chunk k's granules handed out by counter fetch[k * COUNTER_STRIDE]
 // (all FETCH_WORDS words start at 0) to the wavefronts of workgroups
 // b = k (mod C). Sink provides
@@ -149,7 +150,7 @@ __device__ __forceinline__ void trace_persistent(
     // Rays [0, static_span) go out as one batch of TRACE_BLOCK per wavefront
     // (workgroup b's first refill takes rays TRACE_BLOCK b + 0..63), so a
     // small launch takes no atomics. The rest go out in granules of
-    // TRACE_BLOCK rays: granule g belongs to chunk g % C, and chunk k's
+    // GRANULE rays: granule g belongs to chunk g % C, and chunk k's
     // counter hands its granules out in order to the wavefronts of
     // workgroups b = k (mod C). Every chunk spans the whole queue (balanced
     // work) and the granules in flight stay close together (coherent rays).
@@ -268,35 +269,51 @@ __device__ __forceinline__ void trace_persistent(
         const int nidle = (int)__popcll(idle);
         if(nidle >= REFILL_MIN && !exhausted)
         {
-            if(res_lo == res_hi && !dry)
+            // first the rest of the reserved granule, then, for lanes still
+            // idle, the home chunk's next granule (every lane runs this
+            // loop, so lane 0 takes it for the wavefront)
+            const uint32_t base_a = res_lo;
+            const uint32_t take_a = min((uint32_t)nidle, res_hi - res_lo);
+            res_lo += take_a;
+            uint32_t base_b = 0, take_b = 0;
+            if(take_a < (uint32_t)nidle && !dry)
             {
-                // the home chunk's next granule (every lane runs this loop,
-                // so lane 0 takes it for the wavefront)
                 uint32_t g = 0;
                 if(lane == 0)
                     g = atomicAdd(counter, 1u);
                 g = __builtin_amdgcn_readfirstlane(g);
-                const uint64_t lo = (uint64_t)static_span + ((uint64_t)g * chunks + home) * TRACE_BLOCK;
+                const uint64_t lo = (uint64_t)static_span + ((uint64_t)g * chunks + home) * GRANULE;
                 if(lo < (uint64_t)n)
                 {
                     res_lo = (uint32_t)lo;
-                    res_hi = min((uint32_t)n, res_lo + (uint32_t)TRACE_BLOCK);
+                    res_hi = min((uint32_t)n, res_lo + (uint32_t)GRANULE);
+                    base_b = res_lo;
+                    take_b = min((uint32_t)nidle - take_a, res_hi - res_lo);
+                    res_lo += take_b;
                 }
                 else
                     dry = true;
             }
-            const uint32_t base = res_lo;                          // this refill's rays: base + 0 .. take - 1
-            const uint32_t take = min((uint32_t)nidle, res_hi - res_lo);
-            res_lo += take;
             exhausted = dry && res_lo == res_hi;
             if(!active)
             {
+                // idle lane number `rank` takes ray base_a + rank, or ray
+                // base_b + rank - take_a of the new granule
                 const uint32_t rank = (uint32_t)__popcll(idle & lanes_below);
-                if(rank < take)
+                bool got = false;
+                float tmax = 0.0f;
+                if(take_a > 0 && rank < take_a)
                 {
-                    const uint32_t idx = base + rank;
-                    float tmax;
-                    ray = sink.load(base, idx, o, d, tmin, tmax);
+                    ray = sink.load(base_a, base_a + rank, o, d, tmin, tmax);
+                    got = true;
+                }
+                if(take_b > 0 && rank >= take_a && rank < take_a + take_b)
+                {
+                    ray = sink.load(base_b, base_b + (rank - take_a), o, d, tmin, tmax);
+                    got = true;
+                }
+                if(got)
+                {
                     set_ray(o, d);
                     margin = cull_margin(sc.extent, o);
                     cur_inst = 0xffffffffu;
