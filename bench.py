@@ -307,8 +307,9 @@ def main():
 
     cpu = None
     bytes_per_ray = None
+    medium_stats = None
     if not args.no_cpu and media:
-        cpu = media_cpu_baseline(scene, cam, p, args.cpu_seconds)
+        cpu, medium_stats = media_cpu_baseline(scene, cam, p, args.cpu_seconds)
     elif not args.no_cpu:
         # rays of this frame for the work model and the CPU baseline
         log("bench: capturing rays of the first two bounces")
@@ -325,8 +326,8 @@ def main():
             cpu["isects_per_ray"] = float(np.mean(stats[1]))
 
     roofline = None
-    if media and staged["stage_ms"][4] > 0:
-        roofline = medium_roofline(scene, cam, p, staged)
+    if media and staged["stage_ms"][4] > 0 and medium_stats is not None:
+        roofline = medium_roofline(p, staged, medium_stats)
     elif bytes_per_ray is not None and trace_ms > 0:
         launches = max(1, int(staged["launches"][1]))
         rays_per_launch = staged["ext"] / launches
@@ -622,7 +623,7 @@ def nrc_workload(args):
         p.width, p.height, p.spp = 256, 256, 16
         p.min_depth, p.max_depth = MIN_DEPTH, MAX_DEPTH
         p.rr_threshold, p.rr_cont_prob = 0.1, 0.5
-        cpu = media_cpu_baseline(scene, cam, p, args.cpu_seconds)
+        cpu, _ = media_cpu_baseline(scene, cam, p, args.cpu_seconds)
     return {
         "metric": METRIC,
         "value": main_run["mrays_per_s"],
@@ -829,24 +830,14 @@ MEDIUM_BYTES_PER_STATE = 204.0   # a path state and its hit read by the medium s
 MEDIUM_BYTES_PER_LOOKUP = 32.0   # one trilinear density fetch: eight FP32 voxels
 
 
-def medium_roofline(scene, cam, p, staged, bands=8, rows=2):
+def medium_roofline(p, staged, stats):
     """Work model of the medium stage (k_medium): 204 B per path state it
     processes (one per extension ray) plus 32 B per density lookup of delta
-    tracking. Lookups per path come from the oracle's medium stage on the
-    frame's own paths (`bands` bands of `rows` image rows spread over the
-    frame, sample 0); the stage's time is the stage-timed frame's."""
-    from oracle import oracle
-    os_ = oracle.OracleScene(scene)
-    oracle.tracking_stats()
-    W, H = p.width, p.height
-    paths = 0
-    for k in range(bands):
-        r0 = (2 * k + 1) * H // (2 * bands)
-        os_.render(cam, p, first=r0 * W, count=rows * W, threads=host_threads())
-        paths += rows * W
-    segments, lookups = oracle.tracking_stats()
-    lookups_per_path = lookups / paths
-    frame_paths = W * H * p.spp
+    tracking. Lookups per path come from the CPU baseline's run of the same
+    algorithm on paths spread over the frame (media_cpu_baseline); the
+    stage's time is the stage-timed frame's."""
+    lookups_per_path = stats["lookups_per_path"]
+    frame_paths = p.width * p.height * p.spp
     launches = max(1, int(staged["launches"][4]))
     total_bytes = MEDIUM_BYTES_PER_STATE * staged["ext"] + MEDIUM_BYTES_PER_LOOKUP * lookups_per_path * frame_paths
     stage_s = staged["stage_ms"][4] * 1e-3
@@ -857,29 +848,40 @@ def medium_roofline(scene, cam, p, staged, bands=8, rows=2):
                 launch_ms=round(staged["stage_ms"][4] / launches, 4),
                 bytes_per_state=MEDIUM_BYTES_PER_STATE, bytes_per_lookup=MEDIUM_BYTES_PER_LOOKUP,
                 lookups_per_path=round(lookups_per_path, 3),
-                tracking_segments_per_path=round(segments / paths, 3),
-                sample=f"oracle medium stage on {paths} paths of the frame ({bands} bands of {rows} rows, sample 0)")
+                tracking_segments_per_path=round(stats["segments_per_path"], 3),
+                sample=stats["sample"])
 
 
-def media_cpu_baseline(scene, cam, p, budget_s):
-    """the oracle's scalar path tracer (a port of the same algorithm) on 16
-    host threads over the first paths of the frame, about budget_s seconds"""
+def media_cpu_baseline(scene, cam, p, budget_s, bands=8):
+    """CPU baseline of the media workload and the statistics of its work
+    model: the oracle's scalar path tracer (a port of the same algorithm) on
+    the host threads over sample-0 paths of `bands` bands of image rows
+    spread over the frame, a few more rows per band each pass until about
+    budget_s seconds, counting its tracking segments and density lookups"""
     from oracle import oracle
     os_ = oracle.OracleScene(scene)
     threads = host_threads()
-    q = type(p).from_buffer_copy(p)
-    count, rays, dt = 1 << 14, 0, 0.0
-    done = 0
-    while dt < budget_s and done + count <= q.width * q.height * q.spp:
-        t0 = time.perf_counter()
-        _, e, s = os_.render(cam, q, first=done, count=count, threads=threads)
-        dt += time.perf_counter() - t0
-        rays += e + s
-        done += count
-        count *= 2
-    return dict(value=rays / dt / 1e6, unit="Mrays/s", cores=threads, kind="port",
-                sample=f"first {done} paths of the frame traced by the oracle's scalar path tracer "
-                       f"(oracle/oracle.cpp) on {threads} host threads, {dt:.1f} s", host=host_info(threads))
+    W, H = p.width, p.height
+    band_rows = H // bands
+    oracle.tracking_stats()
+    done_rows, rows, rays, dt, paths = 0, 1, 0, 0.0, 0
+    while dt < budget_s and done_rows < band_rows:
+        rows = min(rows, band_rows - done_rows)
+        for k in range(bands):
+            r0 = k * band_rows + done_rows
+            t0 = time.perf_counter()
+            _, e, s = os_.render(cam, p, first=r0 * W, count=rows * W, threads=threads)
+            dt += time.perf_counter() - t0
+            rays += e + s
+            paths += rows * W
+        done_rows += rows
+        rows *= 2
+    segments, lookups = oracle.tracking_stats()
+    sample = (f"{paths} sample-0 paths in {bands} bands of {done_rows} image rows spread over the frame, traced by "
+              f"the oracle's scalar path tracer (oracle/oracle.cpp) on {threads} host threads, {dt:.1f} s")
+    cpu = dict(value=rays / dt / 1e6, unit="Mrays/s", cores=threads, kind="port", sample=sample,
+               host=host_info(threads))
+    return cpu, dict(lookups_per_path=lookups / paths, segments_per_path=segments / paths, sample=sample)
 
 
 def flatten_instances(scene):
