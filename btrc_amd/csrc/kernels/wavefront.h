@@ -68,17 +68,20 @@ struct WaveCounters
 // COUNTER_STRIPES * stripe >= n) at slots k * stripe + 0, 1, ... from its
 // own counter. A pool side therefore holds the states of the last bounce in
 // COUNTER_STRIPES segments and, after them, the states generated for this
-// bounce at slot pool_gen_slot(state_count); a SegmentMap turns dense
-// indices into slots.
+// bounce at slot pool_gen_slot(state_count); pool_slot turns dense
+// indices into slots. The shadow rays of bounce b are traced beside the
+// next bounce's plan, generate and trace (on another stream), so what they
+// use is kept per bounce parity: the stripe size, the shadow stripe counters
+// and the shadow fetch counters.
 constexpr int POOL_SEGMENTS = COUNTER_STRIPES + 1;
 
 struct QueueState
 {
-    int32_t next_n[COUNTER_STRIPES * COUNTER_STRIDE];   // states pushed into the other pool side, per stripe
-    int32_t shadow_n[COUNTER_STRIPES * COUNTER_STRIDE]; // shadow rays pushed, per stripe
-    uint32_t trace_fetch[FETCH_WORDS];                  // extension-ray chunk counters
-    uint32_t shadow_fetch[FETCH_WORDS];                 // shadow-ray chunk counters
-    int32_t stripe;                                     // inputs per stripe of this bounce
+    int32_t next_n[COUNTER_STRIPES * COUNTER_STRIDE];      // states pushed into the other pool side, per stripe
+    int32_t shadow_n[2][COUNTER_STRIPES * COUNTER_STRIDE]; // shadow rays pushed, per parity and stripe
+    uint32_t trace_fetch[FETCH_WORDS];                     // extension-ray chunk counters
+    uint32_t shadow_fetch[2][FETCH_WORDS];                 // shadow-ray chunk counters, per parity
+    int32_t stripe[2];                                     // inputs per stripe of a bounce, per parity
     int32_t pool_first[POOL_SEGMENTS];                  // slot of each segment's first state
     int32_t pool_prefix[POOL_SEGMENTS + 1];             // dense index of each segment's first state
 };
@@ -119,16 +122,16 @@ __device__ __forceinline__ int32_t pool_slot(const QueueState *q, int32_t j0, in
     return slot;
 }
 
-// the same for this bounce's shadow rays, whose segments (stripe k at slot
-// k * stripe, in stripe order) follow from the stripe counters
-__device__ __forceinline__ int32_t shadow_slot(const QueueState *q, int32_t j0, int32_t j)
+// the same for the shadow rays of the bounce of this parity, whose segments
+// (stripe k at slot k * stripe, in stripe order) follow from the counters
+__device__ __forceinline__ int32_t shadow_slot(const QueueState *q, int parity, int32_t j0, int32_t j)
 {
     j0 = __builtin_amdgcn_readfirstlane(j0);
-    const int32_t stripe = q->stripe;
+    const int32_t stripe = q->stripe[parity];
     int32_t slot = j, first = 0;
     for(int k = 0; k < COUNTER_STRIPES && first <= j0 + 63; ++k)
     {
-        const int32_t count = q->shadow_n[k * COUNTER_STRIDE];
+        const int32_t count = q->shadow_n[parity][k * COUNTER_STRIDE];
         if(first + count > j0 && j >= first)
             slot = j - first + k * stripe;
         first += count;
@@ -136,12 +139,12 @@ __device__ __forceinline__ int32_t shadow_slot(const QueueState *q, int32_t j0, 
     return slot;
 }
 
-// the number of shadow rays pushed this bounce
-__device__ __forceinline__ int32_t shadow_count(const QueueState *q)
+// the number of shadow rays pushed by the bounce of this parity
+__device__ __forceinline__ int32_t shadow_count(const QueueState *q, int parity)
 {
     int32_t n = 0;
     for(int k = 0; k < COUNTER_STRIPES; ++k)
-        n += q->shadow_n[k * COUNTER_STRIDE];
+        n += q->shadow_n[parity][k * COUNTER_STRIDE];
     return n;
 }
 
@@ -243,19 +246,21 @@ enum : int
     STAGE_COUNT = 7,
 };
 
-void launch_plan(hipStream_t s, WaveCounters *c, int32_t state_count);
+// parity: the bounce's parity (the pool side it reads)
+void launch_plan(hipStream_t s, WaveCounters *c, int32_t state_count, int parity);
 void launch_generate(hipStream_t s, int grid, int sampler, const RenderConsts *rc, const WaveCounters *c,
                      PathSOA path);
 void launch_trace_extension(hipStream_t s, int grid, int sched, const TraceScene &ts, WaveCounters *c,
                             PathSOA path, HitSOA hit, const TraceOverflow &ovf);
 void launch_shade(hipStream_t s, int grid, int sampler, const RenderConsts *rc, const SceneView &sv,
                   WaveCounters *c, PathSOA path, HitSOA hit, PathSOA next, ShadowSOA shadow, FilmView film,
-                  const NrcView &nrc);
+                  const NrcView &nrc, int parity);
 void launch_medium(hipStream_t s, int grid, int sampler, const RenderConsts *rc, const SceneView &sv,
-                   WaveCounters *c, PathSOA path, HitSOA hit, PathSOA next, ShadowSOA shadow, FilmView film);
+                   WaveCounters *c, PathSOA path, HitSOA hit, PathSOA next, ShadowSOA shadow, FilmView film,
+                   int parity);
 void launch_shadow(hipStream_t s, int grid, int sched, bool media, const TraceScene &ts, const SceneView &sv,
-                   WaveCounters *c, ShadowSOA shadow, FilmView film, const TraceOverflow &ovf);
-void launch_end_iteration(hipStream_t s, WaveCounters *c);
+                   WaveCounters *c, ShadowSOA shadow, FilmView film, const TraceOverflow &ovf, int parity);
+void launch_end_iteration(hipStream_t s, WaveCounters *c, int parity);
 void launch_resolve(hipStream_t s, int mode, const float *value, int32_t pixels, float inv_weight, float4 *out);
 
 // ray batches of the traversal entry points: o.xyz/tmin and d.xyz/tmax

@@ -166,7 +166,7 @@ struct TrSampler
 // GeneratePipeline::generate's refill rule (generate.cpp:140-176), evaluated
 // on the device: refill when at least half of the pool is free or when all
 // remaining paths fit.
-__global__ void k_plan(WaveCounters *c, int32_t state_count)
+__global__ void k_plan(WaveCounters *c, int32_t state_count, int parity)
 {
     QueueState *q = queue_state(c);
     const int32_t active = c->active;
@@ -179,10 +179,7 @@ __global__ void k_plan(WaveCounters *c, int32_t state_count)
             fresh = unfinished < rest ? unfinished : rest;
     }
     for(int k = 0; k < FETCH_WORDS; k += COUNTER_STRIDE)
-    {
         q->trace_fetch[k] = 0;
-        q->shadow_fetch[k] = 0;
-    }
     // the generated states form the last segment of the current side
     const int32_t n = active + (int32_t)fresh;
     q->pool_first[COUNTER_STRIPES] = pool_gen_slot(state_count);
@@ -190,7 +187,7 @@ __global__ void k_plan(WaveCounters *c, int32_t state_count)
     q->pool_prefix[COUNTER_STRIPES + 1] = n;
     // this bounce's pushes: COUNTER_STRIPES stripes over its n inputs
     const int32_t per = (n + COUNTER_STRIPES - 1) / COUNTER_STRIPES;
-    q->stripe = per > 0 ? (per + 63) / 64 * 64 : 64;
+    q->stripe[parity] = per > 0 ? (per + 63) / 64 * 64 : 64;
     c->gen_offset = active;
     c->gen_count = (int32_t)fresh;
     c->gen_base = c->generated;
@@ -360,12 +357,12 @@ __device__ __forceinline__ void train_add(float *acc, uint32_t tid, V3 v)
 template<int SAMPLER>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) k_shade(
     const RenderConsts *__restrict__ rcp, SceneView sv, WaveCounters *c, PathSOA path, HitSOA hit,
-    PathSOA next, ShadowSOA shadow, FilmView film, NrcView nrc)
+    PathSOA next, ShadowSOA shadow, FilmView film, NrcView nrc, int parity)
 {
     const RenderConsts &rc = *rcp;
     QueueState *q = queue_state(c);
     const int32_t n = c->active;
-    const int32_t stripe = q->stripe;
+    const int32_t stripe = q->stripe[parity];
     const int32_t stride = gridDim.x * blockDim.x;
     for(int32_t base = blockIdx.x * blockDim.x; base < n; base += stride)
     {
@@ -532,7 +529,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))
                                            : (shadow_out ? inst.outer_medium : inst.inner_medium);
                         const float cs = fabs_(dot(normalize(ss.d), sp.frame.z));
                         const V3 beta_li = ss.li * beta * shadow_f * cs / (shadow_pdf + ss.light_pdf);
-                        const int32_t si = stripe_push(q->shadow_n, k, stripe, true);
+                        const int32_t si = stripe_push(q->shadow_n[parity], k, stripe, true);
                         shadow.o[si] = f4(ss.o, __uint_as_float(pixel));
                         shadow.d[si] = f4(ss.d, ss.t1);
                         shadow.beta[si] = f4(beta_li, __uint_as_float(med));
@@ -581,12 +578,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))
 template<int SAMPLER>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) k_medium(
     const RenderConsts *__restrict__ rcp, SceneView sv, WaveCounters *c, PathSOA path, HitSOA hit,
-    PathSOA next, ShadowSOA shadow, FilmView film)
+    PathSOA next, ShadowSOA shadow, FilmView film, int parity)
 {
     const RenderConsts &rc = *rcp;
     QueueState *q = queue_state(c);
     const int32_t n = c->active;
-    const int32_t stripe = q->stripe;
+    const int32_t stripe = q->stripe[parity];
     const int32_t stride = gridDim.x * blockDim.x;
     for(int32_t base = blockIdx.x * blockDim.x; base < n; base += stride)
     {
@@ -693,7 +690,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))
                         if(ss.success)
                         {
                             const uint32_t smed = ss.t1 > 1.0f ? (uint32_t)sv.medium_count : med;
-                            const int32_t si = stripe_push(q->shadow_n, k, stripe, true);
+                            const int32_t si = stripe_push(q->shadow_n[parity], k, stripe, true);
                             shadow.o[si] = f4(ss.o, __uint_as_float(pixel));
                             shadow.d[si] = f4(ss.d, ss.t1);
                             shadow.beta[si] = f4(beta_li, __uint_as_float(smed));
@@ -734,11 +731,12 @@ struct ShadowSink
     FilmView film;
     SceneView sv;
     const QueueState *q;
+    int parity; // of the bounce that pushed the rays
 
     // the ray from its shadow queue slot, which is also its key
     __device__ __forceinline__ uint32_t load(uint32_t base, uint32_t j, V3 &o, V3 &d, float &tmin, float &tmax) const
     {
-        const int32_t i = shadow_slot(q, (int32_t)base, (int32_t)j);
+        const int32_t i = shadow_slot(q, parity, (int32_t)base, (int32_t)j);
         const float4 a = shadow.o[i];
         const float4 b = shadow.d[i];
         o = xyz(a);
@@ -793,31 +791,34 @@ struct ShadowSink
 
 template<int SCHED, bool MEDIA>
 __global__ void __launch_bounds__(TRACE_BLOCK) k_shadow(
-    TraceScene ts, SceneView sv, WaveCounters *c, ShadowSOA shadow, FilmView film, TraceOverflow ovf)
+    TraceScene ts, SceneView sv, WaveCounters *c, ShadowSOA shadow, FilmView film, TraceOverflow ovf, int parity)
 {
     QueueState *q = queue_state(c);
-    ShadowSink<MEDIA> sink{ shadow, film, sv, q };
+    ShadowSink<MEDIA> sink{ shadow, film, sv, q, parity };
     trace_persistent<TRI_WATERTIGHT, true, (SCHED & SCHED_POSTPONE) != 0, (SCHED & SCHED_TWO_LEVEL) != 0>(
-        ts, sink, shadow_count(q), q->shadow_fetch, ovf);
+        ts, sink, shadow_count(q, parity), q->shadow_fetch[parity], ovf);
 }
 
 // ---------------------------------------------------------------- bookkeeping
 
 // the states pushed this bounce become the next bounce's pool side: one
-// segment per stripe, then the (empty) generated segment; counters restart
-__global__ void k_end_iteration(WaveCounters *c)
+// segment per stripe, then the (empty) generated segment; the next-state
+// counters restart, and so do the shadow counters of the other parity (the
+// previous bounce's shadow rays were traced before this bounce shaded)
+__global__ void k_end_iteration(WaveCounters *c, int parity)
 {
     QueueState *q = queue_state(c);
-    const int32_t stripe = q->stripe;
+    const int32_t stripe = q->stripe[parity];
     int32_t states = 0, shadows = 0;
     for(int k = 0; k < COUNTER_STRIPES; ++k)
     {
         q->pool_first[k] = k * stripe;
         q->pool_prefix[k] = states;
         states += q->next_n[k * COUNTER_STRIDE];
-        shadows += q->shadow_n[k * COUNTER_STRIDE];
+        shadows += q->shadow_n[parity][k * COUNTER_STRIDE];
         q->next_n[k * COUNTER_STRIDE] = 0;
-        q->shadow_n[k * COUNTER_STRIDE] = 0;
+        q->shadow_n[parity ^ 1][k * COUNTER_STRIDE] = 0;
+        q->shadow_fetch[parity ^ 1][k * COUNTER_STRIDE] = 0;
     }
     q->pool_prefix[COUNTER_STRIPES] = states;
     q->pool_prefix[COUNTER_STRIPES + 1] = states;
@@ -1052,9 +1053,9 @@ __global__ void k_detmath_eval(int32_t fn, const float *x, const float *y, int32
 
 // ---------------------------------------------------------------- launchers
 
-void launch_plan(hipStream_t s, WaveCounters *c, int32_t state_count)
+void launch_plan(hipStream_t s, WaveCounters *c, int32_t state_count, int parity)
 {
-    hipLaunchKernelGGL(k_plan, dim3(1), dim3(1), 0, s, c, state_count);
+    hipLaunchKernelGGL(k_plan, dim3(1), dim3(1), 0, s, c, state_count, parity);
 }
 
 void launch_generate(hipStream_t s, int grid, int sampler, const RenderConsts *rc, const WaveCounters *c,
@@ -1082,54 +1083,55 @@ void launch_trace_extension(hipStream_t s, int grid, int sched, const TraceScene
 
 void launch_shade(hipStream_t s, int grid, int sampler, const RenderConsts *rc, const SceneView &sv,
                   WaveCounters *c, PathSOA path, HitSOA hit, PathSOA next, ShadowSOA shadow, FilmView film,
-                  const NrcView &nrc)
+                  const NrcView &nrc, int parity)
 {
     if(sampler == SAMPLER_INDEPENDENT)
         hipLaunchKernelGGL(k_shade<SAMPLER_INDEPENDENT>, dim3(grid), dim3(256), 0, s, rc, sv, c, path, hit, next,
-                           shadow, film, nrc);
+                           shadow, film, nrc, parity);
     else
         hipLaunchKernelGGL(k_shade<SAMPLER_HALTON>, dim3(grid), dim3(256), 0, s, rc, sv, c, path, hit, next, shadow,
-                           film, nrc);
+                           film, nrc, parity);
 }
 
 void launch_medium(hipStream_t s, int grid, int sampler, const RenderConsts *rc, const SceneView &sv,
-                   WaveCounters *c, PathSOA path, HitSOA hit, PathSOA next, ShadowSOA shadow, FilmView film)
+                   WaveCounters *c, PathSOA path, HitSOA hit, PathSOA next, ShadowSOA shadow, FilmView film,
+                   int parity)
 {
     if(sampler == SAMPLER_INDEPENDENT)
         hipLaunchKernelGGL(k_medium<SAMPLER_INDEPENDENT>, dim3(grid), dim3(256), 0, s, rc, sv, c, path, hit, next,
-                           shadow, film);
+                           shadow, film, parity);
     else
         hipLaunchKernelGGL(k_medium<SAMPLER_HALTON>, dim3(grid), dim3(256), 0, s, rc, sv, c, path, hit, next,
-                           shadow, film);
+                           shadow, film, parity);
 }
 
 template<int SCHED>
 static void launch_shadow_t(hipStream_t s, int grid, bool media, const TraceScene &ts, const SceneView &sv,
-                            WaveCounters *c, ShadowSOA shadow, FilmView film, const TraceOverflow &ovf)
+                            WaveCounters *c, ShadowSOA shadow, FilmView film, const TraceOverflow &ovf, int parity)
 {
     const dim3 g(grid), b(TRACE_BLOCK);
     if(media)
-        hipLaunchKernelGGL((k_shadow<SCHED, true>), g, b, 0, s, ts, sv, c, shadow, film, ovf);
+        hipLaunchKernelGGL((k_shadow<SCHED, true>), g, b, 0, s, ts, sv, c, shadow, film, ovf, parity);
     else
-        hipLaunchKernelGGL((k_shadow<SCHED, false>), g, b, 0, s, ts, sv, c, shadow, film, ovf);
+        hipLaunchKernelGGL((k_shadow<SCHED, false>), g, b, 0, s, ts, sv, c, shadow, film, ovf, parity);
 }
 
 void launch_shadow(hipStream_t s, int grid, int sched, bool media, const TraceScene &ts, const SceneView &sv,
-                   WaveCounters *c, ShadowSOA shadow, FilmView film, const TraceOverflow &ovf)
+                   WaveCounters *c, ShadowSOA shadow, FilmView film, const TraceOverflow &ovf, int parity)
 {
     if(sched == SCHED_TWO_LEVEL_POSTPONE)
-        launch_shadow_t<SCHED_TWO_LEVEL_POSTPONE>(s, grid, media, ts, sv, c, shadow, film, ovf);
+        launch_shadow_t<SCHED_TWO_LEVEL_POSTPONE>(s, grid, media, ts, sv, c, shadow, film, ovf, parity);
     else if(sched == SCHED_TWO_LEVEL)
-        launch_shadow_t<SCHED_TWO_LEVEL>(s, grid, media, ts, sv, c, shadow, film, ovf);
+        launch_shadow_t<SCHED_TWO_LEVEL>(s, grid, media, ts, sv, c, shadow, film, ovf, parity);
     else if(sched == SCHED_POSTPONE)
-        launch_shadow_t<SCHED_POSTPONE>(s, grid, media, ts, sv, c, shadow, film, ovf);
+        launch_shadow_t<SCHED_POSTPONE>(s, grid, media, ts, sv, c, shadow, film, ovf, parity);
     else
-        launch_shadow_t<SCHED_EAGER>(s, grid, media, ts, sv, c, shadow, film, ovf);
+        launch_shadow_t<SCHED_EAGER>(s, grid, media, ts, sv, c, shadow, film, ovf, parity);
 }
 
-void launch_end_iteration(hipStream_t s, WaveCounters *c)
+void launch_end_iteration(hipStream_t s, WaveCounters *c, int parity)
 {
-    hipLaunchKernelGGL(k_end_iteration, dim3(1), dim3(1), 0, s, c);
+    hipLaunchKernelGGL(k_end_iteration, dim3(1), dim3(1), 0, s, c, parity);
 }
 
 void launch_resolve(hipStream_t s, int mode, const float *value, int32_t pixels, float inv_weight, float4 *out)

@@ -80,6 +80,43 @@ def test_graph_loop_equals_direct_launches(ctx):
     ctx.set_tuning(launch_graph=1)
 
 
+@pytest.mark.parametrize("kind", ["field", "media", "cache"])
+def test_shadow_stream_overlap_equals_direct_launches(ctx, kind):
+    """bounces of a million paths run their shadow rays on a second stream
+    beside the next bounce's first stages; the film, the ray counts and the
+    cache's training statistics equal the direct launches' (one stream)"""
+    if kind == "media":
+        scene, cam = scenes.cornell_media(grid_n=24)
+    elif kind == "cache":
+        scene, cam = scenes.cornell()
+    else:
+        scene, cam = scenes.sphere_field(instances=256, lights=8, seed=6, extent=30.0)
+    ctx.upload_scene(scene)
+    w, h, spp = (1024, 1024, 1) if kind == "cache" else (256, 256, 16)
+    ctx.set_camera(cam, w, h)
+    got = {}
+    for graph in (1, 0):
+        ctx.set_tuning(launch_graph=graph)
+        if kind == "cache":
+            ctx.nrc_configure(enabled=1, train=1, train_depth=0)
+            ctx.nrc_reset(3)
+        out = []
+        for frame in range(2):
+            p = render_params(w, h, spp, max_depth=10 if kind == "media" else 8, state_count=1 << 20, film_fp64=1)
+            p.sample_offset, p.accumulate = frame, 1 if frame else 0
+            st = ctx.render(p)
+            stats = ctx.nrc_stats() if kind == "cache" else None
+            out.append((ctx.film(w, h, fp64=True)[1], st.extension_rays, st.shadow_rays,
+                        None if stats is None else (stats.records, stats.queries, tuple(stats.loss[:4]))))
+        got[graph] = out
+        if kind == "cache":
+            ctx.nrc_configure(enabled=0)
+    for (a, ea, sa, na), (b, eb, sb, nb) in zip(got[1], got[0]):
+        assert ea == eb and sa == sb and na == nb
+        assert np.abs(a - b).max() <= 1e-12 * max(1.0, np.abs(b).max())
+    ctx.set_tuning(launch_graph=1)
+
+
 def test_stage_grid_and_graph_block_keep_results(ctx):
     """the stage kernels' grid and the bounces per captured block change
     speed only"""
