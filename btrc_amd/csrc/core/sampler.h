@@ -150,6 +150,20 @@ BTRC_HD void divmod_base(uint64_t a, uint32_t base, uint32_t m, uint64_t *q, uin
     }
 }
 
+// One digit of owen_scrambled_radical_inverse (low_discrepancy.cpp:83-101):
+// the lowest base-`base` digit of a, permuted by the hash of the digits
+// reversed so far, appended to them; a loses the digit. m and w are the
+// base's reciprocal (divmod_base) and permutation mask.
+BTRC_HD uint64_t owen_digit(uint32_t base, uint32_t m, uint32_t w, uint64_t hash, uint64_t &a, uint64_t reversed)
+{
+    uint64_t next, digit_value;
+    divmod_base(a, base, m, &next, &digit_value);
+    const uint64_t digit_hash = mix_bits(hash ^ reversed);
+    digit_value = permutation_element_masked((uint32_t)digit_value, base, w, m, (uint32_t)digit_hash);
+    a = next;
+    return reversed * base + digit_value;
+}
+
 // owen_scrambled_radical_inverse, low_discrepancy.cpp:83-101 (the digit
 // split's reciprocal and the permutation's mask computed once per call)
 BTRC_HD float owen_scrambled_radical_inverse(uint32_t base, uint64_t a, uint64_t hash)
@@ -168,6 +182,50 @@ BTRC_HD float owen_scrambled_radical_inverse(uint32_t base, uint64_t a, uint64_t
         reversed_digits = reversed_digits * base + digit_value;
         inv_base_m = inv_base_m * inv_base;
         a = next;
+    }
+    return fmin_(inv_base_m * (float)reversed_digits, 0x1.fffffep-1f);
+}
+
+// Tables of the first digits. The first K scrambled digits of a depend only
+// on a mod base^K, so for the first HALTON_TABLE_DIMS dimensions a table
+// holds, for every r < base^K (K the largest with base^K <= 4096), the
+// reversed digits after K steps of the loop above on a = r. The rest of the
+// digits continue the loop from there; results are identical.
+constexpr int HALTON_TABLE_DIMS = 256;
+constexpr uint32_t HALTON_TABLE_SPAN = 4096;
+
+// a dimension's table: where its values start, base^K, K, and
+// floor((2^32 - 1) / base^K) for divmod_base
+struct HaltonDim
+{
+    uint32_t first, span, digits, recip;
+};
+
+struct HaltonTables
+{
+    const uint16_t *values; // all dimensions' tables
+    const HaltonDim *dims;  // HALTON_TABLE_DIMS entries
+};
+
+// the same as owen_scrambled_radical_inverse(base, a, hash) with the first
+// info.digits digits from table t
+BTRC_HD float owen_scrambled_radical_inverse_tabled(uint32_t base, uint64_t a, uint64_t hash, const uint16_t *t,
+                                                    HaltonDim info)
+{
+    const float inv_base = 1.0f / (float)base;
+    const uint32_t m = 0xffffffffu / base;
+    const uint32_t w = permutation_mask(base);
+    uint64_t high, low;
+    divmod_base(a, info.span, info.recip, &high, &low); // a / base^K, a mod base^K
+    uint64_t reversed_digits = t[info.first + low];
+    a = high;
+    float inv_base_m = 1.0f;
+    for(uint32_t k = 0; k < info.digits; ++k)
+        inv_base_m = inv_base_m * inv_base;
+    while(1.0f - inv_base_m < 1.0f)
+    {
+        reversed_digits = owen_digit(base, m, w, hash, a, reversed_digits);
+        inv_base_m = inv_base_m * inv_base;
     }
     return fmin_(inv_base_m * (float)reversed_digits, 0x1.fffffep-1f);
 }
@@ -199,11 +257,16 @@ struct HaltonState
     uint64_t index;
 };
 
-// halton.cpp:39-44 (sample_halton)
-BTRC_HD float sample_halton(const int *primes, uint32_t dim, uint64_t halton_index)
+// halton.cpp:39-44 (sample_halton); with tables (non-null values) the
+// first digits of the first HALTON_TABLE_DIMS dimensions come from them
+BTRC_HD float sample_halton(const int *primes, uint32_t dim, uint64_t halton_index,
+                            HaltonTables tab = HaltonTables{ nullptr, nullptr })
 {
-    return owen_scrambled_radical_inverse(
-        (uint32_t)primes[dim], halton_index, mix_bits((uint64_t)(1u + (dim << 4))));
+    const uint64_t hash = mix_bits((uint64_t)(1u + (dim << 4)));
+    if(tab.values && dim < (uint32_t)HALTON_TABLE_DIMS)
+        return owen_scrambled_radical_inverse_tabled((uint32_t)primes[dim], halton_index, hash, tab.values,
+                                                     tab.dims[dim]);
+    return owen_scrambled_radical_inverse((uint32_t)primes[dim], halton_index, hash);
 }
 
 // HaltonSampler(res, pixel, sample_index), halton.cpp:48-66
@@ -230,32 +293,61 @@ BTRC_HD HaltonState halton_start(const HaltonConsts &hc, uint32_t px, uint32_t p
 }
 
 // get1d / get2d / get3d, halton.cpp:80-116
-BTRC_HD float halton_get1d(const int *primes, HaltonState &st)
+BTRC_HD float halton_get1d(const int *primes, HaltonState &st, HaltonTables tab = HaltonTables{ nullptr, nullptr })
 {
     if(st.dimension >= (uint32_t)PRIME_TABLE_SIZE)
         st.dimension = 2;
-    const float r = sample_halton(primes, st.dimension, st.index);
+    const float r = sample_halton(primes, st.dimension, st.index, tab);
     st.dimension = st.dimension + 1;
     return r;
 }
 
-BTRC_HD void halton_get2d(const int *primes, HaltonState &st, float *a, float *b)
+BTRC_HD void halton_get2d(const int *primes, HaltonState &st, float *a, float *b,
+                          HaltonTables tab = HaltonTables{ nullptr, nullptr })
 {
     if(st.dimension + 1 >= (uint32_t)PRIME_TABLE_SIZE)
         st.dimension = 2;
-    *a = sample_halton(primes, st.dimension, st.index);
-    *b = sample_halton(primes, st.dimension + 1, st.index);
+    *a = sample_halton(primes, st.dimension, st.index, tab);
+    *b = sample_halton(primes, st.dimension + 1, st.index, tab);
     st.dimension = st.dimension + 2;
 }
 
-BTRC_HD void halton_get3d(const int *primes, HaltonState &st, float *a, float *b, float *c)
+BTRC_HD void halton_get3d(const int *primes, HaltonState &st, float *a, float *b, float *c,
+                          HaltonTables tab = HaltonTables{ nullptr, nullptr })
 {
     if(st.dimension + 2 >= (uint32_t)PRIME_TABLE_SIZE)
         st.dimension = 2;
-    *a = sample_halton(primes, st.dimension, st.index);
-    *b = sample_halton(primes, st.dimension + 1, st.index);
-    *c = sample_halton(primes, st.dimension + 2, st.index);
+    *a = sample_halton(primes, st.dimension, st.index, tab);
+    *b = sample_halton(primes, st.dimension + 1, st.index, tab);
+    *c = sample_halton(primes, st.dimension + 2, st.index, tab);
     st.dimension = st.dimension + 3;
+}
+
+// one dimension's table (host side): the reversed digits after the first
+// info.digits steps for every a < info.span
+BTRC_HD void fill_halton_table(uint32_t base, uint32_t dim, HaltonDim info, uint16_t *out)
+{
+    const uint64_t hash = mix_bits((uint64_t)(1u + (dim << 4)));
+    const uint32_t m = 0xffffffffu / base, w = permutation_mask(base);
+    for(uint32_t r = 0; r < info.span; ++r)
+    {
+        uint64_t a = r, reversed = 0;
+        for(uint32_t j = 0; j < info.digits; ++j)
+            reversed = owen_digit(base, m, w, hash, a, reversed);
+        out[r] = (uint16_t)reversed;
+    }
+}
+
+// K and base^K of a base's table: the most digits with base^K <= HALTON_TABLE_SPAN
+BTRC_HD HaltonDim halton_dim_shape(uint32_t base, uint32_t first)
+{
+    uint32_t span = 1, k = 0;
+    while(span * base <= HALTON_TABLE_SPAN)
+    {
+        span *= base;
+        ++k;
+    }
+    return HaltonDim{ first, span, k, 0xffffffffu / span };
 }
 
 // ---- independent sampler

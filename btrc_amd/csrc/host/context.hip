@@ -314,6 +314,10 @@ struct btrc_ctx
     int32_t film_spp = 0;
     bool film_has64 = false, film_has_albedo = false, film_has_normal = false;
 
+    // first-digit tables of the Halton sampler (built on first use)
+    DevBuf<uint16_t> halton_values;
+    DevBuf<HaltonDim> halton_dims;
+
     // film filter
     FilterConsts filter{ FILTER_BOX, 0.0f, 0.0f, nullptr };
     DevBuf<AliasUnit> filter_table;
@@ -1696,6 +1700,30 @@ void nrc_collect(btrc_ctx *ctx, int32_t steps)
     n.last = r;
 }
 
+// the Halton sampler's first-digit tables in device memory (a pure function
+// of the prime table: 226,691 values, 443 KB)
+HaltonTables halton_tables(btrc_ctx *ctx)
+{
+    if(!ctx->halton_values.ptr)
+    {
+        constexpr PrimeTable primes = make_prime_table();
+        std::vector<HaltonDim> dims(HALTON_TABLE_DIMS);
+        uint32_t total = 0;
+        for(int d = 0; d < HALTON_TABLE_DIMS; ++d)
+        {
+            dims[d] = halton_dim_shape((uint32_t)primes.p[d], total);
+            total += dims[d].span;
+        }
+        std::vector<uint16_t> values(total);
+        for(int d = 0; d < HALTON_TABLE_DIMS; ++d)
+            fill_halton_table((uint32_t)primes.p[d], (uint32_t)d, dims[d], values.data() + dims[d].first);
+        ctx->halton_dims.upload(dims.data(), dims.size(), ctx->stream);
+        ctx->halton_values.upload(values.data(), values.size(), ctx->stream);
+        BTRC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    }
+    return HaltonTables{ ctx->halton_values.ptr, ctx->halton_dims.ptr };
+}
+
 // Waits for the cache training still running on the training stream and
 // reads the last frame's statistics; every cache entry point calls it first
 void nrc_sync(btrc_ctx *ctx)
@@ -1914,6 +1942,7 @@ void render(btrc_ctx *ctx, const btrc_render_params *p, int32_t view_count, cons
         BTRC_HIP_CHECK(hipMemsetAsync(n.counts.ptr, 0, sizeof(int32_t), s)); // the query count
     }
     rc.filter = ctx->filter;
+    rc.halton = halton_tables(ctx);
 
     const TraceScene ts = ctx->trace_scene();
     SceneView sv = ctx->scene_view();
@@ -2938,7 +2967,8 @@ int btrc_hip_eval_halton(btrc_ctx *ctx, int32_t width, int32_t height, const uin
         dy.upload(py, n, ctx->stream);
         ds.upload(sample_index, n, ctx->stream);
         dout.alloc((size_t)n * dims);
-        launch_halton_eval(ctx->stream, make_halton_consts(width, height), dx.ptr, dy.ptr, ds.ptr, n, dims, dout.ptr);
+        launch_halton_eval(ctx->stream, make_halton_consts(width, height), halton_tables(ctx), dx.ptr, dy.ptr, ds.ptr,
+                           n, dims, dout.ptr);
         BTRC_HIP_CHECK(hipGetLastError());
         BTRC_HIP_CHECK(hipMemcpyAsync(out, dout.ptr, (size_t)n * dims * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
         BTRC_HIP_CHECK(hipStreamSynchronize(ctx->stream));

@@ -208,6 +208,7 @@ struct RenderConsts
     // paths with id pixel / nrc_stride while that is below nrc_max_paths
     int32_t nrc_stride, nrc_phase, nrc_max_paths;
     FilterConsts filter;
+    HaltonTables halton;    // first-digit tables of the Halton sampler
 };
 
 // the view holding film pixel `pixel`
@@ -268,7 +269,8 @@ void launch_trace_batch(hipStream_t s, int grid, int sched, const TraceScene &ts
                         const float4 *rays, int32_t n, uint32_t *out, uint32_t *fetch, const TraceOverflow &ovf);
 
 // test hooks
-void launch_halton_eval(hipStream_t s, const HaltonConsts &hc, const uint32_t *px, const uint32_t *py,
+void launch_halton_eval(hipStream_t s, const HaltonConsts &hc, const HaltonTables &tab, const uint32_t *px,
+                        const uint32_t *py,
                         const int32_t *sample_index, int32_t n, int32_t dims, float *out);
 // one-level structures: world-space triangle records of `geometry`'s
 // instances from its new local positions (9 floats per triangle)

@@ -140,6 +140,40 @@ template<> struct PathSampler<SAMPLER_INDEPENDENT>
     }
 };
 
+// generate's sampler: the path sampler started at the camera sample
+template<int SAMPLER> struct CameraSampler
+{
+    PathSampler<SAMPLER> smp;
+
+    __device__ static CameraSampler start(const RenderConsts &, const ViewRec &view, uint32_t px, uint32_t py,
+                                          int32_t sample)
+    {
+        return CameraSampler{ PathSampler<SAMPLER>::start(view, px, py, sample) };
+    }
+    __device__ float get1d() { return smp.get1d(); }
+    __device__ uint4 pack(uint32_t pixel) const { return smp.pack(pixel); }
+};
+
+// The Halton camera samples read their first digits from the first-digit
+// tables (same values). The later stages (shade, medium) run the digit loop
+// alone: in those large kernels the lookups cost more registers than they save.
+template<> struct CameraSampler<SAMPLER_HALTON>
+{
+    HaltonState st;
+    HaltonTables tab;
+
+    __device__ static CameraSampler start(const RenderConsts &rc, const ViewRec &view, uint32_t px, uint32_t py,
+                                          int32_t sample)
+    {
+        return CameraSampler{ halton_start(view.halton, px, py, sample), rc.halton };
+    }
+    __device__ float get1d() { return halton_get1d(c_primes.p, st, tab); }
+    __device__ uint4 pack(uint32_t pixel) const
+    {
+        return PathSampler<SAMPLER_HALTON>{ st }.pack(pixel);
+    }
+};
+
 // Seed of a shadow ray's transmittance stream: a hash of its path's sampler
 // state when the ray is made, so ratio tracking along the ray is a function
 // of the path alone.
@@ -238,7 +272,7 @@ __global__ void __launch_bounds__(256) k_generate(const RenderConsts *__restrict
         const uint32_t px = local_pixel % (uint32_t)view.width;
         const uint32_t py = local_pixel / (uint32_t)view.width;
 
-        PathSampler<SAMPLER> smp = PathSampler<SAMPLER>::start(view, px, py, sample_index);
+        CameraSampler<SAMPLER> smp = CameraSampler<SAMPLER>::start(rc, view, px, py, sample_index);
         float fx, fy;
         if(rc.filter.type == FILTER_GAUSSIAN)
         {
@@ -988,15 +1022,18 @@ __global__ void k_independent_eval(const uint32_t *px, const uint32_t *py, const
         out[(size_t)i * dims + k] = pcg32_uniform_float(r);
 }
 
-__global__ void k_halton_eval(HaltonConsts hc, const uint32_t *px, const uint32_t *py, const int32_t *si,
-                              int32_t n, int32_t dims, float *out)
+__global__ void k_halton_eval(HaltonConsts hc, HaltonTables tab, const uint32_t *px, const uint32_t *py,
+                              const int32_t *si, int32_t n, int32_t dims, float *out)
 {
     const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if(i >= n)
         return;
+    // even samples read the first-digit tables, odd ones run the digit loop
+    // alone (the camera samples and the later stages' forms)
+    const HaltonTables t = (i & 1) ? HaltonTables{ nullptr, nullptr } : tab;
     HaltonState hs = halton_start(hc, px[i], py[i], si[i]);
     for(int32_t k = 0; k < dims; ++k)
-        out[(size_t)i * dims + k] = halton_get1d(c_primes.p, hs);
+        out[(size_t)i * dims + k] = halton_get1d(c_primes.p, hs, t);
 }
 
 // ACESToneMap's kernel, postprocess/tonemap.cpp:14-24,35-51: Narkowicz's
@@ -1173,10 +1210,11 @@ void launch_trace_batch(hipStream_t s, int grid, int sched, const TraceScene &ts
     }
 }
 
-void launch_halton_eval(hipStream_t s, const HaltonConsts &hc, const uint32_t *px, const uint32_t *py,
-                        const int32_t *sample_index, int32_t n, int32_t dims, float *out)
+void launch_halton_eval(hipStream_t s, const HaltonConsts &hc, const HaltonTables &tab, const uint32_t *px,
+                        const uint32_t *py, const int32_t *sample_index, int32_t n, int32_t dims, float *out)
 {
-    hipLaunchKernelGGL(k_halton_eval, dim3((n + 127) / 128), dim3(128), 0, s, hc, px, py, sample_index, n, dims, out);
+    hipLaunchKernelGGL(k_halton_eval, dim3((n + 127) / 128), dim3(128), 0, s, hc, tab, px, py, sample_index, n, dims,
+                       out);
 }
 
 void launch_volume_eval(hipStream_t s, const VolumeView &vv, const float4 *seg, const uint2 *seed, int32_t n,

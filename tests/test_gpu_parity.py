@@ -41,15 +41,17 @@ def test_detmath_pow_bitwise(ctx):
     assert np.array_equal(ctx.eval_detmath(4, x, y).view(np.uint32), oracle.detmath(4, x, y).view(np.uint32))
 
 
-@pytest.mark.parametrize("w,h", [(256, 256), (1920, 1080), (7, 3)])
-def test_halton_bitwise_equal_to_host(ctx, w, h):
+@pytest.mark.parametrize("w,h,dims", [(256, 256, 48), (1920, 1080, 48), (7, 3, 48), (64, 64, 300)])
+def test_halton_bitwise_equal_to_host(ctx, w, h, dims):
+    """even samples take the first-digit tables (dimensions below 256), odd
+    ones the plain digit loop; 300 dimensions cross the tables' end"""
     rng = np.random.default_rng(w)
     n = 4096
     px = rng.integers(0, w, n)
     py = rng.integers(0, h, n)
     si = rng.integers(0, 64, n)
-    gpu = ctx.eval_halton(w, h, px, py, si, 48)
-    cpu = oracle.halton(w, h, px, py, si, 48)
+    gpu = ctx.eval_halton(w, h, px, py, si, dims)
+    cpu = oracle.halton(w, h, px, py, si, dims)
     assert np.array_equal(gpu.view(np.uint32), cpu.view(np.uint32))
 
 
