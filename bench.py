@@ -100,7 +100,9 @@ def probe(args):
       copy  - one device-to-device copy of --mib MiB (torch's copy kernel),
               to scale FETCH_SIZE / WRITE_SIZE readings to bytes;
       frame - one 1080p frame of the 1M-triangle scene at --spp samples per
-              pixel (the benchmark frame at 64), with its ray counts;
+              pixel (the benchmark frame at 64), with its ray counts, as
+              direct launches on one stream (stage timing), so that every
+              kernel's trace duration is that of the launch alone;
       batch - the extension rays of that frame's first two bounces (up to
               --rays of them) traced --repeats times as one resident batch;
       nrc   - --frames 1 spp frames of the Cornell-style room at 256^2 with
@@ -151,6 +153,7 @@ def probe(args):
     p.rr_threshold, p.rr_cont_prob = 0.1, 0.5
     p.state_count = STATE_COUNT
     if args.probe == "frame":
+        p.time_stages = 1
         st = ctx.render(p)
         print(f"frame: {args.spp} spp, extension_rays {st.extension_rays} shadow_rays {st.shadow_rays} "
               f"iterations {st.iterations}", flush=True)
@@ -275,8 +278,8 @@ def main():
     ps = _abi.RenderParams.from_buffer_copy(p)
     ps.time_stages = 1
     st = ctx.render(ps)
-    staged = dict(seconds=st.seconds, ext=st.extension_rays, stage_ms=np.array(st.stage_ms[:7]),
-                  launches=np.array(st.stage_launches[:7]))
+    staged = dict(seconds=st.seconds, ext=st.extension_rays, shadow=st.shadow_rays,
+                  stage_ms=np.array(st.stage_ms[:7]), launches=np.array(st.stage_launches[:7]))
     log(f"bench: stage-timed frame: {st.seconds:.3f} s, stage ms {[round(x, 1) for x in st.stage_ms]}")
     quality = None
     if args.gt_spp > 0:
@@ -325,6 +328,8 @@ def main():
             cpu["steps_per_ray"] = float(np.mean(stats[0]))
             cpu["isects_per_ray"] = float(np.mean(stats[1]))
 
+    copy_gbs = round(ctx.measure_copy(4096, 8), 1)
+    log(f"bench: device copy {copy_gbs:.0f} GB/s (read + write)")
     roofline = None
     if media and staged["stage_ms"][4] > 0 and medium_stats is not None:
         roofline = medium_roofline(p, staged, medium_stats)
@@ -342,6 +347,10 @@ def main():
                         algorithmic_bytes_per_launch=round(bytes_per_ray * rays_per_launch),
                         rays_per_launch=round(rays_per_launch), launch_ms=round(launch_s * 1e3, 4),
                         traffic_bytes_per_ray=traffic_per_ray)
+    if roofline is not None:
+        # the same achieved rate against the copy bandwidth measured above
+        roofline["copy_measured"] = copy_gbs
+        roofline["frac_of_copy"] = round(roofline["achieved"] / copy_gbs, 4)
 
     bvh = None if media or args.no_bvh_compare else gpu_bvh_comparison(ctx, scene, ps, trace_ms, staged["seconds"])
 
@@ -378,6 +387,14 @@ def main():
                                zip(("generate", "trace", "shade", "shadow", "medium", "cache", "train"),
                                    staged["stage_ms"])},
         "stage_timed_frame_s": round(staged["seconds"], 4),
+        # rays over the traversal kernels' own time (closest hit + shadow
+        # stages of the stage-timed frame), beside the wall-time value
+        "mrays_per_traversal_s": (round((staged["ext"] + staged["shadow"]) /
+                                        ((staged["stage_ms"][1] + staged["stage_ms"][3]) * 1e-3) / 1e6, 1)
+                                  if staged["stage_ms"][1] + staged["stage_ms"][3] > 0 else None),
+        "hbm_copy": {"value": copy_gbs, "unit": "GB/s", "spec": HBM_PEAK_GBS,
+                     "method": "STREAM-style copy kernel (btrc_hip_measure_copy) over 4 GiB, HIP events, median of 8, "
+                               "read + write bytes"},
         "quality": quality,
         "roofline": roofline,
         "cpu_baseline": cpu,

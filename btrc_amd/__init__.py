@@ -290,6 +290,13 @@ class Context:
         self._check(self.lib.btrc_hip_batch_download(self.h, hits.ctypes.data))
         return hits
 
+    def measure_copy(self, mib=4096, repeats=8):
+        """the device's copy bandwidth in GB/s (bytes read + written), from a
+        STREAM-style copy kernel over `mib` MiB, median of `repeats` copies"""
+        gbs = C.c_double()
+        self._check(self.lib.btrc_hip_measure_copy(self.h, int(mib) << 20, repeats, C.byref(gbs)))
+        return gbs.value
+
     def capture_rays(self, params, bounces, max_rays):
         rays = np.empty(max_rays, dtype=_abi.ray_dtype())
         n = C.c_int64()

@@ -207,7 +207,7 @@ EXPORTS = [
     "btrc_hip_nrc_configure", "btrc_hip_nrc_reset", "btrc_hip_nrc_stats", "btrc_hip_nrc_get_weights",
     "btrc_hip_nrc_set_weights", "btrc_hip_nrc_get_ema_weights", "btrc_hip_nrc_eval", "btrc_hip_nrc_gradients", "btrc_hip_nrc_fit",
     "btrc_hip_trace_closest", "btrc_hip_trace_any", "btrc_hip_trace_closest_noalloc",
-    "btrc_hip_batch_upload", "btrc_hip_batch_trace", "btrc_hip_batch_download",
+    "btrc_hip_batch_upload", "btrc_hip_batch_trace", "btrc_hip_batch_download", "btrc_hip_measure_copy",
     "btrc_hip_capture_rays", "btrc_hip_build_bvh", "btrc_hip_eval_halton", "btrc_hip_eval_independent", "btrc_hip_eval_volumes", "btrc_hip_eval_texture",
     "btrc_hip_eval_envir", "btrc_hip_envir_table", "btrc_hip_eval_medium",
     "btrc_hip_eval_detmath",
@@ -256,6 +256,7 @@ def declare(lib):
         "btrc_hip_batch_upload": (C.c_int, [ctx, C.c_void_p, C.c_int64]),
         "btrc_hip_batch_trace": (C.c_int, [ctx, C.c_int32, C.c_int32, C.c_int32, P(C.c_double)]),
         "btrc_hip_batch_download": (C.c_int, [ctx, C.c_void_p]),
+        "btrc_hip_measure_copy": (C.c_int, [ctx, C.c_int64, C.c_int32, P(C.c_double)]),
         "btrc_hip_capture_rays": (C.c_int, [ctx, P(RenderParams), C.c_int32, C.c_void_p, C.c_int64, P(C.c_int64)]),
         "btrc_hip_eval_halton": (C.c_int, [ctx, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p,
                                            C.c_int32, C.c_int32, C.c_void_p]),

@@ -2942,6 +2942,45 @@ int btrc_hip_batch_trace(btrc_ctx *ctx, int32_t tri_test, int32_t any_hit, int32
     });
 }
 
+int btrc_hip_measure_copy(btrc_ctx *ctx, int64_t bytes, int32_t repeats, double *gb_per_s)
+{
+    return guarded(ctx, [&] {
+        if(bytes <= 0 || bytes % 16 != 0)
+            throw InvalidArg("bytes must be a positive multiple of 16");
+        if(repeats <= 0)
+            throw InvalidArg("repeats must be positive");
+        if(!gb_per_s)
+            throw InvalidArg("gb_per_s is null");
+        const int64_t vectors = bytes / 16;
+        DevBuf<float4> src, dst;
+        src.alloc((size_t)vectors);
+        dst.alloc((size_t)vectors);
+        BTRC_HIP_CHECK(hipMemsetAsync(src.ptr, 0, (size_t)bytes, ctx->stream));
+        hipEvent_t a, b;
+        BTRC_HIP_CHECK(hipEventCreate(&a));
+        BTRC_HIP_CHECK(hipEventCreate(&b));
+        std::vector<float> ms;
+        const int grid = ctx->cu_count * 8;
+        for(int32_t k = 0; k <= repeats; ++k) // the first copy is not timed
+        {
+            BTRC_HIP_CHECK(hipEventRecord(a, ctx->stream));
+            launch_stream_copy(ctx->stream, grid, src.ptr, dst.ptr, vectors);
+            BTRC_HIP_CHECK(hipGetLastError());
+            BTRC_HIP_CHECK(hipEventRecord(b, ctx->stream));
+            BTRC_HIP_CHECK(hipEventSynchronize(b));
+            float t = 0;
+            BTRC_HIP_CHECK(hipEventElapsedTime(&t, a, b));
+            if(k > 0)
+                ms.push_back(t);
+        }
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+        std::sort(ms.begin(), ms.end());
+        const double med = ms.size() % 2 ? ms[ms.size() / 2] : 0.5 * (ms[ms.size() / 2 - 1] + ms[ms.size() / 2]);
+        *gb_per_s = 2.0 * (double)bytes / (med * 1e-3) / 1e9;
+    });
+}
+
 int btrc_hip_batch_download(btrc_ctx *ctx, btrc_hit *hits)
 {
     return guarded(ctx, [&] {

@@ -268,6 +268,8 @@ void launch_resolve(hipStream_t s, int mode, const float *value, int32_t pixels,
 void launch_trace_batch(hipStream_t s, int grid, int sched, const TraceScene &ts, int tri_test, bool any_hit,
                         const float4 *rays, int32_t n, uint32_t *out, uint32_t *fetch, const TraceOverflow &ovf);
 
+// `vectors` 16-byte vectors from src to dst (STREAM-style bandwidth measurement)
+void launch_stream_copy(hipStream_t s, int grid, const void *src, void *dst, int64_t vectors);
 // test hooks
 void launch_halton_eval(hipStream_t s, const HaltonConsts &hc, const HaltonTables &tab, const uint32_t *px,
                         const uint32_t *py,

@@ -1022,6 +1022,29 @@ __global__ void k_independent_eval(const uint32_t *px, const uint32_t *py, const
         out[(size_t)i * dims + k] = pcg32_uniform_float(r);
 }
 
+// STREAM-style copy that measures the device's HBM bandwidth: each thread
+// moves four 16-byte vectors per pass, all four loads issued before the
+// stores, with non-temporal hints (the data is not reused)
+typedef float stream_f4 __attribute__((ext_vector_type(4)));
+
+__global__ void __launch_bounds__(256) k_stream_copy(const stream_f4 *__restrict__ a, stream_f4 *__restrict__ b,
+                                                      int64_t n)
+{
+    const int64_t stride = (int64_t)gridDim.x * 1024;
+    for(int64_t base = (int64_t)blockIdx.x * 1024 + threadIdx.x; base < n; base += stride)
+    {
+        stream_f4 v[4];
+#pragma unroll
+        for(int k = 0; k < 4; ++k)
+            if(base + k * 256 < n)
+                v[k] = __builtin_nontemporal_load(a + base + k * 256);
+#pragma unroll
+        for(int k = 0; k < 4; ++k)
+            if(base + k * 256 < n)
+                __builtin_nontemporal_store(v[k], b + base + k * 256);
+    }
+}
+
 __global__ void k_halton_eval(HaltonConsts hc, HaltonTables tab, const uint32_t *px, const uint32_t *py,
                               const int32_t *si, int32_t n, int32_t dims, float *out)
 {
@@ -1208,6 +1231,11 @@ void launch_trace_batch(hipStream_t s, int grid, int sched, const TraceScene &ts
         else
             launch_batch_t<TRI_MOLLER_TRUMBORE, false>(s, grid, sched, ts, rays, n, out, fetch, ovf);
     }
+}
+
+void launch_stream_copy(hipStream_t s, int grid, const void *src, void *dst, int64_t vectors)
+{
+    hipLaunchKernelGGL(k_stream_copy, dim3(grid), dim3(256), 0, s, (const stream_f4 *)src, (stream_f4 *)dst, vectors);
 }
 
 void launch_halton_eval(hipStream_t s, const HaltonConsts &hc, const HaltonTables &tab, const uint32_t *px,

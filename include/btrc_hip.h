@@ -501,6 +501,12 @@ int btrc_hip_batch_upload(btrc_ctx *ctx, const btrc_ray *rays, int64_t n);
 /* traces the resident batch `repeats` times; *kernel_ms = mean kernel time */
 int btrc_hip_batch_trace(btrc_ctx *ctx, int32_t tri_test, int32_t any_hit, int32_t repeats, double *kernel_ms);
 int btrc_hip_batch_download(btrc_ctx *ctx, btrc_hit *hits);
+/* No counterpart in the reference (a measurement aid): the device's copy
+ * bandwidth, from a STREAM-style copy kernel moving `bytes` (a positive
+ * multiple of 16) between two device buffers allocated for the call;
+ * *gb_per_s = bytes read + bytes written per second (GB/s), the median of
+ * `repeats` copies timed with HIP events after one untimed copy */
+int btrc_hip_measure_copy(btrc_ctx *ctx, int64_t bytes, int32_t repeats, double *gb_per_s);
 
 /* ---- rays of a render, for offline comparison ----
  * Renders like btrc_hip_render and copies the extension rays of the first
