@@ -2,7 +2,7 @@
 1M-triangle scene at 1080p.
 
     python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--no-cpu]
-    python bench.py --probe {copy,frame,batch,nrc}   # short workloads for rocprofv3 counter runs
+    python bench.py --probe {copy,frame,batch,media,nrc}   # short workloads for rocprofv3 counter runs
 
 A step renders one full frame: 1920x1080, 64 samples per pixel, max depth 8
 (Russian roulette from depth 3), 4096 Disney-shaded instances of a 256-triangle
@@ -105,6 +105,9 @@ def probe(args):
               kernel's trace duration is that of the launch alone;
       batch - the extension rays of that frame's first two bounces (up to
               --rays of them) traced --repeats times as one resident batch;
+      media - one 1080p frame of the media workload (the Cornell-style room
+              with the 128^3 heterogeneous medium, max depth 16) at --spp
+              samples per pixel, as direct launches on one stream;
       nrc   - --frames 1 spp frames of the Cornell-style room at 256^2 with
               the radiance cache (queries at depth 2, training on 1/16 of the
               pixels), with query and record counts."""
@@ -144,19 +147,20 @@ def probe(args):
         print(f"nrc: {frames} frames, {queries} queries, {records} training records, "
               f"{frames * args.nrc_steps} optimizer steps", flush=True)
         return
-    scene, cam = scenes.sphere_field()
+    media = args.probe == "media"
+    scene, cam = scenes.cornell_media() if media else scenes.sphere_field()
     ctx.upload_scene(scene)
     ctx.set_camera(cam, WIDTH, HEIGHT)
     p = _abi.RenderParams()
     p.width, p.height, p.spp = WIDTH, HEIGHT, args.spp
-    p.min_depth, p.max_depth = MIN_DEPTH, MAX_DEPTH
+    p.min_depth, p.max_depth = MIN_DEPTH, 16 if media else MAX_DEPTH
     p.rr_threshold, p.rr_cont_prob = 0.1, 0.5
     p.state_count = STATE_COUNT
-    if args.probe == "frame":
+    if args.probe in ("frame", "media"):
         p.time_stages = 1
         st = ctx.render(p)
-        print(f"frame: {args.spp} spp, extension_rays {st.extension_rays} shadow_rays {st.shadow_rays} "
-              f"iterations {st.iterations}", flush=True)
+        print(f"{args.probe}: {args.spp} spp, extension_rays {st.extension_rays} shadow_rays {st.shadow_rays} "
+              f"iterations {st.iterations} stage_ms {[round(x, 2) for x in st.stage_ms[:7]]}", flush=True)
         return
     r = ctx.capture_rays(p, 2, args.rays)
     r = r[np.random.default_rng(0).permutation(len(r))]
@@ -205,7 +209,7 @@ def main():
     ap.add_argument("--gt-spp", type=int, default=None,
                     help="samples per pixel of the FP64 ground truth for relMSE (0: skip; default 16384 for the "
                          "field, 256 for media, 1024 for the instanced views)")
-    ap.add_argument("--probe", choices=["copy", "frame", "batch", "nrc"], default=None,
+    ap.add_argument("--probe", choices=["copy", "frame", "batch", "media", "nrc"], default=None,
                     help="run a short fixed workload for rocprofv3 instead of the benchmark")
     ap.add_argument("--mib", type=int, default=1024, help="probe copy: MiB copied")
     ap.add_argument("--rays", type=int, default=1 << 22, help="probe batch: rays captured")
