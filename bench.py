@@ -44,12 +44,13 @@ def algorithmic_bytes_per_ray(steps, isects):
     return 32.0 + 20.0 + 64.0 * float(np.mean(steps)) + 36.0 * float(np.mean(isects))
 
 
-def read_traffic(path):
-    """HBM bytes per traced ray from the committed rocprofv3 counter summary
-    (FETCH_SIZE + WRITE_SIZE of the extension-trace kernel), if present."""
+def read_traffic(path, key="hbm_bytes_per_ray"):
+    """HBM bytes per unit from a committed rocprofv3 counter summary
+    (FETCH_SIZE + WRITE_SIZE of the kernel it covers, per traced ray or per
+    path state), if present."""
     if not os.path.exists(path):
         return None
-    m = re.search(r"hbm_bytes_per_ray:\s*([0-9.eE+-]+)", open(path).read())
+    m = re.search(key + r":\s*([0-9.eE+-]+)", open(path).read())
     return float(m.group(1)) if m else None
 
 
@@ -852,19 +853,25 @@ MEDIUM_BYTES_PER_LOOKUP = 32.0   # one trilinear density fetch: eight FP32 voxel
 
 
 def medium_roofline(p, staged, stats):
-    """Work model of the medium stage (k_medium): 204 B per path state it
-    processes (one per extension ray) plus 32 B per density lookup of delta
-    tracking. Lookups per path come from the CPU baseline's run of the same
-    algorithm on paths spread over the frame (media_cpu_baseline); the
-    stage's time is the stage-timed frame's."""
+    """Work model of the medium stage (k_medium_track + k_medium_scatter):
+    204 B per path state it processes (one per extension ray) plus 32 B per
+    density lookup of delta tracking. Lookups per path come from the CPU
+    baseline's run of the same algorithm on paths spread over the frame
+    (media_cpu_baseline); the stage's time is the stage-timed frame's;
+    `traffic` is the measured HBM bytes per path state of the committed
+    counter summary times the states per launch."""
     lookups_per_path = stats["lookups_per_path"]
     frame_paths = p.width * p.height * p.spp
     launches = max(1, int(staged["launches"][4]))
     total_bytes = MEDIUM_BYTES_PER_STATE * staged["ext"] + MEDIUM_BYTES_PER_LOOKUP * lookups_per_path * frame_paths
     stage_s = staged["stage_ms"][4] * 1e-3
     achieved = total_bytes / stage_s / 1e9
+    per_state = read_traffic(os.path.join(ROOT, "profiles", "medium_stage_cornell_media_1080p_64spp.md"),
+                             "hbm_bytes_per_state")
     return dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None, kernel="k_medium",
+                frac=round(achieved / HBM_PEAK_GBS, 4),
+                traffic=None if per_state is None else round(per_state * staged["ext"] / launches),
+                traffic_bytes_per_state=per_state, kernel="k_medium_track + k_medium_scatter",
                 algorithmic_bytes_per_launch=round(total_bytes / launches),
                 launch_ms=round(staged["stage_ms"][4] / launches, 4),
                 bytes_per_state=MEDIUM_BYTES_PER_STATE, bytes_per_lookup=MEDIUM_BYTES_PER_LOOKUP,

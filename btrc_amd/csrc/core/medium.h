@@ -145,6 +145,64 @@ struct MediumSample
     float g;
 };
 
+// Delta and ratio tracking through a grid medium, one step at a time: the
+// loops of medium_sample and medium_tr below are made of these steps, and
+// the wavefront's medium stage runs them one step per pass so that a lane
+// whose segment is done takes the next path state.
+struct GridTrack
+{
+    V3 a;           // segment start (world space)
+    V3 local_a;     // its grid coordinate
+    V3 local_step;  // grid coordinate per unit of t
+    V3 world_step;  // world position per unit of t
+    float t, t_max; // distance tracked so far and the segment's length
+    int32_t it;     // steps taken (at most MAX_TRACKING_STEPS)
+};
+
+BTRC_HD GridTrack grid_track_start(const MediumRec &m, V3 a, V3 b)
+{
+    GridTrack g;
+    const V3 uvw_a = medium_uvw(m, a), uvw_b = medium_uvw(m, b);
+    g.t_max = length(b - a);
+    g.local_a = v3s(0.5f) + 0.5f * uvw_a;
+    const V3 local_b = v3s(0.5f) + 0.5f * uvw_b;
+    g.local_step = (local_b - g.local_a) / g.t_max;
+    g.world_step = (b - a) / g.t_max;
+    g.a = a;
+    g.t = 0.0f;
+    g.it = 0;
+    return g;
+}
+
+// One free-flight step against the majorant: false once the segment is left
+// (or the step limit reached), else the grid point and its density. Smp
+// provides float get1d().
+template<class Smp>
+BTRC_HD bool grid_track_step(const MediumRec &m, const float *voxels, GridTrack &g, Smp &smp, V3 *uvw,
+                             float *density)
+{
+    if(g.it >= MAX_TRACKING_STEPS)
+        return false;
+    const float dt = -btrc_detmath::log(1.0f - smp.get1d()) * m.inv_max_density;
+    g.t = g.t + dt;
+    if(g.t >= g.t_max)
+        return false;
+    ++g.it;
+    *uvw = g.local_a + g.t * g.local_step;
+    *density = grid_lookup(voxels + m.density_offset, m.nx, m.ny, m.nz, 1, 0, *uvw);
+    return true;
+}
+
+// the phase colour at grid point uvw
+BTRC_HD V3 grid_albedo(const MediumRec &m, const float *voxels, V3 uvw)
+{
+    if(!m.albedo_grid)
+        return v3(m.albedo[0], m.albedo[1], m.albedo[2]);
+    const float *alb = voxels + m.albedo_offset;
+    return v3(grid_lookup(alb, m.nx, m.ny, m.nz, 3, 0, uvw), grid_lookup(alb, m.nx, m.ny, m.nz, 3, 1, uvw),
+              grid_lookup(alb, m.nx, m.ny, m.nz, 3, 2, uvw));
+}
+
 // Medium::sample along the segment a -> b (a, b in world space). Smp
 // provides float get1d().
 template<class Smp>
@@ -168,34 +226,16 @@ BTRC_HD MediumSample medium_sample(const MediumRec &m, const float *voxels, V3 a
         }
         return r;
     }
-    const V3 uvw_a = medium_uvw(m, a), uvw_b = medium_uvw(m, b);
-    const float t_max = length(b - a);
-    const V3 local_a = v3s(0.5f) + 0.5f * uvw_a;
-    const V3 local_b = v3s(0.5f) + 0.5f * uvw_b;
-    const V3 local_step = (local_b - local_a) / t_max;
-    const V3 world_step = (b - a) / t_max;
-    const float *dens = voxels + m.density_offset;
-    float t = 0.0f;
-    for(int32_t it = 0; it < MAX_TRACKING_STEPS; ++it)
+    GridTrack g = grid_track_start(m, a, b);
+    V3 uvw;
+    float density;
+    while(grid_track_step(m, voxels, g, smp, &uvw, &density))
     {
-        const float dt = -btrc_detmath::log(1.0f - smp.get1d()) * m.inv_max_density;
-        t = t + dt;
-        if(t >= t_max)
-            break;
-        const V3 uvw = local_a + t * local_step;
-        const float density = grid_lookup(dens, m.nx, m.ny, m.nz, 1, 0, uvw);
         if(smp.get1d() < density * m.inv_max_density)
         {
             r.scattered = true;
-            r.position = a + t * world_step;
-            if(m.albedo_grid)
-            {
-                const float *alb = voxels + m.albedo_offset;
-                r.color = v3(grid_lookup(alb, m.nx, m.ny, m.nz, 3, 0, uvw), grid_lookup(alb, m.nx, m.ny, m.nz, 3, 1, uvw),
-                             grid_lookup(alb, m.nx, m.ny, m.nz, 3, 2, uvw));
-            }
-            else
-                r.color = v3(m.albedo[0], m.albedo[1], m.albedo[2]);
+            r.position = g.a + g.t * g.world_step;
+            r.color = grid_albedo(m, voxels, uvw);
             break;
         }
     }
@@ -212,23 +252,12 @@ BTRC_HD V3 medium_tr(const MediumRec &m, const float *voxels, V3 a, V3 b, Smp &s
         const V3 e = -length(a - b) * sigma_a;
         return v3(btrc_detmath::exp(e.x), btrc_detmath::exp(e.y), btrc_detmath::exp(e.z));
     }
-    const V3 uvw_a = medium_uvw(m, a), uvw_b = medium_uvw(m, b);
-    float result = 1.0f, t = 0.0f;
-    const float t_max = length(b - a);
-    const V3 local_a = v3s(0.5f) + 0.5f * uvw_a;
-    const V3 local_b = v3s(0.5f) + 0.5f * uvw_b;
-    const V3 local_step = (local_b - local_a) / t_max;
-    const float *dens = voxels + m.density_offset;
-    for(int32_t it = 0; it < MAX_TRACKING_STEPS; ++it)
-    {
-        const float dt = -btrc_detmath::log(1.0f - smp.get1d()) * m.inv_max_density;
-        t = t + dt;
-        if(t >= t_max)
-            break;
-        const V3 uvw = local_a + t * local_step;
-        const float density = grid_lookup(dens, m.nx, m.ny, m.nz, 1, 0, uvw);
+    GridTrack g = grid_track_start(m, a, b);
+    float result = 1.0f;
+    V3 uvw;
+    float density;
+    while(grid_track_step(m, voxels, g, smp, &uvw, &density))
         result = result * (1.0f - density * m.inv_max_density);
-    }
     return v3s(result);
 }
 

@@ -300,6 +300,7 @@ struct btrc_ctx
     DevBuf<float4> hit_tuv;
     DevBuf<float4> sh_o, sh_d, sh_beta;
     DevBuf<uint2> sh_seed;
+    DevBuf<float4> medium_scatter; // scenes with media: tracking results, two per dense index
     DevBuf<uint32_t> sh_train;
     DevBuf<WaveState> counters;       // the counters (read back) and the queue bookkeeping
     WaveCounters *wc() const { return &counters.ptr->c; }
@@ -1841,7 +1842,10 @@ void render(btrc_ctx *ctx, const btrc_render_params *p, int32_t view_count, cons
     }
     const bool media = ctx->medium_count > 0 || ctx->volume_count > 0;
     if(media)
+    {
         ctx->sh_seed.alloc(SS);
+        ctx->medium_scatter.alloc(2 * P);
+    }
     ctx->counters.alloc(1);
     if(!ctx->host_counters)
         BTRC_HIP_CHECK(hipHostMalloc((void **)&ctx->host_counters, sizeof(WaveCounters), hipHostMallocDefault));
@@ -2014,7 +2018,10 @@ void render(btrc_ctx *ctx, const btrc_render_params *p, int32_t view_count, cons
         run(STAGE_TRACE, [&] { launch_trace_extension(s, ctx->trace_grid, ctx->trace_sched(false), ts, ctx->wc(), path, hit, ovf); });
         join_shadow();
         if(media)
-            run(STAGE_MEDIUM, [&] { launch_medium(s, grid256, p->sampler, rcd, sv, ctx->wc(), path, hit, next, sh, fv, side); });
+            run(STAGE_MEDIUM, [&] {
+                launch_medium(s, grid256, ctx->cu_count * MEDIUM_TRACK_WAVES_PER_CU, p->sampler, rcd, sv, ctx->wc(),
+                              path, hit, ctx->medium_scatter.ptr, next, sh, fv, side);
+            });
         run(STAGE_SHADE, [&] { launch_shade(s, grid256, p->sampler, rcd, sv, ctx->wc(), path, hit, next, sh, fv, nv, side); });
         if(overlap)
         {
@@ -2131,7 +2138,10 @@ void render(btrc_ctx *ctx, const btrc_render_params *p, int32_t view_count, cons
             captured += n;
         }
         if(media)
-            timer.run(STAGE_MEDIUM, [&] { launch_medium(s, grid256, p->sampler, rcd, sv, ctx->wc(), path, hit, next, sh, fv, cur); });
+            timer.run(STAGE_MEDIUM, [&] {
+                launch_medium(s, grid256, ctx->cu_count * MEDIUM_TRACK_WAVES_PER_CU, p->sampler, rcd, sv, ctx->wc(),
+                              path, hit, ctx->medium_scatter.ptr, next, sh, fv, cur);
+            });
         timer.run(STAGE_SHADE, [&] { launch_shade(s, grid256, p->sampler, rcd, sv, ctx->wc(), path, hit, next, sh, fv, nv, cur); });
         timer.run(STAGE_SHADOW, [&] { launch_shadow(s, ctx->trace_grid, ctx->trace_sched(true), media, ts, sv, ctx->wc(), sh, fv, ovf, cur); });
         launch_end_iteration(s, ctx->wc(), cur);

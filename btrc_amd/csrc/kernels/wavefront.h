@@ -81,6 +81,7 @@ struct QueueState
     int32_t shadow_n[2][COUNTER_STRIPES * COUNTER_STRIDE]; // shadow rays pushed, per parity and stripe
     uint32_t trace_fetch[FETCH_WORDS];                     // extension-ray chunk counters
     uint32_t shadow_fetch[2][FETCH_WORDS];                 // shadow-ray chunk counters, per parity
+    uint32_t medium_fetch[FETCH_WORDS];                    // medium-tracking chunk counters
     int32_t stripe[2];                                     // inputs per stripe of a bounce, per parity
     int32_t pool_first[POOL_SEGMENTS];                  // slot of each segment's first state
     int32_t pool_prefix[POOL_SEGMENTS + 1];             // dense index of each segment's first state
@@ -256,9 +257,13 @@ void launch_trace_extension(hipStream_t s, int grid, int sched, const TraceScene
 void launch_shade(hipStream_t s, int grid, int sampler, const RenderConsts *rc, const SceneView &sv,
                   WaveCounters *c, PathSOA path, HitSOA hit, PathSOA next, ShadowSOA shadow, FilmView film,
                   const NrcView &nrc, int parity);
-void launch_medium(hipStream_t s, int grid, int sampler, const RenderConsts *rc, const SceneView &sv,
-                   WaveCounters *c, PathSOA path, HitSOA hit, PathSOA next, ShadowSOA shadow, FilmView film,
-                   int parity);
+// wavefronts per CU of the persistent medium-tracking kernel
+constexpr int MEDIUM_TRACK_WAVES_PER_CU = 24;
+// track_grid: wavefronts of the persistent tracking kernel; scatter: two
+// float4 per dense index (the tracking results of scattering states)
+void launch_medium(hipStream_t s, int grid, int track_grid, int sampler, const RenderConsts *rc,
+                   const SceneView &sv, WaveCounters *c, PathSOA path, HitSOA hit, float4 *scatter, PathSOA next,
+                   ShadowSOA shadow, FilmView film, int parity);
 void launch_shadow(hipStream_t s, int grid, int sched, bool media, const TraceScene &ts, const SceneView &sv,
                    WaveCounters *c, ShadowSOA shadow, FilmView film, const TraceOverflow &ovf, int parity);
 void launch_end_iteration(hipStream_t s, WaveCounters *c, int parity);

@@ -140,32 +140,33 @@ template<> struct PathSampler<SAMPLER_INDEPENDENT>
     }
 };
 
-// generate's sampler: the path sampler started at the camera sample
-template<int SAMPLER> struct CameraSampler
+// generate's sampler: the Halton sampler reads the camera samples' first
+// digits from the first-digit tables (same values). The later stages run
+// the digit loop alone: in shade and k_medium_scatter the lookups cost more
+// registers than they save, and in k_medium_track (two draws per tracking
+// step) the dependent table loads cost more time than the digits they save.
+template<int SAMPLER> struct TabledSampler
 {
     PathSampler<SAMPLER> smp;
 
-    __device__ static CameraSampler start(const RenderConsts &, const ViewRec &view, uint32_t px, uint32_t py,
+    __device__ static TabledSampler start(const RenderConsts &, const ViewRec &view, uint32_t px, uint32_t py,
                                           int32_t sample)
     {
-        return CameraSampler{ PathSampler<SAMPLER>::start(view, px, py, sample) };
+        return TabledSampler{ PathSampler<SAMPLER>::start(view, px, py, sample) };
     }
     __device__ float get1d() { return smp.get1d(); }
     __device__ uint4 pack(uint32_t pixel) const { return smp.pack(pixel); }
 };
 
-// The Halton camera samples read their first digits from the first-digit
-// tables (same values). The later stages (shade, medium) run the digit loop
-// alone: in those large kernels the lookups cost more registers than they save.
-template<> struct CameraSampler<SAMPLER_HALTON>
+template<> struct TabledSampler<SAMPLER_HALTON>
 {
     HaltonState st;
     HaltonTables tab;
 
-    __device__ static CameraSampler start(const RenderConsts &rc, const ViewRec &view, uint32_t px, uint32_t py,
+    __device__ static TabledSampler start(const RenderConsts &rc, const ViewRec &view, uint32_t px, uint32_t py,
                                           int32_t sample)
     {
-        return CameraSampler{ halton_start(view.halton, px, py, sample), rc.halton };
+        return TabledSampler{ halton_start(view.halton, px, py, sample), rc.halton };
     }
     __device__ float get1d() { return halton_get1d(c_primes.p, st, tab); }
     __device__ uint4 pack(uint32_t pixel) const
@@ -213,7 +214,10 @@ __global__ void k_plan(WaveCounters *c, int32_t state_count, int parity)
             fresh = unfinished < rest ? unfinished : rest;
     }
     for(int k = 0; k < FETCH_WORDS; k += COUNTER_STRIDE)
+    {
         q->trace_fetch[k] = 0;
+        q->medium_fetch[k] = 0;
+    }
     // the generated states form the last segment of the current side
     const int32_t n = active + (int32_t)fresh;
     q->pool_first[COUNTER_STRIPES] = pool_gen_slot(state_count);
@@ -272,7 +276,7 @@ __global__ void __launch_bounds__(256) k_generate(const RenderConsts *__restrict
         const uint32_t px = local_pixel % (uint32_t)view.width;
         const uint32_t py = local_pixel / (uint32_t)view.width;
 
-        CameraSampler<SAMPLER> smp = CameraSampler<SAMPLER>::start(rc, view, px, py, sample_index);
+        TabledSampler<SAMPLER> smp = TabledSampler<SAMPLER>::start(rc, view, px, py, sample_index);
         float fx, fy;
         if(rc.filter.type == FILTER_GAUSSIAN)
         {
@@ -603,16 +607,211 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))
 
 // ---------------------------------------------------------------- medium
 
-// sample_medium_scattering_kernel, wavefront/medium.cpp:28-162: for every
-// path whose last segment runs through a medium, delta-track a scattering
-// point along it, weight beta_le by the segment's transmittance, and at a
-// scattering point finish the vertex here (Russian roulette, light sample
-// through the phase function, phase-sampled continuation). The shading
-// stage then only adds the emission the segment ended on.
+// sample_medium_scattering_kernel, wavefront/medium.cpp:28-162, in two
+// kernels. For every path whose last segment runs through a medium,
+// k_medium_track tracks a scattering point along the segment (delta
+// tracking) and its transmittance (ratio tracking), weights beta_le by the
+// transmittance and records the scattering point; k_medium_scatter then
+// finishes the vertex at every scattering point (Russian roulette, light
+// sample through the phase function, phase-sampled continuation). The
+// shading stage only adds the emission the segment ended on.
+//
+// The tracking loops run for a number of steps that differs from path to
+// path (and many states have no medium segment at all), so k_medium_track is
+// persistent like the traversal: each wavefront takes states in refills,
+// advances every lane's tracking by one step per pass, and hands a lane
+// whose segment is done the next state. The work distribution is the
+// traversal's: a first batch of MEDIUM_BLOCK states per wavefront by its
+// workgroup index, then GRANULE-state granules dealt round robin to the
+// chunk counters medium_fetch[k * COUNTER_STRIDE].
+constexpr int MEDIUM_BLOCK = 64;       // one wavefront per workgroup
+constexpr int MEDIUM_REFILL_MIN = 16;  // refill once this many lanes are idle
+
+// get_medium_segment, medium.cpp:214-245: the medium of state i's last
+// segment (ray origin ro, direction d) and the segment's end
+__device__ __forceinline__ uint32_t medium_segment(const SceneView &sv, const HitSOA &hit, int32_t i, uint32_t flag,
+                                                  float4 ro, V3 d, V3 *end)
+{
+    const V3 o = xyz(ro);
+    if(flag & PATH_FLAG_HAS_INTERSECTION)
+    {
+        const float4 tuv = hit.tuv[i];
+        const InstanceRec &inst = sv.instances[flag & PATH_FLAG_INSTANCE_ID_MASK];
+        const V3 nor = hit_geometric_normal(sv, inst, __float_as_uint(tuv.w));
+        const uint32_t inct_med = dot(d, nor) < 0.0f ? inst.outer_medium : inst.inner_medium;
+        const uint32_t ray_med = __float_as_uint(ro.w);
+        *end = intersection_offset(o + tuv.x * d, nor, -d);
+        return inct_med < ray_med ? inct_med : ray_med;
+    }
+    // the volume-primitive medium up to the world's diagonal
+    *end = o + normalize(d) * sv.world_diagonal;
+    return (uint32_t)sv.medium_count;
+}
+
+// A state's tracking result for k_medium_scatter, two float4 per dense index:
+// scattering point and phase asymmetry, phase colour
+__device__ __forceinline__ void record_scatter(float4 *scatter, int32_t i, const MediumSample &ms)
+{
+    scatter[2 * (size_t)i] = f4(ms.position, ms.g);
+    scatter[2 * (size_t)i + 1] = f4(ms.color, 0.0f);
+}
+
 template<int SAMPLER>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) k_medium(
+__global__ void __launch_bounds__(MEDIUM_BLOCK) k_medium_track(const RenderConsts *__restrict__ rcp, SceneView sv,
+                                                               WaveCounters *c, PathSOA path, HitSOA hit,
+                                                               float4 *scatter)
+{
+    const RenderConsts &rc = *rcp;
+    QueueState *q = queue_state(c);
+    const uint32_t n = (uint32_t)c->active;
+    const int lane = (int)threadIdx.x;
+    const uint64_t lanes_below = (1ull << lane) - 1ull;
+    const uint32_t static_span = gridDim.x * (uint32_t)MEDIUM_BLOCK;
+    const uint32_t chunks = min((uint32_t)COUNTER_STRIPES, gridDim.x);
+    const uint32_t home = blockIdx.x % chunks;
+    uint32_t *counter = q->medium_fetch + home * COUNTER_STRIDE;
+    // wavefront-uniform: reserved states not yet handed out, whether the
+    // home chunk has run dry, and whether no state is left for this wavefront
+    uint32_t res_lo = min(n, blockIdx.x * (uint32_t)MEDIUM_BLOCK);
+    uint32_t res_hi = min(n, res_lo + (uint32_t)MEDIUM_BLOCK);
+    bool dry = static_span >= n;
+    bool exhausted = dry && res_lo == res_hi;
+
+    // the lane's state: dense index, pool slot, hit flag, film pixel, sampler,
+    // the grid medium being tracked, its progress and results
+    bool active = false, sampling = false, scattered = false;
+    int32_t i = 0, ps = 0;
+    uint32_t flag = 0, pixel = 0, med = 0;
+    PathSampler<SAMPLER> hs{};
+    GridTrack g{};
+    float tr = 1.0f;
+
+    // takes state j of a refill handing out states base .. base + 63
+    auto begin = [&](int32_t base, int32_t j) {
+        i = j;
+        ps = pool_slot(q, base, j);
+        flag = hit.flag[i];
+        const float4 ro = path.ray_o[ps];
+        const float4 rd = path.ray_d[ps];
+        const V3 o = xyz(ro), d = xyz(rd);
+
+        V3 end;
+        med = medium_segment(sv, hit, i, flag, ro, d, &end);
+        path.ray_o[ps] = f4(o, __uint_as_float(med));
+
+        // grid media are tracked here, one step per pass below; segments
+        // in a homogeneous medium or the volume primitives (no long
+        // tracking loops) are left to k_medium_scatter
+        if(med >= (uint32_t)sv.medium_count || sv.media[med].type == MEDIUM_HOMOGENEOUS)
+            return;
+        const MediumRec &m = sv.media[med];
+        const uint4 smp = path.smp[ps];
+        pixel = smp.x;
+        hs = PathSampler<SAMPLER>::load(rc, smp);
+        g = grid_track_start(m, o, end);
+        tr = 1.0f;
+        sampling = true;
+        scattered = false;
+        active = true;
+    };
+
+    for(;;)
+    {
+        // ---- hand new states to idle lanes (as the traversal hands out rays)
+        const uint64_t idle = __ballot(!active);
+        const int nidle = (int)__popcll(idle);
+        if(nidle >= MEDIUM_REFILL_MIN && !exhausted)
+        {
+            const uint32_t base_a = res_lo;
+            const uint32_t take_a = min((uint32_t)nidle, res_hi - res_lo);
+            res_lo += take_a;
+            uint32_t base_b = 0, take_b = 0;
+            if(take_a < (uint32_t)nidle && !dry)
+            {
+                uint32_t gr = 0;
+                if(lane == 0)
+                    gr = atomicAdd(counter, 1u);
+                gr = __builtin_amdgcn_readfirstlane(gr);
+                const uint64_t lo = (uint64_t)static_span + ((uint64_t)gr * chunks + home) * GRANULE;
+                if(lo < (uint64_t)n)
+                {
+                    res_lo = (uint32_t)lo;
+                    res_hi = min(n, res_lo + (uint32_t)GRANULE);
+                    base_b = res_lo;
+                    take_b = min((uint32_t)nidle - take_a, res_hi - res_lo);
+                    res_lo += take_b;
+                }
+                else
+                    dry = true;
+            }
+            exhausted = dry && res_lo == res_hi;
+            if(!active)
+            {
+                const uint32_t rank = (uint32_t)__popcll(idle & lanes_below);
+                if(take_a > 0 && rank < take_a)
+                    begin((int32_t)base_a, (int32_t)(base_a + rank));
+                if(take_b > 0 && rank >= take_a && rank < take_a + take_b)
+                    begin((int32_t)base_b, (int32_t)(base_b + (rank - take_a)));
+            }
+        }
+        if(!__any(active))
+        {
+            if(exhausted)
+                break;
+            continue; // every state taken so far had no grid segment
+        }
+
+        // ---- one tracking step (medium_sample's loop, then medium_tr's)
+        if(active)
+        {
+            const MediumRec &m = sv.media[med];
+            V3 uvw;
+            float density;
+            const bool inside = grid_track_step(m, sv.voxels, g, hs, &uvw, &density);
+            if(sampling)
+            {
+                bool done = !inside;
+                if(inside && hs.get1d() < density * m.inv_max_density)
+                {
+                    MediumSample ms;
+                    ms.scattered = true;
+                    ms.position = g.a + g.t * g.world_step;
+                    ms.color = grid_albedo(m, sv.voxels, uvw);
+                    ms.g = m.g;
+                    record_scatter(scatter, i, ms);
+                    scattered = true;
+                    done = true;
+                }
+                if(done)
+                {
+                    // medium_tr tracks the whole segment again from its start
+                    sampling = false;
+                    g.t = 0.0f;
+                    g.it = 0;
+                }
+            }
+            else if(inside)
+                tr = tr * (1.0f - density * m.inv_max_density);
+            else
+            {
+                const float4 le4 = path.bsdf_le[ps];
+                path.bsdf_le[ps] = f4(xyz(le4) * v3s(tr), le4.w);
+                path.smp[ps] = hs.pack(pixel);
+                if(scattered)
+                    hit.flag[i] = flag | PATH_FLAG_HAS_SCATTERING;
+                active = false;
+            }
+        }
+    }
+}
+
+// The scattering vertices recorded by k_medium_track (the rest of
+// sample_medium_scattering_kernel): Russian roulette, the light sample
+// through the phase function, the phase-sampled continuation.
+template<int SAMPLER>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) k_medium_scatter(
     const RenderConsts *__restrict__ rcp, SceneView sv, WaveCounters *c, PathSOA path, HitSOA hit,
-    PathSOA next, ShadowSOA shadow, FilmView film, int parity)
+    const float4 *scatter, PathSOA next, ShadowSOA shadow, FilmView film, int parity)
 {
     const RenderConsts &rc = *rcp;
     QueueState *q = queue_state(c);
@@ -625,130 +824,127 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))
         const int32_t w0 = base + (int32_t)(threadIdx.x & ~63u); // this wavefront's first input
         const int32_t k = w0 / stripe;                            // its stripe
         // queue pushes where their payload is made, as in k_shade
-        if(i < n)
+        if(i >= n)
+            continue;
+        const uint32_t flag = hit.flag[i];
+        const int32_t ps = pool_slot(q, w0, i); // the state's pool slot
+        const float4 ro = path.ray_o[ps];
+        const uint32_t med = __float_as_uint(ro.w); // the segment's medium (k_medium_track)
+        const bool vol_prim = med == (uint32_t)sv.medium_count && sv.vol.count > 0;
+        const bool homogeneous = med < (uint32_t)sv.medium_count && sv.media[med].type == MEDIUM_HOMOGENEOUS;
+        if((flag & PATH_FLAG_HAS_SCATTERING) || vol_prim || homogeneous)
         {
-            const int32_t ps = pool_slot(q, w0, i); // the state's pool slot
-            const uint32_t flag = hit.flag[i];
-            const float4 ro = path.ray_o[ps];
-            const float4 rd = path.ray_d[ps];
-            const V3 o = xyz(ro), d = xyz(rd);
-
-            // get_medium_segment, medium.cpp:214-245
-            uint32_t med;
-            V3 end;
-            if(flag & PATH_FLAG_HAS_INTERSECTION)
+            const V3 d = xyz(path.ray_d[ps]);
+            const uint4 smp = path.smp[ps];
+            const uint32_t pixel = smp.x;
+            PathSampler<SAMPLER> hs = PathSampler<SAMPLER>::load(rc, smp);
+            V3 position, color;
+            float gph;
+            if(vol_prim || homogeneous)
             {
-                const float4 tuv = hit.tuv[i];
-                const InstanceRec &inst = sv.instances[flag & PATH_FLAG_INSTANCE_ID_MASK];
-                const V3 nor = hit_geometric_normal(sv, inst, __float_as_uint(tuv.w));
-                const uint32_t inct_med = dot(d, nor) < 0.0f ? inst.outer_medium : inst.inner_medium;
-                const uint32_t ray_med = __float_as_uint(ro.w);
-                med = inct_med < ray_med ? inct_med : ray_med;
-                end = intersection_offset(o + tuv.x * d, nor, -d);
-            }
-            else
-            {
-                // the volume-primitive medium up to the world's diagonal
-                med = (uint32_t)sv.medium_count;
-                end = o + normalize(d) * sv.world_diagonal;
-            }
-            path.ray_o[ps] = f4(o, __uint_as_float(med));
-
-            const bool vol_prim = med == (uint32_t)sv.medium_count && sv.vol.count > 0;
-            if(med < (uint32_t)sv.medium_count || vol_prim)
-            {
-                const uint4 smp = path.smp[ps];
-                const uint32_t pixel = smp.x;
-                PathSampler<SAMPLER> hs = PathSampler<SAMPLER>::load(rc, smp);
+                // no long tracking loop: sample and transmittance here
+                V3 end;
+                (void)medium_segment(sv, hit, i, flag, ro, d, &end);
+                const V3 o = xyz(ro);
                 MediumSample ms;
-                V3 tr;
+                V3 trv;
                 if(vol_prim)
                 {
                     ms = volume_sample(sv.vol, o, end, hs);
-                    tr = volume_tr(sv.vol, o, end, hs);
+                    trv = volume_tr(sv.vol, o, end, hs);
                 }
                 else
                 {
                     const MediumRec &m = sv.media[med];
                     ms = medium_sample(m, sv.voxels, o, end, hs);
-                    tr = medium_tr(m, sv.voxels, o, end, hs);
+                    trv = medium_tr(m, sv.voxels, o, end, hs);
                 }
                 const float4 le4 = path.bsdf_le[ps];
-                path.bsdf_le[ps] = f4(xyz(le4) * tr, le4.w);
-
+                path.bsdf_le[ps] = f4(xyz(le4) * trv, le4.w);
                 if(!ms.scattered)
+                {
                     path.smp[ps] = hs.pack(pixel);
+                    continue;
+                }
+                hit.flag[i] = flag | PATH_FLAG_HAS_SCATTERING;
+                position = ms.position;
+                color = ms.color;
+                gph = ms.g;
+            }
+            else
+            {
+                const float4 sa = scatter[2 * (size_t)i], sb = scatter[2 * (size_t)i + 1];
+                position = xyz(sa);
+                color = xyz(sb);
+                gph = sa.w;
+            }
+
+            const float4 beta_depth = path.beta[ps];
+            V3 beta = xyz(beta_depth); // medium throughput is one
+            const int32_t depth = __float_as_int(beta_depth.w);
+            const float4 rad4 = path.rad[ps];
+            const V3 path_rad = xyz(rad4);
+
+            bool rr_exit = false;
+            if(depth >= rc.min_depth)
+            {
+                if(depth >= rc.max_depth)
+                    rr_exit = true;
                 else
                 {
-                    hit.flag[i] = flag | PATH_FLAG_HAS_SCATTERING;
-                    const float4 beta_depth = path.beta[ps];
-                    V3 beta = xyz(beta_depth); // medium throughput is one
-                    const int32_t depth = __float_as_int(beta_depth.w);
-                    const float4 rad4 = path.rad[ps];
-                    const V3 path_rad = xyz(rad4);
-
-                    bool rr_exit = false;
-                    if(depth >= rc.min_depth)
+                    const float u = hs.get1d();
+                    if(luminance(beta) < rc.rr_threshold)
                     {
-                        if(depth >= rc.max_depth)
+                        if(u > rc.rr_cont_prob)
                             rr_exit = true;
                         else
-                        {
-                            const float u = hs.get1d();
-                            if(luminance(beta) < rc.rr_threshold)
-                            {
-                                if(u > rc.rr_cont_prob)
-                                    rr_exit = true;
-                                else
-                                    beta = beta / rc.rr_cont_prob;
-                            }
-                        }
-                    }
-                    if(rr_exit)
-                        film_splat(film, pixel, path_rad);
-                    else
-                    {
-                        float l0, l1, l2;
-                        hs.get3d(&l0, &l1, &l2);
-                        const float lsel = hs.get1d();
-                        ShadowSample ss = sample_light_li(sv, ms.position, v3s(0.0f), false, l0, l1, l2, lsel);
-                        V3 beta_li = v3s(0.0f);
-                        if(ss.success)
-                        {
-                            const float pv = hg_pdf(ms.g, ss.d, -d);
-                            beta_li = ss.li * beta * (ms.color * pv) / (pv + ss.light_pdf);
-                        }
-                        float s0, s1, s2;
-                        hs.get3d(&s0, &s1, &s2);
-                        const PhaseSample ps = hg_sample(ms.g, ms.color, -d, s0, s1);
-                        if(ss.success)
-                        {
-                            const uint32_t smed = ss.t1 > 1.0f ? (uint32_t)sv.medium_count : med;
-                            const int32_t si = stripe_push(q->shadow_n[parity], k, stripe, true);
-                            shadow.o[si] = f4(ss.o, __uint_as_float(pixel));
-                            shadow.d[si] = f4(ss.d, ss.t1);
-                            shadow.beta[si] = f4(beta_li, __uint_as_float(smed));
-                            shadow.seed[si] = shadow_seed(hs.pack(pixel));
-                            if(shadow.train)
-                                shadow.train[si] = __float_as_uint(rad4.w);
-                        }
-                        if(!is_zero(ps.phase))
-                        {
-                            const V3 dir = normalize(ps.dir);
-                            const V3 beta_le = beta * ps.phase;
-                            beta = beta_le / ps.pdf;
-                            const int32_t ni = stripe_push(q->next_n, k, stripe, true);
-                            next.ray_o[ni] = f4(ms.position, __uint_as_float(med));
-                            next.ray_d[ni] = f4(dir, MAX_FLOAT);
-                            next.beta[ni] = f4(beta, __int_as_float(depth + 1));
-                            next.rad[ni] = rad4;
-                            next.bsdf_le[ni] = f4(beta_le, ps.pdf);
-                            next.smp[ni] = hs.pack(pixel);
-                        }
-                        else
-                            film_splat(film, pixel, path_rad);
+                            beta = beta / rc.rr_cont_prob;
                     }
                 }
+            }
+            if(rr_exit)
+                film_splat(film, pixel, path_rad);
+            else
+            {
+                float l0, l1, l2;
+                hs.get3d(&l0, &l1, &l2);
+                const float lsel = hs.get1d();
+                ShadowSample ss = sample_light_li(sv, position, v3s(0.0f), false, l0, l1, l2, lsel);
+                V3 beta_li = v3s(0.0f);
+                if(ss.success)
+                {
+                    const float pv = hg_pdf(gph, ss.d, -d);
+                    beta_li = ss.li * beta * (color * pv) / (pv + ss.light_pdf);
+                }
+                float s0, s1, s2;
+                hs.get3d(&s0, &s1, &s2);
+                const PhaseSample phs = hg_sample(gph, color, -d, s0, s1);
+                if(ss.success)
+                {
+                    const uint32_t smed = ss.t1 > 1.0f ? (uint32_t)sv.medium_count : med;
+                    const int32_t si = stripe_push(q->shadow_n[parity], k, stripe, true);
+                    shadow.o[si] = f4(ss.o, __uint_as_float(pixel));
+                    shadow.d[si] = f4(ss.d, ss.t1);
+                    shadow.beta[si] = f4(beta_li, __uint_as_float(smed));
+                    shadow.seed[si] = shadow_seed(hs.pack(pixel));
+                    if(shadow.train)
+                        shadow.train[si] = __float_as_uint(rad4.w);
+                }
+                if(!is_zero(phs.phase))
+                {
+                    const V3 dir = normalize(phs.dir);
+                    const V3 beta_le = beta * phs.phase;
+                    beta = beta_le / phs.pdf;
+                    const int32_t ni = stripe_push(q->next_n, k, stripe, true);
+                    next.ray_o[ni] = f4(position, __uint_as_float(med));
+                    next.ray_d[ni] = f4(dir, MAX_FLOAT);
+                    next.beta[ni] = f4(beta, __int_as_float(depth + 1));
+                    next.rad[ni] = rad4;
+                    next.bsdf_le[ni] = f4(beta_le, phs.pdf);
+                    next.smp[ni] = hs.pack(pixel);
+                }
+                else
+                    film_splat(film, pixel, path_rad);
             }
         }
     }
@@ -1153,16 +1349,24 @@ void launch_shade(hipStream_t s, int grid, int sampler, const RenderConsts *rc, 
                            film, nrc, parity);
 }
 
-void launch_medium(hipStream_t s, int grid, int sampler, const RenderConsts *rc, const SceneView &sv,
-                   WaveCounters *c, PathSOA path, HitSOA hit, PathSOA next, ShadowSOA shadow, FilmView film,
-                   int parity)
+void launch_medium(hipStream_t s, int grid, int track_grid, int sampler, const RenderConsts *rc,
+                   const SceneView &sv, WaveCounters *c, PathSOA path, HitSOA hit, float4 *scatter, PathSOA next,
+                   ShadowSOA shadow, FilmView film, int parity)
 {
     if(sampler == SAMPLER_INDEPENDENT)
-        hipLaunchKernelGGL(k_medium<SAMPLER_INDEPENDENT>, dim3(grid), dim3(256), 0, s, rc, sv, c, path, hit, next,
-                           shadow, film, parity);
+    {
+        hipLaunchKernelGGL(k_medium_track<SAMPLER_INDEPENDENT>, dim3(track_grid), dim3(MEDIUM_BLOCK), 0, s, rc, sv,
+                           c, path, hit, scatter);
+        hipLaunchKernelGGL(k_medium_scatter<SAMPLER_INDEPENDENT>, dim3(grid), dim3(256), 0, s, rc, sv, c, path, hit,
+                           scatter, next, shadow, film, parity);
+    }
     else
-        hipLaunchKernelGGL(k_medium<SAMPLER_HALTON>, dim3(grid), dim3(256), 0, s, rc, sv, c, path, hit, next,
-                           shadow, film, parity);
+    {
+        hipLaunchKernelGGL(k_medium_track<SAMPLER_HALTON>, dim3(track_grid), dim3(MEDIUM_BLOCK), 0, s, rc, sv, c,
+                           path, hit, scatter);
+        hipLaunchKernelGGL(k_medium_scatter<SAMPLER_HALTON>, dim3(grid), dim3(256), 0, s, rc, sv, c, path, hit,
+                           scatter, next, shadow, film, parity);
+    }
 }
 
 template<int SCHED>
