@@ -258,7 +258,7 @@ void launch_shade(hipStream_t s, int grid, int sampler, const RenderConsts *rc, 
                   WaveCounters *c, PathSOA path, HitSOA hit, PathSOA next, ShadowSOA shadow, FilmView film,
                   const NrcView &nrc, int parity);
 // wavefronts per CU of the persistent medium-tracking kernel
-constexpr int MEDIUM_TRACK_WAVES_PER_CU = 24;
+constexpr int MEDIUM_TRACK_WAVES_PER_CU = 20;
 // track_grid: wavefronts of the persistent tracking kernel; scatter: two
 // float4 per dense index (the tracking results of scattering states)
 void launch_medium(hipStream_t s, int grid, int track_grid, int sampler, const RenderConsts *rc,

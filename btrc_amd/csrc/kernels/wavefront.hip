@@ -617,15 +617,19 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))
 // shading stage only adds the emission the segment ended on.
 //
 // The tracking loops run for a number of steps that differs from path to
-// path (and many states have no medium segment at all), so k_medium_track is
-// persistent like the traversal: each wavefront takes states in refills,
-// advances every lane's tracking by one step per pass, and hands a lane
-// whose segment is done the next state. The work distribution is the
-// traversal's: a first batch of MEDIUM_BLOCK states per wavefront by its
-// workgroup index, then GRANULE-state granules dealt round robin to the
-// chunk counters medium_fetch[k * COUNTER_STRIDE].
+// path, and many states have no grid-medium segment at all, so
+// k_medium_track is persistent like the traversal. A wavefront takes states
+// in batches of 64 (a first batch by its workgroup index, then GRANULE-state
+// granules dealt round robin to the chunk counters medium_fetch[k *
+// COUNTER_STRIDE], as the traversal hands out rays); all 64 lanes find the
+// segments of a batch together and queue the states whose segment runs
+// through a grid medium in the wavefront's LDS. Every pass advances each
+// busy lane's tracking by one step; once MEDIUM_REFILL_MIN lanes are idle
+// they take states from the queue, topped up batch by batch.
 constexpr int MEDIUM_BLOCK = 64;       // one wavefront per workgroup
 constexpr int MEDIUM_REFILL_MIN = 16;  // refill once this many lanes are idle
+constexpr int MEDIUM_QUEUE = 128;      // LDS queue entries (a ring; it never holds more than 127)
+static_assert(GRANULE == MEDIUM_BLOCK, "a granule is one batch of the tracking kernel");
 
 // get_medium_segment, medium.cpp:214-245: the medium of state i's last
 // segment (ray origin ro, direction d) and the segment's end
@@ -661,6 +665,11 @@ __global__ void __launch_bounds__(MEDIUM_BLOCK) k_medium_track(const RenderConst
                                                                WaveCounters *c, PathSOA path, HitSOA hit,
                                                                float4 *scatter)
 {
+    // the queue: dense index, pool slot, hit flag, medium, segment start and end
+    __shared__ int32_t q_i[MEDIUM_QUEUE], q_ps[MEDIUM_QUEUE];
+    __shared__ uint32_t q_flag[MEDIUM_QUEUE], q_med[MEDIUM_QUEUE];
+    __shared__ float q_o[3][MEDIUM_QUEUE], q_end[3][MEDIUM_QUEUE];
+
     const RenderConsts &rc = *rcp;
     QueueState *q = queue_state(c);
     const uint32_t n = (uint32_t)c->active;
@@ -670,12 +679,11 @@ __global__ void __launch_bounds__(MEDIUM_BLOCK) k_medium_track(const RenderConst
     const uint32_t chunks = min((uint32_t)COUNTER_STRIPES, gridDim.x);
     const uint32_t home = blockIdx.x % chunks;
     uint32_t *counter = q->medium_fetch + home * COUNTER_STRIDE;
-    // wavefront-uniform: reserved states not yet handed out, whether the
-    // home chunk has run dry, and whether no state is left for this wavefront
-    uint32_t res_lo = min(n, blockIdx.x * (uint32_t)MEDIUM_BLOCK);
-    uint32_t res_hi = min(n, res_lo + (uint32_t)MEDIUM_BLOCK);
+    // wavefront-uniform: the first batch, whether the home chunk has run
+    // dry, and the LDS queue's head and tail (entries head .. tail - 1)
+    bool first = blockIdx.x * (uint32_t)MEDIUM_BLOCK < n;
     bool dry = static_span >= n;
-    bool exhausted = dry && res_lo == res_hi;
+    uint32_t head = 0, tail = 0;
 
     // the lane's state: dense index, pool slot, hit flag, film pixel, sampler,
     // the grid medium being tracked, its progress and results
@@ -686,79 +694,106 @@ __global__ void __launch_bounds__(MEDIUM_BLOCK) k_medium_track(const RenderConst
     GridTrack g{};
     float tr = 1.0f;
 
-    // takes state j of a refill handing out states base .. base + 63
-    auto begin = [&](int32_t base, int32_t j) {
-        i = j;
-        ps = pool_slot(q, base, j);
-        flag = hit.flag[i];
-        const float4 ro = path.ray_o[ps];
-        const float4 rd = path.ray_d[ps];
-        const V3 o = xyz(ro), d = xyz(rd);
-
-        V3 end;
-        med = medium_segment(sv, hit, i, flag, ro, d, &end);
-        path.ray_o[ps] = f4(o, __uint_as_float(med));
-
-        // grid media are tracked here, one step per pass below; segments
-        // in a homogeneous medium or the volume primitives (no long
-        // tracking loops) are left to k_medium_scatter
-        if(med >= (uint32_t)sv.medium_count || sv.media[med].type == MEDIUM_HOMOGENEOUS)
-            return;
-        const MediumRec &m = sv.media[med];
-        const uint4 smp = path.smp[ps];
-        pixel = smp.x;
-        hs = PathSampler<SAMPLER>::load(rc, smp);
-        g = grid_track_start(m, o, end);
-        tr = 1.0f;
-        sampling = true;
-        scattered = false;
-        active = true;
-    };
-
     for(;;)
     {
-        // ---- hand new states to idle lanes (as the traversal hands out rays)
+        // ---- idle lanes take queued states; the queue is topped up first
         const uint64_t idle = __ballot(!active);
         const int nidle = (int)__popcll(idle);
-        if(nidle >= MEDIUM_REFILL_MIN && !exhausted)
+        if(nidle >= MEDIUM_REFILL_MIN)
         {
-            const uint32_t base_a = res_lo;
-            const uint32_t take_a = min((uint32_t)nidle, res_hi - res_lo);
-            res_lo += take_a;
-            uint32_t base_b = 0, take_b = 0;
-            if(take_a < (uint32_t)nidle && !dry)
+            while(tail - head < (uint32_t)nidle && (first || !dry))
             {
-                uint32_t gr = 0;
-                if(lane == 0)
-                    gr = atomicAdd(counter, 1u);
-                gr = __builtin_amdgcn_readfirstlane(gr);
-                const uint64_t lo = (uint64_t)static_span + ((uint64_t)gr * chunks + home) * GRANULE;
-                if(lo < (uint64_t)n)
+                // the next batch of up to 64 states
+                uint32_t base;
+                if(first)
                 {
-                    res_lo = (uint32_t)lo;
-                    res_hi = min(n, res_lo + (uint32_t)GRANULE);
-                    base_b = res_lo;
-                    take_b = min((uint32_t)nidle - take_a, res_hi - res_lo);
-                    res_lo += take_b;
+                    base = blockIdx.x * (uint32_t)MEDIUM_BLOCK;
+                    first = false;
                 }
                 else
-                    dry = true;
+                {
+                    uint32_t gr = 0;
+                    if(lane == 0)
+                        gr = atomicAdd(counter, 1u);
+                    gr = __builtin_amdgcn_readfirstlane(gr);
+                    const uint64_t lo = (uint64_t)static_span + ((uint64_t)gr * chunks + home) * GRANULE;
+                    if(lo >= (uint64_t)n)
+                    {
+                        dry = true;
+                        break;
+                    }
+                    base = (uint32_t)lo;
+                }
+                const uint32_t cnt = min((uint32_t)MEDIUM_BLOCK, n - base);
+                // every lane finds one state's segment (get_medium_segment)
+                bool grid = false;
+                int32_t bj = 0, bps = 0;
+                uint32_t bflag = 0, bmed = 0;
+                V3 bo = v3s(0.0f), bend = v3s(0.0f);
+                if((uint32_t)lane < cnt)
+                {
+                    bj = (int32_t)(base + (uint32_t)lane);
+                    bps = pool_slot(q, (int32_t)base, bj);
+                    bflag = hit.flag[bj];
+                    const float4 ro = path.ray_o[bps];
+                    const V3 d = xyz(path.ray_d[bps]);
+                    bo = xyz(ro);
+                    bmed = medium_segment(sv, hit, bj, bflag, ro, d, &bend);
+                    path.ray_o[bps] = f4(bo, __uint_as_float(bmed));
+                    // grid media are tracked here; segments in a homogeneous
+                    // medium or the volume primitives (no long tracking
+                    // loops) are left to k_medium_scatter
+                    grid = bmed < (uint32_t)sv.medium_count && sv.media[bmed].type != MEDIUM_HOMOGENEOUS;
+                }
+                const uint64_t gm = __ballot(grid);
+                if(grid)
+                {
+                    const uint32_t e = (tail + (uint32_t)__popcll(gm & lanes_below)) % MEDIUM_QUEUE;
+                    q_i[e] = bj;
+                    q_ps[e] = bps;
+                    q_flag[e] = bflag;
+                    q_med[e] = bmed;
+                    q_o[0][e] = bo.x;
+                    q_o[1][e] = bo.y;
+                    q_o[2][e] = bo.z;
+                    q_end[0][e] = bend.x;
+                    q_end[1][e] = bend.y;
+                    q_end[2][e] = bend.z;
+                }
+                tail += (uint32_t)__popcll(gm);
             }
-            exhausted = dry && res_lo == res_hi;
+            __syncthreads(); // the queue's entries are visible to every lane
+            const uint32_t take = min((uint32_t)nidle, tail - head);
             if(!active)
             {
                 const uint32_t rank = (uint32_t)__popcll(idle & lanes_below);
-                if(take_a > 0 && rank < take_a)
-                    begin((int32_t)base_a, (int32_t)(base_a + rank));
-                if(take_b > 0 && rank >= take_a && rank < take_a + take_b)
-                    begin((int32_t)base_b, (int32_t)(base_b + (rank - take_a)));
+                if(rank < take)
+                {
+                    const uint32_t e = (head + rank) % MEDIUM_QUEUE;
+                    i = q_i[e];
+                    ps = q_ps[e];
+                    flag = q_flag[e];
+                    med = q_med[e];
+                    const V3 o = v3(q_o[0][e], q_o[1][e], q_o[2][e]);
+                    const V3 end = v3(q_end[0][e], q_end[1][e], q_end[2][e]);
+                    const uint4 smp = path.smp[ps];
+                    pixel = smp.x;
+                    hs = PathSampler<SAMPLER>::load(rc, smp);
+                    g = grid_track_start(sv.media[med], o, end);
+                    tr = 1.0f;
+                    sampling = true;
+                    scattered = false;
+                    active = true;
+                }
             }
+            head += take;
+            __syncthreads(); // entries are read before later batches reuse them
         }
         if(!__any(active))
         {
-            if(exhausted)
+            if(head == tail && !first && dry)
                 break;
-            continue; // every state taken so far had no grid segment
+            continue; // the queue ran empty before the states did
         }
 
         // ---- one tracking step (medium_sample's loop, then medium_tr's)
