@@ -87,3 +87,21 @@ def test_media_render_parity(ctx, name, sampler):
     assert rel.max() <= 1e-4, f"max rel err {rel.max()} at {np.unravel_index(rel.argmax(), rel.shape)}"
     assert stats.stage_launches[4] == 0 or stats.stage_ms[4] >= 0
     assert ref.sum() > 0
+
+
+@pytest.mark.parametrize("name", list(SCENES))
+def test_media_pool_sizes_give_one_image(ctx, name):
+    """the medium stage hands states to its lanes in batches and queues;
+    pools smaller than one batch, of odd sizes and large ones trace the same
+    paths: equal ray counts and FP64 films"""
+    scene, cam = SCENES[name]()
+    w, h, spp = 40, 32, 3
+    ctx.upload_scene(scene)
+    ctx.set_camera(cam, w, h)
+    got = []
+    for pool in (37, 1000, 4097, 1 << 16):
+        st = ctx.render(render_params(w, h, spp, min_depth=3, max_depth=10, state_count=pool, film_fp64=1))
+        got.append((st.extension_rays, st.shadow_rays, ctx.film(w, h, fp64=True)[1]))
+    for e, s, img in got[1:]:
+        assert (e, s) == got[0][:2]
+        assert np.abs(img - got[0][2]).max() <= 1e-12 * max(1.0, np.abs(got[0][2]).max())
