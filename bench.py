@@ -880,18 +880,17 @@ def medium_roofline(p, staged, stats):
                 sample=stats["sample"])
 
 
-def media_cpu_baseline(scene, cam, p, budget_s, bands=8):
-    """CPU baseline of the media workload and the statistics of its work
-    model: the oracle's scalar path tracer (a port of the same algorithm) on
-    the host threads over sample-0 paths of `bands` bands of image rows
-    spread over the frame, a few more rows per band each pass until about
-    budget_s seconds, counting its tracking segments and density lookups"""
+def oracle_path_baseline(scene, cam, p, budget_s, bands=8):
+    """The full-path CPU baseline: the oracle's scalar path tracer (a port of
+    the same algorithm, oracle/oracle.cpp) on the host threads over sample-0
+    paths of `bands` bands of image rows spread over the frame, a few more
+    rows per band each pass until about budget_s seconds. Returns the
+    baseline (Mrays/s, with samples/s beside it) and the paths traced."""
     from oracle import oracle
     os_ = oracle.OracleScene(scene)
     threads = host_threads()
     W, H = p.width, p.height
     band_rows = H // bands
-    oracle.tracking_stats()
     done_rows, rows, rays, dt, paths = 0, 1, 0, 0.0, 0
     while dt < budget_s and done_rows < band_rows:
         rows = min(rows, band_rows - done_rows)
@@ -904,12 +903,22 @@ def media_cpu_baseline(scene, cam, p, budget_s, bands=8):
             paths += rows * W
         done_rows += rows
         rows *= 2
-    segments, lookups = oracle.tracking_stats()
     sample = (f"{paths} sample-0 paths in {bands} bands of {done_rows} image rows spread over the frame, traced by "
               f"the oracle's scalar path tracer (oracle/oracle.cpp) on {threads} host threads, {dt:.1f} s")
-    cpu = dict(value=rays / dt / 1e6, unit="Mrays/s", cores=threads, kind="port", sample=sample,
-               host=host_info(threads))
-    return cpu, dict(lookups_per_path=lookups / paths, segments_per_path=segments / paths, sample=sample)
+    cpu = dict(value=rays / dt / 1e6, unit="Mrays/s", samples_per_s=round(paths / dt, 1), cores=threads,
+               kind="port", sample=sample, host=host_info(threads))
+    return cpu, paths
+
+
+def media_cpu_baseline(scene, cam, p, budget_s, bands=8):
+    """CPU baseline of the media workload and the statistics of its work
+    model: the full-path baseline above on the media frame, counting the
+    oracle's tracking segments and density lookups"""
+    from oracle import oracle
+    oracle.tracking_stats()
+    cpu, paths = oracle_path_baseline(scene, cam, p, budget_s, bands)
+    segments, lookups = oracle.tracking_stats()
+    return cpu, dict(lookups_per_path=lookups / paths, segments_per_path=segments / paths, sample=cpu["sample"])
 
 
 def flatten_instances(scene):
