@@ -30,7 +30,13 @@ METRIC = "Mrays/s & samples/s @1080p; equal-time relMSE vs FP64 GT, NRC vs plain
 HBM_PEAK_GBS = 8000.0
 WIDTH, HEIGHT, SPP = 1920, 1080, 64
 MIN_DEPTH, MAX_DEPTH = 3, 8
-STATE_COUNT = 1 << 21
+# path states in flight: 33.5 M (about 17 GB of pools, queues and hits on
+# the device's 288 GB; the reference's default is 1,000,000). Larger pools
+# mean fewer, larger launches per frame: 1080p at 64 spp took 249.6 ms with
+# 2^21 states, 229.2 ms with 2^22, 216.3 ms with 2^23, 207.3 ms with 2^24,
+# 203.9 ms with 2^25 and 202.1 ms with 2^26.
+STATE_COUNT = 1 << 25
+CAPTURE_STATE_COUNT = 1 << 21  # the pool of the frame whose first bounces give the baseline's rays
 
 
 def log(msg):
@@ -163,6 +169,7 @@ def probe(args):
         print(f"{args.probe}: {args.spp} spp, extension_rays {st.extension_rays} shadow_rays {st.shadow_rays} "
               f"iterations {st.iterations} stage_ms {[round(x, 2) for x in st.stage_ms[:7]]}", flush=True)
         return
+    p.state_count = CAPTURE_STATE_COUNT  # the rays the CPU baseline traces
     r = ctx.capture_rays(p, 2, args.rays)
     r = r[np.random.default_rng(0).permutation(len(r))]
     ctx.batch_upload(r)
@@ -323,6 +330,11 @@ def main():
         log("bench: capturing rays of the first two bounces")
         pc = _abi.RenderParams.from_buffer_copy(p)
         pc.time_stages = 0
+        # the first two bounces of a 2^21-state wave: primary rays and the
+        # first bounce's in equal parts, as the traversal meets them over
+        # the frame (the first 4 M rays of a larger wave would all be
+        # primary rays)
+        pc.state_count = CAPTURE_STATE_COUNT
         rays = ctx.capture_rays(pc, 2, 1 << 22)
         rays = rays[np.random.default_rng(0).permutation(len(rays))]
         # world-space triangles in flattened order, as the GPU traverses them

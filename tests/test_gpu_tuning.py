@@ -192,3 +192,20 @@ def test_autotune_picks_a_candidate_and_keeps_results(ctx, two_level):
     assert (before.extension_rays, before.shadow_rays) == (after.extension_rays, after.shadow_rays)
     rel = np.abs(img0 - img1) / np.maximum(np.abs(img0), 1e-2)
     assert rel.max() <= 1e-5
+
+
+def test_large_pool_traces_the_same_paths(ctx):
+    """a frame of 33 M paths in one 2^25-state wave (dense indices up to
+    33 M, pool slots beyond 2^25) and in 2^21-state waves: equal ray counts
+    and FP64 films"""
+    scene, cam = scenes.sphere_field(instances=256, lights=8, seed=6, extent=30.0)
+    ctx.upload_scene(scene)
+    w, h, spp = 1920, 1080, 16
+    ctx.set_camera(cam, w, h)
+    got = []
+    for pool in (1 << 25, 1 << 21):
+        st = ctx.render(render_params(w, h, spp, max_depth=8, state_count=pool, film_fp64=1))
+        got.append((st.extension_rays, st.shadow_rays, ctx.film(w, h, fp64=True)[1]))
+    (ea, sa, a), (eb, sb, b) = got
+    assert ea == eb and sa == sb
+    assert np.abs(a - b).max() <= 1e-12 * max(1.0, np.abs(b).max())
