@@ -23,7 +23,9 @@ from . import _abi
 WAVES = (16, 20, 24, 28)
 STAGE_BLOCKS = (8, 16, 32)
 GRAPH_BOUNCES = (2, 4, 8)
-POOLS = (1 << 20, 1 << 21, 1 << 22)
+# path pools of 2^21 .. 2^25 states (about 1 to 17 GB of device memory;
+# larger pools mean fewer, larger launches per frame)
+POOLS = (1 << 21, 1 << 23, 1 << 25)
 
 
 def _staged(ctx, p):
