@@ -362,7 +362,7 @@ struct btrc_ctx
     DevBuf<float> ovf_t;
     int32_t trace_grid = 0;
     DevBuf<uint32_t> fetch;
-    btrc_tuning tuning{ 1, 0, TRACE_WAVES_PER_CU, 1, 0, 0, 0, 0 };
+    btrc_tuning tuning{ 0, 0, TRACE_WAVES_PER_CU, 1, 0, 0, 0, 0 };
 
     // the bounce loop as a captured graph, kept while its launches stay the same
     RenderConsts *rc_host = nullptr;  // pinned
@@ -429,7 +429,7 @@ struct btrc_ctx
     int persistent_grid(int threads_per_block, int64_t work) const
     {
         const int64_t want = (work + threads_per_block - 1) / threads_per_block;
-        const int64_t cap = (int64_t)cu_count * (tuning.stage_blocks_per_cu > 0 ? tuning.stage_blocks_per_cu : 16);
+        const int64_t cap = (int64_t)cu_count * (tuning.stage_blocks_per_cu > 0 ? tuning.stage_blocks_per_cu : 32);
         return (int)std::max<int64_t>(1, std::min(want, cap));
     }
 };

@@ -363,14 +363,14 @@ typedef struct btrc_hit
  * Picked per device by btrc_amd/autotune.py. */
 typedef struct btrc_tuning
 {
-    int32_t closest_postpone_leaves; /* one-level scenes, 1: leaves wait until every lane holds one (default 1) */
+    int32_t closest_postpone_leaves; /* one-level scenes, 1: leaves wait until every lane holds one (default 0) */
     int32_t any_postpone_leaves;     /* same for visibility rays (default 0) */
     int32_t trace_waves_per_cu;      /* persistent traversal grid, 1..32 (default 20) */
     int32_t launch_graph;            /* 1: the bounce loop replays a captured HIP graph (default 1) */
     int32_t two_level_postpone;      /* two-level scenes: bit 0 closest-hit, bit 1 visibility rays
                                         postpone leaves (default 0: both test leaves when reached) */
     int32_t stage_blocks_per_cu;     /* grid of the 256-lane stage kernels (generate, medium, shade): at most
-                                        this many workgroups per CU, 1..64 (0: 16) */
+                                        this many workgroups per CU, 1..64 (0: 32) */
     int32_t graph_bounces;           /* bounces per captured graph block between counter read-backs,
                                         even, 2..16 (0: 4) */
     int32_t reserved;

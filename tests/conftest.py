@@ -28,7 +28,7 @@ def ctx(_session_ctx):
     filter, the default tuning and the cache switched off"""
     c = _session_ctx
     c.set_filter("box")
-    c.set_tuning(closest_postpone_leaves=1, any_postpone_leaves=0, trace_waves_per_cu=20, launch_graph=1,
+    c.set_tuning(closest_postpone_leaves=0, any_postpone_leaves=0, trace_waves_per_cu=20, launch_graph=1,
                  two_level_postpone=0, stage_blocks_per_cu=0, graph_bounces=0)
     c.nrc_configure(enabled=0, train_depth=0)
     yield c
