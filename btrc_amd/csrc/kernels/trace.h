@@ -28,7 +28,7 @@ namespace btrc {
 
 constexpr int TRACE_BLOCK = 64;   // one wavefront per workgroup
 constexpr int SHORT_STACK = 16;   // LDS stack entries per lane
-constexpr int REFILL_MIN = 32;    // idle lanes that trigger a refill
+constexpr int REFILL_MIN = 16;    // idle lanes that trigger a refill
 constexpr int TRACE_WAVES_PER_CU = 20; // persistent grid: wavefronts per compute unit (default)
 constexpr int TRACE_WAVES_PER_CU_MAX = 32;
 
