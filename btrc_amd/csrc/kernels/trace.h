@@ -3,12 +3,14 @@
 // Persistent "while-while" traversal (Aila and Laine, "Understanding the
 // Efficiency of Ray Traversal on GPUs", HPG 2009) laid out for 64-wide
 // wavefronts:
-//  * one wavefront per workgroup, one lane per ray; a lane whose ray has
-//    ended takes a new one from a global counter (one atomic per wavefront
-//    for all idle lanes), so no lane waits for the slowest ray of its batch;
-//  * inner-node steps and leaf (triangle) steps run in separate loops, and
-//    a lane that has found a leaf keeps stepping through inner nodes until
-//    every lane of the wavefront holds a leaf, so leaf tests run together;
+//  * one wavefront per workgroup, one lane per ray; once REFILL_MIN lanes
+//    have ended their rays they take new ones from the wavefront's granule
+//    (one atomic per wavefront for a new granule, below), so no lane waits
+//    for the slowest ray of its batch;
+//  * a lane tests a leaf's triangles as soon as it meets the leaf (the
+//    default), or, with leaf postponing, keeps stepping through inner nodes
+//    until every lane of the wavefront holds a leaf, so that leaf tests run
+//    together;
 //  * the traversal stack keeps its top SHORT_STACK entries in LDS, laid out
 //    [entry][lane] (a wavefront's pushes hit consecutive words, no bank
 //    conflicts), and deeper entries in a per-lane global overflow area sized
